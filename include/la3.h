@@ -1,0 +1,200 @@
+/*
+ * la3.h — C ABI of the MI355X (gfx950) engine for LA3's vertex-program path.
+ *
+ * The reference boundary is the header-only C++ template pair
+ *   Graph<W>::load_directed / load_undirected      (src/matrix/graph.h:38-47)
+ *   VertexProgram<W,M,A,S>::initialize / execute   (src/vprogram/vertex_program.h:33-220)
+ * with the graph-analytics programs of src/apps/graph_analytics/ (degree, pr, bfs,
+ * sssp, cc, tc). Device code
+ * cannot call host virtuals, so each program is selected by its entry point
+ * here; the C++ drop-in wrappers in include/la3.hpp keep the reference's class
+ * and method names on top of these calls.
+ *
+ * Conventions
+ *  - Every call returns la3_status; la3_last_error(ctx) gives a message for the
+ *    last failure on that context. No C++ exception crosses this boundary.
+ *  - The library owns all device memory; the caller owns every host buffer.
+ *  - Per-vertex outputs are indexed by ORIGINAL vertex id and have
+ *    la3_graph_info.nslots entries (ids 0..n, the one-rank tile of
+ *    src/matrix/matrix2d.hpp:66-70 with n/1+1 slots).
+ *  - One host thread per context at a time (calls on a context serialize on its
+ *    stream); results do not depend on batch composition.
+ */
+#ifndef LA3_H
+#define LA3_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  LA3_OK = 0,
+  LA3_EINVAL = 1,   /* bad argument: null pointer, id out of range, wrong graph mode */
+  LA3_ENOMEM = 2,   /* device or host allocation failed, or admission rejected */
+  LA3_EIO = 3,      /* file could not be opened / read / stat'ed (graph.hpp:186-203) */
+  LA3_EDEVICE = 4,  /* a HIP runtime call failed */
+  LA3_ENOTSUP = 5,  /* size beyond what this build supports (e.g. nnz >= 2^32) */
+  LA3_EAGAIN = 6    /* serving: no completed query yet */
+} la3_status;
+
+/* Load modes: the flag combinations of Graph<W>::load_directed(reverse_edges,
+ * remove_cycles) and load_undirected (src/matrix/graph.hpp:342-364). */
+typedef enum {
+  LA3_DIRECTED = 0,          /* load_directed(reverse=false): stores A^T, messages flow a->b */
+  LA3_REVERSED = 1,          /* load_directed(reverse=true):  messages flow b->a */
+  LA3_UNDIRECTED = 2,        /* load_undirected: both orientations */
+  LA3_ACYCLIC = 3,           /* load_directed(false, remove_cycles=true): low id -> high id */
+  LA3_REVERSED_ACYCLIC = 4   /* load_directed(true, true): high id -> low id */
+} la3_mode;
+
+/* Vertex-id hashing (src/matrix/graph.h:10-15). */
+typedef enum { LA3_HASH_NONE = 0, LA3_HASH_BUCKET = 1 } la3_hashing;
+
+typedef struct la3_ctx la3_ctx;
+typedef struct la3_graph la3_graph;
+
+typedef struct {
+  uint32_t n;          /* nvertices as passed (or read from the header) */
+  uint32_t nslots;     /* n + 1 */
+  uint32_t nregular;   /* vertex classes (src/matrix/processed_matrix2d.hpp:292-317) */
+  uint32_t nsink;
+  uint32_t nsource;
+  uint32_t nisolated;
+  uint32_t ncols;      /* non-empty columns = nregular + nsource */
+  uint32_t directed;
+  uint32_t weighted;
+  uint32_t mode;
+  uint64_t nnz;        /* deduplicated stored entries */
+  uint64_t nnz_regular;/* entries whose row is a regular vertex */
+  uint64_t nnz_sink;   /* entries whose row is a sink vertex */
+  uint64_t nrecords;   /* records read from the input (before self-loop/dedup) */
+  uint64_t device_bytes;
+} la3_graph_info;
+
+/* ------------------------------------------------------------ context --- */
+/* Creates a context on HIP device `device`: one stream, device scratch.
+ * device == -1 creates a host-only context: graphs are ingested and their
+ * layout can be exported (la3_graph_info_get / slots / csc), but programs
+ * return LA3_EDEVICE. */
+la3_status la3_ctx_create(int device, la3_ctx** out);
+void la3_ctx_destroy(la3_ctx* ctx);
+const char* la3_last_error(const la3_ctx* ctx);
+/* Waits for all work queued on the context's stream. */
+la3_status la3_ctx_synchronize(la3_ctx* ctx);
+
+/* -------------------------------------------------------------- graphs --- */
+/* Graph<W>::load_binary (src/matrix/graph.hpp:155-332) at np=1: reads the LA3
+ * binary file (u32 pairs, or u32 triples when weighted != 0), drops
+ * self-loops, orients per `mode`, hashes ids, deduplicates with the first
+ * record winning, classifies vertices and builds the device-resident DCSC and
+ * its row-major companion. nvertices == 0 means the file starts with the
+ * 16-byte header {u32 n-1, u32 m-1, u64 nnz}. Ids must lie in [0, n]. */
+la3_status la3_graph_load_file(la3_ctx* ctx, const char* path, uint32_t nvertices, int weighted,
+                               la3_mode mode, la3_hashing hashing, la3_graph** out);
+/* Same, from records already in host memory (nrecords records of 2 or 3 u32). */
+la3_status la3_graph_load_records(la3_ctx* ctx, const uint32_t* records, uint64_t nrecords,
+                                  uint32_t nvertices, int weighted, la3_mode mode,
+                                  la3_hashing hashing, la3_graph** out);
+/* Graph<W>::free (src/matrix/graph.hpp:146-152). */
+void la3_graph_free(la3_graph* g);
+la3_status la3_graph_info_get(const la3_graph* g, la3_graph_info* out);
+/* Layout export for parity checks: hid_of[nslots] maps a vertex-state slot
+ * (order [regular | sink | source | isolated], each by increasing hashed id,
+ * src/utils/locator.h:84-120) to its hashed id. */
+la3_status la3_graph_slots(const la3_graph* g, uint32_t* hid_of);
+/* Export of one of the tile's two DCSCs (which = 0 regular rows, 1 sink rows),
+ * in the reference's CSC<W> layout (src/utils/csc.h:61-73): colptrs[ncols+1],
+ * colidxs[ncols] (hashed column ids), per entry global_idx (position among the
+ * regular or the sink rows), idx (hashed row id) and weight. */
+la3_status la3_graph_csc(const la3_graph* g, int which, uint32_t* colptrs, uint32_t* colidxs,
+                         uint32_t* global_idx, uint32_t* idx, uint32_t* weight);
+/* hasher->hash / unhash (src/matrix/hashers.h:432-444). */
+uint32_t la3_graph_hash(const la3_graph* g, uint32_t v);
+uint32_t la3_graph_unhash(const la3_graph* g, uint32_t v);
+
+/* ------------------------------------------------- single-query programs --- */
+/* DegVertex (src/apps/graph_analytics/degree.h:25-35) with execute(1): the
+ * in-degree of the stored graph per vertex (out-degree on a REVERSED load). */
+la3_status la3_degree(la3_graph* g, uint32_t* degree);
+
+/* PrVertex (pr.h:28-46, pr.cpp:11-55) on a DIRECTED load: out-degrees as the
+ * reversed-graph DegVertex computes them, then execute(iters) (0 = until no
+ * rank moves by more than 1e-5). Ranks are unnormalized (0.15 + 0.85*sum);
+ * sources and isolated vertices keep 0.15; sinks are computed once after the
+ * loop from the final regular ranks (vertex_program_execute.hpp:131-148).
+ * FP64, deterministic summation. degree and iters_run may be NULL. */
+la3_status la3_pagerank(la3_graph* g, uint32_t iters, double* rank, uint32_t* degree,
+                        uint32_t* iters_run);
+
+/* BfsVertex (bfs.h:27-49), execute() until convergence, on any load mode
+ * (bfs.cpp:14 uses UNDIRECTED). hops: u8 level, 255 = unreached. parent (may
+ * be NULL): the hashed id of the in-neighbour whose message the reference
+ * combines last (last writer wins, bfs.h:40), 0 for the root and unreached. */
+la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
+                   uint32_t* iters_run);
+
+/* SpVertex (sssp.h:26-46) on a weighted DIRECTED load, execute() until
+ * convergence. dist: u32, 2^31-1 for unreachable. */
+la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters_run);
+
+/* CcVertex (cc.h:22-38) on an UNDIRECTED load: min original id per connected
+ * component (label 0 is the reference's "empty" accumulator, cc.h:31; a
+ * graph whose vertex 0 has edges is outside the reference's defined
+ * behaviour). */
+la3_status la3_cc(la3_graph* g, uint32_t* label, uint32_t* iters_run);
+
+/* GnVertex + CtVertex (tc.h:47-99, tc.cpp:11-52) on an ACYCLIC load: total
+ * triangle count (u64) and, if per_vertex != NULL, the count each vertex holds
+ * in the reference (triangles in which it is the middle id). */
+la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex);
+
+/* Personalized PageRank (not in the reference; PR with teleport alpha at
+ * `seed` only, initial rank = teleport, same loop and sink step as
+ * la3_pagerank). FP64 single query. */
+la3_status la3_ppr(la3_graph* g, uint32_t seed, uint32_t iters, double alpha, double* rank);
+
+/* ------------------------------------------------------- batched queries --- */
+/* k BFS roots in one traversal with k-bit frontiers (k <= 1024). hops is
+ * k x nslots, query-major: hops[q * nslots + id]. Identical per query to
+ * la3_bfs levels. */
+la3_status la3_bfs_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint8_t* hops,
+                         uint32_t* iters_run);
+/* k SSSP roots with k-wide u32 distances (k <= 1024); dist[q * nslots + id]. */
+la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint32_t* dist,
+                          uint32_t* iters_run);
+/* k PPR seeds with FP32 ranks and compensated (Kahan) row sums; rank[q * nslots + id]. */
+la3_status la3_ppr_batch(la3_graph* g, const uint32_t* seeds, uint32_t k, uint32_t iters,
+                         float alpha, float* rank);
+
+/* ------------------------------------------------------------- timing --- */
+/* Device time (ms) of the kernels of the last la3_* program call on this
+ * context, measured with HIP events on the context's stream, and the name of
+ * its dominant kernel with its launch count and summed duration. */
+typedef struct {
+  double total_ms;
+  double dominant_ms;
+  uint64_t dominant_launches;
+  uint64_t edges_scanned;   /* entries read by the traversal kernels */
+  char dominant_kernel[64];
+} la3_timing;
+la3_status la3_last_timing(const la3_ctx* ctx, la3_timing* out);
+/* Enables per-launch HIP-event timing (adds one event pair per launch). */
+la3_status la3_set_profiling(la3_ctx* ctx, int enabled);
+
+/* -------------------------------------------------------------- tools --- */
+/* Seeded R-MAT generator (a, b, c, d = 1-a-b-c), 2^scale vertices with ids
+ * 1..2^scale, edgefactor*2^scale samples, self-loops and duplicates removed,
+ * records sorted by (src, dst); with weighted != 0 each record carries a u32
+ * weight uniform in [1, 128]. Call with out == NULL to get the record count
+ * in *nrecords, then with a buffer of nrecords*(2 or 3) u32. Deterministic
+ * for a seed, independent of thread count. */
+la3_status la3_rmat_generate(uint32_t scale, uint32_t edgefactor, double a, double b, double c,
+                             uint64_t seed, int weighted, uint32_t* out, uint64_t* nrecords);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LA3_H */

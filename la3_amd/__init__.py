@@ -1,0 +1,308 @@
+"""Python mirror of LA3's Graph / VertexProgram entry points over the C ABI
+of la3_amd/libla3.so (include/la3.h).
+
+The reference's host interface is C++ (src/matrix/graph.h:38-47,
+src/vprogram/vertex_program.h:33-220); the C++ drop-in lives in
+include/la3.hpp. This module exposes the same loads and programs to Python
+for tests, benchmarks and the serving loop. It loads the HIP library and
+raises if it is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libla3.so")
+
+OK, EINVAL, ENOMEM, EIO, EDEVICE, ENOTSUP, EAGAIN = range(7)
+DIRECTED, REVERSED, UNDIRECTED, ACYCLIC, REVERSED_ACYCLIC = range(5)
+HASH_NONE, HASH_BUCKET = 0, 1
+BFS_INF = 255
+SSSP_INF = 2**31 - 1
+
+# Every symbol include/la3.h declares (checked by tests/test_capi.py).
+EXPORTS = [
+    "la3_ctx_create", "la3_ctx_destroy", "la3_last_error", "la3_ctx_synchronize",
+    "la3_graph_load_file", "la3_graph_load_records", "la3_graph_free", "la3_graph_info_get",
+    "la3_graph_slots", "la3_graph_csc", "la3_graph_hash", "la3_graph_unhash",
+    "la3_degree", "la3_pagerank", "la3_bfs", "la3_sssp", "la3_cc", "la3_tc", "la3_ppr",
+    "la3_bfs_batch", "la3_sssp_batch", "la3_ppr_batch",
+    "la3_last_timing", "la3_set_profiling", "la3_rmat_generate",
+]
+
+
+class LA3Error(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"la3 status {status}: {msg}")
+        self.status = status
+
+
+class GraphInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "n", "nslots", "nregular", "nsink", "nsource", "nisolated", "ncols", "directed",
+        "weighted", "mode")] + [(n, ctypes.c_uint64) for n in (
+            "nnz", "nnz_regular", "nnz_sink", "nrecords", "device_bytes")]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("dominant_ms", ctypes.c_double),
+                ("dominant_launches", ctypes.c_uint64), ("edges_scanned", ctypes.c_uint64),
+                ("dominant_kernel", ctypes.c_char * 64)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
+                              "(the HIP library is required; there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        P, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+        sig = {
+            "la3_ctx_create": (i32, [i32, ctypes.POINTER(P)]),
+            "la3_ctx_destroy": (None, [P]),
+            "la3_last_error": (ctypes.c_char_p, [P]),
+            "la3_ctx_synchronize": (i32, [P]),
+            "la3_graph_load_file": (i32, [P, ctypes.c_char_p, u32, i32, i32, i32, ctypes.POINTER(P)]),
+            "la3_graph_load_records": (i32, [P, P, u64, u32, i32, i32, i32, ctypes.POINTER(P)]),
+            "la3_graph_free": (None, [P]),
+            "la3_graph_info_get": (i32, [P, ctypes.POINTER(GraphInfo)]),
+            "la3_graph_slots": (i32, [P, P]),
+            "la3_graph_csc": (i32, [P, i32, P, P, P, P, P]),
+            "la3_graph_hash": (u32, [P, u32]),
+            "la3_graph_unhash": (u32, [P, u32]),
+            "la3_degree": (i32, [P, P]),
+            "la3_pagerank": (i32, [P, u32, P, P, P]),
+            "la3_bfs": (i32, [P, u32, P, P, P]),
+            "la3_sssp": (i32, [P, u32, P, P]),
+            "la3_cc": (i32, [P, P, P]),
+            "la3_tc": (i32, [P, P, P]),
+            "la3_ppr": (i32, [P, u32, u32, ctypes.c_double, P]),
+            "la3_bfs_batch": (i32, [P, P, u32, P, P]),
+            "la3_sssp_batch": (i32, [P, P, u32, P, P]),
+            "la3_ppr_batch": (i32, [P, P, u32, u32, ctypes.c_float, P]),
+            "la3_last_timing": (i32, [P, ctypes.POINTER(Timing)]),
+            "la3_set_profiling": (i32, [P, i32]),
+            "la3_rmat_generate": (i32, [u32, u32, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, u64, i32, P, ctypes.POINTER(u64)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def rmat(scale, edgefactor=16, a=0.57, b=0.19, c=0.19, seed=1, weighted=False):
+    """Seeded R-MAT records (ids 1..2^scale, no self-loops or duplicates)."""
+    n = ctypes.c_uint64(0)
+    st = lib().la3_rmat_generate(scale, edgefactor, a, b, c, seed, int(weighted), None,
+                                 ctypes.byref(n))
+    if st:
+        raise LA3Error(st, "rmat_generate")
+    out = np.empty(n.value * (3 if weighted else 2), dtype=np.uint32)
+    st = lib().la3_rmat_generate(scale, edgefactor, a, b, c, seed, int(weighted), _ptr(out),
+                                 ctypes.byref(n))
+    if st:
+        raise LA3Error(st, "rmat_generate")
+    return out.reshape(-1, 3 if weighted else 2)
+
+
+class Context:
+    """One device (or, with device=-1, a host-only context for layout work)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        st = lib().la3_ctx_create(device, ctypes.byref(h))
+        if st:
+            raise LA3Error(st, f"la3_ctx_create(device={device}) failed")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().la3_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def check(self, st, what):
+        if st:
+            raise LA3Error(st, f"{what}: {lib().la3_last_error(self.h).decode()}")
+
+    def set_profiling(self, on=True):
+        self.check(lib().la3_set_profiling(self.h, int(on)), "set_profiling")
+
+    def last_timing(self):
+        t = Timing()
+        self.check(lib().la3_last_timing(self.h, ctypes.byref(t)), "last_timing")
+        return dict(total_ms=t.total_ms, dominant_ms=t.dominant_ms,
+                    dominant_launches=t.dominant_launches, edges_scanned=t.edges_scanned,
+                    dominant_kernel=t.dominant_kernel.decode())
+
+    def synchronize(self):
+        self.check(lib().la3_ctx_synchronize(self.h), "synchronize")
+
+
+class Graph:
+    """Graph<W> (src/matrix/graph.h:22-111) at one rank, resident on the device."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+        info = GraphInfo()
+        ctx.check(lib().la3_graph_info_get(self.h, ctypes.byref(info)), "graph_info")
+        self.info = {f: getattr(info, f) for f, _ in GraphInfo._fields_}
+        self.n = info.n
+        self.nslots = info.nslots
+        self.weighted = bool(info.weighted)
+        self.mode = info.mode
+
+    @classmethod
+    def from_records(cls, ctx, records, n, weighted=False, mode=DIRECTED, hashing=HASH_BUCKET):
+        rec = np.ascontiguousarray(records, dtype=np.uint32)
+        nrec = rec.size // (3 if weighted else 2)
+        h = ctypes.c_void_p()
+        ctx.check(lib().la3_graph_load_records(ctx.h, _ptr(rec), nrec, int(n), int(weighted),
+                                               int(mode), int(hashing), ctypes.byref(h)),
+                  "graph_load_records")
+        return cls(ctx, h)
+
+    @classmethod
+    def load_file(cls, ctx, path, n, weighted=False, mode=DIRECTED, hashing=HASH_BUCKET):
+        h = ctypes.c_void_p()
+        ctx.check(lib().la3_graph_load_file(ctx.h, path.encode(), int(n), int(weighted),
+                                            int(mode), int(hashing), ctypes.byref(h)),
+                  "graph_load_file")
+        return cls(ctx, h)
+
+    # graph.h:38-47 spellings
+    @classmethod
+    def load_directed(cls, ctx, path, n, reverse_edges=False, remove_cycles=False,
+                      weighted=False, hashing=HASH_BUCKET):
+        mode = {(False, False): DIRECTED, (True, False): REVERSED,
+                (False, True): ACYCLIC, (True, True): REVERSED_ACYCLIC}[
+                    (bool(reverse_edges), bool(remove_cycles))]
+        return cls.load_file(ctx, path, n, weighted, mode, hashing)
+
+    @classmethod
+    def load_undirected(cls, ctx, path, n, weighted=False, hashing=HASH_BUCKET):
+        return cls.load_file(ctx, path, n, weighted, UNDIRECTED, hashing)
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().la3_graph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.free()
+
+    def hash(self, v):
+        return int(lib().la3_graph_hash(self.h, int(v)))
+
+    def unhash(self, v):
+        return int(lib().la3_graph_unhash(self.h, int(v)))
+
+    def slots(self):
+        out = np.zeros(self.nslots, dtype=np.uint32)
+        self.ctx.check(lib().la3_graph_slots(self.h, _ptr(out)), "graph_slots")
+        return out
+
+    def csc(self, which):
+        nnz = self.info["nnz_sink"] if which else self.info["nnz_regular"]
+        nc = self.info["ncols"]
+        colptrs = np.zeros(nc + 1, dtype=np.uint32)
+        colidxs = np.zeros(nc + 1, dtype=np.uint32)
+        g = np.zeros(max(nnz, 1), dtype=np.uint32)
+        i = np.zeros(max(nnz, 1), dtype=np.uint32)
+        w = np.zeros(max(nnz, 1), dtype=np.uint32)
+        self.ctx.check(lib().la3_graph_csc(self.h, int(which), _ptr(colptrs), _ptr(colidxs),
+                                           _ptr(g), _ptr(i), _ptr(w)), "graph_csc")
+        return dict(colptrs=colptrs, colidxs=colidxs, global_idx=g[:nnz], idx=i[:nnz], w=w[:nnz])
+
+
+# ------------------------------------------------------------- programs ---
+def degree(g):
+    out = np.zeros(g.nslots, dtype=np.uint32)
+    g.ctx.check(lib().la3_degree(g.h, _ptr(out)), "degree")
+    return out
+
+
+def pagerank(g, iters):
+    rank = np.zeros(g.nslots, dtype=np.float64)
+    deg = np.zeros(g.nslots, dtype=np.uint32)
+    it = np.zeros(1, dtype=np.uint32)
+    g.ctx.check(lib().la3_pagerank(g.h, int(iters), _ptr(rank), _ptr(deg), _ptr(it)), "pagerank")
+    return rank, deg, int(it[0])
+
+
+def ppr(g, seed, iters, alpha=0.15):
+    rank = np.zeros(g.nslots, dtype=np.float64)
+    g.ctx.check(lib().la3_ppr(g.h, int(seed), int(iters), float(alpha), _ptr(rank)), "ppr")
+    return rank
+
+
+def bfs(g, root, parents=True):
+    hops = np.zeros(g.nslots, dtype=np.uint8)
+    par = np.zeros(g.nslots, dtype=np.uint32) if parents else None
+    it = np.zeros(1, dtype=np.uint32)
+    g.ctx.check(lib().la3_bfs(g.h, int(root), _ptr(hops), _ptr(par), _ptr(it)), "bfs")
+    return hops, par, int(it[0])
+
+
+def sssp(g, root):
+    dist = np.zeros(g.nslots, dtype=np.uint32)
+    it = np.zeros(1, dtype=np.uint32)
+    g.ctx.check(lib().la3_sssp(g.h, int(root), _ptr(dist), _ptr(it)), "sssp")
+    return dist, int(it[0])
+
+
+def cc(g):
+    label = np.zeros(g.nslots, dtype=np.uint32)
+    it = np.zeros(1, dtype=np.uint32)
+    g.ctx.check(lib().la3_cc(g.h, _ptr(label), _ptr(it)), "cc")
+    return label, int(it[0])
+
+
+def tc(g):
+    total = np.zeros(1, dtype=np.uint64)
+    per = np.zeros(g.nslots, dtype=np.uint32)
+    g.ctx.check(lib().la3_tc(g.h, _ptr(total), _ptr(per)), "tc")
+    return int(total[0]), per
+
+
+def bfs_batch(g, roots):
+    roots = np.ascontiguousarray(roots, dtype=np.uint32)
+    hops = np.zeros((roots.size, g.nslots), dtype=np.uint8)
+    it = np.zeros(1, dtype=np.uint32)
+    g.ctx.check(lib().la3_bfs_batch(g.h, _ptr(roots), roots.size, _ptr(hops), _ptr(it)),
+                "bfs_batch")
+    return hops, int(it[0])
+
+
+def sssp_batch(g, roots):
+    roots = np.ascontiguousarray(roots, dtype=np.uint32)
+    dist = np.zeros((roots.size, g.nslots), dtype=np.uint32)
+    it = np.zeros(1, dtype=np.uint32)
+    g.ctx.check(lib().la3_sssp_batch(g.h, _ptr(roots), roots.size, _ptr(dist), _ptr(it)),
+                "sssp_batch")
+    return dist, int(it[0])
+
+
+def ppr_batch(g, seeds, iters, alpha=0.15):
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+    rank = np.zeros((seeds.size, g.nslots), dtype=np.float32)
+    g.ctx.check(lib().la3_ppr_batch(g.h, _ptr(seeds), seeds.size, int(iters), float(alpha),
+                                    _ptr(rank)), "ppr_batch")
+    return rank

@@ -1,0 +1,47 @@
+"""Builds la3_amd/libla3.so for gfx950 in-tree.
+
+host_build.cpp (ingest, R-MAT) is plain C++ built with g++; capi.hip (C ABI,
+kernels) is built with hipcc for gfx950 only. Both link into one shared
+library with a C ABI (include/la3.h).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libla3.so")
+BUILD = os.path.join(HERE, "_build")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    # Packed FP32 instructions stay off: plain float code only.
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wextra"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build(verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    host_o = os.path.join(BUILD, "host_build.o")
+    capi_o = os.path.join(BUILD, "capi.o")
+    _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "host_build.cpp"), "-o", host_o])
+    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "capi.hip"), "-o", capi_o])
+    tmp = OUT + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", capi_o, host_o, "-o", tmp,
+          "-pthread", "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

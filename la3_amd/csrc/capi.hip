@@ -1,0 +1,705 @@
+// C-ABI implementation (include/la3.h): contexts, device-resident graphs,
+// per-launch HIP-event profiling and the host drivers of each program.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include "../../include/la3.h"
+#include "internal.h"
+#include "kernels_dev.h"
+
+using namespace la3;
+
+// ------------------------------------------------------------------ ctx ---
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct la3_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool profiling = false;
+  struct Ev { const char* name; hipEvent_t a, b; };
+  std::vector<Ev> evs;
+  std::vector<hipEvent_t> free_events;
+  la3_timing last{};
+  uint64_t edges_scanned = 0;
+  std::map<std::string, DevBuf> scratch;
+  void* host_ctr = nullptr;  // pinned, for frontier counters
+  int ncu = 256;
+};
+
+struct la3_graph {
+  la3_ctx* ctx = nullptr;
+  uint32_t n = 0, nslots = 0, nreg = 0, nsink = 0, nsrc = 0, niso = 0, ncols = 0, nrows = 0;
+  bool directed = true, weighted = false;
+  int mode = 0;
+  Hasher h;
+  uint64_t nnz = 0, nnz_reg = 0, nnz_sink = 0, nrecords = 0, device_bytes = 0;
+  std::vector<uint32_t> hid_of, id_of, slot_of_id;
+  std::vector<void*> allocs;
+  DG dg{};
+  std::unique_ptr<HostLayout> host;  // kept instead of device arrays on a host-only context
+};
+
+namespace {
+
+la3_status fail(la3_ctx* c, la3_status s, const std::string& msg) {
+  if (c) c->err = msg;
+  return s;
+}
+
+#define HIPC(ctx, call)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail((ctx), LA3_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+hipEvent_t get_event(la3_ctx* c) {
+  if (!c->free_events.empty()) { hipEvent_t e = c->free_events.back(); c->free_events.pop_back(); return e; }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+template <typename... KArgs, typename... Args>
+void launch(la3_ctx* c, const char* name, void (*k)(KArgs...), uint32_t blocks, Args... args) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (blocks == 0) return;
+  if (c->profiling) { a = get_event(c); b = get_event(c); (void)hipEventRecord(a, c->stream); }
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(BLOCK), 0, c->stream, args...);
+  if (c->profiling) { (void)hipEventRecord(b, c->stream); c->evs.push_back({name, a, b}); }
+}
+
+uint32_t blocks_for(uint64_t threads, uint32_t cap = 8192) {
+  uint64_t b = (threads + BLOCK - 1) / BLOCK;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, cap));
+}
+
+// Starts the timing window of one API call.
+void timing_begin(la3_ctx* c) {
+  c->evs.clear();
+  c->edges_scanned = 0;
+}
+
+// Closes it (the stream must be synchronized): per-kernel totals.
+void timing_end(la3_ctx* c) {
+  la3_timing t{};
+  std::map<std::string, std::pair<double, uint64_t>> per;
+  for (auto& e : c->evs) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e.a, e.b);
+    auto& p = per[e.name];
+    p.first += ms;
+    p.second++;
+    t.total_ms += ms;
+    c->free_events.push_back(e.a);
+    c->free_events.push_back(e.b);
+  }
+  c->evs.clear();
+  for (auto& kv : per)
+    if (kv.second.first > t.dominant_ms) {
+      t.dominant_ms = kv.second.first;
+      t.dominant_launches = kv.second.second;
+      snprintf(t.dominant_kernel, sizeof(t.dominant_kernel), "%s", kv.first.c_str());
+    }
+  t.edges_scanned = c->edges_scanned;
+  c->last = t;
+}
+
+// Grow-only named device scratch.
+void* scratch(la3_ctx* c, const char* name, size_t bytes) {
+  DevBuf& b = c->scratch[name];
+  if (b.bytes >= bytes) return b.p;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+  size_t want = std::max<size_t>(bytes, 256);
+  if (hipMalloc(&b.p, want) != hipSuccess) { b.p = nullptr; return nullptr; }
+  b.bytes = want;
+  return b.p;
+}
+
+template <class T>
+la3_status upload(la3_graph* g, const std::vector<T>& v, const T** out) {
+  void* p = nullptr;
+  size_t bytes = std::max<size_t>(v.size() * sizeof(T), 4);
+  HIPC(g->ctx, hipMalloc(&p, bytes));
+  g->allocs.push_back(p);
+  g->device_bytes += bytes;
+  if (!v.empty()) HIPC(g->ctx, hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  *out = (const T*)p;
+  return LA3_OK;
+}
+
+la3_status graph_from_layout(la3_ctx* ctx, HostLayout& L, la3_graph** out) {
+  auto g = std::make_unique<la3_graph>();
+  g->ctx = ctx;
+  g->n = L.n; g->nslots = L.nslots; g->nreg = L.nreg; g->nsink = L.nsink; g->nsrc = L.nsrc;
+  g->niso = L.niso; g->ncols = L.ncols; g->nrows = L.nrows;
+  g->directed = L.directed; g->weighted = L.weighted; g->mode = L.mode; g->h = L.h;
+  g->nnz = L.nnz; g->nnz_reg = L.nnz_reg; g->nnz_sink = L.nnz_sink; g->nrecords = L.nrecords;
+  if (ctx->device < 0) {
+    g->hid_of = L.hid_of;
+    g->id_of = L.id_of;
+    g->slot_of_id = L.slot_of_id;
+    g->host = std::make_unique<HostLayout>(std::move(L));
+    *out = g.release();
+    return LA3_OK;
+  }
+  HIPC(ctx, hipSetDevice(ctx->device));
+  DG& d = g->dg;
+  d.nslots = L.nslots; d.nreg = L.nreg; d.nsink = L.nsink; d.nsrc = L.nsrc; d.ncols = L.ncols;
+  d.nrows = L.nrows; d.nnz = L.nnz; d.nlong = (uint32_t)L.long_rows.size();
+  la3_status s;
+  auto up = [&](const std::vector<uint32_t>& v, const uint32_t** o) { return upload(g.get(), v, o); };
+  if ((s = up(L.col_ptr, &d.col_ptr)) || (s = up(L.col_split, &d.col_split)) ||
+      (s = up(L.col_rows, &d.col_rows)) || (s = up(L.row_ptr, &d.row_ptr)) ||
+      (s = up(L.row_split, &d.row_split)) || (s = up(L.row_cols, &d.row_cols)) ||
+      (s = up(L.id_of, &d.id_of)) || (s = up(L.hid_of, &d.hid_of)) ||
+      (s = up(L.long_rows, &d.long_rows))) {
+    for (void* p : g->allocs) (void)hipFree(p);
+    return s;
+  }
+  if (L.weighted) {
+    if ((s = up(L.col_w, &d.col_w)) || (s = up(L.row_w, &d.row_w))) {
+      for (void* p : g->allocs) (void)hipFree(p);
+      return s;
+    }
+  }
+  g->hid_of.swap(L.hid_of);
+  g->id_of.swap(L.id_of);
+  g->slot_of_id.swap(L.slot_of_id);
+  *out = g.release();
+  return LA3_OK;
+}
+
+la3_status load_records(la3_ctx* ctx, const uint32_t* rec, uint64_t nrec, uint32_t n, int weighted,
+                        la3_mode mode, la3_hashing hashing, la3_graph** out) {
+  if (!ctx || !out || (nrec && !rec)) return fail(ctx, LA3_EINVAL, "null argument");
+  if ((int)mode < 0 || (int)mode > 4) return fail(ctx, LA3_EINVAL, "bad load mode");
+  if (n == 0xffffffffu) return fail(ctx, LA3_ENOTSUP, "nvertices must be < 2^32-1");
+  HostLayout L;
+  std::string e = build_layout(rec, nrec, n, weighted != 0, (int)mode, hashing != LA3_HASH_NONE, L);
+  if (!e.empty()) return fail(ctx, e.find("2^32") != std::string::npos ? LA3_ENOTSUP : LA3_EINVAL, e);
+  return graph_from_layout(ctx, L, out);
+}
+
+// Per-query frontier buffers (two, ping-ponged).
+struct Fronts {
+  Frontier f[2];
+  uint64_t cap;
+};
+
+la3_status get_fronts(la3_graph* g, Fronts& F, const char* tag) {
+  la3_ctx* c = g->ctx;
+  F.cap = (uint64_t)g->nslots + g->nnz / CHUNK + 2;
+  for (int i = 0; i < 2; i++) {
+    std::string n1 = std::string(tag) + ".items" + char('0' + i);
+    std::string n2 = std::string(tag) + ".ctr" + char('0' + i);
+    F.f[i].items = (uint64_t*)scratch(c, n1.c_str(), F.cap * sizeof(uint64_t));
+    uint8_t* ctr = (uint8_t*)scratch(c, n2.c_str(), 16);
+    if (!F.f[i].items || !ctr) return fail(c, LA3_ENOMEM, "frontier allocation failed");
+    F.f[i].ctr = (uint32_t*)ctr;
+    F.f[i].edges = (unsigned long long*)(ctr + 8);
+  }
+  return LA3_OK;
+}
+
+struct FrontCounts {
+  uint32_t items, verts;
+  unsigned long long edges;
+};
+
+la3_status read_counts(la3_ctx* c, const Frontier& f, FrontCounts& out) {
+  HIPC(c, hipMemcpyAsync(c->host_ctr, f.ctr, 16, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  std::memcpy(&out, c->host_ctr, 16);
+  return LA3_OK;
+}
+
+void zero_counts(la3_ctx* c, const Frontier& f) {
+  launch(c, "fill_u32", k_fill<uint32_t>, 1, f.ctr, (uint64_t)4, 0u);
+}
+
+}  // namespace
+
+// ================================================================== API ===
+extern "C" {
+
+la3_status la3_ctx_create(int device, la3_ctx** out) {
+  if (!out) return LA3_EINVAL;
+  auto c = std::make_unique<la3_ctx>();
+  c->device = device;
+  if (device == -1) { *out = c.release(); return LA3_OK; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return LA3_EDEVICE;
+  if (device < 0 || device >= ndev) return LA3_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return LA3_EDEVICE;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return LA3_EDEVICE;
+  if (hipHostMalloc(&c->host_ctr, 64, hipHostMallocDefault) != hipSuccess) return LA3_EDEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->ncu = prop.multiProcessorCount;
+  *out = c.release();
+  return LA3_OK;
+}
+
+void la3_ctx_destroy(la3_ctx* c) {
+  if (!c) return;
+  if (c->device < 0) { delete c; return; }
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->scratch) if (kv.second.p) (void)hipFree(kv.second.p);
+  for (auto& e : c->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+  for (auto e : c->free_events) (void)hipEventDestroy(e);
+  if (c->host_ctr) (void)hipHostFree(c->host_ctr);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* la3_last_error(const la3_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+la3_status la3_ctx_synchronize(la3_ctx* c) {
+  if (!c) return LA3_EINVAL;
+  if (c->device < 0) return LA3_OK;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LA3_OK;
+}
+
+la3_status la3_set_profiling(la3_ctx* c, int enabled) {
+  if (!c) return LA3_EINVAL;
+  c->profiling = enabled != 0;
+  return LA3_OK;
+}
+
+la3_status la3_last_timing(const la3_ctx* c, la3_timing* out) {
+  if (!c || !out) return LA3_EINVAL;
+  *out = c->last;
+  return LA3_OK;
+}
+
+la3_status la3_graph_load_records(la3_ctx* ctx, const uint32_t* records, uint64_t nrecords,
+                                  uint32_t nvertices, int weighted, la3_mode mode,
+                                  la3_hashing hashing, la3_graph** out) {
+  return load_records(ctx, records, nrecords, nvertices, weighted, mode, hashing, out);
+}
+
+la3_status la3_graph_load_file(la3_ctx* ctx, const char* path, uint32_t nvertices, int weighted,
+                               la3_mode mode, la3_hashing hashing, la3_graph** out) {
+  if (!ctx || !path || !out) return fail(ctx, LA3_EINVAL, "null argument");
+  int fd = open(path, O_RDONLY);
+  if (fd < 0) return fail(ctx, LA3_EIO, std::string("Unable to open input file: ") + path);
+  struct stat st;
+  if (fstat(fd, &st) != 0) { close(fd); return fail(ctx, LA3_EIO, "stat() failure"); }
+  const uint64_t size = (uint64_t)st.st_size;
+  const uint8_t* base = nullptr;
+  if (size > 0) {
+    void* m = mmap(nullptr, size, PROT_READ, MAP_PRIVATE, fd, 0);
+    if (m == MAP_FAILED) { close(fd); return fail(ctx, LA3_EIO, "mmap() failure"); }
+    base = (const uint8_t*)m;
+  }
+  close(fd);
+  uint64_t off = 0;
+  uint32_t n = nvertices;
+  if (n == 0) {
+    // 16-byte header {u32 nrows-1, u32 ncols-1, u64 nnz} (graph.hpp:207-234).
+    if (size < 16) { if (base) munmap((void*)base, size); return fail(ctx, LA3_EIO, "fread() failure: short header"); }
+    uint32_t hr, hc;
+    std::memcpy(&hr, base, 4);
+    std::memcpy(&hc, base + 4, 4);
+    n = std::max(hr, hc) + 1;
+    off = 16;
+  }
+  const uint64_t rsz = weighted ? 12 : 8;
+  const uint64_t nrec = (size - off) / rsz;
+  const uint32_t* rec = base ? (const uint32_t*)(base + off) : nullptr;
+  la3_status s = load_records(ctx, rec, nrec, n, weighted, mode, hashing, out);
+  if (base) munmap((void*)base, size);
+  return s;
+}
+
+void la3_graph_free(la3_graph* g) {
+  if (!g) return;
+  if (g->host) { delete g; return; }
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  for (void* p : g->allocs) (void)hipFree(p);
+  delete g;
+}
+
+la3_status la3_graph_info_get(const la3_graph* g, la3_graph_info* o) {
+  if (!g || !o) return LA3_EINVAL;
+  o->n = g->n; o->nslots = g->nslots; o->nregular = g->nreg; o->nsink = g->nsink;
+  o->nsource = g->nsrc; o->nisolated = g->niso; o->ncols = g->ncols; o->directed = g->directed;
+  o->weighted = g->weighted; o->mode = g->mode; o->nnz = g->nnz; o->nnz_regular = g->nnz_reg;
+  o->nnz_sink = g->nnz_sink; o->nrecords = g->nrecords; o->device_bytes = g->device_bytes;
+  return LA3_OK;
+}
+
+la3_status la3_graph_slots(const la3_graph* g, uint32_t* hid_of) {
+  if (!g || !hid_of) return LA3_EINVAL;
+  std::memcpy(hid_of, g->hid_of.data(), sizeof(uint32_t) * g->nslots);
+  return LA3_OK;
+}
+
+uint32_t la3_graph_hash(const la3_graph* g, uint32_t v) { return g ? (uint32_t)g->h.hash(v) : v; }
+uint32_t la3_graph_unhash(const la3_graph* g, uint32_t v) { return g ? (uint32_t)g->h.unhash(v) : v; }
+
+la3_status la3_graph_csc(const la3_graph* g, int which, uint32_t* colptrs, uint32_t* colidxs,
+                         uint32_t* global_idx, uint32_t* idx, uint32_t* weight) {
+  if (!g || !colptrs || !colidxs || (which != 0 && which != 1)) return LA3_EINVAL;
+  la3_ctx* c = g->ctx;
+  const uint32_t NC = g->ncols;
+  std::vector<uint32_t> cp, cs, rows, w;
+  if (g->host) {
+    cp = g->host->col_ptr; cs = g->host->col_split; rows = g->host->col_rows; w = g->host->col_w;
+  } else {
+    HIPC(c, hipSetDevice(c->device));
+    cp.resize(NC + 1); cs.resize(NC); rows.resize(g->nnz);
+    HIPC(c, hipMemcpy(cp.data(), g->dg.col_ptr, sizeof(uint32_t) * (NC + 1), hipMemcpyDeviceToHost));
+    HIPC(c, hipMemcpy(cs.data(), g->dg.col_split, sizeof(uint32_t) * NC, hipMemcpyDeviceToHost));
+    if (g->nnz) HIPC(c, hipMemcpy(rows.data(), g->dg.col_rows, sizeof(uint32_t) * g->nnz, hipMemcpyDeviceToHost));
+    if (g->weighted && g->nnz) {
+      w.resize(g->nnz);
+      HIPC(c, hipMemcpy(w.data(), g->dg.col_w, sizeof(uint32_t) * g->nnz, hipMemcpyDeviceToHost));
+    }
+  }
+  uint32_t k = 0;
+  for (uint32_t col = 0; col < NC; col++) {
+    const uint32_t b = which ? cs[col] : cp[col], e = which ? cp[col + 1] : cs[col];
+    colptrs[col] = k;
+    colidxs[col] = (e > b) ? g->hid_of[col < g->nreg ? col : col + g->nsink] : 0u;
+    for (uint32_t j = b; j < e; j++, k++) {
+      const uint32_t r = rows[j];
+      if (global_idx) global_idx[k] = which ? r - g->nreg : r;
+      if (idx) idx[k] = g->hid_of[r];
+      if (weight) weight[k] = g->weighted ? w[j] : 0u;
+    }
+  }
+  colptrs[NC] = k;
+  return LA3_OK;
+}
+
+// ------------------------------------------------------------- degree ---
+la3_status la3_degree(la3_graph* g, uint32_t* degree) {
+  if (!g || !degree) return LA3_EINVAL;
+  la3_ctx* c = g->ctx;
+  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+  HIPC(c, hipSetDevice(c->device));
+  timing_begin(c);
+  uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * g->nslots);
+  if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  launch(c, "degree", k_degree, blocks_for(g->nslots), g->dg, out);
+  HIPC(c, hipMemcpyAsync(degree, out, sizeof(uint32_t) * g->nslots, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  timing_end(c);
+  return LA3_OK;
+}
+
+// ---------------------------------------------------------------- BFS ---
+la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent, uint32_t* iters_run) {
+  if (!g || !hops) return LA3_EINVAL;
+  la3_ctx* c = g->ctx;
+  if (root > g->n) return fail(c, LA3_EINVAL, "root id out of range");
+  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+  HIPC(c, hipSetDevice(c->device));
+  timing_begin(c);
+  const uint32_t S = g->nslots;
+  uint8_t* level = (uint8_t*)scratch(c, "bfs.level", S);
+  uint32_t* mark = (uint32_t*)scratch(c, "bfs.mark", sizeof(uint32_t) * S);
+  Fronts F;
+  la3_status st = get_fronts(g, F, "bfs");
+  if (st) return st;
+  if (!level || !mark) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  const DG& d = g->dg;
+  const uint32_t rs = g->slot_of_id[root];
+  launch(c, "bfs_init", k_bfs_init, blocks_for(S), d, level, mark, rs, F.f[0]);
+  FrontCounts fc;
+  if ((st = read_counts(c, F.f[0], fc))) return st;
+  uint32_t iter = 0, cur = 0;
+  bool bottom_up = false;
+  uint64_t explored = fc.edges;
+  const uint32_t wave_blocks = 4 * c->ncu * 8;
+  // The reference stops after the first iteration that activates no regular
+  // vertex (vertex_program_execute.hpp:211-212); a regular vertex is exactly one
+  // that emits work items, so the loop runs while items exist.
+  while (fc.items > 0) {
+    const Frontier& nf = F.f[cur ^ 1];
+    zero_counts(c, nf);
+    if (!bottom_up) {
+      const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
+      if (fc.items) launch(c, "bfs_push", k_bfs_push, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, level, mark, iter, nf);
+      c->edges_scanned += fc.edges;
+    } else {
+      launch(c, "bfs_pull", k_bfs_pull, blocks_for(g->nrows, 4 * c->ncu * 8), d, level, mark, iter, nf);
+      if (d.nlong) launch(c, "bfs_pull_long", k_bfs_pull_long, std::min<uint32_t>((d.nlong + 3) / 4, wave_blocks), d, level, mark, iter, nf);
+    }
+    FrontCounts nc;
+    if ((st = read_counts(c, nf, nc))) return st;
+    if (bottom_up) c->edges_scanned += 0;  // pull steps read row lists; counted below
+    iter++;
+    explored += nc.edges;
+    const uint64_t unexplored = g->nnz > explored ? g->nnz - explored : 0;
+    if (!bottom_up && nc.edges > unexplored / 14 && nc.verts > 0) bottom_up = true;
+    else if (bottom_up && (uint64_t)nc.verts * 24 < S) bottom_up = false;
+    fc = nc;
+    cur ^= 1;
+  }
+  uint8_t* oh = (uint8_t*)scratch(c, "out.u8", S);
+  uint32_t* op = parent ? (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S) : nullptr;
+  if (!oh || (parent && !op)) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  launch(c, "bfs_output", k_bfs_output, blocks_for(S), d, (const uint8_t*)level, (const uint32_t*)mark, oh, op);
+  HIPC(c, hipMemcpyAsync(hops, oh, S, hipMemcpyDeviceToHost, c->stream));
+  if (parent) HIPC(c, hipMemcpyAsync(parent, op, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  timing_end(c);
+  if (iters_run) *iters_run = std::max<uint32_t>(iter, 1);
+  return LA3_OK;
+}
+
+// --------------------------------------------------------------- SSSP ---
+la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters_run) {
+  if (!g || !dist) return LA3_EINVAL;
+  la3_ctx* c = g->ctx;
+  if (root > g->n) return fail(c, LA3_EINVAL, "root id out of range");
+  if (!g->weighted) return fail(c, LA3_EINVAL, "SSSP needs a weighted graph");
+  if (!g->directed) return fail(c, LA3_EINVAL, "SSSP runs on a directed load");
+  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+  HIPC(c, hipSetDevice(c->device));
+  timing_begin(c);
+  const uint32_t S = g->nslots;
+  uint32_t* dv = (uint32_t*)scratch(c, "sssp.dist", sizeof(uint32_t) * S);
+  uint32_t* stamp = (uint32_t*)scratch(c, "sssp.stamp", sizeof(uint32_t) * S);
+  Fronts F;
+  la3_status st = get_fronts(g, F, "bfs");
+  if (st) return st;
+  if (!dv || !stamp) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  const DG& d = g->dg;
+  launch(c, "sssp_init", k_sssp_init, blocks_for(S), d, dv, stamp, g->slot_of_id[root], F.f[0]);
+  FrontCounts fc;
+  if ((st = read_counts(c, F.f[0], fc))) return st;
+  uint32_t round = 0, cur = 0;
+  const uint32_t wave_blocks = 4 * c->ncu * 8;
+  while (fc.items > 0) {
+    const Frontier& nf = F.f[cur ^ 1];
+    zero_counts(c, nf);
+    const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
+    launch(c, "sssp_push", k_sssp_push, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dv, stamp, round, nf);
+    c->edges_scanned += fc.edges;
+    if ((st = read_counts(c, nf, fc))) return st;
+    round++;
+    cur ^= 1;
+  }
+  if (g->nsink) launch(c, "sssp_sinks", k_sssp_sinks, blocks_for(g->nsink), d, dv);
+  uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
+  if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  launch(c, "scatter_u32", k_scatter_by_id<uint32_t>, blocks_for(S), d, (const uint32_t*)dv, out);
+  HIPC(c, hipMemcpyAsync(dist, out, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  timing_end(c);
+  if (iters_run) *iters_run = round + 1;
+  return LA3_OK;
+}
+
+// ----------------------------------------------------------- PageRank ---
+static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t seed_slot,
+                            double* rank, uint32_t* degree, uint32_t* iters_run) {
+  la3_ctx* c = g->ctx;
+  if (!g->directed) return fail(c, LA3_EINVAL, "PageRank runs on a directed load");
+  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+  HIPC(c, hipSetDevice(c->device));
+  timing_begin(c);
+  const uint32_t S = g->nslots;
+  const DG& d = g->dg;
+  double* x0 = (double*)scratch(c, "pr.x0", sizeof(double) * S);
+  double* x1 = (double*)scratch(c, "pr.x1", sizeof(double) * S);
+  double* rk = (double*)scratch(c, "pr.rank", sizeof(double) * S);
+  double* ss = (double*)scratch(c, "pr.srcsum", sizeof(double) * (g->nreg + 1));
+  uint32_t* deg = (uint32_t*)scratch(c, "pr.deg", sizeof(uint32_t) * S);
+  uint32_t* act = (uint32_t*)scratch(c, "pr.active", 64);
+  if (!x0 || !x1 || !rk || !ss || !deg || !act) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  PrArgs a{};
+  double* xa = x0;
+  double* xb = x1;
+  a.x = xa; a.xnext = xb; a.rank = rk; a.srcsum = ss; a.deg = deg; a.active = act;
+  a.alpha = alpha; a.damp = 1.0 - alpha; a.tol = 1e-5; a.seed = seed_slot;
+  launch(c, "outdeg", k_outdeg, blocks_for(S), d, deg);
+  launch(c, "pr_init", k_pr_init, blocks_for(S), d, a);
+  const uint32_t nlb = std::max<uint32_t>(1, std::min<uint32_t>((d.nlong + 3) / 4, 4096));
+  // Constant source contributions per regular row.
+  launch(c, "pr_srcsum", k_pr_rows<0>, blocks_for(g->nreg), d, a, 0u, g->nreg);
+  if (d.nlong) launch(c, "pr_srcsum_long", k_pr_long<0>, nlb, d, a, 0u, g->nreg);
+  const bool until = iters == 0;
+  uint32_t it = 0;
+  for (;;) {
+    if (!until && it >= iters) break;
+    if (until) launch(c, "fill_u32", k_fill<uint32_t>, 1, act, (uint64_t)1, 0u);
+    a.x = xa;
+    a.xnext = xb;
+    launch(c, "pr_rows", k_pr_rows<1>, blocks_for(g->nreg), d, a, 0u, g->nreg);
+    if (d.nlong) launch(c, "pr_rows_long", k_pr_long<1>, nlb, d, a, 0u, g->nreg);
+    std::swap(xa, xb);
+    it++;
+    if (until) {
+      uint32_t h = 0;
+      HIPC(c, hipMemcpyAsync(c->host_ctr, act, 4, hipMemcpyDeviceToHost, c->stream));
+      HIPC(c, hipStreamSynchronize(c->stream));
+      std::memcpy(&h, c->host_ctr, 4);
+      if (!h) break;
+    }
+  }
+  c->edges_scanned = (uint64_t)it * (g->nnz_reg) + g->nnz_sink;
+  // Sink step: xa now holds the final regular messages.
+  a.x = xa;
+  a.xnext = xb;
+  if (g->nsink) {
+    launch(c, "pr_sinks", k_pr_rows<2>, blocks_for(g->nsink), d, a, g->nreg, g->nrows);
+    if (d.nlong) launch(c, "pr_sinks_long", k_pr_long<2>, nlb, d, a, g->nreg, g->nrows);
+  }
+  double* outd = (double*)scratch(c, "out.f64", sizeof(double) * S);
+  if (!outd) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  launch(c, "scatter_f64", k_scatter_by_id<double>, blocks_for(S), d, (const double*)rk, outd);
+  HIPC(c, hipMemcpyAsync(rank, outd, sizeof(double) * S, hipMemcpyDeviceToHost, c->stream));
+  if (degree) {
+    uint32_t* outu = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
+    if (!outu) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+    launch(c, "scatter_u32", k_scatter_by_id<uint32_t>, blocks_for(S), d, (const uint32_t*)deg, outu);
+    HIPC(c, hipMemcpyAsync(degree, outu, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  timing_end(c);
+  if (iters_run) *iters_run = it;
+  return LA3_OK;
+}
+
+la3_status la3_pagerank(la3_graph* g, uint32_t iters, double* rank, uint32_t* degree,
+                        uint32_t* iters_run) {
+  if (!g || !rank) return LA3_EINVAL;
+  return pr_common(g, iters, 0.15, 0xffffffffu, rank, degree, iters_run);
+}
+
+la3_status la3_ppr(la3_graph* g, uint32_t seed, uint32_t iters, double alpha, double* rank) {
+  if (!g || !rank) return LA3_EINVAL;
+  if (seed > g->n) return fail(g->ctx, LA3_EINVAL, "seed id out of range");
+  if (!(alpha > 0.0 && alpha < 1.0)) return fail(g->ctx, LA3_EINVAL, "alpha must be in (0, 1)");
+  if (iters == 0) return fail(g->ctx, LA3_EINVAL, "PPR needs a fixed iteration count");
+  return pr_common(g, iters, alpha, g->slot_of_id[seed], rank, nullptr, nullptr);
+}
+
+// ----------------------------------------------------------------- CC ---
+la3_status la3_cc(la3_graph* g, uint32_t* label, uint32_t* iters_run) {
+  if (!g || !label) return LA3_EINVAL;
+  la3_ctx* c = g->ctx;
+  if (g->directed) return fail(c, LA3_EINVAL, "CC runs on an undirected load");
+  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+  HIPC(c, hipSetDevice(c->device));
+  timing_begin(c);
+  const uint32_t S = g->nslots;
+  const DG& d = g->dg;
+  uint32_t* p = (uint32_t*)scratch(c, "cc.parent", sizeof(uint32_t) * S);
+  uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
+  if (!p || !out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  launch(c, "cc_init", k_cc_init, blocks_for(S), d, p);
+  launch(c, "cc_hook", k_cc_hook, blocks_for(g->nrows), d, p);
+  if (d.nlong) launch(c, "cc_hook_long", k_cc_hook_long, std::min<uint32_t>((d.nlong + 3) / 4, 4096), d, p);
+  launch(c, "cc_output", k_cc_output, blocks_for(S), d, p, out);
+  c->edges_scanned = g->nnz;
+  HIPC(c, hipMemcpyAsync(label, out, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  timing_end(c);
+  if (iters_run) *iters_run = 1;
+  return LA3_OK;
+}
+
+// ----------------------------------------------------------------- TC ---
+la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
+  if (!g || !total) return LA3_EINVAL;
+  la3_ctx* c = g->ctx;
+  if (g->mode != LA3_ACYCLIC) return fail(c, LA3_EINVAL, "TC runs on an ACYCLIC load");
+  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+  HIPC(c, hipSetDevice(c->device));
+  timing_begin(c);
+  const uint32_t S = g->nslots;
+  const DG& d = g->dg;
+  uint32_t* row_of = (uint32_t*)scratch(c, "tc.row_of", sizeof(uint32_t) * (g->nnz + 1));
+  uint32_t* per = (uint32_t*)scratch(c, "tc.per", sizeof(uint32_t) * S);
+  unsigned long long* tot = (unsigned long long*)scratch(c, "tc.total", 64);
+  if (!row_of || !per || !tot) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(S), per, (uint64_t)S, 0u);
+  launch(c, "fill_u64", k_fill<unsigned long long>, 1, tot, (uint64_t)1, 0ull);
+  launch(c, "tc_rows", k_tc_entry_rows, blocks_for(g->nrows), d, row_of);
+  launch(c, "tc_count", k_tc_count, blocks_for(g->nnz, 4 * c->ncu * 8), d, (const uint32_t*)row_of, per, tot);
+  c->edges_scanned = g->nnz;
+  unsigned long long h = 0;
+  HIPC(c, hipMemcpyAsync(c->host_ctr, tot, 8, hipMemcpyDeviceToHost, c->stream));
+  if (per_vertex) {
+    uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
+    if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+    launch(c, "scatter_u32", k_scatter_by_id<uint32_t>, blocks_for(S), d, (const uint32_t*)per, out);
+    HIPC(c, hipMemcpyAsync(per_vertex, out, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  std::memcpy(&h, c->host_ctr, 8);
+  *total = h;
+  timing_end(c);
+  return LA3_OK;
+}
+
+// ------------------------------------------------------------ batched ---
+la3_status la3_bfs_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint8_t* hops,
+                         uint32_t* iters_run) {
+  (void)roots; (void)k; (void)hops; (void)iters_run;
+  return fail(g ? g->ctx : nullptr, LA3_ENOTSUP, "batched BFS not built yet");
+}
+la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint32_t* dist,
+                          uint32_t* iters_run) {
+  (void)roots; (void)k; (void)dist; (void)iters_run;
+  return fail(g ? g->ctx : nullptr, LA3_ENOTSUP, "batched SSSP not built yet");
+}
+la3_status la3_ppr_batch(la3_graph* g, const uint32_t* seeds, uint32_t k, uint32_t iters,
+                         float alpha, float* rank) {
+  (void)seeds; (void)k; (void)iters; (void)alpha; (void)rank;
+  return fail(g ? g->ctx : nullptr, LA3_ENOTSUP, "batched PPR not built yet");
+}
+
+// -------------------------------------------------------------- tools ---
+la3_status la3_rmat_generate(uint32_t scale, uint32_t edgefactor, double a, double b, double c,
+                             uint64_t seed, int weighted, uint32_t* out, uint64_t* nrecords) {
+  if (!nrecords || scale == 0 || scale > 31 || edgefactor == 0) return LA3_EINVAL;
+  if (!(a >= 0 && b >= 0 && c >= 0 && a + b + c <= 1.0)) return LA3_EINVAL;
+  // The generator is deterministic; the count query runs it once and caches
+  // the records for the immediately following fill call with the same args.
+  static std::vector<uint32_t> cache;
+  static std::string key;
+  char kb[160];
+  snprintf(kb, sizeof(kb), "%u/%u/%.17g/%.17g/%.17g/%llu/%d", scale, edgefactor, a, b, c,
+           (unsigned long long)seed, weighted);
+  if (key != kb) {
+    cache = rmat_generate(scale, edgefactor, a, b, c, seed, weighted != 0);
+    key = kb;
+  }
+  const uint64_t W = weighted ? 3 : 2;
+  *nrecords = cache.size() / W;
+  if (out) {
+    std::memcpy(out, cache.data(), cache.size() * sizeof(uint32_t));
+    std::vector<uint32_t>().swap(cache);
+    key.clear();
+  }
+  return LA3_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,547 @@
+// Device code. Included once, by capi.hip, so that launches and the
+// per-launch event profiler live in the same translation unit.
+//
+// Conventions for every kernel here:
+//  - blocks of 256 threads = 4 wavefronts of 64 lanes;
+//  - loops whose body contains a wave-collective (ballot / shfl / emit) have a
+//    wave-uniform trip count, so every lane reaches the collective;
+//  - no floating-point atomics: sums are per-row sequential or fixed-shape
+//    wavefront reductions, so results are run-to-run deterministic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "internal.h"
+#include "kernels.h"
+
+namespace la3 {
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (WAVE - 1); }
+__device__ __forceinline__ uint32_t col_of_slot(const DG& g, uint32_t s) {
+  return s < g.nreg ? s : s - g.nsink;
+}
+__device__ __forceinline__ uint32_t slot_of_col(const DG& g, uint32_t c) {
+  return c < g.nreg ? c : c + g.nsink;
+}
+__device__ __forceinline__ bool slot_has_col(const DG& g, uint32_t s) {
+  return s < g.nreg || (s >= g.nrows && s < g.nrows + g.nsrc);
+}
+// Position of column c in the reference's per-row message order: source
+// columns first, then regular columns (vertex_program_execute.hpp:477-482).
+__device__ __forceinline__ uint32_t msg_key(const DG& g, uint32_t c) {
+  return c >= g.nreg ? c - g.nreg : c + g.nsrc;
+}
+__device__ __forceinline__ uint32_t col_of_key(const DG& g, uint32_t k) {
+  return k < g.nsrc ? k + g.nreg : k - g.nsrc;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    uint32_t t = __shfl_up(v, d, WAVE);
+    if (lane >= (uint32_t)d) v += t;
+  }
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int d = WAVE / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, WAVE);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Traversal frontier bookkeeping, one set per query.
+struct Frontier {
+  uint64_t* items;         // work items: (chunk << 32) | slot
+  uint32_t* ctr;           // [0] items emitted, [1] vertices emitted
+  unsigned long long* edges;  // [0] entries behind the emitted items
+};
+
+// Wave-aggregated append of `nit` work items for `slot` (nit may be 0).
+// Every lane of the wave must call this.
+__device__ __forceinline__ void wave_emit(const Frontier& f, uint32_t slot, uint32_t nit,
+                                          uint32_t nedges, bool counted) {
+  uint32_t incl = wave_incl_scan(nit);
+  uint32_t total = __shfl(incl, WAVE - 1, WAVE);
+  uint64_t b = __ballot(counted);
+  uint32_t e = wave_sum(nedges);
+  if (b == 0) return;
+  uint32_t base = 0;
+  if (lane_id() == WAVE - 1) {
+    if (total) base = atomicAdd(&f.ctr[0], total);
+    atomicAdd(&f.ctr[1], (uint32_t)__popcll(b));
+    atomicAdd(&f.edges[0], (unsigned long long)e);
+  }
+  base = __shfl(base, WAVE - 1, WAVE);
+  uint32_t o = base + incl - nit;
+  for (uint32_t k = 0; k < nit; k++) f.items[o + k] = ((uint64_t)k << 32) | slot;
+}
+
+// ------------------------------------------------------------- utilities ---
+template <class T>
+__global__ void k_fill(T* p, uint64_t n, T v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// out[id_of[s]] = f(v[s]) for every slot.
+template <class T>
+__global__ void k_scatter_by_id(DG g, const T* v, T* out) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x)
+    out[g.id_of[s]] = v[s];
+}
+
+// Degree: in-degree of the stored graph (DegVertex, degree.h:31-34).
+__global__ void k_degree(DG g, uint32_t* out_by_id) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x)
+    out_by_id[g.id_of[s]] = s < g.nrows ? g.row_ptr[s + 1] - g.row_ptr[s] : 0u;
+}
+
+// Out-degree of each slot = length of its column (regular + sink rows).
+__global__ void k_outdeg(DG g, uint32_t* deg) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
+    uint32_t d = 0;
+    if (slot_has_col(g, s)) { uint32_t c = col_of_slot(g, s); d = g.col_ptr[c + 1] - g.col_ptr[c]; }
+    deg[s] = d;
+  }
+}
+
+// ------------------------------------------------------------------- BFS ---
+// BfsVertex (bfs.h:27-49) on the non-optimized loop (vertex_program_execute.hpp:166-248):
+// regular and sink rows every iteration. level[s] = iteration of discovery
+// (u8, 255 = unreached). mark[s] = 1 + message-order key of the winning
+// parent column (0 = unvisited, ~0 = root): among the frontier in-neighbours
+// of a newly reached row, the reference keeps the last one combined
+// (combine: y2 = y1, bfs.h:40), i.e. the largest message-order key.
+
+__global__ void k_bfs_init(DG g, uint8_t* level, uint32_t* mark, uint32_t root, Frontier f) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
+    level[s] = (uint8_t)BFS_INF;
+    mark[s] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    level[root] = 0;
+    mark[root] = 0xffffffffu;
+    uint32_t nit = 0, ne = 0;
+    if (slot_has_col(g, root)) {
+      uint32_t c = col_of_slot(g, root);
+      ne = g.col_ptr[c + 1] - g.col_ptr[c];
+      nit = (ne + CHUNK - 1) / CHUNK;
+    }
+    for (uint32_t k = 0; k < nit; k++) f.items[k] = ((uint64_t)k << 32) | root;
+    f.ctr[0] = nit;
+    f.ctr[1] = 1;
+    f.edges[0] = ne;
+  }
+}
+
+// Top-down (push) step over DCSC columns of the frontier; one work item
+// (<= CHUNK entries of one column) per wavefront iteration.
+__global__ void __launch_bounds__(BLOCK) k_bfs_push(DG g, const uint64_t* items, uint32_t nitems,
+                                                    uint8_t* level, uint32_t* mark, uint32_t iter,
+                                                    Frontier nf) {
+  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  const uint8_t nxt = (uint8_t)(iter + 1);
+  for (uint32_t it = wid; it < nitems; it += nwaves) {
+    const uint64_t item = items[it];
+    const uint32_t s = (uint32_t)item, ch = (uint32_t)(item >> 32);
+    const uint32_t c = col_of_slot(g, s);
+    const uint32_t beg = g.col_ptr[c] + ch * CHUNK;
+    const uint32_t end = min(beg + CHUNK, g.col_ptr[c + 1]);
+    const uint32_t key1 = msg_key(g, c) + 1;
+    for (uint32_t e0 = beg; e0 < end; e0 += WAVE) {
+      const uint32_t e = e0 + lane_id();
+      bool disc = false;
+      uint32_t r = 0;
+      if (e < end) {
+        r = g.col_rows[e];
+        uint8_t lv = level[r];
+        if (lv == (uint8_t)BFS_INF || lv == nxt) {
+          uint32_t old = atomicMax(&mark[r], key1);
+          if (old == 0) { level[r] = nxt; disc = true; }
+        }
+      }
+      uint32_t nit = 0, ne = 0;
+      if (disc && slot_has_col(g, r)) {
+        uint32_t cr = col_of_slot(g, r);
+        ne = g.col_ptr[cr + 1] - g.col_ptr[cr];
+        nit = (ne + CHUNK - 1) / CHUNK;
+      }
+      wave_emit(nf, r, nit, ne, disc);
+    }
+  }
+}
+
+// Bottom-up (pull) step: each unreached row scans its in-entries from the
+// end of the reference's message order; the first frontier member found is
+// the last writer. Rows longer than LONG_ROW are left to k_bfs_pull_long.
+__global__ void __launch_bounds__(BLOCK) k_bfs_pull(DG g, uint8_t* level, uint32_t* mark,
+                                                    uint32_t iter, Frontier nf) {
+  const uint8_t cur = (uint8_t)iter, nxt = (uint8_t)(iter + 1);
+  const uint32_t stride = gridDim.x * BLOCK;
+  for (uint32_t base = blockIdx.x * BLOCK; base < g.nrows; base += stride) {
+    const uint32_t r = base + threadIdx.x;
+    bool disc = false;
+    if (r < g.nrows && level[r] == (uint8_t)BFS_INF) {
+      const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1];
+      if (e - b <= LONG_ROW) {
+        for (uint32_t j = e; j > b; j--) {
+          uint32_t src = g.row_cols[j - 1];
+          if (level[src] == cur) { mark[r] = msg_key(g, col_of_slot(g, src)) + 1; disc = true; break; }
+        }
+      }
+    }
+    uint32_t nit = 0, ne = 0;
+    if (disc) {
+      level[r] = nxt;
+      if (slot_has_col(g, r)) {
+        uint32_t cr = col_of_slot(g, r);
+        ne = g.col_ptr[cr + 1] - g.col_ptr[cr];
+        nit = (ne + CHUNK - 1) / CHUNK;
+      }
+    }
+    wave_emit(nf, r, nit, ne, disc);
+  }
+}
+
+// Bottom-up for long rows: one wavefront per row, scanning 64 entries at a
+// time from the end; the highest lane with a frontier member wins.
+__global__ void __launch_bounds__(BLOCK) k_bfs_pull_long(DG g, uint8_t* level, uint32_t* mark,
+                                                         uint32_t iter, Frontier nf) {
+  const uint8_t cur = (uint8_t)iter, nxt = (uint8_t)(iter + 1);
+  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  for (uint32_t i = wid; i < g.nlong; i += nwaves) {
+    const uint32_t r = g.long_rows[i];
+    if (level[r] != (uint8_t)BFS_INF) continue;  // wave-uniform
+    const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1];
+    bool found = false;
+    for (uint32_t top = e; top > b && !found; top = (top - b > WAVE) ? top - WAVE : b) {
+      const uint32_t lo = (top - b > WAVE) ? top - WAVE : b;
+      const uint32_t j = lo + lane_id();
+      bool hit = false;
+      uint32_t src = 0;
+      if (j < top) { src = g.row_cols[j]; hit = level[src] == cur; }
+      uint64_t m = __ballot(hit);
+      if (m) {
+        const uint32_t win = 63 - __clzll(m);
+        if (lane_id() == win) mark[r] = msg_key(g, col_of_slot(g, src)) + 1;
+        found = true;
+      }
+    }
+    uint32_t nit = 0, ne = 0;
+    bool disc = found && lane_id() == 0;
+    if (disc) {
+      level[r] = nxt;
+      if (slot_has_col(g, r)) {
+        uint32_t cr = col_of_slot(g, r);
+        ne = g.col_ptr[cr + 1] - g.col_ptr[cr];
+        nit = (ne + CHUNK - 1) / CHUNK;
+      }
+    }
+    wave_emit(nf, r, nit, ne, disc);
+  }
+}
+
+// hops / parent by original id. parent = hashed id of the winning column's
+// vertex (Edge.src = csc.colidxs, vertex_program_execute.hpp:593).
+__global__ void k_bfs_output(DG g, const uint8_t* level, const uint32_t* mark, uint8_t* hops,
+                             uint32_t* parent) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
+    const uint32_t id = g.id_of[s];
+    hops[id] = level[s];
+    if (parent) {
+      uint32_t m = mark[s], p = 0;
+      if (m != 0 && m != 0xffffffffu) p = g.hid_of[slot_of_col(g, col_of_key(g, m - 1))];
+      parent[id] = p;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ SSSP ---
+// SpVertex (sssp.h:26-46) on the optimized loop (vertex_program_execute.hpp:55-163):
+// the loop relaxes regular rows only, then one pull pass computes the sinks.
+// Relaxations are asynchronous (Bellman-Ford with a deduplicated work
+// queue); min-plus distances are unique, so the result equals the
+// reference's.
+
+__global__ void k_sssp_init(DG g, uint32_t* dist, uint32_t* stamp, uint32_t root, Frontier f) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
+    dist[s] = SSSP_INF;
+    stamp[s] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    dist[root] = 0;
+    uint32_t nit = 0, ne = 0;
+    if (slot_has_col(g, root)) {
+      uint32_t c = col_of_slot(g, root);
+      ne = g.col_split[c] - g.col_ptr[c];
+      nit = (ne + CHUNK - 1) / CHUNK;
+    }
+    for (uint32_t k = 0; k < nit; k++) f.items[k] = ((uint64_t)k << 32) | root;
+    f.ctr[0] = nit;
+    f.ctr[1] = 1;
+    f.edges[0] = ne;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_sssp_push(DG g, const uint64_t* items, uint32_t nitems,
+                                                     uint32_t* dist, uint32_t* stamp, uint32_t round,
+                                                     Frontier nf) {
+  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  const uint32_t tag = round + 1;
+  for (uint32_t it = wid; it < nitems; it += nwaves) {
+    const uint64_t item = items[it];
+    const uint32_t s = (uint32_t)item, ch = (uint32_t)(item >> 32);
+    const uint32_t c = col_of_slot(g, s);
+    const uint32_t beg = g.col_ptr[c] + ch * CHUNK;
+    const uint32_t end = min(beg + CHUNK, g.col_split[c]);
+    const uint32_t du = ld_relaxed(&dist[s]);
+    for (uint32_t e0 = beg; e0 < end; e0 += WAVE) {
+      const uint32_t e = e0 + lane_id();
+      bool push = false;
+      uint32_t r = 0;
+      if (e < end) {
+        r = g.col_rows[e];
+        const uint32_t nd = du + g.col_w[e];
+        if (nd < ld_relaxed(&dist[r])) {
+          uint32_t old = atomicMin(&dist[r], nd);
+          if (nd < old && atomicMax(&stamp[r], tag) < tag) push = true;
+        }
+      }
+      uint32_t nit = 0, ne = 0;
+      if (push) {  // r is a regular row, hence a regular column with c == r
+        ne = g.col_split[r] - g.col_ptr[r];
+        nit = (ne + CHUNK - 1) / CHUNK;
+      }
+      wave_emit(nf, r, nit, ne, push);
+    }
+  }
+}
+
+// Sink step (vertex_program_execute.hpp:131-148): every sink row takes the
+// min over all its in-entries of dist + w (regular senders with their final
+// distance; source senders other than the root are at INF and add nothing).
+__global__ void k_sssp_sinks(DG g, uint32_t* dist) {
+  const uint32_t n = g.nsink;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t r = g.nreg + i;
+    uint32_t y = SSSP_INF;
+    for (uint32_t j = g.row_ptr[r]; j < g.row_ptr[r + 1]; j++)
+      y = min(y, dist[g.row_cols[j]] + g.row_w[j]);
+    dist[r] = min(dist[r], y);
+  }
+}
+
+// -------------------------------------------------------------- PageRank ---
+// PrVertex (pr.h:34-45) / PPR: rank' = base(r) + (1 - alpha) * y with
+// y = sum of x over the row's in-entries, x = rank / out-degree.
+// base(r) = alpha for PR (seed == ~0), alpha at the seed only for PPR.
+// Rows with <= LONG_ROW entries: one lane per row, sequential sum in the
+// reference's message order (sources first). Longer rows: one wavefront,
+// lane-strided partial sums and a fixed butterfly.
+struct PrArgs {
+  const double* x;       // messages, by slot
+  double* xnext;         // next messages (regular rows), by slot
+  double* rank;          // by slot
+  double* srcsum;        // per regular row: sum over source entries (constant)
+  const uint32_t* deg;   // out-degree by slot
+  uint32_t* active;      // set when some |delta| > tol
+  double alpha, damp, tol;
+  uint32_t seed;         // slot of the PPR seed, ~0 for PR
+};
+
+__device__ __forceinline__ double pr_base(const PrArgs& a, uint32_t r) {
+  return (a.seed == 0xffffffffu || a.seed == r) ? a.alpha : 0.0;
+}
+
+// mode 0: srcsum[r] = sum over source entries (regular rows)
+// mode 1: loop step for regular rows
+// mode 2: sink step for sink rows
+template <int MODE>
+__device__ __forceinline__ void pr_finish(const DG& g, const PrArgs& a, uint32_t r, double y) {
+  if (MODE == 0) { a.srcsum[r] = y; return; }
+  const double nr = __dadd_rn(pr_base(a, r), __dmul_rn(a.damp, y));
+  if (MODE == 1) {
+    const double old = a.rank[r];
+    a.rank[r] = nr;
+    a.xnext[r] = nr / (double)a.deg[r];
+    if (fabs(nr - old) > a.tol) a.active[0] = 1;
+  } else {
+    a.rank[r] = nr;
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(BLOCK) k_pr_rows(DG g, PrArgs a, uint32_t r0, uint32_t r1) {
+  for (uint32_t r = r0 + blockIdx.x * blockDim.x + threadIdx.x; r < r1; r += gridDim.x * blockDim.x) {
+    const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1], sp = g.row_split[r];
+    if (e - b > LONG_ROW) continue;
+    double y;
+    if (MODE == 0) {
+      y = 0.0;
+      for (uint32_t j = b; j < sp; j++) y = __dadd_rn(y, a.x[g.row_cols[j]]);
+    } else if (MODE == 1) {
+      y = a.srcsum[r];
+      for (uint32_t j = sp; j < e; j++) y = __dadd_rn(y, a.x[g.row_cols[j]]);
+    } else {
+      y = 0.0;
+      for (uint32_t j = b; j < e; j++) y = __dadd_rn(y, a.x[g.row_cols[j]]);
+    }
+    pr_finish<MODE>(g, a, r, y);
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(BLOCK) k_pr_long(DG g, PrArgs a, uint32_t r0, uint32_t r1) {
+  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  for (uint32_t i = wid; i < g.nlong; i += nwaves) {
+    const uint32_t r = g.long_rows[i];
+    if (r < r0 || r >= r1) continue;
+    const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1], sp = g.row_split[r];
+    const uint32_t lo = MODE == 1 ? sp : b, hi = MODE == 0 ? sp : e;
+    double p = 0.0;
+    for (uint32_t j = lo + lane_id(); j < hi; j += WAVE) p = __dadd_rn(p, a.x[g.row_cols[j]]);
+    p = wave_sum(p);
+    if (lane_id() == 0) pr_finish<MODE>(g, a, r, MODE == 1 ? __dadd_rn(a.srcsum[r], p) : p);
+  }
+}
+
+// Messages of the initial scatter: x = rank / degree for every vertex with a
+// column (regular and source); pr.h:37.
+__global__ void k_pr_init(DG g, PrArgs a) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
+    const double r0 = pr_base(a, s);
+    a.rank[s] = r0;
+    double x = 0.0;
+    if (slot_has_col(g, s)) x = r0 / (double)a.deg[s];
+    ((double*)a.x)[s] = x;
+    a.xnext[s] = x;
+  }
+}
+
+// -------------------------------------------------------------------- CC ---
+// CcVertex (cc.h:22-38) computes, per connected component, the minimum
+// original id. Here: concurrent union-find over slots where a root is only
+// ever hooked under a root with a smaller original id, so the surviving root
+// of each component is its minimum-id vertex.
+__device__ __forceinline__ uint32_t uf_find(uint32_t* p, uint32_t x) {
+  for (;;) {
+    const uint32_t y = ld_relaxed(&p[x]);
+    if (y == x) return x;
+    const uint32_t z = ld_relaxed(&p[y]);
+    if (z == y) return y;
+    atomicCAS(&p[x], y, z);  // path halving
+    x = z;
+  }
+}
+
+__device__ __forceinline__ void uf_union(const DG& g, uint32_t* p, uint32_t a, uint32_t b) {
+  for (;;) {
+    a = uf_find(p, a);
+    b = uf_find(p, b);
+    if (a == b) return;
+    if (g.id_of[a] < g.id_of[b]) { uint32_t t = a; a = b; b = t; }
+    if (atomicCAS(&p[a], a, b) == a) return;
+  }
+}
+
+__global__ void k_cc_init(DG g, uint32_t* p) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x)
+    p[s] = s;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cc_hook(DG g, uint32_t* p) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < g.nrows; r += gridDim.x * blockDim.x) {
+    const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1];
+    if (e - b > LONG_ROW) continue;
+    for (uint32_t j = b; j < e; j++) {
+      const uint32_t s = g.row_cols[j];
+      if (s < r) uf_union(g, p, r, s);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cc_hook_long(DG g, uint32_t* p) {
+  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  for (uint32_t i = wid; i < g.nlong; i += nwaves) {
+    const uint32_t r = g.long_rows[i];
+    for (uint32_t j = g.row_ptr[r] + lane_id(); j < g.row_ptr[r + 1]; j += WAVE) {
+      const uint32_t s = g.row_cols[j];
+      if (s < r) uf_union(g, p, r, s);
+    }
+  }
+}
+
+__global__ void k_cc_output(DG g, uint32_t* p, uint32_t* label_by_id) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x)
+    label_by_id[g.id_of[s]] = g.id_of[uf_find(p, s)];
+}
+
+// -------------------------------------------------------------------- TC ---
+// GnVertex + CtVertex (tc.h:47-99) on the acyclic load (row = higher id,
+// column = lower id): row u's in-entries are N-(u). For every entry (u, v)
+// the reference adds |N-(u) n N-(v)| to v (tc.h:84-96). Both lists are sorted
+// by message-order key, so one list is binary-searched in the other.
+__device__ __forceinline__ uint32_t slot_key(const DG& g, uint32_t s) {
+  return msg_key(g, col_of_slot(g, s));
+}
+
+__device__ uint32_t tc_intersect(const DG& g, uint32_t ab, uint32_t ae, uint32_t bb, uint32_t be) {
+  if (ae - ab > be - bb) { uint32_t t = ab; ab = bb; bb = t; t = ae; ae = be; be = t; }
+  uint32_t cnt = 0, lo = bb;
+  for (uint32_t i = ab; i < ae && lo < be; i++) {
+    const uint32_t k = slot_key(g, g.row_cols[i]);
+    uint32_t l = lo, h = be;
+    while (l < h) {
+      const uint32_t m = (l + h) >> 1;
+      if (slot_key(g, g.row_cols[m]) < k) l = m + 1; else h = m;
+    }
+    if (l < be && slot_key(g, g.row_cols[l]) == k) { cnt++; l++; }
+    lo = l;
+  }
+  return cnt;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_tc_entry_rows(DG g, uint32_t* row_of) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < g.nrows; r += gridDim.x * blockDim.x)
+    for (uint32_t j = g.row_ptr[r]; j < g.row_ptr[r + 1]; j++) row_of[j] = r;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_tc_count(DG g, const uint32_t* row_of, uint32_t* per_slot,
+                                                    unsigned long long* total) {
+  unsigned long long acc = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < g.nnz; base += stride) {
+    const uint64_t j = base + threadIdx.x;
+    if (j < g.nnz) {
+      const uint32_t u = row_of[j], v = g.row_cols[j];
+      if (v < g.nrows) {
+        const uint32_t c = tc_intersect(g, g.row_ptr[u], g.row_ptr[u + 1], g.row_ptr[v], g.row_ptr[v + 1]);
+        if (c) { atomicAdd(&per_slot[v], c); acc += c; }
+      }
+    }
+  }
+  acc = wave_sum(acc);
+  if (lane_id() == 0 && acc) atomicAdd(total, acc);
+}
+
+}  // namespace la3

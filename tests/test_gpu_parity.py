@@ -1,0 +1,209 @@
+"""HIP path vs the oracle on the same seeded inputs, through the C ABI.
+
+Bar: bit-exact for degree, BFS levels and parents, SSSP distances, CC labels
+and triangle counts; FP64 PageRank / PPR within L1 <= 1e-6 of the oracle
+(the reference's own tolerance for PR) and, tighter, max-abs <= 1e-12, since
+only summation grouping on rows longer than 64 entries differs.
+"""
+import numpy as np
+import pytest
+
+import la3_amd as L
+from test_oracle import random_records
+
+pytestmark = pytest.mark.gpu
+
+PR_L1_TOL = 1e-6
+
+
+def _both(ctx, oracle, rec, n, weighted, mode):
+    return (L.Graph.from_records(ctx, rec, n, weighted, mode),
+            oracle.OracleGraph(rec, n, weighted, mode))
+
+
+def _rmat(scale, seed, weighted=False):
+    return L.rmat(scale, 16, seed=seed, weighted=weighted)
+
+
+def _roots(rec, count, seed):
+    rng = np.random.default_rng(seed)
+    src = np.unique(rec[:, 0])
+    return [int(x) for x in rng.choice(src, size=min(count, src.size), replace=False)]
+
+
+# ------------------------------------------------------------------- g1 ---
+def test_g1_all_programs(gpu_ctx, oracle, g1):
+    rec, recw, n = g1["records"], g1["records_w"], 8
+    g = L.Graph.from_records(gpu_ctx, rec, n, False, L.DIRECTED)
+    rank, deg, it = L.pagerank(g, 20)
+    for k, v in g1["pr_20"].items():
+        assert abs(rank[int(k)] - v) < 5e-10
+    assert it == 20
+    for k, v in g1["out_degree"].items():
+        assert deg[int(k)] == v
+    gu = L.Graph.from_records(gpu_ctx, rec, n, False, L.UNDIRECTED)
+    hops, par, it = L.bfs(gu, 5)
+    for k, v in g1["bfs_undirected_root5_hops"].items():
+        assert hops[int(k)] == v
+    for k, v in g1["bfs_undirected_root5_parent"].items():
+        assert par[int(k)] == v
+    assert it == 3
+    lab, _ = L.cc(gu)
+    for k, v in g1["cc_labels"].items():
+        assert lab[int(k)] == v
+    gw = L.Graph.from_records(gpu_ctx, recw, n, True, L.DIRECTED)
+    dist, _ = L.sssp(gw, 5)
+    assert int(dist.astype(np.int64).sum()) == g1["sssp_checksum"]
+    for k, v in g1["sssp_root5"].items():
+        assert dist[int(k)] == v
+    ga = L.Graph.from_records(gpu_ctx, rec, n, False, L.ACYCLIC)
+    total, per = L.tc(ga)
+    assert total == g1["tc_total"]
+    oa = oracle.OracleGraph(rec, n, False, oracle.ACYCLIC)
+    ora = oracle.OracleGraph(rec, n, False, oracle.REVERSED_ACYCLIC)
+    assert (per == oracle.tc(oa, ora)[1]).all()
+    gd = L.Graph.from_records(gpu_ctx, rec, n, False, L.DIRECTED)
+    for k, v in g1["in_degree"].items():
+        assert L.degree(gd)[int(k)] == v
+
+
+# ------------------------------------------------------------ BFS / SSSP ---
+@pytest.mark.parametrize("scale,seed", [(10, 1), (14, 2), (16, 3)])
+@pytest.mark.parametrize("mode", [L.DIRECTED, L.UNDIRECTED])
+def test_bfs_levels_and_parents(gpu_ctx, oracle, scale, seed, mode):
+    rec = _rmat(scale, seed)
+    n = 1 << scale
+    g, o = _both(gpu_ctx, oracle, rec, n, False, mode)
+    for root in _roots(rec, 4, seed) + [n, 0]:
+        hops, par, it = L.bfs(g, root)
+        eh, ep, eit = oracle.bfs(o, root)
+        assert (hops == eh).all(), f"levels differ, root {root}"
+        assert (par == ep).all(), f"parents differ, root {root}"
+        assert it == eit
+
+
+def test_bfs_random_with_duplicates(gpu_ctx, oracle):
+    for seed in (21, 22):
+        rec = random_records(seed, 3000, 20000)
+        for mode in (L.DIRECTED, L.UNDIRECTED, L.REVERSED):
+            g, o = _both(gpu_ctx, oracle, rec, 3000, False, mode)
+            for root in (1, 2, 3, 1500, 2999):
+                hops, par, it = L.bfs(g, root)
+                eh, ep, eit = oracle.bfs(o, root)
+                assert (hops == eh).all() and (par == ep).all() and it == eit
+
+
+@pytest.mark.parametrize("scale,seed", [(10, 4), (14, 5), (16, 6)])
+def test_sssp_distances(gpu_ctx, oracle, scale, seed):
+    rec = _rmat(scale, seed, weighted=True)
+    n = 1 << scale
+    g, o = _both(gpu_ctx, oracle, rec, n, True, L.DIRECTED)
+    for root in _roots(rec, 4, seed) + [n]:
+        d, _ = L.sssp(g, root)
+        e, _ = oracle.sssp(o, root)
+        assert (d == e).all(), f"distances differ, root {root}"
+
+
+def test_sssp_random_with_duplicates(gpu_ctx, oracle):
+    rec = random_records(23, 3000, 20000, weighted=True)
+    g, o = _both(gpu_ctx, oracle, rec, 3000, True, L.DIRECTED)
+    for root in (1, 2, 3, 100, 2999):
+        assert (L.sssp(g, root)[0] == oracle.sssp(o, root)[0]).all()
+
+
+# -------------------------------------------------------------- CC / TC ---
+@pytest.mark.parametrize("scale,seed", [(10, 7), (16, 8)])
+def test_cc_labels(gpu_ctx, oracle, scale, seed):
+    rec = _rmat(scale, seed)
+    n = 1 << scale
+    g, o = _both(gpu_ctx, oracle, rec, n, False, L.UNDIRECTED)
+    lab, _ = L.cc(g)
+    assert (lab == oracle.cc(o)[0]).all()
+
+
+@pytest.mark.parametrize("scale,seed", [(10, 9), (14, 10)])
+def test_tc_counts(gpu_ctx, oracle, scale, seed):
+    rec = _rmat(scale, seed)
+    n = 1 << scale
+    g = L.Graph.from_records(gpu_ctx, rec, n, False, L.ACYCLIC)
+    oa = oracle.OracleGraph(rec, n, False, oracle.ACYCLIC)
+    ora = oracle.OracleGraph(rec, n, False, oracle.REVERSED_ACYCLIC)
+    total, per = L.tc(g)
+    etotal, eper = oracle.tc(oa, ora)
+    assert total == etotal
+    assert (per == eper).all()
+
+
+# ---------------------------------------------------------- PR / degree ---
+def test_pagerank_c1_config(gpu_ctx, oracle):
+    """PageRank, 20 iterations, on the scale-16 seed-1 R-MAT graph."""
+    rec = _rmat(16, 1)
+    n = 1 << 16
+    g = L.Graph.from_records(gpu_ctx, rec, n, False, L.DIRECTED)
+    o = oracle.OracleGraph(rec, n, False, oracle.DIRECTED)
+    orr = oracle.OracleGraph(rec, n, False, oracle.REVERSED)
+    rank, deg, it = L.pagerank(g, 20)
+    erank, edeg, eit = oracle.pagerank(o, orr, 20)
+    l1 = float(np.abs(rank - erank).sum())
+    print(f"PR C1 L1 vs oracle = {l1:.3e}, max abs = {np.abs(rank - erank).max():.3e}")
+    assert l1 <= PR_L1_TOL
+    assert np.abs(rank - erank).max() <= 1e-12
+    assert (deg == edeg).all() and it == eit == 20
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_pagerank_until_convergence(gpu_ctx, oracle, seed):
+    rec = _rmat(12, seed)
+    n = 1 << 12
+    g = L.Graph.from_records(gpu_ctx, rec, n, False, L.DIRECTED)
+    rank, _, it = L.pagerank(g, 0)
+    erank, _, eit = oracle.pagerank(oracle.OracleGraph(rec, n, False, 0),
+                                    oracle.OracleGraph(rec, n, False, 1), 0)
+    assert it == eit
+    assert float(np.abs(rank - erank).sum()) <= PR_L1_TOL
+
+
+def test_ppr_fp64(gpu_ctx, oracle):
+    rec = _rmat(12, 13)
+    n = 1 << 12
+    g = L.Graph.from_records(gpu_ctx, rec, n, False, L.DIRECTED)
+    o = oracle.OracleGraph(rec, n, False, 0)
+    orr = oracle.OracleGraph(rec, n, False, 1)
+    for s in _roots(rec, 3, 13):
+        r = L.ppr(g, s, 30)
+        e, _ = oracle.ppr(o, orr, s, 30)
+        assert float(np.abs(r - e).sum()) <= 1e-12
+
+
+@pytest.mark.parametrize("mode", [L.DIRECTED, L.REVERSED, L.UNDIRECTED])
+def test_degree(gpu_ctx, oracle, mode):
+    rec = _rmat(14, 14)
+    n = 1 << 14
+    g, o = _both(gpu_ctx, oracle, rec, n, False, mode)
+    assert (L.degree(g) == oracle.degree(o)).all()
+
+
+# --------------------------------------------------------------- edges ---
+def test_edge_cases(gpu_ctx, oracle):
+    # empty graph: every program runs, root is the only reached vertex
+    e = np.zeros((0, 2), dtype=np.uint32)
+    g = L.Graph.from_records(gpu_ctx, e, 4, False, L.DIRECTED)
+    hops, _, it = L.bfs(g, 2)
+    assert hops[2] == 0 and (np.delete(hops, 2) == 255).all() and it == 1
+    rank, _, _ = L.pagerank(g, 5)
+    assert np.allclose(rank, 0.15)
+    # root out of range is an error, not a fault
+    with pytest.raises(L.LA3Error):
+        L.bfs(g, 5)
+    # wrong load mode for the program
+    with pytest.raises(L.LA3Error):
+        L.cc(g)
+    with pytest.raises(L.LA3Error):
+        L.tc(g)
+    # sink / source / isolated roots
+    rec = np.array([[1, 2], [2, 3], [3, 1], [4, 1], [2, 5]], dtype=np.uint32)
+    g, o = _both(gpu_ctx, oracle, rec, 6, False, L.DIRECTED)
+    for root in range(7):
+        hops, par, it = L.bfs(g, root)
+        eh, ep, eit = oracle.bfs(o, root)
+        assert (hops == eh).all() and (par == ep).all() and it == eit
