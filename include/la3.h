@@ -75,6 +75,8 @@ typedef struct {
 
 /* ------------------------------------------------------------ context --- */
 /* Creates a context on HIP device `device`: one stream, device scratch.
+ * la3_ctx_destroy releases the caller's reference; the context is freed when
+ * its last graph is freed.
  * device == -1 creates a host-only context: graphs are ingested and their
  * layout can be exported (la3_graph_info_get / slots / csc), but programs
  * return LA3_EDEVICE. */
@@ -182,6 +184,23 @@ typedef struct {
 la3_status la3_last_timing(const la3_ctx* ctx, la3_timing* out);
 /* Enables per-launch HIP-event timing (adds one event pair per launch). */
 la3_status la3_set_profiling(la3_ctx* ctx, int enabled);
+
+/* Cumulative per-kernel statistics since the last reset, gathered while
+ * profiling is enabled: launches, summed HIP-event duration on the context's
+ * stream, and the algorithmic bytes those launches moved (the compulsory
+ * traffic of the vertex-program step they implement: index and weight
+ * entries scanned, column pointers and messages of active columns, state of
+ * activated rows; per-kernel formulas in the library's driver code). */
+typedef struct {
+  char name[48];
+  uint64_t launches;
+  double ms;
+  double alg_bytes;
+  uint64_t entries;     /* index entries read */
+} la3_kernel_stat;
+la3_status la3_kernel_stats(const la3_ctx* ctx, la3_kernel_stat* out, uint32_t cap,
+                            uint32_t* count);
+la3_status la3_kernel_stats_reset(la3_ctx* ctx);
 
 /* -------------------------------------------------------------- tools --- */
 /* Seeded R-MAT generator (a, b, c, d = 1-a-b-c), 2^scale vertices with ids

@@ -29,6 +29,7 @@ EXPORTS = [
     "la3_degree", "la3_pagerank", "la3_bfs", "la3_sssp", "la3_cc", "la3_tc", "la3_ppr",
     "la3_bfs_batch", "la3_sssp_batch", "la3_ppr_batch",
     "la3_last_timing", "la3_set_profiling", "la3_rmat_generate",
+    "la3_kernel_stats", "la3_kernel_stats_reset",
 ]
 
 
@@ -43,6 +44,12 @@ class GraphInfo(ctypes.Structure):
         "n", "nslots", "nregular", "nsink", "nsource", "nisolated", "ncols", "directed",
         "weighted", "mode")] + [(n, ctypes.c_uint64) for n in (
             "nnz", "nnz_regular", "nnz_sink", "nrecords", "device_bytes")]
+
+
+class KernelStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_uint64),
+                ("ms", ctypes.c_double), ("alg_bytes", ctypes.c_double),
+                ("entries", ctypes.c_uint64)]
 
 
 class Timing(ctypes.Structure):
@@ -88,6 +95,8 @@ def lib():
             "la3_ppr_batch": (i32, [P, P, u32, u32, ctypes.c_float, P]),
             "la3_last_timing": (i32, [P, ctypes.POINTER(Timing)]),
             "la3_set_profiling": (i32, [P, i32]),
+            "la3_kernel_stats": (i32, [P, P, u32, ctypes.POINTER(u32)]),
+            "la3_kernel_stats_reset": (i32, [P]),
             "la3_rmat_generate": (i32, [u32, u32, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, u64, i32, P, ctypes.POINTER(u64)]),
         }
@@ -150,6 +159,18 @@ class Context:
         return dict(total_ms=t.total_ms, dominant_ms=t.dominant_ms,
                     dominant_launches=t.dominant_launches, edges_scanned=t.edges_scanned,
                     dominant_kernel=t.dominant_kernel.decode())
+
+    def kernel_stats(self):
+        """Cumulative per-kernel launches / ms / algorithmic bytes (profiling on)."""
+        n = ctypes.c_uint32(0)
+        self.check(lib().la3_kernel_stats(self.h, None, 0, ctypes.byref(n)), "kernel_stats")
+        arr = (KernelStat * max(n.value, 1))()
+        self.check(lib().la3_kernel_stats(self.h, arr, n.value, ctypes.byref(n)), "kernel_stats")
+        return {a.name.decode(): dict(launches=a.launches, ms=a.ms, alg_bytes=a.alg_bytes,
+                                      entries=a.entries) for a in arr[:n.value]}
+
+    def reset_kernel_stats(self):
+        self.check(lib().la3_kernel_stats_reset(self.h), "kernel_stats_reset")
 
     def synchronize(self):
         self.check(lib().la3_ctx_synchronize(self.h), "synchronize")

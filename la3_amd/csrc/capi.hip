@@ -32,7 +32,9 @@ struct la3_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   bool profiling = false;
-  struct Ev { const char* name; hipEvent_t a, b; };
+  struct Ev { const char* name; hipEvent_t a, b; double bytes; uint64_t entries; };
+  struct Stat { uint64_t launches = 0; double ms = 0, bytes = 0; uint64_t entries = 0; };
+  std::map<std::string, Stat> stats;  // cumulative, while profiling
   std::vector<Ev> evs;
   std::vector<hipEvent_t> free_events;
   la3_timing last{};
@@ -40,7 +42,12 @@ struct la3_ctx {
   std::map<std::string, DevBuf> scratch;
   void* host_ctr = nullptr;  // pinned, for frontier counters
   int ncu = 256;
+  int refs = 1;              // the caller's reference plus one per live graph
 };
+
+namespace {
+void ctx_release(la3_ctx* c);
+}
 
 struct la3_graph {
   la3_ctx* ctx = nullptr;
@@ -56,6 +63,22 @@ struct la3_graph {
 };
 
 namespace {
+
+// Frees the context once the caller has destroyed it and every graph on it
+// has been freed.
+void ctx_release(la3_ctx* c) {
+  if (--c->refs > 0) return;
+  if (c->device >= 0) {
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->scratch) if (kv.second.p) (void)hipFree(kv.second.p);
+    for (auto& e : c->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+    for (auto e : c->free_events) (void)hipEventDestroy(e);
+    if (c->host_ctr) (void)hipHostFree(c->host_ctr);
+    (void)hipStreamDestroy(c->stream);
+  }
+  delete c;
+}
 
 la3_status fail(la3_ctx* c, la3_status s, const std::string& msg) {
   if (c) c->err = msg;
@@ -76,13 +99,25 @@ hipEvent_t get_event(la3_ctx* c) {
   return e;
 }
 
+// Launches a kernel on the context's stream; with profiling on, brackets it
+// with an event pair. Returns the event slot (or -1) so that the driver can
+// attach the launch's algorithmic bytes once its counters are known.
 template <typename... KArgs, typename... Args>
-void launch(la3_ctx* c, const char* name, void (*k)(KArgs...), uint32_t blocks, Args... args) {
+int launch(la3_ctx* c, const char* name, void (*k)(KArgs...), uint32_t blocks, Args... args) {
   hipEvent_t a = nullptr, b = nullptr;
-  if (blocks == 0) return;
+  if (blocks == 0) return -1;
   if (c->profiling) { a = get_event(c); b = get_event(c); (void)hipEventRecord(a, c->stream); }
   hipLaunchKernelGGL(k, dim3(blocks), dim3(BLOCK), 0, c->stream, args...);
-  if (c->profiling) { (void)hipEventRecord(b, c->stream); c->evs.push_back({name, a, b}); }
+  if (!c->profiling) return -1;
+  (void)hipEventRecord(b, c->stream);
+  c->evs.push_back({name, a, b, 0.0, 0});
+  return (int)c->evs.size() - 1;
+}
+
+void annotate(la3_ctx* c, int ev, double bytes, uint64_t entries) {
+  if (ev < 0 || ev >= (int)c->evs.size()) return;
+  c->evs[ev].bytes += bytes;
+  c->evs[ev].entries += entries;
 }
 
 uint32_t blocks_for(uint64_t threads, uint32_t cap = 8192) {
@@ -107,6 +142,11 @@ void timing_end(la3_ctx* c) {
     p.first += ms;
     p.second++;
     t.total_ms += ms;
+    auto& st = c->stats[e.name];
+    st.launches++;
+    st.ms += ms;
+    st.bytes += e.bytes;
+    st.entries += e.entries;
     c->free_events.push_back(e.a);
     c->free_events.push_back(e.b);
   }
@@ -158,6 +198,7 @@ la3_status graph_from_layout(la3_ctx* ctx, HostLayout& L, la3_graph** out) {
     g->id_of = L.id_of;
     g->slot_of_id = L.slot_of_id;
     g->host = std::make_unique<HostLayout>(std::move(L));
+    ctx->refs++;
     *out = g.release();
     return LA3_OK;
   }
@@ -184,6 +225,7 @@ la3_status graph_from_layout(la3_ctx* ctx, HostLayout& L, la3_graph** out) {
   g->hid_of.swap(L.hid_of);
   g->id_of.swap(L.id_of);
   g->slot_of_id.swap(L.slot_of_id);
+  ctx->refs++;
   *out = g.release();
   return LA3_OK;
 }
@@ -212,28 +254,28 @@ la3_status get_fronts(la3_graph* g, Fronts& F, const char* tag) {
     std::string n1 = std::string(tag) + ".items" + char('0' + i);
     std::string n2 = std::string(tag) + ".ctr" + char('0' + i);
     F.f[i].items = (uint64_t*)scratch(c, n1.c_str(), F.cap * sizeof(uint64_t));
-    uint8_t* ctr = (uint8_t*)scratch(c, n2.c_str(), 16);
+    uint8_t* ctr = (uint8_t*)scratch(c, n2.c_str(), 32);
     if (!F.f[i].items || !ctr) return fail(c, LA3_ENOMEM, "frontier allocation failed");
     F.f[i].ctr = (uint32_t*)ctr;
-    F.f[i].edges = (unsigned long long*)(ctr + 8);
+    F.f[i].edges = (unsigned long long*)(ctr + 16);
   }
   return LA3_OK;
 }
 
 struct FrontCounts {
-  uint32_t items, verts;
-  unsigned long long edges;
+  uint32_t items, verts, cols, pad;
+  unsigned long long edges, scanned;
 };
 
 la3_status read_counts(la3_ctx* c, const Frontier& f, FrontCounts& out) {
-  HIPC(c, hipMemcpyAsync(c->host_ctr, f.ctr, 16, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipMemcpyAsync(c->host_ctr, f.ctr, 32, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
-  std::memcpy(&out, c->host_ctr, 16);
+  std::memcpy(&out, c->host_ctr, 32);
   return LA3_OK;
 }
 
 void zero_counts(la3_ctx* c, const Frontier& f) {
-  launch(c, "fill_u32", k_fill<uint32_t>, 1, f.ctr, (uint64_t)4, 0u);
+  launch(c, "fill_u32", k_fill<uint32_t>, 1, f.ctr, (uint64_t)8, 0u);
 }
 
 }  // namespace
@@ -259,16 +301,7 @@ la3_status la3_ctx_create(int device, la3_ctx** out) {
 }
 
 void la3_ctx_destroy(la3_ctx* c) {
-  if (!c) return;
-  if (c->device < 0) { delete c; return; }
-  (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
-  for (auto& kv : c->scratch) if (kv.second.p) (void)hipFree(kv.second.p);
-  for (auto& e : c->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
-  for (auto e : c->free_events) (void)hipEventDestroy(e);
-  if (c->host_ctr) (void)hipHostFree(c->host_ctr);
-  (void)hipStreamDestroy(c->stream);
-  delete c;
+  if (c) ctx_release(c);
 }
 
 const char* la3_last_error(const la3_ctx* c) { return c ? c->err.c_str() : "null context"; }
@@ -289,6 +322,31 @@ la3_status la3_set_profiling(la3_ctx* c, int enabled) {
 la3_status la3_last_timing(const la3_ctx* c, la3_timing* out) {
   if (!c || !out) return LA3_EINVAL;
   *out = c->last;
+  return LA3_OK;
+}
+
+la3_status la3_kernel_stats(const la3_ctx* c, la3_kernel_stat* out, uint32_t cap, uint32_t* count) {
+  if (!c || !count) return LA3_EINVAL;
+  uint32_t i = 0;
+  for (auto& kv : c->stats) {
+    if (out && i < cap) {
+      la3_kernel_stat& o = out[i];
+      std::memset(&o, 0, sizeof(o));
+      snprintf(o.name, sizeof(o.name), "%s", kv.first.c_str());
+      o.launches = kv.second.launches;
+      o.ms = kv.second.ms;
+      o.alg_bytes = kv.second.bytes;
+      o.entries = kv.second.entries;
+    }
+    i++;
+  }
+  *count = i;
+  return LA3_OK;
+}
+
+la3_status la3_kernel_stats_reset(la3_ctx* c) {
+  if (!c) return LA3_EINVAL;
+  c->stats.clear();
   return LA3_OK;
 }
 
@@ -334,11 +392,14 @@ la3_status la3_graph_load_file(la3_ctx* ctx, const char* path, uint32_t nvertice
 
 void la3_graph_free(la3_graph* g) {
   if (!g) return;
-  if (g->host) { delete g; return; }
-  (void)hipSetDevice(g->ctx->device);
-  (void)hipStreamSynchronize(g->ctx->stream);
-  for (void* p : g->allocs) (void)hipFree(p);
+  la3_ctx* c = g->ctx;
+  if (!g->host) {
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (void* p : g->allocs) (void)hipFree(p);
+  }
   delete g;
+  ctx_release(c);
 }
 
 la3_status la3_graph_info_get(const la3_graph* g, la3_graph_info* o) {
@@ -432,7 +493,7 @@ la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
   if ((st = read_counts(c, F.f[0], fc))) return st;
   uint32_t iter = 0, cur = 0;
   bool bottom_up = false;
-  uint64_t explored = fc.edges;
+  uint64_t explored = fc.edges, visited = 1;
   const uint32_t wave_blocks = 4 * c->ncu * 8;
   // The reference stops after the first iteration that activates no regular
   // vertex (vertex_program_execute.hpp:211-212); a regular vertex is exactly one
@@ -440,17 +501,29 @@ la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
   while (fc.items > 0) {
     const Frontier& nf = F.f[cur ^ 1];
     zero_counts(c, nf);
+    int ev = -1;
     if (!bottom_up) {
       const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
-      if (fc.items) launch(c, "bfs_push", k_bfs_push, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, level, mark, iter, nf);
-      c->edges_scanned += fc.edges;
+      ev = launch(c, "bfs_push", k_bfs_push, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, level, mark, iter, nf);
     } else {
-      launch(c, "bfs_pull", k_bfs_pull, blocks_for(g->nrows, 4 * c->ncu * 8), d, level, mark, iter, nf);
+      ev = launch(c, "bfs_pull", k_bfs_pull, blocks_for(g->nrows, 4 * c->ncu * 8), d, level, mark, iter, nf);
       if (d.nlong) launch(c, "bfs_pull_long", k_bfs_pull_long, std::min<uint32_t>((d.nlong + 3) / 4, wave_blocks), d, level, mark, iter, nf);
     }
     FrontCounts nc;
     if ((st = read_counts(c, nf, nc))) return st;
-    if (bottom_up) c->edges_scanned += 0;  // pull steps read row lists; counted below
+    // Algorithmic bytes (m = a = 1/8 B frontier bit, s_r = s_w = 1 B level):
+    // push: 4 B per entry, 8 B pointers + 1/8 B message per active column,
+    //       (a + s_r + s_w) per activated row;
+    // pull: 4 B + 1/8 B per entry read, 8 B pointers + 1 B state per row checked.
+    if (!bottom_up) {
+      annotate(c, ev, 4.0 * fc.edges + 8.125 * fc.cols + 2.25 * nc.verts, fc.edges);
+      c->edges_scanned += fc.edges;
+    } else {
+      const uint64_t unvisited = g->nrows > visited ? g->nrows - visited : 0;
+      annotate(c, ev, 4.125 * nc.scanned + 9.0 * unvisited + 2.25 * nc.verts, nc.scanned);
+      c->edges_scanned += nc.scanned;
+    }
+    visited += nc.verts;
     iter++;
     explored += nc.edges;
     const uint64_t unexplored = g->nnz > explored ? g->nnz - explored : 0;
@@ -498,9 +571,14 @@ la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters
     const Frontier& nf = F.f[cur ^ 1];
     zero_counts(c, nf);
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
-    launch(c, "sssp_push", k_sssp_push, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dv, stamp, round, nf);
+    const int ev = launch(c, "sssp_push", k_sssp_push, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dv, stamp, round, nf);
     c->edges_scanned += fc.edges;
-    if ((st = read_counts(c, nf, fc))) return st;
+    FrontCounts nc;
+    if ((st = read_counts(c, nf, nc))) return st;
+    // Algorithmic bytes (w = m = a = s_r = s_w = 4 B): (4 + w) per entry,
+    // 8 B pointers + m per active column, (a + s_r + s_w) + m per activated row.
+    annotate(c, ev, 8.0 * fc.edges + 12.0 * fc.cols + 16.0 * nc.verts, fc.edges);
+    fc = nc;
     round++;
     cur ^= 1;
   }

@@ -65,9 +65,11 @@ __device__ __forceinline__ void st_relaxed(uint32_t* p, uint32_t v) {
 
 // Traversal frontier bookkeeping, one set per query.
 struct Frontier {
-  uint64_t* items;         // work items: (chunk << 32) | slot
-  uint32_t* ctr;           // [0] items emitted, [1] vertices emitted
-  unsigned long long* edges;  // [0] entries behind the emitted items
+  uint64_t* items;            // work items: (chunk << 32) | slot
+  uint32_t* ctr;              // [0] items emitted, [1] vertices emitted,
+                              // [2] vertices emitted with a non-empty column
+  unsigned long long* edges;  // [0] entries behind the emitted items,
+                              // [1] row entries read by pull steps
 };
 
 // Wave-aggregated append of `nit` work items for `slot` (nit may be 0).
@@ -80,9 +82,11 @@ __device__ __forceinline__ void wave_emit(const Frontier& f, uint32_t slot, uint
   uint32_t e = wave_sum(nedges);
   if (b == 0) return;
   uint32_t base = 0;
+  const uint32_t ncol = (uint32_t)__popcll(__ballot(nit > 0));
   if (lane_id() == WAVE - 1) {
     if (total) base = atomicAdd(&f.ctr[0], total);
     atomicAdd(&f.ctr[1], (uint32_t)__popcll(b));
+    if (ncol) atomicAdd(&f.ctr[2], ncol);
     atomicAdd(&f.edges[0], (unsigned long long)e);
   }
   base = __shfl(base, WAVE - 1, WAVE);
@@ -128,14 +132,14 @@ __global__ void k_outdeg(DG g, uint32_t* deg) {
 // of a newly reached row, the reference keeps the last one combined
 // (combine: y2 = y1, bfs.h:40), i.e. the largest message-order key.
 
+// The root's own slot is written by the thread that resets that slot, so no
+// other thread's reset can land after it.
 __global__ void k_bfs_init(DG g, uint8_t* level, uint32_t* mark, uint32_t root, Frontier f) {
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
-    level[s] = (uint8_t)BFS_INF;
-    mark[s] = 0;
+    level[s] = s == root ? 0 : (uint8_t)BFS_INF;
+    mark[s] = s == root ? 0xffffffffu : 0u;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    level[root] = 0;
-    mark[root] = 0xffffffffu;
     uint32_t nit = 0, ne = 0;
     if (slot_has_col(g, root)) {
       uint32_t c = col_of_slot(g, root);
@@ -145,7 +149,9 @@ __global__ void k_bfs_init(DG g, uint8_t* level, uint32_t* mark, uint32_t root, 
     for (uint32_t k = 0; k < nit; k++) f.items[k] = ((uint64_t)k << 32) | root;
     f.ctr[0] = nit;
     f.ctr[1] = 1;
+    f.ctr[2] = nit > 0;
     f.edges[0] = ne;
+    f.edges[1] = 0;
   }
 }
 
@@ -197,15 +203,19 @@ __global__ void __launch_bounds__(BLOCK) k_bfs_pull(DG g, uint8_t* level, uint32
   for (uint32_t base = blockIdx.x * BLOCK; base < g.nrows; base += stride) {
     const uint32_t r = base + threadIdx.x;
     bool disc = false;
+    uint32_t scanned = 0;
     if (r < g.nrows && level[r] == (uint8_t)BFS_INF) {
       const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1];
       if (e - b <= LONG_ROW) {
         for (uint32_t j = e; j > b; j--) {
           uint32_t src = g.row_cols[j - 1];
+          scanned++;
           if (level[src] == cur) { mark[r] = msg_key(g, col_of_slot(g, src)) + 1; disc = true; break; }
         }
       }
     }
+    scanned = wave_sum(scanned);
+    if (lane_id() == 0 && scanned) atomicAdd(&nf.edges[1], (unsigned long long)scanned);
     uint32_t nit = 0, ne = 0;
     if (disc) {
       level[r] = nxt;
@@ -231,8 +241,10 @@ __global__ void __launch_bounds__(BLOCK) k_bfs_pull_long(DG g, uint8_t* level, u
     if (level[r] != (uint8_t)BFS_INF) continue;  // wave-uniform
     const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1];
     bool found = false;
+    uint32_t scanned = 0;
     for (uint32_t top = e; top > b && !found; top = (top - b > WAVE) ? top - WAVE : b) {
       const uint32_t lo = (top - b > WAVE) ? top - WAVE : b;
+      scanned += top - lo;
       const uint32_t j = lo + lane_id();
       bool hit = false;
       uint32_t src = 0;
@@ -244,6 +256,7 @@ __global__ void __launch_bounds__(BLOCK) k_bfs_pull_long(DG g, uint8_t* level, u
         found = true;
       }
     }
+    if (lane_id() == 0) atomicAdd(&nf.edges[1], (unsigned long long)scanned);
     uint32_t nit = 0, ne = 0;
     bool disc = found && lane_id() == 0;
     if (disc) {
@@ -282,11 +295,10 @@ __global__ void k_bfs_output(DG g, const uint8_t* level, const uint32_t* mark, u
 
 __global__ void k_sssp_init(DG g, uint32_t* dist, uint32_t* stamp, uint32_t root, Frontier f) {
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
-    dist[s] = SSSP_INF;
+    dist[s] = s == root ? 0u : SSSP_INF;
     stamp[s] = 0;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    dist[root] = 0;
     uint32_t nit = 0, ne = 0;
     if (slot_has_col(g, root)) {
       uint32_t c = col_of_slot(g, root);
@@ -296,7 +308,9 @@ __global__ void k_sssp_init(DG g, uint32_t* dist, uint32_t* stamp, uint32_t root
     for (uint32_t k = 0; k < nit; k++) f.items[k] = ((uint64_t)k << 32) | root;
     f.ctr[0] = nit;
     f.ctr[1] = 1;
+    f.ctr[2] = nit > 0;
     f.edges[0] = ne;
+    f.edges[1] = 0;
   }
 }
 
