@@ -18,6 +18,7 @@
 #include "../../include/la3.h"
 #include "internal.h"
 #include "kernels_dev.h"
+#include "kernels_batch.h"
 
 using namespace la3;
 
@@ -212,7 +213,7 @@ la3_status graph_from_layout(la3_ctx* ctx, HostLayout& L, la3_graph** out) {
       (s = up(L.col_rows, &d.col_rows)) || (s = up(L.row_ptr, &d.row_ptr)) ||
       (s = up(L.row_split, &d.row_split)) || (s = up(L.row_cols, &d.row_cols)) ||
       (s = up(L.id_of, &d.id_of)) || (s = up(L.hid_of, &d.hid_of)) ||
-      (s = up(L.long_rows, &d.long_rows))) {
+      (s = up(L.long_rows, &d.long_rows)) || (s = up(L.slot_of_id, &d.slot_of_id))) {
     for (void* p : g->allocs) (void)hipFree(p);
     return s;
   }
@@ -738,11 +739,122 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
 }
 
 // ------------------------------------------------------------ batched ---
+}  // extern "C"
+
+template <int W>
+static la3_status msbfs_run(la3_graph* g, const std::vector<uint32_t>& root_slots, uint32_t k,
+                            uint8_t* hops, uint32_t* iters_run) {
+  la3_ctx* c = g->ctx;
+  const uint32_t S = g->nslots;
+  const uint32_t K = 64 * W;
+  const DG& d = g->dg;
+  uint64_t* vis = (uint64_t*)scratch(c, "msbfs.vis", sizeof(uint64_t) * S * W);
+  uint64_t* fa = (uint64_t*)scratch(c, "msbfs.fa", sizeof(uint64_t) * S * W);
+  uint64_t* fb = (uint64_t*)scratch(c, "msbfs.fb", sizeof(uint64_t) * S * W);
+  uint8_t* lev = (uint8_t*)scratch(c, "msbfs.lev", (size_t)S * K);
+  uint32_t* stamp = (uint32_t*)scratch(c, "msbfs.stamp", sizeof(uint32_t) * S);
+  uint32_t* touched = (uint32_t*)scratch(c, "msbfs.touched", sizeof(uint32_t) * (S + 1));
+  uint32_t* ntouched = (uint32_t*)scratch(c, "msbfs.ntouched", 64);
+  uint32_t* rs = (uint32_t*)scratch(c, "msbfs.roots", sizeof(uint32_t) * k);
+  uint64_t* valid = (uint64_t*)scratch(c, "msbfs.valid", sizeof(uint64_t) * W);
+  uint8_t* out = (uint8_t*)scratch(c, "msbfs.out", (size_t)S * k);
+  Fronts F;
+  la3_status st = get_fronts(g, F, "bfs");
+  if (st) return st;
+  if (!vis || !fa || !fb || !lev || !stamp || !touched || !ntouched || !rs || !valid || !out)
+    return fail(c, LA3_ENOMEM, "batched BFS: device allocation failed");
+  uint64_t hv[W];
+  for (int w = 0; w < W; w++) {
+    const int bits = std::max(0, std::min(64, (int)k - 64 * w));
+    hv[w] = bits == 64 ? ~0ull : ((1ull << bits) - 1);
+  }
+  HIPC(c, hipMemcpyAsync(valid, hv, sizeof(hv), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(rs, root_slots.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream));
+  launch(c, "msbfs_init", k_msbfs_init<W>, blocks_for(S), d, vis, fa, fb, (uint64_t*)lev, stamp);
+  launch(c, "msbfs_roots", k_msbfs_roots<W>, 1, d, (const uint32_t*)rs, k, vis, fa, lev);
+  zero_counts(c, F.f[0]);
+  launch(c, "msbfs_seed", k_msbfs_seed, blocks_for(k), d, (const uint32_t*)rs, k, stamp, F.f[0]);
+  FrontCounts fc;
+  if ((st = read_counts(c, F.f[0], fc))) return st;
+  uint32_t iter = 0, cur = 0;
+  bool bottom_up = false;
+  const uint32_t wave_blocks = 4 * c->ncu * 8;
+  while (fc.items > 0) {
+    const Frontier& nf = F.f[cur ^ 1];
+    zero_counts(c, nf);
+    int ev = -1;
+    if (!bottom_up) {
+      launch(c, "fill_u32", k_fill<uint32_t>, 1, ntouched, (uint64_t)1, 0u);
+      const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
+      ev = launch(c, "msbfs_push", k_msbfs_push<W>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items,
+                  (const uint64_t*)vis, (const uint64_t*)fa, fb, stamp, iter + 2, touched, ntouched);
+      // The touched count bounds the apply grid; read it back with the counters below.
+      HIPC(c, hipMemcpyAsync((uint8_t*)c->host_ctr + 32, ntouched, 4, hipMemcpyDeviceToHost, c->stream));
+      HIPC(c, hipStreamSynchronize(c->stream));
+      uint32_t nt = 0;
+      std::memcpy(&nt, (uint8_t*)c->host_ctr + 32, 4);
+      if (nt) launch(c, "msbfs_apply", k_msbfs_apply<W>, blocks_for(nt), d, (const uint32_t*)touched, nt, vis,
+                     (const uint64_t*)fb, lev, iter, nf);
+    } else {
+      ev = launch(c, "msbfs_pull", k_msbfs_pull<W>, blocks_for(g->nrows, wave_blocks), d, vis,
+                  (const uint64_t*)fa, fb, lev, iter, (const uint64_t*)valid, nf);
+      if (d.nlong) launch(c, "msbfs_pull_long", k_msbfs_pull_long<W>, std::min<uint32_t>((d.nlong + 3) / 4, wave_blocks),
+                          d, vis, (const uint64_t*)fa, fb, lev, iter, (const uint64_t*)valid, nf);
+    }
+    launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for((uint64_t)S * W), (unsigned long long*)fa,
+           (uint64_t)S * W, 0ull);
+    std::swap(fa, fb);
+    FrontCounts nc;
+    if ((st = read_counts(c, nf, nc))) return st;
+    // Algorithmic bytes: k/8 B of frontier per entry and per active column,
+    // plus k/8 B visited + k/8 B next frontier + the new level bytes per row.
+    const double kb = k / 8.0;
+    if (!bottom_up) {
+      annotate(c, ev, (4.0 + kb) * fc.edges + (8.0 + kb) * fc.cols + 3 * kb * nc.verts, fc.edges);
+      c->edges_scanned += fc.edges;
+    } else {
+      annotate(c, ev, (4.0 + kb) * nc.scanned + (8.0 + kb) * g->nrows + 2 * kb * nc.verts, nc.scanned);
+      c->edges_scanned += nc.scanned;
+    }
+    iter++;
+    if (!bottom_up && nc.edges * 20 > g->nnz) bottom_up = true;
+    else if (bottom_up && (uint64_t)nc.verts * 24 < S) bottom_up = false;
+    fc = nc;
+    cur ^= 1;
+  }
+  launch(c, "msbfs_output", k_msbfs_output, blocks_for(S), d, (const uint8_t*)lev, K, k, out);
+  HIPC(c, hipMemcpyAsync(hops, out, (size_t)S * k, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  if (iters_run) *iters_run = std::max<uint32_t>(iter, 1);
+  return LA3_OK;
+}
+
+extern "C" {
+
 la3_status la3_bfs_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint8_t* hops,
                          uint32_t* iters_run) {
-  (void)roots; (void)k; (void)hops; (void)iters_run;
-  return fail(g ? g->ctx : nullptr, LA3_ENOTSUP, "batched BFS not built yet");
+  if (!g || !roots || !hops) return LA3_EINVAL;
+  la3_ctx* c = g->ctx;
+  if (k == 0 || k > 1024) return fail(c, LA3_EINVAL, "batch size must be in [1, 1024]");
+  std::vector<uint32_t> rs(k);
+  for (uint32_t q = 0; q < k; q++) {
+    if (roots[q] > g->n) return fail(c, LA3_EINVAL, "root id out of range");
+    rs[q] = g->slot_of_id[roots[q]];
+  }
+  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+  HIPC(c, hipSetDevice(c->device));
+  timing_begin(c);
+  la3_status st;
+  const uint32_t W = (k + 63) / 64;
+  if (W <= 1) st = msbfs_run<1>(g, rs, k, hops, iters_run);
+  else if (W <= 2) st = msbfs_run<2>(g, rs, k, hops, iters_run);
+  else if (W <= 4) st = msbfs_run<4>(g, rs, k, hops, iters_run);
+  else if (W <= 8) st = msbfs_run<8>(g, rs, k, hops, iters_run);
+  else st = msbfs_run<16>(g, rs, k, hops, iters_run);
+  timing_end(c);
+  return st;
 }
+
 la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint32_t* dist,
                           uint32_t* iters_run) {
   (void)roots; (void)k; (void)dist; (void)iters_run;

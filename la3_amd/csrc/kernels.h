@@ -39,6 +39,7 @@ struct DG {
   const uint32_t* hid_of;
   const uint32_t* long_rows;
   uint32_t nlong;
+  const uint32_t* slot_of_id;
 };
 
 }  // namespace la3

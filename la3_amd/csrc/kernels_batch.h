@@ -1,0 +1,295 @@
+// Batched (multi-source) traversal kernels: k BFS queries in one traversal.
+// Included once, by capi.hip, after kernels_dev.h.
+//
+// Layout, per vertex slot s (W = ceil(k / 64) words, K = 64 * W):
+//   vis[s*W + w], fa[s*W + w], fb[s*W + w]   k-bit visited / frontier / next
+//                                           frontier sets, bit q = query q
+//   lev[s*K + q]                            u8 level of query q (255 = unreached)
+// Every per-query result equals the single-query BfsVertex run from that
+// root: a vertex joins query q's frontier at step i exactly when the
+// single-query traversal discovers it at level i + 1.
+#pragma once
+
+#include "kernels_dev.h"
+
+namespace la3 {
+
+template <int W>
+__global__ void k_msbfs_init(DG g, uint64_t* vis, uint64_t* fa, uint64_t* fb, uint64_t* lev64,
+                             uint32_t* stamp) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      vis[(size_t)s * W + w] = 0;
+      fa[(size_t)s * W + w] = 0;
+      fb[(size_t)s * W + w] = 0;
+    }
+#pragma unroll
+    for (int w = 0; w < 8 * W; w++) lev64[(size_t)s * 8 * W + w] = ~0ull;
+    stamp[s] = 0;
+  }
+}
+
+// One thread sets every root (roots may repeat, so no concurrent writers to a
+// word), then the distinct root slots are turned into push work items.
+template <int W>
+__global__ void k_msbfs_roots(DG g, const uint32_t* root_slots, uint32_t k, uint64_t* vis,
+                              uint64_t* fa, uint8_t* lev) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (uint32_t q = 0; q < k; q++) {
+    const uint32_t s = root_slots[q];
+    const uint64_t bit = 1ull << (q & 63);
+    vis[(size_t)s * W + q / 64] |= bit;
+    fa[(size_t)s * W + q / 64] |= bit;
+    lev[(size_t)s * 64 * W + q] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_msbfs_seed(DG g, const uint32_t* root_slots, uint32_t k,
+                                                      uint32_t* stamp, Frontier f) {
+  const uint32_t stride = gridDim.x * BLOCK;
+  for (uint32_t base = blockIdx.x * BLOCK; base < k; base += stride) {
+    const uint32_t q = base + threadIdx.x;
+    bool first = false;
+    uint32_t s = 0, nit = 0, ne = 0;
+    if (q < k) {
+      s = root_slots[q];
+      first = atomicMax(&stamp[s], 1u) < 1u;
+      if (first && slot_has_col(g, s)) {
+        const uint32_t c = col_of_slot(g, s);
+        ne = g.col_ptr[c + 1] - g.col_ptr[c];
+        nit = (ne + CHUNK - 1) / CHUNK;
+      }
+    }
+    wave_emit(f, s, nit, ne, first);
+  }
+}
+
+// Top-down step: every entry (c -> r) of an active column ORs c's frontier
+// bits that r has not yet seen into r's next-frontier words. The first
+// toucher of r in this step (stamp) appends r to the touched list.
+template <int W>
+__global__ void __launch_bounds__(BLOCK) k_msbfs_push(DG g, const uint64_t* items, uint32_t nitems,
+                                                      const uint64_t* vis, const uint64_t* fa,
+                                                      uint64_t* fb, uint32_t* stamp, uint32_t tag,
+                                                      uint32_t* touched, uint32_t* ntouched) {
+  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  for (uint32_t it = wid; it < nitems; it += nwaves) {
+    const uint64_t item = items[it];
+    const uint32_t s = (uint32_t)item, ch = (uint32_t)(item >> 32);
+    const uint32_t c = col_of_slot(g, s);
+    const uint32_t beg = g.col_ptr[c] + ch * CHUNK;
+    const uint32_t end = min(beg + CHUNK, g.col_ptr[c + 1]);
+    uint64_t f[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) f[w] = fa[(size_t)s * W + w];
+    for (uint32_t e0 = beg; e0 < end; e0 += WAVE) {
+      const uint32_t e = e0 + lane_id();
+      bool first = false;
+      uint32_t r = 0;
+      if (e < end) {
+        r = g.col_rows[e];
+        bool any = false;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+          const uint64_t m = f[w] & ~vis[(size_t)r * W + w];
+          if (m) { atomicOr((unsigned long long*)&fb[(size_t)r * W + w], (unsigned long long)m); any = true; }
+        }
+        if (any) first = atomicMax(&stamp[r], tag) < tag;
+      }
+      const uint64_t b = __ballot(first);
+      if (b) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(ntouched, (uint32_t)__popcll(b));
+        base = __shfl(base, 0, WAVE);
+        if (first) touched[base + __popcll(b & ((1ull << lane_id()) - 1))] = r;
+      }
+    }
+  }
+}
+
+// Marks the new bits of each touched vertex visited, records their level and
+// emits the vertex's column as push work for the next step.
+template <int W>
+__global__ void __launch_bounds__(BLOCK) k_msbfs_apply(DG g, const uint32_t* touched,
+                                                       uint32_t ntouched, uint64_t* vis,
+                                                       const uint64_t* fb, uint8_t* lev,
+                                                       uint32_t iter, Frontier nf) {
+  const uint32_t stride = gridDim.x * BLOCK;
+  const uint8_t nxt = (uint8_t)(iter + 1);
+  for (uint32_t base = blockIdx.x * BLOCK; base < ntouched; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    bool act = false;
+    uint32_t r = 0, nit = 0, ne = 0;
+    if (i < ntouched) {
+      r = touched[i];
+#pragma unroll
+      for (int w = 0; w < W; w++) {
+        uint64_t m = fb[(size_t)r * W + w];
+        if (!m) continue;
+        act = true;
+        vis[(size_t)r * W + w] |= m;
+        while (m) {
+          const int q = __ffsll((long long)m) - 1;
+          lev[(size_t)r * 64 * W + w * 64 + q] = nxt;
+          m &= m - 1;
+        }
+      }
+      if (act && slot_has_col(g, r)) {
+        const uint32_t c = col_of_slot(g, r);
+        ne = g.col_ptr[c + 1] - g.col_ptr[c];
+        nit = (ne + CHUNK - 1) / CHUNK;
+      }
+    }
+    wave_emit(nf, r, nit, ne, act);
+  }
+}
+
+// Clears the current frontier words of the vertices that were active.
+template <int W>
+__global__ void k_msbfs_clear_items(const uint64_t* items, uint32_t nitems, uint64_t* fa) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += gridDim.x * blockDim.x) {
+    const uint32_t s = (uint32_t)items[i];
+#pragma unroll
+    for (int w = 0; w < W; w++) fa[(size_t)s * W + w] = 0;
+  }
+}
+
+// Bottom-up step: each row ORs the frontier words of its in-neighbours,
+// restricted to the queries that have not reached it, stopping once all
+// of them are covered. Rows longer than LONG_ROW go to k_msbfs_pull_long.
+template <int W>
+__global__ void __launch_bounds__(BLOCK) k_msbfs_pull(DG g, uint64_t* vis, const uint64_t* fa,
+                                                      uint64_t* fb, uint8_t* lev, uint32_t iter,
+                                                      const uint64_t* valid, Frontier nf) {
+  const uint32_t stride = gridDim.x * BLOCK;
+  const uint8_t nxt = (uint8_t)(iter + 1);
+  for (uint32_t base = blockIdx.x * BLOCK; base < g.nrows; base += stride) {
+    const uint32_t r = base + threadIdx.x;
+    bool act = false;
+    uint32_t nit = 0, ne = 0, scanned = 0;
+    if (r < g.nrows) {
+      const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1];
+      uint64_t need[W], acc[W];
+      bool any = false;
+#pragma unroll
+      for (int w = 0; w < W; w++) {
+        need[w] = ~vis[(size_t)r * W + w] & valid[w];
+        acc[w] = 0;
+        any |= need[w] != 0;
+      }
+      if (any && e - b <= LONG_ROW) {
+        for (uint32_t j = b; j < e; j++) {
+          const uint32_t src = g.row_cols[j];
+          bool done = true;
+          scanned++;
+#pragma unroll
+          for (int w = 0; w < W; w++) {
+            acc[w] |= fa[(size_t)src * W + w] & need[w];
+            done &= acc[w] == need[w];
+          }
+          if (done) break;
+        }
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+          uint64_t m = acc[w];
+          if (!m) continue;
+          act = true;
+          vis[(size_t)r * W + w] |= m;
+          fb[(size_t)r * W + w] = m;
+          while (m) {
+            const int q = __ffsll((long long)m) - 1;
+            lev[(size_t)r * 64 * W + w * 64 + q] = nxt;
+            m &= m - 1;
+          }
+        }
+      }
+      if (act && slot_has_col(g, r)) {
+        const uint32_t c = col_of_slot(g, r);
+        ne = g.col_ptr[c + 1] - g.col_ptr[c];
+        nit = (ne + CHUNK - 1) / CHUNK;
+      }
+    }
+    scanned = wave_sum(scanned);
+    if (lane_id() == 0 && scanned) atomicAdd(&nf.edges[1], (unsigned long long)scanned);
+    wave_emit(nf, r, nit, ne, act);
+  }
+}
+
+template <int W>
+__global__ void __launch_bounds__(BLOCK) k_msbfs_pull_long(DG g, uint64_t* vis, const uint64_t* fa,
+                                                           uint64_t* fb, uint8_t* lev, uint32_t iter,
+                                                           const uint64_t* valid, Frontier nf) {
+  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  const uint8_t nxt = (uint8_t)(iter + 1);
+  for (uint32_t i = wid; i < g.nlong; i += nwaves) {
+    const uint32_t r = g.long_rows[i];
+    uint64_t need[W], acc[W];
+    bool any = false;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+      need[w] = ~vis[(size_t)r * W + w] & valid[w];
+      acc[w] = 0;
+      any |= need[w] != 0;
+    }
+    if (!any) continue;  // wave-uniform: every lane read the same words
+    const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1];
+    uint32_t scanned = 0;
+    for (uint32_t j0 = b; j0 < e; j0 += WAVE) {
+      const uint32_t j = j0 + lane_id();
+      if (j < e) {
+        const uint32_t src = g.row_cols[j];
+        scanned++;
+#pragma unroll
+        for (int w = 0; w < W; w++) acc[w] |= fa[(size_t)src * W + w] & need[w];
+      }
+      bool done = true;
+#pragma unroll
+      for (int w = 0; w < W; w++) {
+        uint64_t x = acc[w];
+#pragma unroll
+        for (int d = WAVE / 2; d > 0; d >>= 1) x |= __shfl_xor(x, d, WAVE);
+        acc[w] = x;
+        done &= x == need[w];
+      }
+      if (done) break;
+    }
+    scanned = wave_sum(scanned);
+    bool act = false;
+    uint32_t nit = 0, ne = 0;
+    if (lane_id() == 0) {
+      atomicAdd(&nf.edges[1], (unsigned long long)scanned);
+#pragma unroll
+      for (int w = 0; w < W; w++) {
+        uint64_t m = acc[w];
+        if (!m) continue;
+        act = true;
+        vis[(size_t)r * W + w] |= m;
+        fb[(size_t)r * W + w] = m;
+        while (m) {
+          const int q = __ffsll((long long)m) - 1;
+          lev[(size_t)r * 64 * W + w * 64 + q] = nxt;
+          m &= m - 1;
+        }
+      }
+      if (act && slot_has_col(g, r)) {
+        const uint32_t c = col_of_slot(g, r);
+        ne = g.col_ptr[c + 1] - g.col_ptr[c];
+        nit = (ne + CHUNK - 1) / CHUNK;
+      }
+    }
+    wave_emit(nf, r, nit, ne, act);
+  }
+}
+
+// hops[q * nslots + id] = lev[slot(id) * K + q] for q < k.
+__global__ void k_msbfs_output(DG g, const uint8_t* lev, uint32_t K, uint32_t k, uint8_t* hops) {
+  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < g.nslots; id += gridDim.x * blockDim.x) {
+    const uint8_t* row = lev + (size_t)g.slot_of_id[id] * K;
+    for (uint32_t q = 0; q < k; q++) hops[(size_t)q * g.nslots + id] = row[q];
+  }
+}
+
+}  // namespace la3

@@ -207,3 +207,34 @@ def test_edge_cases(gpu_ctx, oracle):
         hops, par, it = L.bfs(g, root)
         eh, ep, eit = oracle.bfs(o, root)
         assert (hops == eh).all() and (par == ep).all() and it == eit
+
+
+# -------------------------------------------------------------- batched ---
+@pytest.mark.parametrize("scale,seed,k", [(10, 31, 1), (12, 32, 64), (14, 33, 100), (16, 34, 256)])
+def test_bfs_batch_equals_single(gpu_ctx, oracle, scale, seed, k):
+    rec = _rmat(scale, seed)
+    n = 1 << scale
+    g, o = _both(gpu_ctx, oracle, rec, n, False, L.DIRECTED)
+    roots = _roots(rec, k, seed)
+    if k > 2:
+        roots[1] = roots[0]          # repeated root
+        roots[2] = n                  # likely isolated / sink
+    hops, it = L.bfs_batch(g, roots)
+    its = []
+    for q, r in enumerate(roots):
+        h1, _, it1 = L.bfs(g, r, parents=False)
+        assert (hops[q] == h1).all(), f"query {q} (root {r}) differs from single-query"
+        its.append(it1)
+    assert it == max(its)
+    for q in range(0, len(roots), max(1, len(roots) // 4)):
+        assert (hops[q] == oracle.bfs(o, roots[q])[0]).all()
+
+
+def test_bfs_batch_undirected(gpu_ctx, oracle):
+    rec = _rmat(14, 35)
+    n = 1 << 14
+    g, o = _both(gpu_ctx, oracle, rec, n, False, L.UNDIRECTED)
+    roots = _roots(rec, 70, 35)
+    hops, _ = L.bfs_batch(g, roots)
+    for q in (0, 33, 69):
+        assert (hops[q] == oracle.bfs(o, roots[q])[0]).all()
