@@ -242,6 +242,19 @@ la3_status load_records(la3_ctx* ctx, const uint32_t* rec, uint64_t nrec, uint32
   return graph_from_layout(ctx, L, out);
 }
 
+// Frontier queues: work items plus the counter block
+// {u32 nitems, pad to 64 B, CtrSlot[NSLOT]}.
+constexpr size_t CTR_BYTES = 64 + NSLOT * sizeof(CtrSlot);
+
+la3_status get_queue(la3_ctx* c, const std::string& name, uint64_t cap, Frontier& f) {
+  f.items = (uint64_t*)scratch(c, (name + ".items").c_str(), cap * sizeof(uint64_t));
+  uint8_t* ctr = (uint8_t*)scratch(c, (name + ".ctr").c_str(), CTR_BYTES);
+  if (!f.items || !ctr) return fail(c, LA3_ENOMEM, "frontier allocation failed");
+  f.nitems = (uint32_t*)ctr;
+  f.slots = (CtrSlot*)(ctr + 64);
+  return LA3_OK;
+}
+
 // Per-query frontier buffers (two, ping-ponged).
 struct Fronts {
   Frontier f[2];
@@ -249,34 +262,37 @@ struct Fronts {
 };
 
 la3_status get_fronts(la3_graph* g, Fronts& F, const char* tag) {
-  la3_ctx* c = g->ctx;
   F.cap = (uint64_t)g->nslots + g->nnz / CHUNK + 2;
   for (int i = 0; i < 2; i++) {
-    std::string n1 = std::string(tag) + ".items" + char('0' + i);
-    std::string n2 = std::string(tag) + ".ctr" + char('0' + i);
-    F.f[i].items = (uint64_t*)scratch(c, n1.c_str(), F.cap * sizeof(uint64_t));
-    uint8_t* ctr = (uint8_t*)scratch(c, n2.c_str(), 32);
-    if (!F.f[i].items || !ctr) return fail(c, LA3_ENOMEM, "frontier allocation failed");
-    F.f[i].ctr = (uint32_t*)ctr;
-    F.f[i].edges = (unsigned long long*)(ctr + 16);
+    la3_status st = get_queue(g->ctx, std::string(tag) + char('0' + i), F.cap, F.f[i]);
+    if (st) return st;
   }
   return LA3_OK;
 }
 
 struct FrontCounts {
-  uint32_t items, verts, cols, pad;
-  unsigned long long edges, scanned;
+  uint32_t items = 0, verts = 0, cols = 0;
+  unsigned long long edges = 0, scanned = 0;
 };
 
 la3_status read_counts(la3_ctx* c, const Frontier& f, FrontCounts& out) {
-  HIPC(c, hipMemcpyAsync(c->host_ctr, f.ctr, 32, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipMemcpyAsync(c->host_ctr, f.nitems, CTR_BYTES, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
-  std::memcpy(&out, c->host_ctr, 32);
+  const uint8_t* h = (const uint8_t*)c->host_ctr;
+  out = FrontCounts();
+  std::memcpy(&out.items, h, 4);
+  const CtrSlot* sl = (const CtrSlot*)(h + 64);
+  for (uint32_t i = 0; i < NSLOT; i++) {
+    out.verts += sl[i].verts;
+    out.cols += sl[i].cols;
+    out.edges += sl[i].edges;
+    out.scanned += sl[i].scanned;
+  }
   return LA3_OK;
 }
 
 void zero_counts(la3_ctx* c, const Frontier& f) {
-  launch(c, "fill_u32", k_fill<uint32_t>, 1, f.ctr, (uint64_t)8, 0u);
+  launch(c, "fill_u32", k_fill<uint32_t>, 3, f.nitems, (uint64_t)(CTR_BYTES / 4), 0u);
 }
 
 }  // namespace
@@ -294,7 +310,7 @@ la3_status la3_ctx_create(int device, la3_ctx** out) {
   if (device < 0 || device >= ndev) return LA3_EINVAL;
   if (hipSetDevice(device) != hipSuccess) return LA3_EDEVICE;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return LA3_EDEVICE;
-  if (hipHostMalloc(&c->host_ctr, 64, hipHostMallocDefault) != hipSuccess) return LA3_EDEVICE;
+  if (hipHostMalloc(&c->host_ctr, 4096, hipHostMallocDefault) != hipSuccess) return LA3_EDEVICE;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->ncu = prop.multiProcessorCount;
   *out = c.release();
@@ -489,6 +505,7 @@ la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
   if (!level || !mark) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   const DG& d = g->dg;
   const uint32_t rs = g->slot_of_id[root];
+  zero_counts(c, F.f[0]);
   launch(c, "bfs_init", k_bfs_init, blocks_for(S), d, level, mark, rs, F.f[0]);
   FrontCounts fc;
   if ((st = read_counts(c, F.f[0], fc))) return st;
@@ -563,6 +580,7 @@ la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters
   if (st) return st;
   if (!dv || !stamp) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   const DG& d = g->dg;
+  zero_counts(c, F.f[0]);
   launch(c, "sssp_init", k_sssp_init, blocks_for(S), d, dv, stamp, g->slot_of_id[root], F.f[0]);
   FrontCounts fc;
   if ((st = read_counts(c, F.f[0], fc))) return st;
@@ -753,15 +771,15 @@ static la3_status msbfs_run(la3_graph* g, const std::vector<uint32_t>& root_slot
   uint64_t* fb = (uint64_t*)scratch(c, "msbfs.fb", sizeof(uint64_t) * S * W);
   uint8_t* lev = (uint8_t*)scratch(c, "msbfs.lev", (size_t)S * K);
   uint32_t* stamp = (uint32_t*)scratch(c, "msbfs.stamp", sizeof(uint32_t) * S);
-  uint32_t* touched = (uint32_t*)scratch(c, "msbfs.touched", sizeof(uint32_t) * (S + 1));
-  uint32_t* ntouched = (uint32_t*)scratch(c, "msbfs.ntouched", 64);
   uint32_t* rs = (uint32_t*)scratch(c, "msbfs.roots", sizeof(uint32_t) * k);
   uint64_t* valid = (uint64_t*)scratch(c, "msbfs.valid", sizeof(uint64_t) * W);
   uint8_t* out = (uint8_t*)scratch(c, "msbfs.out", (size_t)S * k);
   Fronts F;
   la3_status st = get_fronts(g, F, "bfs");
   if (st) return st;
-  if (!vis || !fa || !fb || !lev || !stamp || !touched || !ntouched || !rs || !valid || !out)
+  Frontier T;
+  if ((st = get_queue(c, "msbfs.touched", (uint64_t)S + 1, T))) return st;
+  if (!vis || !fa || !fb || !lev || !stamp || !rs || !valid || !out)
     return fail(c, LA3_ENOMEM, "batched BFS: device allocation failed");
   uint64_t hv[W];
   for (int w = 0; w < W; w++) {
@@ -784,17 +802,15 @@ static la3_status msbfs_run(la3_graph* g, const std::vector<uint32_t>& root_slot
     zero_counts(c, nf);
     int ev = -1;
     if (!bottom_up) {
-      launch(c, "fill_u32", k_fill<uint32_t>, 1, ntouched, (uint64_t)1, 0u);
+      zero_counts(c, T);
       const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
       ev = launch(c, "msbfs_push", k_msbfs_push<W>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items,
-                  (const uint64_t*)vis, (const uint64_t*)fa, fb, stamp, iter + 2, touched, ntouched);
-      // The touched count bounds the apply grid; read it back with the counters below.
-      HIPC(c, hipMemcpyAsync((uint8_t*)c->host_ctr + 32, ntouched, 4, hipMemcpyDeviceToHost, c->stream));
-      HIPC(c, hipStreamSynchronize(c->stream));
-      uint32_t nt = 0;
-      std::memcpy(&nt, (uint8_t*)c->host_ctr + 32, 4);
-      if (nt) launch(c, "msbfs_apply", k_msbfs_apply<W>, blocks_for(nt), d, (const uint32_t*)touched, nt, vis,
-                     (const uint64_t*)fb, lev, iter, nf);
+                  (const uint64_t*)vis, (const uint64_t*)fa, fb, stamp, iter + 2, T);
+      // The touched count bounds the apply grid.
+      FrontCounts tc;
+      if ((st = read_counts(c, T, tc))) return st;
+      if (tc.items) launch(c, "msbfs_apply", k_msbfs_apply<W>, blocks_for(tc.items), d, (const uint64_t*)T.items,
+                           tc.items, vis, (const uint64_t*)fb, lev, iter, nf);
     } else {
       ev = launch(c, "msbfs_pull", k_msbfs_pull<W>, blocks_for(g->nrows, wave_blocks), d, vis,
                   (const uint64_t*)fa, fb, lev, iter, (const uint64_t*)valid, nf);

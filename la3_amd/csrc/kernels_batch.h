@@ -48,6 +48,7 @@ __global__ void k_msbfs_roots(DG g, const uint32_t* root_slots, uint32_t k, uint
 __global__ void __launch_bounds__(BLOCK) k_msbfs_seed(DG g, const uint32_t* root_slots, uint32_t k,
                                                       uint32_t* stamp, Frontier f) {
   const uint32_t stride = gridDim.x * BLOCK;
+  LA3_EMITTER(em);
   for (uint32_t base = blockIdx.x * BLOCK; base < k; base += stride) {
     const uint32_t q = base + threadIdx.x;
     bool first = false;
@@ -61,8 +62,9 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_seed(DG g, const uint32_t* root
         nit = (ne + CHUNK - 1) / CHUNK;
       }
     }
-    wave_emit(f, s, nit, ne, first);
+    em_push(em, f, s, nit, ne, first);
   }
+  em_finish(em, f);
 }
 
 // Top-down step: every entry (c -> r) of an active column ORs c's frontier
@@ -72,9 +74,10 @@ template <int W>
 __global__ void __launch_bounds__(BLOCK) k_msbfs_push(DG g, const uint64_t* items, uint32_t nitems,
                                                       const uint64_t* vis, const uint64_t* fa,
                                                       uint64_t* fb, uint32_t* stamp, uint32_t tag,
-                                                      uint32_t* touched, uint32_t* ntouched) {
+                                                      Frontier touched) {
   const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
   const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  LA3_EMITTER(em);
   for (uint32_t it = wid; it < nitems; it += nwaves) {
     const uint64_t item = items[it];
     const uint32_t s = (uint32_t)item, ch = (uint32_t)(item >> 32);
@@ -98,32 +101,28 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_push(DG g, const uint64_t* item
         }
         if (any) first = atomicMax(&stamp[r], tag) < tag;
       }
-      const uint64_t b = __ballot(first);
-      if (b) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(ntouched, (uint32_t)__popcll(b));
-        base = __shfl(base, 0, WAVE);
-        if (first) touched[base + __popcll(b & ((1ull << lane_id()) - 1))] = r;
-      }
+      em_push(em, touched, r, first ? 1u : 0u, 0, first);
     }
   }
+  em_finish(em, touched);
 }
 
 // Marks the new bits of each touched vertex visited, records their level and
 // emits the vertex's column as push work for the next step.
 template <int W>
-__global__ void __launch_bounds__(BLOCK) k_msbfs_apply(DG g, const uint32_t* touched,
+__global__ void __launch_bounds__(BLOCK) k_msbfs_apply(DG g, const uint64_t* touched,
                                                        uint32_t ntouched, uint64_t* vis,
                                                        const uint64_t* fb, uint8_t* lev,
                                                        uint32_t iter, Frontier nf) {
   const uint32_t stride = gridDim.x * BLOCK;
   const uint8_t nxt = (uint8_t)(iter + 1);
+  LA3_EMITTER(em);
   for (uint32_t base = blockIdx.x * BLOCK; base < ntouched; base += stride) {
     const uint32_t i = base + threadIdx.x;
     bool act = false;
     uint32_t r = 0, nit = 0, ne = 0;
     if (i < ntouched) {
-      r = touched[i];
+      r = (uint32_t)touched[i];
 #pragma unroll
       for (int w = 0; w < W; w++) {
         uint64_t m = fb[(size_t)r * W + w];
@@ -142,8 +141,9 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_apply(DG g, const uint32_t* tou
         nit = (ne + CHUNK - 1) / CHUNK;
       }
     }
-    wave_emit(nf, r, nit, ne, act);
+    em_push(em, nf, r, nit, ne, act);
   }
+  em_finish(em, nf);
 }
 
 // Clears the current frontier words of the vertices that were active.
@@ -165,6 +165,7 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_pull(DG g, uint64_t* vis, const
                                                       const uint64_t* valid, Frontier nf) {
   const uint32_t stride = gridDim.x * BLOCK;
   const uint8_t nxt = (uint8_t)(iter + 1);
+  LA3_EMITTER(em);
   for (uint32_t base = blockIdx.x * BLOCK; base < g.nrows; base += stride) {
     const uint32_t r = base + threadIdx.x;
     bool act = false;
@@ -211,10 +212,10 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_pull(DG g, uint64_t* vis, const
         nit = (ne + CHUNK - 1) / CHUNK;
       }
     }
-    scanned = wave_sum(scanned);
-    if (lane_id() == 0 && scanned) atomicAdd(&nf.edges[1], (unsigned long long)scanned);
-    wave_emit(nf, r, nit, ne, act);
+    em.scanned += scanned;
+    em_push(em, nf, r, nit, ne, act);
   }
+  em_finish(em, nf);
 }
 
 template <int W>
@@ -224,6 +225,7 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_pull_long(DG g, uint64_t* vis, 
   const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
   const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
   const uint8_t nxt = (uint8_t)(iter + 1);
+  LA3_EMITTER(em);
   for (uint32_t i = wid; i < g.nlong; i += nwaves) {
     const uint32_t r = g.long_rows[i];
     uint64_t need[W], acc[W];
@@ -260,7 +262,7 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_pull_long(DG g, uint64_t* vis, 
     bool act = false;
     uint32_t nit = 0, ne = 0;
     if (lane_id() == 0) {
-      atomicAdd(&nf.edges[1], (unsigned long long)scanned);
+      em.scanned += scanned;
 #pragma unroll
       for (int w = 0; w < W; w++) {
         uint64_t m = acc[w];
@@ -280,8 +282,9 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_pull_long(DG g, uint64_t* vis, 
         nit = (ne + CHUNK - 1) / CHUNK;
       }
     }
-    wave_emit(nf, r, nit, ne, act);
+    em_push(em, nf, r, nit, ne, act);
   }
+  em_finish(em, nf);
 }
 
 // hops[q * nslots + id] = lev[slot(id) * K + q] for q < k.

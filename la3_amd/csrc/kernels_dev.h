@@ -64,35 +64,102 @@ __device__ __forceinline__ void st_relaxed(uint32_t* p, uint32_t v) {
 }
 
 // Traversal frontier bookkeeping, one set per query.
+// Counters are spread over NSLOT slots (block b adds into slot b % NSLOT) so
+// that no single address takes an atomic from every wavefront; the host sums
+// the slots. Work items go through a per-wavefront LDS queue and reach HBM
+// with one reservation per QCAP items.
+constexpr uint32_t NSLOT = 64;
+constexpr uint32_t QCAP = 256;   // items per wavefront LDS queue
+constexpr uint32_t BIG_NIT = 4;  // vertices with more items bypass the LDS queue
+
+struct CtrSlot {
+  uint32_t verts, cols;          // vertices emitted / with a non-empty column
+  unsigned long long edges;      // entries behind the emitted items
+  unsigned long long scanned;    // row entries read by pull steps
+  unsigned long long pad;
+};
 struct Frontier {
-  uint64_t* items;            // work items: (chunk << 32) | slot
-  uint32_t* ctr;              // [0] items emitted, [1] vertices emitted,
-                              // [2] vertices emitted with a non-empty column
-  unsigned long long* edges;  // [0] entries behind the emitted items,
-                              // [1] row entries read by pull steps
+  uint64_t* items;               // work items: (chunk << 32) | slot
+  uint32_t* nitems;              // [0] items emitted
+  CtrSlot* slots;                // [NSLOT]
 };
 
-// Wave-aggregated append of `nit` work items for `slot` (nit may be 0).
-// Every lane of the wave must call this.
-__device__ __forceinline__ void wave_emit(const Frontier& f, uint32_t slot, uint32_t nit,
-                                          uint32_t nedges, bool counted) {
-  uint32_t incl = wave_incl_scan(nit);
-  uint32_t total = __shfl(incl, WAVE - 1, WAVE);
-  uint64_t b = __ballot(counted);
-  uint32_t e = wave_sum(nedges);
-  if (b == 0) return;
-  uint32_t base = 0;
-  const uint32_t ncol = (uint32_t)__popcll(__ballot(nit > 0));
-  if (lane_id() == WAVE - 1) {
-    if (total) base = atomicAdd(&f.ctr[0], total);
-    atomicAdd(&f.ctr[1], (uint32_t)__popcll(b));
-    if (ncol) atomicAdd(&f.ctr[2], ncol);
-    atomicAdd(&f.edges[0], (unsigned long long)e);
-  }
-  base = __shfl(base, WAVE - 1, WAVE);
-  uint32_t o = base + incl - nit;
-  for (uint32_t k = 0; k < nit; k++) f.items[o + k] = ((uint64_t)k << 32) | slot;
+// Per-wavefront emitter. cnt is wave-uniform; the counters are per lane.
+struct Emitter {
+  uint64_t* q;
+  uint32_t cnt;
+  uint32_t verts, cols;
+  unsigned long long edges, scanned;
+};
+
+__device__ __forceinline__ void em_init(Emitter& em, uint64_t* lds) {
+  em.q = lds + (threadIdx.x / WAVE) * QCAP;
+  em.cnt = 0;
+  em.verts = em.cols = 0;
+  em.edges = em.scanned = 0;
 }
+
+__device__ __forceinline__ void em_flush(Emitter& em, const Frontier& f) {
+  if (em.cnt == 0) return;
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(f.nitems, em.cnt);
+  base = __shfl(base, 0, WAVE);
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t i = lane_id(); i < em.cnt; i += WAVE) f.items[base + i] = em.q[i];
+  __builtin_amdgcn_wave_barrier();
+  em.cnt = 0;
+}
+
+// Every lane of the wave must call this (nit may be 0).
+__device__ __forceinline__ void em_push(Emitter& em, const Frontier& f, uint32_t slot, uint32_t nit,
+                                        uint32_t nedges, bool counted) {
+  em.verts += counted;
+  em.cols += nit > 0;
+  em.edges += nedges;
+  const bool big = nit > BIG_NIT;
+  const uint32_t small = big ? 0 : nit;
+  const uint32_t incl = wave_incl_scan(small);
+  const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
+  if (total) {
+    if (em.cnt + total > QCAP) em_flush(em, f);
+    const uint32_t o = em.cnt + incl - small;
+    for (uint32_t k = 0; k < small; k++) em.q[o + k] = ((uint64_t)k << 32) | slot;
+    em.cnt += total;
+  }
+  if (big) {  // hubs: reserve their own range
+    const uint32_t base = atomicAdd(f.nitems, nit);
+    for (uint32_t k = 0; k < nit; k++) f.items[base + k] = ((uint64_t)k << 32) | slot;
+  }
+}
+
+// Flushes the queue and adds the block's counters into its slot. Must be
+// called once by every thread of the block (contains a block barrier).
+__device__ __forceinline__ void em_finish(Emitter& em, const Frontier& f) {
+  em_flush(em, f);
+  __shared__ unsigned long long red[4][BLOCK / WAVE];
+  const uint32_t w = threadIdx.x / WAVE;
+  const unsigned long long v0 = wave_sum((unsigned long long)em.verts);
+  const unsigned long long v1 = wave_sum((unsigned long long)em.cols);
+  const unsigned long long v2 = wave_sum(em.edges);
+  const unsigned long long v3 = wave_sum(em.scanned);
+  if (lane_id() == 0) { red[0][w] = v0; red[1][w] = v1; red[2][w] = v2; red[3][w] = v3; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 4; i++)
+      for (uint32_t j = 0; j < BLOCK / WAVE; j++) t[i] += red[i][j];
+    CtrSlot& cs = f.slots[blockIdx.x % NSLOT];
+    if (t[0]) atomicAdd(&cs.verts, (uint32_t)t[0]);
+    if (t[1]) atomicAdd(&cs.cols, (uint32_t)t[1]);
+    if (t[2]) atomicAdd(&cs.edges, t[2]);
+    if (t[3]) atomicAdd(&cs.scanned, t[3]);
+  }
+}
+
+#define LA3_EMITTER(em)                                  \
+  __shared__ uint64_t em##_lds[(BLOCK / WAVE) * QCAP];   \
+  Emitter em;                                            \
+  em_init(em, em##_lds)
 
 // ------------------------------------------------------------- utilities ---
 template <class T>
@@ -147,11 +214,10 @@ __global__ void k_bfs_init(DG g, uint8_t* level, uint32_t* mark, uint32_t root, 
       nit = (ne + CHUNK - 1) / CHUNK;
     }
     for (uint32_t k = 0; k < nit; k++) f.items[k] = ((uint64_t)k << 32) | root;
-    f.ctr[0] = nit;
-    f.ctr[1] = 1;
-    f.ctr[2] = nit > 0;
-    f.edges[0] = ne;
-    f.edges[1] = 0;
+    f.nitems[0] = nit;
+    f.slots[0].verts = 1;
+    f.slots[0].cols = nit > 0;
+    f.slots[0].edges = ne;
   }
 }
 
@@ -163,6 +229,7 @@ __global__ void __launch_bounds__(BLOCK) k_bfs_push(DG g, const uint64_t* items,
   const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
   const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
   const uint8_t nxt = (uint8_t)(iter + 1);
+  LA3_EMITTER(em);
   for (uint32_t it = wid; it < nitems; it += nwaves) {
     const uint64_t item = items[it];
     const uint32_t s = (uint32_t)item, ch = (uint32_t)(item >> 32);
@@ -188,9 +255,10 @@ __global__ void __launch_bounds__(BLOCK) k_bfs_push(DG g, const uint64_t* items,
         ne = g.col_ptr[cr + 1] - g.col_ptr[cr];
         nit = (ne + CHUNK - 1) / CHUNK;
       }
-      wave_emit(nf, r, nit, ne, disc);
+      em_push(em, nf, r, nit, ne, disc);
     }
   }
+  em_finish(em, nf);
 }
 
 // Bottom-up (pull) step: each unreached row scans its in-entries from the
@@ -200,6 +268,7 @@ __global__ void __launch_bounds__(BLOCK) k_bfs_pull(DG g, uint8_t* level, uint32
                                                     uint32_t iter, Frontier nf) {
   const uint8_t cur = (uint8_t)iter, nxt = (uint8_t)(iter + 1);
   const uint32_t stride = gridDim.x * BLOCK;
+  LA3_EMITTER(em);
   for (uint32_t base = blockIdx.x * BLOCK; base < g.nrows; base += stride) {
     const uint32_t r = base + threadIdx.x;
     bool disc = false;
@@ -214,8 +283,7 @@ __global__ void __launch_bounds__(BLOCK) k_bfs_pull(DG g, uint8_t* level, uint32
         }
       }
     }
-    scanned = wave_sum(scanned);
-    if (lane_id() == 0 && scanned) atomicAdd(&nf.edges[1], (unsigned long long)scanned);
+    em.scanned += scanned;
     uint32_t nit = 0, ne = 0;
     if (disc) {
       level[r] = nxt;
@@ -225,8 +293,9 @@ __global__ void __launch_bounds__(BLOCK) k_bfs_pull(DG g, uint8_t* level, uint32
         nit = (ne + CHUNK - 1) / CHUNK;
       }
     }
-    wave_emit(nf, r, nit, ne, disc);
+    em_push(em, nf, r, nit, ne, disc);
   }
+  em_finish(em, nf);
 }
 
 // Bottom-up for long rows: one wavefront per row, scanning 64 entries at a
@@ -236,6 +305,7 @@ __global__ void __launch_bounds__(BLOCK) k_bfs_pull_long(DG g, uint8_t* level, u
   const uint8_t cur = (uint8_t)iter, nxt = (uint8_t)(iter + 1);
   const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
   const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  LA3_EMITTER(em);
   for (uint32_t i = wid; i < g.nlong; i += nwaves) {
     const uint32_t r = g.long_rows[i];
     if (level[r] != (uint8_t)BFS_INF) continue;  // wave-uniform
@@ -256,7 +326,7 @@ __global__ void __launch_bounds__(BLOCK) k_bfs_pull_long(DG g, uint8_t* level, u
         found = true;
       }
     }
-    if (lane_id() == 0) atomicAdd(&nf.edges[1], (unsigned long long)scanned);
+    if (lane_id() == 0) em.scanned += scanned;
     uint32_t nit = 0, ne = 0;
     bool disc = found && lane_id() == 0;
     if (disc) {
@@ -267,8 +337,9 @@ __global__ void __launch_bounds__(BLOCK) k_bfs_pull_long(DG g, uint8_t* level, u
         nit = (ne + CHUNK - 1) / CHUNK;
       }
     }
-    wave_emit(nf, r, nit, ne, disc);
+    em_push(em, nf, r, nit, ne, disc);
   }
+  em_finish(em, nf);
 }
 
 // hops / parent by original id. parent = hashed id of the winning column's
@@ -306,11 +377,10 @@ __global__ void k_sssp_init(DG g, uint32_t* dist, uint32_t* stamp, uint32_t root
       nit = (ne + CHUNK - 1) / CHUNK;
     }
     for (uint32_t k = 0; k < nit; k++) f.items[k] = ((uint64_t)k << 32) | root;
-    f.ctr[0] = nit;
-    f.ctr[1] = 1;
-    f.ctr[2] = nit > 0;
-    f.edges[0] = ne;
-    f.edges[1] = 0;
+    f.nitems[0] = nit;
+    f.slots[0].verts = 1;
+    f.slots[0].cols = nit > 0;
+    f.slots[0].edges = ne;
   }
 }
 
@@ -320,6 +390,7 @@ __global__ void __launch_bounds__(BLOCK) k_sssp_push(DG g, const uint64_t* items
   const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
   const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
   const uint32_t tag = round + 1;
+  LA3_EMITTER(em);
   for (uint32_t it = wid; it < nitems; it += nwaves) {
     const uint64_t item = items[it];
     const uint32_t s = (uint32_t)item, ch = (uint32_t)(item >> 32);
@@ -344,9 +415,10 @@ __global__ void __launch_bounds__(BLOCK) k_sssp_push(DG g, const uint64_t* items
         ne = g.col_split[r] - g.col_ptr[r];
         nit = (ne + CHUNK - 1) / CHUNK;
       }
-      wave_emit(nf, r, nit, ne, push);
+      em_push(em, nf, r, nit, ne, push);
     }
   }
+  em_finish(em, nf);
 }
 
 // Sink step (vertex_program_execute.hpp:131-148): every sink row takes the
