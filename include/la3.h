@@ -202,6 +202,12 @@ la3_status la3_kernel_stats(const la3_ctx* ctx, la3_kernel_stat* out, uint32_t c
                             uint32_t* count);
 la3_status la3_kernel_stats_reset(la3_ctx* ctx);
 
+/* Page-locked host memory (hipHostMalloc) for result buffers: device-to-host
+ * copies into it run at full link speed. Any la3_* output pointer may point
+ * into it. */
+la3_status la3_host_alloc(uint64_t bytes, void** out);
+void la3_host_free(void* p);
+
 /* -------------------------------------------------------------- tools --- */
 /* Seeded R-MAT generator (a, b, c, d = 1-a-b-c), 2^scale vertices with ids
  * 1..2^scale, edgefactor*2^scale samples, self-loops and duplicates removed,

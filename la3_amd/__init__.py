@@ -9,6 +9,7 @@ raises if it is missing: there is no CPU fallback.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -29,7 +30,7 @@ EXPORTS = [
     "la3_degree", "la3_pagerank", "la3_bfs", "la3_sssp", "la3_cc", "la3_tc", "la3_ppr",
     "la3_bfs_batch", "la3_sssp_batch", "la3_ppr_batch",
     "la3_last_timing", "la3_set_profiling", "la3_rmat_generate",
-    "la3_kernel_stats", "la3_kernel_stats_reset",
+    "la3_kernel_stats", "la3_kernel_stats_reset", "la3_host_alloc", "la3_host_free",
 ]
 
 
@@ -97,6 +98,8 @@ def lib():
             "la3_set_profiling": (i32, [P, i32]),
             "la3_kernel_stats": (i32, [P, P, u32, ctypes.POINTER(u32)]),
             "la3_kernel_stats_reset": (i32, [P]),
+            "la3_host_alloc": (i32, [u64, ctypes.POINTER(P)]),
+            "la3_host_free": (None, [P]),
             "la3_rmat_generate": (i32, [u32, u32, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, u64, i32, P, ctypes.POINTER(u64)]),
         }
@@ -110,6 +113,26 @@ def lib():
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def pinned(shape, dtype):
+    """A numpy array in page-locked host memory (freed with the array)."""
+    dtype = np.dtype(dtype)
+    nbytes = int(np.prod(shape)) * dtype.itemsize
+    p = ctypes.c_void_p()
+    st = lib().la3_host_alloc(max(nbytes, 1), ctypes.byref(p))
+    if st:
+        raise LA3Error(st, f"host_alloc({nbytes})")
+    buf = (ctypes.c_char * max(nbytes, 1)).from_address(p.value)
+    weakref.finalize(buf, lib().la3_host_free, p.value)
+    return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+
+def _out(out, shape, dtype):
+    if out is None:
+        return np.zeros(shape, dtype=dtype)
+    assert out.shape == tuple(shape) and out.dtype == np.dtype(dtype) and out.flags.c_contiguous
+    return out
 
 
 def rmat(scale, edgefactor=16, a=0.57, b=0.19, c=0.19, seed=1, weighted=False):
@@ -274,16 +297,16 @@ def ppr(g, seed, iters, alpha=0.15):
     return rank
 
 
-def bfs(g, root, parents=True):
-    hops = np.zeros(g.nslots, dtype=np.uint8)
+def bfs(g, root, parents=True, out=None):
+    hops = _out(out, (g.nslots,), np.uint8)
     par = np.zeros(g.nslots, dtype=np.uint32) if parents else None
     it = np.zeros(1, dtype=np.uint32)
     g.ctx.check(lib().la3_bfs(g.h, int(root), _ptr(hops), _ptr(par), _ptr(it)), "bfs")
     return hops, par, int(it[0])
 
 
-def sssp(g, root):
-    dist = np.zeros(g.nslots, dtype=np.uint32)
+def sssp(g, root, out=None):
+    dist = _out(out, (g.nslots,), np.uint32)
     it = np.zeros(1, dtype=np.uint32)
     g.ctx.check(lib().la3_sssp(g.h, int(root), _ptr(dist), _ptr(it)), "sssp")
     return dist, int(it[0])
@@ -303,27 +326,27 @@ def tc(g):
     return int(total[0]), per
 
 
-def bfs_batch(g, roots):
+def bfs_batch(g, roots, out=None):
     roots = np.ascontiguousarray(roots, dtype=np.uint32)
-    hops = np.zeros((roots.size, g.nslots), dtype=np.uint8)
+    hops = _out(out, (roots.size, g.nslots), np.uint8)
     it = np.zeros(1, dtype=np.uint32)
     g.ctx.check(lib().la3_bfs_batch(g.h, _ptr(roots), roots.size, _ptr(hops), _ptr(it)),
                 "bfs_batch")
     return hops, int(it[0])
 
 
-def sssp_batch(g, roots):
+def sssp_batch(g, roots, out=None):
     roots = np.ascontiguousarray(roots, dtype=np.uint32)
-    dist = np.zeros((roots.size, g.nslots), dtype=np.uint32)
+    dist = _out(out, (roots.size, g.nslots), np.uint32)
     it = np.zeros(1, dtype=np.uint32)
     g.ctx.check(lib().la3_sssp_batch(g.h, _ptr(roots), roots.size, _ptr(dist), _ptr(it)),
                 "sssp_batch")
     return dist, int(it[0])
 
 
-def ppr_batch(g, seeds, iters, alpha=0.15):
+def ppr_batch(g, seeds, iters, alpha=0.15, out=None):
     seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
-    rank = np.zeros((seeds.size, g.nslots), dtype=np.float32)
+    rank = _out(out, (seeds.size, g.nslots), np.float32)
     g.ctx.check(lib().la3_ppr_batch(g.h, _ptr(seeds), seeds.size, int(iters), float(alpha),
                                     _ptr(rank)), "ppr_batch")
     return rank

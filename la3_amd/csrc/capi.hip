@@ -505,8 +505,7 @@ la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
   if (!level || !mark) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   const DG& d = g->dg;
   const uint32_t rs = g->slot_of_id[root];
-  zero_counts(c, F.f[0]);
-  launch(c, "bfs_init", k_bfs_init, blocks_for(S), d, level, mark, rs, F.f[0]);
+  launch(c, "bfs_init", k_bfs_init, blocks_for(S), d, level, mark, rs, F.f[0], F.f[1].nitems);
   FrontCounts fc;
   if ((st = read_counts(c, F.f[0], fc))) return st;
   uint32_t iter = 0, cur = 0;
@@ -518,13 +517,12 @@ la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
   // that emits work items, so the loop runs while items exist.
   while (fc.items > 0) {
     const Frontier& nf = F.f[cur ^ 1];
-    zero_counts(c, nf);
     int ev = -1;
     if (!bottom_up) {
       const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
-      ev = launch(c, "bfs_push", k_bfs_push, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, level, mark, iter, nf);
+      ev = launch(c, "bfs_push", k_bfs_push, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, level, mark, iter, nf, F.f[cur].nitems);
     } else {
-      ev = launch(c, "bfs_pull", k_bfs_pull, blocks_for(g->nrows, 4 * c->ncu * 8), d, level, mark, iter, nf);
+      ev = launch(c, "bfs_pull", k_bfs_pull, blocks_for(g->nrows, 4 * c->ncu * 8), d, level, mark, iter, nf, F.f[cur].nitems);
       if (d.nlong) launch(c, "bfs_pull_long", k_bfs_pull_long, std::min<uint32_t>((d.nlong + 3) / 4, wave_blocks), d, level, mark, iter, nf);
     }
     FrontCounts nc;
@@ -580,17 +578,15 @@ la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters
   if (st) return st;
   if (!dv || !stamp) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   const DG& d = g->dg;
-  zero_counts(c, F.f[0]);
-  launch(c, "sssp_init", k_sssp_init, blocks_for(S), d, dv, stamp, g->slot_of_id[root], F.f[0]);
+  launch(c, "sssp_init", k_sssp_init, blocks_for(S), d, dv, stamp, g->slot_of_id[root], F.f[0], F.f[1].nitems);
   FrontCounts fc;
   if ((st = read_counts(c, F.f[0], fc))) return st;
   uint32_t round = 0, cur = 0;
   const uint32_t wave_blocks = 4 * c->ncu * 8;
   while (fc.items > 0) {
     const Frontier& nf = F.f[cur ^ 1];
-    zero_counts(c, nf);
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
-    const int ev = launch(c, "sssp_push", k_sssp_push, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dv, stamp, round, nf);
+    const int ev = launch(c, "sssp_push", k_sssp_push, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dv, stamp, round, nf, F.f[cur].nitems);
     c->edges_scanned += fc.edges;
     FrontCounts nc;
     if ((st = read_counts(c, nf, nc))) return st;
@@ -604,7 +600,7 @@ la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters
   if (g->nsink) launch(c, "sssp_sinks", k_sssp_sinks, blocks_for(g->nsink), d, dv);
   uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
   if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
-  launch(c, "scatter_u32", k_scatter_by_id<uint32_t>, blocks_for(S), d, (const uint32_t*)dv, out);
+  launch(c, "gather_u32", k_gather_by_id<uint32_t>, blocks_for(S), d, (const uint32_t*)dv, out);
   HIPC(c, hipMemcpyAsync(dist, out, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   timing_end(c);
@@ -669,12 +665,12 @@ static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t
   }
   double* outd = (double*)scratch(c, "out.f64", sizeof(double) * S);
   if (!outd) return fail(c, LA3_ENOMEM, "scratch allocation failed");
-  launch(c, "scatter_f64", k_scatter_by_id<double>, blocks_for(S), d, (const double*)rk, outd);
+  launch(c, "gather_f64", k_gather_by_id<double>, blocks_for(S), d, (const double*)rk, outd);
   HIPC(c, hipMemcpyAsync(rank, outd, sizeof(double) * S, hipMemcpyDeviceToHost, c->stream));
   if (degree) {
     uint32_t* outu = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
     if (!outu) return fail(c, LA3_ENOMEM, "scratch allocation failed");
-    launch(c, "scatter_u32", k_scatter_by_id<uint32_t>, blocks_for(S), d, (const uint32_t*)deg, outu);
+    launch(c, "gather_u32", k_gather_by_id<uint32_t>, blocks_for(S), d, (const uint32_t*)deg, outu);
     HIPC(c, hipMemcpyAsync(degree, outu, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
   }
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -746,7 +742,7 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
   if (per_vertex) {
     uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
     if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
-    launch(c, "scatter_u32", k_scatter_by_id<uint32_t>, blocks_for(S), d, (const uint32_t*)per, out);
+    launch(c, "gather_u32", k_gather_by_id<uint32_t>, blocks_for(S), d, (const uint32_t*)per, out);
     HIPC(c, hipMemcpyAsync(per_vertex, out, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
   }
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -791,7 +787,7 @@ static la3_status msbfs_run(la3_graph* g, const std::vector<uint32_t>& root_slot
   launch(c, "msbfs_init", k_msbfs_init<W>, blocks_for(S), d, vis, fa, fb, (uint64_t*)lev, stamp);
   launch(c, "msbfs_roots", k_msbfs_roots<W>, 1, d, (const uint32_t*)rs, k, vis, fa, lev);
   zero_counts(c, F.f[0]);
-  launch(c, "msbfs_seed", k_msbfs_seed, blocks_for(k), d, (const uint32_t*)rs, k, stamp, F.f[0]);
+  launch(c, "msbfs_seed", k_msbfs_seed, blocks_for(k), d, (const uint32_t*)rs, k, stamp, F.f[0], 0);
   FrontCounts fc;
   if ((st = read_counts(c, F.f[0], fc))) return st;
   uint32_t iter = 0, cur = 0;
@@ -871,15 +867,123 @@ la3_status la3_bfs_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint8_
   return st;
 }
 
+}  // extern "C"
+
+template <int QW>
+static la3_status mssp_run(la3_graph* g, const std::vector<uint32_t>& root_slots, uint32_t k,
+                           uint32_t* dist_out, uint32_t* iters_run) {
+  la3_ctx* c = g->ctx;
+  const uint32_t S = g->nslots;
+  constexpr uint32_t K = 64 * QW;
+  const DG& d = g->dg;
+  uint32_t* dist = (uint32_t*)scratch(c, "mssp.dist", sizeof(uint32_t) * (size_t)S * K);
+  uint64_t* cm = (uint64_t*)scratch(c, "mssp.cm", sizeof(uint64_t) * (size_t)S * QW);
+  uint64_t* nm = (uint64_t*)scratch(c, "mssp.nm", sizeof(uint64_t) * (size_t)S * QW);
+  uint32_t* stamp = (uint32_t*)scratch(c, "mssp.stamp", sizeof(uint32_t) * S);
+  uint32_t* rs = (uint32_t*)scratch(c, "mssp.roots", sizeof(uint32_t) * k);
+  uint32_t* out = (uint32_t*)scratch(c, "mssp.out", sizeof(uint32_t) * (size_t)S * k);
+  Fronts F;
+  la3_status st = get_fronts(g, F, "bfs");
+  if (st) return st;
+  Frontier T;
+  if ((st = get_queue(c, "mssp.touched", (uint64_t)S + 1, T))) return st;
+  if (!dist || !cm || !nm || !stamp || !rs || !out)
+    return fail(c, LA3_ENOMEM, "batched SSSP: device allocation failed");
+  HIPC(c, hipMemcpyAsync(rs, root_slots.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream));
+  const uint64_t inf2 = ((uint64_t)SSSP_INF << 32) | SSSP_INF;
+  launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for((uint64_t)S * K / 2), (unsigned long long*)dist,
+         (uint64_t)S * K / 2, (unsigned long long)inf2);
+  launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for((uint64_t)S * QW), (unsigned long long*)cm,
+         (uint64_t)S * QW, 0ull);
+  launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for((uint64_t)S * QW), (unsigned long long*)nm,
+         (uint64_t)S * QW, 0ull);
+  launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(S), stamp, (uint64_t)S, 0u);
+  launch(c, "mssp_roots", k_mssp_roots<QW>, 1, d, (const uint32_t*)rs, k, dist, cm);
+  zero_counts(c, F.f[0]);
+  zero_counts(c, F.f[1]);
+  zero_counts(c, T);
+  launch(c, "msbfs_seed", k_msbfs_seed, blocks_for(k), d, (const uint32_t*)rs, k, stamp, F.f[0], 1);
+  FrontCounts fc;
+  if ((st = read_counts(c, F.f[0], fc))) return st;
+  uint32_t round = 0, cur = 0;
+  const uint32_t wave_blocks = 4 * c->ncu * 8;
+  while (fc.items > 0) {
+    const Frontier& nf = F.f[cur ^ 1];
+    const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
+    const int ev = launch(c, "mssp_push", k_mssp_push<QW>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items,
+                          dist, (const uint64_t*)cm, nm, stamp, round + 2, T, F.f[cur].nitems);
+    FrontCounts tc;
+    if ((st = read_counts(c, T, tc))) return st;
+    // The touched list becomes the next frontier's work items; clearing T's
+    // counters happens inside that launch.
+    launch(c, "mssp_items", k_mssp_items, blocks_for(std::max<uint32_t>(tc.items, 1)), d, (const uint64_t*)T.items,
+           tc.items, nf, T.nitems);
+    // cm of the finished step is cleared; nm becomes the next step's cm.
+    launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for((uint64_t)S * QW), (unsigned long long*)cm,
+           (uint64_t)S * QW, 0ull);
+    std::swap(cm, nm);
+    FrontCounts nc;
+    if ((st = read_counts(c, nf, nc))) return st;
+    // Algorithmic bytes: (4 + w) per entry plus K x 4 B of distances read per
+    // entry (the k-wide message row), K x 4 B per active column, and
+    // K/8 B of masks per touched row.
+    annotate(c, ev, (8.0 + 4.0 * K) * fc.edges + (8.0 + 4.0 * K) * fc.cols + (K / 8.0 + 4) * tc.items, fc.edges);
+    c->edges_scanned += fc.edges;
+    round++;
+    fc = nc;
+    cur ^= 1;
+  }
+  if (g->nsink) launch(c, "mssp_sinks", k_mssp_sinks<QW>, std::min<uint32_t>((g->nsink + 3) / 4, wave_blocks), d, dist);
+  launch(c, "mssp_output", k_batch_output<uint32_t>, blocks_for(S), d, (const uint32_t*)dist, K, k, out);
+  HIPC(c, hipMemcpyAsync(dist_out, out, sizeof(uint32_t) * (size_t)S * k, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  if (iters_run) *iters_run = round + 1;
+  return LA3_OK;
+}
+
+extern "C" {
+
 la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint32_t* dist,
                           uint32_t* iters_run) {
-  (void)roots; (void)k; (void)dist; (void)iters_run;
-  return fail(g ? g->ctx : nullptr, LA3_ENOTSUP, "batched SSSP not built yet");
+  if (!g || !roots || !dist) return LA3_EINVAL;
+  la3_ctx* c = g->ctx;
+  if (k == 0 || k > 1024) return fail(c, LA3_EINVAL, "batch size must be in [1, 1024]");
+  if (!g->weighted) return fail(c, LA3_EINVAL, "SSSP needs a weighted graph");
+  if (!g->directed) return fail(c, LA3_EINVAL, "SSSP runs on a directed load");
+  std::vector<uint32_t> rs(k);
+  for (uint32_t q = 0; q < k; q++) {
+    if (roots[q] > g->n) return fail(c, LA3_EINVAL, "root id out of range");
+    rs[q] = g->slot_of_id[roots[q]];
+  }
+  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+  HIPC(c, hipSetDevice(c->device));
+  timing_begin(c);
+  la3_status st;
+  const uint32_t QW = (k + 63) / 64;
+  if (QW <= 1) st = mssp_run<1>(g, rs, k, dist, iters_run);
+  else if (QW <= 2) st = mssp_run<2>(g, rs, k, dist, iters_run);
+  else if (QW <= 4) st = mssp_run<4>(g, rs, k, dist, iters_run);
+  else if (QW <= 8) st = mssp_run<8>(g, rs, k, dist, iters_run);
+  else st = mssp_run<16>(g, rs, k, dist, iters_run);
+  timing_end(c);
+  return st;
 }
+
 la3_status la3_ppr_batch(la3_graph* g, const uint32_t* seeds, uint32_t k, uint32_t iters,
                          float alpha, float* rank) {
   (void)seeds; (void)k; (void)iters; (void)alpha; (void)rank;
   return fail(g ? g->ctx : nullptr, LA3_ENOTSUP, "batched PPR not built yet");
+}
+
+la3_status la3_host_alloc(uint64_t bytes, void** out) {
+  if (!out) return LA3_EINVAL;
+  *out = nullptr;
+  if (hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) return LA3_ENOMEM;
+  return LA3_OK;
+}
+
+void la3_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
 }
 
 // -------------------------------------------------------------- tools ---

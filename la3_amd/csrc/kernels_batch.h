@@ -46,7 +46,7 @@ __global__ void k_msbfs_roots(DG g, const uint32_t* root_slots, uint32_t k, uint
 }
 
 __global__ void __launch_bounds__(BLOCK) k_msbfs_seed(DG g, const uint32_t* root_slots, uint32_t k,
-                                                      uint32_t* stamp, Frontier f) {
+                                                      uint32_t* stamp, Frontier f, int regular_only) {
   const uint32_t stride = gridDim.x * BLOCK;
   LA3_EMITTER(em);
   for (uint32_t base = blockIdx.x * BLOCK; base < k; base += stride) {
@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_seed(DG g, const uint32_t* root
       first = atomicMax(&stamp[s], 1u) < 1u;
       if (first && slot_has_col(g, s)) {
         const uint32_t c = col_of_slot(g, s);
-        ne = g.col_ptr[c + 1] - g.col_ptr[c];
+        ne = (regular_only ? g.col_split[c] : g.col_ptr[c + 1]) - g.col_ptr[c];
         nit = (ne + CHUNK - 1) / CHUNK;
       }
     }
@@ -285,6 +285,133 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_pull_long(DG g, uint64_t* vis, 
     em_push(em, nf, r, nit, ne, act);
   }
   em_finish(em, nf);
+}
+
+// ------------------------------------------------------------ batched SSSP ---
+// k SSSP queries (SpVertex, sssp.h:26-46) in one Bellman-Ford traversal.
+// dist[s*K + q] (K = 64*QW); lane l of a wavefront owns queries l, l+64, ...
+// cm / nm hold, per vertex, the queries whose distance improved in the last /
+// current step; only those queries are relaxed from it. As in the single
+// query, the loop relaxes regular rows and one pull pass computes the sinks.
+
+// Appends `slot` from lane 0 when `flag` (wave-uniform) is set.
+__device__ __forceinline__ void em_push_one(Emitter& em, const Frontier& f, uint32_t slot, bool flag) {
+  if (!flag) return;
+  if (em.cnt + 1 > QCAP) em_flush(em, f);
+  if (lane_id() == 0) { em.q[em.cnt] = slot; em.verts++; em.cols++; }
+  em.cnt++;
+}
+
+template <int QW>
+__global__ void k_mssp_roots(DG g, const uint32_t* root_slots, uint32_t k, uint32_t* dist, uint64_t* cm) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (uint32_t q = 0; q < k; q++) {
+    const uint32_t s = root_slots[q];
+    dist[(size_t)s * 64 * QW + q] = 0;
+    cm[(size_t)s * QW + q / 64] |= 1ull << (q & 63);
+  }
+}
+
+template <int QW>
+__global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items, uint32_t nitems,
+                                                     uint32_t* dist, const uint64_t* cm, uint64_t* nm,
+                                                     uint32_t* stamp, uint32_t tag, Frontier touched,
+                                                     uint32_t* zc) {
+  zero_ctr(zc);
+  constexpr uint32_t K = 64 * QW;
+  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  const uint32_t lane = lane_id();
+  LA3_EMITTER(em);
+  for (uint32_t it = wid; it < nitems; it += nwaves) {
+    const uint64_t item = items[it];
+    const uint32_t s = (uint32_t)item, ch = (uint32_t)(item >> 32);
+    const uint32_t c = col_of_slot(g, s);
+    const uint32_t beg = g.col_ptr[c] + ch * CHUNK;
+    const uint32_t end = min(beg + CHUNK, g.col_split[c]);
+    uint32_t ds[QW];
+    bool act[QW];
+#pragma unroll
+    for (int j = 0; j < QW; j++) {
+      act[j] = (cm[(size_t)s * QW + j] >> lane) & 1;
+      ds[j] = act[j] ? ld_relaxed(&dist[(size_t)s * K + 64 * j + lane]) : SSSP_INF;
+    }
+    for (uint32_t e = beg; e < end; e++) {
+      const uint32_t r = g.col_rows[e], w = g.col_w[e];
+      bool any = false;
+#pragma unroll
+      for (int j = 0; j < QW; j++) {
+        bool imp = false;
+        if (act[j]) {
+          uint32_t* p = &dist[(size_t)r * K + 64 * j + lane];
+          const uint32_t nd = ds[j] + w;
+          if (nd < ld_relaxed(p)) imp = nd < atomicMin(p, nd);
+        }
+        const uint64_t b = __ballot(imp);
+        if (b) {
+          if (lane == 0) atomicOr((unsigned long long*)&nm[(size_t)r * QW + j], (unsigned long long)b);
+          any = true;
+        }
+      }
+      bool first = false;
+      if (any && lane == 0) first = atomicMax(&stamp[r], tag) < tag;
+      first = __shfl(first, 0, WAVE);
+      em_push_one(em, touched, r, first);
+    }
+  }
+  em_finish(em, touched);
+}
+
+// Turns the touched list into push work items (regular-row part of each column).
+__global__ void __launch_bounds__(BLOCK) k_mssp_items(DG g, const uint64_t* touched, uint32_t ntouched,
+                                                      Frontier nf, uint32_t* zc) {
+  zero_ctr(zc);
+  const uint32_t stride = gridDim.x * BLOCK;
+  LA3_EMITTER(em);
+  for (uint32_t base = blockIdx.x * BLOCK; base < ntouched; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    uint32_t r = 0, nit = 0, ne = 0;
+    if (i < ntouched) {
+      r = (uint32_t)touched[i];
+      ne = g.col_split[r] - g.col_ptr[r];  // touched rows are regular: column index == slot
+      nit = (ne + CHUNK - 1) / CHUNK;
+    }
+    em_push(em, nf, r, nit, ne, i < ntouched);
+  }
+  em_finish(em, nf);
+}
+
+template <int QW>
+__global__ void __launch_bounds__(BLOCK) k_mssp_sinks(DG g, uint32_t* dist) {
+  constexpr uint32_t K = 64 * QW;
+  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  const uint32_t lane = lane_id();
+  for (uint32_t i = wid; i < g.nsink; i += nwaves) {
+    const uint32_t r = g.nreg + i;
+    uint32_t y[QW];
+#pragma unroll
+    for (int j = 0; j < QW; j++) y[j] = SSSP_INF;
+    for (uint32_t e = g.row_ptr[r]; e < g.row_ptr[r + 1]; e++) {
+      const uint32_t src = g.row_cols[e], w = g.row_w[e];
+#pragma unroll
+      for (int j = 0; j < QW; j++) y[j] = min(y[j], dist[(size_t)src * K + 64 * j + lane] + w);
+    }
+#pragma unroll
+    for (int j = 0; j < QW; j++) {
+      uint32_t* p = &dist[(size_t)r * K + 64 * j + lane];
+      *p = min(*p, y[j]);
+    }
+  }
+}
+
+// out[q * nslots + id] = v[slot(id) * K + q] for q < k.
+template <class T>
+__global__ void k_batch_output(DG g, const T* v, uint32_t K, uint32_t k, T* out) {
+  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < g.nslots; id += gridDim.x * blockDim.x) {
+    const T* row = v + (size_t)g.slot_of_id[id] * K;
+    for (uint32_t q = 0; q < k; q++) out[(size_t)q * g.nslots + id] = row[q];
+  }
 }
 
 // hops[q * nslots + id] = lev[slot(id) * K + q] for q < k.

@@ -169,17 +169,28 @@ __global__ void k_fill(T* p, uint64_t n, T v) {
     p[i] = v;
 }
 
-// out[id_of[s]] = f(v[s]) for every slot.
+// out[id] = v[slot(id)] for every original id (coalesced writes).
 template <class T>
-__global__ void k_scatter_by_id(DG g, const T* v, T* out) {
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x)
-    out[g.id_of[s]] = v[s];
+__global__ void k_gather_by_id(DG g, const T* v, T* out) {
+  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < g.nslots; id += gridDim.x * blockDim.x)
+    out[id] = v[g.slot_of_id[id]];
 }
 
 // Degree: in-degree of the stored graph (DegVertex, degree.h:31-34).
 __global__ void k_degree(DG g, uint32_t* out_by_id) {
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x)
-    out_by_id[g.id_of[s]] = s < g.nrows ? g.row_ptr[s + 1] - g.row_ptr[s] : 0u;
+  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < g.nslots; id += gridDim.x * blockDim.x) {
+    const uint32_t s = g.slot_of_id[id];
+    out_by_id[id] = s < g.nrows ? g.row_ptr[s + 1] - g.row_ptr[s] : 0u;
+  }
+}
+
+// Block 0 clears a queue's counter block (the queue the kernel reads from:
+// its counts were already taken), so the next step finds it zeroed without
+// a separate launch.
+constexpr uint32_t CTR_WORDS = (64 + NSLOT * sizeof(CtrSlot)) / 4;
+__device__ __forceinline__ void zero_ctr(uint32_t* zc) {
+  if (zc && blockIdx.x == 0)
+    for (uint32_t i = threadIdx.x; i < CTR_WORDS; i += blockDim.x) zc[i] = 0;
 }
 
 // Out-degree of each slot = length of its column (regular + sink rows).
@@ -201,11 +212,14 @@ __global__ void k_outdeg(DG g, uint32_t* deg) {
 
 // The root's own slot is written by the thread that resets that slot, so no
 // other thread's reset can land after it.
-__global__ void k_bfs_init(DG g, uint8_t* level, uint32_t* mark, uint32_t root, Frontier f) {
+__global__ void k_bfs_init(DG g, uint8_t* level, uint32_t* mark, uint32_t root, Frontier f, uint32_t* zc_other) {
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
     level[s] = s == root ? 0 : (uint8_t)BFS_INF;
     mark[s] = s == root ? 0xffffffffu : 0u;
   }
+  zero_ctr(zc_other);
+  zero_ctr(f.nitems);
+  __syncthreads();
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     uint32_t nit = 0, ne = 0;
     if (slot_has_col(g, root)) {
@@ -225,7 +239,8 @@ __global__ void k_bfs_init(DG g, uint8_t* level, uint32_t* mark, uint32_t root, 
 // (<= CHUNK entries of one column) per wavefront iteration.
 __global__ void __launch_bounds__(BLOCK) k_bfs_push(DG g, const uint64_t* items, uint32_t nitems,
                                                     uint8_t* level, uint32_t* mark, uint32_t iter,
-                                                    Frontier nf) {
+                                                    Frontier nf, uint32_t* zc) {
+  zero_ctr(zc);
   const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
   const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
   const uint8_t nxt = (uint8_t)(iter + 1);
@@ -265,7 +280,8 @@ __global__ void __launch_bounds__(BLOCK) k_bfs_push(DG g, const uint64_t* items,
 // end of the reference's message order; the first frontier member found is
 // the last writer. Rows longer than LONG_ROW are left to k_bfs_pull_long.
 __global__ void __launch_bounds__(BLOCK) k_bfs_pull(DG g, uint8_t* level, uint32_t* mark,
-                                                    uint32_t iter, Frontier nf) {
+                                                    uint32_t iter, Frontier nf, uint32_t* zc) {
+  zero_ctr(zc);
   const uint8_t cur = (uint8_t)iter, nxt = (uint8_t)(iter + 1);
   const uint32_t stride = gridDim.x * BLOCK;
   LA3_EMITTER(em);
@@ -346,8 +362,8 @@ __global__ void __launch_bounds__(BLOCK) k_bfs_pull_long(DG g, uint8_t* level, u
 // vertex (Edge.src = csc.colidxs, vertex_program_execute.hpp:593).
 __global__ void k_bfs_output(DG g, const uint8_t* level, const uint32_t* mark, uint8_t* hops,
                              uint32_t* parent) {
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
-    const uint32_t id = g.id_of[s];
+  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < g.nslots; id += gridDim.x * blockDim.x) {
+    const uint32_t s = g.slot_of_id[id];
     hops[id] = level[s];
     if (parent) {
       uint32_t m = mark[s], p = 0;
@@ -364,11 +380,14 @@ __global__ void k_bfs_output(DG g, const uint8_t* level, const uint32_t* mark, u
 // queue); min-plus distances are unique, so the result equals the
 // reference's.
 
-__global__ void k_sssp_init(DG g, uint32_t* dist, uint32_t* stamp, uint32_t root, Frontier f) {
+__global__ void k_sssp_init(DG g, uint32_t* dist, uint32_t* stamp, uint32_t root, Frontier f, uint32_t* zc_other) {
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
     dist[s] = s == root ? 0u : SSSP_INF;
     stamp[s] = 0;
   }
+  zero_ctr(zc_other);
+  zero_ctr(f.nitems);
+  __syncthreads();
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     uint32_t nit = 0, ne = 0;
     if (slot_has_col(g, root)) {
@@ -386,7 +405,8 @@ __global__ void k_sssp_init(DG g, uint32_t* dist, uint32_t* stamp, uint32_t root
 
 __global__ void __launch_bounds__(BLOCK) k_sssp_push(DG g, const uint64_t* items, uint32_t nitems,
                                                      uint32_t* dist, uint32_t* stamp, uint32_t round,
-                                                     Frontier nf) {
+                                                     Frontier nf, uint32_t* zc) {
+  zero_ctr(zc);
   const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
   const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
   const uint32_t tag = round + 1;
@@ -578,8 +598,8 @@ __global__ void __launch_bounds__(BLOCK) k_cc_hook_long(DG g, uint32_t* p) {
 }
 
 __global__ void k_cc_output(DG g, uint32_t* p, uint32_t* label_by_id) {
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x)
-    label_by_id[g.id_of[s]] = g.id_of[uf_find(p, s)];
+  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < g.nslots; id += gridDim.x * blockDim.x)
+    label_by_id[id] = g.id_of[uf_find(p, g.slot_of_id[id])];
 }
 
 // -------------------------------------------------------------------- TC ---

@@ -238,3 +238,20 @@ def test_bfs_batch_undirected(gpu_ctx, oracle):
     hops, _ = L.bfs_batch(g, roots)
     for q in (0, 33, 69):
         assert (hops[q] == oracle.bfs(o, roots[q])[0]).all()
+
+
+@pytest.mark.parametrize("scale,seed,k", [(10, 41, 1), (12, 42, 64), (14, 43, 100), (16, 44, 128)])
+def test_sssp_batch_equals_single(gpu_ctx, oracle, scale, seed, k):
+    rec = _rmat(scale, seed, weighted=True)
+    n = 1 << scale
+    g, o = _both(gpu_ctx, oracle, rec, n, True, L.DIRECTED)
+    roots = _roots(rec, k, seed)
+    if k > 2:
+        roots[1] = roots[0]
+        roots[2] = n
+    dist, _ = L.sssp_batch(g, roots)
+    for q, r in enumerate(roots):
+        d1, _ = L.sssp(g, r)
+        assert (dist[q] == d1).all(), f"query {q} (root {r}) differs from single-query"
+    for q in range(0, len(roots), max(1, len(roots) // 4)):
+        assert (dist[q] == oracle.sssp(o, roots[q])[0]).all()
