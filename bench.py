@@ -225,6 +225,8 @@ def main():
         }
         log(f"single: {el_s * 1e3 / args.steps:.1f} ms/step, {gteps_s:.2f} GTEPS; kernels {json.dumps(kernel_table(st_s))}")
 
+    copy_gbs = ctx.copy_bandwidth(1 << 30, 10)
+    log(f"device copy bandwidth {copy_gbs:.0f} GB/s (1 GiB buffer, read + write)")
     lat_b_ms = np.array(lat_b) * 1e3
     out = {
         "metric": METRIC,
@@ -253,6 +255,7 @@ def main():
         "single_query": single,
         "speedup_vs_single": round(gteps_b / single["value"], 2) if single else None,
         "roofline": roofline(st_b),
+        "device_copy_gbs": round(copy_gbs, 1),
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(rec, n, roots[:args.cpu_roots], outdeg, hops, dist)

@@ -208,6 +208,13 @@ la3_status la3_kernel_stats_reset(la3_ctx* ctx);
 la3_status la3_host_alloc(uint64_t bytes, void** out);
 void la3_host_free(void* p);
 
+/* Device copy bandwidth: copies a `bytes` buffer `reps` times on the
+ * context's stream with a plain vectorized copy kernel (la3_copy) and
+ * returns (read + written bytes) / time in GB/s. A measured counterpart to
+ * the 8 TB/s datasheet peak, and the known-size kernel that calibrates HBM
+ * counter readings. */
+la3_status la3_copy_bandwidth(la3_ctx* ctx, uint64_t bytes, uint32_t reps, double* gbs);
+
 /* -------------------------------------------------------------- tools --- */
 /* Seeded R-MAT generator (a, b, c, d = 1-a-b-c), 2^scale vertices with ids
  * 1..2^scale, edgefactor*2^scale samples, self-loops and duplicates removed,

@@ -31,6 +31,7 @@ EXPORTS = [
     "la3_bfs_batch", "la3_sssp_batch", "la3_ppr_batch",
     "la3_last_timing", "la3_set_profiling", "la3_rmat_generate",
     "la3_kernel_stats", "la3_kernel_stats_reset", "la3_host_alloc", "la3_host_free",
+    "la3_copy_bandwidth",
 ]
 
 
@@ -100,6 +101,7 @@ def lib():
             "la3_kernel_stats_reset": (i32, [P]),
             "la3_host_alloc": (i32, [u64, ctypes.POINTER(P)]),
             "la3_host_free": (None, [P]),
+            "la3_copy_bandwidth": (i32, [P, u64, u32, ctypes.POINTER(ctypes.c_double)]),
             "la3_rmat_generate": (i32, [u32, u32, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, u64, i32, P, ctypes.POINTER(u64)]),
         }
@@ -197,6 +199,13 @@ class Context:
 
     def synchronize(self):
         self.check(lib().la3_ctx_synchronize(self.h), "synchronize")
+
+    def copy_bandwidth(self, nbytes=1 << 30, reps=20):
+        """Measured device copy GB/s (read + write) of a plain copy kernel."""
+        v = ctypes.c_double(0)
+        self.check(lib().la3_copy_bandwidth(self.h, int(nbytes), int(reps), ctypes.byref(v)),
+                   "copy_bandwidth")
+        return v.value
 
 
 class Graph:

@@ -975,6 +975,31 @@ la3_status la3_ppr_batch(la3_graph* g, const uint32_t* seeds, uint32_t k, uint32
   return fail(g ? g->ctx : nullptr, LA3_ENOTSUP, "batched PPR not built yet");
 }
 
+la3_status la3_copy_bandwidth(la3_ctx* c, uint64_t bytes, uint32_t reps, double* gbs) {
+  if (!c || !gbs || bytes < 16 || reps == 0) return LA3_EINVAL;
+  if (c->device < 0) return fail(c, LA3_EDEVICE, "host-only context");
+  HIPC(c, hipSetDevice(c->device));
+  const uint64_t n = bytes / 16;
+  uint4* a = (uint4*)scratch(c, "copy.src", n * 16);
+  uint4* b = (uint4*)scratch(c, "copy.dst", n * 16);
+  if (!a || !b) return fail(c, LA3_ENOMEM, "copy buffers");
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + BLOCK - 1) / BLOCK, (uint64_t)c->ncu * 32);
+  launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for(n * 2), (unsigned long long*)a, n * 2, 1ull);
+  hipLaunchKernelGGL(la3_copy, dim3(blocks), dim3(BLOCK), 0, c->stream, (const uint4*)a, b, n);
+  hipEvent_t e0 = get_event(c), e1 = get_event(c);
+  HIPC(c, hipEventRecord(e0, c->stream));
+  for (uint32_t r = 0; r < reps; r++)
+    hipLaunchKernelGGL(la3_copy, dim3(blocks), dim3(BLOCK), 0, c->stream, (const uint4*)a, b, n);
+  HIPC(c, hipEventRecord(e1, c->stream));
+  HIPC(c, hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPC(c, hipEventElapsedTime(&ms, e0, e1));
+  c->free_events.push_back(e0);
+  c->free_events.push_back(e1);
+  *gbs = 2.0 * n * 16 * reps / (ms * 1e-3) / 1e9;
+  return LA3_OK;
+}
+
 la3_status la3_host_alloc(uint64_t bytes, void** out) {
   if (!out) return LA3_EINVAL;
   *out = nullptr;

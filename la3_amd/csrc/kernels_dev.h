@@ -169,6 +169,13 @@ __global__ void k_fill(T* p, uint64_t n, T v) {
     p[i] = v;
 }
 
+// Plain copy (16-byte vectors); the known-size kernel for bandwidth checks.
+__global__ void __launch_bounds__(BLOCK) la3_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                  uint64_t n) {
+  for (uint64_t i = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK)
+    dst[i] = src[i];
+}
+
 // out[id] = v[slot(id)] for every original id (coalesced writes).
 template <class T>
 __global__ void k_gather_by_id(DG g, const T* v, T* out) {
