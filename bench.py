@@ -88,7 +88,8 @@ def cpu_baseline(rec, n, roots, outdeg, gpu_hops, gpu_dist):
 
 
 def run_single(L, g, roots, hbuf, dbuf, steps):
-    """One step = every root as its own BFS query, then its own SSSP query."""
+    """One step = every root as its own BFS query, then its own SSSP query.
+    hbuf/dbuf = L.DEVICE keeps results in HBM."""
     lat = []
     for _ in range(steps):
         for r in roots:
@@ -103,7 +104,8 @@ def run_single(L, g, roots, hbuf, dbuf, steps):
 
 
 def run_batched(L, g, roots, hb, db, steps):
-    """One step = all roots in one batched BFS, then one batched SSSP."""
+    """One step = all roots in one batched BFS, then one batched SSSP.
+    hb/db = L.DEVICE keeps results in HBM."""
     lat = []
     for _ in range(steps):
         q = time.perf_counter()
@@ -205,14 +207,20 @@ def main():
         hops[r], dist[r] = h1, d1
     log(f"exact: batched == single-query for all {k} BFS and {k} SSSP roots")
 
-    el_b, lat_b, st_b = timed(ctx, lambda s: run_batched(L, g, roots, hb, db, s), args.steps)
+    # value: results stay in HBM (inputs resident, no PCIe in the timed region)
+    D = L.DEVICE
+    el_b, lat_b, st_b = timed(ctx, lambda s: run_batched(L, g, roots, D, D, s), args.steps)
     gteps_b = q_edges * args.steps / el_b / 1e9
     log(f"batched: {el_b * 1e3 / args.steps:.1f} ms/step, {gteps_b:.2f} GTEPS; kernels {json.dumps(kernel_table(st_b))}")
+    # the same with every result copied to page-locked host memory
+    el_p, _, _ = timed(ctx, lambda s: run_batched(L, g, roots, hb, db, s), args.steps)
+    gteps_p = q_edges * args.steps / el_p / 1e9
+    log(f"batched incl. PCIe result copies: {el_p * 1e3 / args.steps:.1f} ms/step, {gteps_p:.2f} GTEPS")
     single = None
     if not args.no_single:
         for _ in range(max(args.warmup, 1)):
-            run_single(L, g, roots[:4], hbuf, dbuf, 1)
-        el_s, lat_s, st_s = timed(ctx, lambda s: run_single(L, g, roots, hbuf, dbuf, s), args.steps)
+            run_single(L, g, roots[:4], D, D, 1)
+        el_s, lat_s, st_s = timed(ctx, lambda s: run_single(L, g, roots, D, D, s), args.steps)
         gteps_s = q_edges * args.steps / el_s / 1e9
         lat_ms = np.array(lat_s) * 1e3
         single = {
@@ -245,13 +253,17 @@ def main():
             "workload": f"BFS+SSSP, {k} roots, directed R-MAT scale {scale} edge factor 16 "
                         f"(a=.57 b=.19 c=.19) seed {seed}, u32 weights [1,128]",
             "mode": f"batched: {k} BFS roots in one k-bit traversal, then {k} SSSP roots in one "
-                    f"k-wide traversal; results equal the single-query path for every root",
+                    f"k-wide traversal; results equal the single-query path for every root and "
+                    f"stay in HBM (PCIe-inclusive rate under pcie_inclusive)",
             "graph_edges": int(g.info["nnz"]),
             "queries_per_step": 2 * k,
         },
         "queries_per_s": round(2 * k * args.steps / el_b, 2),
         "p50_ms": round(float(np.percentile(lat_b_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_b_ms, 99)), 3),
+        "pcie_inclusive": {"value": round(gteps_p, 4), "unit": "GTEPS",
+                           "ms_per_step": round(el_p * 1e3 / args.steps, 3),
+                           "note": "batched path with every per-query result copied to page-locked host memory"},
         "single_query": single,
         "speedup_vs_single": round(gteps_b / single["value"], 2) if single else None,
         "roofline": roofline(st_b),

@@ -202,6 +202,13 @@ la3_status la3_kernel_stats(const la3_ctx* ctx, la3_kernel_stat* out, uint32_t c
                             uint32_t* count);
 la3_status la3_kernel_stats_reset(la3_ctx* ctx);
 
+/* Every program's main per-vertex output (hops, dist, rank, label, ...) may be
+ * NULL: the result then stays in device memory, and la3_result_fetch copies
+ * the last program's main output (as laid out for the host buffer) into
+ * `dst`. Device-resident results let a caller time or chain programs without
+ * the PCIe transfer. */
+la3_status la3_result_fetch(la3_ctx* ctx, void* dst, uint64_t bytes);
+
 /* Page-locked host memory (hipHostMalloc) for result buffers: device-to-host
  * copies into it run at full link speed. Any la3_* output pointer may point
  * into it. */

@@ -31,7 +31,7 @@ EXPORTS = [
     "la3_bfs_batch", "la3_sssp_batch", "la3_ppr_batch",
     "la3_last_timing", "la3_set_profiling", "la3_rmat_generate",
     "la3_kernel_stats", "la3_kernel_stats_reset", "la3_host_alloc", "la3_host_free",
-    "la3_copy_bandwidth",
+    "la3_copy_bandwidth", "la3_result_fetch",
 ]
 
 
@@ -102,6 +102,7 @@ def lib():
             "la3_host_alloc": (i32, [u64, ctypes.POINTER(P)]),
             "la3_host_free": (None, [P]),
             "la3_copy_bandwidth": (i32, [P, u64, u32, ctypes.POINTER(ctypes.c_double)]),
+            "la3_result_fetch": (i32, [P, P, u64]),
             "la3_rmat_generate": (i32, [u32, u32, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, u64, i32, P, ctypes.POINTER(u64)]),
         }
@@ -130,7 +131,12 @@ def pinned(shape, dtype):
     return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
 
 
+DEVICE = "device"  # pass as out= to leave a program's result in device memory
+
+
 def _out(out, shape, dtype):
+    if out is DEVICE:
+        return None
     if out is None:
         return np.zeros(shape, dtype=dtype)
     assert out.shape == tuple(shape) and out.dtype == np.dtype(dtype) and out.flags.c_contiguous
@@ -199,6 +205,11 @@ class Context:
 
     def synchronize(self):
         self.check(lib().la3_ctx_synchronize(self.h), "synchronize")
+
+    def fetch(self, out):
+        """Copies the last program's device-resident main output into `out`."""
+        self.check(lib().la3_result_fetch(self.h, _ptr(out), out.nbytes), "result_fetch")
+        return out
 
     def copy_bandwidth(self, nbytes=1 << 30, reps=20):
         """Measured device copy GB/s (read + write) of a plain copy kernel."""

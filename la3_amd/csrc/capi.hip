@@ -44,6 +44,8 @@ struct la3_ctx {
   void* host_ctr = nullptr;  // pinned, for frontier counters
   int ncu = 256;
   int refs = 1;              // the caller's reference plus one per live graph
+  const void* last_result = nullptr;  // device buffer of the last program's main output
+  size_t last_result_bytes = 0;
 };
 
 namespace {
@@ -297,6 +299,16 @@ void zero_counts(la3_ctx* c, const Frontier& f) {
 
 }  // namespace
 
+// Hands a device result to the caller: copies it into `dst` (host memory)
+// when dst != NULL, and in every case records it as the context's last
+// result so that la3_result_fetch can copy it later.
+la3_status deliver(la3_ctx* c, void* dst, const void* dev, size_t bytes) {
+  c->last_result = dev;
+  c->last_result_bytes = bytes;
+  if (dst) HIPC(c, hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+  return LA3_OK;
+}
+
 // ================================================================== API ===
 extern "C" {
 
@@ -474,7 +486,7 @@ la3_status la3_graph_csc(const la3_graph* g, int which, uint32_t* colptrs, uint3
 
 // ------------------------------------------------------------- degree ---
 la3_status la3_degree(la3_graph* g, uint32_t* degree) {
-  if (!g || !degree) return LA3_EINVAL;
+  if (!g) return LA3_EINVAL;
   la3_ctx* c = g->ctx;
   if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
   HIPC(c, hipSetDevice(c->device));
@@ -482,7 +494,8 @@ la3_status la3_degree(la3_graph* g, uint32_t* degree) {
   uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * g->nslots);
   if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   launch(c, "degree", k_degree, blocks_for(g->nslots), g->dg, out);
-  HIPC(c, hipMemcpyAsync(degree, out, sizeof(uint32_t) * g->nslots, hipMemcpyDeviceToHost, c->stream));
+  la3_status st = deliver(c, degree, out, sizeof(uint32_t) * g->nslots);
+  if (st) return st;
   HIPC(c, hipStreamSynchronize(c->stream));
   timing_end(c);
   return LA3_OK;
@@ -490,7 +503,7 @@ la3_status la3_degree(la3_graph* g, uint32_t* degree) {
 
 // ---------------------------------------------------------------- BFS ---
 la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent, uint32_t* iters_run) {
-  if (!g || !hops) return LA3_EINVAL;
+  if (!g) return LA3_EINVAL;
   la3_ctx* c = g->ctx;
   if (root > g->n) return fail(c, LA3_EINVAL, "root id out of range");
   if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
@@ -552,7 +565,7 @@ la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
   uint32_t* op = parent ? (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S) : nullptr;
   if (!oh || (parent && !op)) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   launch(c, "bfs_output", k_bfs_output, blocks_for(S), d, (const uint8_t*)level, (const uint32_t*)mark, oh, op);
-  HIPC(c, hipMemcpyAsync(hops, oh, S, hipMemcpyDeviceToHost, c->stream));
+  if ((st = deliver(c, hops, oh, S))) return st;
   if (parent) HIPC(c, hipMemcpyAsync(parent, op, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   timing_end(c);
@@ -562,7 +575,7 @@ la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
 
 // --------------------------------------------------------------- SSSP ---
 la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters_run) {
-  if (!g || !dist) return LA3_EINVAL;
+  if (!g) return LA3_EINVAL;
   la3_ctx* c = g->ctx;
   if (root > g->n) return fail(c, LA3_EINVAL, "root id out of range");
   if (!g->weighted) return fail(c, LA3_EINVAL, "SSSP needs a weighted graph");
@@ -601,7 +614,7 @@ la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters
   uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
   if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   launch(c, "gather_u32", k_gather_by_id<uint32_t>, blocks_for(S), d, (const uint32_t*)dv, out);
-  HIPC(c, hipMemcpyAsync(dist, out, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
+  if ((st = deliver(c, dist, out, sizeof(uint32_t) * S))) return st;
   HIPC(c, hipStreamSynchronize(c->stream));
   timing_end(c);
   if (iters_run) *iters_run = round + 1;
@@ -666,7 +679,8 @@ static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t
   double* outd = (double*)scratch(c, "out.f64", sizeof(double) * S);
   if (!outd) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   launch(c, "gather_f64", k_gather_by_id<double>, blocks_for(S), d, (const double*)rk, outd);
-  HIPC(c, hipMemcpyAsync(rank, outd, sizeof(double) * S, hipMemcpyDeviceToHost, c->stream));
+  la3_status st = deliver(c, rank, outd, sizeof(double) * S);
+  if (st) return st;
   if (degree) {
     uint32_t* outu = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
     if (!outu) return fail(c, LA3_ENOMEM, "scratch allocation failed");
@@ -681,12 +695,12 @@ static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t
 
 la3_status la3_pagerank(la3_graph* g, uint32_t iters, double* rank, uint32_t* degree,
                         uint32_t* iters_run) {
-  if (!g || !rank) return LA3_EINVAL;
+  if (!g) return LA3_EINVAL;
   return pr_common(g, iters, 0.15, 0xffffffffu, rank, degree, iters_run);
 }
 
 la3_status la3_ppr(la3_graph* g, uint32_t seed, uint32_t iters, double alpha, double* rank) {
-  if (!g || !rank) return LA3_EINVAL;
+  if (!g) return LA3_EINVAL;
   if (seed > g->n) return fail(g->ctx, LA3_EINVAL, "seed id out of range");
   if (!(alpha > 0.0 && alpha < 1.0)) return fail(g->ctx, LA3_EINVAL, "alpha must be in (0, 1)");
   if (iters == 0) return fail(g->ctx, LA3_EINVAL, "PPR needs a fixed iteration count");
@@ -695,7 +709,7 @@ la3_status la3_ppr(la3_graph* g, uint32_t seed, uint32_t iters, double alpha, do
 
 // ----------------------------------------------------------------- CC ---
 la3_status la3_cc(la3_graph* g, uint32_t* label, uint32_t* iters_run) {
-  if (!g || !label) return LA3_EINVAL;
+  if (!g) return LA3_EINVAL;
   la3_ctx* c = g->ctx;
   if (g->directed) return fail(c, LA3_EINVAL, "CC runs on an undirected load");
   if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
@@ -711,7 +725,8 @@ la3_status la3_cc(la3_graph* g, uint32_t* label, uint32_t* iters_run) {
   if (d.nlong) launch(c, "cc_hook_long", k_cc_hook_long, std::min<uint32_t>((d.nlong + 3) / 4, 4096), d, p);
   launch(c, "cc_output", k_cc_output, blocks_for(S), d, p, out);
   c->edges_scanned = g->nnz;
-  HIPC(c, hipMemcpyAsync(label, out, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
+  la3_status st = deliver(c, label, out, sizeof(uint32_t) * S);
+  if (st) return st;
   HIPC(c, hipStreamSynchronize(c->stream));
   timing_end(c);
   if (iters_run) *iters_run = 1;
@@ -743,7 +758,8 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
     uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
     if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
     launch(c, "gather_u32", k_gather_by_id<uint32_t>, blocks_for(S), d, (const uint32_t*)per, out);
-    HIPC(c, hipMemcpyAsync(per_vertex, out, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
+    la3_status st = deliver(c, per_vertex, out, sizeof(uint32_t) * S);
+    if (st) return st;
   }
   HIPC(c, hipStreamSynchronize(c->stream));
   std::memcpy(&h, c->host_ctr, 8);
@@ -835,7 +851,7 @@ static la3_status msbfs_run(la3_graph* g, const std::vector<uint32_t>& root_slot
     cur ^= 1;
   }
   launch(c, "msbfs_output", k_msbfs_output, blocks_for(S), d, (const uint8_t*)lev, K, k, out);
-  HIPC(c, hipMemcpyAsync(hops, out, (size_t)S * k, hipMemcpyDeviceToHost, c->stream));
+  if ((st = deliver(c, hops, out, (size_t)S * k))) return st;
   HIPC(c, hipStreamSynchronize(c->stream));
   if (iters_run) *iters_run = std::max<uint32_t>(iter, 1);
   return LA3_OK;
@@ -845,7 +861,7 @@ extern "C" {
 
 la3_status la3_bfs_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint8_t* hops,
                          uint32_t* iters_run) {
-  if (!g || !roots || !hops) return LA3_EINVAL;
+  if (!g || !roots) return LA3_EINVAL;
   la3_ctx* c = g->ctx;
   if (k == 0 || k > 1024) return fail(c, LA3_EINVAL, "batch size must be in [1, 1024]");
   std::vector<uint32_t> rs(k);
@@ -935,7 +951,7 @@ static la3_status mssp_run(la3_graph* g, const std::vector<uint32_t>& root_slots
   }
   if (g->nsink) launch(c, "mssp_sinks", k_mssp_sinks<QW>, std::min<uint32_t>((g->nsink + 3) / 4, wave_blocks), d, dist);
   launch(c, "mssp_output", k_batch_output<uint32_t>, blocks_for(S), d, (const uint32_t*)dist, K, k, out);
-  HIPC(c, hipMemcpyAsync(dist_out, out, sizeof(uint32_t) * (size_t)S * k, hipMemcpyDeviceToHost, c->stream));
+  if ((st = deliver(c, dist_out, out, sizeof(uint32_t) * (size_t)S * k))) return st;
   HIPC(c, hipStreamSynchronize(c->stream));
   if (iters_run) *iters_run = round + 1;
   return LA3_OK;
@@ -945,7 +961,7 @@ extern "C" {
 
 la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint32_t* dist,
                           uint32_t* iters_run) {
-  if (!g || !roots || !dist) return LA3_EINVAL;
+  if (!g || !roots) return LA3_EINVAL;
   la3_ctx* c = g->ctx;
   if (k == 0 || k > 1024) return fail(c, LA3_EINVAL, "batch size must be in [1, 1024]");
   if (!g->weighted) return fail(c, LA3_EINVAL, "SSSP needs a weighted graph");
@@ -997,6 +1013,15 @@ la3_status la3_copy_bandwidth(la3_ctx* c, uint64_t bytes, uint32_t reps, double*
   c->free_events.push_back(e0);
   c->free_events.push_back(e1);
   *gbs = 2.0 * n * 16 * reps / (ms * 1e-3) / 1e9;
+  return LA3_OK;
+}
+
+la3_status la3_result_fetch(la3_ctx* c, void* dst, uint64_t bytes) {
+  if (!c || !dst) return LA3_EINVAL;
+  if (!c->last_result) return fail(c, LA3_EINVAL, "no result on the device");
+  if (bytes > c->last_result_bytes) return fail(c, LA3_EINVAL, "fetch larger than the last result");
+  HIPC(c, hipMemcpyAsync(dst, c->last_result, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
   return LA3_OK;
 }
 
