@@ -1,0 +1,252 @@
+// la3.hpp — C++ drop-in over the C ABI (include/la3.h).
+//
+// Keeps the reference's host-side names so that an LA3 app driver ports by
+// changing includes and handing a Context to the Graph:
+//   Graph<W>::load_directed / load_undirected / free / get_nvertices
+//       (src/matrix/graph.h:38-109)
+//   DegVertex, PrVertex, BfsVertex, SpVertex, CcVertex, GnVertex, CtVertex
+//       (src/apps/graph_analytics/*.h) with initialize(), initialize(other),
+//       execute(max_iters), reduce(mapper, reducer) and a public `root` where
+//       the reference has one (src/vprogram/vertex_program.h:158-202).
+// States are exposed per vertex with the reference's field names (degree,
+// rank, parent, hops, distance, label, ntriangles). Errors throw la3::Error
+// (the reference exits or asserts; a C++ caller can catch here).
+#ifndef LA3_HPP
+#define LA3_HPP
+
+#include <cstdint>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "la3.h"
+
+namespace la3 {
+
+struct Error : std::runtime_error {
+  la3_status status;
+  Error(la3_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+enum Hashing { NONE = LA3_HASH_NONE, BUCKET = LA3_HASH_BUCKET };
+struct Empty {};
+
+class Context {
+ public:
+  explicit Context(int device = 0) {
+    la3_status s = la3_ctx_create(device, &c_);
+    if (s) throw Error(s, "la3_ctx_create failed");
+  }
+  ~Context() { la3_ctx_destroy(c_); }
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+  la3_ctx* get() const { return c_; }
+  void check(la3_status s, const char* what) const {
+    if (s) throw Error(s, std::string(what) + ": " + la3_last_error(c_));
+  }
+
+ private:
+  la3_ctx* c_ = nullptr;
+};
+
+// Graph<W> at one rank (np = 1), resident on the device. W = uint32_t loads
+// weighted triples, anything else loads pairs.
+template <class W = Empty>
+class Graph {
+ public:
+  explicit Graph(Context& ctx) : ctx_(&ctx) {}
+  ~Graph() { free(); }
+  Graph(const Graph&) = delete;
+  Graph& operator=(const Graph&) = delete;
+
+  void load_directed(bool binary, std::string filepath, uint32_t nvertices = 0,
+                     bool reverse_edges = false, bool remove_cycles = false,
+                     Hashing hashing = BUCKET) {
+    la3_mode m = reverse_edges ? (remove_cycles ? LA3_REVERSED_ACYCLIC : LA3_REVERSED)
+                               : (remove_cycles ? LA3_ACYCLIC : LA3_DIRECTED);
+    load(binary, filepath, nvertices, m, hashing);
+  }
+  void load_undirected(bool binary, std::string filepath, uint32_t nvertices = 0,
+                       Hashing hashing = BUCKET) {
+    load(binary, filepath, nvertices, LA3_UNDIRECTED, hashing);
+  }
+  void free() {
+    if (g_) la3_graph_free(g_);
+    g_ = nullptr;
+  }
+
+  uint32_t get_nvertices() const { return info_.n; }
+  uint64_t get_nedges() const { return info_.nrecords; }
+  bool is_directed() const { return info_.directed != 0; }
+  const la3_graph_info& info() const { return info_; }
+  la3_graph* handle() const { return g_; }
+  Context& context() const { return *ctx_; }
+  uint32_t unhash(uint32_t v) const { return la3_graph_unhash(g_, v); }
+  // Slot order of the reference's vertex-state vector (for reduce order).
+  std::vector<uint32_t> slot_ids() const {
+    std::vector<uint32_t> hid(info_.nslots), ids(info_.nslots);
+    ctx_->check(la3_graph_slots(g_, hid.data()), "graph_slots");
+    for (uint32_t p = 0; p < info_.nslots; p++) ids[p] = la3_graph_unhash(g_, hid[p]);
+    return ids;
+  }
+
+ private:
+  void load(bool binary, const std::string& path, uint32_t n, la3_mode m, Hashing h) {
+    if (!binary) throw Error(LA3_ENOTSUP, "text input is not implemented (as in graph.hpp:335-339)");
+    free();
+    ctx_->check(la3_graph_load_file(ctx_->get(), path.c_str(), n, weighted, m, (la3_hashing)h, &g_),
+                "graph_load_file");
+    ctx_->check(la3_graph_info_get(g_, &info_), "graph_info");
+  }
+  static constexpr int weighted = std::is_same<W, uint32_t>::value ? 1 : 0;
+  Context* ctx_;
+  la3_graph* g_ = nullptr;
+  la3_graph_info info_{};
+};
+
+// Common part of the program wrappers: per-vertex states by original id and
+// reduce() in the reference's slot order (vertex_program.hpp:558-604).
+template <class G, class S>
+class ProgramBase {
+ public:
+  using State = S;
+  explicit ProgramBase(const G* graph) : G_(graph) {}
+  const G* get_graph() const { return G_; }
+  const std::vector<S>& states() const { return states_; }
+  const S& state(uint32_t vid) const { return states_.at(vid); }
+
+  template <class Value, class Mapper, class Reducer>
+  Value reduce(Mapper map, Reducer red, bool /*active_only*/ = false) const {
+    Value acc = Value();
+    for (uint32_t vid : G_->slot_ids()) red(acc, map(vid, states_[vid]));
+    return acc;
+  }
+
+ protected:
+  const G* G_;
+  std::vector<S> states_;
+  uint32_t nslots() const { return G_->info().nslots; }
+  Context& ctx() const { return G_->context(); }
+};
+
+struct DegState { uint32_t degree = 0; };
+struct PrState : DegState { double rank = 0.15; };
+struct BfsState { uint32_t parent = 0; uint8_t hops = 255; };
+struct SpState { uint32_t distance = 0x7fffffffu; };
+struct CcState { uint32_t label = 0; };
+struct CtState { uint32_t ntriangles = 0; };
+
+// DegVertex (degree.h:25-35): in-degree of the stored graph.
+template <class W = Empty>
+class DegVertex : public ProgramBase<Graph<W>, DegState> {
+ public:
+  DegVertex(const Graph<W>* G, bool /*stationary*/ = true) : ProgramBase<Graph<W>, DegState>(G) {}
+  void initialize() {}
+  void execute(uint32_t /*max_iters*/ = 1) {
+    std::vector<uint32_t> d(this->nslots());
+    this->ctx().check(la3_degree(this->G_->handle(), d.data()), "degree");
+    this->states_.assign(d.size(), DegState());
+    for (size_t i = 0; i < d.size(); i++) this->states_[i].degree = d[i];
+  }
+};
+
+// PrVertex (pr.h:28-46). initialize(vp_degree) is accepted for drop-in: the
+// out-degrees the reference gets from DegVertex on the reversed load are the
+// column lengths of this graph, which the device reads directly.
+class PrVertex : public ProgramBase<Graph<Empty>, PrState> {
+ public:
+  PrVertex(const Graph<Empty>* G, bool /*stationary*/ = true) : ProgramBase(G) {}
+  template <class Other>
+  void initialize(const Other&) {}
+  void initialize() {}
+  void execute(uint32_t max_iters = 0) {
+    std::vector<double> r(nslots());
+    std::vector<uint32_t> d(nslots());
+    ctx().check(la3_pagerank(G_->handle(), max_iters, r.data(), d.data(), &iters_), "pagerank");
+    states_.assign(r.size(), PrState());
+    for (size_t i = 0; i < r.size(); i++) { states_[i].rank = r[i]; states_[i].degree = d[i]; }
+  }
+  uint32_t iterations() const { return iters_; }
+
+ private:
+  uint32_t iters_ = 0;
+};
+
+// BfsVertex (bfs.h:27-49).
+class BfsVertex : public ProgramBase<Graph<Empty>, BfsState> {
+ public:
+  BfsVertex(const Graph<Empty>* G, bool = false) : ProgramBase(G) {}
+  uint32_t root = 0;
+  void initialize() {}
+  void execute(uint32_t = 0) {
+    std::vector<uint8_t> h(nslots());
+    std::vector<uint32_t> p(nslots());
+    ctx().check(la3_bfs(G_->handle(), root, h.data(), p.data(), &iters_), "bfs");
+    states_.assign(h.size(), BfsState());
+    for (size_t i = 0; i < h.size(); i++) { states_[i].hops = h[i]; states_[i].parent = p[i]; }
+  }
+  uint32_t iterations() const { return iters_; }
+
+ private:
+  uint32_t iters_ = 0;
+};
+
+// SpVertex (sssp.h:26-46).
+class SpVertex : public ProgramBase<Graph<uint32_t>, SpState> {
+ public:
+  SpVertex(const Graph<uint32_t>* G, bool = false) : ProgramBase(G) {}
+  uint32_t root = 0;
+  void initialize() {}
+  void execute(uint32_t = 0) {
+    std::vector<uint32_t> d(nslots());
+    ctx().check(la3_sssp(G_->handle(), root, d.data(), nullptr), "sssp");
+    states_.assign(d.size(), SpState());
+    for (size_t i = 0; i < d.size(); i++) states_[i].distance = d[i];
+  }
+};
+
+// CcVertex (cc.h:22-38).
+class CcVertex : public ProgramBase<Graph<Empty>, CcState> {
+ public:
+  CcVertex(const Graph<Empty>* G, bool = false) : ProgramBase(G) {}
+  void initialize() {}
+  void execute(uint32_t = 0) {
+    std::vector<uint32_t> l(nslots());
+    ctx().check(la3_cc(G_->handle(), l.data(), nullptr), "cc");
+    states_.assign(l.size(), CcState());
+    for (size_t i = 0; i < l.size(); i++) states_[i].label = l[i];
+  }
+};
+
+// GnVertex + CtVertex (tc.h:47-99): the pair runs as one masked SpGEMM on the
+// acyclic load GnVertex was constructed with; CtVertex(GR).initialize(gn)
+// takes that graph over, execute(1) fills ntriangles (middle-id counts).
+class GnVertex : public ProgramBase<Graph<Empty>, CtState> {
+ public:
+  GnVertex(const Graph<Empty>* G, bool = true) : ProgramBase(G) {}
+  void initialize() {}
+  void execute(uint32_t = 1) {}
+};
+
+class CtVertex : public ProgramBase<Graph<Empty>, CtState> {
+ public:
+  CtVertex(const Graph<Empty>* GR, bool = true) : ProgramBase(GR) {}
+  void initialize(const GnVertex& gn) { acyclic_ = gn.get_graph(); }
+  void execute(uint32_t = 1) {
+    if (!acyclic_) throw Error(LA3_EINVAL, "CtVertex::initialize(GnVertex) was not called");
+    std::vector<uint32_t> per(acyclic_->info().nslots);
+    acyclic_->context().check(la3_tc(acyclic_->handle(), &total_, per.data()), "tc");
+    states_.assign(per.size(), CtState());
+    for (size_t i = 0; i < per.size(); i++) states_[i].ntriangles = per[i];
+  }
+  uint64_t total() const { return total_; }
+
+ private:
+  const Graph<Empty>* acyclic_ = nullptr;
+  uint64_t total_ = 0;
+};
+
+}  // namespace la3
+
+#endif  // LA3_HPP

@@ -39,6 +39,11 @@ def build(verbose=True):
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", capi_o, host_o, "-o", tmp,
           "-pthread", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, OUT)
+    # C++ drivers over include/la3.hpp (the reference's app mains).
+    os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "apps", "la3_run.cpp"),
+          "-o", os.path.join(HERE, "bin", "la3_run"), f"-L{HERE}", "-lla3",
+          "-Wl,-rpath,$ORIGIN/..", "-pthread"])
     return OUT
 
 

@@ -1,0 +1,52 @@
+"""The C++ drop-in (include/la3.hpp) through la3_amd/bin/la3_run, the
+counterpart of the reference's app drivers: same arguments, same output
+lines, and the g1 checksums the reference prints at np=1."""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUN = os.path.join(ROOT, "la3_amd", "bin", "la3_run")
+
+
+@pytest.fixture(scope="module")
+def g1_files(tmp_path_factory, g1):
+    d = tmp_path_factory.mktemp("g1")
+    p, pw = d / "g1_8_8_13.bin", d / "g1_8_8_13.w.bin"
+    g1["records"].tofile(p)
+    g1["records_w"].tofile(pw)
+    return str(p), str(pw)
+
+
+def run(*args):
+    out = subprocess.run([RUN, *map(str, args)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stderr
+    return out.stdout
+
+
+def test_pr_driver(g1_files, g1):
+    out = run("pr", g1_files[0], 8, 20)
+    assert f"Pagerank Checksum = {g1['pr_checksum']:f}" in out
+    assert f"Degree Checksum = {g1['degree_checksum']}" in out
+    assert "Timer Pagerank Execution stats:" in out
+
+
+def test_sssp_driver(g1_files, g1):
+    out = run("sssp", g1_files[1], 8, 5)
+    assert f"Reachable Vertices = {g1['sssp_reachable']}" in out
+    assert f"Checksum = {g1['sssp_checksum']}" in out
+
+
+def test_bfs_driver(g1_files, g1):
+    out = run("bfs", g1_files[0], 8, 5)
+    assert f"Reachable Vertices = {g1['bfs_reachable']}" in out
+    assert f"Checksum = {g1['bfs_checksum']}" in out
+
+
+def test_cc_tc_degree_drivers(g1_files, g1):
+    assert f"Checksum = {g1['cc_checksum']}" in run("cc", g1_files[0], 8)
+    assert f"Triangles = {g1['tc_total']}" in run("tc", g1_files[0], 8)
+    assert "Out-Degree Checksum = 13" in run("degree", g1_files[0], 8, "-o")
