@@ -166,9 +166,15 @@ la3_status la3_bfs_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint8_
 /* k SSSP roots with k-wide u32 distances (k <= 1024); dist[q * nslots + id]. */
 la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint32_t* dist,
                           uint32_t* iters_run);
-/* k PPR seeds with FP32 ranks and compensated (Kahan) row sums; rank[q * nslots + id]. */
+/* k PPR seeds (k <= 1024), `iters` iterations, FP32 with compensation: each
+ * vertex and query carries its rank (f32) and the rank's compensation term
+ * (f32); messages are f32; each row sums its messages with Kahan summation in
+ * the reference's per-row message order. rank[q * nslots + id] is the f32
+ * rank, rank_comp (may be NULL) its compensation (rank + rank_comp is the
+ * carried value), residual (may be NULL) per query the FP64 L1 change of the
+ * last iteration. */
 la3_status la3_ppr_batch(la3_graph* g, const uint32_t* seeds, uint32_t k, uint32_t iters,
-                         float alpha, float* rank);
+                         double alpha, float* rank, float* rank_comp, double* residual);
 
 /* ------------------------------------------------------------- timing --- */
 /* Device time (ms) of the kernels of the last la3_* program call on this

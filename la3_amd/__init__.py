@@ -94,7 +94,7 @@ def lib():
             "la3_ppr": (i32, [P, u32, u32, ctypes.c_double, P]),
             "la3_bfs_batch": (i32, [P, P, u32, P, P]),
             "la3_sssp_batch": (i32, [P, P, u32, P, P]),
-            "la3_ppr_batch": (i32, [P, P, u32, u32, ctypes.c_float, P]),
+            "la3_ppr_batch": (i32, [P, P, u32, u32, ctypes.c_double, P, P, P]),
             "la3_last_timing": (i32, [P, ctypes.POINTER(Timing)]),
             "la3_set_profiling": (i32, [P, i32]),
             "la3_kernel_stats": (i32, [P, P, u32, ctypes.POINTER(u32)]),
@@ -364,9 +364,14 @@ def sssp_batch(g, roots, out=None):
     return dist, int(it[0])
 
 
-def ppr_batch(g, seeds, iters, alpha=0.15, out=None):
+def ppr_batch(g, seeds, iters, alpha=0.15, out=None, comp=False):
+    """k PPR queries in FP32 with compensation. Returns (rank, comp or None,
+    residual): rank[q, id] f32, comp[q, id] its compensation term, residual[q]
+    the FP64 L1 change of the last iteration."""
     seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
     rank = _out(out, (seeds.size, g.nslots), np.float32)
+    cmp = np.zeros((seeds.size, g.nslots), dtype=np.float32) if comp else None
+    res = np.zeros(seeds.size, dtype=np.float64)
     g.ctx.check(lib().la3_ppr_batch(g.h, _ptr(seeds), seeds.size, int(iters), float(alpha),
-                                    _ptr(rank)), "ppr_batch")
-    return rank
+                                    _ptr(rank), _ptr(cmp), _ptr(res)), "ppr_batch")
+    return rank, cmp, res

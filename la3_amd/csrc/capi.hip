@@ -209,13 +209,17 @@ la3_status graph_from_layout(la3_ctx* ctx, HostLayout& L, la3_graph** out) {
   DG& d = g->dg;
   d.nslots = L.nslots; d.nreg = L.nreg; d.nsink = L.nsink; d.nsrc = L.nsrc; d.ncols = L.ncols;
   d.nrows = L.nrows; d.nnz = L.nnz; d.nlong = (uint32_t)L.long_rows.size();
+  d.nplong = (uint32_t)L.plong.size(); d.nplong_reg = L.nplong_reg;
+  d.npseg = (uint32_t)L.pseg_row.size(); d.npseg_reg = L.npseg_reg;
   la3_status s;
   auto up = [&](const std::vector<uint32_t>& v, const uint32_t** o) { return upload(g.get(), v, o); };
   if ((s = up(L.col_ptr, &d.col_ptr)) || (s = up(L.col_split, &d.col_split)) ||
       (s = up(L.col_rows, &d.col_rows)) || (s = up(L.row_ptr, &d.row_ptr)) ||
       (s = up(L.row_split, &d.row_split)) || (s = up(L.row_cols, &d.row_cols)) ||
       (s = up(L.id_of, &d.id_of)) || (s = up(L.hid_of, &d.hid_of)) ||
-      (s = up(L.long_rows, &d.long_rows)) || (s = up(L.slot_of_id, &d.slot_of_id))) {
+      (s = up(L.long_rows, &d.long_rows)) || (s = up(L.slot_of_id, &d.slot_of_id)) ||
+      (s = up(L.plong, &d.plong)) || (s = up(L.pseg_first, &d.pseg_first)) ||
+      (s = up(L.pseg_row, &d.pseg_row)) || (s = up(L.pseg_beg, &d.pseg_beg))) {
     for (void* p : g->allocs) (void)hipFree(p);
     return s;
   }
@@ -985,10 +989,124 @@ la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint3
   return st;
 }
 
+}  // extern "C"
+
+template <int QW>
+static la3_status mppr_run(la3_graph* g, const std::vector<uint32_t>& seed_slots, uint32_t k,
+                           uint32_t iters, double alpha, float* rank, float* comp, double* residual) {
+  la3_ctx* c = g->ctx;
+  const uint32_t S = g->nslots;
+  constexpr uint32_t K = 64 * QW;
+  const DG& d = g->dg;
+  const size_t SK = (size_t)S * K;
+  const uint32_t wave_blocks = 4 * c->ncu * 8;
+  const uint32_t rblocks = std::max<uint32_t>(1, std::min<uint32_t>((g->nrows + 3) / 4, wave_blocks));
+  const uint32_t cblocks = std::max<uint32_t>(1, std::min<uint32_t>((d.nplong + 3) / 4, wave_blocks));
+  float* x0 = (float*)scratch(c, "mppr.x0", 4 * SK);
+  float* x1 = (float*)scratch(c, "mppr.x1", 4 * SK);
+  float* hi = (float*)scratch(c, "mppr.hi", 4 * SK);
+  float* lo = (float*)scratch(c, "mppr.lo", 4 * SK);
+  uint32_t* deg = (uint32_t*)scratch(c, "pr.deg", sizeof(uint32_t) * S);
+  uint32_t* sd = (uint32_t*)scratch(c, "mppr.seeds", sizeof(uint32_t) * K);
+  double* res_r = (double*)scratch(c, "mppr.res_rows", sizeof(double) * rblocks * K);
+  double* res_c = (double*)scratch(c, "mppr.res_comb", sizeof(double) * cblocks * K);
+  double* res_s = (double*)scratch(c, "mppr.res_sink", sizeof(double) * std::max(rblocks, cblocks) * K);
+  float* part = (float*)scratch(c, "mppr.part", sizeof(float) * 2 * K * ((size_t)d.npseg + 1));
+  float* out = (float*)scratch(c, "mppr.out", sizeof(float) * (size_t)S * k);
+  float* outc = comp ? (float*)scratch(c, "mppr.outc", sizeof(float) * (size_t)S * k) : nullptr;
+  if (!x0 || !x1 || !hi || !lo || !deg || !sd || !res_r || !res_c || !res_s || !part || !out || (comp && !outc))
+    return fail(c, LA3_ENOMEM, "batched PPR: device allocation failed");
+  std::vector<uint32_t> seeds(K, 0xffffffffu);
+  std::copy(seed_slots.begin(), seed_slots.end(), seeds.begin());
+  HIPC(c, hipMemcpyAsync(sd, seeds.data(), sizeof(uint32_t) * K, hipMemcpyHostToDevice, c->stream));
+  PprBatchArgs a{};
+  a.x = x0; a.xnext = x1; a.hi = hi; a.lo = lo; a.deg = deg; a.seeds = sd; a.k = k;
+  a.alpha_hi = (float)alpha;
+  a.alpha_lo = (float)(alpha - (double)a.alpha_hi);
+  const double damp = 1.0 - alpha;
+  a.damp_hi = (float)damp;
+  a.damp_lo = (float)(damp - (double)a.damp_hi);
+  a.part = part; a.seg_row = d.pseg_row; a.seg_beg = d.pseg_beg; a.long_ppr = d.plong;
+  a.nlong_ppr = d.nplong; a.seg_first = d.pseg_first;
+  launch(c, "outdeg", k_outdeg, blocks_for(S), d, deg);
+  launch(c, "mppr_init", k_mppr_init<QW>, blocks_for(SK), d, a);
+  float* xa = x0;
+  float* xb = x1;
+  auto step = [&](uint32_t r0, uint32_t r1, uint32_t s0, uint32_t s1, int loop, double* rr, double* rc) {
+    a.x = xa; a.xnext = xb;
+    a.resid = rr;
+    const int ev = launch(c, "mppr_rows", k_mppr_rows<QW>, rblocks, d, a, r0, r1, loop);
+    if (s1 > s0) {
+      PprBatchArgs as = a;
+      as.seg_row = d.pseg_row + s0; as.seg_beg = d.pseg_beg + s0; as.part = part + (size_t)s0 * 2 * K;
+      launch(c, "mppr_segs", k_mppr_segs<QW>, std::max<uint32_t>(1, std::min<uint32_t>((s1 - s0 + 3) / 4, wave_blocks)),
+             d, as, s1 - s0);
+      a.resid = rc;
+      launch(c, "mppr_combine", k_mppr_combine<QW>, cblocks, d, a, r0, r1, loop);
+    }
+    return ev;
+  };
+  for (uint32_t it = 0; it < iters; it++) {
+    const int ev = step(0, g->nreg, 0, d.npseg_reg, 1, res_r, res_c);
+    // Algorithmic bytes per iteration: 4 B index + 4K B message row per
+    // entry of a regular row, 8K B state read + 12K B state/message written
+    // per regular row.
+    annotate(c, ev, (4.0 + 4.0 * K) * g->nnz_reg + 20.0 * K * g->nreg, g->nnz_reg);
+    c->edges_scanned += g->nnz_reg;
+    std::swap(xa, xb);
+  }
+  if (residual) {
+    std::vector<double> hr((size_t)rblocks * K), hc((size_t)cblocks * K);
+    HIPC(c, hipMemcpyAsync(hr.data(), res_r, sizeof(double) * hr.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipMemcpyAsync(hc.data(), res_c, sizeof(double) * hc.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    for (uint32_t q = 0; q < k; q++) {
+      double t = 0;
+      for (uint32_t b = 0; b < rblocks; b++) t += hr[(size_t)b * K + q];
+      if (d.npseg_reg) for (uint32_t b = 0; b < cblocks; b++) t += hc[(size_t)b * K + q];
+      residual[q] = iters ? t : 0.0;
+    }
+  }
+  // Sink step from the final messages (vertex_program_execute.hpp:131-148).
+  if (g->nsink) step(g->nreg, g->nrows, d.npseg_reg, d.npseg, 0, res_s, res_s);
+  launch(c, "mppr_output", k_batch_output<float>, blocks_for(S), d, (const float*)hi, K, k, out);
+  la3_status st = deliver(c, rank, out, sizeof(float) * (size_t)S * k);
+  if (st) return st;
+  if (comp) {
+    launch(c, "mppr_output", k_batch_output<float>, blocks_for(S), d, (const float*)lo, K, k, outc);
+    HIPC(c, hipMemcpyAsync(comp, outc, sizeof(float) * (size_t)S * k, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LA3_OK;
+}
+
+extern "C" {
+
 la3_status la3_ppr_batch(la3_graph* g, const uint32_t* seeds, uint32_t k, uint32_t iters,
-                         float alpha, float* rank) {
-  (void)seeds; (void)k; (void)iters; (void)alpha; (void)rank;
-  return fail(g ? g->ctx : nullptr, LA3_ENOTSUP, "batched PPR not built yet");
+                         double alpha, float* rank, float* rank_comp, double* residual) {
+  if (!g || !seeds) return LA3_EINVAL;
+  la3_ctx* c = g->ctx;
+  if (k == 0 || k > 1024) return fail(c, LA3_EINVAL, "batch size must be in [1, 1024]");
+  if (!g->directed) return fail(c, LA3_EINVAL, "PPR runs on a directed load");
+  if (!(alpha > 0.0 && alpha < 1.0)) return fail(c, LA3_EINVAL, "alpha must be in (0, 1)");
+  if (iters == 0) return fail(c, LA3_EINVAL, "PPR needs a fixed iteration count");
+  std::vector<uint32_t> ss(k);
+  for (uint32_t q = 0; q < k; q++) {
+    if (seeds[q] > g->n) return fail(c, LA3_EINVAL, "seed id out of range");
+    ss[q] = g->slot_of_id[seeds[q]];
+  }
+  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+  HIPC(c, hipSetDevice(c->device));
+  timing_begin(c);
+  la3_status st;
+  const uint32_t QW = (k + 63) / 64;
+  if (QW <= 1) st = mppr_run<1>(g, ss, k, iters, alpha, rank, rank_comp, residual);
+  else if (QW <= 2) st = mppr_run<2>(g, ss, k, iters, alpha, rank, rank_comp, residual);
+  else if (QW <= 4) st = mppr_run<4>(g, ss, k, iters, alpha, rank, rank_comp, residual);
+  else if (QW <= 8) st = mppr_run<8>(g, ss, k, iters, alpha, rank, rank_comp, residual);
+  else st = mppr_run<16>(g, ss, k, iters, alpha, rank, rank_comp, residual);
+  timing_end(c);
+  return st;
 }
 
 la3_status la3_copy_bandwidth(la3_ctx* c, uint64_t bytes, uint32_t reps, double* gbs) {

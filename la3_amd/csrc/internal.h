@@ -53,7 +53,14 @@ struct HostLayout {
   // row_cols holds the SLOT of the sending vertex.
   std::vector<uint32_t> row_ptr, row_split, row_cols, row_w;
   std::vector<uint32_t> long_rows;  // rows with more than LONG_ROW in-entries
+  // Rows with more than PSEG_ENTRIES in-entries, split into segments of that
+  // many entries (batched PPR): row list (regular rows first, then sinks),
+  // first segment of each row, and per segment its row and first entry.
+  std::vector<uint32_t> plong, pseg_first, pseg_row, pseg_beg;
+  uint32_t nplong_reg = 0, npseg_reg = 0;
 };
+
+constexpr uint32_t PSEG_ENTRIES = 256;
 
 // Builds the layout from LA3 binary records. Returns an empty string on
 // success, else an error message (ids beyond n, nnz >= 2^32).

@@ -40,6 +40,12 @@ struct DG {
   const uint32_t* long_rows;
   uint32_t nlong;
   const uint32_t* slot_of_id;
+  // batched-PPR row segments (see HostLayout)
+  const uint32_t* plong;
+  const uint32_t* pseg_first;
+  const uint32_t* pseg_row;
+  const uint32_t* pseg_beg;
+  uint32_t nplong, nplong_reg, npseg, npseg_reg;
 };
 
 }  // namespace la3

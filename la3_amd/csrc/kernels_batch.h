@@ -423,3 +423,230 @@ __global__ void k_msbfs_output(DG g, const uint8_t* lev, uint32_t K, uint32_t k,
 }
 
 }  // namespace la3
+
+namespace la3 {
+
+// ------------------------------------------------------------- batched PPR ---
+// k personalized-PageRank queries (the PR iteration of pr.h:34-45 with the
+// teleport alpha at each query's seed) in FP32 with compensation:
+//   state per (vertex, query): rank hi (f32) + its compensation lo (f32);
+//   message x = f32((hi + lo) / degree);
+//   row sum y = Kahan sum of the messages in the reference's per-row order
+//   (sources first, then regular columns), one wavefront per row, lane = query;
+//   rank' = tele + (1 - alpha) * y in error-free transforms, renormalized to
+//   (hi, lo).
+// Rows longer than PSEG entries are summed in PSEG-entry segments whose
+// (sum, compensation) pairs are combined in segment order by k_mppr_combine.
+constexpr uint32_t PSEG = PSEG_ENTRIES;
+
+struct PprBatchArgs {
+  const float* x;        // messages [slot * K + q]
+  float* xnext;
+  float* hi;             // rank [slot * K + q]
+  float* lo;             // compensation
+  const uint32_t* deg;   // out-degree by slot
+  const uint32_t* seeds; // seed slot per query [k]
+  uint32_t k;
+  float alpha_hi, alpha_lo;  // alpha as an unevaluated f32 pair (teleport at the seed)
+  float damp_hi, damp_lo;    // (1 - alpha) as an unevaluated f32 pair
+  double* resid;            // per block x K partial |delta| sums (FP64)
+  float* part;              // long-row segment partials [seg * 2K] (sum, comp)
+  const uint32_t* seg_row;  // segment -> row
+  const uint32_t* seg_beg;  // segment -> first entry
+  const uint32_t* long_ppr; // rows with > PSEG entries
+  uint32_t nlong_ppr;
+  const uint32_t* seg_first;  // long row i -> first segment index (nlong_ppr + 1 entries)
+};
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+  const float y = v - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// (hi, lo) of (th + tl) + (dh + dl) * (s - c), all error-free transforms.
+__device__ __forceinline__ void ppr_apply(float th, float tl, float dh, float dl, float s, float c,
+                                          float& nh, float& nl) {
+  const float p = dh * s;
+  const float pe = fmaf(dh, s, -p) + fmaf(dl, s, -dh * c);   // product error + low-order terms
+  const float t = th + p;                                     // TwoSum(th, p)
+  const float bv = t - th;
+  const float te = (th - (t - bv)) + (p - bv);
+  const float e = te + pe + tl;
+  nh = t + e;                                                 // Fast2Sum renormalization
+  nl = e - (nh - t);
+}
+
+__device__ __forceinline__ float ppr_msg(float h, float l, uint32_t d) {
+  const float fd = (float)d;
+  const float q = h / fd;
+  const float r = fmaf(-q, fd, h);   // exact remainder of h / d
+  return q + (r + l) / fd;
+}
+
+template <int QW>
+__device__ __forceinline__ void ppr_finish(const PprBatchArgs& a, uint32_t r, const float* s,
+                                           const float* c, bool loop, double* res) {
+  constexpr uint32_t K = 64 * QW;
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int w = 0; w < QW; w++) {
+    const uint32_t q = 64 * w + lane;
+    const size_t o = (size_t)r * K + q;
+    const bool seed = q < a.k && a.seeds[q] == r;
+    float nh, nl;
+    ppr_apply(seed ? a.alpha_hi : 0.f, seed ? a.alpha_lo : 0.f, a.damp_hi, a.damp_lo, s[w], c[w], nh, nl);
+    const double d = fabs(((double)nh + nl) - ((double)a.hi[o] + a.lo[o]));
+    res[w] += d;
+    a.hi[o] = nh;
+    a.lo[o] = nl;
+    if (loop) a.xnext[o] = ppr_msg(nh, nl, a.deg[r]);
+  }
+}
+
+// Per-block, per-query reduction of the residual partials (FP64, fixed order).
+template <int QW>
+__device__ __forceinline__ void ppr_resid_flush(const PprBatchArgs& a, double* res) {
+  constexpr uint32_t K = 64 * QW;
+  __shared__ double red[BLOCK / WAVE][K];
+  const uint32_t w0 = threadIdx.x / WAVE, lane = lane_id();
+#pragma unroll
+  for (int w = 0; w < QW; w++) red[w0][64 * w + lane] = res[w];
+  __syncthreads();
+  for (uint32_t q = threadIdx.x; q < K; q += BLOCK) {
+    double t = 0;
+    for (uint32_t j = 0; j < BLOCK / WAVE; j++) t += red[j][q];
+    a.resid[(size_t)blockIdx.x * K + q] = t;
+  }
+}
+
+// Rows [r0, r1) with at most PSEG entries: one wavefront per row.
+template <int QW>
+__global__ void __launch_bounds__(BLOCK) k_mppr_rows(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1,
+                                                     int loop) {
+  constexpr uint32_t K = 64 * QW;
+  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  const uint32_t lane = lane_id();
+  double res[QW];
+#pragma unroll
+  for (int w = 0; w < QW; w++) res[w] = 0.0;
+  for (uint32_t r = r0 + wid; r < r1; r += nwaves) {
+    const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1];
+    if (e - b > PSEG) continue;  // wave-uniform; segmented path
+    float s[QW], c[QW];
+#pragma unroll
+    for (int w = 0; w < QW; w++) { s[w] = 0.f; c[w] = 0.f; }
+    uint32_t j = b;
+    for (; j + 4 <= e; j += 4) {
+      const uint32_t s0 = g.row_cols[j], s1 = g.row_cols[j + 1], s2 = g.row_cols[j + 2],
+                     s3 = g.row_cols[j + 3];
+#pragma unroll
+      for (int w = 0; w < QW; w++) {
+        const float v0 = a.x[(size_t)s0 * K + 64 * w + lane], v1 = a.x[(size_t)s1 * K + 64 * w + lane];
+        const float v2 = a.x[(size_t)s2 * K + 64 * w + lane], v3 = a.x[(size_t)s3 * K + 64 * w + lane];
+        kahan_add(s[w], c[w], v0);
+        kahan_add(s[w], c[w], v1);
+        kahan_add(s[w], c[w], v2);
+        kahan_add(s[w], c[w], v3);
+      }
+    }
+    for (; j < e; j++) {
+      const uint32_t s0 = g.row_cols[j];
+#pragma unroll
+      for (int w = 0; w < QW; w++) kahan_add(s[w], c[w], a.x[(size_t)s0 * K + 64 * w + lane]);
+    }
+    ppr_finish<QW>(a, r, s, c, loop != 0, res);
+  }
+  ppr_resid_flush<QW>(a, res);
+}
+
+// One wavefront per PSEG-entry segment of a long row: partial (sum, comp).
+template <int QW>
+__global__ void __launch_bounds__(BLOCK) k_mppr_segs(DG g, PprBatchArgs a, uint32_t nseg) {
+  constexpr uint32_t K = 64 * QW;
+  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  const uint32_t lane = lane_id();
+  for (uint32_t sg = wid; sg < nseg; sg += nwaves) {
+    const uint32_t r = a.seg_row[sg], b = a.seg_beg[sg];
+    const uint32_t e = min(b + PSEG, g.row_ptr[r + 1]);
+    float s[QW], c[QW];
+#pragma unroll
+    for (int w = 0; w < QW; w++) { s[w] = 0.f; c[w] = 0.f; }
+    uint32_t j = b;
+    for (; j + 4 <= e; j += 4) {
+      const uint32_t s0 = g.row_cols[j], s1 = g.row_cols[j + 1], s2 = g.row_cols[j + 2],
+                     s3 = g.row_cols[j + 3];
+#pragma unroll
+      for (int w = 0; w < QW; w++) {
+        const float v0 = a.x[(size_t)s0 * K + 64 * w + lane], v1 = a.x[(size_t)s1 * K + 64 * w + lane];
+        const float v2 = a.x[(size_t)s2 * K + 64 * w + lane], v3 = a.x[(size_t)s3 * K + 64 * w + lane];
+        kahan_add(s[w], c[w], v0);
+        kahan_add(s[w], c[w], v1);
+        kahan_add(s[w], c[w], v2);
+        kahan_add(s[w], c[w], v3);
+      }
+    }
+    for (; j < e; j++) {
+      const uint32_t s0 = g.row_cols[j];
+#pragma unroll
+      for (int w = 0; w < QW; w++) kahan_add(s[w], c[w], a.x[(size_t)s0 * K + 64 * w + lane]);
+    }
+#pragma unroll
+    for (int w = 0; w < QW; w++) {
+      a.part[(size_t)sg * 2 * K + 64 * w + lane] = s[w];
+      a.part[(size_t)sg * 2 * K + K + 64 * w + lane] = c[w];
+    }
+  }
+}
+
+// Combines each long row's segment partials in segment order, then applies.
+template <int QW>
+__global__ void __launch_bounds__(BLOCK) k_mppr_combine(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1,
+                                                        int loop) {
+  constexpr uint32_t K = 64 * QW;
+  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  const uint32_t lane = lane_id();
+  double res[QW];
+#pragma unroll
+  for (int w = 0; w < QW; w++) res[w] = 0.0;
+  for (uint32_t i = wid; i < a.nlong_ppr; i += nwaves) {
+    const uint32_t r = a.long_ppr[i];
+    if (r < r0 || r >= r1) continue;
+    float s[QW], c[QW];
+#pragma unroll
+    for (int w = 0; w < QW; w++) { s[w] = 0.f; c[w] = 0.f; }
+    for (uint32_t sg = a.seg_first[i]; sg < a.seg_first[i + 1]; sg++) {
+#pragma unroll
+      for (int w = 0; w < QW; w++) {
+        kahan_add(s[w], c[w], a.part[(size_t)sg * 2 * K + 64 * w + lane]);
+        kahan_add(s[w], c[w], -a.part[(size_t)sg * 2 * K + K + 64 * w + lane]);
+      }
+    }
+    ppr_finish<QW>(a, r, s, c, loop != 0, res);
+  }
+  ppr_resid_flush<QW>(a, res);
+}
+
+// hi = lo = tele; x = message for every vertex with a column (constant for sources).
+template <int QW>
+__global__ void k_mppr_init(DG g, PprBatchArgs a) {
+  constexpr uint32_t K = 64 * QW;
+  const uint64_t total = (uint64_t)g.nslots * K;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = (uint32_t)(i / K), q = (uint32_t)(i % K);
+    const bool seed = q < a.k && a.seeds[q] == s;
+    const float th = seed ? a.alpha_hi : 0.f, tl = seed ? a.alpha_lo : 0.f;
+    a.hi[i] = th;
+    a.lo[i] = tl;
+    const float x = slot_has_col(g, s) ? ppr_msg(th, tl, a.deg[s]) : 0.f;
+    ((float*)a.x)[i] = x;
+    a.xnext[i] = x;
+  }
+}
+
+}  // namespace la3

@@ -255,3 +255,48 @@ def test_sssp_batch_equals_single(gpu_ctx, oracle, scale, seed, k):
         assert (dist[q] == d1).all(), f"query {q} (root {r}) differs from single-query"
     for q in range(0, len(roots), max(1, len(roots) // 4)):
         assert (dist[q] == oracle.sssp(o, roots[q])[0]).all()
+
+
+# ---------------------------------------------------------- batched PPR ---
+PPR_L1_TOL = 1e-7
+
+
+@pytest.mark.parametrize("scale,seed,k", [(12, 51, 5), (14, 52, 64), (16, 53, 128)])
+def test_ppr_batch_fp32_vs_fp64_oracle(gpu_ctx, oracle, scale, seed, k):
+    """FP32 + compensation within L1 <= 1e-7 of the FP64 oracle, per query."""
+    rec = _rmat(scale, seed)
+    n = 1 << scale
+    g = L.Graph.from_records(gpu_ctx, rec, n, False, L.DIRECTED)
+    o = oracle.OracleGraph(rec, n, False, oracle.DIRECTED)
+    orr = oracle.OracleGraph(rec, n, False, oracle.REVERSED)
+    seeds = _roots(rec, k, seed)
+    if k > 2:
+        seeds[1] = seeds[0]
+    rank, comp, resid = L.ppr_batch(g, seeds, 30, comp=True)
+    worst = 0.0
+    for q in range(0, k, max(1, k // 6)):
+        e, _ = oracle.ppr(o, orr, seeds[q], 30)
+        l1 = float(np.abs(rank[q].astype(np.float64) - e).sum())
+        l1c = float(np.abs(rank[q].astype(np.float64) + comp[q] - e).sum())
+        nz = e > 0
+        mrel = float((np.abs(rank[q] - e)[nz] / e[nz]).max())
+        worst = max(worst, l1)
+        assert l1 <= PPR_L1_TOL, f"query {q}: L1 {l1:.3e}"
+        assert l1c <= l1 + 1e-12
+        assert mrel < 1e-5
+        assert np.isfinite(resid[q]) and resid[q] >= 0
+    print(f"PPR batch k={k} scale {scale}: worst L1 vs FP64 {worst:.3e}")
+    # a query in a batch equals the same query alone
+    r1, _, _ = L.ppr_batch(g, [seeds[3]], 30)
+    assert (r1[0] == rank[3]).all()
+
+
+def test_ppr_batch_matches_fp64_single(gpu_ctx):
+    rec = _rmat(14, 54)
+    n = 1 << 14
+    g = L.Graph.from_records(gpu_ctx, rec, n, False, L.DIRECTED)
+    seeds = _roots(rec, 3, 54)
+    rank, _, _ = L.ppr_batch(g, seeds, 20)
+    for q, s in enumerate(seeds):
+        r64 = L.ppr(g, s, 20)
+        assert float(np.abs(rank[q] - r64).sum()) <= PPR_L1_TOL
