@@ -176,6 +176,63 @@ la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint3
 la3_status la3_ppr_batch(la3_graph* g, const uint32_t* seeds, uint32_t k, uint32_t iters,
                          double alpha, float* rank, float* rank_comp, double* residual);
 
+/* The batched programs on an explicit context: the batch runs on `ctx`'s
+ * stream with `ctx`'s scratch instead of the graph's own context, so several
+ * host threads (each with its own context on the same device) can run
+ * batches on one resident graph concurrently. */
+la3_status la3_bfs_batch_ctx(la3_ctx* ctx, la3_graph* g, const uint32_t* roots, uint32_t k,
+                             uint8_t* hops, uint32_t* iters_run);
+la3_status la3_sssp_batch_ctx(la3_ctx* ctx, la3_graph* g, const uint32_t* roots, uint32_t k,
+                              uint32_t* dist, uint32_t* iters_run);
+la3_status la3_ppr_batch_ctx(la3_ctx* ctx, la3_graph* g, const uint32_t* seeds, uint32_t k,
+                             uint32_t iters, double alpha, float* rank, float* rank_comp,
+                             double* residual);
+
+/* ------------------------------------------------------------ serving --- */
+/* A request loop over resident graphs. Queries (BFS, SSSP, PPR) are
+ * submitted one by one; `nworkers` host threads, each with its own context
+ * (stream, scratch) on the device, repeatedly form a batch from the pending
+ * queries — the oldest pending query fixes the graph, the program and (for
+ * PPR) the iteration count, and up to max_batch pending queries with that
+ * key join it in arrival order — as long as the per-query state bytes of
+ * the admitted batches stay within state_budget, run it as one batched
+ * launch sequence, and record each query's completion. Queries that arrive
+ * while batches are on the GPU are admitted as workers and budget free up;
+ * batches of different workers overlap on the device. Each query's result
+ * is the one the query gets alone (la3_*_batch with k = 1), whatever batch
+ * it lands in. No counterpart in the reference. */
+typedef enum { LA3_Q_BFS = 0, LA3_Q_SSSP = 1, LA3_Q_PPR = 2 } la3_query_kind;
+typedef struct la3_server la3_server;
+typedef struct {
+  uint32_t nworkers;      /* concurrent batches (host threads, one context each) */
+  uint32_t max_batch;     /* queries per batch, 1..1024 */
+  uint64_t state_budget;  /* bytes of per-query state admitted at once */
+  int keep_results;       /* copy each result to host and keep it for la3_server_result */
+  double ppr_alpha;       /* teleport probability of PPR queries */
+} la3_server_config;
+typedef struct {
+  uint64_t qid;
+  uint32_t kind, graph, root, batch_size;
+  int32_t status;                   /* la3_status of the batch that ran it */
+  double submit_s, start_s, done_s; /* server clock, seconds */
+} la3_query_info;
+la3_status la3_server_create(int device, const la3_server_config* cfg, la3_server** out);
+/* Registers a resident graph (loaded on any context of the same device). */
+la3_status la3_server_add_graph(la3_server* s, la3_graph* g, uint32_t* graph_id);
+/* Bytes of per-query state a query of `kind` on the graph is admitted with. */
+uint64_t la3_server_query_bytes(const la3_server* s, la3_query_kind kind, uint32_t graph_id);
+la3_status la3_server_submit(la3_server* s, la3_query_kind kind, uint32_t graph_id, uint32_t root,
+                             uint32_t iters, uint64_t* qid);
+/* Blocks until every submitted query has completed. */
+la3_status la3_server_drain(la3_server* s);
+/* Pops up to `cap` completion records (oldest first). */
+la3_status la3_server_completed(la3_server* s, la3_query_info* out, uint32_t cap, uint32_t* n);
+/* Copies a kept result (BFS u8 hops, SSSP u32 dist, PPR f32 rank; nslots
+ * entries by original id) and releases it. */
+la3_status la3_server_result(la3_server* s, uint64_t qid, void* dst, uint64_t bytes);
+double la3_server_clock(const la3_server* s);
+void la3_server_destroy(la3_server* s);
+
 /* ------------------------------------------------------------- timing --- */
 /* Device time (ms) of the kernels of the last la3_* program call on this
  * context, measured with HIP events on the context's stream, and the name of

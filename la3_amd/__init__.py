@@ -32,7 +32,12 @@ EXPORTS = [
     "la3_last_timing", "la3_set_profiling", "la3_rmat_generate",
     "la3_kernel_stats", "la3_kernel_stats_reset", "la3_host_alloc", "la3_host_free",
     "la3_copy_bandwidth", "la3_result_fetch",
+    "la3_bfs_batch_ctx", "la3_sssp_batch_ctx", "la3_ppr_batch_ctx",
+    "la3_server_create", "la3_server_add_graph", "la3_server_query_bytes", "la3_server_submit",
+    "la3_server_drain", "la3_server_completed", "la3_server_result", "la3_server_clock",
+    "la3_server_destroy",
 ]
+Q_BFS, Q_SSSP, Q_PPR = 0, 1, 2
 
 
 class LA3Error(RuntimeError):
@@ -52,6 +57,19 @@ class KernelStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_uint64),
                 ("ms", ctypes.c_double), ("alg_bytes", ctypes.c_double),
                 ("entries", ctypes.c_uint64)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("nworkers", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
+                ("state_budget", ctypes.c_uint64), ("keep_results", ctypes.c_int),
+                ("ppr_alpha", ctypes.c_double)]
+
+
+class QueryInfo(ctypes.Structure):
+    _fields_ = [("qid", ctypes.c_uint64), ("kind", ctypes.c_uint32), ("graph", ctypes.c_uint32),
+                ("root", ctypes.c_uint32), ("batch_size", ctypes.c_uint32),
+                ("status", ctypes.c_int32), ("submit_s", ctypes.c_double),
+                ("start_s", ctypes.c_double), ("done_s", ctypes.c_double)]
 
 
 class Timing(ctypes.Structure):
@@ -103,6 +121,18 @@ def lib():
             "la3_host_free": (None, [P]),
             "la3_copy_bandwidth": (i32, [P, u64, u32, ctypes.POINTER(ctypes.c_double)]),
             "la3_result_fetch": (i32, [P, P, u64]),
+            "la3_bfs_batch_ctx": (i32, [P, P, P, u32, P, P]),
+            "la3_sssp_batch_ctx": (i32, [P, P, P, u32, P, P]),
+            "la3_ppr_batch_ctx": (i32, [P, P, P, u32, u32, ctypes.c_double, P, P, P]),
+            "la3_server_create": (i32, [i32, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]),
+            "la3_server_add_graph": (i32, [P, P, ctypes.POINTER(u32)]),
+            "la3_server_query_bytes": (u64, [P, i32, u32]),
+            "la3_server_submit": (i32, [P, i32, u32, u32, u32, ctypes.POINTER(u64)]),
+            "la3_server_drain": (i32, [P]),
+            "la3_server_completed": (i32, [P, P, u32, ctypes.POINTER(u32)]),
+            "la3_server_result": (i32, [P, u64, P, u64]),
+            "la3_server_clock": (ctypes.c_double, [P]),
+            "la3_server_destroy": (None, [P]),
             "la3_rmat_generate": (i32, [u32, u32, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, u64, i32, P, ctypes.POINTER(u64)]),
         }
@@ -375,3 +405,66 @@ def ppr_batch(g, seeds, iters, alpha=0.15, out=None, comp=False):
     g.ctx.check(lib().la3_ppr_batch(g.h, _ptr(seeds), seeds.size, int(iters), float(alpha),
                                     _ptr(rank), _ptr(cmp), _ptr(res)), "ppr_batch")
     return rank, cmp, res
+
+
+class Server:
+    """Serving loop (include/la3.h, "serving"): resident graphs, continuous
+    batching by worker threads, admission by per-query state bytes."""
+
+    def __init__(self, device=0, nworkers=2, max_batch=64, state_budget=32 << 30,
+                 keep_results=False, ppr_alpha=0.15):
+        cfg = ServerConfig(nworkers, max_batch, state_budget, int(keep_results), ppr_alpha)
+        h = ctypes.c_void_p()
+        st = lib().la3_server_create(device, ctypes.byref(cfg), ctypes.byref(h))
+        if st:
+            raise LA3Error(st, "la3_server_create")
+        self.h = h
+        self.graphs = []
+
+    def add_graph(self, g):
+        gid = ctypes.c_uint32()
+        st = lib().la3_server_add_graph(self.h, g.h, ctypes.byref(gid))
+        if st:
+            raise LA3Error(st, "la3_server_add_graph")
+        self.graphs.append(g)  # keeps the graph alive while served
+        return gid.value
+
+    def query_bytes(self, kind, gid):
+        return int(lib().la3_server_query_bytes(self.h, kind, gid))
+
+    def submit(self, kind, gid, root, iters=0):
+        q = ctypes.c_uint64()
+        st = lib().la3_server_submit(self.h, kind, gid, int(root), int(iters), ctypes.byref(q))
+        if st:
+            raise LA3Error(st, "la3_server_submit")
+        return q.value
+
+    def clock(self):
+        return lib().la3_server_clock(self.h)
+
+    def drain(self):
+        lib().la3_server_drain(self.h)
+
+    def completed(self, cap=4096):
+        out, arr, n = [], (QueryInfo * cap)(), ctypes.c_uint32()
+        while True:
+            lib().la3_server_completed(self.h, arr, cap, ctypes.byref(n))
+            out += [{f: getattr(a, f) for f, _ in QueryInfo._fields_} for a in arr[:n.value]]
+            if n.value < cap:
+                return out
+
+    def result(self, qid, kind, nslots):
+        dt = {Q_BFS: np.uint8, Q_SSSP: np.uint32, Q_PPR: np.float32}[kind]
+        out = np.zeros(nslots, dtype=dt)
+        st = lib().la3_server_result(self.h, qid, _ptr(out), out.nbytes)
+        if st:
+            raise LA3Error(st, f"la3_server_result({qid})")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().la3_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -32,11 +32,13 @@ def _run(cmd):
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     host_o = os.path.join(BUILD, "host_build.o")
+    serve_o = os.path.join(BUILD, "server.o")
     capi_o = os.path.join(BUILD, "capi.o")
     _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "host_build.cpp"), "-o", host_o])
+    _run(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, "server.cpp"), "-o", serve_o])
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "capi.hip"), "-o", capi_o])
     tmp = OUT + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", capi_o, host_o, "-o", tmp,
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", capi_o, host_o, serve_o, "-o", tmp,
           "-pthread", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, OUT)
     # C++ drivers over include/la3.hpp (the reference's app mains).

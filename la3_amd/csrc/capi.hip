@@ -267,10 +267,10 @@ struct Fronts {
   uint64_t cap;
 };
 
-la3_status get_fronts(la3_graph* g, Fronts& F, const char* tag) {
+la3_status get_fronts(la3_ctx* c, la3_graph* g, Fronts& F, const char* tag) {
   F.cap = (uint64_t)g->nslots + g->nnz / CHUNK + 2;
   for (int i = 0; i < 2; i++) {
-    la3_status st = get_queue(g->ctx, std::string(tag) + char('0' + i), F.cap, F.f[i]);
+    la3_status st = get_queue(c, std::string(tag) + char('0' + i), F.cap, F.f[i]);
     if (st) return st;
   }
   return LA3_OK;
@@ -517,7 +517,7 @@ la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
   uint8_t* level = (uint8_t*)scratch(c, "bfs.level", S);
   uint32_t* mark = (uint32_t*)scratch(c, "bfs.mark", sizeof(uint32_t) * S);
   Fronts F;
-  la3_status st = get_fronts(g, F, "bfs");
+  la3_status st = get_fronts(c, g, F, "bfs");
   if (st) return st;
   if (!level || !mark) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   const DG& d = g->dg;
@@ -591,7 +591,7 @@ la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters
   uint32_t* dv = (uint32_t*)scratch(c, "sssp.dist", sizeof(uint32_t) * S);
   uint32_t* stamp = (uint32_t*)scratch(c, "sssp.stamp", sizeof(uint32_t) * S);
   Fronts F;
-  la3_status st = get_fronts(g, F, "bfs");
+  la3_status st = get_fronts(c, g, F, "bfs");
   if (st) return st;
   if (!dv || !stamp) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   const DG& d = g->dg;
@@ -776,9 +776,8 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
 }  // extern "C"
 
 template <int W>
-static la3_status msbfs_run(la3_graph* g, const std::vector<uint32_t>& root_slots, uint32_t k,
+static la3_status msbfs_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>& root_slots, uint32_t k,
                             uint8_t* hops, uint32_t* iters_run) {
-  la3_ctx* c = g->ctx;
   const uint32_t S = g->nslots;
   const uint32_t K = 64 * W;
   const DG& d = g->dg;
@@ -791,7 +790,7 @@ static la3_status msbfs_run(la3_graph* g, const std::vector<uint32_t>& root_slot
   uint64_t* valid = (uint64_t*)scratch(c, "msbfs.valid", sizeof(uint64_t) * W);
   uint8_t* out = (uint8_t*)scratch(c, "msbfs.out", (size_t)S * k);
   Fronts F;
-  la3_status st = get_fronts(g, F, "bfs");
+  la3_status st = get_fronts(c, g, F, "bfs");
   if (st) return st;
   Frontier T;
   if ((st = get_queue(c, "msbfs.touched", (uint64_t)S + 1, T))) return st;
@@ -865,8 +864,14 @@ extern "C" {
 
 la3_status la3_bfs_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint8_t* hops,
                          uint32_t* iters_run) {
-  if (!g || !roots) return LA3_EINVAL;
-  la3_ctx* c = g->ctx;
+  if (!g) return LA3_EINVAL;
+  return la3_bfs_batch_ctx(g->ctx, g, roots, k, hops, iters_run);
+}
+
+la3_status la3_bfs_batch_ctx(la3_ctx* c, la3_graph* g, const uint32_t* roots, uint32_t k,
+                             uint8_t* hops, uint32_t* iters_run) {
+  if (!c || !g || !roots) return LA3_EINVAL;
+  if (c->device != g->ctx->device) return fail(c, LA3_EINVAL, "graph lives on another device");
   if (k == 0 || k > 1024) return fail(c, LA3_EINVAL, "batch size must be in [1, 1024]");
   std::vector<uint32_t> rs(k);
   for (uint32_t q = 0; q < k; q++) {
@@ -878,11 +883,11 @@ la3_status la3_bfs_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint8_
   timing_begin(c);
   la3_status st;
   const uint32_t W = (k + 63) / 64;
-  if (W <= 1) st = msbfs_run<1>(g, rs, k, hops, iters_run);
-  else if (W <= 2) st = msbfs_run<2>(g, rs, k, hops, iters_run);
-  else if (W <= 4) st = msbfs_run<4>(g, rs, k, hops, iters_run);
-  else if (W <= 8) st = msbfs_run<8>(g, rs, k, hops, iters_run);
-  else st = msbfs_run<16>(g, rs, k, hops, iters_run);
+  if (W <= 1) st = msbfs_run<1>(c, g, rs, k, hops, iters_run);
+  else if (W <= 2) st = msbfs_run<2>(c, g, rs, k, hops, iters_run);
+  else if (W <= 4) st = msbfs_run<4>(c, g, rs, k, hops, iters_run);
+  else if (W <= 8) st = msbfs_run<8>(c, g, rs, k, hops, iters_run);
+  else st = msbfs_run<16>(c, g, rs, k, hops, iters_run);
   timing_end(c);
   return st;
 }
@@ -890,9 +895,8 @@ la3_status la3_bfs_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint8_
 }  // extern "C"
 
 template <int QW>
-static la3_status mssp_run(la3_graph* g, const std::vector<uint32_t>& root_slots, uint32_t k,
+static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>& root_slots, uint32_t k,
                            uint32_t* dist_out, uint32_t* iters_run) {
-  la3_ctx* c = g->ctx;
   const uint32_t S = g->nslots;
   constexpr uint32_t K = 64 * QW;
   const DG& d = g->dg;
@@ -903,7 +907,7 @@ static la3_status mssp_run(la3_graph* g, const std::vector<uint32_t>& root_slots
   uint32_t* rs = (uint32_t*)scratch(c, "mssp.roots", sizeof(uint32_t) * k);
   uint32_t* out = (uint32_t*)scratch(c, "mssp.out", sizeof(uint32_t) * (size_t)S * k);
   Fronts F;
-  la3_status st = get_fronts(g, F, "bfs");
+  la3_status st = get_fronts(c, g, F, "bfs");
   if (st) return st;
   Frontier T;
   if ((st = get_queue(c, "mssp.touched", (uint64_t)S + 1, T))) return st;
@@ -965,8 +969,14 @@ extern "C" {
 
 la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint32_t* dist,
                           uint32_t* iters_run) {
-  if (!g || !roots) return LA3_EINVAL;
-  la3_ctx* c = g->ctx;
+  if (!g) return LA3_EINVAL;
+  return la3_sssp_batch_ctx(g->ctx, g, roots, k, dist, iters_run);
+}
+
+la3_status la3_sssp_batch_ctx(la3_ctx* c, la3_graph* g, const uint32_t* roots, uint32_t k,
+                              uint32_t* dist, uint32_t* iters_run) {
+  if (!c || !g || !roots) return LA3_EINVAL;
+  if (c->device != g->ctx->device) return fail(c, LA3_EINVAL, "graph lives on another device");
   if (k == 0 || k > 1024) return fail(c, LA3_EINVAL, "batch size must be in [1, 1024]");
   if (!g->weighted) return fail(c, LA3_EINVAL, "SSSP needs a weighted graph");
   if (!g->directed) return fail(c, LA3_EINVAL, "SSSP runs on a directed load");
@@ -980,11 +990,11 @@ la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint3
   timing_begin(c);
   la3_status st;
   const uint32_t QW = (k + 63) / 64;
-  if (QW <= 1) st = mssp_run<1>(g, rs, k, dist, iters_run);
-  else if (QW <= 2) st = mssp_run<2>(g, rs, k, dist, iters_run);
-  else if (QW <= 4) st = mssp_run<4>(g, rs, k, dist, iters_run);
-  else if (QW <= 8) st = mssp_run<8>(g, rs, k, dist, iters_run);
-  else st = mssp_run<16>(g, rs, k, dist, iters_run);
+  if (QW <= 1) st = mssp_run<1>(c, g, rs, k, dist, iters_run);
+  else if (QW <= 2) st = mssp_run<2>(c, g, rs, k, dist, iters_run);
+  else if (QW <= 4) st = mssp_run<4>(c, g, rs, k, dist, iters_run);
+  else if (QW <= 8) st = mssp_run<8>(c, g, rs, k, dist, iters_run);
+  else st = mssp_run<16>(c, g, rs, k, dist, iters_run);
   timing_end(c);
   return st;
 }
@@ -992,9 +1002,8 @@ la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint3
 }  // extern "C"
 
 template <int QW>
-static la3_status mppr_run(la3_graph* g, const std::vector<uint32_t>& seed_slots, uint32_t k,
+static la3_status mppr_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>& seed_slots, uint32_t k,
                            uint32_t iters, double alpha, float* rank, float* comp, double* residual) {
-  la3_ctx* c = g->ctx;
   const uint32_t S = g->nslots;
   constexpr uint32_t K = 64 * QW;
   const DG& d = g->dg;
@@ -1084,8 +1093,15 @@ extern "C" {
 
 la3_status la3_ppr_batch(la3_graph* g, const uint32_t* seeds, uint32_t k, uint32_t iters,
                          double alpha, float* rank, float* rank_comp, double* residual) {
-  if (!g || !seeds) return LA3_EINVAL;
-  la3_ctx* c = g->ctx;
+  if (!g) return LA3_EINVAL;
+  return la3_ppr_batch_ctx(g->ctx, g, seeds, k, iters, alpha, rank, rank_comp, residual);
+}
+
+la3_status la3_ppr_batch_ctx(la3_ctx* c, la3_graph* g, const uint32_t* seeds, uint32_t k,
+                             uint32_t iters, double alpha, float* rank, float* rank_comp,
+                             double* residual) {
+  if (!c || !g || !seeds) return LA3_EINVAL;
+  if (c->device != g->ctx->device) return fail(c, LA3_EINVAL, "graph lives on another device");
   if (k == 0 || k > 1024) return fail(c, LA3_EINVAL, "batch size must be in [1, 1024]");
   if (!g->directed) return fail(c, LA3_EINVAL, "PPR runs on a directed load");
   if (!(alpha > 0.0 && alpha < 1.0)) return fail(c, LA3_EINVAL, "alpha must be in (0, 1)");
@@ -1100,11 +1116,11 @@ la3_status la3_ppr_batch(la3_graph* g, const uint32_t* seeds, uint32_t k, uint32
   timing_begin(c);
   la3_status st;
   const uint32_t QW = (k + 63) / 64;
-  if (QW <= 1) st = mppr_run<1>(g, ss, k, iters, alpha, rank, rank_comp, residual);
-  else if (QW <= 2) st = mppr_run<2>(g, ss, k, iters, alpha, rank, rank_comp, residual);
-  else if (QW <= 4) st = mppr_run<4>(g, ss, k, iters, alpha, rank, rank_comp, residual);
-  else if (QW <= 8) st = mppr_run<8>(g, ss, k, iters, alpha, rank, rank_comp, residual);
-  else st = mppr_run<16>(g, ss, k, iters, alpha, rank, rank_comp, residual);
+  if (QW <= 1) st = mppr_run<1>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
+  else if (QW <= 2) st = mppr_run<2>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
+  else if (QW <= 4) st = mppr_run<4>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
+  else if (QW <= 8) st = mppr_run<8>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
+  else st = mppr_run<16>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
   timing_end(c);
   return st;
 }
