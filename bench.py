@@ -153,6 +153,207 @@ def kernel_table(stats):
             for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])[:6]}
 
 
+def base_line(args, value, unit, ms_per_step, dtype, workload, **extra):
+    out = {"metric": METRIC, "value": round(value, 4), "unit": unit, "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic", "config": {"workload": workload}}
+    out.update(extra)
+    return out
+
+
+def workload_pr_s16(args, L):
+    """PageRank, 20 iterations, FP64, directed R-MAT scale 16 seed 1."""
+    from oracle import oracle as O
+    scale, n = 16, 1 << 16
+    rec = L.rmat(scale, 16, 0.57, 0.19, 0.19, 1)
+    ctx = L.Context(0)
+    g = L.Graph.from_records(ctx, rec, n, False, L.DIRECTED)
+    rank, _, _ = L.pagerank(g, 20)
+    o, orr = O.OracleGraph(rec, n, False, O.DIRECTED), O.OracleGraph(rec, n, False, O.REVERSED)
+    t = time.perf_counter()
+    erank, _, _ = O.pagerank(o, orr, 20)
+    cpu_s = time.perf_counter() - t
+    l1 = float(np.abs(rank - erank).sum())
+    for _ in range(args.warmup):
+        L.pagerank(g, 20)
+    el, _, st = timed(ctx, lambda k: [L.pagerank(g, 20, out=L.DEVICE) for _ in range(k)], args.steps)
+    edges = g.info["nnz"] * 20
+    return base_line(args, edges * args.steps / el / 1e9, "GTEPS", el * 1e3 / args.steps, "f64",
+                     "PageRank 20 iterations FP64, directed R-MAT scale 16 ef 16 seed 1",
+                     l1_vs_oracle=l1, roofline=roofline(st), kernels=kernel_table(st),
+                     cpu_baseline={"value": edges / cpu_s / 1e9, "unit": "GTEPS", "cores": 1,
+                                   "kind": "port", "sample": "one full 20-iteration run (oracle)"})
+
+
+def workload_cc_tc_s20(args, L):
+    """CC (undirected load) and TC (acyclic load) on R-MAT scale 20 seed 3, symmetrized."""
+    from oracle import oracle as O
+    scale, n = 20, 1 << 20
+    rec = L.rmat(scale, 16, 0.57, 0.19, 0.19, 3)
+    ctx = L.Context(0)
+    gu = L.Graph.from_records(ctx, rec, n, False, L.UNDIRECTED)
+    ga = L.Graph.from_records(ctx, rec, n, False, L.ACYCLIC)
+    lab, _ = L.cc(gu)
+    total, _ = L.tc(ga)
+    el_cc, _, st_cc = timed(ctx, lambda k: [L.cc(gu, out=L.DEVICE) for _ in range(k)], args.steps)
+    el_tc, _, st_tc = timed(ctx, lambda k: [L.tc(ga, per_vertex=False) for _ in range(k)], args.steps)
+    ou = O.OracleGraph(rec, n, False, O.UNDIRECTED)
+    t = time.perf_counter()
+    elab, _ = O.cc(ou)
+    cpu_cc = time.perf_counter() - t
+    oa, ora = O.OracleGraph(rec, n, False, O.ACYCLIC), O.OracleGraph(rec, n, False, O.REVERSED_ACYCLIC)
+    t = time.perf_counter()
+    etotal, _ = O.tc(oa, ora)
+    cpu_tc = time.perf_counter() - t
+    cc_e = gu.info["nnz"]
+    return base_line(args, cc_e * args.steps / el_cc / 1e9, "GTEPS", el_cc * 1e3 / args.steps, "u32",
+                     "CC (value) and TC on symmetrized R-MAT scale 20 ef 16 seed 3",
+                     cc_exact=bool((lab == elab).all()), tc_total=total, tc_exact=total == etotal,
+                     tc_ms=round(el_tc * 1e3 / args.steps, 3), roofline=roofline(st_cc),
+                     tc_roofline=roofline(st_tc),
+                     cpu_baseline={"value": cc_e / cpu_cc / 1e9, "unit": "GTEPS", "cores": 1,
+                                   "kind": "port", "sample": "one CC run (oracle label propagation)",
+                                   "tc_s": round(cpu_tc, 3)})
+
+
+def workload_batched_s24(args, L):
+    """256 BFS + 256 SSSP + 128 PPR (30 iterations) on R-MAT scale 24 seed 4,
+    batched vs one query at a time."""
+    scale, seed, n = 24, 4, 1 << 24
+    rec = L.rmat(scale, 16, 0.57, 0.19, 0.19, seed, weighted=True)
+    outdeg = np.bincount(rec[:, 0], minlength=n + 1)
+    ctx = L.Context(0)
+    g = L.Graph.from_records(ctx, rec, n, True, L.DIRECTED)
+    log(f"scale 24 loaded: {g.info}")
+    bfs_r, sssp_r, ppr_s = choose_roots(rec, 256, 41), choose_roots(rec, 256, 42), choose_roots(rec, 128, 43)
+    D = L.DEVICE
+    hb, _ = L.bfs_batch(g, bfs_r)
+    db, _ = L.sssp_batch(g, sssp_r)
+    pb, _, resid = L.ppr_batch(g, ppr_s, 30)
+    edges = 0
+    for q, r in enumerate(bfs_r):
+        h1 = L.bfs(g, r, parents=False)[0]
+        assert (hb[q] == h1).all()
+        edges += traversed_edges(outdeg, h1 != L.BFS_INF)
+    for q, r in enumerate(sssp_r):
+        d1 = L.sssp(g, r)[0]
+        assert (db[q] == d1).all()
+        edges += traversed_edges(outdeg, d1 != L.SSSP_INF)
+    l1s, rels = [], []
+    for q in range(0, 128, 16):
+        r64 = L.ppr(g, ppr_s[q], 30)
+        l1s.append(float(np.abs(pb[q] - r64).sum()))
+        nz = r64 > 0
+        rels.append(float((np.abs(pb[q] - r64)[nz] / r64[nz]).max()))
+    assert max(l1s) <= 1e-7
+    ppr_edges = g.info["nnz_regular"] * 30
+    edges += ppr_edges * 128
+    del hb, db, pb
+
+    def batched(k):
+        for _ in range(k):
+            L.bfs_batch(g, bfs_r, out=D)
+            L.sssp_batch(g, sssp_r, out=D)
+            L.ppr_batch(g, ppr_s, 30, out=D)
+
+    def single(k):
+        for _ in range(k):
+            for r in bfs_r:
+                L.bfs(g, r, parents=False, out=D)
+            for r in sssp_r:
+                L.sssp(g, r, out=D)
+            for s_ in ppr_s:
+                L.ppr(g, s_, 30, out=D)
+
+    el_b, _, st_b = timed(ctx, batched, args.steps)
+    el_s, _, st_s = timed(ctx, single, 1)
+    return base_line(args, edges * args.steps / el_b / 1e9, "GTEPS", el_b * 1e3 / args.steps, "u32/f32",
+                     "256 BFS + 256 SSSP + 128 PPR(30 it, FP32+comp), directed R-MAT scale 24 ef 16 seed 4, "
+                     "u32 weights [1,128]",
+                     queries_per_s=round(640 * args.steps / el_b, 2),
+                     single_query={"value": round(edges / el_s / 1e9, 4), "unit": "GTEPS",
+                                   "queries_per_s": round(640 / el_s, 2), "ms": round(el_s * 1e3, 1),
+                                   "roofline": roofline(st_s)},
+                     speedup_vs_single=round(el_s * args.steps / el_b, 2),
+                     ppr_l1_vs_fp64=max(l1s), ppr_max_rel=max(rels),
+                     ppr_residual_last_iter=float(np.max(resid)),
+                     roofline=roofline(st_b), kernels=kernel_table(st_b))
+
+
+def workload_serve(args, L):
+    """Six resident R-MAT graphs (scales 18-22 and 24, seeds 10-15, weights
+    [1,128]); a seeded Poisson stream at fixed QPS mixing 50% BFS, 30% SSSP,
+    20% PPR (30 iterations, FP32); graphs chosen Zipf(1.1) over size rank
+    (smallest most frequent); roots uniform over non-isolated vertices.
+    Served with continuous batching vs one query at a time, same stream."""
+    scales = [18, 19, 20, 21, 22, 24]
+    ctx = L.Context(0)
+    graphs = []
+    for i, sc in enumerate(scales):
+        rec = L.rmat(sc, 16, 0.57, 0.19, 0.19, 10 + i, weighted=True)
+        g = L.Graph.from_records(ctx, rec, 1 << sc, True, L.DIRECTED)
+        nonisol = np.unique(rec[:, :2])
+        outdeg = np.bincount(rec[:, 0], minlength=(1 << sc) + 1)
+        graphs.append((g, nonisol, outdeg))
+        del rec
+    log("six graphs resident: " + ", ".join(f"s{sc}:{g.info['nnz']}" for sc, (g, _, _) in zip(scales, graphs)))
+    rng = np.random.default_rng(2026)
+    nq, qps = args.serve_queries, args.qps
+    zipf = 1.0 / np.arange(1, 7) ** 1.1
+    zipf /= zipf.sum()
+    gi = rng.choice(6, size=nq, p=zipf)
+    kinds = rng.choice(3, size=nq, p=[0.5, 0.3, 0.2])
+    roots = [int(rng.choice(graphs[i][1])) for i in gi]
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, size=nq))
+
+    def run(max_batch, nworkers):
+        srv = L.Server(0, nworkers=nworkers, max_batch=max_batch, state_budget=64 << 30)
+        gids = [srv.add_graph(g) for g, _, _ in graphs]
+        # warm every (graph, program) once
+        for i in range(6):
+            for kd in (L.Q_BFS, L.Q_SSSP, L.Q_PPR):
+                srv.submit(kd, gids[i], int(graphs[i][1][0]), 30 if kd == L.Q_PPR else 0)
+        srv.drain()
+        srv.completed()
+        t0 = srv.clock()
+        for j in range(nq):
+            while srv.clock() - t0 < arrivals[j]:
+                time.sleep(0.0002)
+            srv.submit(int(kinds[j]), gids[gi[j]], roots[j], 30 if kinds[j] == L.Q_PPR else 0)
+        srv.drain()
+        done = srv.completed()
+        srv.close()
+        lat = np.array([d["done_s"] - d["submit_s"] for d in done]) * 1e3
+        span = max(d["done_s"] for d in done) - t0
+        bs = np.array([d["batch_size"] for d in done])
+        return lat, span, bs
+
+    # traversed edges per query (deterministic, computed from single-query results)
+    edges = 0
+    for j in range(nq):
+        g, _, outdeg = graphs[gi[j]]
+        if kinds[j] == L.Q_PPR:
+            edges += g.info["nnz_regular"] * 30
+        elif kinds[j] == L.Q_BFS:
+            edges += traversed_edges(outdeg, L.bfs(g, roots[j], parents=False)[0] != L.BFS_INF)
+        else:
+            edges += traversed_edges(outdeg, L.sssp(g, roots[j])[0] != L.SSSP_INF)
+    lat_b, span_b, bs = run(args.serve_batch, 3)
+    lat_s, span_s, _ = run(1, 1)
+    return base_line(args, edges / span_b / 1e9, "GTEPS", span_b * 1e3, "u32/f32",
+                     f"serving: {nq} queries, Poisson {qps} QPS, 50/30/20 BFS/SSSP/PPR(30 it), "
+                     "six resident R-MAT graphs s18-22,24 seeds 10-15, Zipf(1.1) by size rank",
+                     queries_per_s=round(nq / span_b, 2),
+                     p50_ms=round(float(np.percentile(lat_b, 50)), 3),
+                     p99_ms=round(float(np.percentile(lat_b, 99)), 3),
+                     mean_batch=round(float(bs.mean()), 2), max_batch=int(bs.max()),
+                     single_query={"value": round(edges / span_s / 1e9, 4), "unit": "GTEPS",
+                                   "queries_per_s": round(nq / span_s, 2),
+                                   "p50_ms": round(float(np.percentile(lat_s, 50)), 3),
+                                   "p99_ms": round(float(np.percentile(lat_s, 99)), 3)})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +364,11 @@ def main():
     ap.add_argument("--cpu-roots", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the single-query baseline")
+    ap.add_argument("--workload", default="bfs_sssp_s22",
+                    choices=["bfs_sssp_s22", "pr_s16", "cc_tc_s20", "batched_s24", "serve"])
+    ap.add_argument("--qps", type=float, default=400.0)
+    ap.add_argument("--serve-queries", type=int, default=2000)
+    ap.add_argument("--serve-batch", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the whole graph is one rank's tile set on one "
@@ -170,6 +376,11 @@ def main():
         sys.exit(2)
 
     import la3_amd as L
+    other = {"pr_s16": workload_pr_s16, "cc_tc_s20": workload_cc_tc_s20,
+             "batched_s24": workload_batched_s24, "serve": workload_serve}
+    if args.workload in other:
+        print(json.dumps(other[args.workload](args, L)), flush=True)
+        return
 
     scale, seed = args.scale, 2
     n = 1 << scale

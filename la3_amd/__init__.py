@@ -327,22 +327,22 @@ class Graph:
 
 
 # ------------------------------------------------------------- programs ---
-def degree(g):
-    out = np.zeros(g.nslots, dtype=np.uint32)
+def degree(g, out=None):
+    out = _out(out, (g.nslots,), np.uint32)
     g.ctx.check(lib().la3_degree(g.h, _ptr(out)), "degree")
     return out
 
 
-def pagerank(g, iters):
-    rank = np.zeros(g.nslots, dtype=np.float64)
-    deg = np.zeros(g.nslots, dtype=np.uint32)
+def pagerank(g, iters, out=None):
+    rank = _out(out, (g.nslots,), np.float64)
+    deg = np.zeros(g.nslots, dtype=np.uint32) if out is not DEVICE else None
     it = np.zeros(1, dtype=np.uint32)
     g.ctx.check(lib().la3_pagerank(g.h, int(iters), _ptr(rank), _ptr(deg), _ptr(it)), "pagerank")
     return rank, deg, int(it[0])
 
 
-def ppr(g, seed, iters, alpha=0.15):
-    rank = np.zeros(g.nslots, dtype=np.float64)
+def ppr(g, seed, iters, alpha=0.15, out=None):
+    rank = _out(out, (g.nslots,), np.float64)
     g.ctx.check(lib().la3_ppr(g.h, int(seed), int(iters), float(alpha), _ptr(rank)), "ppr")
     return rank
 
@@ -362,16 +362,16 @@ def sssp(g, root, out=None):
     return dist, int(it[0])
 
 
-def cc(g):
-    label = np.zeros(g.nslots, dtype=np.uint32)
+def cc(g, out=None):
+    label = _out(out, (g.nslots,), np.uint32)
     it = np.zeros(1, dtype=np.uint32)
     g.ctx.check(lib().la3_cc(g.h, _ptr(label), _ptr(it)), "cc")
     return label, int(it[0])
 
 
-def tc(g):
+def tc(g, per_vertex=True):
     total = np.zeros(1, dtype=np.uint64)
-    per = np.zeros(g.nslots, dtype=np.uint32)
+    per = np.zeros(g.nslots, dtype=np.uint32) if per_vertex else None
     g.ctx.check(lib().la3_tc(g.h, _ptr(total), _ptr(per)), "tc")
     return int(total[0]), per
 

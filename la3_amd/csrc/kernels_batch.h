@@ -336,27 +336,44 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items
       act[j] = (cm[(size_t)s * QW + j] >> lane) & 1;
       ds[j] = act[j] ? ld_relaxed(&dist[(size_t)s * K + 64 * j + lane]) : SSSP_INF;
     }
-    for (uint32_t e = beg; e < end; e++) {
-      const uint32_t r = g.col_rows[e], w = g.col_w[e];
-      bool any = false;
+    // Edges in groups of U: the U distance rows are loaded before any is
+    // used, so a wavefront keeps U row loads in flight.
+    constexpr uint32_t U = QW <= 2 ? 4 : 2;
+    for (uint32_t e0 = beg; e0 < end; e0 += U) {
+      const uint32_t n = min(U, end - e0);
+      uint32_t r[U], w[U], cur[U][QW];
 #pragma unroll
-      for (int j = 0; j < QW; j++) {
-        bool imp = false;
-        if (act[j]) {
-          uint32_t* p = &dist[(size_t)r * K + 64 * j + lane];
-          const uint32_t nd = ds[j] + w;
-          if (nd < ld_relaxed(p)) imp = nd < atomicMin(p, nd);
-        }
-        const uint64_t b = __ballot(imp);
-        if (b) {
-          if (lane == 0) atomicOr((unsigned long long*)&nm[(size_t)r * QW + j], (unsigned long long)b);
-          any = true;
-        }
+      for (uint32_t u = 0; u < U; u++) {
+        r[u] = u < n ? g.col_rows[e0 + u] : 0;
+        w[u] = u < n ? g.col_w[e0 + u] : 0;
       }
-      bool first = false;
-      if (any && lane == 0) first = atomicMax(&stamp[r], tag) < tag;
-      first = __shfl(first, 0, WAVE);
-      em_push_one(em, touched, r, first);
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++)
+#pragma unroll
+        for (int j = 0; j < QW; j++)
+          cur[u][j] = (u < n && act[j]) ? ld_relaxed(&dist[(size_t)r[u] * K + 64 * j + lane]) : 0u;
+#pragma unroll
+      for (uint32_t u = 0; u < U; u++) {
+        if (u >= n) break;  // wave-uniform
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < QW; j++) {
+          bool imp = false;
+          if (act[j]) {
+            const uint32_t nd = ds[j] + w[u];
+            if (nd < cur[u][j]) imp = nd < atomicMin(&dist[(size_t)r[u] * K + 64 * j + lane], nd);
+          }
+          const uint64_t b = __ballot(imp);
+          if (b) {
+            if (lane == 0) atomicOr((unsigned long long*)&nm[(size_t)r[u] * QW + j], (unsigned long long)b);
+            any = true;
+          }
+        }
+        bool first = false;
+        if (any && lane == 0) first = atomicMax(&stamp[r[u]], tag) < tag;
+        first = __shfl(first, 0, WAVE);
+        em_push_one(em, touched, r[u], first);
+      }
     }
   }
   em_finish(em, touched);
