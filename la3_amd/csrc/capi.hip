@@ -59,6 +59,7 @@ struct la3_graph {
   int mode = 0;
   Hasher h;
   uint64_t nnz = 0, nnz_reg = 0, nnz_sink = 0, nrecords = 0, device_bytes = 0;
+  uint64_t pnnz_reg = 0, nnz_long_reg = 0;
   std::vector<uint32_t> hid_of, id_of, slot_of_id;
   std::vector<void*> allocs;
   DG dg{};
@@ -196,6 +197,7 @@ la3_status graph_from_layout(la3_ctx* ctx, HostLayout& L, la3_graph** out) {
   g->niso = L.niso; g->ncols = L.ncols; g->nrows = L.nrows;
   g->directed = L.directed; g->weighted = L.weighted; g->mode = L.mode; g->h = L.h;
   g->nnz = L.nnz; g->nnz_reg = L.nnz_reg; g->nnz_sink = L.nnz_sink; g->nrecords = L.nrecords;
+  g->pnnz_reg = L.pnnz_reg; g->nnz_long_reg = L.nnz_long_reg;
   if (ctx->device < 0) {
     g->hid_of = L.hid_of;
     g->id_of = L.id_of;
@@ -660,8 +662,12 @@ static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t
     if (until) launch(c, "fill_u32", k_fill<uint32_t>, 1, act, (uint64_t)1, 0u);
     a.x = xa;
     a.xnext = xb;
-    launch(c, "pr_rows", k_pr_rows<1>, blocks_for(g->nreg), d, a, 0u, g->nreg);
-    if (d.nlong) launch(c, "pr_rows_long", k_pr_long<1>, nlb, d, a, 0u, g->nreg);
+    // Algorithmic bytes: 4 B index + 8 B message per regular-column entry,
+    // 8 B source sum + 16 B rank read/written + 8 B next message per row.
+    const int ev_s = launch(c, "pr_rows", k_pr_rows<1>, blocks_for(g->nreg), d, a, 0u, g->nreg);
+    const int ev_l = d.nlong ? launch(c, "pr_rows_long", k_pr_long<1>, nlb, d, a, 0u, g->nreg) : -1;
+    annotate(c, ev_s, 12.0 * (g->nnz_reg - g->nnz_long_reg) + 32.0 * g->nreg, g->nnz_reg - g->nnz_long_reg);
+    annotate(c, ev_l, 12.0 * g->nnz_long_reg, g->nnz_long_reg);
     std::swap(xa, xb);
     it++;
     if (until) {
@@ -1041,26 +1047,31 @@ static la3_status mppr_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   launch(c, "mppr_init", k_mppr_init<QW>, blocks_for(SK), d, a);
   float* xa = x0;
   float* xb = x1;
+  int ev_rows = -1, ev_segs = -1;
   auto step = [&](uint32_t r0, uint32_t r1, uint32_t s0, uint32_t s1, int loop, double* rr, double* rc) {
     a.x = xa; a.xnext = xb;
     a.resid = rr;
-    const int ev = launch(c, "mppr_rows", k_mppr_rows<QW>, rblocks, d, a, r0, r1, loop);
+    ev_rows = launch(c, "mppr_rows", k_mppr_rows<QW>, rblocks, d, a, r0, r1, loop);
+    ev_segs = -1;
     if (s1 > s0) {
       PprBatchArgs as = a;
       as.seg_row = d.pseg_row + s0; as.seg_beg = d.pseg_beg + s0; as.part = part + (size_t)s0 * 2 * K;
-      launch(c, "mppr_segs", k_mppr_segs<QW>, std::max<uint32_t>(1, std::min<uint32_t>((s1 - s0 + 3) / 4, wave_blocks)),
-             d, as, s1 - s0);
+      ev_segs = launch(c, "mppr_segs", k_mppr_segs<QW>, std::max<uint32_t>(1, std::min<uint32_t>((s1 - s0 + 3) / 4, wave_blocks)),
+                       d, as, s1 - s0);
       a.resid = rc;
       launch(c, "mppr_combine", k_mppr_combine<QW>, cblocks, d, a, r0, r1, loop);
     }
-    return ev;
   };
+  // Algorithmic bytes per iteration: 4 B index + 4k B message row per entry
+  // of a regular row (live query lanes only), 20k B of state and messages
+  // per regular row; entries of rows longer than PSEG are read by
+  // mppr_segs, the rest by mppr_rows.
+  const double kb = 4.0 * k;
+  const uint64_t nnz_short = g->nnz_reg - g->pnnz_reg;
   for (uint32_t it = 0; it < iters; it++) {
-    const int ev = step(0, g->nreg, 0, d.npseg_reg, 1, res_r, res_c);
-    // Algorithmic bytes per iteration: 4 B index + 4K B message row per
-    // entry of a regular row, 8K B state read + 12K B state/message written
-    // per regular row.
-    annotate(c, ev, (4.0 + 4.0 * K) * g->nnz_reg + 20.0 * K * g->nreg, g->nnz_reg);
+    step(0, g->nreg, 0, d.npseg_reg, 1, res_r, res_c);
+    annotate(c, ev_rows, (4.0 + kb) * nnz_short + 5.0 * kb * g->nreg, nnz_short);
+    annotate(c, ev_segs, (4.0 + kb) * g->pnnz_reg + 2.0 * kb * d.npseg_reg, g->pnnz_reg);
     c->edges_scanned += g->nnz_reg;
     std::swap(xa, xb);
   }

@@ -321,11 +321,14 @@ std::string build_layout(const uint32_t* rec, uint64_t nrec, uint32_t n, bool we
   for (uint32_t r = 0; r < NR; r++) L.row_split[r] = cur[r];
   for (uint32_t c = 0; c < nreg; c++) fill_col(c);
   for (uint32_t r = 0; r < NR; r++)
-    if (L.row_ptr[r + 1] - L.row_ptr[r] > LONG_ROW) L.long_rows.push_back(r);
+    if (L.row_ptr[r + 1] - L.row_ptr[r] > LONG_ROW) {
+      L.long_rows.push_back(r);
+      if (r < nreg) L.nnz_long_reg += L.row_ptr[r + 1] - L.row_ptr[r];
+    }
   for (uint32_t r = 0; r < NR; r++) {
     const uint32_t b = L.row_ptr[r], e = L.row_ptr[r + 1];
     if (e - b <= PSEG_ENTRIES) continue;
-    if (r < nreg) L.nplong_reg++;
+    if (r < nreg) { L.nplong_reg++; L.pnnz_reg += e - b; }
     L.plong.push_back(r);
     L.pseg_first.push_back((uint32_t)L.pseg_row.size());
     for (uint32_t j = b; j < e; j += PSEG_ENTRIES) {

@@ -58,6 +58,8 @@ struct HostLayout {
   // first segment of each row, and per segment its row and first entry.
   std::vector<uint32_t> plong, pseg_first, pseg_row, pseg_beg;
   uint32_t nplong_reg = 0, npseg_reg = 0;
+  uint64_t pnnz_reg = 0;       // entries of the regular rows in plong
+  uint64_t nnz_long_reg = 0;   // entries of the regular rows in long_rows
 };
 
 constexpr uint32_t PSEG_ENTRIES = 256;

@@ -338,7 +338,7 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items
     }
     // Edges in groups of U: the U distance rows are loaded before any is
     // used, so a wavefront keeps U row loads in flight.
-    constexpr uint32_t U = QW <= 2 ? 4 : 2;
+    constexpr uint32_t U = QW == 1 ? 8 : (QW <= 2 ? 4 : 2);
     for (uint32_t e0 = beg; e0 < end; e0 += U) {
       const uint32_t n = min(U, end - e0);
       uint32_t r[U], w[U], cur[U][QW];
@@ -422,21 +422,39 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_sinks(DG g, uint32_t* dist) {
   }
 }
 
-// out[q * nslots + id] = v[slot(id) * K + q] for q < k.
+// out[q * nslots + id] = v[slot(id) * K + q] for q < k, through a 64 x 64
+// LDS tile: rows are read whole (lane = query, coalesced) and written per
+// query as runs of 64 consecutive ids (lane = id, coalesced).
 template <class T>
-__global__ void k_batch_output(DG g, const T* v, uint32_t K, uint32_t k, T* out) {
-  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < g.nslots; id += gridDim.x * blockDim.x) {
-    const T* row = v + (size_t)g.slot_of_id[id] * K;
-    for (uint32_t q = 0; q < k; q++) out[(size_t)q * g.nslots + id] = row[q];
+__device__ __forceinline__ void k_batch_output_body(const DG& g, const T* v, uint32_t K, uint32_t k, T* out) {
+  __shared__ T tile[64][65];
+  const uint32_t lane = lane_id(), wv = threadIdx.x / WAVE;
+  const uint32_t ntile = (g.nslots + 63) / 64;
+  for (uint32_t t = blockIdx.x; t < ntile; t += gridDim.x) {
+    const uint32_t id0 = t * 64;
+    for (uint32_t q0 = 0; q0 < k; q0 += 64) {
+      for (uint32_t i = wv; i < 64; i += BLOCK / WAVE) {
+        const uint32_t id = id0 + i;
+        if (id < g.nslots && q0 + lane < k) tile[i][lane] = v[(size_t)g.slot_of_id[id] * K + q0 + lane];
+      }
+      __syncthreads();
+      for (uint32_t qi = wv; qi < 64 && q0 + qi < k; qi += BLOCK / WAVE) {
+        const uint32_t id = id0 + lane;
+        if (id < g.nslots) out[(size_t)(q0 + qi) * g.nslots + id] = tile[lane][qi];
+      }
+      __syncthreads();
+    }
   }
 }
 
-// hops[q * nslots + id] = lev[slot(id) * K + q] for q < k.
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_batch_output(DG g, const T* v, uint32_t K, uint32_t k, T* out) {
+  k_batch_output_body(g, v, K, k, out);
+}
+
+// hops[q * nslots + id] = lev[slot(id) * K + q] for q < k (same tiling).
 __global__ void k_msbfs_output(DG g, const uint8_t* lev, uint32_t K, uint32_t k, uint8_t* hops) {
-  for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < g.nslots; id += gridDim.x * blockDim.x) {
-    const uint8_t* row = lev + (size_t)g.slot_of_id[id] * K;
-    for (uint32_t q = 0; q < k; q++) hops[(size_t)q * g.nslots + id] = row[q];
-  }
+  k_batch_output_body(g, lev, K, k, hops);
 }
 
 }  // namespace la3
@@ -510,8 +528,9 @@ __device__ __forceinline__ void ppr_finish(const PprBatchArgs& a, uint32_t r, co
 #pragma unroll
   for (int w = 0; w < QW; w++) {
     const uint32_t q = 64 * w + lane;
+    if (q >= a.k) continue;
     const size_t o = (size_t)r * K + q;
-    const bool seed = q < a.k && a.seeds[q] == r;
+    const bool seed = a.seeds[q] == r;
     float nh, nl;
     ppr_apply(seed ? a.alpha_hi : 0.f, seed ? a.alpha_lo : 0.f, a.damp_hi, a.damp_lo, s[w], c[w], nh, nl);
     const double d = fabs(((double)nh + nl) - ((double)a.hi[o] + a.lo[o]));
@@ -538,14 +557,46 @@ __device__ __forceinline__ void ppr_resid_flush(const PprBatchArgs& a, double* r
   }
 }
 
+// Kahan sums of the messages of entries [b, e), lane = query. Lanes of
+// queries beyond k load nothing (their vector-load lanes are off), so a
+// small batch reads only the cache lines of its live queries.
+template <int QW>
+__device__ __forceinline__ void ppr_row_sum(const DG& g, const PprBatchArgs& a, uint32_t b, uint32_t e,
+                                            float* s, float* c) {
+  constexpr uint32_t K = 64 * QW;
+  const uint32_t lane = lane_id();
+  bool live[QW];
+#pragma unroll
+  for (int w = 0; w < QW; w++) live[w] = 64 * w + lane < a.k;
+  uint32_t j = b;
+  for (; j + 4 <= e; j += 4) {
+    const uint32_t s0 = g.row_cols[j], s1 = g.row_cols[j + 1], s2 = g.row_cols[j + 2],
+                   s3 = g.row_cols[j + 3];
+#pragma unroll
+    for (int w = 0; w < QW; w++) {
+      if (!live[w]) continue;
+      const float v0 = a.x[(size_t)s0 * K + 64 * w + lane], v1 = a.x[(size_t)s1 * K + 64 * w + lane];
+      const float v2 = a.x[(size_t)s2 * K + 64 * w + lane], v3 = a.x[(size_t)s3 * K + 64 * w + lane];
+      kahan_add(s[w], c[w], v0);
+      kahan_add(s[w], c[w], v1);
+      kahan_add(s[w], c[w], v2);
+      kahan_add(s[w], c[w], v3);
+    }
+  }
+  for (; j < e; j++) {
+    const uint32_t s0 = g.row_cols[j];
+#pragma unroll
+    for (int w = 0; w < QW; w++)
+      if (live[w]) kahan_add(s[w], c[w], a.x[(size_t)s0 * K + 64 * w + lane]);
+  }
+}
+
 // Rows [r0, r1) with at most PSEG entries: one wavefront per row.
 template <int QW>
 __global__ void __launch_bounds__(BLOCK) k_mppr_rows(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1,
                                                      int loop) {
-  constexpr uint32_t K = 64 * QW;
   const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
   const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
-  const uint32_t lane = lane_id();
   double res[QW];
 #pragma unroll
   for (int w = 0; w < QW; w++) res[w] = 0.0;
@@ -555,25 +606,7 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_rows(DG g, PprBatchArgs a, uint3
     float s[QW], c[QW];
 #pragma unroll
     for (int w = 0; w < QW; w++) { s[w] = 0.f; c[w] = 0.f; }
-    uint32_t j = b;
-    for (; j + 4 <= e; j += 4) {
-      const uint32_t s0 = g.row_cols[j], s1 = g.row_cols[j + 1], s2 = g.row_cols[j + 2],
-                     s3 = g.row_cols[j + 3];
-#pragma unroll
-      for (int w = 0; w < QW; w++) {
-        const float v0 = a.x[(size_t)s0 * K + 64 * w + lane], v1 = a.x[(size_t)s1 * K + 64 * w + lane];
-        const float v2 = a.x[(size_t)s2 * K + 64 * w + lane], v3 = a.x[(size_t)s3 * K + 64 * w + lane];
-        kahan_add(s[w], c[w], v0);
-        kahan_add(s[w], c[w], v1);
-        kahan_add(s[w], c[w], v2);
-        kahan_add(s[w], c[w], v3);
-      }
-    }
-    for (; j < e; j++) {
-      const uint32_t s0 = g.row_cols[j];
-#pragma unroll
-      for (int w = 0; w < QW; w++) kahan_add(s[w], c[w], a.x[(size_t)s0 * K + 64 * w + lane]);
-    }
+    ppr_row_sum<QW>(g, a, b, e, s, c);
     ppr_finish<QW>(a, r, s, c, loop != 0, res);
   }
   ppr_resid_flush<QW>(a, res);
@@ -592,25 +625,7 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_segs(DG g, PprBatchArgs a, uint3
     float s[QW], c[QW];
 #pragma unroll
     for (int w = 0; w < QW; w++) { s[w] = 0.f; c[w] = 0.f; }
-    uint32_t j = b;
-    for (; j + 4 <= e; j += 4) {
-      const uint32_t s0 = g.row_cols[j], s1 = g.row_cols[j + 1], s2 = g.row_cols[j + 2],
-                     s3 = g.row_cols[j + 3];
-#pragma unroll
-      for (int w = 0; w < QW; w++) {
-        const float v0 = a.x[(size_t)s0 * K + 64 * w + lane], v1 = a.x[(size_t)s1 * K + 64 * w + lane];
-        const float v2 = a.x[(size_t)s2 * K + 64 * w + lane], v3 = a.x[(size_t)s3 * K + 64 * w + lane];
-        kahan_add(s[w], c[w], v0);
-        kahan_add(s[w], c[w], v1);
-        kahan_add(s[w], c[w], v2);
-        kahan_add(s[w], c[w], v3);
-      }
-    }
-    for (; j < e; j++) {
-      const uint32_t s0 = g.row_cols[j];
-#pragma unroll
-      for (int w = 0; w < QW; w++) kahan_add(s[w], c[w], a.x[(size_t)s0 * K + 64 * w + lane]);
-    }
+    ppr_row_sum<QW>(g, a, b, e, s, c);
 #pragma unroll
     for (int w = 0; w < QW; w++) {
       a.part[(size_t)sg * 2 * K + 64 * w + lane] = s[w];
@@ -639,6 +654,7 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_combine(DG g, PprBatchArgs a, ui
     for (uint32_t sg = a.seg_first[i]; sg < a.seg_first[i + 1]; sg++) {
 #pragma unroll
       for (int w = 0; w < QW; w++) {
+        if (64 * w + lane >= a.k) continue;
         kahan_add(s[w], c[w], a.part[(size_t)sg * 2 * K + 64 * w + lane]);
         kahan_add(s[w], c[w], -a.part[(size_t)sg * 2 * K + K + 64 * w + lane]);
       }
