@@ -227,6 +227,8 @@ def workload_batched_s24(args, L):
     g = L.Graph.from_records(ctx, rec, n, True, L.DIRECTED)
     log(f"scale 24 loaded: {g.info}")
     bfs_r, sssp_r, ppr_s = choose_roots(rec, 256, 41), choose_roots(rec, 256, 42), choose_roots(rec, 128, 43)
+    tune = g.autotune(sssp_r)
+    log(f"autotune: {tune}")
     D = L.DEVICE
     hb, _ = L.bfs_batch(g, bfs_r)
     db, _ = L.sssp_batch(g, sssp_r)
@@ -278,7 +280,8 @@ def workload_batched_s24(args, L):
                      speedup_vs_single=round(el_s * args.steps / el_b, 2),
                      ppr_l1_vs_fp64=max(l1s), ppr_max_rel=max(rels),
                      ppr_residual_last_iter=float(np.max(resid)),
-                     roofline=roofline(st_b), kernels=kernel_table(st_b))
+                     roofline=roofline(st_b), kernels=kernel_table(st_b),
+                     autotune={"sssp_delta": tune["sssp_delta"], "batch_sssp_delta": tune["batch_sssp_delta"]})
 
 
 def workload_serve(args, L):
@@ -395,6 +398,9 @@ def main():
     t = time.time()
     g = L.Graph.from_records(ctx, rec, n, True, L.DIRECTED)
     log(f"ingest + upload {time.time() - t:.1f}s: {g.info}")
+    t = time.time()
+    tune = g.autotune(roots)
+    log(f"autotune ({time.time() - t:.1f}s, not timed): {tune}")
     S = g.nslots
     hb = L.pinned((k, S), np.uint8)
     db = L.pinned((k, S), np.uint32)
@@ -479,6 +485,10 @@ def main():
         "speedup_vs_single": round(gteps_b / single["value"], 2) if single else None,
         "roofline": roofline(st_b),
         "device_copy_gbs": round(copy_gbs, 1),
+        "autotune": {"sssp_delta": tune["sssp_delta"], "batch_sssp_delta": tune["batch_sssp_delta"],
+                     "candidates": tune["candidates"],
+                     "single_ms": [round(x, 3) for x in tune["single_ms"]],
+                     "batch_ms": [round(x, 3) for x in tune["batch_ms"]]},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(rec, n, roots[:args.cpu_roots], outdeg, hops, dist)

@@ -188,6 +188,28 @@ la3_status la3_ppr_batch_ctx(la3_ctx* ctx, la3_graph* g, const uint32_t* seeds, 
                              uint32_t iters, double alpha, float* rank, float* rank_comp,
                              double* residual);
 
+/* ------------------------------------------------------------- tuning --- */
+/* SSSP relaxes in near/far rounds: only (vertex, query) pairs whose distance
+ * is below a threshold T are relaxed; the others stay in the frontier, and
+ * T rises to (smallest frontier distance + delta) when nothing below it
+ * remains. delta = 0 relaxes everything every round (Bellman-Ford). The
+ * distances are exact for every delta; delta only changes the work. */
+typedef enum { LA3_TUNE_SSSP_DELTA = 0, LA3_TUNE_BATCH_SSSP_DELTA = 1 } la3_tunable;
+la3_status la3_graph_set_tuning(la3_graph* g, la3_tunable what, uint32_t value);
+la3_status la3_graph_get_tuning(const la3_graph* g, la3_tunable what, uint32_t* value);
+/* Autotuner: times single-query and batched SSSP from `roots` (a batch of
+ * min(nroots, 1024) and up to 4 single queries) for each candidate delta and
+ * keeps the fastest of each on the graph. */
+typedef struct {
+  uint32_t ncandidates;
+  uint32_t candidates[8];
+  double single_ms[8];   /* mean single-query time per candidate */
+  double batch_ms[8];    /* batch time per candidate */
+  uint32_t sssp_delta, batch_sssp_delta, batch_k;
+} la3_autotune_report;
+la3_status la3_autotune(la3_graph* g, const uint32_t* roots, uint32_t nroots,
+                        la3_autotune_report* report);
+
 /* ------------------------------------------------------------ serving --- */
 /* A request loop over resident graphs. Queries (BFS, SSSP, PPR) are
  * submitted one by one; `nworkers` host threads, each with its own context

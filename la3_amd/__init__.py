@@ -35,8 +35,9 @@ EXPORTS = [
     "la3_bfs_batch_ctx", "la3_sssp_batch_ctx", "la3_ppr_batch_ctx",
     "la3_server_create", "la3_server_add_graph", "la3_server_query_bytes", "la3_server_submit",
     "la3_server_drain", "la3_server_completed", "la3_server_result", "la3_server_clock",
-    "la3_server_destroy",
+    "la3_server_destroy", "la3_graph_set_tuning", "la3_graph_get_tuning", "la3_autotune",
 ]
+TUNE_SSSP_DELTA, TUNE_BATCH_SSSP_DELTA = 0, 1
 Q_BFS, Q_SSSP, Q_PPR = 0, 1, 2
 
 
@@ -70,6 +71,13 @@ class QueryInfo(ctypes.Structure):
                 ("root", ctypes.c_uint32), ("batch_size", ctypes.c_uint32),
                 ("status", ctypes.c_int32), ("submit_s", ctypes.c_double),
                 ("start_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+
+
+class AutotuneReport(ctypes.Structure):
+    _fields_ = [("ncandidates", ctypes.c_uint32), ("candidates", ctypes.c_uint32 * 8),
+                ("single_ms", ctypes.c_double * 8), ("batch_ms", ctypes.c_double * 8),
+                ("sssp_delta", ctypes.c_uint32), ("batch_sssp_delta", ctypes.c_uint32),
+                ("batch_k", ctypes.c_uint32)]
 
 
 class Timing(ctypes.Structure):
@@ -133,6 +141,9 @@ def lib():
             "la3_server_result": (i32, [P, u64, P, u64]),
             "la3_server_clock": (ctypes.c_double, [P]),
             "la3_server_destroy": (None, [P]),
+            "la3_graph_set_tuning": (i32, [P, i32, u32]),
+            "la3_graph_get_tuning": (i32, [P, i32, ctypes.POINTER(u32)]),
+            "la3_autotune": (i32, [P, P, u32, ctypes.POINTER(AutotuneReport)]),
             "la3_rmat_generate": (i32, [u32, u32, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, u64, i32, P, ctypes.POINTER(u64)]),
         }
@@ -312,6 +323,24 @@ class Graph:
         out = np.zeros(self.nslots, dtype=np.uint32)
         self.ctx.check(lib().la3_graph_slots(self.h, _ptr(out)), "graph_slots")
         return out
+
+    def set_tuning(self, what, value):
+        self.ctx.check(lib().la3_graph_set_tuning(self.h, int(what), int(value)), "set_tuning")
+
+    def get_tuning(self, what):
+        v = ctypes.c_uint32()
+        self.ctx.check(lib().la3_graph_get_tuning(self.h, int(what), ctypes.byref(v)), "get_tuning")
+        return v.value
+
+    def autotune(self, roots):
+        """Picks the SSSP near/far delta (single and batched) by timing."""
+        r = np.ascontiguousarray(roots, dtype=np.uint32)
+        rep = AutotuneReport()
+        self.ctx.check(lib().la3_autotune(self.h, _ptr(r), r.size, ctypes.byref(rep)), "autotune")
+        n = rep.ncandidates
+        return dict(candidates=list(rep.candidates[:n]), single_ms=list(rep.single_ms[:n]),
+                    batch_ms=list(rep.batch_ms[:n]), sssp_delta=rep.sssp_delta,
+                    batch_sssp_delta=rep.batch_sssp_delta, batch_k=rep.batch_k)
 
     def csc(self, which):
         nnz = self.info["nnz_sink"] if which else self.info["nnz_regular"]

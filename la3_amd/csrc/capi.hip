@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -60,6 +61,9 @@ struct la3_graph {
   Hasher h;
   uint64_t nnz = 0, nnz_reg = 0, nnz_sink = 0, nrecords = 0, device_bytes = 0;
   uint64_t pnnz_reg = 0, nnz_long_reg = 0;
+  // Near/far thresholds of the SSSP relaxation (0 = plain Bellman-Ford),
+  // single-query and batched; set by la3_graph_set_tuning or la3_autotune.
+  uint32_t sssp_delta = 0, mssp_delta = 0;
   std::vector<uint32_t> hid_of, id_of, slot_of_id;
   std::vector<void*> allocs;
   DG dg{};
@@ -281,6 +285,7 @@ la3_status get_fronts(la3_ctx* c, la3_graph* g, Fronts& F, const char* tag) {
 struct FrontCounts {
   uint32_t items = 0, verts = 0, cols = 0;
   unsigned long long edges = 0, scanned = 0;
+  uint32_t nmin = 0xffffffffu;  // smallest distance put in the next frontier
 };
 
 la3_status read_counts(la3_ctx* c, const Frontier& f, FrontCounts& out) {
@@ -290,12 +295,15 @@ la3_status read_counts(la3_ctx* c, const Frontier& f, FrontCounts& out) {
   out = FrontCounts();
   std::memcpy(&out.items, h, 4);
   const CtrSlot* sl = (const CtrSlot*)(h + 64);
+  uint32_t mx = 0;
   for (uint32_t i = 0; i < NSLOT; i++) {
     out.verts += sl[i].verts;
     out.cols += sl[i].cols;
     out.edges += sl[i].edges;
     out.scanned += sl[i].scanned;
+    mx = std::max(mx, sl[i].nmin_inv);
   }
+  out.nmin = ~mx;
   return LA3_OK;
 }
 
@@ -304,6 +312,14 @@ void zero_counts(la3_ctx* c, const Frontier& f) {
 }
 
 }  // namespace
+
+// Next near/far threshold: unchanged while the next frontier holds pairs
+// below it, else the frontier's smallest distance + delta.
+uint32_t next_threshold(uint32_t T, uint32_t delta, uint32_t nmin) {
+  if (delta == 0) return 0xffffffffu;
+  if (nmin < T) return T;
+  return (uint32_t)std::min<uint64_t>(0xffffffffull, (uint64_t)nmin + delta);
+}
 
 // Hands a device result to the caller: copies it into `dst` (host memory)
 // when dst != NULL, and in every case records it as the context's last
@@ -602,16 +618,19 @@ la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters
   if ((st = read_counts(c, F.f[0], fc))) return st;
   uint32_t round = 0, cur = 0;
   const uint32_t wave_blocks = 4 * c->ncu * 8;
+  const uint32_t delta = g->sssp_delta;
+  uint32_t T = delta ? delta : 0xffffffffu;
   while (fc.items > 0) {
     const Frontier& nf = F.f[cur ^ 1];
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
-    const int ev = launch(c, "sssp_push", k_sssp_push, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dv, stamp, round, nf, F.f[cur].nitems);
+    const int ev = launch(c, "sssp_push", k_sssp_push, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dv, stamp, round, T, nf, F.f[cur].nitems);
     c->edges_scanned += fc.edges;
     FrontCounts nc;
     if ((st = read_counts(c, nf, nc))) return st;
     // Algorithmic bytes (w = m = a = s_r = s_w = 4 B): (4 + w) per entry,
     // 8 B pointers + m per active column, (a + s_r + s_w) + m per activated row.
     annotate(c, ev, 8.0 * fc.edges + 12.0 * fc.cols + 16.0 * nc.verts, fc.edges);
+    T = next_threshold(T, delta, nc.nmin);
     fc = nc;
     round++;
     cur ^= 1;
@@ -937,11 +956,13 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   if ((st = read_counts(c, F.f[0], fc))) return st;
   uint32_t round = 0, cur = 0;
   const uint32_t wave_blocks = 4 * c->ncu * 8;
+  const uint32_t delta = g->mssp_delta;
+  uint32_t thr = delta ? delta : 0xffffffffu;
   while (fc.items > 0) {
     const Frontier& nf = F.f[cur ^ 1];
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
     const int ev = launch(c, "mssp_push", k_mssp_push<QW>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items,
-                          dist, (const uint64_t*)cm, nm, stamp, round + 2, T, F.f[cur].nitems);
+                          dist, (const uint64_t*)cm, nm, stamp, round + 2, thr, T, F.f[cur].nitems);
     FrontCounts tc;
     if ((st = read_counts(c, T, tc))) return st;
     // The touched list becomes the next frontier's work items; clearing T's
@@ -959,6 +980,7 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
     // K/8 B of masks per touched row.
     annotate(c, ev, (8.0 + 4.0 * K) * fc.edges + (8.0 + 4.0 * K) * fc.cols + (K / 8.0 + 4) * tc.items, fc.edges);
     c->edges_scanned += fc.edges;
+    thr = next_threshold(thr, delta, tc.nmin);
     round++;
     fc = nc;
     cur ^= 1;
@@ -1134,6 +1156,62 @@ la3_status la3_ppr_batch_ctx(la3_ctx* c, la3_graph* g, const uint32_t* seeds, ui
   else st = mppr_run<16>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
   timing_end(c);
   return st;
+}
+
+la3_status la3_graph_set_tuning(la3_graph* g, la3_tunable what, uint32_t value) {
+  if (!g) return LA3_EINVAL;
+  if (what == LA3_TUNE_SSSP_DELTA) g->sssp_delta = value;
+  else if (what == LA3_TUNE_BATCH_SSSP_DELTA) g->mssp_delta = value;
+  else return fail(g->ctx, LA3_EINVAL, "unknown tunable");
+  return LA3_OK;
+}
+
+la3_status la3_graph_get_tuning(const la3_graph* g, la3_tunable what, uint32_t* value) {
+  if (!g || !value) return LA3_EINVAL;
+  if (what == LA3_TUNE_SSSP_DELTA) *value = g->sssp_delta;
+  else if (what == LA3_TUNE_BATCH_SSSP_DELTA) *value = g->mssp_delta;
+  else return LA3_EINVAL;
+  return LA3_OK;
+}
+
+la3_status la3_autotune(la3_graph* g, const uint32_t* roots, uint32_t nroots, la3_autotune_report* rep) {
+  if (!g || !roots || nroots == 0) return LA3_EINVAL;
+  la3_ctx* c = g->ctx;
+  if (!g->weighted || !g->directed) return fail(c, LA3_EINVAL, "autotune: SSSP needs a weighted directed load");
+  static const uint32_t cand[] = {0, 32, 64, 128, 256, 512, 1024};
+  const uint32_t nc = sizeof(cand) / sizeof(cand[0]);
+  const uint32_t nsingle = std::min<uint32_t>(nroots, 4);
+  const uint32_t kb = std::min<uint32_t>(nroots, 1024);
+  auto clock = [] { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+  double best_s = 1e30, best_b = 1e30;
+  uint32_t pick_s = 0, pick_b = 0;
+  for (uint32_t i = 0; i < nc; i++) {
+    g->sssp_delta = cand[i];
+    la3_status st = la3_sssp(g, roots[0], nullptr, nullptr);  // warm
+    if (st) return st;
+    double t0 = clock();
+    for (uint32_t r = 0; r < nsingle; r++)
+      if ((st = la3_sssp(g, roots[r], nullptr, nullptr))) return st;
+    const double ts = clock() - t0;
+    if (rep) rep->single_ms[i] = ts * 1e3 / nsingle;
+    if (ts < best_s) { best_s = ts; pick_s = cand[i]; }
+    g->mssp_delta = cand[i];
+    t0 = clock();
+    if ((st = la3_sssp_batch(g, roots, kb, nullptr, nullptr))) return st;
+    const double tb = clock() - t0;
+    if (rep) rep->batch_ms[i] = tb * 1e3;
+    if (tb < best_b) { best_b = tb; pick_b = cand[i]; }
+  }
+  g->sssp_delta = pick_s;
+  g->mssp_delta = pick_b;
+  if (rep) {
+    rep->ncandidates = nc;
+    for (uint32_t i = 0; i < nc; i++) rep->candidates[i] = cand[i];
+    rep->sssp_delta = pick_s;
+    rep->batch_sssp_delta = pick_b;
+    rep->batch_k = kb;
+  }
+  return LA3_OK;
 }
 
 la3_status la3_copy_bandwidth(la3_ctx* c, uint64_t bytes, uint32_t reps, double* gbs) {

@@ -312,11 +312,14 @@ __global__ void k_mssp_roots(DG g, const uint32_t* root_slots, uint32_t k, uint3
   }
 }
 
+// Near/far split as in k_sssp_push, per (vertex, query) pair with one
+// threshold T for the batch: far pairs keep their bit for the next step and
+// their vertex is re-added once (chunk 0) without scanning the column.
 template <int QW>
 __global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items, uint32_t nitems,
                                                      uint32_t* dist, const uint64_t* cm, uint64_t* nm,
-                                                     uint32_t* stamp, uint32_t tag, Frontier touched,
-                                                     uint32_t* zc) {
+                                                     uint32_t* stamp, uint32_t tag, uint32_t T,
+                                                     Frontier touched, uint32_t* zc) {
   zero_ctr(zc);
   constexpr uint32_t K = 64 * QW;
   const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
@@ -331,11 +334,30 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items
     const uint32_t end = min(beg + CHUNK, g.col_split[c]);
     uint32_t ds[QW];
     bool act[QW];
+    bool any_near = false, any_far = false;
 #pragma unroll
     for (int j = 0; j < QW; j++) {
-      act[j] = (cm[(size_t)s * QW + j] >> lane) & 1;
-      ds[j] = act[j] ? ld_relaxed(&dist[(size_t)s * K + 64 * j + lane]) : SSSP_INF;
+      const bool a0 = (cm[(size_t)s * QW + j] >> lane) & 1;
+      ds[j] = a0 ? ld_relaxed(&dist[(size_t)s * K + 64 * j + lane]) : SSSP_INF;
+      const bool far = a0 && ds[j] >= T;
+      act[j] = a0 && !far;
+      any_near |= act[j];
+      if (far && ch == 0) {
+        any_far = true;
+        em.nmin_inv = max(em.nmin_inv, ~ds[j]);
+      }
+      const uint64_t fb = __ballot(far);
+      if (fb && ch == 0 && lane == 0) atomicOr((unsigned long long*)&nm[(size_t)s * QW + j], (unsigned long long)fb);
     }
+    {
+      // far pairs of this vertex: re-add it once for the next step
+      bool first = false;
+      const bool f_any = __ballot(any_far) != 0;
+      if (f_any && lane == 0) first = atomicMax(&stamp[s], tag) < tag;
+      first = __shfl(first, 0, WAVE);
+      em_push_one(em, touched, s, first);
+    }
+    if (__ballot(any_near) == 0) continue;  // nothing near: no column scan
     // Edges in groups of U: the U distance rows are loaded before any is
     // used, so a wavefront keeps U row loads in flight.
     constexpr uint32_t U = QW == 1 ? 8 : (QW <= 2 ? 4 : 2);
@@ -362,6 +384,7 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items
           if (act[j]) {
             const uint32_t nd = ds[j] + w[u];
             if (nd < cur[u][j]) imp = nd < atomicMin(&dist[(size_t)r[u] * K + 64 * j + lane], nd);
+            if (imp) em.nmin_inv = max(em.nmin_inv, ~nd);
           }
           const uint64_t b = __ballot(imp);
           if (b) {

@@ -76,7 +76,8 @@ struct CtrSlot {
   uint32_t verts, cols;          // vertices emitted / with a non-empty column
   unsigned long long edges;      // entries behind the emitted items
   unsigned long long scanned;    // row entries read by pull steps
-  unsigned long long pad;
+  uint32_t nmin_inv;             // ~(min distance in the next frontier), via atomicMax
+  uint32_t pad;
 };
 struct Frontier {
   uint64_t* items;               // work items: (chunk << 32) | slot
@@ -90,6 +91,7 @@ struct Emitter {
   uint32_t cnt;
   uint32_t verts, cols;
   unsigned long long edges, scanned;
+  uint32_t nmin_inv;  // ~(smallest distance this lane put in the next frontier)
 };
 
 __device__ __forceinline__ void em_init(Emitter& em, uint64_t* lds) {
@@ -97,6 +99,7 @@ __device__ __forceinline__ void em_init(Emitter& em, uint64_t* lds) {
   em.cnt = 0;
   em.verts = em.cols = 0;
   em.edges = em.scanned = 0;
+  em.nmin_inv = 0;
 }
 
 __device__ __forceinline__ void em_flush(Emitter& em, const Frontier& f) {
@@ -136,23 +139,28 @@ __device__ __forceinline__ void em_push(Emitter& em, const Frontier& f, uint32_t
 // called once by every thread of the block (contains a block barrier).
 __device__ __forceinline__ void em_finish(Emitter& em, const Frontier& f) {
   em_flush(em, f);
-  __shared__ unsigned long long red[4][BLOCK / WAVE];
+  __shared__ unsigned long long red[5][BLOCK / WAVE];
   const uint32_t w = threadIdx.x / WAVE;
   const unsigned long long v0 = wave_sum((unsigned long long)em.verts);
   const unsigned long long v1 = wave_sum((unsigned long long)em.cols);
   const unsigned long long v2 = wave_sum(em.edges);
   const unsigned long long v3 = wave_sum(em.scanned);
-  if (lane_id() == 0) { red[0][w] = v0; red[1][w] = v1; red[2][w] = v2; red[3][w] = v3; }
+  uint32_t mx = em.nmin_inv;
+#pragma unroll
+  for (int d = WAVE / 2; d > 0; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, WAVE));
+  if (lane_id() == 0) { red[0][w] = v0; red[1][w] = v1; red[2][w] = v2; red[3][w] = v3; red[4][w] = mx; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long t[4] = {0, 0, 0, 0};
+    unsigned long long t[5] = {0, 0, 0, 0, 0};
     for (int i = 0; i < 4; i++)
       for (uint32_t j = 0; j < BLOCK / WAVE; j++) t[i] += red[i][j];
+    for (uint32_t j = 0; j < BLOCK / WAVE; j++) t[4] = max(t[4], red[4][j]);
     CtrSlot& cs = f.slots[blockIdx.x % NSLOT];
     if (t[0]) atomicAdd(&cs.verts, (uint32_t)t[0]);
     if (t[1]) atomicAdd(&cs.cols, (uint32_t)t[1]);
     if (t[2]) atomicAdd(&cs.edges, t[2]);
     if (t[3]) atomicAdd(&cs.scanned, t[3]);
+    if (t[4]) atomicMax(&cs.nmin_inv, (uint32_t)t[4]);
   }
 }
 
@@ -410,9 +418,14 @@ __global__ void k_sssp_init(DG g, uint32_t* dist, uint32_t* stamp, uint32_t root
   }
 }
 
+// Near/far split: an item whose vertex distance is >= T is not relaxed but
+// put back in the next frontier (chunk 0 re-adds the whole column once);
+// the host raises T when nothing below it remains. T = ~0 is plain
+// Bellman-Ford. Every improvement is still relaxed eventually, so the fixed
+// point — the exact distances — is unchanged.
 __global__ void __launch_bounds__(BLOCK) k_sssp_push(DG g, const uint64_t* items, uint32_t nitems,
                                                      uint32_t* dist, uint32_t* stamp, uint32_t round,
-                                                     Frontier nf, uint32_t* zc) {
+                                                     uint32_t T, Frontier nf, uint32_t* zc) {
   zero_ctr(zc);
   const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
   const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
@@ -422,9 +435,21 @@ __global__ void __launch_bounds__(BLOCK) k_sssp_push(DG g, const uint64_t* items
     const uint64_t item = items[it];
     const uint32_t s = (uint32_t)item, ch = (uint32_t)(item >> 32);
     const uint32_t c = col_of_slot(g, s);
+    const uint32_t du = ld_relaxed(&dist[s]);
+    if (du >= T) {  // wave-uniform: far, keep it for a later round
+      bool again = false;
+      uint32_t nit = 0, ne = 0;
+      if (ch == 0 && lane_id() == 0 && atomicMax(&stamp[s], tag) < tag) {
+        again = true;
+        ne = g.col_split[c] - g.col_ptr[c];
+        nit = (ne + CHUNK - 1) / CHUNK;
+        em.nmin_inv = max(em.nmin_inv, ~du);
+      }
+      em_push(em, nf, s, nit, ne, again);
+      continue;
+    }
     const uint32_t beg = g.col_ptr[c] + ch * CHUNK;
     const uint32_t end = min(beg + CHUNK, g.col_split[c]);
-    const uint32_t du = ld_relaxed(&dist[s]);
     for (uint32_t e0 = beg; e0 < end; e0 += WAVE) {
       const uint32_t e = e0 + lane_id();
       bool push = false;
@@ -434,7 +459,10 @@ __global__ void __launch_bounds__(BLOCK) k_sssp_push(DG g, const uint64_t* items
         const uint32_t nd = du + g.col_w[e];
         if (nd < ld_relaxed(&dist[r])) {
           uint32_t old = atomicMin(&dist[r], nd);
-          if (nd < old && atomicMax(&stamp[r], tag) < tag) push = true;
+          if (nd < old) {
+            em.nmin_inv = max(em.nmin_inv, ~nd);
+            if (atomicMax(&stamp[r], tag) < tag) push = true;
+          }
         }
       }
       uint32_t nit = 0, ne = 0;

@@ -300,3 +300,32 @@ def test_ppr_batch_matches_fp64_single(gpu_ctx):
     for q, s in enumerate(seeds):
         r64 = L.ppr(g, s, 20)
         assert float(np.abs(rank[q] - r64).sum()) <= PPR_L1_TOL
+
+
+@pytest.mark.parametrize("delta", [0, 1, 32, 128, 1000])
+def test_sssp_near_far_exact_for_any_delta(gpu_ctx, oracle, delta):
+    rec = _rmat(14, 71, weighted=True)
+    n = 1 << 14
+    g, o = _both(gpu_ctx, oracle, rec, n, True, L.DIRECTED)
+    g.set_tuning(L.TUNE_SSSP_DELTA, delta)
+    g.set_tuning(L.TUNE_BATCH_SSSP_DELTA, delta)
+    assert g.get_tuning(L.TUNE_SSSP_DELTA) == delta
+    roots = _roots(rec, 70, 71)
+    db, _ = L.sssp_batch(g, roots)
+    for q in (0, 1, 35, 69):
+        e, _ = oracle.sssp(o, roots[q])
+        assert (L.sssp(g, roots[q])[0] == e).all()
+        assert (db[q] == e).all()
+
+
+def test_autotune_keeps_results_exact(gpu_ctx, oracle):
+    rec = _rmat(13, 72, weighted=True)
+    n = 1 << 13
+    g, o = _both(gpu_ctx, oracle, rec, n, True, L.DIRECTED)
+    roots = _roots(rec, 64, 72)
+    rep = g.autotune(roots)
+    assert rep["sssp_delta"] in rep["candidates"] and rep["batch_sssp_delta"] in rep["candidates"]
+    assert g.get_tuning(L.TUNE_SSSP_DELTA) == rep["sssp_delta"]
+    db, _ = L.sssp_batch(g, roots)
+    for q in (0, 63):
+        assert (db[q] == oracle.sssp(o, roots[q])[0]).all()
