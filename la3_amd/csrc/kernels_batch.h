@@ -382,9 +382,16 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items
         for (int j = 0; j < QW; j++) {
           bool imp = false;
           if (act[j]) {
+            // The pre-loaded value decides "improved"; the atomic's result is
+            // not waited for. A stale value can only over-report (the pair
+            // is re-activated and relaxes with its current, smaller
+            // distance); a real improvement is never missed.
             const uint32_t nd = ds[j] + w[u];
-            if (nd < cur[u][j]) imp = nd < atomicMin(&dist[(size_t)r[u] * K + 64 * j + lane], nd);
-            if (imp) em.nmin_inv = max(em.nmin_inv, ~nd);
+            if (nd < cur[u][j]) {
+              atomicMin(&dist[(size_t)r[u] * K + 64 * j + lane], nd);
+              imp = true;
+              em.nmin_inv = max(em.nmin_inv, ~nd);
+            }
           }
           const uint64_t b = __ballot(imp);
           if (b) {

@@ -458,11 +458,11 @@ __global__ void __launch_bounds__(BLOCK) k_sssp_push(DG g, const uint64_t* items
         r = g.col_rows[e];
         const uint32_t nd = du + g.col_w[e];
         if (nd < ld_relaxed(&dist[r])) {
-          uint32_t old = atomicMin(&dist[r], nd);
-          if (nd < old) {
-            em.nmin_inv = max(em.nmin_inv, ~nd);
-            if (atomicMax(&stamp[r], tag) < tag) push = true;
-          }
+          // Fire-and-forget: the comparison with the loaded value marks the
+          // improvement (a stale load only re-activates r harmlessly).
+          atomicMin(&dist[r], nd);
+          em.nmin_inv = max(em.nmin_inv, ~nd);
+          if (atomicMax(&stamp[r], tag) < tag) push = true;
         }
       }
       uint32_t nit = 0, ne = 0;
