@@ -935,7 +935,9 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   la3_status st = get_fronts(c, g, F, "bfs");
   if (st) return st;
   Frontier T;
-  if ((st = get_queue(c, "mssp.touched", (uint64_t)S + 1, T))) return st;
+  // One entry per improving scanned entry at most (rows repeat until
+  // k_mssp_items deduplicates them).
+  if ((st = get_queue(c, "mssp.touched", (uint64_t)S + g->nnz + 1, T))) return st;
   if (!dist || !cm || !nm || !stamp || !rs || !out)
     return fail(c, LA3_ENOMEM, "batched SSSP: device allocation failed");
   HIPC(c, hipMemcpyAsync(rs, root_slots.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream));
@@ -962,13 +964,13 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
     const Frontier& nf = F.f[cur ^ 1];
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
     const int ev = launch(c, "mssp_push", k_mssp_push<QW>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items,
-                          dist, (const uint64_t*)cm, nm, stamp, round + 2, thr, T, F.f[cur].nitems);
+                          dist, (const uint64_t*)cm, nm, thr, T, F.f[cur].nitems);
     FrontCounts tc;
     if ((st = read_counts(c, T, tc))) return st;
     // The touched list becomes the next frontier's work items; clearing T's
     // counters happens inside that launch.
     launch(c, "mssp_items", k_mssp_items, blocks_for(std::max<uint32_t>(tc.items, 1)), d, (const uint64_t*)T.items,
-           tc.items, nf, T.nitems);
+           tc.items, stamp, round + 2, nf, T.nitems);
     // cm of the finished step is cleared; nm becomes the next step's cm.
     launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for((uint64_t)S * QW), (unsigned long long*)cm,
            (uint64_t)S * QW, 0ull);
@@ -978,7 +980,7 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
     // Algorithmic bytes: (4 + w) per entry plus K x 4 B of distances read per
     // entry (the k-wide message row), K x 4 B per active column, and
     // K/8 B of masks per touched row.
-    annotate(c, ev, (8.0 + 4.0 * K) * fc.edges + (8.0 + 4.0 * K) * fc.cols + (K / 8.0 + 4) * tc.items, fc.edges);
+    annotate(c, ev, (8.0 + 4.0 * K) * fc.edges + (8.0 + 4.0 * K) * fc.cols + (K / 8.0 + 4) * nc.verts, fc.edges);
     c->edges_scanned += fc.edges;
     thr = next_threshold(thr, delta, tc.nmin);
     round++;

@@ -314,12 +314,12 @@ __global__ void k_mssp_roots(DG g, const uint32_t* root_slots, uint32_t k, uint3
 
 // Near/far split as in k_sssp_push, per (vertex, query) pair with one
 // threshold T for the batch: far pairs keep their bit for the next step and
-// their vertex is re-added once (chunk 0) without scanning the column.
+// their vertex is re-added (chunk 0) without scanning the column. Improved
+// rows are queued without deduplication, so no relaxation waits on an atomic.
 template <int QW>
 __global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items, uint32_t nitems,
                                                      uint32_t* dist, const uint64_t* cm, uint64_t* nm,
-                                                     uint32_t* stamp, uint32_t tag, uint32_t T,
-                                                     Frontier touched, uint32_t* zc) {
+                                                     uint32_t T, Frontier touched, uint32_t* zc) {
   zero_ctr(zc);
   constexpr uint32_t K = 64 * QW;
   const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
@@ -349,14 +349,8 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items
       const uint64_t fb = __ballot(far);
       if (fb && ch == 0 && lane == 0) atomicOr((unsigned long long*)&nm[(size_t)s * QW + j], (unsigned long long)fb);
     }
-    {
-      // far pairs of this vertex: re-add it once for the next step
-      bool first = false;
-      const bool f_any = __ballot(any_far) != 0;
-      if (f_any && lane == 0) first = atomicMax(&stamp[s], tag) < tag;
-      first = __shfl(first, 0, WAVE);
-      em_push_one(em, touched, s, first);
-    }
+    // far pairs of this vertex: queue it again (k_mssp_items drops repeats)
+    em_push_one(em, touched, s, __ballot(any_far) != 0);
     if (__ballot(any_near) == 0) continue;  // nothing near: no column scan
     // Edges in groups of U: the U distance rows are loaded before any is
     // used, so a wavefront keeps U row loads in flight.
@@ -399,31 +393,36 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items
             any = true;
           }
         }
-        bool first = false;
-        if (any && lane == 0) first = atomicMax(&stamp[r[u]], tag) < tag;
-        first = __shfl(first, 0, WAVE);
-        em_push_one(em, touched, r[u], first);
+        // queued on every improvement; k_mssp_items keeps the first per row
+        em_push_one(em, touched, r[u], any);
       }
     }
   }
   em_finish(em, touched);
 }
 
-// Turns the touched list into push work items (regular-row part of each column).
+// Turns the touched list (rows may repeat) into push work items for the
+// regular-row part of each column, keeping the first occurrence of a row in
+// this step (stamp).
 __global__ void __launch_bounds__(BLOCK) k_mssp_items(DG g, const uint64_t* touched, uint32_t ntouched,
-                                                      Frontier nf, uint32_t* zc) {
+                                                      uint32_t* stamp, uint32_t tag, Frontier nf,
+                                                      uint32_t* zc) {
   zero_ctr(zc);
   const uint32_t stride = gridDim.x * BLOCK;
   LA3_EMITTER(em);
   for (uint32_t base = blockIdx.x * BLOCK; base < ntouched; base += stride) {
     const uint32_t i = base + threadIdx.x;
     uint32_t r = 0, nit = 0, ne = 0;
+    bool first = false;
     if (i < ntouched) {
       r = (uint32_t)touched[i];
-      ne = g.col_split[r] - g.col_ptr[r];  // touched rows are regular: column index == slot
-      nit = (ne + CHUNK - 1) / CHUNK;
+      first = atomicMax(&stamp[r], tag) < tag;
+      if (first) {
+        ne = g.col_split[r] - g.col_ptr[r];  // touched rows are regular: column index == slot
+        nit = (ne + CHUNK - 1) / CHUNK;
+      }
     }
-    em_push(em, nf, r, nit, ne, i < ntouched);
+    em_push(em, nf, r, nit, ne, first);
   }
   em_finish(em, nf);
 }
