@@ -117,6 +117,17 @@ def run_batched(L, g, roots, hb, db, steps):
     return lat
 
 
+def run_fused(L, ctx, g, roots, out, steps):
+    """One step = the BFS batch and the SSSP batch advancing together in one
+    fused, graph-captured run (la3_batch_run)."""
+    lat = []
+    for _ in range(steps):
+        q = time.perf_counter()
+        L.batch_run(ctx, [(g, L.Q_BFS, roots), (g, L.Q_SSSP, roots)], out=out)
+        lat.append(time.perf_counter() - q)
+    return lat
+
+
 def timed(ctx, fn, steps):
     ctx.set_profiling(True)
     ctx.reset_kernel_stats()
@@ -429,6 +440,12 @@ def main():
     el_b, lat_b, st_b = timed(ctx, lambda s: run_batched(L, g, roots, D, D, s), args.steps)
     gteps_b = q_edges * args.steps / el_b / 1e9
     log(f"batched: {el_b * 1e3 / args.steps:.1f} ms/step, {gteps_b:.2f} GTEPS; kernels {json.dumps(kernel_table(st_b))}")
+    # fused: both batches in one graph-captured run; exact vs the batched path
+    fr = L.batch_run(ctx, [(g, L.Q_BFS, roots), (g, L.Q_SSSP, roots)])
+    assert (fr[0]["out"] == hb).all() and (fr[1]["out"] == db).all(), "fused run differs from batched"
+    el_f, lat_f, st_f = timed(ctx, lambda s: run_fused(L, ctx, g, roots, D, s), args.steps)
+    gteps_f = q_edges * args.steps / el_f / 1e9
+    log(f"fused: {el_f * 1e3 / args.steps:.1f} ms/step, {gteps_f:.2f} GTEPS; kernels {json.dumps(kernel_table(st_f))}")
     # the same with every result copied to page-locked host memory
     el_p, _, _ = timed(ctx, lambda s: run_batched(L, g, roots, hb, db, s), args.steps)
     gteps_p = q_edges * args.steps / el_p / 1e9
@@ -481,6 +498,9 @@ def main():
         "pcie_inclusive": {"value": round(gteps_p, 4), "unit": "GTEPS",
                            "ms_per_step": round(el_p * 1e3 / args.steps, 3),
                            "note": "batched path with every per-query result copied to page-locked host memory"},
+        "fused": {"value": round(gteps_f, 4), "unit": "GTEPS", "ms_per_step": round(el_f * 1e3 / args.steps, 3),
+                  "note": "BFS and SSSP batches as one fused, hipGraph-captured run (la3_batch_run)",
+                  "kernels": kernel_table(st_f)},
         "single_query": single,
         "speedup_vs_single": round(gteps_b / single["value"], 2) if single else None,
         "roofline": roofline(st_b),

@@ -188,6 +188,36 @@ la3_status la3_ppr_batch_ctx(la3_ctx* ctx, la3_graph* g, const uint32_t* seeds, 
                              uint32_t iters, double alpha, float* rank, float* rank_comp,
                              double* residual);
 
+typedef enum { LA3_Q_BFS = 0, LA3_Q_SSSP = 1, LA3_Q_PPR = 2 } la3_query_kind;
+
+/* ------------------------------------------------- fused batch runs --- */
+/* Several batches — BFS, SSSP and PPR, on one or several resident graphs of
+ * the context's device — run as ONE traversal: each step is one fused launch
+ * over the work of every batch (blocks split among the batches in
+ * proportion to their work), one launch for the work that depends on it, and
+ * a one-block control launch that advances each batch's state on the device
+ * (BFS direction, SSSP threshold, buffer flips, termination). The steps are
+ * captured once per context in a hipGraph and replayed; the host reads one
+ * word per replay of 4 steps. Batches of more than 256 queries are split into
+ * tasks of 256 (at most 64 tasks per run). Every query's result is identical
+ * to the single-family batch call (la3_bfs_batch / la3_sssp_batch /
+ * la3_ppr_batch) and so to the query alone. No counterpart in the reference;
+ * it replaces several VertexProgram::execute calls (vertex_program_execute.hpp:
+ * 11-52) issued back to back. */
+typedef struct {
+  la3_graph* graph;
+  la3_query_kind kind;     /* LA3_Q_BFS, LA3_Q_SSSP or LA3_Q_PPR */
+  uint32_t k;              /* queries, 1..1024 */
+  const uint32_t* roots;   /* k root ids (PPR: seed ids) */
+  uint32_t iters;          /* PPR iteration count (> 0); ignored otherwise */
+  void* out;               /* k x nslots by id: u8 hops / u32 distances / f32 ranks;
+                              NULL leaves the result in device memory */
+  float* out_comp;         /* PPR: rank compensation terms, or NULL */
+  double* residual;        /* PPR: FP64 L1 change of the last iteration per query, or NULL */
+  uint32_t iters_run;      /* out: as the single-family batch call reports it */
+} la3_batch;
+la3_status la3_batch_run(la3_ctx* ctx, la3_batch* batches, uint32_t nbatches, double ppr_alpha);
+
 /* ------------------------------------------------------------- tuning --- */
 /* SSSP relaxes in near/far rounds: only (vertex, query) pairs whose distance
  * is below a threshold T are relaxed; the others stay in the frontier, and
@@ -223,7 +253,6 @@ la3_status la3_autotune(la3_graph* g, const uint32_t* roots, uint32_t nroots,
  * batches of different workers overlap on the device. Each query's result
  * is the one the query gets alone (la3_*_batch with k = 1), whatever batch
  * it lands in. No counterpart in the reference. */
-typedef enum { LA3_Q_BFS = 0, LA3_Q_SSSP = 1, LA3_Q_PPR = 2 } la3_query_kind;
 typedef struct la3_server la3_server;
 typedef struct {
   uint32_t nworkers;      /* concurrent batches (host threads, one context each) */

@@ -36,6 +36,7 @@ EXPORTS = [
     "la3_server_create", "la3_server_add_graph", "la3_server_query_bytes", "la3_server_submit",
     "la3_server_drain", "la3_server_completed", "la3_server_result", "la3_server_clock",
     "la3_server_destroy", "la3_graph_set_tuning", "la3_graph_get_tuning", "la3_autotune",
+    "la3_batch_run",
 ]
 TUNE_SSSP_DELTA, TUNE_BATCH_SSSP_DELTA = 0, 1
 Q_BFS, Q_SSSP, Q_PPR = 0, 1, 2
@@ -78,6 +79,13 @@ class AutotuneReport(ctypes.Structure):
                 ("single_ms", ctypes.c_double * 8), ("batch_ms", ctypes.c_double * 8),
                 ("sssp_delta", ctypes.c_uint32), ("batch_sssp_delta", ctypes.c_uint32),
                 ("batch_k", ctypes.c_uint32)]
+
+
+class Batch(ctypes.Structure):
+    _fields_ = [("graph", ctypes.c_void_p), ("kind", ctypes.c_int), ("k", ctypes.c_uint32),
+                ("roots", ctypes.c_void_p), ("iters", ctypes.c_uint32), ("out", ctypes.c_void_p),
+                ("out_comp", ctypes.c_void_p), ("residual", ctypes.c_void_p),
+                ("iters_run", ctypes.c_uint32)]
 
 
 class Timing(ctypes.Structure):
@@ -144,6 +152,7 @@ def lib():
             "la3_graph_set_tuning": (i32, [P, i32, u32]),
             "la3_graph_get_tuning": (i32, [P, i32, ctypes.POINTER(u32)]),
             "la3_autotune": (i32, [P, P, u32, ctypes.POINTER(AutotuneReport)]),
+            "la3_batch_run": (i32, [P, ctypes.POINTER(Batch), u32, ctypes.c_double]),
             "la3_rmat_generate": (i32, [u32, u32, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, u64, i32, P, ctypes.POINTER(u64)]),
         }
@@ -434,6 +443,40 @@ def ppr_batch(g, seeds, iters, alpha=0.15, out=None, comp=False):
     g.ctx.check(lib().la3_ppr_batch(g.h, _ptr(seeds), seeds.size, int(iters), float(alpha),
                                     _ptr(rank), _ptr(cmp), _ptr(res)), "ppr_batch")
     return rank, cmp, res
+
+
+def batch_run(ctx, batches, alpha=0.15, out=None):
+    """Runs several batches as one fused, graph-captured traversal
+    (la3_batch_run). `batches` is a list of (graph, kind, roots[, iters]) with
+    kind Q_BFS / Q_SSSP / Q_PPR. Returns one dict per batch with the result
+    ("out": k x nslots by id; None with out=DEVICE), for PPR also "comp" and
+    "residual", and "iters"."""
+    arr = (Batch * len(batches))()
+    keep = []
+    res = []
+    for i, b in enumerate(batches):
+        g, kind, roots = b[0], b[1], np.ascontiguousarray(b[2], dtype=np.uint32)
+        iters = int(b[3]) if len(b) > 3 else 0
+        dt = {Q_BFS: np.uint8, Q_SSSP: np.uint32, Q_PPR: np.float32}[kind]
+        o = _out(out, (roots.size, g.nslots), dt)
+        r = {"out": o}
+        arr[i].graph = g.h
+        arr[i].kind = kind
+        arr[i].k = roots.size
+        arr[i].roots = roots.ctypes.data
+        arr[i].iters = iters
+        arr[i].out = o.ctypes.data if o is not None else None
+        if kind == Q_PPR:
+            r["comp"] = np.zeros((roots.size, g.nslots), dtype=np.float32) if o is not None else None
+            r["residual"] = np.zeros(roots.size, dtype=np.float64)
+            arr[i].out_comp = r["comp"].ctypes.data if r["comp"] is not None else None
+            arr[i].residual = r["residual"].ctypes.data
+        keep.append(roots)
+        res.append(r)
+    ctx.check(lib().la3_batch_run(ctx.h, arr, len(batches), float(alpha)), "batch_run")
+    for i, r in enumerate(res):
+        r["iters"] = int(arr[i].iters_run)
+    return res
 
 
 class Server:

@@ -20,6 +20,7 @@
 #include "internal.h"
 #include "kernels_dev.h"
 #include "kernels_batch.h"
+#include "kernels_fused.h"
 
 using namespace la3;
 
@@ -47,6 +48,11 @@ struct la3_ctx {
   int refs = 1;              // the caller's reference plus one per live graph
   const void* last_result = nullptr;  // device buffer of the last program's main output
   size_t last_result_bytes = 0;
+  // Fused batch runs: task table and control block (fixed addresses for the
+  // context's life) and the captured step graph that reads them.
+  FTask* fz_tasks = nullptr;
+  uint8_t* fz_ctl = nullptr;
+  hipGraphExec_t fz_exec = nullptr;
 };
 
 namespace {
@@ -82,6 +88,7 @@ void ctx_release(la3_ctx* c) {
     for (auto& kv : c->scratch) if (kv.second.p) (void)hipFree(kv.second.p);
     for (auto& e : c->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
+    if (c->fz_exec) (void)hipGraphExecDestroy(c->fz_exec);
     if (c->host_ctr) (void)hipHostFree(c->host_ctr);
     (void)hipStreamDestroy(c->stream);
   }
@@ -850,15 +857,16 @@ static la3_status msbfs_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t
       FrontCounts tc;
       if ((st = read_counts(c, T, tc))) return st;
       if (tc.items) launch(c, "msbfs_apply", k_msbfs_apply<W>, blocks_for(tc.items), d, (const uint64_t*)T.items,
-                           tc.items, vis, (const uint64_t*)fb, lev, iter, nf);
+                           tc.items, vis, fb, lev, iter, nf);
     } else {
       ev = launch(c, "msbfs_pull", k_msbfs_pull<W>, blocks_for(g->nrows, wave_blocks), d, vis,
                   (const uint64_t*)fa, fb, lev, iter, (const uint64_t*)valid, nf);
       if (d.nlong) launch(c, "msbfs_pull_long", k_msbfs_pull_long<W>, std::min<uint32_t>((d.nlong + 3) / 4, wave_blocks),
                           d, vis, (const uint64_t*)fa, fb, lev, iter, (const uint64_t*)valid, nf);
     }
-    launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for((uint64_t)S * W), (unsigned long long*)fa,
-           (uint64_t)S * W, 0ull);
+    // The consumed frontier words are non-zero only at this step's work-list
+    // vertices: clear those.
+    launch(c, "msbfs_clear", k_clear_items<W>, blocks_for(fc.items), (const uint64_t*)F.f[cur].items, fc.items, fa);
     std::swap(fa, fb);
     FrontCounts nc;
     if ((st = read_counts(c, nf, nc))) return st;
@@ -971,9 +979,9 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
     // counters happens inside that launch.
     launch(c, "mssp_items", k_mssp_items, blocks_for(std::max<uint32_t>(tc.items, 1)), d, (const uint64_t*)T.items,
            tc.items, stamp, round + 2, nf, T.nitems);
-    // cm of the finished step is cleared; nm becomes the next step's cm.
-    launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for((uint64_t)S * QW), (unsigned long long*)cm,
-           (uint64_t)S * QW, 0ull);
+    // cm of the finished step is cleared at the work-list vertices (the only
+    // ones whose masks a push reads); nm becomes the next step's cm.
+    launch(c, "mssp_clear", k_clear_items<QW>, blocks_for(fc.items), (const uint64_t*)F.f[cur].items, fc.items, cm);
     std::swap(cm, nm);
     FrontCounts nc;
     if ((st = read_counts(c, nf, nc))) return st;
@@ -1285,6 +1293,325 @@ la3_status la3_rmat_generate(uint32_t scale, uint32_t edgefactor, double a, doub
     key.clear();
   }
   return LA3_OK;
+}
+
+}  // extern "C"
+
+// ====================================================== fused batch runs ===
+// la3_batch_run: every batch is split into tasks of at most 256 queries;
+// all tasks advance together through k_fused_a / k_fused_b / k_fused_c
+// (kernels_fused.h), FZ_STEPS steps per replay of one captured hipGraph.
+namespace {
+
+constexpr uint32_t FZ_STEPS = 4;
+
+struct FzTask {
+  uint32_t batch, q0, k;
+};
+
+uint32_t fz_ga(const la3_ctx* c) { return 4u * c->ncu * 8; }
+uint32_t fz_gb(const la3_ctx* c) { return 4u * c->ncu * 2; }
+
+std::string fz_name(uint32_t t, const char* what) { return "fz" + std::to_string(t) + "." + what; }
+
+// Allocates and initialises one task's state on the context's stream and
+// fills its FTask entry (the single-family drivers' initialisation).
+la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t, double alpha, FTask& T) {
+  la3_graph* g = b.graph;
+  const DG& d = g->dg;
+  const uint32_t S = g->nslots;
+  const uint32_t k = ft.k, W = (k + 63) / 64 == 3 ? 4 : (k + 63) / 64, K = 64 * W;
+  std::memset(&T, 0, sizeof(T));
+  T.g = d;
+  T.kind = (uint32_t)b.kind;
+  T.width = W;
+  T.k = k;
+  T.iters = b.iters;
+  std::vector<uint32_t> slots(k);
+  for (uint32_t q = 0; q < k; q++) slots[q] = g->slot_of_id[b.roots[ft.q0 + q]];
+  la3_status st;
+  auto buf = [&](const char* what, size_t bytes) { return scratch(c, fz_name(t, what).c_str(), bytes); };
+  const size_t SW = (size_t)S * W, SK = (size_t)S * K;
+  if (b.kind != LA3_Q_PPR) {
+    const bool bfs = b.kind == LA3_Q_BFS;
+    uint32_t* rs = (uint32_t*)buf("roots", sizeof(uint32_t) * k);
+    T.stamp = (uint32_t*)buf("stamp", sizeof(uint32_t) * S);
+    if (!rs || !T.stamp) return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
+    const uint64_t qcap = (uint64_t)S + g->nnz / CHUNK + 2;
+    if ((st = get_queue(c, fz_name(t, "q0"), qcap, T.q[0])) || (st = get_queue(c, fz_name(t, "q1"), qcap, T.q[1])) ||
+        (st = get_queue(c, fz_name(t, "touched"), bfs ? (uint64_t)S + 1 : (uint64_t)S + g->nnz + 1, T.touched)))
+      return st;
+    HIPC(c, hipMemcpyAsync(rs, slots.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream));
+    if (bfs) {
+      T.vis = (uint64_t*)buf("vis", 8 * SW);
+      T.fr[0] = (uint64_t*)buf("fr0", 8 * SW);
+      T.fr[1] = (uint64_t*)buf("fr1", 8 * SW);
+      T.lev = (uint8_t*)buf("lev", SK);
+      uint64_t* valid = (uint64_t*)buf("valid", 8 * W);
+      if (!T.vis || !T.fr[0] || !T.fr[1] || !T.lev || !valid) return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
+      uint64_t hv[FMAX_WIDTH] = {0, 0, 0, 0};
+      for (uint32_t w = 0; w < W; w++) {
+        const int bits = std::max(0, std::min(64, (int)k - 64 * (int)w));
+        hv[w] = bits == 64 ? ~0ull : ((1ull << bits) - 1);
+      }
+      HIPC(c, hipMemcpyAsync(valid, hv, 8 * W, hipMemcpyHostToDevice, c->stream));
+      T.valid = valid;
+      if (W == 1) {
+        launch(c, "msbfs_init", k_msbfs_init<1>, blocks_for(S), d, T.vis, T.fr[0], T.fr[1], (uint64_t*)T.lev, T.stamp);
+        launch(c, "msbfs_roots", k_msbfs_roots<1>, 1, d, (const uint32_t*)rs, k, T.vis, T.fr[0], T.lev);
+      } else if (W == 2) {
+        launch(c, "msbfs_init", k_msbfs_init<2>, blocks_for(S), d, T.vis, T.fr[0], T.fr[1], (uint64_t*)T.lev, T.stamp);
+        launch(c, "msbfs_roots", k_msbfs_roots<2>, 1, d, (const uint32_t*)rs, k, T.vis, T.fr[0], T.lev);
+      } else {
+        launch(c, "msbfs_init", k_msbfs_init<4>, blocks_for(S), d, T.vis, T.fr[0], T.fr[1], (uint64_t*)T.lev, T.stamp);
+        launch(c, "msbfs_roots", k_msbfs_roots<4>, 1, d, (const uint32_t*)rs, k, T.vis, T.fr[0], T.lev);
+      }
+    } else {
+      T.delta = g->mssp_delta;
+      T.dist = (uint32_t*)buf("dist", 4 * SK);
+      T.m[0] = (uint64_t*)buf("m0", 8 * SW);
+      T.m[1] = (uint64_t*)buf("m1", 8 * SW);
+      if (!T.dist || !T.m[0] || !T.m[1]) return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
+      const uint64_t inf2 = ((uint64_t)SSSP_INF << 32) | SSSP_INF;
+      launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for(SK / 2), (unsigned long long*)T.dist,
+             (uint64_t)(SK / 2), (unsigned long long)inf2);
+      launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for(SW), (unsigned long long*)T.m[0], (uint64_t)SW, 0ull);
+      launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for(SW), (unsigned long long*)T.m[1], (uint64_t)SW, 0ull);
+      launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(S), T.stamp, (uint64_t)S, 0u);
+      if (W == 1) launch(c, "mssp_roots", k_mssp_roots<1>, 1, d, (const uint32_t*)rs, k, T.dist, T.m[0]);
+      else if (W == 2) launch(c, "mssp_roots", k_mssp_roots<2>, 1, d, (const uint32_t*)rs, k, T.dist, T.m[0]);
+      else launch(c, "mssp_roots", k_mssp_roots<4>, 1, d, (const uint32_t*)rs, k, T.dist, T.m[0]);
+    }
+    zero_counts(c, T.q[0]);
+    zero_counts(c, T.q[1]);
+    zero_counts(c, T.touched);
+    launch(c, "msbfs_seed", k_msbfs_seed, blocks_for(k), d, (const uint32_t*)rs, k, T.stamp, T.q[0], bfs ? 0 : 1);
+    return LA3_OK;
+  }
+  // PPR
+  const uint32_t ga = fz_ga(c) + FMAX_TASKS, gb = fz_gb(c) + FMAX_TASKS;
+  T.xs[0] = (float*)buf("x0", 4 * SK);
+  T.xs[1] = (float*)buf("x1", 4 * SK);
+  float* hi = (float*)buf("hi", 4 * SK);
+  float* lo = (float*)buf("lo", 4 * SK);
+  uint32_t* deg = (uint32_t*)buf("deg", sizeof(uint32_t) * S);
+  uint32_t* sd = (uint32_t*)buf("seeds", sizeof(uint32_t) * K);
+  float* part = (float*)buf("part", sizeof(float) * 2 * K * ((size_t)d.npseg + 1));
+  T.res_rows = (double*)buf("res_rows", sizeof(double) * ga * K);
+  T.res_comb = (double*)buf("res_comb", sizeof(double) * gb * K);
+  T.resid = (double*)buf("resid", sizeof(double) * K);
+  if (!T.xs[0] || !T.xs[1] || !hi || !lo || !deg || !sd || !part || !T.res_rows || !T.res_comb || !T.resid)
+    return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
+  std::vector<uint32_t> seeds(K, 0xffffffffu);
+  std::copy(slots.begin(), slots.end(), seeds.begin());
+  HIPC(c, hipMemcpyAsync(sd, seeds.data(), sizeof(uint32_t) * K, hipMemcpyHostToDevice, c->stream));
+  PprBatchArgs& a = T.pa;
+  a.x = T.xs[0]; a.xnext = T.xs[1]; a.hi = hi; a.lo = lo; a.deg = deg; a.seeds = sd; a.k = k;
+  a.alpha_hi = (float)alpha;
+  a.alpha_lo = (float)(alpha - (double)a.alpha_hi);
+  const double damp = 1.0 - alpha;
+  a.damp_hi = (float)damp;
+  a.damp_lo = (float)(damp - (double)a.damp_hi);
+  a.part = part; a.seg_row = d.pseg_row; a.seg_beg = d.pseg_beg; a.long_ppr = d.plong;
+  a.nlong_ppr = d.nplong; a.seg_first = d.pseg_first;
+  // Bytes and entries per iteration, as la3_ppr_batch accounts them.
+  const double kb = 4.0 * k;
+  const uint64_t nnz_short = g->nnz_reg - g->pnnz_reg;
+  T.step_bytes = (4.0 + kb) * nnz_short + 5.0 * kb * g->nreg + (4.0 + kb) * g->pnnz_reg + 2.0 * kb * d.npseg_reg;
+  T.step_edges = g->nnz_reg;
+  launch(c, "outdeg", k_outdeg, blocks_for(S), d, deg);
+  if (W == 1) launch(c, "mppr_init", k_mppr_init<1>, blocks_for(SK), d, a);
+  else if (W == 2) launch(c, "mppr_init", k_mppr_init<2>, blocks_for(SK), d, a);
+  else launch(c, "mppr_init", k_mppr_init<4>, blocks_for(SK), d, a);
+  return LA3_OK;
+}
+
+// Post-loop work of one task and delivery of its results.
+template <int QW>
+la3_status fz_finish(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t, const FTask& T, const FCtl& C) {
+  la3_graph* g = b.graph;
+  const DG& d = g->dg;
+  const uint32_t S = g->nslots, K = 64 * QW, k = ft.k;
+  const uint32_t wave_blocks = 4 * c->ncu * 8;
+  const size_t esz = b.kind == LA3_Q_BFS ? 1 : 4;
+  void* out = scratch(c, fz_name(t, "out").c_str(), esz * (size_t)S * k);
+  if (!out) return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
+  void* host = b.out ? (uint8_t*)b.out + esz * (size_t)S * ft.q0 : nullptr;
+  if (b.kind == LA3_Q_BFS) {
+    launch(c, "msbfs_output", k_msbfs_output, blocks_for(S), d, (const uint8_t*)T.lev, K, k, (uint8_t*)out);
+  } else if (b.kind == LA3_Q_SSSP) {
+    if (g->nsink) launch(c, "mssp_sinks", k_mssp_sinks<QW>, std::min<uint32_t>((g->nsink + 3) / 4, wave_blocks), d, T.dist);
+    launch(c, "mssp_output", k_batch_output<uint32_t>, blocks_for(S), d, (const uint32_t*)T.dist, K, k, (uint32_t*)out);
+  } else {
+    // Sink step from the final messages (vertex_program_execute.hpp:131-148).
+    if (g->nsink) {
+      PprBatchArgs a = T.pa;
+      a.x = T.xs[C.cur];
+      a.xnext = T.xs[C.cur ^ 1];
+      a.resid = (double*)scratch(c, fz_name(t, "res_sink").c_str(), sizeof(double) * wave_blocks * K);
+      if (!a.resid) return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
+      const uint32_t rblocks = std::max<uint32_t>(1, std::min<uint32_t>((g->nrows + 3) / 4, wave_blocks));
+      const uint32_t cblocks = std::max<uint32_t>(1, std::min<uint32_t>((d.nplong + 3) / 4, wave_blocks));
+      launch(c, "mppr_rows", k_mppr_rows<QW>, rblocks, d, a, g->nreg, g->nrows, 0);
+      if (d.npseg > d.npseg_reg) {
+        PprBatchArgs as = a;
+        const uint32_t s0 = d.npseg_reg;
+        as.seg_row = d.pseg_row + s0; as.seg_beg = d.pseg_beg + s0; as.part = a.part + (size_t)s0 * 2 * K;
+        launch(c, "mppr_segs", k_mppr_segs<QW>, std::max<uint32_t>(1, std::min<uint32_t>((d.npseg - s0 + 3) / 4, wave_blocks)),
+               d, as, d.npseg - s0);
+        launch(c, "mppr_combine", k_mppr_combine<QW>, cblocks, d, a, g->nreg, g->nrows, 0);
+      }
+    }
+    launch(c, "mppr_output", k_batch_output<float>, blocks_for(S), d, (const float*)T.pa.hi, K, k, (float*)out);
+    if (b.out_comp) {
+      float* oc = (float*)scratch(c, fz_name(t, "outc").c_str(), sizeof(float) * (size_t)S * k);
+      if (!oc) return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
+      launch(c, "mppr_output", k_batch_output<float>, blocks_for(S), d, (const float*)T.pa.lo, K, k, oc);
+      HIPC(c, hipMemcpyAsync(b.out_comp + (size_t)S * ft.q0, oc, sizeof(float) * (size_t)S * k, hipMemcpyDeviceToHost,
+                             c->stream));
+    }
+    if (b.residual)
+      HIPC(c, hipMemcpyAsync(b.residual + ft.q0, T.resid, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
+  }
+  return deliver(c, host, out, esz * (size_t)S * k);
+}
+
+la3_status fz_capture(la3_ctx* c) {
+  if (c->fz_exec) return LA3_OK;
+  const uint32_t ga = fz_ga(c) + FMAX_TASKS, gb = fz_gb(c) + FMAX_TASKS;
+  FHdr* hdr = (FHdr*)c->fz_ctl;
+  FCtl* ctl = (FCtl*)(c->fz_ctl + sizeof(FHdr));
+  hipGraph_t graph = nullptr;
+  HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  for (uint32_t i = 0; i < FZ_STEPS; i++) {
+    hipLaunchKernelGGL(k_fused_a, dim3(ga), dim3(BLOCK), 0, c->stream, (const FTask*)c->fz_tasks, (const FHdr*)hdr,
+                       (const FCtl*)ctl);
+    hipLaunchKernelGGL(k_fused_b, dim3(gb), dim3(BLOCK), 0, c->stream, (const FTask*)c->fz_tasks, (const FHdr*)hdr,
+                       (const FCtl*)ctl);
+    hipLaunchKernelGGL(k_fused_c, dim3(1), dim3(BLOCK), 0, c->stream, (const FTask*)c->fz_tasks, hdr, ctl, 0u);
+  }
+  hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (e != hipSuccess) return fail(c, LA3_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(&c->fz_exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    c->fz_exec = nullptr;
+    return fail(c, LA3_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  return LA3_OK;
+}
+
+la3_status fz_run(la3_ctx* c, la3_batch* batches, uint32_t nb, double alpha) {
+  std::vector<FzTask> tasks;
+  for (uint32_t i = 0; i < nb; i++)
+    for (uint32_t q0 = 0; q0 < batches[i].k; q0 += 64 * FMAX_WIDTH)
+      tasks.push_back({i, q0, std::min<uint32_t>(64 * FMAX_WIDTH, batches[i].k - q0)});
+  if (tasks.size() > FMAX_TASKS) return fail(c, LA3_EINVAL, "fused run: more than 64 tasks of 256 queries");
+  if (!c->fz_tasks) {
+    c->fz_tasks = (FTask*)scratch(c, "fz.tasks", sizeof(FTask) * FMAX_TASKS);
+    c->fz_ctl = (uint8_t*)scratch(c, "fz.ctl", sizeof(FHdr) + sizeof(FCtl) * FMAX_TASKS);
+    if (!c->fz_tasks || !c->fz_ctl) {
+      c->fz_tasks = nullptr;
+      return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
+    }
+  }
+  const uint32_t nt = (uint32_t)tasks.size();
+  std::vector<FTask> ht(nt);
+  la3_status st;
+  for (uint32_t t = 0; t < nt; t++)
+    if ((st = fz_setup(c, batches[tasks[t].batch], tasks[t], t, alpha, ht[t]))) return st;
+  FHdr h{};
+  h.ntasks = nt;
+  h.ga = fz_ga(c);
+  h.gb = fz_gb(c);
+  HIPC(c, hipMemcpyAsync(c->fz_tasks, ht.data(), sizeof(FTask) * nt, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(c->fz_ctl, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+  FHdr* dh = (FHdr*)c->fz_ctl;
+  FCtl* dc = (FCtl*)(c->fz_ctl + sizeof(FHdr));
+  launch(c, "fused_init", k_fused_c, 1, (const FTask*)c->fz_tasks, dh, dc, 1u);
+  if ((st = fz_capture(c))) return st;
+  // Safety net against a run that does not terminate: far more steps than
+  // any traversal here takes (BFS <= nslots + 1; SSSP rounds are bounded by
+  // the near/far phases; PPR = iters).
+  uint64_t max_steps = 4096;
+  for (uint32_t t = 0; t < nt; t++)
+    max_steps = std::max<uint64_t>(max_steps, ht[t].kind == FK_PPR ? (uint64_t)ht[t].iters + 8
+                                                                   : 8ull * ht[t].g.nslots + 4096);
+  int last_ev = -1;
+  for (uint64_t steps = 0;; steps += FZ_STEPS) {
+    HIPC(c, hipMemcpyAsync(c->host_ctr, dh, sizeof(FHdr), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    FHdr now;
+    std::memcpy(&now, c->host_ctr, sizeof(now));
+    if (now.active == 0) break;
+    if (steps >= max_steps) return fail(c, LA3_EDEVICE, "fused run: step limit reached");
+    hipEvent_t a = nullptr, b = nullptr;
+    if (c->profiling) { a = get_event(c); b = get_event(c); (void)hipEventRecord(a, c->stream); }
+    HIPC(c, hipGraphLaunch(c->fz_exec, c->stream));
+    if (c->profiling) {
+      (void)hipEventRecord(b, c->stream);
+      c->evs.push_back({"fused_steps", a, b, 0.0, 0});
+      last_ev = (int)c->evs.size() - 1;
+    }
+  }
+  std::vector<FCtl> hc(nt);
+  HIPC(c, hipMemcpyAsync(hc.data(), dc, sizeof(FCtl) * nt, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  double bytes = 0;
+  uint64_t edges = 0;
+  for (uint32_t i = 0; i < nb; i++) batches[i].iters_run = 0;
+  for (uint32_t t = 0; t < nt; t++) {
+    bytes += hc[t].bytes;
+    edges += hc[t].edges;
+    la3_batch& b = batches[tasks[t].batch];
+    const uint32_t it = b.kind == LA3_Q_BFS ? std::max<uint32_t>(hc[t].iter, 1)
+                        : b.kind == LA3_Q_SSSP ? hc[t].iter + 1 : hc[t].iter;
+    b.iters_run = std::max(b.iters_run, it);
+  }
+  annotate(c, last_ev, bytes, edges);
+  c->edges_scanned += edges;
+  for (uint32_t t = 0; t < nt; t++) {
+    const la3_batch& b = batches[tasks[t].batch];
+    const uint32_t W = ht[t].width;
+    st = W == 1 ? fz_finish<1>(c, b, tasks[t], t, ht[t], hc[t])
+         : W == 2 ? fz_finish<2>(c, b, tasks[t], t, ht[t], hc[t])
+                  : fz_finish<4>(c, b, tasks[t], t, ht[t], hc[t]);
+    if (st) return st;
+  }
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LA3_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+la3_status la3_batch_run(la3_ctx* c, la3_batch* batches, uint32_t nbatches, double ppr_alpha) {
+  if (!c || !batches || nbatches == 0) return LA3_EINVAL;
+  if (c->device < 0) return fail(c, LA3_EDEVICE, "host-only context");
+  bool ppr = false;
+  for (uint32_t i = 0; i < nbatches; i++) {
+    const la3_batch& b = batches[i];
+    la3_graph* g = b.graph;
+    if (!g || !b.roots) return fail(c, LA3_EINVAL, "batch without graph or roots");
+    if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+    if (g->ctx->device != c->device) return fail(c, LA3_EINVAL, "graph lives on another device");
+    if (b.k == 0 || b.k > 1024) return fail(c, LA3_EINVAL, "batch size must be in [1, 1024]");
+    if (b.kind != LA3_Q_BFS && b.kind != LA3_Q_SSSP && b.kind != LA3_Q_PPR)
+      return fail(c, LA3_EINVAL, "unknown query kind");
+    if (b.kind == LA3_Q_SSSP && !g->weighted) return fail(c, LA3_EINVAL, "SSSP needs a weighted graph");
+    if (b.kind != LA3_Q_BFS && !g->directed) return fail(c, LA3_EINVAL, "SSSP and PPR run on a directed load");
+    if (b.kind == LA3_Q_PPR && b.iters == 0) return fail(c, LA3_EINVAL, "PPR needs a fixed iteration count");
+    ppr |= b.kind == LA3_Q_PPR;
+    for (uint32_t q = 0; q < b.k; q++)
+      if (b.roots[q] > g->n) return fail(c, LA3_EINVAL, "root id out of range");
+  }
+  if (ppr && !(ppr_alpha > 0.0 && ppr_alpha < 1.0)) return fail(c, LA3_EINVAL, "alpha must be in (0, 1)");
+  HIPC(c, hipSetDevice(c->device));
+  timing_begin(c);
+  la3_status st = fz_run(c, batches, nbatches, ppr_alpha);
+  timing_end(c);
+  return st;
 }
 
 }  // extern "C"

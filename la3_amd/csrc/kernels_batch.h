@@ -40,7 +40,7 @@ __global__ void k_msbfs_roots(DG g, const uint32_t* root_slots, uint32_t k, uint
     const uint32_t s = root_slots[q];
     const uint64_t bit = 1ull << (q & 63);
     vis[(size_t)s * W + q / 64] |= bit;
-    fa[(size_t)s * W + q / 64] |= bit;
+    if (slot_has_col(g, s)) fa[(size_t)s * W + q / 64] |= bit;  // senders only
     lev[(size_t)s * 64 * W + q] = 0;
   }
 }
@@ -71,14 +71,11 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_seed(DG g, const uint32_t* root
 // bits that r has not yet seen into r's next-frontier words. The first
 // toucher of r in this step (stamp) appends r to the touched list.
 template <int W>
-__global__ void __launch_bounds__(BLOCK) k_msbfs_push(DG g, const uint64_t* items, uint32_t nitems,
-                                                      const uint64_t* vis, const uint64_t* fa,
-                                                      uint64_t* fb, uint32_t* stamp, uint32_t tag,
-                                                      Frontier touched) {
-  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
-  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
-  LA3_EMITTER(em);
-  for (uint32_t it = wid; it < nitems; it += nwaves) {
+__device__ __forceinline__ void msbfs_push_body(const DG& g, Part p, Emitter& em, const uint64_t* items,
+                                                uint32_t nitems, const uint64_t* vis, const uint64_t* fa,
+                                                uint64_t* fb, uint32_t* stamp, uint32_t tag,
+                                                const Frontier& touched) {
+  for (uint32_t it = p.wid(); it < nitems; it += p.nwaves()) {
     const uint64_t item = items[it];
     const uint32_t s = (uint32_t)item, ch = (uint32_t)(item >> 32);
     const uint32_t c = col_of_slot(g, s);
@@ -104,38 +101,50 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_push(DG g, const uint64_t* item
       em_push(em, touched, r, first ? 1u : 0u, 0, first);
     }
   }
+}
+
+template <int W>
+__global__ void __launch_bounds__(BLOCK) k_msbfs_push(DG g, const uint64_t* items, uint32_t nitems,
+                                                      const uint64_t* vis, const uint64_t* fa,
+                                                      uint64_t* fb, uint32_t* stamp, uint32_t tag,
+                                                      Frontier touched) {
+  LA3_EMITTER(em);
+  msbfs_push_body<W>(g, grid_part(), em, items, nitems, vis, fa, fb, stamp, tag, touched);
   em_finish(em, touched);
 }
 
 // Marks the new bits of each touched vertex visited, records their level and
-// emits the vertex's column as push work for the next step.
+// emits the vertex's column as push work for the next step. New bits are the
+// next-frontier bits not yet visited; a sink's next-frontier words are
+// cleared here (no step reads a sink's frontier bits), so after the step the
+// only non-zero frontier words are those of emitted vertices.
 template <int W>
-__global__ void __launch_bounds__(BLOCK) k_msbfs_apply(DG g, const uint64_t* touched,
-                                                       uint32_t ntouched, uint64_t* vis,
-                                                       const uint64_t* fb, uint8_t* lev,
-                                                       uint32_t iter, Frontier nf) {
-  const uint32_t stride = gridDim.x * BLOCK;
+__device__ __forceinline__ void msbfs_apply_body(const DG& g, Part p, Emitter& em, const uint64_t* touched,
+                                                 uint32_t ntouched, uint64_t* vis, uint64_t* fb,
+                                                 uint8_t* lev, uint32_t iter, const Frontier& nf) {
   const uint8_t nxt = (uint8_t)(iter + 1);
-  LA3_EMITTER(em);
-  for (uint32_t base = blockIdx.x * BLOCK; base < ntouched; base += stride) {
+  for (uint32_t base = p.bid * BLOCK; base < ntouched; base += p.nthreads()) {
     const uint32_t i = base + threadIdx.x;
     bool act = false;
     uint32_t r = 0, nit = 0, ne = 0;
     if (i < ntouched) {
       r = (uint32_t)touched[i];
+      const bool has_col = slot_has_col(g, r);
 #pragma unroll
       for (int w = 0; w < W; w++) {
-        uint64_t m = fb[(size_t)r * W + w];
+        const uint64_t v = vis[(size_t)r * W + w];
+        uint64_t m = fb[(size_t)r * W + w] & ~v;
+        if (!has_col) fb[(size_t)r * W + w] = 0;
         if (!m) continue;
         act = true;
-        vis[(size_t)r * W + w] |= m;
+        vis[(size_t)r * W + w] = v | m;
         while (m) {
           const int q = __ffsll((long long)m) - 1;
           lev[(size_t)r * 64 * W + w * 64 + q] = nxt;
           m &= m - 1;
         }
       }
-      if (act && slot_has_col(g, r)) {
+      if (act && has_col) {
         const uint32_t c = col_of_slot(g, r);
         ne = g.col_ptr[c + 1] - g.col_ptr[c];
         nit = (ne + CHUNK - 1) / CHUNK;
@@ -143,30 +152,47 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_apply(DG g, const uint64_t* tou
     }
     em_push(em, nf, r, nit, ne, act);
   }
+}
+
+template <int W>
+__global__ void __launch_bounds__(BLOCK) k_msbfs_apply(DG g, const uint64_t* touched,
+                                                       uint32_t ntouched, uint64_t* vis,
+                                                       uint64_t* fb, uint8_t* lev,
+                                                       uint32_t iter, Frontier nf) {
+  LA3_EMITTER(em);
+  msbfs_apply_body<W>(g, grid_part(), em, touched, ntouched, vis, fb, lev, iter, nf);
   em_finish(em, nf);
 }
 
-// Clears the current frontier words of the vertices that were active.
+// Clears the W words of `bits` of every vertex in the work list (chunk 0
+// items: one per vertex). Used on the frontier words a step has consumed.
 template <int W>
-__global__ void k_msbfs_clear_items(const uint64_t* items, uint32_t nitems, uint64_t* fa) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += gridDim.x * blockDim.x) {
-    const uint32_t s = (uint32_t)items[i];
+__device__ __forceinline__ void clear_items_body(Part p, const uint64_t* items, uint32_t nitems,
+                                                 uint64_t* bits) {
+  for (uint32_t i = p.tid(); i < nitems; i += p.nthreads()) {
+    const uint64_t item = items[i];
+    if ((uint32_t)(item >> 32) != 0) continue;
+    const uint32_t s = (uint32_t)item;
 #pragma unroll
-    for (int w = 0; w < W; w++) fa[(size_t)s * W + w] = 0;
+    for (int w = 0; w < W; w++) bits[(size_t)s * W + w] = 0;
   }
+}
+
+template <int W>
+__global__ void __launch_bounds__(BLOCK) k_clear_items(const uint64_t* items, uint32_t nitems, uint64_t* bits) {
+  clear_items_body<W>(grid_part(), items, nitems, bits);
 }
 
 // Bottom-up step: each row ORs the frontier words of its in-neighbours,
 // restricted to the queries that have not reached it, stopping once all
-// of them are covered. Rows longer than LONG_ROW go to k_msbfs_pull_long.
+// of them are covered. Rows longer than LONG_ROW go to msbfs_pull_long_body.
+// Only rows with a column (senders of the next step) keep frontier words.
 template <int W>
-__global__ void __launch_bounds__(BLOCK) k_msbfs_pull(DG g, uint64_t* vis, const uint64_t* fa,
-                                                      uint64_t* fb, uint8_t* lev, uint32_t iter,
-                                                      const uint64_t* valid, Frontier nf) {
-  const uint32_t stride = gridDim.x * BLOCK;
+__device__ __forceinline__ void msbfs_pull_body(const DG& g, Part p, Emitter& em, uint64_t* vis,
+                                                const uint64_t* fa, uint64_t* fb, uint8_t* lev,
+                                                uint32_t iter, const uint64_t* valid, const Frontier& nf) {
   const uint8_t nxt = (uint8_t)(iter + 1);
-  LA3_EMITTER(em);
-  for (uint32_t base = blockIdx.x * BLOCK; base < g.nrows; base += stride) {
+  for (uint32_t base = p.bid * BLOCK; base < g.nrows; base += p.nthreads()) {
     const uint32_t r = base + threadIdx.x;
     bool act = false;
     uint32_t nit = 0, ne = 0, scanned = 0;
@@ -192,13 +218,14 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_pull(DG g, uint64_t* vis, const
           }
           if (done) break;
         }
+        const bool has_col = slot_has_col(g, r);
 #pragma unroll
         for (int w = 0; w < W; w++) {
           uint64_t m = acc[w];
           if (!m) continue;
           act = true;
           vis[(size_t)r * W + w] |= m;
-          fb[(size_t)r * W + w] = m;
+          if (has_col) fb[(size_t)r * W + w] = m;
           while (m) {
             const int q = __ffsll((long long)m) - 1;
             lev[(size_t)r * 64 * W + w * 64 + q] = nxt;
@@ -215,18 +242,24 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_pull(DG g, uint64_t* vis, const
     em.scanned += scanned;
     em_push(em, nf, r, nit, ne, act);
   }
+}
+
+template <int W>
+__global__ void __launch_bounds__(BLOCK) k_msbfs_pull(DG g, uint64_t* vis, const uint64_t* fa,
+                                                      uint64_t* fb, uint8_t* lev, uint32_t iter,
+                                                      const uint64_t* valid, Frontier nf) {
+  LA3_EMITTER(em);
+  msbfs_pull_body<W>(g, grid_part(), em, vis, fa, fb, lev, iter, valid, nf);
   em_finish(em, nf);
 }
 
 template <int W>
-__global__ void __launch_bounds__(BLOCK) k_msbfs_pull_long(DG g, uint64_t* vis, const uint64_t* fa,
-                                                           uint64_t* fb, uint8_t* lev, uint32_t iter,
-                                                           const uint64_t* valid, Frontier nf) {
-  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
-  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+__device__ __forceinline__ void msbfs_pull_long_body(const DG& g, Part p, Emitter& em, uint64_t* vis,
+                                                     const uint64_t* fa, uint64_t* fb, uint8_t* lev,
+                                                     uint32_t iter, const uint64_t* valid,
+                                                     const Frontier& nf) {
   const uint8_t nxt = (uint8_t)(iter + 1);
-  LA3_EMITTER(em);
-  for (uint32_t i = wid; i < g.nlong; i += nwaves) {
+  for (uint32_t i = p.wid(); i < g.nlong; i += p.nwaves()) {
     const uint32_t r = g.long_rows[i];
     uint64_t need[W], acc[W];
     bool any = false;
@@ -263,20 +296,21 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_pull_long(DG g, uint64_t* vis, 
     uint32_t nit = 0, ne = 0;
     if (lane_id() == 0) {
       em.scanned += scanned;
+      const bool has_col = slot_has_col(g, r);
 #pragma unroll
       for (int w = 0; w < W; w++) {
         uint64_t m = acc[w];
         if (!m) continue;
         act = true;
         vis[(size_t)r * W + w] |= m;
-        fb[(size_t)r * W + w] = m;
+        if (has_col) fb[(size_t)r * W + w] = m;
         while (m) {
           const int q = __ffsll((long long)m) - 1;
           lev[(size_t)r * 64 * W + w * 64 + q] = nxt;
           m &= m - 1;
         }
       }
-      if (act && slot_has_col(g, r)) {
+      if (act && has_col) {
         const uint32_t c = col_of_slot(g, r);
         ne = g.col_ptr[c + 1] - g.col_ptr[c];
         nit = (ne + CHUNK - 1) / CHUNK;
@@ -284,6 +318,14 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_pull_long(DG g, uint64_t* vis, 
     }
     em_push(em, nf, r, nit, ne, act);
   }
+}
+
+template <int W>
+__global__ void __launch_bounds__(BLOCK) k_msbfs_pull_long(DG g, uint64_t* vis, const uint64_t* fa,
+                                                           uint64_t* fb, uint8_t* lev, uint32_t iter,
+                                                           const uint64_t* valid, Frontier nf) {
+  LA3_EMITTER(em);
+  msbfs_pull_long_body<W>(g, grid_part(), em, vis, fa, fb, lev, iter, valid, nf);
   em_finish(em, nf);
 }
 
@@ -317,16 +359,12 @@ __global__ void k_mssp_roots(DG g, const uint32_t* root_slots, uint32_t k, uint3
 // their vertex is re-added (chunk 0) without scanning the column. Improved
 // rows are queued without deduplication, so no relaxation waits on an atomic.
 template <int QW>
-__global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items, uint32_t nitems,
-                                                     uint32_t* dist, const uint64_t* cm, uint64_t* nm,
-                                                     uint32_t T, Frontier touched, uint32_t* zc) {
-  zero_ctr(zc);
+__device__ __forceinline__ void mssp_push_body(const DG& g, Part p, Emitter& em, const uint64_t* items,
+                                               uint32_t nitems, uint32_t* dist, const uint64_t* cm,
+                                               uint64_t* nm, uint32_t T, const Frontier& touched) {
   constexpr uint32_t K = 64 * QW;
-  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
-  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
   const uint32_t lane = lane_id();
-  LA3_EMITTER(em);
-  for (uint32_t it = wid; it < nitems; it += nwaves) {
+  for (uint32_t it = p.wid(); it < nitems; it += p.nwaves()) {
     const uint64_t item = items[it];
     const uint32_t s = (uint32_t)item, ch = (uint32_t)(item >> 32);
     const uint32_t c = col_of_slot(g, s);
@@ -398,19 +436,25 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items
       }
     }
   }
+}
+
+template <int QW>
+__global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items, uint32_t nitems,
+                                                     uint32_t* dist, const uint64_t* cm, uint64_t* nm,
+                                                     uint32_t T, Frontier touched, uint32_t* zc) {
+  zero_ctr(zc);
+  LA3_EMITTER(em);
+  mssp_push_body<QW>(g, grid_part(), em, items, nitems, dist, cm, nm, T, touched);
   em_finish(em, touched);
 }
 
 // Turns the touched list (rows may repeat) into push work items for the
 // regular-row part of each column, keeping the first occurrence of a row in
 // this step (stamp).
-__global__ void __launch_bounds__(BLOCK) k_mssp_items(DG g, const uint64_t* touched, uint32_t ntouched,
-                                                      uint32_t* stamp, uint32_t tag, Frontier nf,
-                                                      uint32_t* zc) {
-  zero_ctr(zc);
-  const uint32_t stride = gridDim.x * BLOCK;
-  LA3_EMITTER(em);
-  for (uint32_t base = blockIdx.x * BLOCK; base < ntouched; base += stride) {
+__device__ __forceinline__ void mssp_items_body(const DG& g, Part p, Emitter& em, const uint64_t* touched,
+                                                uint32_t ntouched, uint32_t* stamp, uint32_t tag,
+                                                const Frontier& nf) {
+  for (uint32_t base = p.bid * BLOCK; base < ntouched; base += p.nthreads()) {
     const uint32_t i = base + threadIdx.x;
     uint32_t r = 0, nit = 0, ne = 0;
     bool first = false;
@@ -424,6 +468,14 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_items(DG g, const uint64_t* touc
     }
     em_push(em, nf, r, nit, ne, first);
   }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_mssp_items(DG g, const uint64_t* touched, uint32_t ntouched,
+                                                      uint32_t* stamp, uint32_t tag, Frontier nf,
+                                                      uint32_t* zc) {
+  zero_ctr(zc);
+  LA3_EMITTER(em);
+  mssp_items_body(g, grid_part(), em, touched, ntouched, stamp, tag, nf);
   em_finish(em, nf);
 }
 
@@ -572,7 +624,7 @@ __device__ __forceinline__ void ppr_finish(const PprBatchArgs& a, uint32_t r, co
 
 // Per-block, per-query reduction of the residual partials (FP64, fixed order).
 template <int QW>
-__device__ __forceinline__ void ppr_resid_flush(const PprBatchArgs& a, double* res) {
+__device__ __forceinline__ void ppr_resid_flush(const PprBatchArgs& a, double* res, uint32_t bid) {
   constexpr uint32_t K = 64 * QW;
   __shared__ double red[BLOCK / WAVE][K];
   const uint32_t w0 = threadIdx.x / WAVE, lane = lane_id();
@@ -582,7 +634,7 @@ __device__ __forceinline__ void ppr_resid_flush(const PprBatchArgs& a, double* r
   for (uint32_t q = threadIdx.x; q < K; q += BLOCK) {
     double t = 0;
     for (uint32_t j = 0; j < BLOCK / WAVE; j++) t += red[j][q];
-    a.resid[(size_t)blockIdx.x * K + q] = t;
+    a.resid[(size_t)bid * K + q] = t;
   }
 }
 
@@ -622,33 +674,36 @@ __device__ __forceinline__ void ppr_row_sum(const DG& g, const PprBatchArgs& a, 
 
 // Rows [r0, r1) with at most PSEG entries: one wavefront per row.
 template <int QW>
-__global__ void __launch_bounds__(BLOCK) k_mppr_rows(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1,
-                                                     int loop) {
-  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
-  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+__device__ __forceinline__ void mppr_rows_body(const DG& g, Part p, const PprBatchArgs& a, uint32_t r0,
+                                               uint32_t r1, bool loop) {
   double res[QW];
 #pragma unroll
   for (int w = 0; w < QW; w++) res[w] = 0.0;
-  for (uint32_t r = r0 + wid; r < r1; r += nwaves) {
+  for (uint32_t r = r0 + p.wid(); r < r1; r += p.nwaves()) {
     const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1];
     if (e - b > PSEG) continue;  // wave-uniform; segmented path
     float s[QW], c[QW];
 #pragma unroll
     for (int w = 0; w < QW; w++) { s[w] = 0.f; c[w] = 0.f; }
     ppr_row_sum<QW>(g, a, b, e, s, c);
-    ppr_finish<QW>(a, r, s, c, loop != 0, res);
+    ppr_finish<QW>(a, r, s, c, loop, res);
   }
-  ppr_resid_flush<QW>(a, res);
+  ppr_resid_flush<QW>(a, res, p.bid);
+}
+
+template <int QW>
+__global__ void __launch_bounds__(BLOCK) k_mppr_rows(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1,
+                                                     int loop) {
+  mppr_rows_body<QW>(g, grid_part(), a, r0, r1, loop != 0);
 }
 
 // One wavefront per PSEG-entry segment of a long row: partial (sum, comp).
+// Segments [0, nseg) of a.seg_row / a.seg_beg / a.part.
 template <int QW>
-__global__ void __launch_bounds__(BLOCK) k_mppr_segs(DG g, PprBatchArgs a, uint32_t nseg) {
+__device__ __forceinline__ void mppr_segs_body(const DG& g, Part p, const PprBatchArgs& a, uint32_t nseg) {
   constexpr uint32_t K = 64 * QW;
-  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
-  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
   const uint32_t lane = lane_id();
-  for (uint32_t sg = wid; sg < nseg; sg += nwaves) {
+  for (uint32_t sg = p.wid(); sg < nseg; sg += p.nwaves()) {
     const uint32_t r = a.seg_row[sg], b = a.seg_beg[sg];
     const uint32_t e = min(b + PSEG, g.row_ptr[r + 1]);
     float s[QW], c[QW];
@@ -663,18 +718,21 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_segs(DG g, PprBatchArgs a, uint3
   }
 }
 
+template <int QW>
+__global__ void __launch_bounds__(BLOCK) k_mppr_segs(DG g, PprBatchArgs a, uint32_t nseg) {
+  mppr_segs_body<QW>(g, grid_part(), a, nseg);
+}
+
 // Combines each long row's segment partials in segment order, then applies.
 template <int QW>
-__global__ void __launch_bounds__(BLOCK) k_mppr_combine(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1,
-                                                        int loop) {
+__device__ __forceinline__ void mppr_combine_body(const DG& g, Part p, const PprBatchArgs& a, uint32_t r0,
+                                                  uint32_t r1, bool loop) {
   constexpr uint32_t K = 64 * QW;
-  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
-  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
   const uint32_t lane = lane_id();
   double res[QW];
 #pragma unroll
   for (int w = 0; w < QW; w++) res[w] = 0.0;
-  for (uint32_t i = wid; i < a.nlong_ppr; i += nwaves) {
+  for (uint32_t i = p.wid(); i < a.nlong_ppr; i += p.nwaves()) {
     const uint32_t r = a.long_ppr[i];
     if (r < r0 || r >= r1) continue;
     float s[QW], c[QW];
@@ -688,9 +746,15 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_combine(DG g, PprBatchArgs a, ui
         kahan_add(s[w], c[w], -a.part[(size_t)sg * 2 * K + K + 64 * w + lane]);
       }
     }
-    ppr_finish<QW>(a, r, s, c, loop != 0, res);
+    ppr_finish<QW>(a, r, s, c, loop, res);
   }
-  ppr_resid_flush<QW>(a, res);
+  ppr_resid_flush<QW>(a, res, p.bid);
+}
+
+template <int QW>
+__global__ void __launch_bounds__(BLOCK) k_mppr_combine(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1,
+                                                        int loop) {
+  mppr_combine_body<QW>(g, grid_part(), a, r0, r1, loop != 0);
 }
 
 // hi = lo = tele; x = message for every vertex with a column (constant for sources).

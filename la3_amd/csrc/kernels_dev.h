@@ -135,9 +135,10 @@ __device__ __forceinline__ void em_push(Emitter& em, const Frontier& f, uint32_t
   }
 }
 
-// Flushes the queue and adds the block's counters into its slot. Must be
-// called once by every thread of the block (contains a block barrier).
-__device__ __forceinline__ void em_finish(Emitter& em, const Frontier& f) {
+// Flushes the queue and adds the block's counters into slot bid % NSLOT
+// (bid = the block's index in its launch or in its share of a fused launch).
+// Must be called once by every thread of the block (contains a block barrier).
+__device__ __forceinline__ void em_finish(Emitter& em, const Frontier& f, uint32_t bid) {
   em_flush(em, f);
   __shared__ unsigned long long red[5][BLOCK / WAVE];
   const uint32_t w = threadIdx.x / WAVE;
@@ -155,7 +156,7 @@ __device__ __forceinline__ void em_finish(Emitter& em, const Frontier& f) {
     for (int i = 0; i < 4; i++)
       for (uint32_t j = 0; j < BLOCK / WAVE; j++) t[i] += red[i][j];
     for (uint32_t j = 0; j < BLOCK / WAVE; j++) t[4] = max(t[4], red[4][j]);
-    CtrSlot& cs = f.slots[blockIdx.x % NSLOT];
+    CtrSlot& cs = f.slots[bid % NSLOT];
     if (t[0]) atomicAdd(&cs.verts, (uint32_t)t[0]);
     if (t[1]) atomicAdd(&cs.cols, (uint32_t)t[1]);
     if (t[2]) atomicAdd(&cs.edges, t[2]);
@@ -163,6 +164,19 @@ __device__ __forceinline__ void em_finish(Emitter& em, const Frontier& f) {
     if (t[4]) atomicMax(&cs.nmin_inv, (uint32_t)t[4]);
   }
 }
+
+__device__ __forceinline__ void em_finish(Emitter& em, const Frontier& f) { em_finish(em, f, blockIdx.x); }
+
+// Blocks of a launch, or of one task's share of a fused launch: block bid of
+// nb, with the waves numbered bid * 4 + wave of nb * 4.
+struct Part {
+  uint32_t bid, nb;
+  __device__ uint32_t wid() const { return bid * (BLOCK / WAVE) + threadIdx.x / WAVE; }
+  __device__ uint32_t nwaves() const { return nb * (BLOCK / WAVE); }
+  __device__ uint32_t tid() const { return bid * BLOCK + threadIdx.x; }
+  __device__ uint32_t nthreads() const { return nb * BLOCK; }
+};
+__device__ __forceinline__ Part grid_part() { return Part{blockIdx.x, gridDim.x}; }
 
 #define LA3_EMITTER(em)                                  \
   __shared__ uint64_t em##_lds[(BLOCK / WAVE) * QCAP];   \

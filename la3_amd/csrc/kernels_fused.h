@@ -1,0 +1,336 @@
+// Fused, device-driven steps for a mix of batched queries: any number of
+// batches (BFS k-bit, SSSP k-wide, PPR FP32+compensation), on one or several
+// resident graphs, advance together, one step per iteration of three
+// launches:
+//   k_fused_a  the traversal / row-sum work of every batch (one launch);
+//   k_fused_b  the work that needs A's results (BFS apply, SSSP work items,
+//              PPR long-row combine) and the clearing of consumed masks;
+//   k_fused_c  one block: reads each batch's counters on the device, picks
+//              the next BFS direction and SSSP threshold, flips buffers,
+//              zeroes counters and splits the next launches' blocks among
+//              the batches in proportion to their work.
+// No kernel takes a per-step host argument, so a run of steps is captured
+// once in a hipGraph and replayed; the host reads one word per replay.
+// Included once, by capi.hip, after kernels_batch.h.
+#pragma once
+
+#include "kernels_batch.h"
+
+namespace la3 {
+
+enum : uint32_t { FK_BFS = 0, FK_SSSP = 1, FK_PPR = 2 };
+constexpr uint32_t FMAX_TASKS = 64;   // batches of <= 256 queries in one run
+constexpr uint32_t FMAX_WIDTH = 4;    // 64-query words per batch
+
+// One batch ("task"): immutable during a run.
+struct FTask {
+  DG g;
+  uint32_t kind, width, k, iters;  // width = ceil(k / 64); iters: PPR iterations
+  uint32_t delta;                  // SSSP near/far delta (0 = Bellman-Ford)
+  uint32_t nslots_pad;
+  uint32_t* stamp;
+  Frontier q[2];                   // work lists, ping-pong by FCtl::cur
+  Frontier touched;
+  // BFS
+  uint64_t* vis;
+  uint64_t* fr[2];                 // frontier words (current = fr[cur])
+  uint8_t* lev;
+  const uint64_t* valid;
+  // SSSP
+  uint32_t* dist;
+  uint64_t* m[2];                  // improved-query masks (current = m[cur])
+  // PPR
+  PprBatchArgs pa;                 // x / xnext / resid are set per phase
+  float* xs[2];                    // messages (current = xs[cur])
+  double* res_rows;                // per A-block partial residuals [block * K]
+  double* res_comb;                // per B-block partial residuals
+  double* resid;                   // [K] FP64 residual of the last iteration
+  double step_bytes;               // PPR: algorithmic bytes per iteration
+  unsigned long long step_edges;   // PPR: entries read per iteration
+};
+
+// Mutable per-task control word block, written by k_fused_c.
+struct FCtl {
+  uint32_t iter, cur, done, bottom_up;
+  uint32_t thr, nitems, a_beg, a_end;
+  uint32_t b_beg, b_end, fcols, pad;
+  unsigned long long fedges;   // entries behind the current work list
+  unsigned long long edges;    // entries scanned over the run
+  double bytes;                // algorithmic bytes over the run
+};
+
+struct FHdr {
+  uint32_t ntasks, ga, gb, active;  // active: tasks not done
+  uint32_t steps, pad[11];
+};
+
+__device__ __forceinline__ const FTask* find_task(const FTask* tasks, const FHdr* h, const FCtl* ctl,
+                                                  bool phase_a, Part& p, uint32_t& t) {
+  const uint32_t nt = h->ntasks;
+  for (t = 0; t < nt; t++) {
+    const uint32_t b = phase_a ? ctl[t].a_beg : ctl[t].b_beg;
+    const uint32_t e = phase_a ? ctl[t].a_end : ctl[t].b_end;
+    if (blockIdx.x >= b && blockIdx.x < e) {
+      p.bid = blockIdx.x - b;
+      p.nb = e - b;
+      return &tasks[t];
+    }
+  }
+  return nullptr;
+}
+
+template <int W>
+__device__ __forceinline__ void fused_bfs_a(const FTask& T, const FCtl& C, Part p, Emitter& em) {
+  const uint32_t cur = C.cur;
+  if (!C.bottom_up) {
+    msbfs_push_body<W>(T.g, p, em, T.q[cur].items, C.nitems, T.vis, T.fr[cur], T.fr[cur ^ 1], T.stamp,
+                       C.iter + 2, T.touched);
+    em_finish(em, T.touched, p.bid);
+  } else {
+    msbfs_pull_body<W>(T.g, p, em, T.vis, T.fr[cur], T.fr[cur ^ 1], T.lev, C.iter, T.valid, T.q[cur ^ 1]);
+    msbfs_pull_long_body<W>(T.g, p, em, T.vis, T.fr[cur], T.fr[cur ^ 1], T.lev, C.iter, T.valid,
+                            T.q[cur ^ 1]);
+    em_finish(em, T.q[cur ^ 1], p.bid);
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void fused_bfs_b(const FTask& T, const FCtl& C, Part p, Emitter& em) {
+  const uint32_t cur = C.cur;
+  if (!C.bottom_up) {
+    msbfs_apply_body<W>(T.g, p, em, T.touched.items, *T.touched.nitems, T.vis, T.fr[cur ^ 1], T.lev, C.iter,
+                        T.q[cur ^ 1]);
+    em_finish(em, T.q[cur ^ 1], p.bid);
+  }
+  clear_items_body<W>(p, T.q[cur].items, C.nitems, T.fr[cur]);
+}
+
+template <int QW>
+__device__ __forceinline__ void fused_sssp_a(const FTask& T, const FCtl& C, Part p, Emitter& em) {
+  const uint32_t cur = C.cur;
+  mssp_push_body<QW>(T.g, p, em, T.q[cur].items, C.nitems, T.dist, T.m[cur], T.m[cur ^ 1], C.thr, T.touched);
+  em_finish(em, T.touched, p.bid);
+}
+
+template <int QW>
+__device__ __forceinline__ void fused_sssp_b(const FTask& T, const FCtl& C, Part p, Emitter& em) {
+  const uint32_t cur = C.cur;
+  mssp_items_body(T.g, p, em, T.touched.items, *T.touched.nitems, T.stamp, C.iter + 2, T.q[cur ^ 1]);
+  em_finish(em, T.q[cur ^ 1], p.bid);
+  clear_items_body<QW>(p, T.q[cur].items, C.nitems, T.m[cur]);
+}
+
+template <int QW>
+__device__ __forceinline__ void fused_ppr_a(const FTask& T, const FCtl& C, Part p) {
+  PprBatchArgs a = T.pa;
+  a.x = T.xs[C.cur];
+  a.xnext = T.xs[C.cur ^ 1];
+  a.resid = T.res_rows;
+  mppr_rows_body<QW>(T.g, p, a, 0, T.g.nreg, true);
+  mppr_segs_body<QW>(T.g, p, a, T.g.npseg_reg);
+}
+
+template <int QW>
+__device__ __forceinline__ void fused_ppr_b(const FTask& T, const FCtl& C, Part p) {
+  PprBatchArgs a = T.pa;
+  a.x = T.xs[C.cur];
+  a.xnext = T.xs[C.cur ^ 1];
+  a.resid = T.res_comb;
+  mppr_combine_body<QW>(T.g, p, a, 0, T.g.nreg, true);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fused_a(const FTask* __restrict__ tasks, const FHdr* hdr,
+                                                   const FCtl* ctl) {
+  Part p;
+  uint32_t t;
+  const FTask* T = find_task(tasks, hdr, ctl, true, p, t);
+  if (!T) return;  // block-uniform
+  const FCtl& C = ctl[t];
+  LA3_EMITTER(em);
+  switch (T->kind * 8 + T->width) {
+    case FK_BFS * 8 + 1: fused_bfs_a<1>(*T, C, p, em); break;
+    case FK_BFS * 8 + 2: fused_bfs_a<2>(*T, C, p, em); break;
+    case FK_BFS * 8 + 4: fused_bfs_a<4>(*T, C, p, em); break;
+    case FK_SSSP * 8 + 1: fused_sssp_a<1>(*T, C, p, em); break;
+    case FK_SSSP * 8 + 2: fused_sssp_a<2>(*T, C, p, em); break;
+    case FK_SSSP * 8 + 4: fused_sssp_a<4>(*T, C, p, em); break;
+    case FK_PPR * 8 + 1: fused_ppr_a<1>(*T, C, p); break;
+    case FK_PPR * 8 + 2: fused_ppr_a<2>(*T, C, p); break;
+    case FK_PPR * 8 + 4: fused_ppr_a<4>(*T, C, p); break;
+    default: break;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fused_b(const FTask* __restrict__ tasks, const FHdr* hdr,
+                                                   const FCtl* ctl) {
+  Part p;
+  uint32_t t;
+  const FTask* T = find_task(tasks, hdr, ctl, false, p, t);
+  if (!T) return;  // block-uniform
+  const FCtl& C = ctl[t];
+  LA3_EMITTER(em);
+  switch (T->kind * 8 + T->width) {
+    case FK_BFS * 8 + 1: fused_bfs_b<1>(*T, C, p, em); break;
+    case FK_BFS * 8 + 2: fused_bfs_b<2>(*T, C, p, em); break;
+    case FK_BFS * 8 + 4: fused_bfs_b<4>(*T, C, p, em); break;
+    case FK_SSSP * 8 + 1: fused_sssp_b<1>(*T, C, p, em); break;
+    case FK_SSSP * 8 + 2: fused_sssp_b<2>(*T, C, p, em); break;
+    case FK_SSSP * 8 + 4: fused_sssp_b<4>(*T, C, p, em); break;
+    case FK_PPR * 8 + 1: fused_ppr_b<1>(*T, C, p); break;
+    case FK_PPR * 8 + 2: fused_ppr_b<2>(*T, C, p); break;
+    case FK_PPR * 8 + 4: fused_ppr_b<4>(*T, C, p); break;
+    default: break;
+  }
+}
+
+// Sums of one queue's counter slots (one wavefront; NSLOT == WAVE).
+struct FSums {
+  uint32_t verts, cols, nmin;
+  unsigned long long edges, scanned;
+};
+__device__ __forceinline__ FSums fsum(const Frontier& f) {
+  const CtrSlot& s = f.slots[lane_id()];
+  FSums o;
+  o.verts = wave_sum(s.verts);
+  o.cols = wave_sum(s.cols);
+  o.edges = wave_sum(s.edges);
+  o.scanned = wave_sum(s.scanned);
+  uint32_t mx = s.nmin_inv;
+#pragma unroll
+  for (int d = WAVE / 2; d > 0; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, WAVE));
+  o.nmin = ~mx;
+  return o;
+}
+
+// Zeroes a queue's item count and counter slots (one wavefront).
+__device__ __forceinline__ void fzero(const Frontier& f) {
+  uint32_t* w = f.nitems;
+  for (uint32_t i = lane_id(); i < CTR_WORDS; i += WAVE) w[i] = 0;
+}
+
+// Control step (one block of BLOCK threads). init != 0: set up from the
+// seeded work lists; else account for the step just run and advance.
+__global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tasks, FHdr* hdr, FCtl* ctl,
+                                                   uint32_t init) {
+  __shared__ double est[FMAX_TASKS];
+  __shared__ uint32_t useful[FMAX_TASKS];
+  const uint32_t nt = hdr->ntasks;
+  const uint32_t lane = lane_id();
+  for (uint32_t t = threadIdx.x / WAVE; t < nt; t += BLOCK / WAVE) {
+    const FTask& T = tasks[t];
+    FCtl& C = ctl[t];
+    const DG& g = T.g;
+    const uint32_t K = 64 * T.width;
+    if (!init && C.done) {
+      if (lane == 0) { est[t] = 0; useful[t] = 0; }
+      continue;
+    }
+    if (T.kind == FK_PPR) {
+      if (!init) {
+        // FP64 residual of the step, in a fixed order: A-block partials,
+        // then B-block partials.
+        const uint32_t na = C.a_end - C.a_beg, nb = C.b_end - C.b_beg;
+        for (uint32_t q = lane; q < K; q += WAVE) {
+          double r = 0;
+          for (uint32_t b = 0; b < na; b++) r += T.res_rows[(size_t)b * K + q];
+          if (g.npseg_reg)
+            for (uint32_t b = 0; b < nb; b++) r += T.res_comb[(size_t)b * K + q];
+          T.resid[q] = r;
+        }
+        if (lane == 0) {
+          C.bytes += T.step_bytes;
+          C.edges += T.step_edges;
+          C.iter++;
+          C.cur ^= 1;
+        }
+      } else if (lane == 0) {
+        C.iter = 0; C.cur = 0; C.bytes = 0; C.edges = 0;
+      }
+      if (lane == 0) {
+        C.done = C.iter >= T.iters;
+        est[t] = C.done ? 0.0 : (4.0 + 4.0 * T.k) * (double)g.nnz + 20.0 * T.k * (double)g.nreg;
+        useful[t] = C.done ? 0 : (g.nreg + 3) / 4 + (g.npseg_reg + 3) / 4;
+      }
+      continue;
+    }
+    // Traversals: the work list just produced is q[cur ^ 1] (q[0] at init).
+    const uint32_t nxt = init ? 0 : (C.cur ^ 1);
+    const Frontier& nf = T.q[nxt];
+    const FSums ns = fsum(nf);
+    const uint32_t nitems = *nf.nitems;
+    if (!init) {
+      const FSums ts = fsum(T.touched);  // push counters: smallest queued distance
+      if (lane == 0) {
+        const double kb = T.k / 8.0;
+        double b = 0;
+        unsigned long long e = 0;
+        if (T.kind == FK_BFS) {
+          if (!C.bottom_up) {
+            b = (4.0 + kb) * C.fedges + (8.0 + kb) * C.fcols + 3 * kb * ns.verts;
+            e = C.fedges;
+          } else {
+            b = (4.0 + kb) * ns.scanned + (8.0 + kb) * g.nrows + 2 * kb * ns.verts;
+            e = ns.scanned;
+          }
+        } else {
+          b = (8.0 + 4.0 * K) * C.fedges + (8.0 + 4.0 * K) * C.fcols + (K / 8.0 + 4) * ns.verts;
+          e = C.fedges;
+        }
+        C.bytes += b;
+        C.edges += e;
+        C.iter++;
+        if (T.kind == FK_BFS) {
+          if (!C.bottom_up && ns.edges * 20 > g.nnz) C.bottom_up = 1;
+          else if (C.bottom_up && (uint64_t)ns.verts * 24 < g.nslots) C.bottom_up = 0;
+        } else if (T.delta) {
+          // Threshold unchanged while the new work list holds pairs below it.
+          if (ts.nmin >= C.thr) C.thr = (uint32_t)min(0xffffffffull, (unsigned long long)ts.nmin + T.delta);
+        }
+        C.cur = nxt;
+      }
+    } else if (lane == 0) {
+      C.iter = 0; C.cur = 0; C.bottom_up = 0; C.bytes = 0; C.edges = 0;
+      C.thr = T.delta ? T.delta : 0xffffffffu;
+    }
+    // The list just consumed (and the touched list) are written next step.
+    __builtin_amdgcn_wave_barrier();
+    fzero(T.q[nxt ^ 1]);
+    fzero(T.touched);
+    if (lane == 0) {
+      C.nitems = nitems;
+      C.fedges = ns.edges;
+      C.fcols = ns.cols;
+      C.done = nitems == 0;
+      const double K8 = T.kind == FK_BFS ? 4.0 + T.k / 8.0 : 8.0 + 4.0 * K;
+      const bool pull = T.kind == FK_BFS && C.bottom_up;
+      est[t] = C.done ? 0.0 : (pull ? K8 * (double)g.nnz / 4 : K8 * (double)ns.edges + 64.0 * nitems);
+      useful[t] = C.done ? 0 : (pull ? (g.nrows + BLOCK - 1) / BLOCK + (g.nlong + 3) / 4 : (nitems + 3) / 4);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0;
+    uint32_t act = 0;
+    for (uint32_t t = 0; t < nt; t++) {
+      tot += est[t];
+      act += ctl[t].done ? 0 : 1;
+    }
+    const uint32_t ga = hdr->ga, gb = hdr->gb;
+    uint32_t a = 0, b = 0;
+    for (uint32_t t = 0; t < nt; t++) {
+      FCtl& C = ctl[t];
+      uint32_t na = 0, nb = 0;
+      if (!C.done) {
+        na = tot > 0 ? (uint32_t)(ga * (est[t] / tot)) : 1;
+        na = max(1u, min(na, max(1u, useful[t])));
+        nb = max(1u, gb / max(act, 1u));
+      }
+      C.a_beg = a; C.a_end = a + na; a += na;
+      C.b_beg = b; C.b_end = b + nb; b += nb;
+    }
+    hdr->active = act;
+    if (!init && act) hdr->steps++;
+  }
+}
+
+}  // namespace la3

@@ -1,0 +1,113 @@
+"""Fused, graph-captured batch runs (la3_batch_run): batches of BFS, SSSP and
+PPR on several graphs advance as one traversal. Every query's result must be
+the single-family batch call's result: bit-exact for BFS levels and SSSP
+distances, and for PPR ranks and compensation terms (the row sums run in the
+same order whatever block a row lands on); the FP64 residual is a sum of
+per-block partials whose grouping depends on the block split, so it is
+compared to 1e-12 relative. The single-family batch calls themselves are
+pinned to the single-query programs and the oracle in test_gpu_parity.py;
+here a few roots are also checked against the oracle directly."""
+import numpy as np
+import pytest
+
+import la3_amd as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _graph(ctx, scale, seed, weighted=True, mode=L.DIRECTED):
+    rec = L.rmat(scale, 16, seed=seed, weighted=weighted)
+    g = L.Graph.from_records(ctx, rec, 1 << scale, weighted, mode)
+    return g, rec
+
+
+def _roots(rec, k, seed, extra=()):
+    rng = np.random.default_rng(seed)
+    src = np.unique(rec[:, 0])
+    r = rng.choice(src, size=k - len(extra), replace=True).astype(np.uint32)
+    return np.concatenate([np.asarray(extra, dtype=np.uint32), r])
+
+
+@pytest.fixture(scope="module")
+def two_graphs(gpu_ctx):
+    return _graph(gpu_ctx, 12, 71), _graph(gpu_ctx, 11, 72)
+
+
+def _check(batches, res):
+    for (g, kind, roots, *rest), r in zip(batches, res):
+        if kind == L.Q_BFS:
+            ref, it = L.bfs_batch(g, roots)
+            assert np.array_equal(r["out"], ref)
+            assert r["iters"] == it
+        elif kind == L.Q_SSSP:
+            ref, it = L.sssp_batch(g, roots)
+            assert np.array_equal(r["out"], ref)
+            assert r["iters"] == it
+        else:
+            rank, comp, resid = L.ppr_batch(g, roots, rest[0], comp=True)
+            assert np.array_equal(r["out"], rank)
+            assert np.array_equal(r["comp"], comp)
+            np.testing.assert_allclose(r["residual"], resid, rtol=1e-12, atol=0)
+            assert r["iters"] == rest[0]
+
+
+def test_mixed_two_graphs(gpu_ctx, two_graphs):
+    (ga, ra), (gb, rb) = two_graphs
+    # id 0 is isolated (ids are 1-based), id 2^scale is likely a sink or
+    # isolated; repeated roots share a frontier entry
+    sink = ga.nslots - 1
+    batches = [
+        (ga, L.Q_BFS, _roots(ra, 70, 1, extra=(0, 0, sink))),
+        (ga, L.Q_SSSP, _roots(ra, 130, 2, extra=(0, sink, sink))),
+        (gb, L.Q_PPR, _roots(rb, 5, 3), 10),
+        (gb, L.Q_BFS, _roots(rb, 300, 4)),      # two tasks: 256 + 44
+        (gb, L.Q_SSSP, _roots(rb, 1, 5)),
+    ]
+    res = L.batch_run(gpu_ctx, batches)
+    _check(batches, res)
+
+
+def test_repeated_runs_reuse_the_graph(gpu_ctx, two_graphs):
+    (ga, ra), (gb, rb) = two_graphs
+    first = [(ga, L.Q_SSSP, _roots(ra, 64, 11)), (gb, L.Q_PPR, _roots(rb, 64, 12), 7)]
+    second = [(gb, L.Q_BFS, _roots(rb, 9, 13))]
+    for batches in (first, second, first):
+        _check(batches, L.batch_run(gpu_ctx, batches))
+
+
+def test_near_far_threshold_in_fused_run(gpu_ctx, two_graphs):
+    (ga, ra), _ = two_graphs
+    roots = _roots(ra, 100, 21)
+    want, _ = L.sssp_batch(ga, roots)
+    for delta in (0, 16, 200):
+        ga.set_tuning(L.TUNE_BATCH_SSSP_DELTA, delta)
+        res = L.batch_run(gpu_ctx, [(ga, L.Q_SSSP, roots)])
+        assert np.array_equal(res[0]["out"], want)
+    ga.set_tuning(L.TUNE_BATCH_SSSP_DELTA, 0)
+
+
+def test_fused_against_oracle(gpu_ctx, oracle):
+    rec = L.rmat(10, 8, seed=31, weighted=True)
+    n = 1 << 10
+    g = L.Graph.from_records(gpu_ctx, rec, n, True, L.DIRECTED)
+    og = oracle.OracleGraph(rec, n, True, L.DIRECTED)
+    roots = _roots(rec, 6, 32)
+    res = L.batch_run(gpu_ctx, [(g, L.Q_BFS, roots), (g, L.Q_SSSP, roots)])
+    for q, r in enumerate(roots):
+        hops = oracle.bfs(og, int(r))[0]
+        dist = oracle.sssp(og, int(r))[0]
+        assert np.array_equal(res[0]["out"][q], hops)
+        assert np.array_equal(res[1]["out"][q], dist)
+
+
+def test_undirected_bfs_and_errors(gpu_ctx):
+    g, rec = _graph(gpu_ctx, 10, 41, weighted=False, mode=L.UNDIRECTED)
+    roots = _roots(rec, 20, 42)
+    _check([(g, L.Q_BFS, roots)], L.batch_run(gpu_ctx, [(g, L.Q_BFS, roots)]))
+    with pytest.raises(L.LA3Error):
+        L.batch_run(gpu_ctx, [(g, L.Q_SSSP, roots)])     # unweighted, undirected
+    gd, recd = _graph(gpu_ctx, 10, 43)
+    with pytest.raises(L.LA3Error):
+        L.batch_run(gpu_ctx, [(gd, L.Q_PPR, roots, 0)])  # PPR needs iterations
+    with pytest.raises(L.LA3Error):
+        L.batch_run(gpu_ctx, [(gd, L.Q_BFS, np.array([(1 << 10) + 5], dtype=np.uint32))])
