@@ -260,6 +260,12 @@ typedef struct {
   uint64_t state_budget;  /* bytes of per-query state admitted at once */
   int keep_results;       /* copy each result to host and keep it for la3_server_result */
   double ppr_alpha;       /* teleport probability of PPR queries */
+  int fused;              /* 0: a batch is one (kind, graph, iters) group run by
+                             la3_*_batch_ctx; 1: a batch takes pending queries of
+                             every kind and graph in arrival order, grouped by
+                             (kind, graph, iters), each group up to max_batch, and
+                             runs them as one la3_batch_run (one fused launch per
+                             step across graphs and programs) */
 } la3_server_config;
 typedef struct {
   uint64_t qid;

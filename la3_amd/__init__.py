@@ -64,7 +64,7 @@ class KernelStat(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("nworkers", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
                 ("state_budget", ctypes.c_uint64), ("keep_results", ctypes.c_int),
-                ("ppr_alpha", ctypes.c_double)]
+                ("ppr_alpha", ctypes.c_double), ("fused", ctypes.c_int)]
 
 
 class QueryInfo(ctypes.Structure):
@@ -450,7 +450,8 @@ def batch_run(ctx, batches, alpha=0.15, out=None):
     (la3_batch_run). `batches` is a list of (graph, kind, roots[, iters]) with
     kind Q_BFS / Q_SSSP / Q_PPR. Returns one dict per batch with the result
     ("out": k x nslots by id; None with out=DEVICE), for PPR also "comp" and
-    "residual", and "iters"."""
+    "residual", and "iters". `out` may also be a list with one output array
+    (or DEVICE) per batch."""
     arr = (Batch * len(batches))()
     keep = []
     res = []
@@ -458,7 +459,7 @@ def batch_run(ctx, batches, alpha=0.15, out=None):
         g, kind, roots = b[0], b[1], np.ascontiguousarray(b[2], dtype=np.uint32)
         iters = int(b[3]) if len(b) > 3 else 0
         dt = {Q_BFS: np.uint8, Q_SSSP: np.uint32, Q_PPR: np.float32}[kind]
-        o = _out(out, (roots.size, g.nslots), dt)
+        o = _out(out[i] if isinstance(out, (list, tuple)) else out, (roots.size, g.nslots), dt)
         r = {"out": o}
         arr[i].graph = g.h
         arr[i].kind = kind
@@ -484,8 +485,9 @@ class Server:
     batching by worker threads, admission by per-query state bytes."""
 
     def __init__(self, device=0, nworkers=2, max_batch=64, state_budget=32 << 30,
-                 keep_results=False, ppr_alpha=0.15):
-        cfg = ServerConfig(nworkers, max_batch, state_budget, int(keep_results), ppr_alpha)
+                 keep_results=False, ppr_alpha=0.15, fused=False):
+        cfg = ServerConfig(nworkers, max_batch, state_budget, int(keep_results), ppr_alpha,
+                           int(fused))
         h = ctypes.c_void_p()
         st = lib().la3_server_create(device, ctypes.byref(cfg), ctypes.byref(h))
         if st:

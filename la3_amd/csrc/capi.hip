@@ -1545,13 +1545,17 @@ la3_status fz_run(la3_ctx* c, la3_batch* batches, uint32_t nb, double alpha) {
     std::memcpy(&now, c->host_ctr, sizeof(now));
     if (now.active == 0) break;
     if (steps >= max_steps) return fail(c, LA3_EDEVICE, "fused run: step limit reached");
-    hipEvent_t a = nullptr, b = nullptr;
-    if (c->profiling) { a = get_event(c); b = get_event(c); (void)hipEventRecord(a, c->stream); }
-    HIPC(c, hipGraphLaunch(c->fz_exec, c->stream));
-    if (c->profiling) {
-      (void)hipEventRecord(b, c->stream);
-      c->evs.push_back({"fused_steps", a, b, 0.0, 0});
-      last_ev = (int)c->evs.size() - 1;
+    if (!c->profiling) {
+      HIPC(c, hipGraphLaunch(c->fz_exec, c->stream));
+      continue;
+    }
+    // Profiling: the same FZ_STEPS steps as separate launches, each
+    // bracketed by events (per-kernel device time; the graph has none).
+    const uint32_t ga = fz_ga(c) + FMAX_TASKS, gb = fz_gb(c) + FMAX_TASKS;
+    for (uint32_t i = 0; i < FZ_STEPS; i++) {
+      last_ev = launch(c, "fused_a", k_fused_a, ga, (const FTask*)c->fz_tasks, (const FHdr*)dh, (const FCtl*)dc);
+      launch(c, "fused_b", k_fused_b, gb, (const FTask*)c->fz_tasks, (const FHdr*)dh, (const FCtl*)dc);
+      launch(c, "fused_c", k_fused_c, 1, (const FTask*)c->fz_tasks, dh, dc, 0u);
     }
   }
   std::vector<FCtl> hc(nt);
@@ -1568,6 +1572,8 @@ la3_status fz_run(la3_ctx* c, la3_batch* batches, uint32_t nb, double alpha) {
                         : b.kind == LA3_Q_SSSP ? hc[t].iter + 1 : hc[t].iter;
     b.iters_run = std::max(b.iters_run, it);
   }
+  // The run's algorithmic bytes are attributed to k_fused_a, the launch
+  // that does the traversal and row-sum work.
   annotate(c, last_ev, bytes, edges);
   c->edges_scanned += edges;
   for (uint32_t t = 0; t < nt; t++) {

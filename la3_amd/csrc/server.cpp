@@ -70,11 +70,117 @@ uint64_t batch_bytes(const la3_server* s, la3_query_kind k, uint32_t graph, uint
 
 size_t elem_bytes(la3_query_kind k) { return k == LA3_Q_BFS ? 1 : 4; }
 
+// Mixed admission (cfg.fused): pending queries in arrival order join groups
+// keyed by (kind, graph, iters), each up to max_batch, while the admitted
+// state bytes fit the budget (a lone query is admitted when nothing runs)
+// and the groups fit one fused run (at most 64 tasks of 256 queries).
+// Called with the lock held; moves the picked queries into `groups`.
+uint64_t pick_mixed(la3_server* s, std::vector<std::vector<Query>>& groups) {
+  struct G { la3_query_kind kind; uint32_t graph, iters; std::vector<size_t> idx; };
+  std::vector<G> gs;
+  uint64_t bytes = 0;
+  uint32_t tasks = 0;
+  for (size_t i = 0; i < s->pending.size(); i++) {
+    const Query& q = s->pending[i];
+    G* gp = nullptr;
+    for (G& g : gs)
+      if (g.kind == q.kind && g.graph == q.graph && g.iters == q.iters) gp = &g;
+    const uint32_t have = gp ? (uint32_t)gp->idx.size() : 0;
+    if (have >= s->cfg.max_batch) continue;
+    const uint64_t db = batch_bytes(s, q.kind, q.graph, have + 1) - (have ? batch_bytes(s, q.kind, q.graph, have) : 0);
+    const uint32_t dt = (have % 256 == 0) ? 1 : 0;
+    const bool first = gs.empty();
+    if (!first && (s->in_flight_bytes + bytes + db > s->cfg.state_budget || tasks + dt > 64)) break;
+    if (first && s->in_flight_bytes + db > s->cfg.state_budget && s->running_batches != 0) break;
+    if (!gp) {
+      gs.push_back({q.kind, q.graph, q.iters, {}});
+      gp = &gs.back();
+    }
+    gp->idx.push_back(i);
+    bytes += db;
+    tasks += dt;
+  }
+  if (gs.empty()) return 0;
+  std::vector<size_t> all;
+  for (G& g : gs) {
+    std::vector<Query> v;
+    for (size_t i : g.idx) { v.push_back(s->pending[i]); all.push_back(i); }
+    groups.push_back(std::move(v));
+  }
+  std::sort(all.begin(), all.end());
+  for (size_t j = all.size(); j-- > 0;) s->pending.erase(s->pending.begin() + all[j]);
+  return bytes;
+}
+
+// Runs the picked groups as one fused batch run and records completions.
+void run_mixed(la3_server* s, la3_ctx* ctx, std::vector<std::vector<Query>>& groups, uint64_t bytes,
+               void*& host, uint64_t& host_bytes) {
+  const double start = s->now();
+  std::vector<la3_batch> bs(groups.size());
+  std::vector<std::vector<uint32_t>> roots(groups.size());
+  std::vector<size_t> off(groups.size());
+  uint64_t need = 0;
+  for (size_t i = 0; i < groups.size(); i++) {
+    const Query& h = groups[i].front();
+    off[i] = need;
+    need += (uint64_t)groups[i].size() * s->graphs[h.graph].nslots * elem_bytes(h.kind);
+  }
+  la3_status st = LA3_OK;
+  if (s->cfg.keep_results && host_bytes < need) {
+    if (host) la3_host_free(host);
+    host = nullptr;
+    host_bytes = 0;
+    if (la3_host_alloc(need, &host) == LA3_OK) host_bytes = need;
+    else st = LA3_ENOMEM;
+  }
+  for (size_t i = 0; i < groups.size(); i++) {
+    const Query& h = groups[i].front();
+    for (const Query& q : groups[i]) roots[i].push_back(q.root);
+    la3_batch& b = bs[i];
+    std::memset(&b, 0, sizeof(b));
+    b.graph = s->graphs[h.graph].g;
+    b.kind = h.kind;
+    b.k = (uint32_t)groups[i].size();
+    b.roots = roots[i].data();
+    b.iters = h.iters;
+    b.out = s->cfg.keep_results && host ? (uint8_t*)host + off[i] : nullptr;
+  }
+  if (st == LA3_OK) st = la3_batch_run(ctx, bs.data(), (uint32_t)bs.size(), s->cfg.ppr_alpha);
+  const double fin = s->now();
+  uint32_t total = 0;
+  for (auto& g : groups) total += (uint32_t)g.size();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (size_t i = 0; i < groups.size(); i++) {
+      const uint32_t k = (uint32_t)groups[i].size();
+      for (uint32_t j = 0; j < k; j++) {
+        const Query& q = groups[i][j];
+        la3_query_info info{};
+        info.qid = q.qid; info.kind = q.kind; info.graph = q.graph; info.root = q.root;
+        info.batch_size = total; info.status = st; info.submit_s = q.submit_s;
+        info.start_s = start; info.done_s = fin;
+        s->done.push_back(info);
+        if (s->cfg.keep_results && st == LA3_OK) {
+          const size_t row = (size_t)s->graphs[q.graph].nslots * elem_bytes(q.kind);
+          std::vector<uint8_t> v(row);
+          std::memcpy(v.data(), (const uint8_t*)host + off[i] + (size_t)j * row, row);
+          s->results[q.qid] = std::move(v);
+        }
+      }
+    }
+    s->completed += total;
+    s->in_flight_bytes -= bytes;
+    s->running_batches--;
+  }
+  s->cv.notify_all();
+}
+
 void worker_main(la3_server* s, la3_ctx* ctx) {
   void* host = nullptr;
   uint64_t host_bytes = 0;
   for (;;) {
     std::vector<Query> batch;
+    std::vector<std::vector<Query>> groups;
     uint64_t bytes = 0;
     {
       std::unique_lock<std::mutex> lk(s->mu);
@@ -85,7 +191,14 @@ void worker_main(la3_server* s, la3_ctx* ctx) {
           la3_ctx_destroy(ctx);
           return;
         }
-        if (!s->pending.empty()) {
+        if (!s->pending.empty() && s->cfg.fused) {
+          bytes = pick_mixed(s, groups);
+          if (!groups.empty()) {
+            s->in_flight_bytes += bytes;
+            s->running_batches++;
+            break;
+          }
+        } else if (!s->pending.empty()) {
           const Query& head = s->pending.front();
           // Grow the batch with same-key queries in arrival order while the
           // budget admits it; a lone query is admitted when nothing runs.
@@ -112,6 +225,10 @@ void worker_main(la3_server* s, la3_ctx* ctx) {
         }
         s->cv.wait(lk);
       }
+    }
+    if (!groups.empty()) {
+      run_mixed(s, ctx, groups, bytes, host, host_bytes);
+      continue;
     }
     const Query& h = batch.front();
     const uint32_t k = (uint32_t)batch.size();

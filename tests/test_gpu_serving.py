@@ -1,7 +1,9 @@
 """Serving loop: resident graphs of different sizes, a mixed BFS / SSSP / PPR
 stream submitted while batches run, and every served result equal to the
 same query run alone (bit-exact; PPR bit-identical to a batch of one and
-within L1 1e-7 of the FP64 single-query program)."""
+within L1 1e-7 of the FP64 single-query program). Both admission modes:
+per-(program, graph) batches, and mixed batches across programs and graphs
+run as one fused batch run."""
 import numpy as np
 import pytest
 
@@ -20,8 +22,9 @@ def graphs(gpu_ctx):
     return out
 
 
-def test_mixed_stream_equals_alone(gpu_ctx, graphs):
-    srv = L.Server(0, nworkers=3, max_batch=16, state_budget=1 << 30, keep_results=True)
+@pytest.mark.parametrize("fused", [False, True])
+def test_mixed_stream_equals_alone(gpu_ctx, graphs, fused):
+    srv = L.Server(0, nworkers=3, max_batch=16, state_budget=1 << 30, keep_results=True, fused=fused)
     gids = [srv.add_graph(g) for g, _ in graphs]
     rng = np.random.default_rng(7)
     subs = []
@@ -51,10 +54,10 @@ def test_mixed_stream_equals_alone(gpu_ctx, graphs):
     srv.close()
 
 
-def test_admission_budget_limits_batch(gpu_ctx, graphs):
+@pytest.mark.parametrize("fused", [False, True])
+def test_admission_budget_limits_batch(gpu_ctx, graphs, fused):
     g, roots = graphs[2]
-    one = None
-    srv = L.Server(0, nworkers=1, max_batch=64, state_budget=1, keep_results=False)
+    srv = L.Server(0, nworkers=1, max_batch=64, state_budget=1, keep_results=False, fused=fused)
     gid = srv.add_graph(g)
     one = srv.query_bytes(L.Q_SSSP, gid)
     assert one > 0
