@@ -52,7 +52,7 @@ struct la3_ctx {
   // context's life) and the captured step graph that reads them.
   FTask* fz_tasks = nullptr;
   uint8_t* fz_ctl = nullptr;
-  hipGraphExec_t fz_exec = nullptr;
+  hipGraphExec_t fz_exec[64] = {};  // by launch mask (fz_step)
 };
 
 namespace {
@@ -88,7 +88,7 @@ void ctx_release(la3_ctx* c) {
     for (auto& kv : c->scratch) if (kv.second.p) (void)hipFree(kv.second.p);
     for (auto& e : c->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
-    if (c->fz_exec) (void)hipGraphExecDestroy(c->fz_exec);
+    for (auto e : c->fz_exec) if (e) (void)hipGraphExecDestroy(e);
     if (c->host_ctr) (void)hipHostFree(c->host_ctr);
     (void)hipStreamDestroy(c->stream);
   }
@@ -1476,26 +1476,48 @@ la3_status fz_finish(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t 
   return deliver(c, host, out, esz * (size_t)S * k);
 }
 
-la3_status fz_capture(la3_ctx* c) {
-  if (c->fz_exec) return LA3_OK;
+// One step: the A and B launches of each width class in `mask` (bit 0: 64
+// queries, bit 1: 128, bit 2: 256), the bottom-up BFS launch of each width
+// class with BFS tasks (bits 3-5), then the control launch. With profiling
+// each launch is bracketed by events (last_a: the event slot of the last A).
+void fz_step(la3_ctx* c, uint32_t mask, bool events, int* last_a, int* last_p) {
   const uint32_t ga = fz_ga(c) + FMAX_TASKS, gb = fz_gb(c) + FMAX_TASKS;
+  const FTask* tk = (const FTask*)c->fz_tasks;
   FHdr* hdr = (FHdr*)c->fz_ctl;
   FCtl* ctl = (FCtl*)(c->fz_ctl + sizeof(FHdr));
+  auto go = [&](const char* name, auto kern, uint32_t grid) {
+    if (events) {
+      const int ev = launch(c, name, kern, grid, tk, (const FHdr*)hdr, (const FCtl*)ctl);
+      if (name[6] == 'a') *last_a = ev;
+      if (name[6] == 'p') *last_p = ev;
+    } else {
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, c->stream, tk, (const FHdr*)hdr, (const FCtl*)ctl);
+    }
+  };
+  if (mask & 1) go("fused_a", k_fused_a<1>, ga);
+  if (mask & 2) go("fused_a", k_fused_a<2>, ga);
+  if (mask & 4) go("fused_a", k_fused_a<4>, ga);
+  if (mask & 8) go("fused_pull", k_fused_pull<1>, ga);
+  if (mask & 16) go("fused_pull", k_fused_pull<2>, ga);
+  if (mask & 32) go("fused_pull", k_fused_pull<4>, ga);
+  if (mask & 1) go("fused_b", k_fused_b<1>, gb);
+  if (mask & 2) go("fused_b", k_fused_b<2>, gb);
+  if (mask & 4) go("fused_b", k_fused_b<4>, gb);
+  if (events) launch(c, "fused_c", k_fused_c, 1, tk, hdr, ctl, 0u);
+  else hipLaunchKernelGGL(k_fused_c, dim3(1), dim3(BLOCK), 0, c->stream, tk, hdr, ctl, 0u);
+}
+
+la3_status fz_capture(la3_ctx* c, uint32_t mask) {
+  if (c->fz_exec[mask]) return LA3_OK;
   hipGraph_t graph = nullptr;
   HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-  for (uint32_t i = 0; i < FZ_STEPS; i++) {
-    hipLaunchKernelGGL(k_fused_a, dim3(ga), dim3(BLOCK), 0, c->stream, (const FTask*)c->fz_tasks, (const FHdr*)hdr,
-                       (const FCtl*)ctl);
-    hipLaunchKernelGGL(k_fused_b, dim3(gb), dim3(BLOCK), 0, c->stream, (const FTask*)c->fz_tasks, (const FHdr*)hdr,
-                       (const FCtl*)ctl);
-    hipLaunchKernelGGL(k_fused_c, dim3(1), dim3(BLOCK), 0, c->stream, (const FTask*)c->fz_tasks, hdr, ctl, 0u);
-  }
+  for (uint32_t i = 0; i < FZ_STEPS; i++) fz_step(c, mask, false, nullptr, nullptr);
   hipError_t e = hipStreamEndCapture(c->stream, &graph);
   if (e != hipSuccess) return fail(c, LA3_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  e = hipGraphInstantiate(&c->fz_exec, graph, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&c->fz_exec[mask], graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) {
-    c->fz_exec = nullptr;
+    c->fz_exec[mask] = nullptr;
     return fail(c, LA3_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
   return LA3_OK;
@@ -1529,7 +1551,12 @@ la3_status fz_run(la3_ctx* c, la3_batch* batches, uint32_t nb, double alpha) {
   FHdr* dh = (FHdr*)c->fz_ctl;
   FCtl* dc = (FCtl*)(c->fz_ctl + sizeof(FHdr));
   launch(c, "fused_init", k_fused_c, 1, (const FTask*)c->fz_tasks, dh, dc, 1u);
-  if ((st = fz_capture(c))) return st;
+  uint32_t mask = 0;
+  for (uint32_t t = 0; t < nt; t++) {
+    const uint32_t wb = ht[t].width == 1 ? 1u : ht[t].width == 2 ? 2u : 4u;
+    mask |= wb | (ht[t].kind == FK_BFS ? wb << 3 : 0u);
+  }
+  if (!c->profiling && (st = fz_capture(c, mask))) return st;
   // Safety net against a run that does not terminate: far more steps than
   // any traversal here takes (BFS <= nslots + 1; SSSP rounds are bounded by
   // the near/far phases; PPR = iters).
@@ -1537,7 +1564,7 @@ la3_status fz_run(la3_ctx* c, la3_batch* batches, uint32_t nb, double alpha) {
   for (uint32_t t = 0; t < nt; t++)
     max_steps = std::max<uint64_t>(max_steps, ht[t].kind == FK_PPR ? (uint64_t)ht[t].iters + 8
                                                                    : 8ull * ht[t].g.nslots + 4096);
-  int last_ev = -1;
+  int last_ev = -1, last_pull = -1;
   for (uint64_t steps = 0;; steps += FZ_STEPS) {
     HIPC(c, hipMemcpyAsync(c->host_ctr, dh, sizeof(FHdr), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
@@ -1546,35 +1573,32 @@ la3_status fz_run(la3_ctx* c, la3_batch* batches, uint32_t nb, double alpha) {
     if (now.active == 0) break;
     if (steps >= max_steps) return fail(c, LA3_EDEVICE, "fused run: step limit reached");
     if (!c->profiling) {
-      HIPC(c, hipGraphLaunch(c->fz_exec, c->stream));
+      HIPC(c, hipGraphLaunch(c->fz_exec[mask], c->stream));
       continue;
     }
     // Profiling: the same FZ_STEPS steps as separate launches, each
     // bracketed by events (per-kernel device time; the graph has none).
-    const uint32_t ga = fz_ga(c) + FMAX_TASKS, gb = fz_gb(c) + FMAX_TASKS;
-    for (uint32_t i = 0; i < FZ_STEPS; i++) {
-      last_ev = launch(c, "fused_a", k_fused_a, ga, (const FTask*)c->fz_tasks, (const FHdr*)dh, (const FCtl*)dc);
-      launch(c, "fused_b", k_fused_b, gb, (const FTask*)c->fz_tasks, (const FHdr*)dh, (const FCtl*)dc);
-      launch(c, "fused_c", k_fused_c, 1, (const FTask*)c->fz_tasks, dh, dc, 0u);
-    }
+    for (uint32_t i = 0; i < FZ_STEPS; i++) fz_step(c, mask, true, &last_ev, &last_pull);
   }
   std::vector<FCtl> hc(nt);
   HIPC(c, hipMemcpyAsync(hc.data(), dc, sizeof(FCtl) * nt, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
-  double bytes = 0;
+  double bytes = 0, pbytes = 0;
   uint64_t edges = 0;
   for (uint32_t i = 0; i < nb; i++) batches[i].iters_run = 0;
   for (uint32_t t = 0; t < nt; t++) {
-    bytes += hc[t].bytes;
+    bytes += hc[t].bytes - hc[t].pbytes;
+    pbytes += hc[t].pbytes;
     edges += hc[t].edges;
     la3_batch& b = batches[tasks[t].batch];
     const uint32_t it = b.kind == LA3_Q_BFS ? std::max<uint32_t>(hc[t].iter, 1)
                         : b.kind == LA3_Q_SSSP ? hc[t].iter + 1 : hc[t].iter;
     b.iters_run = std::max(b.iters_run, it);
   }
-  // The run's algorithmic bytes are attributed to k_fused_a, the launch
-  // that does the traversal and row-sum work.
+  // The run's algorithmic bytes: bottom-up BFS steps to k_fused_pull, the
+  // rest to k_fused_a (the B and control launches move no counted bytes).
   annotate(c, last_ev, bytes, edges);
+  annotate(c, last_pull, pbytes, 0);
   c->edges_scanned += edges;
   for (uint32_t t = 0; t < nt; t++) {
     const la3_batch& b = batches[tasks[t].batch];

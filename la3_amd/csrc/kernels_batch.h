@@ -623,17 +623,17 @@ __device__ __forceinline__ void ppr_finish(const PprBatchArgs& a, uint32_t r, co
 }
 
 // Per-block, per-query reduction of the residual partials (FP64, fixed order).
+// red: LDS of (BLOCK / WAVE) x K doubles.
 template <int QW>
-__device__ __forceinline__ void ppr_resid_flush(const PprBatchArgs& a, double* res, uint32_t bid) {
+__device__ __forceinline__ void ppr_resid_flush(const PprBatchArgs& a, double* res, uint32_t bid, double* red) {
   constexpr uint32_t K = 64 * QW;
-  __shared__ double red[BLOCK / WAVE][K];
   const uint32_t w0 = threadIdx.x / WAVE, lane = lane_id();
 #pragma unroll
-  for (int w = 0; w < QW; w++) red[w0][64 * w + lane] = res[w];
+  for (int w = 0; w < QW; w++) red[w0 * K + 64 * w + lane] = res[w];
   __syncthreads();
   for (uint32_t q = threadIdx.x; q < K; q += BLOCK) {
     double t = 0;
-    for (uint32_t j = 0; j < BLOCK / WAVE; j++) t += red[j][q];
+    for (uint32_t j = 0; j < BLOCK / WAVE; j++) t += red[j * K + q];
     a.resid[(size_t)bid * K + q] = t;
   }
 }
@@ -675,7 +675,7 @@ __device__ __forceinline__ void ppr_row_sum(const DG& g, const PprBatchArgs& a, 
 // Rows [r0, r1) with at most PSEG entries: one wavefront per row.
 template <int QW>
 __device__ __forceinline__ void mppr_rows_body(const DG& g, Part p, const PprBatchArgs& a, uint32_t r0,
-                                               uint32_t r1, bool loop) {
+                                               uint32_t r1, bool loop, double* red) {
   double res[QW];
 #pragma unroll
   for (int w = 0; w < QW; w++) res[w] = 0.0;
@@ -688,13 +688,14 @@ __device__ __forceinline__ void mppr_rows_body(const DG& g, Part p, const PprBat
     ppr_row_sum<QW>(g, a, b, e, s, c);
     ppr_finish<QW>(a, r, s, c, loop, res);
   }
-  ppr_resid_flush<QW>(a, res, p.bid);
+  ppr_resid_flush<QW>(a, res, p.bid, red);
 }
 
 template <int QW>
 __global__ void __launch_bounds__(BLOCK) k_mppr_rows(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1,
                                                      int loop) {
-  mppr_rows_body<QW>(g, grid_part(), a, r0, r1, loop != 0);
+  __shared__ double red[(BLOCK / WAVE) * 64 * QW];
+  mppr_rows_body<QW>(g, grid_part(), a, r0, r1, loop != 0, red);
 }
 
 // One wavefront per PSEG-entry segment of a long row: partial (sum, comp).
@@ -726,7 +727,7 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_segs(DG g, PprBatchArgs a, uint3
 // Combines each long row's segment partials in segment order, then applies.
 template <int QW>
 __device__ __forceinline__ void mppr_combine_body(const DG& g, Part p, const PprBatchArgs& a, uint32_t r0,
-                                                  uint32_t r1, bool loop) {
+                                                  uint32_t r1, bool loop, double* red) {
   constexpr uint32_t K = 64 * QW;
   const uint32_t lane = lane_id();
   double res[QW];
@@ -748,13 +749,14 @@ __device__ __forceinline__ void mppr_combine_body(const DG& g, Part p, const Ppr
     }
     ppr_finish<QW>(a, r, s, c, loop, res);
   }
-  ppr_resid_flush<QW>(a, res, p.bid);
+  ppr_resid_flush<QW>(a, res, p.bid, red);
 }
 
 template <int QW>
 __global__ void __launch_bounds__(BLOCK) k_mppr_combine(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1,
                                                         int loop) {
-  mppr_combine_body<QW>(g, grid_part(), a, r0, r1, loop != 0);
+  __shared__ double red[(BLOCK / WAVE) * 64 * QW];
+  mppr_combine_body<QW>(g, grid_part(), a, r0, r1, loop != 0, red);
 }
 
 // hi = lo = tele; x = message for every vertex with a column (constant for sources).

@@ -1,8 +1,8 @@
 // Fused, device-driven steps for a mix of batched queries: any number of
 // batches (BFS k-bit, SSSP k-wide, PPR FP32+compensation), on one or several
-// resident graphs, advance together, one step per iteration of three
-// launches:
-//   k_fused_a  the traversal / row-sum work of every batch (one launch);
+// resident graphs, advance together, one step per iteration of:
+//   k_fused_a  the traversal / row-sum work of every batch (one launch per
+//              width class of 64, 128 or 256 queries present);
 //   k_fused_b  the work that needs A's results (BFS apply, SSSP work items,
 //              PPR long-row combine) and the clearing of consumed masks;
 //   k_fused_c  one block: reads each batch's counters on the device, picks
@@ -53,10 +53,12 @@ struct FTask {
 struct FCtl {
   uint32_t iter, cur, done, bottom_up;
   uint32_t thr, nitems, a_beg, a_end;
-  uint32_t b_beg, b_end, fcols, pad;
+  uint32_t b_beg, b_end, p_beg, p_end;   // block shares: A, B and pull launches
+  uint32_t fcols, pad[3];
   unsigned long long fedges;   // entries behind the current work list
   unsigned long long edges;    // entries scanned over the run
   double bytes;                // algorithmic bytes over the run
+  double pbytes;               // of which in bottom-up BFS steps (pull launch)
 };
 
 struct FHdr {
@@ -64,12 +66,17 @@ struct FHdr {
   uint32_t steps, pad[11];
 };
 
+// The task of width W whose share of this launch holds blockIdx.x.
+enum : uint32_t { PH_A = 0, PH_B = 1, PH_PULL = 2 };
+
+template <int W>
 __device__ __forceinline__ const FTask* find_task(const FTask* tasks, const FHdr* h, const FCtl* ctl,
-                                                  bool phase_a, Part& p, uint32_t& t) {
+                                                  uint32_t phase, Part& p, uint32_t& t) {
   const uint32_t nt = h->ntasks;
   for (t = 0; t < nt; t++) {
-    const uint32_t b = phase_a ? ctl[t].a_beg : ctl[t].b_beg;
-    const uint32_t e = phase_a ? ctl[t].a_end : ctl[t].b_end;
+    if (tasks[t].width != (uint32_t)W) continue;
+    const uint32_t b = phase == PH_A ? ctl[t].a_beg : phase == PH_B ? ctl[t].b_beg : ctl[t].p_beg;
+    const uint32_t e = phase == PH_A ? ctl[t].a_end : phase == PH_B ? ctl[t].b_end : ctl[t].p_end;
     if (blockIdx.x >= b && blockIdx.x < e) {
       p.bid = blockIdx.x - b;
       p.nb = e - b;
@@ -79,28 +86,25 @@ __device__ __forceinline__ const FTask* find_task(const FTask* tasks, const FHdr
   return nullptr;
 }
 
+// Top-down BFS step (bottom-up steps run in k_fused_pull: the W-bit pull
+// bodies need more registers than any other body, and kept in their own
+// launch they do not force spills into the SSSP / PPR loops of k_fused_a).
 template <int W>
 __device__ __forceinline__ void fused_bfs_a(const FTask& T, const FCtl& C, Part p, Emitter& em) {
   const uint32_t cur = C.cur;
-  if (!C.bottom_up) {
-    msbfs_push_body<W>(T.g, p, em, T.q[cur].items, C.nitems, T.vis, T.fr[cur], T.fr[cur ^ 1], T.stamp,
-                       C.iter + 2, T.touched);
-    em_finish(em, T.touched, p.bid);
-  } else {
-    msbfs_pull_body<W>(T.g, p, em, T.vis, T.fr[cur], T.fr[cur ^ 1], T.lev, C.iter, T.valid, T.q[cur ^ 1]);
-    msbfs_pull_long_body<W>(T.g, p, em, T.vis, T.fr[cur], T.fr[cur ^ 1], T.lev, C.iter, T.valid,
-                            T.q[cur ^ 1]);
-    em_finish(em, T.q[cur ^ 1], p.bid);
-  }
+  msbfs_push_body<W>(T.g, p, em, T.q[cur].items, C.nitems, T.vis, T.fr[cur], T.fr[cur ^ 1], T.stamp,
+                     C.iter + 2, T.touched);
+  em_finish(em, T.touched, p.bid);
 }
 
 template <int W>
 __device__ __forceinline__ void fused_bfs_b(const FTask& T, const FCtl& C, Part p, Emitter& em) {
   const uint32_t cur = C.cur;
   if (!C.bottom_up) {
-    msbfs_apply_body<W>(T.g, p, em, T.touched.items, *T.touched.nitems, T.vis, T.fr[cur ^ 1], T.lev, C.iter,
-                        T.q[cur ^ 1]);
-    em_finish(em, T.q[cur ^ 1], p.bid);
+    const DG g = T.g;
+    const Frontier nf = T.q[cur ^ 1];
+    msbfs_apply_body<W>(g, p, em, T.touched.items, *T.touched.nitems, T.vis, T.fr[cur ^ 1], T.lev, C.iter, nf);
+    em_finish(em, nf, p.bid);
   }
   clear_items_body<W>(p, T.q[cur].items, C.nitems, T.fr[cur]);
 }
@@ -115,70 +119,88 @@ __device__ __forceinline__ void fused_sssp_a(const FTask& T, const FCtl& C, Part
 template <int QW>
 __device__ __forceinline__ void fused_sssp_b(const FTask& T, const FCtl& C, Part p, Emitter& em) {
   const uint32_t cur = C.cur;
-  mssp_items_body(T.g, p, em, T.touched.items, *T.touched.nitems, T.stamp, C.iter + 2, T.q[cur ^ 1]);
-  em_finish(em, T.q[cur ^ 1], p.bid);
+  const DG g = T.g;
+  const Frontier nf = T.q[cur ^ 1];
+  mssp_items_body(g, p, em, T.touched.items, *T.touched.nitems, T.stamp, C.iter + 2, nf);
+  em_finish(em, nf, p.bid);
   clear_items_body<QW>(p, T.q[cur].items, C.nitems, T.m[cur]);
 }
 
+// PPR steps reduce their residual partials in the emitter's LDS (unused by
+// PPR): (BLOCK / WAVE) x 64 * QW doubles <= the emitter's 4 x QCAP words.
+static_assert((BLOCK / WAVE) * 64 * FMAX_WIDTH <= (BLOCK / WAVE) * QCAP, "LDS reuse");
+
 template <int QW>
-__device__ __forceinline__ void fused_ppr_a(const FTask& T, const FCtl& C, Part p) {
+__device__ __forceinline__ void fused_ppr_a(const FTask& T, const FCtl& C, Part p, double* red) {
   PprBatchArgs a = T.pa;
   a.x = T.xs[C.cur];
   a.xnext = T.xs[C.cur ^ 1];
   a.resid = T.res_rows;
-  mppr_rows_body<QW>(T.g, p, a, 0, T.g.nreg, true);
+  mppr_rows_body<QW>(T.g, p, a, 0, T.g.nreg, true, red);
   mppr_segs_body<QW>(T.g, p, a, T.g.npseg_reg);
 }
 
 template <int QW>
-__device__ __forceinline__ void fused_ppr_b(const FTask& T, const FCtl& C, Part p) {
+__device__ __forceinline__ void fused_ppr_b(const FTask& T, const FCtl& C, Part p, double* red) {
   PprBatchArgs a = T.pa;
   a.x = T.xs[C.cur];
   a.xnext = T.xs[C.cur ^ 1];
   a.resid = T.res_comb;
-  mppr_combine_body<QW>(T.g, p, a, 0, T.g.nreg, true);
+  const DG g = T.g;
+  mppr_combine_body<QW>(g, p, a, 0, g.nreg, true, red);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_fused_a(const FTask* __restrict__ tasks, const FHdr* hdr,
+// One launch per width class W present in the run (1, 2 or 4 query words):
+// each holds only the W-wide bodies, so its registers are those of its own
+// widest body.
+template <int W>
+__global__ void __launch_bounds__(BLOCK, 8) k_fused_a(const FTask* __restrict__ tasks, const FHdr* hdr,
                                                    const FCtl* ctl) {
   Part p;
   uint32_t t;
-  const FTask* T = find_task(tasks, hdr, ctl, true, p, t);
+  const FTask* T = find_task<W>(tasks, hdr, ctl, PH_A, p, t);
   if (!T) return;  // block-uniform
   const FCtl& C = ctl[t];
   LA3_EMITTER(em);
-  switch (T->kind * 8 + T->width) {
-    case FK_BFS * 8 + 1: fused_bfs_a<1>(*T, C, p, em); break;
-    case FK_BFS * 8 + 2: fused_bfs_a<2>(*T, C, p, em); break;
-    case FK_BFS * 8 + 4: fused_bfs_a<4>(*T, C, p, em); break;
-    case FK_SSSP * 8 + 1: fused_sssp_a<1>(*T, C, p, em); break;
-    case FK_SSSP * 8 + 2: fused_sssp_a<2>(*T, C, p, em); break;
-    case FK_SSSP * 8 + 4: fused_sssp_a<4>(*T, C, p, em); break;
-    case FK_PPR * 8 + 1: fused_ppr_a<1>(*T, C, p); break;
-    case FK_PPR * 8 + 2: fused_ppr_a<2>(*T, C, p); break;
-    case FK_PPR * 8 + 4: fused_ppr_a<4>(*T, C, p); break;
+  switch (T->kind) {
+    case FK_BFS: fused_bfs_a<W>(*T, C, p, em); break;
+    case FK_SSSP: fused_sssp_a<W>(*T, C, p, em); break;
+    case FK_PPR: fused_ppr_a<W>(*T, C, p, (double*)em_lds); break;
     default: break;
   }
 }
 
+// Bottom-up BFS steps of width W.
+template <int W>
+__global__ void __launch_bounds__(BLOCK) k_fused_pull(const FTask* __restrict__ tasks, const FHdr* hdr,
+                                                      const FCtl* ctl) {
+  Part p;
+  uint32_t t;
+  const FTask* T = find_task<W>(tasks, hdr, ctl, PH_PULL, p, t);
+  if (!T) return;  // block-uniform
+  const FCtl& C = ctl[t];
+  const uint32_t cur = C.cur;
+  const DG g = T->g;
+  const Frontier nf = T->q[cur ^ 1];
+  LA3_EMITTER(em);
+  msbfs_pull_body<W>(g, p, em, T->vis, T->fr[cur], T->fr[cur ^ 1], T->lev, C.iter, T->valid, nf);
+  msbfs_pull_long_body<W>(g, p, em, T->vis, T->fr[cur], T->fr[cur ^ 1], T->lev, C.iter, T->valid, nf);
+  em_finish(em, nf, p.bid);
+}
+
+template <int W>
 __global__ void __launch_bounds__(BLOCK) k_fused_b(const FTask* __restrict__ tasks, const FHdr* hdr,
                                                    const FCtl* ctl) {
   Part p;
   uint32_t t;
-  const FTask* T = find_task(tasks, hdr, ctl, false, p, t);
+  const FTask* T = find_task<W>(tasks, hdr, ctl, PH_B, p, t);
   if (!T) return;  // block-uniform
   const FCtl& C = ctl[t];
   LA3_EMITTER(em);
-  switch (T->kind * 8 + T->width) {
-    case FK_BFS * 8 + 1: fused_bfs_b<1>(*T, C, p, em); break;
-    case FK_BFS * 8 + 2: fused_bfs_b<2>(*T, C, p, em); break;
-    case FK_BFS * 8 + 4: fused_bfs_b<4>(*T, C, p, em); break;
-    case FK_SSSP * 8 + 1: fused_sssp_b<1>(*T, C, p, em); break;
-    case FK_SSSP * 8 + 2: fused_sssp_b<2>(*T, C, p, em); break;
-    case FK_SSSP * 8 + 4: fused_sssp_b<4>(*T, C, p, em); break;
-    case FK_PPR * 8 + 1: fused_ppr_b<1>(*T, C, p); break;
-    case FK_PPR * 8 + 2: fused_ppr_b<2>(*T, C, p); break;
-    case FK_PPR * 8 + 4: fused_ppr_b<4>(*T, C, p); break;
+  switch (T->kind) {
+    case FK_BFS: fused_bfs_b<W>(*T, C, p, em); break;
+    case FK_SSSP: fused_sssp_b<W>(*T, C, p, em); break;
+    case FK_PPR: fused_ppr_b<W>(*T, C, p, (double*)em_lds); break;
     default: break;
   }
 }
@@ -214,6 +236,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
                                                    uint32_t init) {
   __shared__ double est[FMAX_TASKS];
   __shared__ uint32_t useful[FMAX_TASKS];
+  __shared__ uint32_t fin[FMAX_TASKS];   // PPR tasks whose last iteration just ran
   const uint32_t nt = hdr->ntasks;
   const uint32_t lane = lane_id();
   for (uint32_t t = threadIdx.x / WAVE; t < nt; t += BLOCK / WAVE) {
@@ -221,33 +244,23 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
     FCtl& C = ctl[t];
     const DG& g = T.g;
     const uint32_t K = 64 * T.width;
+    if (lane == 0) fin[t] = 0;
     if (!init && C.done) {
       if (lane == 0) { est[t] = 0; useful[t] = 0; }
       continue;
     }
     if (T.kind == FK_PPR) {
-      if (!init) {
-        // FP64 residual of the step, in a fixed order: A-block partials,
-        // then B-block partials.
-        const uint32_t na = C.a_end - C.a_beg, nb = C.b_end - C.b_beg;
-        for (uint32_t q = lane; q < K; q += WAVE) {
-          double r = 0;
-          for (uint32_t b = 0; b < na; b++) r += T.res_rows[(size_t)b * K + q];
-          if (g.npseg_reg)
-            for (uint32_t b = 0; b < nb; b++) r += T.res_comb[(size_t)b * K + q];
-          T.resid[q] = r;
-        }
-        if (lane == 0) {
+      if (lane == 0) {
+        if (!init) {
           C.bytes += T.step_bytes;
           C.edges += T.step_edges;
           C.iter++;
           C.cur ^= 1;
+        } else {
+          C.iter = 0; C.cur = 0; C.bytes = 0; C.pbytes = 0; C.edges = 0;
         }
-      } else if (lane == 0) {
-        C.iter = 0; C.cur = 0; C.bytes = 0; C.edges = 0;
-      }
-      if (lane == 0) {
         C.done = C.iter >= T.iters;
+        fin[t] = !init && C.done;
         est[t] = C.done ? 0.0 : (4.0 + 4.0 * T.k) * (double)g.nnz + 20.0 * T.k * (double)g.nreg;
         useful[t] = C.done ? 0 : (g.nreg + 3) / 4 + (g.npseg_reg + 3) / 4;
       }
@@ -271,6 +284,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
           } else {
             b = (4.0 + kb) * ns.scanned + (8.0 + kb) * g.nrows + 2 * kb * ns.verts;
             e = ns.scanned;
+            C.pbytes += b;
           }
         } else {
           b = (8.0 + 4.0 * K) * C.fedges + (8.0 + 4.0 * K) * C.fcols + (K / 8.0 + 4) * ns.verts;
@@ -289,7 +303,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
         C.cur = nxt;
       }
     } else if (lane == 0) {
-      C.iter = 0; C.cur = 0; C.bottom_up = 0; C.bytes = 0; C.edges = 0;
+      C.iter = 0; C.cur = 0; C.bottom_up = 0; C.bytes = 0; C.pbytes = 0; C.edges = 0;
       C.thr = T.delta ? T.delta : 0xffffffffu;
     }
     // The list just consumed (and the touched list) are written next step.
@@ -308,28 +322,57 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
     }
   }
   __syncthreads();
+  // FP64 residual of a PPR task's last iteration, per query, in a fixed
+  // order: the A-block partials, then the B-block partials of that step.
+  for (uint32_t t = 0; t < nt; t++) {
+    if (!fin[t]) continue;  // block-uniform
+    const FTask& T = tasks[t];
+    const FCtl& C = ctl[t];
+    const uint32_t K = 64 * T.width;
+    const uint32_t na = C.a_end - C.a_beg, nb = C.b_end - C.b_beg;
+    for (uint32_t q = threadIdx.x; q < K; q += BLOCK) {
+      double r = 0;
+      for (uint32_t b = 0; b < na; b++) r += T.res_rows[(size_t)b * K + q];
+      if (T.g.npseg_reg)
+        for (uint32_t b = 0; b < nb; b++) r += T.res_comb[(size_t)b * K + q];
+      T.resid[q] = r;
+    }
+  }
+  __syncthreads();  // the partitions read above are rewritten below
   if (threadIdx.x == 0) {
-    double tot = 0;
-    uint32_t act = 0;
+    // Blocks of each width class's launches, split among its tasks: A (and,
+    // for bottom-up BFS steps, the pull launch) in proportion to the
+    // estimated bytes (at least 1, at most the useful count), B evenly.
+    double tot[5] = {0, 0, 0, 0, 0}, totp[5] = {0, 0, 0, 0, 0};
+    uint32_t act[5] = {0, 0, 0, 0, 0}, a[5] = {0, 0, 0, 0, 0}, b[5] = {0, 0, 0, 0, 0};
+    uint32_t pp[5] = {0, 0, 0, 0, 0};
+    uint32_t active = 0;
     for (uint32_t t = 0; t < nt; t++) {
-      tot += est[t];
-      act += ctl[t].done ? 0 : 1;
+      const uint32_t w = tasks[t].width;
+      const bool pull = tasks[t].kind == FK_BFS && ctl[t].bottom_up;
+      (pull ? totp : tot)[w] += est[t];
+      act[w] += ctl[t].done ? 0 : 1;
+      active += ctl[t].done ? 0 : 1;
     }
     const uint32_t ga = hdr->ga, gb = hdr->gb;
-    uint32_t a = 0, b = 0;
     for (uint32_t t = 0; t < nt; t++) {
       FCtl& C = ctl[t];
-      uint32_t na = 0, nb = 0;
+      const uint32_t w = tasks[t].width;
+      const bool pull = tasks[t].kind == FK_BFS && C.bottom_up;
+      uint32_t na = 0, nb = 0, np = 0;
       if (!C.done) {
-        na = tot > 0 ? (uint32_t)(ga * (est[t] / tot)) : 1;
-        na = max(1u, min(na, max(1u, useful[t])));
-        nb = max(1u, gb / max(act, 1u));
+        const double tw = pull ? totp[w] : tot[w];
+        uint32_t n = tw > 0 ? (uint32_t)(ga * (est[t] / tw)) : 1;
+        n = max(1u, min(n, max(1u, useful[t])));
+        (pull ? np : na) = n;
+        nb = max(1u, gb / max(act[w], 1u));
       }
-      C.a_beg = a; C.a_end = a + na; a += na;
-      C.b_beg = b; C.b_end = b + nb; b += nb;
+      C.a_beg = a[w]; C.a_end = a[w] + na; a[w] += na;
+      C.p_beg = pp[w]; C.p_end = pp[w] + np; pp[w] += np;
+      C.b_beg = b[w]; C.b_end = b[w] + nb; b[w] += nb;
     }
-    hdr->active = act;
-    if (!init && act) hdr->steps++;
+    hdr->active = active;
+    if (!init && active) hdr->steps++;
   }
 }
 
