@@ -218,6 +218,27 @@ typedef struct {
 } la3_batch;
 la3_status la3_batch_run(la3_ctx* ctx, la3_batch* batches, uint32_t nbatches, double ppr_alpha);
 
+/* Sessions: continuous batching at step granularity. A session is a fused
+ * run that batches join and leave while it runs: la3_session_add sets a batch
+ * up (it joins the next step), la3_session_step runs 4 steps of every running
+ * batch (one replay of the captured graph) and finishes the batches whose
+ * traversals ended (results copied to their `out`, which must stay valid
+ * until then), and la3_session_poll returns the ids of finished batches.
+ * la3_batch_run is one session run to completion. A context holds at most one
+ * session at a time (LA3_EAGAIN otherwise); a session has 64 task slots of
+ * 256 queries (la3_session_add returns LA3_EAGAIN when the batch does not
+ * fit). Each batch's results are those of la3_batch_run for it alone. */
+typedef struct la3_session la3_session;
+la3_status la3_session_create(la3_ctx* ctx, double ppr_alpha, la3_session** out);
+la3_status la3_session_add(la3_session* s, const la3_batch* batch, uint64_t* id);
+/* *running = batches not finished after the step. */
+la3_status la3_session_step(la3_session* s, uint32_t* running);
+/* Pops up to cap finished batches: ids, iters_run and status (may be NULL). */
+la3_status la3_session_poll(la3_session* s, uint64_t* ids, uint32_t* iters_run, int32_t* status,
+                            uint32_t cap, uint32_t* n);
+uint32_t la3_session_free_slots(const la3_session* s);
+void la3_session_destroy(la3_session* s);
+
 /* ------------------------------------------------------------- tuning --- */
 /* SSSP relaxes in near/far rounds: only (vertex, query) pairs whose distance
  * is below a threshold T are relaxed; the others stay in the frontier, and
@@ -261,11 +282,12 @@ typedef struct {
   int keep_results;       /* copy each result to host and keep it for la3_server_result */
   double ppr_alpha;       /* teleport probability of PPR queries */
   int fused;              /* 0: a batch is one (kind, graph, iters) group run by
-                             la3_*_batch_ctx; 1: a batch takes pending queries of
-                             every kind and graph in arrival order, grouped by
-                             (kind, graph, iters), each group up to max_batch, and
-                             runs them as one la3_batch_run (one fused launch per
-                             step across graphs and programs) */
+                             la3_*_batch_ctx; 1: each worker keeps a session
+                             (la3_session_*): between replays of 4 steps it admits
+                             pending queries as new (kind, graph, iters) groups of up
+                             to max_batch while task slots and budget allow, and
+                             retires finished groups, so queries of every program and
+                             graph share each fused launch */
 } la3_server_config;
 typedef struct {
   uint64_t qid;

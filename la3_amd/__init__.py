@@ -36,7 +36,8 @@ EXPORTS = [
     "la3_server_create", "la3_server_add_graph", "la3_server_query_bytes", "la3_server_submit",
     "la3_server_drain", "la3_server_completed", "la3_server_result", "la3_server_clock",
     "la3_server_destroy", "la3_graph_set_tuning", "la3_graph_get_tuning", "la3_autotune",
-    "la3_batch_run",
+    "la3_batch_run", "la3_session_create", "la3_session_add", "la3_session_step",
+    "la3_session_poll", "la3_session_free_slots", "la3_session_destroy",
 ]
 TUNE_SSSP_DELTA, TUNE_BATCH_SSSP_DELTA = 0, 1
 Q_BFS, Q_SSSP, Q_PPR = 0, 1, 2
@@ -153,6 +154,12 @@ def lib():
             "la3_graph_get_tuning": (i32, [P, i32, ctypes.POINTER(u32)]),
             "la3_autotune": (i32, [P, P, u32, ctypes.POINTER(AutotuneReport)]),
             "la3_batch_run": (i32, [P, ctypes.POINTER(Batch), u32, ctypes.c_double]),
+            "la3_session_create": (i32, [P, ctypes.c_double, ctypes.POINTER(P)]),
+            "la3_session_add": (i32, [P, ctypes.POINTER(Batch), ctypes.POINTER(u64)]),
+            "la3_session_step": (i32, [P, ctypes.POINTER(u32)]),
+            "la3_session_poll": (i32, [P, P, P, P, u32, ctypes.POINTER(u32)]),
+            "la3_session_free_slots": (u32, [P]),
+            "la3_session_destroy": (None, [P]),
             "la3_rmat_generate": (i32, [u32, u32, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, u64, i32, P, ctypes.POINTER(u64)]),
         }
@@ -478,6 +485,64 @@ def batch_run(ctx, batches, alpha=0.15, out=None):
     for i, r in enumerate(res):
         r["iters"] = int(arr[i].iters_run)
     return res
+
+
+class Session:
+    """Continuous batching at step granularity (la3_session_*): batches join
+    a running fused run with add(), advance with step(), and finished ones
+    come back from poll()."""
+
+    def __init__(self, ctx, alpha=0.15):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(lib().la3_session_create(ctx.h, float(alpha), ctypes.byref(h)), "session_create")
+        self.h = h
+        self._keep = {}
+
+    def add(self, g, kind, roots, iters=0, out=None):
+        """Adds a batch; returns (id, result array or None). The array is
+        filled when the batch finishes. Raises LA3Error(EAGAIN) if it does
+        not fit the free task slots."""
+        roots = np.ascontiguousarray(roots, dtype=np.uint32)
+        dt = {Q_BFS: np.uint8, Q_SSSP: np.uint32, Q_PPR: np.float32}[kind]
+        o = _out(out, (roots.size, g.nslots), dt)
+        b = Batch()
+        b.graph, b.kind, b.k, b.roots, b.iters = g.h, kind, roots.size, roots.ctypes.data, int(iters)
+        b.out = o.ctypes.data if o is not None else None
+        bid = ctypes.c_uint64()
+        self.ctx.check(lib().la3_session_add(self.h, ctypes.byref(b), ctypes.byref(bid)), "session_add")
+        self._keep[bid.value] = (roots, o)
+        return bid.value, o
+
+    def step(self):
+        """Runs one replay of steps; returns the number of running batches."""
+        n = ctypes.c_uint32()
+        self.ctx.check(lib().la3_session_step(self.h, ctypes.byref(n)), "session_step")
+        return n.value
+
+    def poll(self, cap=256):
+        """Finished batches: list of (id, iters_run, status)."""
+        ids = np.zeros(cap, dtype=np.uint64)
+        its = np.zeros(cap, dtype=np.uint32)
+        sts = np.zeros(cap, dtype=np.int32)
+        n = ctypes.c_uint32()
+        self.ctx.check(lib().la3_session_poll(self.h, _ptr(ids), _ptr(its), _ptr(sts), cap, ctypes.byref(n)),
+                       "session_poll")
+        out = [(int(ids[i]), int(its[i]), int(sts[i])) for i in range(n.value)]
+        for i, _, _ in out:
+            self._keep.pop(i, None)
+        return out
+
+    def free_slots(self):
+        return int(lib().la3_session_free_slots(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().la3_session_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Server:

@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <string>
@@ -53,6 +54,7 @@ struct la3_ctx {
   FTask* fz_tasks = nullptr;
   uint8_t* fz_ctl = nullptr;
   hipGraphExec_t fz_exec[64] = {};  // by launch mask (fz_step)
+  bool fz_busy = false;             // a session holds the task table
 };
 
 namespace {
@@ -142,6 +144,10 @@ uint32_t blocks_for(uint64_t threads, uint32_t cap = 8192) {
 
 // Starts the timing window of one API call.
 void timing_begin(la3_ctx* c) {
+  for (auto& e : c->evs) {  // launches outside any timing window
+    c->free_events.push_back(e.a);
+    c->free_events.push_back(e.b);
+  }
   c->evs.clear();
   c->edges_scanned = 0;
 }
@@ -1523,12 +1529,178 @@ la3_status fz_capture(la3_ctx* c, uint32_t mask) {
   return LA3_OK;
 }
 
-la3_status fz_run(la3_ctx* c, la3_batch* batches, uint32_t nb, double alpha) {
-  std::vector<FzTask> tasks;
-  for (uint32_t i = 0; i < nb; i++)
-    for (uint32_t q0 = 0; q0 < batches[i].k; q0 += 64 * FMAX_WIDTH)
-      tasks.push_back({i, q0, std::min<uint32_t>(64 * FMAX_WIDTH, batches[i].k - q0)});
-  if (tasks.size() > FMAX_TASKS) return fail(c, LA3_EINVAL, "fused run: more than 64 tasks of 256 queries");
+la3_status fz_check(la3_ctx* c, const la3_batch& b, double alpha) {
+  la3_graph* g = b.graph;
+  if (!g || !b.roots) return fail(c, LA3_EINVAL, "batch without graph or roots");
+  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+  if (g->ctx->device != c->device) return fail(c, LA3_EINVAL, "graph lives on another device");
+  if (b.k == 0 || b.k > 1024) return fail(c, LA3_EINVAL, "batch size must be in [1, 1024]");
+  if (b.kind != LA3_Q_BFS && b.kind != LA3_Q_SSSP && b.kind != LA3_Q_PPR)
+    return fail(c, LA3_EINVAL, "unknown query kind");
+  if (b.kind == LA3_Q_SSSP && !g->weighted) return fail(c, LA3_EINVAL, "SSSP needs a weighted graph");
+  if (b.kind != LA3_Q_BFS && !g->directed) return fail(c, LA3_EINVAL, "SSSP and PPR run on a directed load");
+  if (b.kind == LA3_Q_PPR && b.iters == 0) return fail(c, LA3_EINVAL, "PPR needs a fixed iteration count");
+  if (b.kind == LA3_Q_PPR && !(alpha > 0.0 && alpha < 1.0)) return fail(c, LA3_EINVAL, "alpha must be in (0, 1)");
+  for (uint32_t q = 0; q < b.k; q++)
+    if (b.roots[q] > g->n) return fail(c, LA3_EINVAL, "root id out of range");
+  return LA3_OK;
+}
+
+struct FzBatch {
+  la3_batch b;
+  std::vector<uint32_t> roots;
+  uint32_t tasks_left = 0, iters_run = 0;
+  la3_status st = LA3_OK;
+};
+
+}  // namespace
+
+// A fused run that batches join and leave between steps (include/la3.h).
+struct la3_session {
+  la3_ctx* c = nullptr;
+  double alpha = 0.15;
+  struct Slot {
+    bool used = false;
+    uint64_t batch = 0;
+    FzTask ft{};
+    uint64_t max_steps = 0;
+  };
+  Slot slot[FMAX_TASKS];
+  FTask ht[FMAX_TASKS];     // host copies of the task table (staging for uploads)
+  FCtl hc[FMAX_TASKS];      // control blocks read back after each replay
+  FCtl fresh[FMAX_TASKS];   // staging for control blocks written by the host
+  FHdr hdr{};               // staging for the header
+  uint32_t ntasks = 0;      // slots [0, ntasks) are known to the device
+  bool pending_init = false;
+  std::map<uint64_t, FzBatch> batches;
+  std::deque<uint64_t> done;
+  uint64_t next_id = 1;
+};
+
+namespace {
+
+uint32_t sess_free_slots(const la3_session* s) {
+  uint32_t n = 0;
+  for (const auto& sl : s->slot) n += !sl.used;
+  return n;
+}
+
+la3_status sess_add(la3_session* s, const la3_batch& in, uint64_t* id) {
+  la3_ctx* c = s->c;
+  la3_status st;
+  if ((st = fz_check(c, in, s->alpha))) return st;
+  const uint32_t per = 64 * FMAX_WIDTH;
+  const uint32_t need = (in.k + per - 1) / per;
+  if (need > sess_free_slots(s)) return fail(c, LA3_EAGAIN, "session: no free task slots");
+  const uint64_t bid = s->next_id++;
+  FzBatch& fb = s->batches[bid];
+  fb.b = in;
+  fb.roots.assign(in.roots, in.roots + in.k);
+  fb.b.roots = fb.roots.data();
+  fb.tasks_left = need;
+  FHdr* dh = (FHdr*)c->fz_ctl;
+  FCtl* dc = (FCtl*)(c->fz_ctl + sizeof(FHdr));
+  uint32_t t = 0;
+  for (uint32_t q0 = 0; q0 < in.k; q0 += per) {
+    while (s->slot[t].used) t++;
+    la3_session::Slot& sl = s->slot[t];
+    sl.used = true;
+    sl.batch = bid;
+    sl.ft = FzTask{0, q0, std::min(per, in.k - q0)};
+    if ((st = fz_setup(c, fb.b, sl.ft, t, s->alpha, s->ht[t]))) return st;
+    const FTask& T = s->ht[t];
+    // Safety net against a run that does not terminate: far more steps than
+    // any traversal here takes (BFS <= nslots + 1; SSSP rounds are bounded
+    // by the near/far phases; PPR = iters).
+    sl.max_steps = T.kind == FK_PPR ? (uint64_t)T.iters + 8 : 8ull * T.g.nslots + 4096;
+    s->fresh[t] = FCtl{};
+    s->fresh[t].state = FS_NEW;
+    HIPC(c, hipMemcpyAsync(c->fz_tasks + t, &s->ht[t], sizeof(FTask), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(dc + t, &s->fresh[t], sizeof(FCtl), hipMemcpyHostToDevice, c->stream));
+    s->ntasks = std::max(s->ntasks, t + 1);
+  }
+  s->hdr.ntasks = s->ntasks;
+  HIPC(c, hipMemcpyAsync(&dh->ntasks, &s->hdr.ntasks, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  s->pending_init = true;
+  if (id) *id = bid;
+  return LA3_OK;
+}
+
+// One replay (FZ_STEPS steps) of the running tasks, then the post-loop work
+// of every task that finished; a batch completes when its last task does.
+la3_status sess_step(la3_session* s) {
+  la3_ctx* c = s->c;
+  FHdr* dh = (FHdr*)c->fz_ctl;
+  FCtl* dc = (FCtl*)(c->fz_ctl + sizeof(FHdr));
+  la3_status st;
+  if (s->pending_init) {
+    launch(c, "fused_c", k_fused_c, 1, (const FTask*)c->fz_tasks, dh, dc, 1u);
+    s->pending_init = false;
+  }
+  uint32_t mask = 0;
+  for (uint32_t t = 0; t < s->ntasks; t++) {
+    if (!s->slot[t].used) continue;
+    const uint32_t w = s->ht[t].width;
+    const uint32_t wb = w == 1 ? 1u : w == 2 ? 2u : 4u;
+    mask |= wb | (s->ht[t].kind == FK_BFS ? wb << 3 : 0u);
+  }
+  int last_a = -1, last_pull = -1;
+  if (mask) {
+    if (!c->profiling) {
+      if ((st = fz_capture(c, mask))) return st;
+      HIPC(c, hipGraphLaunch(c->fz_exec[mask], c->stream));
+    } else {
+      // Profiling: the same steps as separate launches, each bracketed by
+      // events (per-kernel device time; the graph has none).
+      for (uint32_t i = 0; i < FZ_STEPS; i++) fz_step(c, mask, true, &last_a, &last_pull);
+    }
+  }
+  HIPC(c, hipMemcpyAsync(s->hc, dc, sizeof(FCtl) * s->ntasks, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  double bytes = 0, pbytes = 0;
+  uint64_t edges = 0;
+  bool finished = false;
+  for (uint32_t t = 0; t < s->ntasks; t++) {
+    la3_session::Slot& sl = s->slot[t];
+    if (!sl.used) continue;
+    const FCtl& C = s->hc[t];
+    FzBatch& fb = s->batches[sl.batch];
+    if (C.state == FS_RUN && !C.done && C.iter > sl.max_steps)
+      return fail(c, LA3_EDEVICE, "fused run: step limit reached");
+    if (C.state != FS_RUN || !C.done) continue;
+    bytes += C.bytes - C.pbytes;
+    pbytes += C.pbytes;
+    edges += C.edges;
+    const la3_query_kind kind = fb.b.kind;
+    const uint32_t it = kind == LA3_Q_BFS ? std::max<uint32_t>(C.iter, 1) : kind == LA3_Q_SSSP ? C.iter + 1 : C.iter;
+    fb.iters_run = std::max(fb.iters_run, it);
+    const uint32_t W = s->ht[t].width;
+    st = W == 1 ? fz_finish<1>(c, fb.b, sl.ft, t, s->ht[t], C)
+         : W == 2 ? fz_finish<2>(c, fb.b, sl.ft, t, s->ht[t], C)
+                  : fz_finish<4>(c, fb.b, sl.ft, t, s->ht[t], C);
+    if (st) fb.st = st;
+    // The slot is free again: the device skips it from now on.
+    s->fresh[t] = FCtl{};
+    s->fresh[t].state = FS_FREE;
+    s->fresh[t].done = 1;
+    HIPC(c, hipMemcpyAsync(dc + t, &s->fresh[t], sizeof(FCtl), hipMemcpyHostToDevice, c->stream));
+    sl.used = false;
+    if (--fb.tasks_left == 0) s->done.push_back(sl.batch);
+    finished = true;
+  }
+  // Bytes of the finished tasks: bottom-up BFS steps to k_fused_pull, the
+  // rest to k_fused_a (the B and control launches move no counted bytes).
+  annotate(c, last_a, bytes, edges);
+  annotate(c, last_pull, pbytes, 0);
+  c->edges_scanned += edges;
+  if (finished) HIPC(c, hipStreamSynchronize(c->stream));
+  return LA3_OK;
+}
+
+la3_status sess_init(la3_ctx* c, double alpha, la3_session* s) {
+  if (c->device < 0) return fail(c, LA3_EDEVICE, "host-only context");
+  if (c->fz_busy) return fail(c, LA3_EAGAIN, "the context already runs a session");
+  HIPC(c, hipSetDevice(c->device));
   if (!c->fz_tasks) {
     c->fz_tasks = (FTask*)scratch(c, "fz.tasks", sizeof(FTask) * FMAX_TASKS);
     c->fz_ctl = (uint8_t*)scratch(c, "fz.ctl", sizeof(FHdr) + sizeof(FCtl) * FMAX_TASKS);
@@ -1537,79 +1709,21 @@ la3_status fz_run(la3_ctx* c, la3_batch* batches, uint32_t nb, double alpha) {
       return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
     }
   }
-  const uint32_t nt = (uint32_t)tasks.size();
-  std::vector<FTask> ht(nt);
-  la3_status st;
-  for (uint32_t t = 0; t < nt; t++)
-    if ((st = fz_setup(c, batches[tasks[t].batch], tasks[t], t, alpha, ht[t]))) return st;
-  FHdr h{};
-  h.ntasks = nt;
-  h.ga = fz_ga(c);
-  h.gb = fz_gb(c);
-  HIPC(c, hipMemcpyAsync(c->fz_tasks, ht.data(), sizeof(FTask) * nt, hipMemcpyHostToDevice, c->stream));
-  HIPC(c, hipMemcpyAsync(c->fz_ctl, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
-  FHdr* dh = (FHdr*)c->fz_ctl;
-  FCtl* dc = (FCtl*)(c->fz_ctl + sizeof(FHdr));
-  launch(c, "fused_init", k_fused_c, 1, (const FTask*)c->fz_tasks, dh, dc, 1u);
-  uint32_t mask = 0;
-  for (uint32_t t = 0; t < nt; t++) {
-    const uint32_t wb = ht[t].width == 1 ? 1u : ht[t].width == 2 ? 2u : 4u;
-    mask |= wb | (ht[t].kind == FK_BFS ? wb << 3 : 0u);
-  }
-  if (!c->profiling && (st = fz_capture(c, mask))) return st;
-  // Safety net against a run that does not terminate: far more steps than
-  // any traversal here takes (BFS <= nslots + 1; SSSP rounds are bounded by
-  // the near/far phases; PPR = iters).
-  uint64_t max_steps = 4096;
-  for (uint32_t t = 0; t < nt; t++)
-    max_steps = std::max<uint64_t>(max_steps, ht[t].kind == FK_PPR ? (uint64_t)ht[t].iters + 8
-                                                                   : 8ull * ht[t].g.nslots + 4096);
-  int last_ev = -1, last_pull = -1;
-  for (uint64_t steps = 0;; steps += FZ_STEPS) {
-    HIPC(c, hipMemcpyAsync(c->host_ctr, dh, sizeof(FHdr), hipMemcpyDeviceToHost, c->stream));
-    HIPC(c, hipStreamSynchronize(c->stream));
-    FHdr now;
-    std::memcpy(&now, c->host_ctr, sizeof(now));
-    if (now.active == 0) break;
-    if (steps >= max_steps) return fail(c, LA3_EDEVICE, "fused run: step limit reached");
-    if (!c->profiling) {
-      HIPC(c, hipGraphLaunch(c->fz_exec[mask], c->stream));
-      continue;
-    }
-    // Profiling: the same FZ_STEPS steps as separate launches, each
-    // bracketed by events (per-kernel device time; the graph has none).
-    for (uint32_t i = 0; i < FZ_STEPS; i++) fz_step(c, mask, true, &last_ev, &last_pull);
-  }
-  std::vector<FCtl> hc(nt);
-  HIPC(c, hipMemcpyAsync(hc.data(), dc, sizeof(FCtl) * nt, hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipStreamSynchronize(c->stream));
-  double bytes = 0, pbytes = 0;
-  uint64_t edges = 0;
-  for (uint32_t i = 0; i < nb; i++) batches[i].iters_run = 0;
-  for (uint32_t t = 0; t < nt; t++) {
-    bytes += hc[t].bytes - hc[t].pbytes;
-    pbytes += hc[t].pbytes;
-    edges += hc[t].edges;
-    la3_batch& b = batches[tasks[t].batch];
-    const uint32_t it = b.kind == LA3_Q_BFS ? std::max<uint32_t>(hc[t].iter, 1)
-                        : b.kind == LA3_Q_SSSP ? hc[t].iter + 1 : hc[t].iter;
-    b.iters_run = std::max(b.iters_run, it);
-  }
-  // The run's algorithmic bytes: bottom-up BFS steps to k_fused_pull, the
-  // rest to k_fused_a (the B and control launches move no counted bytes).
-  annotate(c, last_ev, bytes, edges);
-  annotate(c, last_pull, pbytes, 0);
-  c->edges_scanned += edges;
-  for (uint32_t t = 0; t < nt; t++) {
-    const la3_batch& b = batches[tasks[t].batch];
-    const uint32_t W = ht[t].width;
-    st = W == 1 ? fz_finish<1>(c, b, tasks[t], t, ht[t], hc[t])
-         : W == 2 ? fz_finish<2>(c, b, tasks[t], t, ht[t], hc[t])
-                  : fz_finish<4>(c, b, tasks[t], t, ht[t], hc[t]);
-    if (st) return st;
-  }
-  HIPC(c, hipStreamSynchronize(c->stream));
+  s->c = c;
+  s->alpha = alpha;
+  s->hdr = FHdr{};
+  s->hdr.ga = fz_ga(c);
+  s->hdr.gb = fz_gb(c);
+  HIPC(c, hipMemcpyAsync(c->fz_ctl, &s->hdr, sizeof(FHdr), hipMemcpyHostToDevice, c->stream));
+  c->fz_busy = true;
   return LA3_OK;
+}
+
+void sess_release(la3_session* s) {
+  if (!s->c) return;
+  (void)hipStreamSynchronize(s->c->stream);
+  s->c->fz_busy = false;
+  s->c = nullptr;
 }
 
 }  // namespace
@@ -1618,30 +1732,86 @@ extern "C" {
 
 la3_status la3_batch_run(la3_ctx* c, la3_batch* batches, uint32_t nbatches, double ppr_alpha) {
   if (!c || !batches || nbatches == 0) return LA3_EINVAL;
-  if (c->device < 0) return fail(c, LA3_EDEVICE, "host-only context");
-  bool ppr = false;
+  uint32_t need = 0;
   for (uint32_t i = 0; i < nbatches; i++) {
-    const la3_batch& b = batches[i];
-    la3_graph* g = b.graph;
-    if (!g || !b.roots) return fail(c, LA3_EINVAL, "batch without graph or roots");
-    if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
-    if (g->ctx->device != c->device) return fail(c, LA3_EINVAL, "graph lives on another device");
-    if (b.k == 0 || b.k > 1024) return fail(c, LA3_EINVAL, "batch size must be in [1, 1024]");
-    if (b.kind != LA3_Q_BFS && b.kind != LA3_Q_SSSP && b.kind != LA3_Q_PPR)
-      return fail(c, LA3_EINVAL, "unknown query kind");
-    if (b.kind == LA3_Q_SSSP && !g->weighted) return fail(c, LA3_EINVAL, "SSSP needs a weighted graph");
-    if (b.kind != LA3_Q_BFS && !g->directed) return fail(c, LA3_EINVAL, "SSSP and PPR run on a directed load");
-    if (b.kind == LA3_Q_PPR && b.iters == 0) return fail(c, LA3_EINVAL, "PPR needs a fixed iteration count");
-    ppr |= b.kind == LA3_Q_PPR;
-    for (uint32_t q = 0; q < b.k; q++)
-      if (b.roots[q] > g->n) return fail(c, LA3_EINVAL, "root id out of range");
+    la3_status st = fz_check(c, batches[i], ppr_alpha);
+    if (st) return st;
+    need += (batches[i].k + 64 * FMAX_WIDTH - 1) / (64 * FMAX_WIDTH);
   }
-  if (ppr && !(ppr_alpha > 0.0 && ppr_alpha < 1.0)) return fail(c, LA3_EINVAL, "alpha must be in (0, 1)");
+  if (need > FMAX_TASKS) return fail(c, LA3_EINVAL, "fused run: more than 64 tasks of 256 queries");
+  auto s = std::make_unique<la3_session>();
+  la3_status st = sess_init(c, ppr_alpha, s.get());
+  if (st) return st;
+  timing_begin(c);
+  std::vector<uint64_t> ids(nbatches);
+  for (uint32_t i = 0; i < nbatches && !st; i++) st = sess_add(s.get(), batches[i], &ids[i]);
+  while (!st && s->done.size() < nbatches) st = sess_step(s.get());
+  timing_end(c);
+  for (uint32_t i = 0; i < nbatches && !st; i++) {
+    const FzBatch& fb = s->batches[ids[i]];
+    batches[i].iters_run = fb.iters_run;
+    if (fb.st) st = fb.st;
+  }
+  sess_release(s.get());
+  return st;
+}
+
+la3_status la3_session_create(la3_ctx* c, double ppr_alpha, la3_session** out) {
+  if (!c || !out) return LA3_EINVAL;
+  if (!(ppr_alpha > 0.0 && ppr_alpha < 1.0)) return fail(c, LA3_EINVAL, "alpha must be in (0, 1)");
+  auto s = std::make_unique<la3_session>();
+  la3_status st = sess_init(c, ppr_alpha, s.get());
+  if (st) return st;
+  *out = s.release();
+  return LA3_OK;
+}
+
+la3_status la3_session_add(la3_session* s, const la3_batch* batch, uint64_t* id) {
+  if (!s || !s->c || !batch) return LA3_EINVAL;
+  HIPC(s->c, hipSetDevice(s->c->device));
+  return sess_add(s, *batch, id);
+}
+
+la3_status la3_session_step(la3_session* s, uint32_t* running) {
+  if (!s || !s->c) return LA3_EINVAL;
+  la3_ctx* c = s->c;
   HIPC(c, hipSetDevice(c->device));
   timing_begin(c);
-  la3_status st = fz_run(c, batches, nbatches, ppr_alpha);
+  la3_status st = LA3_OK;
+  if (s->batches.size() > s->done.size()) st = sess_step(s);
   timing_end(c);
+  if (running) {
+    uint32_t n = 0;
+    for (const auto& kv : s->batches) n += kv.second.tasks_left > 0;
+    *running = n;
+  }
   return st;
+}
+
+la3_status la3_session_poll(la3_session* s, uint64_t* ids, uint32_t* iters_run, int32_t* status, uint32_t cap,
+                            uint32_t* n) {
+  if (!s || !n || (cap && !ids)) return LA3_EINVAL;
+  uint32_t i = 0;
+  while (i < cap && !s->done.empty()) {
+    const uint64_t id = s->done.front();
+    s->done.pop_front();
+    auto it = s->batches.find(id);
+    ids[i] = id;
+    if (iters_run) iters_run[i] = it->second.iters_run;
+    if (status) status[i] = it->second.st;
+    s->batches.erase(it);
+    i++;
+  }
+  *n = i;
+  return LA3_OK;
+}
+
+uint32_t la3_session_free_slots(const la3_session* s) { return s ? sess_free_slots(s) : 0; }
+
+void la3_session_destroy(la3_session* s) {
+  if (!s) return;
+  sess_release(s);
+  delete s;
 }
 
 }  // extern "C"

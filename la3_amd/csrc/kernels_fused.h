@@ -50,11 +50,14 @@ struct FTask {
 };
 
 // Mutable per-task control word block, written by k_fused_c.
+enum : uint32_t { FS_FREE = 0, FS_NEW = 1, FS_RUN = 2 };
+
 struct FCtl {
   uint32_t iter, cur, done, bottom_up;
   uint32_t thr, nitems, a_beg, a_end;
   uint32_t b_beg, b_end, p_beg, p_end;   // block shares: A, B and pull launches
-  uint32_t fcols, pad[3];
+  uint32_t fcols, state, useful, pad;    // state: FS_*; useful: blocks the next A/pull can use
+  double est;                  // estimated bytes of the next step (block split)
   unsigned long long fedges;   // entries behind the current work list
   unsigned long long edges;    // entries scanned over the run
   double bytes;                // algorithmic bytes over the run
@@ -230,12 +233,13 @@ __device__ __forceinline__ void fzero(const Frontier& f) {
   for (uint32_t i = lane_id(); i < CTR_WORDS; i += WAVE) w[i] = 0;
 }
 
-// Control step (one block of BLOCK threads). init != 0: set up from the
-// seeded work lists; else account for the step just run and advance.
+// Control step (one block of BLOCK threads). Tasks in state FS_NEW are set
+// up from their seeded work lists and become FS_RUN; with init_only == 0,
+// running tasks account for the step just run and advance (done tasks and
+// FS_FREE slots are skipped). Then the blocks of the next launches are split
+// among the running tasks.
 __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tasks, FHdr* hdr, FCtl* ctl,
-                                                   uint32_t init) {
-  __shared__ double est[FMAX_TASKS];
-  __shared__ uint32_t useful[FMAX_TASKS];
+                                                   uint32_t init_only) {
   __shared__ uint32_t fin[FMAX_TASKS];   // PPR tasks whose last iteration just ran
   const uint32_t nt = hdr->ntasks;
   const uint32_t lane = lane_id();
@@ -245,10 +249,9 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
     const DG& g = T.g;
     const uint32_t K = 64 * T.width;
     if (lane == 0) fin[t] = 0;
-    if (!init && C.done) {
-      if (lane == 0) { est[t] = 0; useful[t] = 0; }
-      continue;
-    }
+    const uint32_t state = C.state;
+    const bool init = state == FS_NEW;
+    if (state == FS_FREE || (state == FS_RUN && (C.done || init_only))) continue;
     if (T.kind == FK_PPR) {
       if (lane == 0) {
         if (!init) {
@@ -257,12 +260,13 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
           C.iter++;
           C.cur ^= 1;
         } else {
-          C.iter = 0; C.cur = 0; C.bytes = 0; C.pbytes = 0; C.edges = 0;
+          C.iter = 0; C.cur = 0; C.bytes = 0; C.pbytes = 0; C.edges = 0; C.bottom_up = 0;
         }
         C.done = C.iter >= T.iters;
+        C.state = FS_RUN;
         fin[t] = !init && C.done;
-        est[t] = C.done ? 0.0 : (4.0 + 4.0 * T.k) * (double)g.nnz + 20.0 * T.k * (double)g.nreg;
-        useful[t] = C.done ? 0 : (g.nreg + 3) / 4 + (g.npseg_reg + 3) / 4;
+        C.est = C.done ? 0.0 : (4.0 + 4.0 * T.k) * (double)g.nnz + 20.0 * T.k * (double)g.nreg;
+        C.useful = C.done ? 0 : (g.nreg + 3) / 4 + (g.npseg_reg + 3) / 4;
       }
       continue;
     }
@@ -315,10 +319,11 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
       C.fedges = ns.edges;
       C.fcols = ns.cols;
       C.done = nitems == 0;
+      C.state = FS_RUN;
       const double K8 = T.kind == FK_BFS ? 4.0 + T.k / 8.0 : 8.0 + 4.0 * K;
       const bool pull = T.kind == FK_BFS && C.bottom_up;
-      est[t] = C.done ? 0.0 : (pull ? K8 * (double)g.nnz / 4 : K8 * (double)ns.edges + 64.0 * nitems);
-      useful[t] = C.done ? 0 : (pull ? (g.nrows + BLOCK - 1) / BLOCK + (g.nlong + 3) / 4 : (nitems + 3) / 4);
+      C.est = C.done ? 0.0 : (pull ? K8 * (double)g.nnz / 4 : K8 * (double)ns.edges + 64.0 * nitems);
+      C.useful = C.done ? 0 : (pull ? (g.nrows + BLOCK - 1) / BLOCK + (g.nlong + 3) / 4 : (nitems + 3) / 4);
     }
   }
   __syncthreads();
@@ -340,30 +345,33 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
   }
   __syncthreads();  // the partitions read above are rewritten below
   if (threadIdx.x == 0) {
-    // Blocks of each width class's launches, split among its tasks: A (and,
-    // for bottom-up BFS steps, the pull launch) in proportion to the
+    // Blocks of each width class's launches, split among its running tasks:
+    // A (and, for bottom-up BFS steps, the pull launch) in proportion to the
     // estimated bytes (at least 1, at most the useful count), B evenly.
     double tot[5] = {0, 0, 0, 0, 0}, totp[5] = {0, 0, 0, 0, 0};
     uint32_t act[5] = {0, 0, 0, 0, 0}, a[5] = {0, 0, 0, 0, 0}, b[5] = {0, 0, 0, 0, 0};
     uint32_t pp[5] = {0, 0, 0, 0, 0};
     uint32_t active = 0;
     for (uint32_t t = 0; t < nt; t++) {
-      const uint32_t w = tasks[t].width;
-      const bool pull = tasks[t].kind == FK_BFS && ctl[t].bottom_up;
-      (pull ? totp : tot)[w] += est[t];
-      act[w] += ctl[t].done ? 0 : 1;
-      active += ctl[t].done ? 0 : 1;
+      const FCtl& C = ctl[t];
+      if (C.state != FS_RUN || C.done) continue;
+      const uint32_t w = min(tasks[t].width, 4u);
+      const bool pull = tasks[t].kind == FK_BFS && C.bottom_up;
+      (pull ? totp : tot)[w] += C.est;
+      act[w]++;
+      active++;
     }
     const uint32_t ga = hdr->ga, gb = hdr->gb;
     for (uint32_t t = 0; t < nt; t++) {
       FCtl& C = ctl[t];
-      const uint32_t w = tasks[t].width;
-      const bool pull = tasks[t].kind == FK_BFS && C.bottom_up;
+      const bool run = C.state == FS_RUN && !C.done;
+      const uint32_t w = run ? min(tasks[t].width, 4u) : 0;
+      const bool pull = run && tasks[t].kind == FK_BFS && C.bottom_up;
       uint32_t na = 0, nb = 0, np = 0;
-      if (!C.done) {
+      if (run) {
         const double tw = pull ? totp[w] : tot[w];
-        uint32_t n = tw > 0 ? (uint32_t)(ga * (est[t] / tw)) : 1;
-        n = max(1u, min(n, max(1u, useful[t])));
+        uint32_t n = tw > 0 ? (uint32_t)(ga * (C.est / tw)) : 1;
+        n = max(1u, min(n, max(1u, C.useful)));
         (pull ? np : na) = n;
         nb = max(1u, gb / max(act[w], 1u));
       }
@@ -372,7 +380,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
       C.b_beg = b[w]; C.b_end = b[w] + nb; b[w] += nb;
     }
     hdr->active = active;
-    if (!init && active) hdr->steps++;
+    if (!init_only && active) hdr->steps++;
   }
 }
 

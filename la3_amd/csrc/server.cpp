@@ -70,117 +70,168 @@ uint64_t batch_bytes(const la3_server* s, la3_query_kind k, uint32_t graph, uint
 
 size_t elem_bytes(la3_query_kind k) { return k == LA3_Q_BFS ? 1 : 4; }
 
-// Mixed admission (cfg.fused): pending queries in arrival order join groups
-// keyed by (kind, graph, iters), each up to max_batch, while the admitted
-// state bytes fit the budget (a lone query is admitted when nothing runs)
-// and the groups fit one fused run (at most 64 tasks of 256 queries).
-// Called with the lock held; moves the picked queries into `groups`.
-uint64_t pick_mixed(la3_server* s, std::vector<std::vector<Query>>& groups) {
-  struct G { la3_query_kind kind; uint32_t graph, iters; std::vector<size_t> idx; };
-  std::vector<G> gs;
-  uint64_t bytes = 0;
-  uint32_t tasks = 0;
-  for (size_t i = 0; i < s->pending.size(); i++) {
+// Takes, with the lock held, the oldest pending query and the later pending
+// queries with its (kind, graph, iters), up to `cap`, while the admitted state
+// bytes fit the budget (a lone query is admitted when nothing runs). Returns
+// the picked queries (arrival order) and their state bytes.
+std::vector<Query> pick_group(la3_server* s, uint32_t cap, uint64_t& bytes) {
+  std::vector<Query> batch;
+  bytes = 0;
+  if (s->pending.empty() || cap == 0) return batch;
+  const Query head = s->pending.front();
+  std::vector<size_t> pick;
+  for (size_t i = 0; i < s->pending.size() && pick.size() < cap; i++) {
     const Query& q = s->pending[i];
-    G* gp = nullptr;
-    for (G& g : gs)
-      if (g.kind == q.kind && g.graph == q.graph && g.iters == q.iters) gp = &g;
-    const uint32_t have = gp ? (uint32_t)gp->idx.size() : 0;
-    if (have >= s->cfg.max_batch) continue;
-    const uint64_t db = batch_bytes(s, q.kind, q.graph, have + 1) - (have ? batch_bytes(s, q.kind, q.graph, have) : 0);
-    const uint32_t dt = (have % 256 == 0) ? 1 : 0;
-    const bool first = gs.empty();
-    if (!first && (s->in_flight_bytes + bytes + db > s->cfg.state_budget || tasks + dt > 64)) break;
-    if (first && s->in_flight_bytes + db > s->cfg.state_budget && s->running_batches != 0) break;
-    if (!gp) {
-      gs.push_back({q.kind, q.graph, q.iters, {}});
-      gp = &gs.back();
-    }
-    gp->idx.push_back(i);
-    bytes += db;
-    tasks += dt;
+    if (q.kind != head.kind || q.graph != head.graph || q.iters != head.iters) continue;
+    const uint64_t b = batch_bytes(s, head.kind, head.graph, (uint32_t)pick.size() + 1);
+    if (s->in_flight_bytes + b > s->cfg.state_budget && !(pick.empty() && s->running_batches == 0)) break;
+    pick.push_back(i);
   }
-  if (gs.empty()) return 0;
-  std::vector<size_t> all;
-  for (G& g : gs) {
-    std::vector<Query> v;
-    for (size_t i : g.idx) { v.push_back(s->pending[i]); all.push_back(i); }
-    groups.push_back(std::move(v));
+  if (pick.empty()) return batch;
+  bytes = batch_bytes(s, head.kind, head.graph, (uint32_t)pick.size());
+  for (size_t j = pick.size(); j-- > 0;) {
+    batch.push_back(s->pending[pick[j]]);
+    s->pending.erase(s->pending.begin() + pick[j]);
   }
-  std::sort(all.begin(), all.end());
-  for (size_t j = all.size(); j-- > 0;) s->pending.erase(s->pending.begin() + all[j]);
-  return bytes;
+  std::reverse(batch.begin(), batch.end());
+  s->in_flight_bytes += bytes;
+  s->running_batches++;
+  return batch;
 }
 
-// Runs the picked groups as one fused batch run and records completions.
-void run_mixed(la3_server* s, la3_ctx* ctx, std::vector<std::vector<Query>>& groups, uint64_t bytes,
-               void*& host, uint64_t& host_bytes) {
-  const double start = s->now();
-  std::vector<la3_batch> bs(groups.size());
-  std::vector<std::vector<uint32_t>> roots(groups.size());
-  std::vector<size_t> off(groups.size());
-  uint64_t need = 0;
-  for (size_t i = 0; i < groups.size(); i++) {
-    const Query& h = groups[i].front();
-    off[i] = need;
-    need += (uint64_t)groups[i].size() * s->graphs[h.graph].nslots * elem_bytes(h.kind);
+// Records the completion of one batch's queries (lock held).
+void complete(la3_server* s, const std::vector<Query>& qs, uint32_t batch_size, la3_status st, double start,
+              double fin, const uint8_t* out, uint64_t bytes) {
+  for (uint32_t j = 0; j < qs.size(); j++) {
+    const Query& q = qs[j];
+    la3_query_info info{};
+    info.qid = q.qid; info.kind = q.kind; info.graph = q.graph; info.root = q.root;
+    info.batch_size = batch_size; info.status = st; info.submit_s = q.submit_s;
+    info.start_s = start; info.done_s = fin;
+    s->done.push_back(info);
+    if (s->cfg.keep_results && st == LA3_OK && out) {
+      const size_t row = (size_t)s->graphs[q.graph].nslots * elem_bytes(q.kind);
+      s->results[q.qid] = std::vector<uint8_t>(out + (size_t)j * row, out + (size_t)(j + 1) * row);
+    }
   }
-  la3_status st = LA3_OK;
-  if (s->cfg.keep_results && host_bytes < need) {
-    if (host) la3_host_free(host);
-    host = nullptr;
-    host_bytes = 0;
-    if (la3_host_alloc(need, &host) == LA3_OK) host_bytes = need;
-    else st = LA3_ENOMEM;
-  }
-  for (size_t i = 0; i < groups.size(); i++) {
-    const Query& h = groups[i].front();
-    for (const Query& q : groups[i]) roots[i].push_back(q.root);
-    la3_batch& b = bs[i];
-    std::memset(&b, 0, sizeof(b));
-    b.graph = s->graphs[h.graph].g;
-    b.kind = h.kind;
-    b.k = (uint32_t)groups[i].size();
-    b.roots = roots[i].data();
-    b.iters = h.iters;
-    b.out = s->cfg.keep_results && host ? (uint8_t*)host + off[i] : nullptr;
-  }
-  if (st == LA3_OK) st = la3_batch_run(ctx, bs.data(), (uint32_t)bs.size(), s->cfg.ppr_alpha);
-  const double fin = s->now();
-  uint32_t total = 0;
-  for (auto& g : groups) total += (uint32_t)g.size();
-  {
-    std::lock_guard<std::mutex> lk(s->mu);
-    for (size_t i = 0; i < groups.size(); i++) {
-      const uint32_t k = (uint32_t)groups[i].size();
-      for (uint32_t j = 0; j < k; j++) {
-        const Query& q = groups[i][j];
-        la3_query_info info{};
-        info.qid = q.qid; info.kind = q.kind; info.graph = q.graph; info.root = q.root;
-        info.batch_size = total; info.status = st; info.submit_s = q.submit_s;
-        info.start_s = start; info.done_s = fin;
-        s->done.push_back(info);
-        if (s->cfg.keep_results && st == LA3_OK) {
-          const size_t row = (size_t)s->graphs[q.graph].nslots * elem_bytes(q.kind);
-          std::vector<uint8_t> v(row);
-          std::memcpy(v.data(), (const uint8_t*)host + off[i] + (size_t)j * row, row);
-          s->results[q.qid] = std::move(v);
+  s->completed += qs.size();
+  s->in_flight_bytes -= bytes;
+  s->running_batches--;
+}
+
+// Fused mode: the worker keeps one session (a fused run on its context).
+// Between replays of 4 steps it admits pending queries as new batches —
+// grouped by (kind, graph, iters), any number of groups, while task slots
+// and budget allow — and retires the batches that finished, so queries of
+// every program and graph share each fused launch and new ones join as
+// slots and memory free up.
+void worker_fused(la3_server* s, la3_ctx* ctx) {
+  struct Running {
+    std::vector<Query> qs;
+    uint64_t bytes = 0;
+    std::vector<uint8_t> out;
+    double start = 0;
+  };
+  la3_session* sess = nullptr;
+  const la3_status cst = la3_session_create(ctx, s->cfg.ppr_alpha, &sess);
+  std::map<uint64_t, Running> running;
+  for (;;) {
+    std::vector<Running> admit;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      for (;;) {
+        if (s->stop) {
+          lk.unlock();
+          if (sess) la3_session_destroy(sess);
+          la3_ctx_destroy(ctx);
+          return;
         }
+        uint32_t free = sess ? la3_session_free_slots(sess) : 0;
+        while (free > 0 || !sess) {
+          uint64_t bytes = 0;
+          const uint32_t cap = sess ? std::min<uint32_t>(s->cfg.max_batch, 256 * free) : s->cfg.max_batch;
+          std::vector<Query> qs = pick_group(s, cap, bytes);
+          if (qs.empty()) break;
+          if (!sess) {  // no session: every query fails with the creation status
+            complete(s, qs, (uint32_t)qs.size(), cst, s->now(), s->now(), nullptr, bytes);
+            continue;
+          }
+          free -= ((uint32_t)qs.size() + 255) / 256;
+          Running r;
+          r.qs = std::move(qs);
+          r.bytes = bytes;
+          admit.push_back(std::move(r));
+        }
+        if (!sess) s->cv.notify_all();
+        if (!admit.empty() || !running.empty()) break;
+        s->cv.wait(lk);
       }
     }
-    s->completed += total;
-    s->in_flight_bytes -= bytes;
-    s->running_batches--;
+    const double now = s->now();
+    std::vector<std::pair<Running, la3_status>> failed;
+    for (Running& r : admit) {
+      const Query& h = r.qs.front();
+      const GraphEntry& ge = s->graphs[h.graph];
+      std::vector<uint32_t> roots;
+      for (const Query& q : r.qs) roots.push_back(q.root);
+      if (s->cfg.keep_results) r.out.resize((size_t)r.qs.size() * ge.nslots * elem_bytes(h.kind));
+      la3_batch b;
+      std::memset(&b, 0, sizeof(b));
+      b.graph = ge.g;
+      b.kind = h.kind;
+      b.k = (uint32_t)r.qs.size();
+      b.roots = roots.data();
+      b.iters = h.iters;
+      b.out = r.out.empty() ? nullptr : r.out.data();
+      r.start = now;
+      uint64_t id = 0;
+      const la3_status st = la3_session_add(sess, &b, &id);
+      if (st) failed.push_back({std::move(r), st});
+      else running[id] = std::move(r);
+    }
+    uint32_t nrun = 0;
+    la3_status sst = running.empty() ? LA3_OK : la3_session_step(sess, &nrun);
+    uint64_t ids[64];
+    uint32_t its[64];
+    int32_t sts[64];
+    uint32_t n = 0;
+    std::vector<std::pair<uint64_t, la3_status>> fin;
+    do {
+      la3_session_poll(sess, ids, its, sts, 64, &n);
+      for (uint32_t i = 0; i < n; i++) fin.push_back({ids[i], (la3_status)sts[i]});
+    } while (n == 64);
+    if (sst) {  // a failed step ends every running batch with its status
+      for (auto& kv : running) fin.push_back({kv.first, sst});
+    }
+    const double done_t = s->now();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      for (auto& f : failed) complete(s, f.first.qs, (uint32_t)f.first.qs.size(), f.second, f.first.start, done_t,
+                                      nullptr, f.first.bytes);
+      for (auto& f : fin) {
+        auto it = running.find(f.first);
+        if (it == running.end()) continue;
+        Running& r = it->second;
+        complete(s, r.qs, (uint32_t)r.qs.size(), f.second, r.start, done_t,
+                 r.out.empty() ? nullptr : r.out.data(), r.bytes);
+        running.erase(it);
+      }
+    }
+    if (!fin.empty() || !failed.empty()) s->cv.notify_all();
+    if (sst) {  // the session is unusable after a failed step: start a new one
+      la3_session_destroy(sess);
+      sess = nullptr;
+      if (la3_session_create(ctx, s->cfg.ppr_alpha, &sess) != LA3_OK) sess = nullptr;
+    }
   }
-  s->cv.notify_all();
 }
 
 void worker_main(la3_server* s, la3_ctx* ctx) {
+  if (s->cfg.fused) return worker_fused(s, ctx);
   void* host = nullptr;
   uint64_t host_bytes = 0;
   for (;;) {
     std::vector<Query> batch;
-    std::vector<std::vector<Query>> groups;
     uint64_t bytes = 0;
     {
       std::unique_lock<std::mutex> lk(s->mu);
@@ -191,44 +242,10 @@ void worker_main(la3_server* s, la3_ctx* ctx) {
           la3_ctx_destroy(ctx);
           return;
         }
-        if (!s->pending.empty() && s->cfg.fused) {
-          bytes = pick_mixed(s, groups);
-          if (!groups.empty()) {
-            s->in_flight_bytes += bytes;
-            s->running_batches++;
-            break;
-          }
-        } else if (!s->pending.empty()) {
-          const Query& head = s->pending.front();
-          // Grow the batch with same-key queries in arrival order while the
-          // budget admits it; a lone query is admitted when nothing runs.
-          std::vector<size_t> pick;
-          for (size_t i = 0; i < s->pending.size() && pick.size() < s->cfg.max_batch; i++) {
-            const Query& q = s->pending[i];
-            if (q.kind != head.kind || q.graph != head.graph || q.iters != head.iters) continue;
-            const uint64_t b = batch_bytes(s, head.kind, head.graph, (uint32_t)pick.size() + 1);
-            if (s->in_flight_bytes + b > s->cfg.state_budget && !(pick.empty() && s->running_batches == 0))
-              break;
-            pick.push_back(i);
-          }
-          if (!pick.empty()) {
-            bytes = batch_bytes(s, head.kind, head.graph, (uint32_t)pick.size());
-            for (size_t j = pick.size(); j-- > 0;) {
-              batch.push_back(s->pending[pick[j]]);
-              s->pending.erase(s->pending.begin() + pick[j]);
-            }
-            std::reverse(batch.begin(), batch.end());
-            s->in_flight_bytes += bytes;
-            s->running_batches++;
-            break;
-          }
-        }
+        batch = pick_group(s, s->cfg.max_batch, bytes);
+        if (!batch.empty()) break;
         s->cv.wait(lk);
       }
-    }
-    if (!groups.empty()) {
-      run_mixed(s, ctx, groups, bytes, host, host_bytes);
-      continue;
     }
     const Query& h = batch.front();
     const uint32_t k = (uint32_t)batch.size();
@@ -260,23 +277,7 @@ void worker_main(la3_server* s, la3_ctx* ctx) {
     const double fin = s->now();
     {
       std::lock_guard<std::mutex> lk(s->mu);
-      for (uint32_t i = 0; i < k; i++) {
-        const Query& q = batch[i];
-        la3_query_info info{};
-        info.qid = q.qid; info.kind = q.kind; info.graph = q.graph; info.root = q.root;
-        info.batch_size = k; info.status = st; info.submit_s = q.submit_s;
-        info.start_s = start; info.done_s = fin;
-        s->done.push_back(info);
-        if (s->cfg.keep_results && st == LA3_OK) {
-          const size_t row = (size_t)ge.nslots * elem_bytes(h.kind);
-          std::vector<uint8_t> v(row);
-          std::memcpy(v.data(), (const uint8_t*)out + (size_t)i * row, row);
-          s->results[q.qid] = std::move(v);
-        }
-      }
-      s->completed += k;
-      s->in_flight_bytes -= bytes;
-      s->running_batches--;
+      complete(s, batch, k, st, start, fin, (const uint8_t*)out, bytes);
     }
     s->cv.notify_all();
   }

@@ -111,3 +111,47 @@ def test_undirected_bfs_and_errors(gpu_ctx):
         L.batch_run(gpu_ctx, [(gd, L.Q_PPR, roots, 0)])  # PPR needs iterations
     with pytest.raises(L.LA3Error):
         L.batch_run(gpu_ctx, [(gd, L.Q_BFS, np.array([(1 << 10) + 5], dtype=np.uint32))])
+
+
+def test_session_join_while_running(gpu_ctx, two_graphs):
+    """Batches added while others run (step-level continuous batching) get
+    the same results as alone."""
+    (ga, ra), (gb, rb) = two_graphs
+    s = L.Session(gpu_ctx)
+    r1 = _roots(ra, 64, 51)
+    id1, o1 = s.add(ga, L.Q_SSSP, r1)
+    s.step()
+    r2 = _roots(rb, 40, 52)
+    id2, o2 = s.add(gb, L.Q_BFS, r2)
+    r3 = _roots(rb, 3, 53)
+    id3, o3 = s.add(gb, L.Q_PPR, r3, 6)
+    done = {}
+    for _ in range(10000):
+        for i, it, st in s.poll():
+            assert st == 0
+            done[i] = it
+        if len(done) == 3:
+            break
+        s.step()
+    assert len(done) == 3
+    assert np.array_equal(o1, L.sssp_batch(ga, r1)[0])
+    assert np.array_equal(o2, L.bfs_batch(gb, r2)[0])
+    assert np.array_equal(o3, L.ppr_batch(gb, r3, 6)[0])
+    assert done[id3] == 6
+    # the context holds one session at a time
+    with pytest.raises(L.LA3Error):
+        L.batch_run(gpu_ctx, [(gb, L.Q_BFS, r2)])
+    s.close()
+    L.batch_run(gpu_ctx, [(gb, L.Q_BFS, r2)])
+
+
+def test_session_slots_exhausted(gpu_ctx, two_graphs):
+    _, (gb, rb) = two_graphs
+    s = L.Session(gpu_ctx)
+    added = 0
+    with pytest.raises(L.LA3Error):
+        for i in range(17):           # 16 batches x 4 tasks fill the 64 slots
+            s.add(gb, L.Q_BFS, _roots(rb, 1024, 60 + i))
+            added += 1
+    assert added == 16 and s.free_slots() == 0
+    s.close()
