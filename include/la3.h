@@ -220,8 +220,8 @@ la3_status la3_batch_run(la3_ctx* ctx, la3_batch* batches, uint32_t nbatches, do
 
 /* Sessions: continuous batching at step granularity. A session is a fused
  * run that batches join and leave while it runs: la3_session_add sets a batch
- * up (it joins the next step), la3_session_step runs 4 steps of every running
- * batch (one replay of the captured graph) and finishes the batches whose
+ * up (it joins the next step), la3_session_step runs one step of every
+ * running batch (one replay of a captured graph) and finishes the batches whose
  * traversals ended (results copied to their `out`, which must stay valid
  * until then), and la3_session_poll returns the ids of finished batches.
  * la3_batch_run is one session run to completion. A context holds at most one
@@ -283,7 +283,7 @@ typedef struct {
   double ppr_alpha;       /* teleport probability of PPR queries */
   int fused;              /* 0: a batch is one (kind, graph, iters) group run by
                              la3_*_batch_ctx; 1: each worker keeps a session
-                             (la3_session_*): between replays of 4 steps it admits
+                             (la3_session_*): between steps it admits
                              pending queries as new (kind, graph, iters) groups of up
                              to max_batch while task slots and budget allow, and
                              retires finished groups, so queries of every program and

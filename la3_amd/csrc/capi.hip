@@ -53,7 +53,7 @@ struct la3_ctx {
   // context's life) and the captured step graph that reads them.
   FTask* fz_tasks = nullptr;
   uint8_t* fz_ctl = nullptr;
-  hipGraphExec_t fz_exec[64] = {};  // by launch mask (fz_step)
+  hipGraphExec_t fz_exec[2][64] = {};  // [one step per replay][launch mask (fz_step)]
   bool fz_busy = false;             // a session holds the task table
 };
 
@@ -90,7 +90,8 @@ void ctx_release(la3_ctx* c) {
     for (auto& kv : c->scratch) if (kv.second.p) (void)hipFree(kv.second.p);
     for (auto& e : c->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
-    for (auto e : c->fz_exec) if (e) (void)hipGraphExecDestroy(e);
+    for (auto& row : c->fz_exec)
+      for (auto e : row) if (e) (void)hipGraphExecDestroy(e);
     if (c->host_ctr) (void)hipHostFree(c->host_ctr);
     (void)hipStreamDestroy(c->stream);
   }
@@ -1306,7 +1307,9 @@ la3_status la3_rmat_generate(uint32_t scale, uint32_t edgefactor, double a, doub
 // ====================================================== fused batch runs ===
 // la3_batch_run: every batch is split into tasks of at most 256 queries;
 // all tasks advance together through k_fused_a / k_fused_b / k_fused_c
-// (kernels_fused.h), FZ_STEPS steps per replay of one captured hipGraph.
+// (kernels_fused.h), several steps per replay of one captured hipGraph:
+// FZ_STEPS for la3_batch_run (whose batches all start together), one for a
+// session, so that its batches can join and leave at every step.
 namespace {
 
 constexpr uint32_t FZ_STEPS = 4;
@@ -1513,17 +1516,18 @@ void fz_step(la3_ctx* c, uint32_t mask, bool events, int* last_a, int* last_p) {
   else hipLaunchKernelGGL(k_fused_c, dim3(1), dim3(BLOCK), 0, c->stream, tk, hdr, ctl, 0u);
 }
 
-la3_status fz_capture(la3_ctx* c, uint32_t mask) {
-  if (c->fz_exec[mask]) return LA3_OK;
+la3_status fz_capture(la3_ctx* c, uint32_t mask, uint32_t steps) {
+  hipGraphExec_t& ex = c->fz_exec[steps == 1][mask];
+  if (ex) return LA3_OK;
   hipGraph_t graph = nullptr;
   HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-  for (uint32_t i = 0; i < FZ_STEPS; i++) fz_step(c, mask, false, nullptr, nullptr);
+  for (uint32_t i = 0; i < steps; i++) fz_step(c, mask, false, nullptr, nullptr);
   hipError_t e = hipStreamEndCapture(c->stream, &graph);
   if (e != hipSuccess) return fail(c, LA3_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  e = hipGraphInstantiate(&c->fz_exec[mask], graph, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) {
-    c->fz_exec[mask] = nullptr;
+    ex = nullptr;
     return fail(c, LA3_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
   return LA3_OK;
@@ -1571,6 +1575,7 @@ struct la3_session {
   FCtl fresh[FMAX_TASKS];   // staging for control blocks written by the host
   FHdr hdr{};               // staging for the header
   uint32_t ntasks = 0;      // slots [0, ntasks) are known to the device
+  uint32_t steps = 1;       // steps per replay
   bool pending_init = false;
   std::map<uint64_t, FzBatch> batches;
   std::deque<uint64_t> done;
@@ -1626,7 +1631,7 @@ la3_status sess_add(la3_session* s, const la3_batch& in, uint64_t* id) {
   return LA3_OK;
 }
 
-// One replay (FZ_STEPS steps) of the running tasks, then the post-loop work
+// One replay (s->steps steps) of the running tasks, then the post-loop work
 // of every task that finished; a batch completes when its last task does.
 la3_status sess_step(la3_session* s) {
   la3_ctx* c = s->c;
@@ -1647,12 +1652,12 @@ la3_status sess_step(la3_session* s) {
   int last_a = -1, last_pull = -1;
   if (mask) {
     if (!c->profiling) {
-      if ((st = fz_capture(c, mask))) return st;
-      HIPC(c, hipGraphLaunch(c->fz_exec[mask], c->stream));
+      if ((st = fz_capture(c, mask, s->steps))) return st;
+      HIPC(c, hipGraphLaunch(c->fz_exec[s->steps == 1][mask], c->stream));
     } else {
       // Profiling: the same steps as separate launches, each bracketed by
       // events (per-kernel device time; the graph has none).
-      for (uint32_t i = 0; i < FZ_STEPS; i++) fz_step(c, mask, true, &last_a, &last_pull);
+      for (uint32_t i = 0; i < s->steps; i++) fz_step(c, mask, true, &last_a, &last_pull);
     }
   }
   HIPC(c, hipMemcpyAsync(s->hc, dc, sizeof(FCtl) * s->ntasks, hipMemcpyDeviceToHost, c->stream));
@@ -1742,6 +1747,7 @@ la3_status la3_batch_run(la3_ctx* c, la3_batch* batches, uint32_t nbatches, doub
   auto s = std::make_unique<la3_session>();
   la3_status st = sess_init(c, ppr_alpha, s.get());
   if (st) return st;
+  s->steps = FZ_STEPS;
   timing_begin(c);
   std::vector<uint64_t> ids(nbatches);
   for (uint32_t i = 0; i < nbatches && !st; i++) st = sess_add(s.get(), batches[i], &ids[i]);

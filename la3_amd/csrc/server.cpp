@@ -120,7 +120,7 @@ void complete(la3_server* s, const std::vector<Query>& qs, uint32_t batch_size, 
 }
 
 // Fused mode: the worker keeps one session (a fused run on its context).
-// Between replays of 4 steps it admits pending queries as new batches —
+// Between steps it admits pending queries as new batches —
 // grouped by (kind, graph, iters), any number of groups, while task slots
 // and budget allow — and retires the batches that finished, so queries of
 // every program and graph share each fused launch and new ones join as
