@@ -661,6 +661,27 @@ la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters
 }
 
 // ----------------------------------------------------------- PageRank ---
+}  // extern "C"
+
+// One PR pass in MODE over the regular rows (sinks = false) or the sink rows:
+// k_pr_step, then k_pr_combine for the rows longer than PSEG_ENTRIES.
+// Returns the event slot of the k_pr_step launch.
+template <int MODE>
+static int pr_pass(la3_ctx* c, la3_graph* g, const PrArgs& a, bool sinks, double* part, const char* name) {
+  const DG& d = g->dg;
+  const uint32_t r0 = sinks ? g->nreg : 0, r1 = sinks ? g->nrows : g->nreg;
+  const uint32_t s0 = sinks ? d.npseg_reg : 0, s1 = sinks ? d.npseg : d.npseg_reg;
+  const uint32_t p0 = sinks ? d.nplong_reg : 0, p1 = sinks ? d.nplong : d.nplong_reg;
+  if (r1 <= r0) return -1;
+  const uint32_t nbr = blocks_for(r1 - r0);
+  const uint32_t nbw = std::min<uint32_t>(((uint64_t)d.nlong + (s1 - s0) + 3) / 4, 4096);
+  const int ev = launch(c, name, k_pr_step<MODE>, nbr + nbw, d, a, r0, r1, nbr, s0, s1, part);
+  if (p1 > p0) launch(c, "pr_combine", k_pr_combine<MODE>, blocks_for(p1 - p0), d, a, p0, p1, (const double*)part);
+  return ev;
+}
+
+extern "C" {
+
 static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t seed_slot,
                             double* rank, uint32_t* degree, uint32_t* iters_run) {
   la3_ctx* c = g->ctx;
@@ -676,7 +697,8 @@ static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t
   double* ss = (double*)scratch(c, "pr.srcsum", sizeof(double) * (g->nreg + 1));
   uint32_t* deg = (uint32_t*)scratch(c, "pr.deg", sizeof(uint32_t) * S);
   uint32_t* act = (uint32_t*)scratch(c, "pr.active", 64);
-  if (!x0 || !x1 || !rk || !ss || !deg || !act) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  double* part = (double*)scratch(c, "pr.part", sizeof(double) * ((size_t)d.npseg + 1));
+  if (!x0 || !x1 || !rk || !ss || !deg || !act || !part) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   PrArgs a{};
   double* xa = x0;
   double* xb = x1;
@@ -684,10 +706,8 @@ static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t
   a.alpha = alpha; a.damp = 1.0 - alpha; a.tol = 1e-5; a.seed = seed_slot;
   launch(c, "outdeg", k_outdeg, blocks_for(S), d, deg);
   launch(c, "pr_init", k_pr_init, blocks_for(S), d, a);
-  const uint32_t nlb = std::max<uint32_t>(1, std::min<uint32_t>((d.nlong + 3) / 4, 4096));
   // Constant source contributions per regular row.
-  launch(c, "pr_srcsum", k_pr_rows<0>, blocks_for(g->nreg), d, a, 0u, g->nreg);
-  if (d.nlong) launch(c, "pr_srcsum_long", k_pr_long<0>, nlb, d, a, 0u, g->nreg);
+  pr_pass<0>(c, g, a, false, part, "pr_srcsum");
   const bool until = iters == 0;
   uint32_t it = 0;
   for (;;) {
@@ -697,10 +717,8 @@ static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t
     a.xnext = xb;
     // Algorithmic bytes: 4 B index + 8 B message per regular-column entry,
     // 8 B source sum + 16 B rank read/written + 8 B next message per row.
-    const int ev_s = launch(c, "pr_rows", k_pr_rows<1>, blocks_for(g->nreg), d, a, 0u, g->nreg);
-    const int ev_l = d.nlong ? launch(c, "pr_rows_long", k_pr_long<1>, nlb, d, a, 0u, g->nreg) : -1;
-    annotate(c, ev_s, 12.0 * (g->nnz_reg - g->nnz_long_reg) + 32.0 * g->nreg, g->nnz_reg - g->nnz_long_reg);
-    annotate(c, ev_l, 12.0 * g->nnz_long_reg, g->nnz_long_reg);
+    const int ev = pr_pass<1>(c, g, a, false, part, "pr_step");
+    annotate(c, ev, 12.0 * g->nnz_reg + 32.0 * g->nreg, g->nnz_reg);
     std::swap(xa, xb);
     it++;
     if (until) {
@@ -716,8 +734,7 @@ static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t
   a.x = xa;
   a.xnext = xb;
   if (g->nsink) {
-    launch(c, "pr_sinks", k_pr_rows<2>, blocks_for(g->nsink), d, a, g->nreg, g->nrows);
-    if (d.nlong) launch(c, "pr_sinks_long", k_pr_long<2>, nlb, d, a, g->nreg, g->nrows);
+    pr_pass<2>(c, g, a, true, part, "pr_sinks");
   }
   double* outd = (double*)scratch(c, "out.f64", sizeof(double) * S);
   if (!outd) return fail(c, LA3_ENOMEM, "scratch allocation failed");

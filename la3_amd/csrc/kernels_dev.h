@@ -543,44 +543,101 @@ __device__ __forceinline__ void pr_finish(const DG& g, const PrArgs& a, uint32_t
   }
 }
 
+// Entry range of row r summed in MODE: 0 = source columns [b, split),
+// 1 = regular columns [split, e), 2 = all [b, e).
 template <int MODE>
-__global__ void __launch_bounds__(BLOCK) k_pr_rows(DG g, PrArgs a, uint32_t r0, uint32_t r1) {
-  for (uint32_t r = r0 + blockIdx.x * blockDim.x + threadIdx.x; r < r1; r += gridDim.x * blockDim.x) {
-    const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1], sp = g.row_split[r];
-    if (e - b > LONG_ROW) continue;
-    double y;
-    if (MODE == 0) {
-      y = 0.0;
-      for (uint32_t j = b; j < sp; j++) y = __dadd_rn(y, a.x[g.row_cols[j]]);
-    } else if (MODE == 1) {
-      y = a.srcsum[r];
-      for (uint32_t j = sp; j < e; j++) y = __dadd_rn(y, a.x[g.row_cols[j]]);
-    } else {
-      y = 0.0;
-      for (uint32_t j = b; j < e; j++) y = __dadd_rn(y, a.x[g.row_cols[j]]);
+__device__ __forceinline__ void pr_range(const DG& g, uint32_t r, uint32_t& lo, uint32_t& hi) {
+  const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1], sp = g.row_split[r];
+  lo = MODE == 1 ? sp : b;
+  hi = MODE == 0 ? sp : e;
+}
+
+// y + x[cols[lo]] + ... + x[cols[hi - 1]], added in entry order; the loads
+// of four entries are issued before their adds.
+__device__ __forceinline__ double seq_sum(const DG& g, const double* x, uint32_t j, uint32_t hi, double y) {
+  for (; j + 4 <= hi; j += 4) {
+    const uint32_t c0 = g.row_cols[j], c1 = g.row_cols[j + 1], c2 = g.row_cols[j + 2], c3 = g.row_cols[j + 3];
+    const double v0 = x[c0], v1 = x[c1], v2 = x[c2], v3 = x[c3];
+    y = __dadd_rn(y, v0);
+    y = __dadd_rn(y, v1);
+    y = __dadd_rn(y, v2);
+    y = __dadd_rn(y, v3);
+  }
+  for (; j < hi; j++) y = __dadd_rn(y, x[g.row_cols[j]]);
+  return y;
+}
+
+// Wavefront sum of x[cols[j]], j in [lo, hi): lane l adds entries lo + l,
+// lo + l + 64, ... in order (four loads in flight), then a fixed-shape
+// wavefront reduction.
+__device__ __forceinline__ double wave_range_sum(const DG& g, const double* x, uint32_t lo, uint32_t hi) {
+  double p = 0.0;
+  uint32_t j = lo + lane_id();
+  for (; j + 3 * WAVE < hi; j += 4 * WAVE) {
+    const uint32_t c0 = g.row_cols[j], c1 = g.row_cols[j + WAVE], c2 = g.row_cols[j + 2 * WAVE],
+                   c3 = g.row_cols[j + 3 * WAVE];
+    const double v0 = x[c0], v1 = x[c1], v2 = x[c2], v3 = x[c3];
+    p = __dadd_rn(p, v0);
+    p = __dadd_rn(p, v1);
+    p = __dadd_rn(p, v2);
+    p = __dadd_rn(p, v3);
+  }
+  for (; j < hi; j += WAVE) p = __dadd_rn(p, x[g.row_cols[j]]);
+  return wave_sum(p);
+}
+
+// One PR step over rows [r0, r1) (regular rows, or sink rows for MODE 2):
+//   blocks [0, nbr): rows with <= LONG_ROW entries, lane per row;
+//   blocks [nbr, grid): long rows with <= PSEG_ENTRIES entries, wavefront
+//   per row; then the PSEG_ENTRIES-entry segments [s0, s1) of the longer
+//   rows, wavefront per segment, partial sums into part[segment]
+//   (k_pr_combine finishes those rows).
+template <int MODE>
+__global__ void __launch_bounds__(BLOCK) k_pr_step(DG g, PrArgs a, uint32_t r0, uint32_t r1, uint32_t nbr,
+                                                   uint32_t s0, uint32_t s1, double* part) {
+  if (blockIdx.x < nbr) {
+    for (uint32_t r = r0 + blockIdx.x * BLOCK + threadIdx.x; r < r1; r += nbr * BLOCK) {
+      if (g.row_ptr[r + 1] - g.row_ptr[r] > LONG_ROW) continue;
+      uint32_t lo, hi;
+      pr_range<MODE>(g, r, lo, hi);
+      pr_finish<MODE>(g, a, r, seq_sum(g, a.x, lo, hi, MODE == 1 ? a.srcsum[r] : 0.0));
     }
+    return;
+  }
+  const uint32_t nwaves = (gridDim.x - nbr) * (BLOCK / WAVE);
+  const uint32_t wid = (blockIdx.x - nbr) * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  for (uint32_t i = wid; i < g.nlong; i += nwaves) {
+    const uint32_t r = g.long_rows[i];
+    if (r < r0 || r >= r1 || g.row_ptr[r + 1] - g.row_ptr[r] > PSEG_ENTRIES) continue;  // wave-uniform
+    uint32_t lo, hi;
+    pr_range<MODE>(g, r, lo, hi);
+    const double p = wave_range_sum(g, a.x, lo, hi);
+    if (lane_id() == 0) pr_finish<MODE>(g, a, r, MODE == 1 ? __dadd_rn(a.srcsum[r], p) : p);
+  }
+  for (uint32_t sg = s0 + wid; sg < s1; sg += nwaves) {
+    const uint32_t r = g.pseg_row[sg];
+    uint32_t lo, hi;
+    pr_range<MODE>(g, r, lo, hi);
+    lo = max(lo, g.pseg_beg[sg]);
+    hi = min(hi, g.pseg_beg[sg] + PSEG_ENTRIES);
+    const double p = wave_range_sum(g, a.x, lo, max(lo, hi));
+    if (lane_id() == 0) part[sg] = p;
+  }
+}
+
+// Rows plong[p0, p1) (rows longer than PSEG_ENTRIES): their segment partials
+// in segment order, then the row's update.
+template <int MODE>
+__global__ void __launch_bounds__(BLOCK) k_pr_combine(DG g, PrArgs a, uint32_t p0, uint32_t p1,
+                                                      const double* part) {
+  for (uint32_t i = p0 + blockIdx.x * BLOCK + threadIdx.x; i < p1; i += gridDim.x * BLOCK) {
+    const uint32_t r = g.plong[i];
+    double y = MODE == 1 ? a.srcsum[r] : 0.0;
+    for (uint32_t sg = g.pseg_first[i]; sg < g.pseg_first[i + 1]; sg++) y = __dadd_rn(y, part[sg]);
     pr_finish<MODE>(g, a, r, y);
   }
 }
 
-template <int MODE>
-__global__ void __launch_bounds__(BLOCK) k_pr_long(DG g, PrArgs a, uint32_t r0, uint32_t r1) {
-  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
-  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
-  for (uint32_t i = wid; i < g.nlong; i += nwaves) {
-    const uint32_t r = g.long_rows[i];
-    if (r < r0 || r >= r1) continue;
-    const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1], sp = g.row_split[r];
-    const uint32_t lo = MODE == 1 ? sp : b, hi = MODE == 0 ? sp : e;
-    double p = 0.0;
-    for (uint32_t j = lo + lane_id(); j < hi; j += WAVE) p = __dadd_rn(p, a.x[g.row_cols[j]]);
-    p = wave_sum(p);
-    if (lane_id() == 0) pr_finish<MODE>(g, a, r, MODE == 1 ? __dadd_rn(a.srcsum[r], p) : p);
-  }
-}
-
-// Messages of the initial scatter: x = rank / degree for every vertex with a
-// column (regular and source); pr.h:37.
 __global__ void k_pr_init(DG g, PrArgs a) {
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
     const double r0 = pr_base(a, s);
