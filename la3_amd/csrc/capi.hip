@@ -76,6 +76,10 @@ struct la3_graph {
   std::vector<void*> allocs;
   DG dg{};
   std::unique_ptr<HostLayout> host;  // kept instead of device arrays on a host-only context
+  // Fixed-iteration PR / PPR loop captured as a hipGraph, with the arguments
+  // it was captured for (iterations, buffers, seed, alpha).
+  hipGraphExec_t pr_exec = nullptr;
+  std::vector<uint64_t> pr_key;
 };
 
 namespace {
@@ -461,6 +465,7 @@ void la3_graph_free(la3_graph* g) {
   if (!g->host) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    if (g->pr_exec) (void)hipGraphExecDestroy(g->pr_exec);
     for (void* p : g->allocs) (void)hipFree(p);
   }
   delete g;
@@ -710,7 +715,41 @@ static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t
   pr_pass<0>(c, g, a, false, part, "pr_srcsum");
   const bool until = iters == 0;
   uint32_t it = 0;
-  for (;;) {
+  if (!until && !c->profiling) {
+    // A fixed number of iterations needs no host decision between them: the
+    // loop is captured once (per buffers and parameters) and replayed.
+    uint64_t ab;
+    std::memcpy(&ab, &alpha, 8);
+    const std::vector<uint64_t> key = {(uint64_t)(uintptr_t)c, iters, (uint64_t)(uintptr_t)x0, (uint64_t)(uintptr_t)x1,
+                                       (uint64_t)(uintptr_t)rk, (uint64_t)(uintptr_t)ss, (uint64_t)(uintptr_t)deg,
+                                       (uint64_t)(uintptr_t)act, (uint64_t)(uintptr_t)part, seed_slot, ab};
+    if (!g->pr_exec || g->pr_key != key) {
+      if (g->pr_exec) (void)hipGraphExecDestroy(g->pr_exec);
+      g->pr_exec = nullptr;
+      hipGraph_t graph = nullptr;
+      HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+      PrArgs b = a;
+      for (uint32_t i = 0; i < iters; i++) {
+        b.x = (i & 1) ? xb : xa;
+        b.xnext = (i & 1) ? xa : xb;
+        pr_pass<1>(c, g, b, false, part, "pr_step");
+      }
+      hipError_t e = hipStreamEndCapture(c->stream, &graph);
+      if (e != hipSuccess) return fail(c, LA3_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&g->pr_exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        g->pr_exec = nullptr;
+        return fail(c, LA3_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+      g->pr_key = key;
+    }
+    HIPC(c, hipGraphLaunch(g->pr_exec, c->stream));
+    it = iters;
+    if (iters & 1) std::swap(xa, xb);  // xa: the final messages
+  }
+  const bool replayed = it > 0;
+  for (; !replayed;) {
     if (!until && it >= iters) break;
     if (until) launch(c, "fill_u32", k_fill<uint32_t>, 1, act, (uint64_t)1, 0u);
     a.x = xa;
