@@ -820,8 +820,13 @@ la3_status la3_cc(la3_graph* g, uint32_t* label, uint32_t* iters_run) {
   uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
   if (!p || !out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   launch(c, "cc_init", k_cc_init, blocks_for(S), d, p);
-  launch(c, "cc_hook", k_cc_hook, blocks_for(g->nrows), d, p);
-  if (d.nlong) launch(c, "cc_hook_long", k_cc_hook_long, std::min<uint32_t>((d.nlong + 3) / 4, 4096), d, p);
+  // Algorithmic bytes: 4 B column + 4 B parent read per entry, 8 B row
+  // pointers and a parent word per row (the finds' extra hops not counted).
+  const int ev_s = launch(c, "cc_hook", k_cc_hook, blocks_for(g->nrows), d, p);
+  const int ev_l = d.nlong ? launch(c, "cc_hook_long", k_cc_hook_long, std::min<uint32_t>((d.nlong + 3) / 4, 4096), d, p)
+                           : -1;
+  annotate(c, ev_s, 8.0 * (g->nnz - g->nnz_long_reg) + 12.0 * g->nrows, g->nnz - g->nnz_long_reg);
+  annotate(c, ev_l, 8.0 * g->nnz_long_reg, g->nnz_long_reg);
   launch(c, "cc_output", k_cc_output, blocks_for(S), d, p, out);
   c->edges_scanned = g->nnz;
   la3_status st = deliver(c, label, out, sizeof(uint32_t) * S);
@@ -847,12 +852,13 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
   unsigned long long* tot = (unsigned long long*)scratch(c, "tc.total", 64);
   if (!row_of || !per || !tot) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(S), per, (uint64_t)S, 0u);
-  launch(c, "fill_u64", k_fill<unsigned long long>, 1, tot, (uint64_t)1, 0ull);
+  launch(c, "fill_u64", k_fill<unsigned long long>, 1, tot, (uint64_t)2, 0ull);
   launch(c, "tc_rows", k_tc_entry_rows, blocks_for(g->nrows), d, row_of);
-  launch(c, "tc_count", k_tc_count, blocks_for(g->nnz, 4 * c->ncu * 8), d, (const uint32_t*)row_of, per, tot);
+  const int ev = launch(c, "tc_count", k_tc_count, blocks_for(g->nnz, 4 * c->ncu * 8), d, (const uint32_t*)row_of,
+                        per, tot);
   c->edges_scanned = g->nnz;
-  unsigned long long h = 0;
-  HIPC(c, hipMemcpyAsync(c->host_ctr, tot, 8, hipMemcpyDeviceToHost, c->stream));
+  unsigned long long h[2] = {0, 0};
+  HIPC(c, hipMemcpyAsync(c->host_ctr, tot, 16, hipMemcpyDeviceToHost, c->stream));
   if (per_vertex) {
     uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
     if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
@@ -861,8 +867,11 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
     if (st) return st;
   }
   HIPC(c, hipStreamSynchronize(c->stream));
-  std::memcpy(&h, c->host_ctr, 8);
-  *total = h;
+  std::memcpy(h, c->host_ctr, 16);
+  *total = h[0];
+  // Algorithmic bytes: 4 B per list entry a merge intersection reads, plus
+  // row id, column and both row pointers (16 B) per entry.
+  annotate(c, ev, 4.0 * h[1] + 16.0 * g->nnz, g->nnz);
   timing_end(c);
   return LA3_OK;
 }

@@ -738,22 +738,28 @@ __global__ void __launch_bounds__(BLOCK) k_tc_entry_rows(DG g, uint32_t* row_of)
     for (uint32_t j = g.row_ptr[r]; j < g.row_ptr[r + 1]; j++) row_of[j] = r;
 }
 
+// total[0] += triangles; total[1] += sum over entries of |N-(u)| + |N-(v)|
+// (the list entries a merge intersection reads: the algorithmic bytes).
 __global__ void __launch_bounds__(BLOCK) k_tc_count(DG g, const uint32_t* row_of, uint32_t* per_slot,
                                                     unsigned long long* total) {
-  unsigned long long acc = 0;
+  unsigned long long acc = 0, work = 0;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < g.nnz; base += stride) {
     const uint64_t j = base + threadIdx.x;
     if (j < g.nnz) {
       const uint32_t u = row_of[j], v = g.row_cols[j];
       if (v < g.nrows) {
-        const uint32_t c = tc_intersect(g, g.row_ptr[u], g.row_ptr[u + 1], g.row_ptr[v], g.row_ptr[v + 1]);
+        const uint32_t ub = g.row_ptr[u], ue = g.row_ptr[u + 1], vb = g.row_ptr[v], ve = g.row_ptr[v + 1];
+        const uint32_t c = tc_intersect(g, ub, ue, vb, ve);
+        work += (ue - ub) + (ve - vb);
         if (c) { atomicAdd(&per_slot[v], c); acc += c; }
       }
     }
   }
   acc = wave_sum(acc);
-  if (lane_id() == 0 && acc) atomicAdd(total, acc);
+  work = wave_sum(work);
+  if (lane_id() == 0 && acc) atomicAdd(&total[0], acc);
+  if (lane_id() == 0 && work) atomicAdd(&total[1], work);
 }
 
 }  // namespace la3
