@@ -333,7 +333,10 @@ def workload_batched_s24(args, L):
                      ppr_l1_vs_fp64=max(l1s), ppr_max_rel=max(rels),
                      ppr_residual_last_iter=float(np.max(resid)),
                      roofline=roofline(st_f), kernels=kernel_table(st_f),
-                     autotune={"sssp_delta": tune["sssp_delta"], "batch_sssp_delta": tune["batch_sssp_delta"]})
+                     autotune={"sssp_delta": tune["sssp_delta"], "batch_sssp_delta": tune["batch_sssp_delta"],
+                               "task_k": tune["task_k"], "fused_blocks_per_cu": tune["fused_blocks"],
+                               "task_ms_64_128_256": [[round(x, 3) for x in r] for r in tune["task_ms"]],
+                               "fused_ms_8_16_32": [round(x, 3) for x in tune["fused_ms"]]})
 
 
 def workload_serve(args, L):
@@ -573,7 +576,10 @@ def main():
         "autotune": {"sssp_delta": tune["sssp_delta"], "batch_sssp_delta": tune["batch_sssp_delta"],
                      "candidates": tune["candidates"],
                      "single_ms": [round(x, 3) for x in tune["single_ms"]],
-                     "batch_ms": [round(x, 3) for x in tune["batch_ms"]]},
+                     "batch_ms": [round(x, 3) for x in tune["batch_ms"]],
+                     "task_k": tune["task_k"], "task_ms_64_128_256": [[round(x, 3) for x in r] for r in tune["task_ms"]],
+                     "fused_blocks_per_cu": tune["fused_blocks"],
+                     "fused_ms_8_16_32": [round(x, 3) for x in tune["fused_ms"]]},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(rec, n, roots[:args.cpu_roots], outdeg, hops, dist)

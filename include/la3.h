@@ -245,18 +245,37 @@ void la3_session_destroy(la3_session* s);
  * T rises to (smallest frontier distance + delta) when nothing below it
  * remains. delta = 0 relaxes everything every round (Bellman-Ford). The
  * distances are exact for every delta; delta only changes the work. */
-typedef enum { LA3_TUNE_SSSP_DELTA = 0, LA3_TUNE_BATCH_SSSP_DELTA = 1 } la3_tunable;
+/* Fused runs (la3_batch_run, sessions) split a batch of this graph into
+ * tasks of LA3_TUNE_{BFS,SSSP,PPR}_TASK_K queries (64, 128 or 256: the k-wide
+ * rows each task streams), and give their A / pull launches
+ * LA3_TUNE_FUSED_BLOCKS blocks per CU (8, 16 or 32; a run uses the largest
+ * value among its graphs). */
+typedef enum {
+  LA3_TUNE_SSSP_DELTA = 0,
+  LA3_TUNE_BATCH_SSSP_DELTA = 1,
+  LA3_TUNE_BFS_TASK_K = 2,
+  LA3_TUNE_SSSP_TASK_K = 3,
+  LA3_TUNE_PPR_TASK_K = 4,
+  LA3_TUNE_FUSED_BLOCKS = 5
+} la3_tunable;
 la3_status la3_graph_set_tuning(la3_graph* g, la3_tunable what, uint32_t value);
 la3_status la3_graph_get_tuning(const la3_graph* g, la3_tunable what, uint32_t* value);
 /* Autotuner: times single-query and batched SSSP from `roots` (a batch of
  * min(nroots, 1024) and up to 4 single queries) for each candidate delta and
- * keeps the fastest of each on the graph. */
+ * keeps the fastest of each on the graph. With at least 64 roots it then
+ * times fused runs of min(nroots, 256) queries: per program (BFS, SSSP, PPR
+ * of 5 iterations) each task width 64 / 128 / 256, then a BFS + SSSP run at
+ * 8 / 16 / 32 blocks per CU, and keeps the fastest of each. */
 typedef struct {
   uint32_t ncandidates;
   uint32_t candidates[8];
   double single_ms[8];   /* mean single-query time per candidate */
   double batch_ms[8];    /* batch time per candidate */
   uint32_t sssp_delta, batch_sssp_delta, batch_k;
+  uint32_t task_k[3];        /* chosen task width: BFS, SSSP, PPR (0: not tuned) */
+  double task_ms[3][3];      /* fused-run time per program at widths 64 / 128 / 256 */
+  uint32_t fused_blocks;     /* chosen blocks per CU (0: not tuned) */
+  double fused_ms[3];        /* BFS + SSSP fused run at 8 / 16 / 32 blocks per CU */
 } la3_autotune_report;
 la3_status la3_autotune(la3_graph* g, const uint32_t* roots, uint32_t nroots,
                         la3_autotune_report* report);

@@ -39,7 +39,8 @@ EXPORTS = [
     "la3_batch_run", "la3_session_create", "la3_session_add", "la3_session_step",
     "la3_session_poll", "la3_session_free_slots", "la3_session_destroy",
 ]
-TUNE_SSSP_DELTA, TUNE_BATCH_SSSP_DELTA = 0, 1
+(TUNE_SSSP_DELTA, TUNE_BATCH_SSSP_DELTA, TUNE_BFS_TASK_K, TUNE_SSSP_TASK_K, TUNE_PPR_TASK_K,
+ TUNE_FUSED_BLOCKS) = range(6)
 Q_BFS, Q_SSSP, Q_PPR = 0, 1, 2
 
 
@@ -79,7 +80,9 @@ class AutotuneReport(ctypes.Structure):
     _fields_ = [("ncandidates", ctypes.c_uint32), ("candidates", ctypes.c_uint32 * 8),
                 ("single_ms", ctypes.c_double * 8), ("batch_ms", ctypes.c_double * 8),
                 ("sssp_delta", ctypes.c_uint32), ("batch_sssp_delta", ctypes.c_uint32),
-                ("batch_k", ctypes.c_uint32)]
+                ("batch_k", ctypes.c_uint32), ("task_k", ctypes.c_uint32 * 3),
+                ("task_ms", (ctypes.c_double * 3) * 3), ("fused_blocks", ctypes.c_uint32),
+                ("fused_ms", ctypes.c_double * 3)]
 
 
 class Batch(ctypes.Structure):
@@ -349,14 +352,17 @@ class Graph:
         return v.value
 
     def autotune(self, roots):
-        """Picks the SSSP near/far delta (single and batched) by timing."""
+        """Picks by timing the SSSP near/far delta (single and batched) and,
+        with >= 64 roots, the fused runs' task widths and blocks per CU."""
         r = np.ascontiguousarray(roots, dtype=np.uint32)
         rep = AutotuneReport()
         self.ctx.check(lib().la3_autotune(self.h, _ptr(r), r.size, ctypes.byref(rep)), "autotune")
         n = rep.ncandidates
         return dict(candidates=list(rep.candidates[:n]), single_ms=list(rep.single_ms[:n]),
                     batch_ms=list(rep.batch_ms[:n]), sssp_delta=rep.sssp_delta,
-                    batch_sssp_delta=rep.batch_sssp_delta, batch_k=rep.batch_k)
+                    batch_sssp_delta=rep.batch_sssp_delta, batch_k=rep.batch_k,
+                    task_k=list(rep.task_k), task_ms=[list(r) for r in rep.task_ms],
+                    fused_blocks=rep.fused_blocks, fused_ms=list(rep.fused_ms))
 
     def csc(self, which):
         nnz = self.info["nnz_sink"] if which else self.info["nnz_regular"]

@@ -53,7 +53,7 @@ struct la3_ctx {
   // context's life) and the captured step graph that reads them.
   FTask* fz_tasks = nullptr;
   uint8_t* fz_ctl = nullptr;
-  hipGraphExec_t fz_exec[2][64] = {};  // [one step per replay][launch mask (fz_step)]
+  std::map<uint64_t, hipGraphExec_t> fz_exec;  // by (steps per replay, A grid, launch mask)
   bool fz_busy = false;             // a session holds the task table
 };
 
@@ -72,6 +72,10 @@ struct la3_graph {
   // Near/far thresholds of the SSSP relaxation (0 = plain Bellman-Ford),
   // single-query and batched; set by la3_graph_set_tuning or la3_autotune.
   uint32_t sssp_delta = 0, mssp_delta = 0;
+  // Fused runs: queries per task by kind (BFS, SSSP, PPR; 64, 128 or 256)
+  // and blocks per CU of the A / pull launches; la3_autotune sets them.
+  uint32_t task_k[3] = {256, 256, 256};
+  uint32_t fused_blocks = 32;
   std::vector<uint32_t> hid_of, id_of, slot_of_id;
   std::vector<void*> allocs;
   DG dg{};
@@ -94,8 +98,7 @@ void ctx_release(la3_ctx* c) {
     for (auto& kv : c->scratch) if (kv.second.p) (void)hipFree(kv.second.p);
     for (auto& e : c->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
-    for (auto& row : c->fz_exec)
-      for (auto e : row) if (e) (void)hipGraphExecDestroy(e);
+    for (auto& kv : c->fz_exec) (void)hipGraphExecDestroy(kv.second);
     if (c->host_ctr) (void)hipHostFree(c->host_ctr);
     (void)hipStreamDestroy(c->stream);
   }
@@ -1244,7 +1247,15 @@ la3_status la3_graph_set_tuning(la3_graph* g, la3_tunable what, uint32_t value) 
   if (!g) return LA3_EINVAL;
   if (what == LA3_TUNE_SSSP_DELTA) g->sssp_delta = value;
   else if (what == LA3_TUNE_BATCH_SSSP_DELTA) g->mssp_delta = value;
-  else return fail(g->ctx, LA3_EINVAL, "unknown tunable");
+  else if (what >= LA3_TUNE_BFS_TASK_K && what <= LA3_TUNE_PPR_TASK_K) {
+    if (value != 64 && value != 128 && value != 256) return fail(g->ctx, LA3_EINVAL, "task width must be 64, 128 or 256");
+    g->task_k[what - LA3_TUNE_BFS_TASK_K] = value;
+  } else if (what == LA3_TUNE_FUSED_BLOCKS) {
+    if (value != 8 && value != 16 && value != 32) return fail(g->ctx, LA3_EINVAL, "blocks per CU must be 8, 16 or 32");
+    g->fused_blocks = value;
+  } else {
+    return fail(g->ctx, LA3_EINVAL, "unknown tunable");
+  }
   return LA3_OK;
 }
 
@@ -1252,6 +1263,8 @@ la3_status la3_graph_get_tuning(const la3_graph* g, la3_tunable what, uint32_t* 
   if (!g || !value) return LA3_EINVAL;
   if (what == LA3_TUNE_SSSP_DELTA) *value = g->sssp_delta;
   else if (what == LA3_TUNE_BATCH_SSSP_DELTA) *value = g->mssp_delta;
+  else if (what >= LA3_TUNE_BFS_TASK_K && what <= LA3_TUNE_PPR_TASK_K) *value = g->task_k[what - LA3_TUNE_BFS_TASK_K];
+  else if (what == LA3_TUNE_FUSED_BLOCKS) *value = g->fused_blocks;
   else return LA3_EINVAL;
   return LA3_OK;
 }
@@ -1292,7 +1305,56 @@ la3_status la3_autotune(la3_graph* g, const uint32_t* roots, uint32_t nroots, la
     rep->sssp_delta = pick_s;
     rep->batch_sssp_delta = pick_b;
     rep->batch_k = kb;
+    rep->fused_blocks = 0;
+    for (int i = 0; i < 3; i++) rep->task_k[i] = 0;
   }
+  if (nroots < 64) return LA3_OK;
+  // Fused runs: task width per program, then blocks per CU.
+  const uint32_t kf = std::min<uint32_t>(nroots, 256);
+  auto run = [&](std::vector<la3_batch>& bs, double& ms) -> la3_status {
+    la3_status st = la3_batch_run(c, bs.data(), (uint32_t)bs.size(), 0.15);  // warm (scratch, graph capture)
+    if (st) return st;
+    const double t0 = clock();
+    if ((st = la3_batch_run(c, bs.data(), (uint32_t)bs.size(), 0.15))) return st;
+    ms = (clock() - t0) * 1e3;
+    return LA3_OK;
+  };
+  auto batch = [&](la3_query_kind kind) {
+    la3_batch b;
+    std::memset(&b, 0, sizeof(b));
+    b.graph = g; b.kind = kind; b.k = kf; b.roots = roots; b.iters = kind == LA3_Q_PPR ? 5 : 0;
+    return b;
+  };
+  static const uint32_t widths[3] = {64, 128, 256};
+  static const uint32_t blocks[3] = {8, 16, 32};
+  for (int kind = 0; kind < 3; kind++) {
+    double best = 1e30;
+    uint32_t pick = 256;
+    for (int wi = 0; wi < 3; wi++) {
+      g->task_k[kind] = widths[wi];
+      std::vector<la3_batch> bs = {batch((la3_query_kind)kind)};
+      double ms = 0;
+      la3_status st = run(bs, ms);
+      if (st) return st;
+      if (rep) rep->task_ms[kind][wi] = ms;
+      if (ms < best) { best = ms; pick = widths[wi]; }
+    }
+    g->task_k[kind] = pick;
+    if (rep) rep->task_k[kind] = pick;
+  }
+  double best = 1e30;
+  uint32_t pick = 32;
+  for (int bi = 0; bi < 3; bi++) {
+    g->fused_blocks = blocks[bi];
+    std::vector<la3_batch> bs = {batch(LA3_Q_BFS), batch(LA3_Q_SSSP)};
+    double ms = 0;
+    la3_status st = run(bs, ms);
+    if (st) return st;
+    if (rep) rep->fused_ms[bi] = ms;
+    if (ms < best) { best = ms; pick = blocks[bi]; }
+  }
+  g->fused_blocks = pick;
+  if (rep) rep->fused_blocks = pick;
   return LA3_OK;
 }
 
@@ -1378,19 +1440,23 @@ la3_status la3_rmat_generate(uint32_t scale, uint32_t edgefactor, double a, doub
 namespace {
 
 constexpr uint32_t FZ_STEPS = 4;
+constexpr uint32_t FZ_BLOCKS_PER_CU = 32;
 
 struct FzTask {
   uint32_t batch, q0, k;
 };
 
-uint32_t fz_ga(const la3_ctx* c) { return 4u * c->ncu * 8; }
+// Blocks of the A / pull launches (split among the tasks by k_fused_c):
+// blocks_per_cu per CU (LA3_TUNE_FUSED_BLOCKS, default 32); B launches 8 per CU.
+uint32_t fz_ga(const la3_ctx* c, uint32_t blocks_per_cu) { return blocks_per_cu * c->ncu; }
 uint32_t fz_gb(const la3_ctx* c) { return 4u * c->ncu * 2; }
 
 std::string fz_name(uint32_t t, const char* what) { return "fz" + std::to_string(t) + "." + what; }
 
 // Allocates and initialises one task's state on the context's stream and
 // fills its FTask entry (the single-family drivers' initialisation).
-la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t, double alpha, FTask& T) {
+la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t, double alpha, uint32_t ga_blocks,
+                    FTask& T) {
   la3_graph* g = b.graph;
   const DG& d = g->dg;
   const uint32_t S = g->nslots;
@@ -1463,7 +1529,7 @@ la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t
     return LA3_OK;
   }
   // PPR
-  const uint32_t ga = fz_ga(c) + FMAX_TASKS, gb = fz_gb(c) + FMAX_TASKS;
+  const uint32_t ga = ga_blocks + FMAX_TASKS, gb = fz_gb(c) + FMAX_TASKS;
   T.xs[0] = (float*)buf("x0", 4 * SK);
   T.xs[1] = (float*)buf("x1", 4 * SK);
   float* hi = (float*)buf("hi", 4 * SK);
@@ -1554,8 +1620,8 @@ la3_status fz_finish(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t 
 // queries, bit 1: 128, bit 2: 256), the bottom-up BFS launch of each width
 // class with BFS tasks (bits 3-5), then the control launch. With profiling
 // each launch is bracketed by events (last_a: the event slot of the last A).
-void fz_step(la3_ctx* c, uint32_t mask, bool events, int* last_a, int* last_p) {
-  const uint32_t ga = fz_ga(c) + FMAX_TASKS, gb = fz_gb(c) + FMAX_TASKS;
+void fz_step(la3_ctx* c, uint32_t mask, uint32_t ga_blocks, bool events, int* last_a, int* last_p) {
+  const uint32_t ga = ga_blocks + FMAX_TASKS, gb = fz_gb(c) + FMAX_TASKS;
   const FTask* tk = (const FTask*)c->fz_tasks;
   FHdr* hdr = (FHdr*)c->fz_ctl;
   FCtl* ctl = (FCtl*)(c->fz_ctl + sizeof(FHdr));
@@ -1581,12 +1647,13 @@ void fz_step(la3_ctx* c, uint32_t mask, bool events, int* last_a, int* last_p) {
   else hipLaunchKernelGGL(k_fused_c, dim3(1), dim3(BLOCK), 0, c->stream, tk, hdr, ctl, 0u);
 }
 
-la3_status fz_capture(la3_ctx* c, uint32_t mask, uint32_t steps) {
-  hipGraphExec_t& ex = c->fz_exec[steps == 1][mask];
+la3_status fz_capture(la3_ctx* c, uint32_t mask, uint32_t steps, uint32_t ga_blocks, hipGraphExec_t* out) {
+  hipGraphExec_t& ex = c->fz_exec[((uint64_t)steps << 40) | ((uint64_t)ga_blocks << 8) | mask];
+  *out = ex;
   if (ex) return LA3_OK;
   hipGraph_t graph = nullptr;
   HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-  for (uint32_t i = 0; i < steps; i++) fz_step(c, mask, false, nullptr, nullptr);
+  for (uint32_t i = 0; i < steps; i++) fz_step(c, mask, ga_blocks, false, nullptr, nullptr);
   hipError_t e = hipStreamEndCapture(c->stream, &graph);
   if (e != hipSuccess) return fail(c, LA3_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
@@ -1595,6 +1662,7 @@ la3_status fz_capture(la3_ctx* c, uint32_t mask, uint32_t steps) {
     ex = nullptr;
     return fail(c, LA3_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
+  *out = ex;
   return LA3_OK;
 }
 
@@ -1659,7 +1727,7 @@ la3_status sess_add(la3_session* s, const la3_batch& in, uint64_t* id) {
   la3_ctx* c = s->c;
   la3_status st;
   if ((st = fz_check(c, in, s->alpha))) return st;
-  const uint32_t per = 64 * FMAX_WIDTH;
+  const uint32_t per = in.graph->task_k[in.kind];  // queries per task (LA3_TUNE_*_TASK_K)
   const uint32_t need = (in.k + per - 1) / per;
   if (need > sess_free_slots(s)) return fail(c, LA3_EAGAIN, "session: no free task slots");
   const uint64_t bid = s->next_id++;
@@ -1677,7 +1745,7 @@ la3_status sess_add(la3_session* s, const la3_batch& in, uint64_t* id) {
     sl.used = true;
     sl.batch = bid;
     sl.ft = FzTask{0, q0, std::min(per, in.k - q0)};
-    if ((st = fz_setup(c, fb.b, sl.ft, t, s->alpha, s->ht[t]))) return st;
+    if ((st = fz_setup(c, fb.b, sl.ft, t, s->alpha, s->hdr.ga, s->ht[t]))) return st;
     const FTask& T = s->ht[t];
     // Safety net against a run that does not terminate: far more steps than
     // any traversal here takes (BFS <= nslots + 1; SSSP rounds are bounded
@@ -1717,12 +1785,13 @@ la3_status sess_step(la3_session* s) {
   int last_a = -1, last_pull = -1;
   if (mask) {
     if (!c->profiling) {
-      if ((st = fz_capture(c, mask, s->steps))) return st;
-      HIPC(c, hipGraphLaunch(c->fz_exec[s->steps == 1][mask], c->stream));
+      hipGraphExec_t ex = nullptr;
+      if ((st = fz_capture(c, mask, s->steps, s->hdr.ga, &ex))) return st;
+      HIPC(c, hipGraphLaunch(ex, c->stream));
     } else {
       // Profiling: the same steps as separate launches, each bracketed by
       // events (per-kernel device time; the graph has none).
-      for (uint32_t i = 0; i < s->steps; i++) fz_step(c, mask, true, &last_a, &last_pull);
+      for (uint32_t i = 0; i < s->steps; i++) fz_step(c, mask, s->hdr.ga, true, &last_a, &last_pull);
     }
   }
   HIPC(c, hipMemcpyAsync(s->hc, dc, sizeof(FCtl) * s->ntasks, hipMemcpyDeviceToHost, c->stream));
@@ -1767,7 +1836,7 @@ la3_status sess_step(la3_session* s) {
   return LA3_OK;
 }
 
-la3_status sess_init(la3_ctx* c, double alpha, la3_session* s) {
+la3_status sess_init(la3_ctx* c, double alpha, uint32_t blocks_per_cu, la3_session* s) {
   if (c->device < 0) return fail(c, LA3_EDEVICE, "host-only context");
   if (c->fz_busy) return fail(c, LA3_EAGAIN, "the context already runs a session");
   HIPC(c, hipSetDevice(c->device));
@@ -1782,7 +1851,7 @@ la3_status sess_init(la3_ctx* c, double alpha, la3_session* s) {
   s->c = c;
   s->alpha = alpha;
   s->hdr = FHdr{};
-  s->hdr.ga = fz_ga(c);
+  s->hdr.ga = fz_ga(c, blocks_per_cu);
   s->hdr.gb = fz_gb(c);
   HIPC(c, hipMemcpyAsync(c->fz_ctl, &s->hdr, sizeof(FHdr), hipMemcpyHostToDevice, c->stream));
   c->fz_busy = true;
@@ -1802,15 +1871,17 @@ extern "C" {
 
 la3_status la3_batch_run(la3_ctx* c, la3_batch* batches, uint32_t nbatches, double ppr_alpha) {
   if (!c || !batches || nbatches == 0) return LA3_EINVAL;
-  uint32_t need = 0;
+  uint32_t need = 0, bpc = 0;
   for (uint32_t i = 0; i < nbatches; i++) {
     la3_status st = fz_check(c, batches[i], ppr_alpha);
     if (st) return st;
-    need += (batches[i].k + 64 * FMAX_WIDTH - 1) / (64 * FMAX_WIDTH);
+    const uint32_t per = batches[i].graph->task_k[batches[i].kind];
+    need += (batches[i].k + per - 1) / per;
+    bpc = std::max(bpc, batches[i].graph->fused_blocks);
   }
-  if (need > FMAX_TASKS) return fail(c, LA3_EINVAL, "fused run: more than 64 tasks of 256 queries");
+  if (need > FMAX_TASKS) return fail(c, LA3_EINVAL, "fused run: more than 64 tasks");
   auto s = std::make_unique<la3_session>();
-  la3_status st = sess_init(c, ppr_alpha, s.get());
+  la3_status st = sess_init(c, ppr_alpha, bpc, s.get());
   if (st) return st;
   s->steps = FZ_STEPS;
   timing_begin(c);
@@ -1831,7 +1902,7 @@ la3_status la3_session_create(la3_ctx* c, double ppr_alpha, la3_session** out) {
   if (!c || !out) return LA3_EINVAL;
   if (!(ppr_alpha > 0.0 && ppr_alpha < 1.0)) return fail(c, LA3_EINVAL, "alpha must be in (0, 1)");
   auto s = std::make_unique<la3_session>();
-  la3_status st = sess_init(c, ppr_alpha, s.get());
+  la3_status st = sess_init(c, ppr_alpha, FZ_BLOCKS_PER_CU, s.get());
   if (st) return st;
   *out = s.release();
   return LA3_OK;

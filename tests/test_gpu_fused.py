@@ -155,3 +155,33 @@ def test_session_slots_exhausted(gpu_ctx, two_graphs):
             added += 1
     assert added == 16 and s.free_slots() == 0
     s.close()
+
+
+def test_task_width_and_grid_keep_results(gpu_ctx, two_graphs):
+    """Task widths 64 / 128 / 256 and 8 / 16 / 32 blocks per CU (the
+    autotuner's choices) change the split of the work, never the results."""
+    (ga, ra), _ = two_graphs
+    roots = _roots(ra, 200, 81)
+    want_b, _ = L.bfs_batch(ga, roots)
+    want_s, _ = L.sssp_batch(ga, roots)
+    want_p, _, _ = L.ppr_batch(ga, roots[:70], 4)
+    for k in (64, 128, 256):
+        for blocks in (8, 32):
+            for t in (L.TUNE_BFS_TASK_K, L.TUNE_SSSP_TASK_K, L.TUNE_PPR_TASK_K):
+                ga.set_tuning(t, k)
+            ga.set_tuning(L.TUNE_FUSED_BLOCKS, blocks)
+            res = L.batch_run(gpu_ctx, [(ga, L.Q_BFS, roots), (ga, L.Q_SSSP, roots), (ga, L.Q_PPR, roots[:70], 4)])
+            assert np.array_equal(res[0]["out"], want_b)
+            assert np.array_equal(res[1]["out"], want_s)
+            assert np.array_equal(res[2]["out"], want_p)
+    with pytest.raises(L.LA3Error):
+        ga.set_tuning(L.TUNE_SSSP_TASK_K, 100)
+    rep = ga.autotune(roots)
+    assert rep["task_k"][1] in (64, 128, 256) and rep["fused_blocks"] in (8, 16, 32)
+    res = L.batch_run(gpu_ctx, [(ga, L.Q_SSSP, roots)])
+    assert np.array_equal(res[0]["out"], want_s)
+    for t in (L.TUNE_BFS_TASK_K, L.TUNE_SSSP_TASK_K, L.TUNE_PPR_TASK_K):
+        ga.set_tuning(t, 256)
+    ga.set_tuning(L.TUNE_FUSED_BLOCKS, 32)
+    ga.set_tuning(L.TUNE_SSSP_DELTA, 0)
+    ga.set_tuning(L.TUNE_BATCH_SSSP_DELTA, 0)
