@@ -50,15 +50,16 @@ def traversed_edges(outdeg, reached_mask):
     return int(outdeg[reached_mask].sum())
 
 
-def load_traffic(kernel):
-    """HBM bytes per launch of `kernel` measured by rocprofv3 PMC runs,
-    recorded in profiles/traffic.json by the profiling recipe (null if absent)."""
+def load_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` in `workload` measured by rocprofv3 PMC
+    runs, recorded in profiles/traffic.json by the profiling recipe (null if
+    absent)."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
         t = json.load(f)
-    k = t.get(kernel)
+    k = t.get(workload, {}).get(kernel)
     return None if k is None else k.get("bytes_per_dispatch")
 
 
@@ -159,7 +160,7 @@ def timed(ctx, fn, steps, profile=True):
     return elapsed, lat, stats
 
 
-def roofline(stats):
+def roofline(stats, workload=None):
     name, dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
     achieved = dom["alg_bytes"] / (dom["ms"] * 1e-3) / 1e9 if dom["ms"] > 0 else 0.0
     return {
@@ -169,7 +170,7 @@ def roofline(stats):
         "peak": PEAK_HBM_GBS,
         "unit": "GB/s",
         "frac": round(achieved / PEAK_HBM_GBS, 4),
-        "traffic": load_traffic(name),
+        "traffic": load_traffic(name, workload) if workload else None,
         "launches": dom["launches"],
         "avg_launch_us": round(dom["ms"] * 1e3 / max(dom["launches"], 1), 2),
         "alg_bytes_per_launch": round(dom["alg_bytes"] / max(dom["launches"], 1)),
@@ -568,10 +569,10 @@ def main():
         "batched_sequential": {"value": round(gteps_b, 4), "unit": "GTEPS",
                                "ms_per_step": round(el_b * 1e3 / args.steps, 3),
                                "note": "la3_bfs_batch then la3_sssp_batch (host-driven steps)",
-                               "roofline": roofline(st_b)},
+                               "roofline": roofline(st_b, "bfs_sssp_s22")},
         "single_query": single,
         "speedup_vs_single": round(gteps_f / single["value"], 2) if single else None,
-        "roofline": roofline(st_f),
+        "roofline": roofline(st_f, "bfs_sssp_s22"),
         "device_copy_gbs": round(copy_gbs, 1),
         "autotune": {"sssp_delta": tune["sssp_delta"], "batch_sssp_delta": tune["batch_sssp_delta"],
                      "candidates": tune["candidates"],

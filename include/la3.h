@@ -85,6 +85,10 @@ void la3_ctx_destroy(la3_ctx* ctx);
 const char* la3_last_error(const la3_ctx* ctx);
 /* Waits for all work queued on the context's stream. */
 la3_status la3_ctx_synchronize(la3_ctx* ctx);
+/* Frees the device scratch the context caches between calls (per-query
+ * state, work lists, outputs left in device memory, captured step graphs):
+ * later calls allocate again. LA3_EAGAIN while a session runs on it. */
+la3_status la3_ctx_release_scratch(la3_ctx* ctx);
 
 /* -------------------------------------------------------------- graphs --- */
 /* Graph<W>::load_binary (src/matrix/graph.hpp:155-332) at np=1: reads the LA3

@@ -37,7 +37,7 @@ EXPORTS = [
     "la3_server_drain", "la3_server_completed", "la3_server_result", "la3_server_clock",
     "la3_server_destroy", "la3_graph_set_tuning", "la3_graph_get_tuning", "la3_autotune",
     "la3_batch_run", "la3_session_create", "la3_session_add", "la3_session_step",
-    "la3_session_poll", "la3_session_free_slots", "la3_session_destroy",
+    "la3_session_poll", "la3_session_free_slots", "la3_session_destroy", "la3_ctx_release_scratch",
 ]
 (TUNE_SSSP_DELTA, TUNE_BATCH_SSSP_DELTA, TUNE_BFS_TASK_K, TUNE_SSSP_TASK_K, TUNE_PPR_TASK_K,
  TUNE_FUSED_BLOCKS) = range(6)
@@ -163,6 +163,7 @@ def lib():
             "la3_session_poll": (i32, [P, P, P, P, u32, ctypes.POINTER(u32)]),
             "la3_session_free_slots": (u32, [P]),
             "la3_session_destroy": (None, [P]),
+            "la3_ctx_release_scratch": (i32, [P]),
             "la3_rmat_generate": (i32, [u32, u32, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, u64, i32, P, ctypes.POINTER(u64)]),
         }
@@ -265,6 +266,10 @@ class Context:
 
     def synchronize(self):
         self.check(lib().la3_ctx_synchronize(self.h), "synchronize")
+
+    def release_scratch(self):
+        """Frees the device scratch cached between calls."""
+        self.check(lib().la3_ctx_release_scratch(self.h), "release_scratch")
 
     def fetch(self, out):
         """Copies the last program's device-resident main output into `out`."""

@@ -55,6 +55,7 @@ struct la3_ctx {
   uint8_t* fz_ctl = nullptr;
   std::map<uint64_t, hipGraphExec_t> fz_exec;  // by (steps per replay, A grid, launch mask)
   bool fz_busy = false;             // a session holds the task table
+  uint64_t scratch_epoch = 0;       // bumped when all scratch is released
 };
 
 namespace {
@@ -201,6 +202,19 @@ void* scratch(la3_ctx* c, const char* name, size_t bytes) {
   if (hipMalloc(&b.p, want) != hipSuccess) { b.p = nullptr; return nullptr; }
   b.bytes = want;
   return b.p;
+}
+
+// Frees the cached scratch whose names start with `prefix` (all with "").
+void scratch_release(la3_ctx* c, const std::string& prefix) {
+  (void)hipStreamSynchronize(c->stream);
+  for (auto it = c->scratch.begin(); it != c->scratch.end();) {
+    if (it->first.compare(0, prefix.size(), prefix) == 0) {
+      if (it->second.p) (void)hipFree(it->second.p);
+      it = c->scratch.erase(it);
+    } else {
+      ++it;
+    }
+  }
 }
 
 template <class T>
@@ -377,6 +391,23 @@ void la3_ctx_destroy(la3_ctx* c) {
 }
 
 const char* la3_last_error(const la3_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+la3_status la3_ctx_release_scratch(la3_ctx* c) {
+  if (!c) return LA3_EINVAL;
+  if (c->device < 0) return LA3_OK;
+  if (c->fz_busy) return fail(c, LA3_EAGAIN, "a session is running on the context");
+  HIPC(c, hipSetDevice(c->device));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  for (auto& kv : c->fz_exec) (void)hipGraphExecDestroy(kv.second);
+  c->fz_exec.clear();
+  c->fz_tasks = nullptr;
+  c->fz_ctl = nullptr;
+  c->last_result = nullptr;
+  c->last_result_bytes = 0;
+  scratch_release(c, "");
+  c->scratch_epoch++;
+  return LA3_OK;
+}
 
 la3_status la3_ctx_synchronize(la3_ctx* c) {
   if (!c) return LA3_EINVAL;
@@ -723,7 +754,8 @@ static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t
     // loop is captured once (per buffers and parameters) and replayed.
     uint64_t ab;
     std::memcpy(&ab, &alpha, 8);
-    const std::vector<uint64_t> key = {(uint64_t)(uintptr_t)c, iters, (uint64_t)(uintptr_t)x0, (uint64_t)(uintptr_t)x1,
+    const std::vector<uint64_t> key = {(uint64_t)(uintptr_t)c, c->scratch_epoch, iters, (uint64_t)(uintptr_t)x0,
+                                       (uint64_t)(uintptr_t)x1,
                                        (uint64_t)(uintptr_t)rk, (uint64_t)(uintptr_t)ss, (uint64_t)(uintptr_t)deg,
                                        (uint64_t)(uintptr_t)act, (uint64_t)(uintptr_t)part, seed_slot, ab};
     if (!g->pr_exec || g->pr_key != key) {
@@ -1309,7 +1341,8 @@ la3_status la3_autotune(la3_graph* g, const uint32_t* roots, uint32_t nroots, la
     for (int i = 0; i < 3; i++) rep->task_k[i] = 0;
   }
   if (nroots < 64) return LA3_OK;
-  // Fused runs: task width per program, then blocks per CU.
+  // Fused runs: task width per program, then blocks per CU. PPR state is 16
+  // B per vertex and query lane, so its runs take at most 128 seeds.
   const uint32_t kf = std::min<uint32_t>(nroots, 256);
   auto run = [&](std::vector<la3_batch>& bs, double& ms) -> la3_status {
     la3_status st = la3_batch_run(c, bs.data(), (uint32_t)bs.size(), 0.15);  // warm (scratch, graph capture)
@@ -1322,7 +1355,9 @@ la3_status la3_autotune(la3_graph* g, const uint32_t* roots, uint32_t nroots, la
   auto batch = [&](la3_query_kind kind) {
     la3_batch b;
     std::memset(&b, 0, sizeof(b));
-    b.graph = g; b.kind = kind; b.k = kf; b.roots = roots; b.iters = kind == LA3_Q_PPR ? 5 : 0;
+    b.graph = g; b.kind = kind; b.roots = roots;
+    b.k = kind == LA3_Q_PPR ? std::min<uint32_t>(kf, 128) : kf;
+    b.iters = kind == LA3_Q_PPR ? 5 : 0;
     return b;
   };
   static const uint32_t widths[3] = {64, 128, 256};
@@ -1331,6 +1366,8 @@ la3_status la3_autotune(la3_graph* g, const uint32_t* roots, uint32_t nroots, la
     double best = 1e30;
     uint32_t pick = 256;
     for (int wi = 0; wi < 3; wi++) {
+      if (rep) rep->task_ms[kind][wi] = 0;
+      if (wi > 0 && widths[wi - 1] >= batch((la3_query_kind)kind).k) continue;  // same split as the narrower width
       g->task_k[kind] = widths[wi];
       std::vector<la3_batch> bs = {batch((la3_query_kind)kind)};
       double ms = 0;
@@ -1355,6 +1392,8 @@ la3_status la3_autotune(la3_graph* g, const uint32_t* roots, uint32_t nroots, la
   }
   g->fused_blocks = pick;
   if (rep) rep->fused_blocks = pick;
+  // The trial runs' task-slot buffers are not kept.
+  scratch_release(c, "fz");
   return LA3_OK;
 }
 
@@ -1841,8 +1880,8 @@ la3_status sess_init(la3_ctx* c, double alpha, uint32_t blocks_per_cu, la3_sessi
   if (c->fz_busy) return fail(c, LA3_EAGAIN, "the context already runs a session");
   HIPC(c, hipSetDevice(c->device));
   if (!c->fz_tasks) {
-    c->fz_tasks = (FTask*)scratch(c, "fz.tasks", sizeof(FTask) * FMAX_TASKS);
-    c->fz_ctl = (uint8_t*)scratch(c, "fz.ctl", sizeof(FHdr) + sizeof(FCtl) * FMAX_TASKS);
+    c->fz_tasks = (FTask*)scratch(c, "fused.tasks", sizeof(FTask) * FMAX_TASKS);
+    c->fz_ctl = (uint8_t*)scratch(c, "fused.ctl", sizeof(FHdr) + sizeof(FCtl) * FMAX_TASKS);
     if (!c->fz_tasks || !c->fz_ctl) {
       c->fz_tasks = nullptr;
       return fail(c, LA3_ENOMEM, "fused run: device allocation failed");

@@ -185,3 +185,19 @@ def test_task_width_and_grid_keep_results(gpu_ctx, two_graphs):
     ga.set_tuning(L.TUNE_FUSED_BLOCKS, 32)
     ga.set_tuning(L.TUNE_SSSP_DELTA, 0)
     ga.set_tuning(L.TUNE_BATCH_SSSP_DELTA, 0)
+
+
+def test_release_scratch_then_run_again(gpu_ctx, two_graphs):
+    (ga, ra), (gb, rb) = two_graphs
+    roots = _roots(ra, 50, 91)
+    first = L.batch_run(gpu_ctx, [(ga, L.Q_SSSP, roots), (gb, L.Q_PPR, roots[:5] % 2048, 5)])
+    pr1 = L.pagerank(ga, 7)[0]
+    gpu_ctx.release_scratch()
+    again = L.batch_run(gpu_ctx, [(ga, L.Q_SSSP, roots), (gb, L.Q_PPR, roots[:5] % 2048, 5)])
+    assert np.array_equal(first[0]["out"], again[0]["out"])
+    assert np.array_equal(first[1]["out"], again[1]["out"])
+    assert np.array_equal(pr1, L.pagerank(ga, 7)[0])
+    s = L.Session(gpu_ctx)
+    with pytest.raises(L.LA3Error):
+        gpu_ctx.release_scratch()      # not while a session runs
+    s.close()
