@@ -738,22 +738,67 @@ __global__ void __launch_bounds__(BLOCK) k_tc_entry_rows(DG g, uint32_t* row_of)
     for (uint32_t j = g.row_ptr[r]; j < g.row_ptr[r + 1]; j++) row_of[j] = r;
 }
 
+// Number of keys of list A = cols[ab, ae) found in list B = cols[bb, be)
+// (both sorted by message-order key), with the 64 lanes of a wavefront each
+// binary-searching one element of A in B. Every lane returns the count.
+__device__ uint32_t tc_intersect_wave(const DG& g, uint32_t ab, uint32_t ae, uint32_t bb, uint32_t be) {
+  uint32_t found = 0;
+  for (uint32_t i0 = ab; i0 < ae; i0 += WAVE) {
+    const uint32_t i = i0 + lane_id();
+    bool hit = false;
+    if (i < ae) {
+      const uint32_t k = slot_key(g, g.row_cols[i]);
+      uint32_t l = bb, h = be;
+      while (l < h) {
+        const uint32_t m = (l + h) >> 1;
+        if (slot_key(g, g.row_cols[m]) < k) l = m + 1; else h = m;
+      }
+      hit = l < be && slot_key(g, g.row_cols[l]) == k;
+    }
+    found += __popcll(__ballot(hit));
+  }
+  return found;
+}
+
 // total[0] += triangles; total[1] += sum over entries of |N-(u)| + |N-(v)|
 // (the list entries a merge intersection reads: the algorithmic bytes).
+// Lane per entry (u, v); a pair whose shorter list has more than
+// TC_WAVE_MIN entries is intersected by the whole wavefront instead, so a
+// pair of hub lists is not one lane's serial chain of binary searches.
+constexpr uint32_t TC_WAVE_MIN = 32;
+
 __global__ void __launch_bounds__(BLOCK) k_tc_count(DG g, const uint32_t* row_of, uint32_t* per_slot,
                                                     unsigned long long* total) {
   unsigned long long acc = 0, work = 0;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < g.nnz; base += stride) {
     const uint64_t j = base + threadIdx.x;
+    uint32_t u = 0, v = 0, ub = 0, ue = 0, vb = 0, ve = 0;
+    bool pair = false;
     if (j < g.nnz) {
-      const uint32_t u = row_of[j], v = g.row_cols[j];
+      u = row_of[j];
+      v = g.row_cols[j];
       if (v < g.nrows) {
-        const uint32_t ub = g.row_ptr[u], ue = g.row_ptr[u + 1], vb = g.row_ptr[v], ve = g.row_ptr[v + 1];
-        const uint32_t c = tc_intersect(g, ub, ue, vb, ve);
+        pair = true;
+        ub = g.row_ptr[u]; ue = g.row_ptr[u + 1]; vb = g.row_ptr[v]; ve = g.row_ptr[v + 1];
         work += (ue - ub) + (ve - vb);
-        if (c) { atomicAdd(&per_slot[v], c); acc += c; }
       }
+    }
+    const bool heavy = pair && min(ue - ub, ve - vb) > TC_WAVE_MIN;
+    if (pair && !heavy) {
+      const uint32_t c = tc_intersect(g, ub, ue, vb, ve);
+      if (c) { atomicAdd(&per_slot[v], c); acc += c; }
+    }
+    uint64_t hm = __ballot(heavy);
+    while (hm) {  // wave-uniform
+      const int src = __ffsll((long long)hm) - 1;
+      hm &= hm - 1;
+      const uint32_t hv = __shfl(v, src, WAVE);
+      uint32_t ab = __shfl(ub, src, WAVE), ae = __shfl(ue, src, WAVE);
+      uint32_t bb = __shfl(vb, src, WAVE), be = __shfl(ve, src, WAVE);
+      if (ae - ab > be - bb) { uint32_t t = ab; ab = bb; bb = t; t = ae; ae = be; be = t; }
+      const uint32_t c = tc_intersect_wave(g, ab, ae, bb, be);
+      if (lane_id() == 0 && c) { atomicAdd(&per_slot[hv], c); acc += c; }
     }
   }
   acc = wave_sum(acc);
