@@ -374,6 +374,21 @@ la3_status la3_kernel_stats_reset(la3_ctx* ctx);
  * the PCIe transfer. */
 la3_status la3_result_fetch(la3_ctx* ctx, void* dst, uint64_t bytes);
 
+/* Top-k of each row of the last device-resident result (VertexProgram::topk,
+ * vertex_program.hpp:607-680, one row per query for batched results): the
+ * result is read as rows x n values of `dtype` by original id (the layout of
+ * every program's output; rows = 1 for single queries); per row the k
+ * largest (largest != 0) or smallest values, ordered by value and then by
+ * smaller id (the reference's std::partial_sort leaves the order of equal
+ * values unspecified). ids[r * k + j] and values[r * k + j] (values may be
+ * NULL); rows with fewer than k values pad with id 0xffffffff. Selection
+ * runs on the device (radix select and an ordered compaction); only k
+ * entries per row cross PCIe. A fused run leaves its last task's rows as the
+ * last result. */
+typedef enum { LA3_DT_U8 = 0, LA3_DT_U32 = 1, LA3_DT_F32 = 2, LA3_DT_F64 = 3 } la3_dtype;
+la3_status la3_result_topk(la3_ctx* ctx, la3_dtype dtype, uint32_t rows, uint64_t n, uint32_t k, int largest,
+                           uint32_t* ids, void* values);
+
 /* Page-locked host memory (hipHostMalloc) for result buffers: device-to-host
  * copies into it run at full link speed. Any la3_* output pointer may point
  * into it. */

@@ -14,6 +14,7 @@
 #ifndef LA3_HPP
 #define LA3_HPP
 
+#include <algorithm>
 #include <cstdint>
 #include <functional>
 #include <stdexcept>
@@ -121,6 +122,21 @@ class ProgramBase {
     Value acc = Value();
     for (uint32_t vid : G_->slot_ids()) red(acc, map(vid, states_[vid]));
     return acc;
+  }
+
+  // topk (vertex_program.hpp:607-680) at one rank: (id, map(id, state)) pairs
+  // in slot order, std::partial_sort by cmp, the first k. Same input order
+  // and algorithm, so equal values come out as in the reference. For large
+  // outputs left on the device, la3_result_topk selects on the GPU.
+  template <class I, class V, class Mapper, class Comparator>
+  void topk(uint32_t k, std::vector<std::pair<I, V>>& out, Mapper map, Comparator cmp,
+            bool /*active_only*/ = false) const {
+    std::vector<std::pair<I, V>> ivs;
+    for (uint32_t vid : G_->slot_ids()) ivs.push_back(std::pair<I, V>((I)vid, map(vid, states_[vid])));
+    const size_t kk = std::min<size_t>(k, ivs.size());
+    std::partial_sort(ivs.begin(), ivs.begin() + kk, ivs.end(), cmp);
+    ivs.resize(kk);
+    out = ivs;
   }
 
  protected:

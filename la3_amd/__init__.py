@@ -38,6 +38,7 @@ EXPORTS = [
     "la3_server_destroy", "la3_graph_set_tuning", "la3_graph_get_tuning", "la3_autotune",
     "la3_batch_run", "la3_session_create", "la3_session_add", "la3_session_step",
     "la3_session_poll", "la3_session_free_slots", "la3_session_destroy", "la3_ctx_release_scratch",
+    "la3_result_topk",
 ]
 (TUNE_SSSP_DELTA, TUNE_BATCH_SSSP_DELTA, TUNE_BFS_TASK_K, TUNE_SSSP_TASK_K, TUNE_PPR_TASK_K,
  TUNE_FUSED_BLOCKS) = range(6)
@@ -164,6 +165,7 @@ def lib():
             "la3_session_free_slots": (u32, [P]),
             "la3_session_destroy": (None, [P]),
             "la3_ctx_release_scratch": (i32, [P]),
+            "la3_result_topk": (i32, [P, i32, u32, u64, u32, i32, P, P]),
             "la3_rmat_generate": (i32, [u32, u32, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, u64, i32, P, ctypes.POINTER(u64)]),
         }
@@ -275,6 +277,19 @@ class Context:
         """Copies the last program's device-resident main output into `out`."""
         self.check(lib().la3_result_fetch(self.h, _ptr(out), out.nbytes), "result_fetch")
         return out
+
+    def topk(self, dtype, rows, n, k, largest=True):
+        """Top-k of each row of the last device-resident result (rows x n
+        values of numpy `dtype` by id): (ids[rows, k], values[rows, k]),
+        ordered by value, then smaller id; padded with id 0xffffffff."""
+        dt = np.dtype(dtype)
+        code = {np.dtype(np.uint8): 0, np.dtype(np.uint32): 1, np.dtype(np.float32): 2,
+                np.dtype(np.float64): 3}[dt]
+        ids = np.zeros((rows, k), dtype=np.uint32)
+        vals = np.zeros((rows, k), dtype=dt)
+        self.check(lib().la3_result_topk(self.h, code, int(rows), int(n), int(k), int(bool(largest)),
+                                         _ptr(ids), _ptr(vals)), "result_topk")
+        return ids, vals
 
     def copy_bandwidth(self, nbytes=1 << 30, reps=20):
         """Measured device copy GB/s (read + write) of a plain copy kernel."""

@@ -129,6 +129,14 @@ int main(int argc, char** argv) {
       long sum = vp.reduce<long>([](uint32_t, const la3::DegState& s) -> long { return s.degree; },
                                  [](long& a, const long& b) { a += b; });
       printf("%s-Degree Checksum = %lu \n", out ? "Out" : "In", sum);
+      // degree.cpp:34-43
+      constexpr int k = 3;
+      using iv_t = std::pair<uint32_t, uint32_t>;
+      std::vector<iv_t> top;
+      vp.topk<uint32_t, uint32_t>(k, top, [](uint32_t, const la3::DegState& s) { return s.degree; },
+                                  [](const iv_t& a, const iv_t& b) { return a.second > b.second; });
+      printf("Top-%i: \n", k);
+      for (const auto& iv : top) printf("idx %u: degree %u \n", iv.first, iv.second);
     } else {
       fprintf(stderr, "unknown app %s\n", app.c_str());
       return 1;

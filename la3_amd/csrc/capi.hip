@@ -22,6 +22,7 @@
 #include "kernels_dev.h"
 #include "kernels_batch.h"
 #include "kernels_fused.h"
+#include "kernels_topk.h"
 
 using namespace la3;
 
@@ -1993,6 +1994,114 @@ void la3_session_destroy(la3_session* s) {
   if (!s) return;
   sess_release(s);
   delete s;
+}
+
+}  // extern "C"
+
+// ============================================================== top-k ===
+extern "C" {
+
+la3_status la3_result_topk(la3_ctx* c, la3_dtype dtype, uint32_t rows, uint64_t n, uint32_t k, int largest,
+                           uint32_t* ids, void* values) {
+  if (!c || !ids || rows == 0 || n == 0 || k == 0 || n > 0xffffffffull) return LA3_EINVAL;
+  if (dtype < LA3_DT_U8 || dtype > LA3_DT_F64) return fail(c, LA3_EINVAL, "unknown value type");
+  if (c->device < 0) return fail(c, LA3_EDEVICE, "host-only context");
+  const uint32_t esz = dtype == LA3_DT_U8 ? 1 : dtype == LA3_DT_F64 ? 8 : 4;
+  if (!c->last_result || (uint64_t)rows * n * esz > c->last_result_bytes)
+    return fail(c, LA3_EINVAL, "no device-resident result of that size (call a program with a NULL output first)");
+  HIPC(c, hipSetDevice(c->device));
+  timing_begin(c);
+  const void* v = c->last_result;
+  const uint32_t bits = dtype == LA3_DT_U8 ? 8 : dtype == LA3_DT_F64 ? 64 : 32;
+  const uint32_t kk = (uint32_t)std::min<uint64_t>(k, n);
+  const uint32_t chunks = (uint32_t)((n + TK_CHUNK - 1) / TK_CHUNK);
+  const uint32_t nb = rows * chunks;
+  uint64_t* dprefix = (uint64_t*)scratch(c, "topk.prefix", 8ull * rows);
+  uint32_t* dhist = (uint32_t*)scratch(c, "topk.hist", 4ull * 256 * rows);
+  uint32_t* dceq = (uint32_t*)scratch(c, "topk.ceq", 4ull * nb);
+  uint32_t* dsmall = (uint32_t*)scratch(c, "topk.small", 4ull * 4 * rows);  // need, gtc, gt_next, cnt
+  uint32_t* dbase = (uint32_t*)scratch(c, "topk.base", 4ull * nb);
+  uint32_t* dids = (uint32_t*)scratch(c, "topk.ids", 4ull * rows * k);
+  void* dvals = scratch(c, "topk.vals", (size_t)esz * rows * k);
+  if (!dprefix || !dhist || !dceq || !dsmall || !dbase || !dids || !dvals)
+    return fail(c, LA3_ENOMEM, "top-k: device allocation failed");
+  // 1. radix select of the kk-th key per row, 8 bits per pass from the top
+  std::vector<uint64_t> prefix(rows, 0);
+  std::vector<uint32_t> above(rows, 0), hist(256ull * rows);
+  uint64_t hi_mask = 0;
+  for (int shift = (int)bits - 8; shift >= 0; shift -= 8) {
+    HIPC(c, hipMemcpyAsync(dprefix, prefix.data(), 8ull * rows, hipMemcpyHostToDevice, c->stream));
+    launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(256ull * rows), dhist, 256ull * rows, 0u);
+    launch(c, "topk_hist", k_topk_hist, nb, v, (uint32_t)dtype, largest, n, chunks, (const uint64_t*)dprefix, hi_mask,
+           (uint32_t)shift, dhist);
+    HIPC(c, hipMemcpyAsync(hist.data(), dhist, 4ull * 256 * rows, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    for (uint32_t r = 0; r < rows; r++) {
+      uint32_t acc = above[r];
+      int d = 255;
+      for (; d > 0; d--) {
+        if (acc + hist[256ull * r + d] >= kk) break;
+        acc += hist[256ull * r + d];
+      }
+      prefix[r] |= (uint64_t)d << shift;
+      above[r] = acc;
+    }
+    hi_mask |= 0xffull << shift;
+  }
+  // 2. ordered compaction: every key above the threshold, equal keys by id
+  std::vector<uint32_t> small(4ull * rows, 0), ceq(nb), base(nb);
+  HIPC(c, hipMemcpyAsync(dprefix, prefix.data(), 8ull * rows, hipMemcpyHostToDevice, c->stream));
+  launch(c, "topk_count", k_topk_count_eq, nb, v, (uint32_t)dtype, largest, n, chunks, (const uint64_t*)dprefix, dceq);
+  HIPC(c, hipMemcpyAsync(ceq.data(), dceq, 4ull * nb, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  for (uint32_t r = 0; r < rows; r++) {
+    uint32_t b = 0;
+    for (uint32_t ch = 0; ch < chunks; ch++) { base[(size_t)r * chunks + ch] = b; b += ceq[(size_t)r * chunks + ch]; }
+    small[r] = kk - above[r];           // need
+    small[rows + r] = above[r];         // gtc
+    small[3ull * rows + r] = kk;        // cnt
+  }
+  HIPC(c, hipMemcpyAsync(dsmall, small.data(), 16ull * rows, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, hipMemcpyAsync(dbase, base.data(), 4ull * nb, hipMemcpyHostToDevice, c->stream));
+  launch(c, "topk_write", k_topk_write, nb, v, (uint32_t)dtype, largest, n, chunks, (const uint64_t*)dprefix,
+         (const uint32_t*)dsmall, (const uint32_t*)(dsmall + rows), (const uint32_t*)dbase, dsmall + 2 * rows, k, dids);
+  launch(c, "topk_gather", k_topk_gather, blocks_for((uint64_t)rows * k), v, esz, n, rows, k,
+         (const uint32_t*)(dsmall + 3 * rows), (const uint32_t*)dids, dvals);
+  std::vector<uint32_t> hid((size_t)rows * k);
+  std::vector<uint8_t> hv((size_t)rows * k * esz);
+  HIPC(c, hipMemcpyAsync(hid.data(), dids, 4ull * rows * k, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipMemcpyAsync(hv.data(), dvals, (size_t)esz * rows * k, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  timing_end(c);
+  // 3. order each row's kk entries: value (largest or smallest first), then id
+  for (uint32_t r = 0; r < rows; r++) {
+    std::vector<uint32_t> idx(kk);
+    for (uint32_t j = 0; j < kk; j++) idx[j] = j;
+    auto val = [&](uint32_t j) -> double {
+      const size_t x = (size_t)r * k + j;
+      if (esz == 1) return hv[x];
+      if (dtype == LA3_DT_U32) { uint32_t u; std::memcpy(&u, &hv[x * 4], 4); return u; }
+      if (dtype == LA3_DT_F32) { float f; std::memcpy(&f, &hv[x * 4], 4); return f; }
+      double d; std::memcpy(&d, &hv[x * 8], 8); return d;
+    };
+    std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+      const double va = val(a), vb = val(b);
+      if (va != vb) return largest ? va > vb : va < vb;
+      return hid[(size_t)r * k + a] < hid[(size_t)r * k + b];
+    });
+    for (uint32_t j = 0; j < k; j++) {
+      const size_t o = (size_t)r * k + j;
+      if (j >= kk) {
+        ids[o] = 0xffffffffu;
+        if (values) std::memset((uint8_t*)values + o * esz, 0, esz);
+        continue;
+      }
+      const size_t x = (size_t)r * k + idx[j];
+      ids[o] = hid[x];
+      if (values) std::memcpy((uint8_t*)values + o * esz, &hv[x * esz], esz);
+    }
+  }
+  return LA3_OK;
 }
 
 }  // extern "C"

@@ -50,3 +50,18 @@ def test_cc_tc_degree_drivers(g1_files, g1):
     assert f"Checksum = {g1['cc_checksum']}" in run("cc", g1_files[0], 8)
     assert f"Triangles = {g1['tc_total']}" in run("tc", g1_files[0], 8)
     assert "Out-Degree Checksum = 13" in run("degree", g1_files[0], 8, "-o")
+
+
+def test_degree_driver_top3(g1_files, g1):
+    """degree.cpp:34-43: Top-3 by in-degree (std::partial_sort over the
+    states in slot order; equal degrees may come in either order)."""
+    import re
+    import numpy as np
+    out = run("degree", g1_files[0], 8)
+    top = [(int(a), int(b)) for a, b in re.findall(r"idx (\d+): degree (\d+)", out)]
+    assert "Top-3:" in out and len(top) == 3
+    rec = g1["records"]
+    indeg = np.bincount(rec[:, 1], minlength=9)
+    assert [d for _, d in top] == sorted(indeg, reverse=True)[:3]
+    assert all(indeg[i] == d for i, d in top)
+
