@@ -231,7 +231,10 @@ la3_status la3_batch_run(la3_ctx* ctx, la3_batch* batches, uint32_t nbatches, do
  * la3_batch_run is one session run to completion. A context holds at most one
  * session at a time (LA3_EAGAIN otherwise); a session has 64 task slots of
  * 256 queries (la3_session_add returns LA3_EAGAIN when the batch does not
- * fit). Each batch's results are those of la3_batch_run for it alone. */
+ * fit). Each batch's results are those of la3_batch_run for it alone. A
+ * task slot keeps its device buffers for the next task up to 256 MB and
+ * frees larger ones when its task finishes (la3_batch_run keeps them all
+ * until la3_ctx_release_scratch). */
 typedef struct la3_session la3_session;
 la3_status la3_session_create(la3_ctx* ctx, double ppr_alpha, la3_session** out);
 la3_status la3_session_add(la3_session* s, const la3_batch* batch, uint64_t* id);

@@ -1481,6 +1481,7 @@ namespace {
 
 constexpr uint32_t FZ_STEPS = 4;
 constexpr uint32_t FZ_BLOCKS_PER_CU = 32;
+constexpr size_t FZ_SLOT_KEEP_BYTES = 256ull << 20;  // a finished session slot keeps up to this much
 
 struct FzTask {
   uint32_t batch, q0, k;
@@ -1749,6 +1750,7 @@ struct la3_session {
   FHdr hdr{};               // staging for the header
   uint32_t ntasks = 0;      // slots [0, ntasks) are known to the device
   uint32_t steps = 1;       // steps per replay
+  bool keep_slots = false;  // keep every slot's buffers (la3_batch_run: the caller releases)
   bool pending_init = false;
   std::map<uint64_t, FzBatch> batches;
   std::deque<uint64_t> done;
@@ -1779,13 +1781,29 @@ la3_status sess_add(la3_session* s, const la3_batch& in, uint64_t* id) {
   FHdr* dh = (FHdr*)c->fz_ctl;
   FCtl* dc = (FCtl*)(c->fz_ctl + sizeof(FHdr));
   uint32_t t = 0;
+  std::vector<uint32_t> taken;
+  // A failed set-up (device memory) gives the batch's slots back and frees
+  // what they allocated.
+  auto undo = [&](la3_status e) {
+    for (uint32_t u : taken) {
+      s->slot[u].used = false;
+      s->fresh[u] = FCtl{};
+      s->fresh[u].state = FS_FREE;
+      s->fresh[u].done = 1;
+      (void)hipMemcpyAsync(dc + u, &s->fresh[u], sizeof(FCtl), hipMemcpyHostToDevice, c->stream);
+      scratch_release(c, fz_name(u, ""));
+    }
+    s->batches.erase(bid);
+    return e;
+  };
   for (uint32_t q0 = 0; q0 < in.k; q0 += per) {
     while (s->slot[t].used) t++;
     la3_session::Slot& sl = s->slot[t];
     sl.used = true;
     sl.batch = bid;
+    taken.push_back(t);
     sl.ft = FzTask{0, q0, std::min(per, in.k - q0)};
-    if ((st = fz_setup(c, fb.b, sl.ft, t, s->alpha, s->hdr.ga, s->ht[t]))) return st;
+    if ((st = fz_setup(c, fb.b, sl.ft, t, s->alpha, s->hdr.ga, s->ht[t]))) return undo(st);
     const FTask& T = s->ht[t];
     // Safety net against a run that does not terminate: far more steps than
     // any traversal here takes (BFS <= nslots + 1; SSSP rounds are bounded
@@ -1793,8 +1811,9 @@ la3_status sess_add(la3_session* s, const la3_batch& in, uint64_t* id) {
     sl.max_steps = T.kind == FK_PPR ? (uint64_t)T.iters + 8 : 8ull * T.g.nslots + 4096;
     s->fresh[t] = FCtl{};
     s->fresh[t].state = FS_NEW;
-    HIPC(c, hipMemcpyAsync(c->fz_tasks + t, &s->ht[t], sizeof(FTask), hipMemcpyHostToDevice, c->stream));
-    HIPC(c, hipMemcpyAsync(dc + t, &s->fresh[t], sizeof(FCtl), hipMemcpyHostToDevice, c->stream));
+    if (hipMemcpyAsync(c->fz_tasks + t, &s->ht[t], sizeof(FTask), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(dc + t, &s->fresh[t], sizeof(FCtl), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+      return undo(fail(c, LA3_EDEVICE, "session: task upload failed"));
     s->ntasks = std::max(s->ntasks, t + 1);
   }
   s->hdr.ntasks = s->ntasks;
@@ -1839,6 +1858,7 @@ la3_status sess_step(la3_session* s) {
   double bytes = 0, pbytes = 0;
   uint64_t edges = 0;
   bool finished = false;
+  std::vector<uint32_t> freed;
   for (uint32_t t = 0; t < s->ntasks; t++) {
     la3_session::Slot& sl = s->slot[t];
     if (!sl.used) continue;
@@ -1866,6 +1886,7 @@ la3_status sess_step(la3_session* s) {
     sl.used = false;
     if (--fb.tasks_left == 0) s->done.push_back(sl.batch);
     finished = true;
+    freed.push_back(t);
   }
   // Bytes of the finished tasks: bottom-up BFS steps to k_fused_pull, the
   // rest to k_fused_a (the B and control launches move no counted bytes).
@@ -1873,6 +1894,17 @@ la3_status sess_step(la3_session* s) {
   annotate(c, last_pull, pbytes, 0);
   c->edges_scanned += edges;
   if (finished) HIPC(c, hipStreamSynchronize(c->stream));
+  // A slot keeps its buffers for the next task unless they are large: each
+  // of the 64 slots growing to the largest task it ever ran would hold far
+  // more memory than the running tasks need.
+  for (uint32_t t : freed) {
+    if (s->keep_slots) break;
+    const std::string pre = fz_name(t, "");
+    size_t held = 0;
+    for (const auto& kv : c->scratch)
+      if (kv.first.compare(0, pre.size(), pre) == 0) held += kv.second.bytes;
+    if (held > FZ_SLOT_KEEP_BYTES) scratch_release(c, pre);
+  }
   return LA3_OK;
 }
 
@@ -1924,6 +1956,7 @@ la3_status la3_batch_run(la3_ctx* c, la3_batch* batches, uint32_t nbatches, doub
   la3_status st = sess_init(c, ppr_alpha, bpc, s.get());
   if (st) return st;
   s->steps = FZ_STEPS;
+  s->keep_slots = true;
   timing_begin(c);
   std::vector<uint64_t> ids(nbatches);
   for (uint32_t i = 0; i < nbatches && !st; i++) st = sess_add(s.get(), batches[i], &ids[i]);
