@@ -89,6 +89,11 @@ la3_status la3_ctx_synchronize(la3_ctx* ctx);
  * state, work lists, outputs left in device memory, captured step graphs):
  * later calls allocate again. LA3_EAGAIN while a session runs on it. */
 la3_status la3_ctx_release_scratch(la3_ctx* ctx);
+/* Where la3_graph_load_* builds a graph of a device context: on the device
+ * (default: records uploaded, classes / slots / sorted DCSC and row-major
+ * companion built with device scans and radix sorts) or on the host
+ * (multi-threaded builder). Both give the same arrays. */
+la3_status la3_ctx_set_ingest(la3_ctx* ctx, int on_device);
 
 /* -------------------------------------------------------------- graphs --- */
 /* Graph<W>::load_binary (src/matrix/graph.hpp:155-332) at np=1: reads the LA3
@@ -116,6 +121,20 @@ la3_status la3_graph_slots(const la3_graph* g, uint32_t* hid_of);
  * regular or the sink rows), idx (hashed row id) and weight. */
 la3_status la3_graph_csc(const la3_graph* g, int which, uint32_t* colptrs, uint32_t* colidxs,
                          uint32_t* global_idx, uint32_t* idx, uint32_t* weight);
+/* Any array of the graph's device layout, as stored (the layout the kernels
+ * read: DCSC with a regular/sink split per column, row-major companion in
+ * message order, id maps by slot, long-row and PPR-segment lists). With
+ * out == NULL only *count is set; cap = capacity of out in elements. For
+ * parity checks of the ingest paths and for debugging; no counterpart in the
+ * reference. */
+typedef enum {
+  LA3_ARR_COL_PTR = 0, LA3_ARR_COL_SPLIT, LA3_ARR_COL_ROWS, LA3_ARR_COL_W,
+  LA3_ARR_ROW_PTR, LA3_ARR_ROW_SPLIT, LA3_ARR_ROW_COLS, LA3_ARR_ROW_W,
+  LA3_ARR_ID_OF, LA3_ARR_SLOT_OF_ID, LA3_ARR_LONG_ROWS,
+  LA3_ARR_PLONG, LA3_ARR_PSEG_FIRST, LA3_ARR_PSEG_ROW, LA3_ARR_PSEG_BEG
+} la3_array;
+la3_status la3_graph_array(const la3_graph* g, la3_array which, uint32_t* out, uint64_t cap,
+                           uint64_t* count);
 /* hasher->hash / unhash (src/matrix/hashers.h:432-444). */
 uint32_t la3_graph_hash(const la3_graph* g, uint32_t v);
 uint32_t la3_graph_unhash(const la3_graph* g, uint32_t v);

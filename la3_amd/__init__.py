@@ -38,7 +38,7 @@ EXPORTS = [
     "la3_server_destroy", "la3_graph_set_tuning", "la3_graph_get_tuning", "la3_autotune",
     "la3_batch_run", "la3_session_create", "la3_session_add", "la3_session_step",
     "la3_session_poll", "la3_session_free_slots", "la3_session_destroy", "la3_ctx_release_scratch",
-    "la3_result_topk",
+    "la3_result_topk", "la3_ctx_set_ingest", "la3_graph_array",
 ]
 (TUNE_SSSP_DELTA, TUNE_BATCH_SSSP_DELTA, TUNE_BFS_TASK_K, TUNE_SSSP_TASK_K, TUNE_PPR_TASK_K,
  TUNE_FUSED_BLOCKS) = range(6)
@@ -166,6 +166,8 @@ def lib():
             "la3_session_destroy": (None, [P]),
             "la3_ctx_release_scratch": (i32, [P]),
             "la3_result_topk": (i32, [P, i32, u32, u64, u32, i32, P, P]),
+            "la3_ctx_set_ingest": (i32, [P, i32]),
+            "la3_graph_array": (i32, [P, i32, P, u64, ctypes.POINTER(u64)]),
             "la3_rmat_generate": (i32, [u32, u32, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, u64, i32, P, ctypes.POINTER(u64)]),
         }
@@ -268,6 +270,10 @@ class Context:
 
     def synchronize(self):
         self.check(lib().la3_ctx_synchronize(self.h), "synchronize")
+
+    def set_ingest(self, on_device):
+        """Build loaded graphs on the device (True, default) or the host."""
+        self.check(lib().la3_ctx_set_ingest(self.h, int(bool(on_device))), "set_ingest")
 
     def release_scratch(self):
         """Frees the device scratch cached between calls."""
@@ -383,6 +389,20 @@ class Graph:
                     batch_sssp_delta=rep.batch_sssp_delta, batch_k=rep.batch_k,
                     task_k=list(rep.task_k), task_ms=[list(r) for r in rep.task_ms],
                     fused_blocks=rep.fused_blocks, fused_ms=list(rep.fused_ms))
+
+    ARRAYS = ("col_ptr", "col_split", "col_rows", "col_w", "row_ptr", "row_split", "row_cols",
+              "row_w", "id_of", "slot_of_id", "long_rows", "plong", "pseg_first", "pseg_row",
+              "pseg_beg")
+
+    def array(self, name):
+        """One array of the stored layout (la3_graph_array)."""
+        which = self.ARRAYS.index(name)
+        n = ctypes.c_uint64()
+        self.ctx.check(lib().la3_graph_array(self.h, which, None, 0, ctypes.byref(n)), "graph_array")
+        out = np.zeros(n.value, dtype=np.uint32)
+        self.ctx.check(lib().la3_graph_array(self.h, which, _ptr(out), n.value, ctypes.byref(n)),
+                       "graph_array")
+        return out
 
     def csc(self, which):
         nnz = self.info["nnz_sink"] if which else self.info["nnz_regular"]

@@ -23,6 +23,9 @@
 #include "kernels_batch.h"
 #include "kernels_fused.h"
 #include "kernels_topk.h"
+#include "kernels_ingest.h"
+
+#include <hipcub/hipcub.hpp>
 
 using namespace la3;
 
@@ -56,6 +59,7 @@ struct la3_ctx {
   uint8_t* fz_ctl = nullptr;
   std::map<uint64_t, hipGraphExec_t> fz_exec;  // by (steps per replay, A grid, launch mask)
   bool fz_busy = false;             // a session holds the task table
+  bool ingest_device = true;        // build loaded graphs on the device (else on the host)
   uint64_t scratch_epoch = 0;       // bumped when all scratch is released
 };
 
@@ -279,11 +283,277 @@ la3_status graph_from_layout(la3_ctx* ctx, HostLayout& L, la3_graph** out) {
   return LA3_OK;
 }
 
+// ------------------------------------------------------ device ingest ---
+// Device buffers of one ingest, freed together unless kept for the graph.
+struct DevTemps {
+  std::vector<void*> ptrs;
+  ~DevTemps() { for (void* p : ptrs) (void)hipFree(p); }
+  template <class T>
+  T* get(size_t count) {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(count * sizeof(T), 4)) != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    return (T*)p;
+  }
+  // Hands a buffer over to the graph (not freed here).
+  void keep(void* p) {
+    for (auto& q : ptrs) if (q == p) q = nullptr;
+  }
+};
+
+// The layout of kernels_ingest.h built on the device; fills g (counts, DG,
+// host id maps). Equal, array for array, to build_layout + graph_from_layout.
+la3_status ingest_device(la3_ctx* c, const uint32_t* rec, uint64_t nrec, uint32_t n, bool weighted, int mode,
+                         bool bucket, la3_graph* g) {
+  hipStream_t st = c->stream;
+  const bool directed = mode != 2, reverse = mode == 1 || mode == 4, acyclic = mode == 3 || mode == 4;
+  const uint32_t RW = weighted ? 3 : 2;
+  const uint32_t S = n + 1;
+  DevTemps T;
+  g->n = n; g->nslots = S; g->directed = directed; g->weighted = weighted; g->mode = mode;
+  g->nrecords = nrec;
+  g->h = Hasher(n, bucket);
+  IngestHash ih{bucket ? 1u : 0u, (uint64_t)g->h.nparts, (uint64_t)g->h.height, (uint64_t)g->h.max_range};
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  // hipcub temporary storage, grown as needed
+  auto temp = [&](size_t need) -> void* {
+    if (need <= tmp_bytes) return tmp;
+    tmp = T.get<uint8_t>(need);
+    tmp_bytes = tmp ? need : 0;
+    return tmp;
+  };
+  auto exclusive_sum = [&](const uint32_t* in, uint32_t* out, uint64_t cnt) -> bool {
+    if (cnt == 0) return true;
+    size_t need = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, (int)cnt, st) != hipSuccess) return false;
+    void* t = temp(need);
+    return t && hipcub::DeviceScan::ExclusiveSum(t, need, in, out, (int)cnt, st) == hipSuccess;
+  };
+  auto sort_pairs = [&](hipcub::DoubleBuffer<uint64_t>& k, hipcub::DoubleBuffer<uint32_t>& v, uint64_t cnt,
+                        int bits) -> bool {
+    if (cnt == 0) return true;
+    size_t need = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, need, k, v, (int)cnt, 0, bits, st) != hipSuccess) return false;
+    void* t = temp(need);
+    return t && hipcub::DeviceRadixSort::SortPairs(t, need, k, v, (int)cnt, 0, bits, st) == hipSuccess;
+  };
+  auto total = [&](const uint32_t* in, const uint32_t* ex, uint64_t cnt, uint64_t& out) -> la3_status {
+    out = 0;
+    if (cnt == 0) return LA3_OK;
+    uint32_t a = 0, b = 0;
+    HIPC(c, hipMemcpyAsync(&a, in + cnt - 1, 4, hipMemcpyDeviceToHost, st));
+    HIPC(c, hipMemcpyAsync(&b, ex + cnt - 1, 4, hipMemcpyDeviceToHost, st));
+    HIPC(c, hipStreamSynchronize(st));
+    out = (uint64_t)a + b;
+    return LA3_OK;
+  };
+  const char* oom = "device ingest: allocation failed";
+  // 1. records: keep flags, positions, orient + hash
+  uint32_t* drec = T.get<uint32_t>(nrec * RW);
+  uint32_t* keep = T.get<uint32_t>(nrec);
+  uint32_t* pos = T.get<uint32_t>(nrec);
+  uint32_t* bad = T.get<uint32_t>(1);
+  if (!drec || !keep || !pos || !bad) return fail(c, LA3_ENOMEM, oom);
+  if (nrec) HIPC(c, hipMemcpyAsync(drec, rec, 4 * nrec * RW, hipMemcpyHostToDevice, st));
+  HIPC(c, hipMemsetAsync(bad, 0, 4, st));
+  launch(c, "ingest", k_ing_keep, blocks_for(nrec), (const uint32_t*)drec, nrec, RW, n, directed ? 1u : 2u, keep, bad);
+  if (!exclusive_sum(keep, pos, nrec)) return fail(c, LA3_EDEVICE, "device ingest: scan failed");
+  uint64_t m = 0;
+  la3_status e;
+  if ((e = total(keep, pos, nrec, m))) return e;
+  uint32_t hbad = 0;
+  HIPC(c, hipMemcpy(&hbad, bad, 4, hipMemcpyDeviceToHost));
+  if (hbad) return fail(c, LA3_EINVAL, "vertex id out of range [0, nvertices]");
+  if (m >= 0xffffffffull) return fail(c, LA3_ENOTSUP, "more than 2^32-1 stored entries");
+  uint32_t* R = T.get<uint32_t>(m);
+  uint32_t* C = T.get<uint32_t>(m);
+  uint32_t* W = weighted ? T.get<uint32_t>(m) : nullptr;
+  if (!R || !C || (weighted && !W)) return fail(c, LA3_ENOMEM, oom);
+  launch(c, "ingest", k_ing_orient, blocks_for(nrec), (const uint32_t*)drec, nrec, RW, (const uint32_t*)pos,
+         (int)directed, (int)reverse, (int)acyclic, ih, R, C, W);
+  // 2. classes and slots
+  uint8_t* rowf = T.get<uint8_t>(S);
+  uint8_t* colf = T.get<uint8_t>(S);
+  uint32_t* cls = T.get<uint32_t>(4ull * S);
+  uint32_t* rank = T.get<uint32_t>(4ull * S);
+  uint32_t* slot = T.get<uint32_t>(S);
+  uint32_t* hid_of = T.get<uint32_t>(S);
+  uint32_t* id_of = T.get<uint32_t>(S);
+  uint32_t* slot_of_id = T.get<uint32_t>(S);
+  if (!rowf || !colf || !cls || !rank || !slot || !hid_of || !id_of || !slot_of_id) return fail(c, LA3_ENOMEM, oom);
+  HIPC(c, hipMemsetAsync(rowf, 0, S, st));
+  HIPC(c, hipMemsetAsync(colf, 0, S, st));
+  launch(c, "ingest", k_ing_occupy, blocks_for(m), (const uint32_t*)R, (const uint32_t*)C, m, rowf, colf);
+  launch(c, "ingest", k_ing_class, blocks_for(S), (const uint8_t*)rowf, (const uint8_t*)colf, S, cls);
+  uint64_t cnt[4];
+  for (int q = 0; q < 4; q++) {
+    if (!exclusive_sum(cls + (uint64_t)q * S, rank + (uint64_t)q * S, S))
+      return fail(c, LA3_EDEVICE, "device ingest: scan failed");
+    if ((e = total(cls + (uint64_t)q * S, rank + (uint64_t)q * S, S, cnt[q]))) return e;
+  }
+  g->nreg = (uint32_t)cnt[0]; g->nsink = (uint32_t)cnt[1]; g->nsrc = (uint32_t)cnt[2]; g->niso = (uint32_t)cnt[3];
+  g->ncols = g->nreg + g->nsrc;
+  g->nrows = g->nreg + g->nsink;
+  const uint32_t nreg = g->nreg, nsink = g->nsink, nsrc = g->nsrc, NC = g->ncols, NR = g->nrows;
+  launch(c, "ingest", k_ing_slots, blocks_for(S), (const uint8_t*)rowf, (const uint8_t*)colf, S, (const uint32_t*)rank,
+         nreg, nreg + nsink, nreg + nsink + nsrc, ih, slot, hid_of, id_of, slot_of_id);
+  // 3. entries by (column, row slot), first record of equal keys kept
+  uint32_t b = 1;
+  while ((1ull << b) < S) b++;
+  uint64_t* k0 = T.get<uint64_t>(m);
+  uint64_t* k1 = T.get<uint64_t>(m);
+  uint32_t* v0 = T.get<uint32_t>(m);
+  uint32_t* v1 = T.get<uint32_t>(m);
+  uint32_t* first = T.get<uint32_t>(m);
+  uint32_t* upos = T.get<uint32_t>(m);
+  if (!k0 || !k1 || !v0 || !v1 || !first || !upos) return fail(c, LA3_ENOMEM, oom);
+  launch(c, "ingest", k_ing_colkeys, blocks_for(m), (const uint32_t*)R, (const uint32_t*)C, m, (const uint32_t*)slot,
+         nreg, nsink, b, k0, v0);
+  hipcub::DoubleBuffer<uint64_t> kb(k0, k1);
+  hipcub::DoubleBuffer<uint32_t> vb(v0, v1);
+  if (!sort_pairs(kb, vb, m, 2 * b)) return fail(c, LA3_EDEVICE, "device ingest: sort failed");
+  launch(c, "ingest", k_ing_firsts, blocks_for(m), (const uint64_t*)kb.Current(), m, first);
+  if (!exclusive_sum(first, upos, m)) return fail(c, LA3_EDEVICE, "device ingest: scan failed");
+  uint64_t nnz = 0;
+  if ((e = total(first, upos, m, nnz))) return e;
+  uint32_t* col_rows = T.get<uint32_t>(nnz);
+  uint32_t* col_w = weighted ? T.get<uint32_t>(nnz) : nullptr;
+  uint32_t* col_of = T.get<uint32_t>(nnz);
+  uint32_t* col_ptr = T.get<uint32_t>((uint64_t)NC + 1);
+  uint32_t* col_split = T.get<uint32_t>(NC);
+  if (!col_rows || (weighted && !col_w) || !col_of || !col_ptr || !col_split) return fail(c, LA3_ENOMEM, oom);
+  launch(c, "ingest", k_ing_compact, blocks_for(m), (const uint64_t*)kb.Current(), (const uint32_t*)vb.Current(),
+         (const uint32_t*)first, (const uint32_t*)upos, m, b, (const uint32_t*)W, col_rows, col_w, col_of);
+  HIPC(c, hipMemsetAsync(col_ptr, 0, 4ull * (NC + 1), st));
+  launch(c, "ingest", k_ing_ptr, blocks_for(nnz), (const uint32_t*)col_of, nnz, NC, col_ptr);
+  launch(c, "ingest", k_ing_split, blocks_for(NC), (const uint32_t*)col_of, (const uint32_t*)col_rows,
+         (const uint32_t*)col_ptr, nnz, NC, nreg, col_split);
+  launch(c, "ingest", k_ing_split2, blocks_for(nnz), (const uint32_t*)col_of, (const uint32_t*)col_rows,
+         (const uint32_t*)col_ptr, nnz, nreg, col_split);
+  // 4. row-major companion by (row slot, message-order key)
+  uint32_t* row_cols = T.get<uint32_t>(nnz);
+  uint32_t* row_w = weighted ? T.get<uint32_t>(nnz) : nullptr;
+  uint32_t* row_of = T.get<uint32_t>(nnz);
+  uint32_t* mkey = T.get<uint32_t>(nnz);
+  uint32_t* row_ptr = T.get<uint32_t>((uint64_t)NR + 1);
+  uint32_t* row_split = T.get<uint32_t>(NR);
+  if (!row_cols || (weighted && !row_w) || !row_of || !mkey || !row_ptr || !row_split) return fail(c, LA3_ENOMEM, oom);
+  launch(c, "ingest", k_ing_rowkeys, blocks_for(nnz), (const uint32_t*)col_rows, (const uint32_t*)col_of, nnz, nreg,
+         nsrc, b, k0, v0);
+  hipcub::DoubleBuffer<uint64_t> kr(k0, k1);
+  hipcub::DoubleBuffer<uint32_t> vr(v0, v1);
+  if (!sort_pairs(kr, vr, nnz, 2 * b)) return fail(c, LA3_EDEVICE, "device ingest: sort failed");
+  launch(c, "ingest", k_ing_rowfill, blocks_for(nnz), (const uint64_t*)kr.Current(), (const uint32_t*)vr.Current(), nnz,
+         b, (const uint32_t*)col_of, (const uint32_t*)col_w, nreg, nsink, row_cols, row_w, row_of, mkey);
+  HIPC(c, hipMemsetAsync(row_ptr, 0, 4ull * (NR + 1), st));
+  launch(c, "ingest", k_ing_ptr, blocks_for(nnz), (const uint32_t*)row_of, nnz, NR, row_ptr);
+  launch(c, "ingest", k_ing_split, blocks_for(NR), (const uint32_t*)row_of, (const uint32_t*)mkey,
+         (const uint32_t*)row_ptr, nnz, NR, nsrc, row_split);
+  launch(c, "ingest", k_ing_split2, blocks_for(nnz), (const uint32_t*)row_of, (const uint32_t*)mkey,
+         (const uint32_t*)row_ptr, nnz, nsrc, row_split);
+  // 5. long rows and PPR segments
+  uint32_t* lflag = T.get<uint32_t>(NR);
+  uint32_t* lpos = T.get<uint32_t>(NR);
+  if (!lflag || !lpos) return fail(c, LA3_ENOMEM, oom);
+  auto compact_rows = [&](uint32_t lim, uint32_t*& list, uint64_t& count) -> la3_status {
+    launch(c, "ingest", k_ing_longflag, blocks_for(NR), (const uint32_t*)row_ptr, NR, lim, lflag);
+    if (!exclusive_sum(lflag, lpos, NR)) return fail(c, LA3_EDEVICE, "device ingest: scan failed");
+    la3_status x;
+    if ((x = total(lflag, lpos, NR, count))) return x;
+    list = T.get<uint32_t>(count);
+    if (!list) return fail(c, LA3_ENOMEM, oom);
+    launch(c, "ingest", k_ing_compact_ids, blocks_for(NR), (const uint32_t*)lflag, (const uint32_t*)lpos, NR, list);
+    return LA3_OK;
+  };
+  uint32_t* long_rows = nullptr;
+  uint32_t* plong = nullptr;
+  uint64_t nlong = 0, np = 0;
+  if ((e = compact_rows(LONG_ROW, long_rows, nlong))) return e;
+  if ((e = compact_rows(PSEG_ENTRIES, plong, np))) return e;
+  uint32_t* nseg = T.get<uint32_t>(np);
+  uint32_t* pseg_first = T.get<uint32_t>(np + 1);
+  if (!nseg || !pseg_first) return fail(c, LA3_ENOMEM, oom);
+  launch(c, "ingest", k_ing_segcount, blocks_for(np), (const uint32_t*)plong, (uint32_t)np, (const uint32_t*)row_ptr,
+         PSEG_ENTRIES, nseg);
+  if (!exclusive_sum(nseg, pseg_first, np)) return fail(c, LA3_EDEVICE, "device ingest: scan failed");
+  uint64_t npseg = 0;
+  if ((e = total(nseg, pseg_first, np, npseg))) return e;
+  const uint32_t npseg32 = (uint32_t)npseg;
+  HIPC(c, hipMemcpyAsync(pseg_first + np, &npseg32, 4, hipMemcpyHostToDevice, st));
+  uint32_t* pseg_row = T.get<uint32_t>(npseg);
+  uint32_t* pseg_beg = T.get<uint32_t>(npseg);
+  unsigned long long* sums = T.get<unsigned long long>(2);
+  if (!pseg_row || !pseg_beg || !sums) return fail(c, LA3_ENOMEM, oom);
+  launch(c, "ingest", k_ing_segfill, blocks_for(np), (const uint32_t*)plong, (uint32_t)np, (const uint32_t*)row_ptr,
+         (const uint32_t*)pseg_first, PSEG_ENTRIES, pseg_row, pseg_beg);
+  HIPC(c, hipMemsetAsync(sums, 0, 16, st));
+  launch(c, "ingest", k_ing_lensum, blocks_for(nlong), (const uint32_t*)long_rows, (uint32_t)nlong,
+         (const uint32_t*)row_ptr, nreg, sums);
+  launch(c, "ingest", k_ing_lensum, blocks_for(np), (const uint32_t*)plong, (uint32_t)np, (const uint32_t*)row_ptr,
+         nreg, sums + 1);
+  // Host copies: counts, id maps, the regular part of the PPR lists.
+  unsigned long long hs[2];
+  std::vector<uint32_t> hplong(np), hpf(np + 1), hcp((size_t)NC + 1), hcs(NC);
+  g->hid_of.resize(S); g->id_of.resize(S); g->slot_of_id.resize(S);
+  HIPC(c, hipMemcpyAsync(hs, sums, 16, hipMemcpyDeviceToHost, st));
+  if (np) HIPC(c, hipMemcpyAsync(hplong.data(), plong, 4 * np, hipMemcpyDeviceToHost, st));
+  HIPC(c, hipMemcpyAsync(hpf.data(), pseg_first, 4 * (np + 1), hipMemcpyDeviceToHost, st));
+  HIPC(c, hipMemcpyAsync(hcp.data(), col_ptr, 4ull * (NC + 1), hipMemcpyDeviceToHost, st));
+  if (NC) HIPC(c, hipMemcpyAsync(hcs.data(), col_split, 4ull * NC, hipMemcpyDeviceToHost, st));
+  HIPC(c, hipMemcpyAsync(g->hid_of.data(), hid_of, 4ull * S, hipMemcpyDeviceToHost, st));
+  HIPC(c, hipMemcpyAsync(g->id_of.data(), id_of, 4ull * S, hipMemcpyDeviceToHost, st));
+  HIPC(c, hipMemcpyAsync(g->slot_of_id.data(), slot_of_id, 4ull * S, hipMemcpyDeviceToHost, st));
+  HIPC(c, hipStreamSynchronize(st));
+  uint64_t nnz_reg = 0;
+  for (uint32_t col = 0; col < NC; col++) nnz_reg += hcs[col] - hcp[col];
+  uint32_t nplong_reg = 0;
+  while (nplong_reg < np && hplong[nplong_reg] < nreg) nplong_reg++;
+  g->nnz = nnz; g->nnz_reg = nnz_reg; g->nnz_sink = nnz - nnz_reg;
+  g->nnz_long_reg = hs[0]; g->pnnz_reg = hs[1];
+  DG& d = g->dg;
+  d.nslots = S; d.nreg = nreg; d.nsink = nsink; d.nsrc = nsrc; d.ncols = NC; d.nrows = NR; d.nnz = nnz;
+  d.col_ptr = col_ptr; d.col_split = col_split; d.col_rows = col_rows; d.col_w = col_w;
+  d.row_ptr = row_ptr; d.row_split = row_split; d.row_cols = row_cols; d.row_w = row_w;
+  d.id_of = id_of; d.hid_of = hid_of; d.long_rows = long_rows; d.nlong = (uint32_t)nlong;
+  d.slot_of_id = slot_of_id; d.plong = plong; d.pseg_first = pseg_first; d.pseg_row = pseg_row;
+  d.pseg_beg = pseg_beg; d.nplong = (uint32_t)np; d.nplong_reg = nplong_reg; d.npseg = npseg32;
+  d.npseg_reg = hpf[nplong_reg];
+  const std::pair<void*, uint64_t> kept[] = {
+      {col_ptr, NC + 1ull}, {col_split, NC}, {col_rows, nnz}, {col_w, weighted ? nnz : 0}, {row_ptr, NR + 1ull},
+      {row_split, NR}, {row_cols, nnz}, {row_w, weighted ? nnz : 0}, {id_of, S}, {hid_of, S}, {long_rows, nlong},
+      {slot_of_id, S}, {plong, np}, {pseg_first, np + 1}, {pseg_row, npseg}, {pseg_beg, npseg}};
+  for (const auto& kv : kept) {
+    if (!kv.first) continue;
+    T.keep(kv.first);
+    g->allocs.push_back(kv.first);
+    g->device_bytes += std::max<uint64_t>(4 * kv.second, 4);
+  }
+  return LA3_OK;
+}
+
 la3_status load_records(la3_ctx* ctx, const uint32_t* rec, uint64_t nrec, uint32_t n, int weighted,
                         la3_mode mode, la3_hashing hashing, la3_graph** out) {
   if (!ctx || !out || (nrec && !rec)) return fail(ctx, LA3_EINVAL, "null argument");
   if ((int)mode < 0 || (int)mode > 4) return fail(ctx, LA3_EINVAL, "bad load mode");
   if (n == 0xffffffffu) return fail(ctx, LA3_ENOTSUP, "nvertices must be < 2^32-1");
+  // Device ingest sorts and scans with int counts: records and stored
+  // entries (twice the records when undirected) below 2^31; larger graphs
+  // take the host builder.
+  const uint64_t stored = mode == LA3_UNDIRECTED ? 2 * nrec : nrec;
+  if (ctx->device >= 0 && ctx->ingest_device && stored < (1ull << 31)) {
+    HIPC(ctx, hipSetDevice(ctx->device));
+    auto g = std::make_unique<la3_graph>();
+    g->ctx = ctx;
+    la3_status st = ingest_device(ctx, rec, nrec, n, weighted != 0, (int)mode, hashing != LA3_HASH_NONE, g.get());
+    if (st) {
+      for (void* p : g->allocs) (void)hipFree(p);
+      return st;
+    }
+    ctx->refs++;
+    *out = g.release();
+    return LA3_OK;
+  }
   HostLayout L;
   std::string e = build_layout(rec, nrec, n, weighted != 0, (int)mode, hashing != LA3_HASH_NONE, L);
   if (!e.empty()) return fail(ctx, e.find("2^32") != std::string::npos ? LA3_ENOTSUP : LA3_EINVAL, e);
@@ -392,6 +662,51 @@ void la3_ctx_destroy(la3_ctx* c) {
 }
 
 const char* la3_last_error(const la3_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+la3_status la3_ctx_set_ingest(la3_ctx* c, int on_device) {
+  if (!c) return LA3_EINVAL;
+  c->ingest_device = on_device != 0;
+  return LA3_OK;
+}
+
+la3_status la3_graph_array(const la3_graph* g, la3_array which, uint32_t* out, uint64_t cap, uint64_t* count) {
+  if (!g || !count) return LA3_EINVAL;
+  la3_ctx* c = g->ctx;
+  const uint32_t* dev = nullptr;
+  const std::vector<uint32_t>* hv = nullptr;
+  uint64_t n = 0;
+  const HostLayout* L = g->host.get();
+  const DG& d = g->dg;
+  switch (which) {
+    case LA3_ARR_COL_PTR: n = g->ncols + 1ull; dev = d.col_ptr; if (L) hv = &L->col_ptr; break;
+    case LA3_ARR_COL_SPLIT: n = g->ncols; dev = d.col_split; if (L) hv = &L->col_split; break;
+    case LA3_ARR_COL_ROWS: n = g->nnz; dev = d.col_rows; if (L) hv = &L->col_rows; break;
+    case LA3_ARR_COL_W: n = g->weighted ? g->nnz : 0; dev = d.col_w; if (L) hv = &L->col_w; break;
+    case LA3_ARR_ROW_PTR: n = g->nrows + 1ull; dev = d.row_ptr; if (L) hv = &L->row_ptr; break;
+    case LA3_ARR_ROW_SPLIT: n = g->nrows; dev = d.row_split; if (L) hv = &L->row_split; break;
+    case LA3_ARR_ROW_COLS: n = g->nnz; dev = d.row_cols; if (L) hv = &L->row_cols; break;
+    case LA3_ARR_ROW_W: n = g->weighted ? g->nnz : 0; dev = d.row_w; if (L) hv = &L->row_w; break;
+    case LA3_ARR_ID_OF: n = g->nslots; hv = &g->id_of; break;
+    case LA3_ARR_SLOT_OF_ID: n = g->nslots; hv = &g->slot_of_id; break;
+    case LA3_ARR_LONG_ROWS: n = L ? L->long_rows.size() : d.nlong; dev = d.long_rows; if (L) hv = &L->long_rows; break;
+    case LA3_ARR_PLONG: n = L ? L->plong.size() : d.nplong; dev = d.plong; if (L) hv = &L->plong; break;
+    case LA3_ARR_PSEG_FIRST: n = L ? L->pseg_first.size() : d.nplong + 1ull; dev = d.pseg_first; if (L) hv = &L->pseg_first; break;
+    case LA3_ARR_PSEG_ROW: n = L ? L->pseg_row.size() : d.npseg; dev = d.pseg_row; if (L) hv = &L->pseg_row; break;
+    case LA3_ARR_PSEG_BEG: n = L ? L->pseg_beg.size() : d.npseg; dev = d.pseg_beg; if (L) hv = &L->pseg_beg; break;
+    default: return fail(c, LA3_EINVAL, "unknown layout array");
+  }
+  *count = n;
+  if (!out) return LA3_OK;
+  if (cap < n) return fail(c, LA3_EINVAL, "output too small");
+  if (n == 0) return LA3_OK;
+  if (hv) {
+    std::memcpy(out, hv->data(), 4 * n);
+    return LA3_OK;
+  }
+  HIPC(c, hipSetDevice(c->device));
+  HIPC(c, hipMemcpy(out, dev, 4 * n, hipMemcpyDeviceToHost));
+  return LA3_OK;
+}
 
 la3_status la3_ctx_release_scratch(la3_ctx* c) {
   if (!c) return LA3_EINVAL;
