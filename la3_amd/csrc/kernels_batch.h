@@ -336,6 +336,20 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_pull_long(DG g, uint64_t* vis, 
 // current step; only those queries are relaxed from it. As in the single
 // query, the loop relaxes regular rows and one pull pass computes the sinks.
 
+// Appends `slot` of every lane whose `flag` is set (in lane order).
+__device__ __forceinline__ void em_push_lanes(Emitter& em, const Frontier& f, uint32_t slot, bool flag) {
+  const uint64_t m = __ballot(flag);
+  if (!m) return;  // wave-uniform
+  const uint32_t tot = __popcll(m);
+  if (em.cnt + tot > QCAP) em_flush(em, f);
+  if (flag) {
+    em.q[em.cnt + __popcll(m & ((1ull << lane_id()) - 1))] = slot;
+    em.verts++;
+    em.cols++;
+  }
+  em.cnt += tot;
+}
+
 // Appends `slot` from lane 0 when `flag` (wave-uniform) is set.
 __device__ __forceinline__ void em_push_one(Emitter& em, const Frontier& f, uint32_t slot, bool flag) {
   if (!flag) return;
@@ -354,85 +368,143 @@ __global__ void k_mssp_roots(DG g, const uint32_t* root_slots, uint32_t k, uint3
   }
 }
 
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ uint64_t rdlane64(uint64_t v, uint32_t l) {
+  return ((uint64_t)rdlane((uint32_t)(v >> 32), l) << 32) | rdlane((uint32_t)v, l);
+}
+
 // Near/far split as in k_sssp_push, per (vertex, query) pair with one
 // threshold T for the batch: far pairs keep their bit for the next step and
 // their vertex is re-added (chunk 0) without scanning the column. Improved
 // rows are queued without deduplication, so no relaxation waits on an atomic.
+//
+// Wavefront w takes items w, w + nw, w + 2 nw, ... of the work list (the
+// chunks of a hub's column, adjacent in the list, go to different
+// wavefronts). The items' slots and entry ranges are read 64 at a time, one
+// item per lane, and handed to the whole wavefront with readlane; a column's
+// entries are read 64 at a time the same way. Per item, the vertex's
+// distance row, its query mask and its first 64 entries are loaded
+// together, instead of behind a chain of dependent loads (item, column
+// pointers, mask, distances, entries), and the entry indices live in one
+// register, not U.
 template <int QW>
 __device__ __forceinline__ void mssp_push_body(const DG& g, Part p, Emitter& em, const uint64_t* items,
                                                uint32_t nitems, uint32_t* dist, const uint64_t* cm,
                                                uint64_t* nm, uint32_t T, const Frontier& touched) {
   constexpr uint32_t K = 64 * QW;
+  // Edges in groups of U: the U distance rows are loaded before any is
+  // used, so a wavefront keeps U row loads in flight.
+  constexpr uint32_t U = QW == 1 ? 8 : (QW <= 2 ? 4 : 2);
   const uint32_t lane = lane_id();
-  for (uint32_t it = p.wid(); it < nitems; it += p.nwaves()) {
-    const uint64_t item = items[it];
-    const uint32_t s = (uint32_t)item, ch = (uint32_t)(item >> 32);
-    const uint32_t c = col_of_slot(g, s);
-    const uint32_t beg = g.col_ptr[c] + ch * CHUNK;
-    const uint32_t end = min(beg + CHUNK, g.col_split[c]);
-    uint32_t ds[QW];
-    bool act[QW];
-    bool any_near = false, any_far = false;
-#pragma unroll
-    for (int j = 0; j < QW; j++) {
-      const bool a0 = (cm[(size_t)s * QW + j] >> lane) & 1;
-      ds[j] = a0 ? ld_relaxed(&dist[(size_t)s * K + 64 * j + lane]) : SSSP_INF;
-      const bool far = a0 && ds[j] >= T;
-      act[j] = a0 && !far;
-      any_near |= act[j];
-      if (far && ch == 0) {
-        any_far = true;
-        em.nmin_inv = max(em.nmin_inv, ~ds[j]);
-      }
-      const uint64_t fb = __ballot(far);
-      if (fb && ch == 0 && lane == 0) atomicOr((unsigned long long*)&nm[(size_t)s * QW + j], (unsigned long long)fb);
+  const uint32_t nw = p.nwaves(), wid = p.wid();
+  const uint32_t mine = wid < nitems ? (nitems - wid + nw - 1) / nw : 0;
+  for (uint32_t b0 = 0; b0 < mine; b0 += WAVE) {  // wave-uniform
+    const uint32_t nb = min((uint32_t)WAVE, mine - b0);
+    uint32_t ms = 0, mbeg = 0, mend = 0, mfirst = 0;
+    if (lane < nb) {
+      const uint64_t item = items[wid + (uint64_t)(b0 + lane) * nw];
+      ms = (uint32_t)item;
+      const uint32_t ch = (uint32_t)(item >> 32);
+      const uint32_t c = col_of_slot(g, ms);
+      mbeg = g.col_ptr[c] + ch * CHUNK;
+      mend = max(mbeg, min(mbeg + CHUNK, g.col_split[c]));
+      mfirst = ch == 0;  // first chunk of the column
     }
-    // far pairs of this vertex: queue it again (k_mssp_items drops repeats)
-    em_push_one(em, touched, s, __ballot(any_far) != 0);
-    if (__ballot(any_near) == 0) continue;  // nothing near: no column scan
-    // Edges in groups of U: the U distance rows are loaded before any is
-    // used, so a wavefront keeps U row loads in flight.
-    constexpr uint32_t U = QW == 1 ? 8 : (QW <= 2 ? 4 : 2);
-    for (uint32_t e0 = beg; e0 < end; e0 += U) {
-      const uint32_t n = min(U, end - e0);
-      uint32_t r[U], w[U], cur[U][QW];
+    for (uint32_t x = 0; x < nb; x++) {
+      const uint32_t s = rdlane(ms, x), beg = rdlane(mbeg, x), end = rdlane(mend, x);
+      const bool first = rdlane(mfirst, x) != 0;
+      // distance row, query mask and first entries: all in flight together
+      uint32_t ds[QW];
+      uint64_t qm[QW];
 #pragma unroll
-      for (uint32_t u = 0; u < U; u++) {
-        r[u] = u < n ? g.col_rows[e0 + u] : 0;
-        w[u] = u < n ? g.col_w[e0 + u] : 0;
+      for (int j = 0; j < QW; j++) {
+        ds[j] = ld_relaxed(&dist[(size_t)s * K + 64 * j + lane]);
+        qm[j] = cm[(size_t)s * QW + j];
       }
+      uint32_t ne = min((uint32_t)WAVE, end - beg);
+      uint32_t rv = lane < ne ? g.col_rows[beg + lane] : 0;
+      uint32_t wv = lane < ne ? g.col_w[beg + lane] : 0;
+      bool act[QW];
+      bool any_near = false, any_far = false;
 #pragma unroll
-      for (uint32_t u = 0; u < U; u++)
+      for (int j = 0; j < QW; j++) {
+        const bool a0 = (qm[j] >> lane) & 1;
+        if (!a0) ds[j] = SSSP_INF;
+        const bool far = a0 && ds[j] >= T;
+        act[j] = a0 && !far;
+        any_near |= act[j];
+        if (far && first) {
+          any_far = true;
+          em.nmin_inv = max(em.nmin_inv, ~ds[j]);
+        }
+        const uint64_t fb = __ballot(far);
+        if (fb && first && lane == 0) atomicOr((unsigned long long*)&nm[(size_t)s * QW + j], (unsigned long long)fb);
+      }
+      // far pairs of this vertex: queue it again (k_mssp_items drops repeats)
+      em_push_one(em, touched, s, __ballot(any_far) != 0);
+      if (__ballot(any_near) == 0) continue;  // nothing near: no column scan
+      bool actw[QW];  // wave-uniform: some lane relaxes in word j
 #pragma unroll
-        for (int j = 0; j < QW; j++)
-          cur[u][j] = (u < n && act[j]) ? ld_relaxed(&dist[(size_t)r[u] * K + 64 * j + lane]) : 0u;
+      for (int j = 0; j < QW; j++) actw[j] = __ballot(act[j]) != 0;
+      for (uint32_t e0 = beg; e0 < end; e0 += WAVE) {  // wave-uniform
+        if (e0 != beg) {
+          ne = min((uint32_t)WAVE, end - e0);
+          rv = lane < ne ? g.col_rows[e0 + lane] : 0;
+          wv = lane < ne ? g.col_w[e0 + lane] : 0;
+        }
+        for (uint32_t u0 = 0; u0 < ne; u0 += U) {
+          const uint32_t n = min(U, ne - u0);
+          // Row bases are wave-uniform (scalar registers); lanes add their
+          // query offset. A word of a row is loaded whole when any lane
+          // relaxes in it (the lines are fetched either way), so the loads
+          // need no per-lane branches.
+          uint32_t r[U], cur[U][QW];
 #pragma unroll
-      for (uint32_t u = 0; u < U; u++) {
-        if (u >= n) break;  // wave-uniform
-        bool any = false;
+          for (uint32_t u = 0; u < U; u++) {
+            r[u] = rdlane(rv, min(u0 + u, (uint32_t)WAVE - 1));
+            const uint32_t* row = dist + (size_t)r[u] * K;
 #pragma unroll
-        for (int j = 0; j < QW; j++) {
-          bool imp = false;
-          if (act[j]) {
-            // The pre-loaded value decides "improved"; the atomic's result is
-            // not waited for. A stale value can only over-report (the pair
-            // is re-activated and relaxes with its current, smaller
-            // distance); a real improvement is never missed.
-            const uint32_t nd = ds[j] + w[u];
-            if (nd < cur[u][j]) {
-              atomicMin(&dist[(size_t)r[u] * K + 64 * j + lane], nd);
-              imp = true;
-              em.nmin_inv = max(em.nmin_inv, ~nd);
+            for (int j = 0; j < QW; j++) cur[u][j] = (u < n && actw[j]) ? ld_relaxed(row + 64 * j + lane) : 0u;
+          }
+          // Lane u collects entry u's row and improved-query masks, so the
+          // mask updates and the queue appends of the whole group are one
+          // lane-parallel atomic and one compaction.
+          uint32_t vrow = 0;
+          uint64_t vbits[QW];
+#pragma unroll
+          for (int j = 0; j < QW; j++) vbits[j] = 0;
+#pragma unroll
+          for (uint32_t u = 0; u < U; u++) {
+            if (u >= n) continue;  // wave-uniform; no break, so the loop unrolls
+            const uint32_t w = rdlane(wv, u0 + u);
+            vrow = lane == u ? r[u] : vrow;
+#pragma unroll
+            for (int j = 0; j < QW; j++) {
+              // The pre-loaded value decides "improved"; the atomic's result
+              // is not waited for. A stale value can only over-report (the
+              // pair is re-activated and relaxes with its current, smaller
+              // distance); a real improvement is never missed.
+              const uint32_t nd = ds[j] + w;
+              const bool imp = act[j] && nd < cur[u][j];
+              if (imp) {
+                atomicMin(dist + (size_t)r[u] * K + 64 * j + lane, nd);
+                em.nmin_inv = max(em.nmin_inv, ~nd);
+              }
+              const uint64_t b = __ballot(imp);
+              vbits[j] = lane == u ? b : vbits[j];
             }
           }
-          const uint64_t b = __ballot(imp);
-          if (b) {
-            if (lane == 0) atomicOr((unsigned long long*)&nm[(size_t)r[u] * QW + j], (unsigned long long)b);
-            any = true;
+          bool any = false;
+#pragma unroll
+          for (int j = 0; j < QW; j++) {
+            if (vbits[j]) atomicOr((unsigned long long*)&nm[(size_t)vrow * QW + j], (unsigned long long)vbits[j]);
+            any |= vbits[j] != 0;
           }
+          // queued on every improvement; k_mssp_items keeps the first per row
+          em_push_lanes(em, touched, vrow, any);
         }
-        // queued on every improvement; k_mssp_items keeps the first per row
-        em_push_one(em, touched, r[u], any);
       }
     }
   }

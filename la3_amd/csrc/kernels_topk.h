@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(BLOCK) k_topk_hist(const void* v, uint32_t dty
   const uint32_t row = blockIdx.x / chunks, ch = blockIdx.x % chunks;
   h[threadIdx.x] = 0;
   __syncthreads();
-  const uint64_t beg = (uint64_t)ch * TK_CHUNK, end = min(beg + TK_CHUNK, n);
+  const uint64_t beg = (uint64_t)ch * TK_CHUNK, end = beg + TK_CHUNK < n ? beg + TK_CHUNK : n;
   const uint64_t p = prefix[row];
   for (uint64_t i = beg + threadIdx.x; i < end; i += BLOCK) {
     const uint64_t k = tk_key(v, dtype, (uint64_t)row * n + i, largest != 0);
@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(BLOCK) k_topk_count_eq(const void* v, uint32_t
   const uint32_t row = blockIdx.x / chunks, ch = blockIdx.x % chunks;
   if (threadIdx.x == 0) s = 0;
   __syncthreads();
-  const uint64_t beg = (uint64_t)ch * TK_CHUNK, end = min(beg + TK_CHUNK, n);
+  const uint64_t beg = (uint64_t)ch * TK_CHUNK, end = beg + TK_CHUNK < n ? beg + TK_CHUNK : n;
   const uint64_t t = thr[row];
   uint32_t c = 0;
   for (uint64_t i = beg + threadIdx.x; i < end; i += BLOCK)
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(BLOCK) k_topk_write(const void* v, uint32_t dt
   __shared__ uint32_t wsum[BLOCK / WAVE];
   __shared__ uint32_t run;  // equal keys of this chunk before the current 256 ids
   const uint32_t row = blockIdx.x / chunks, ch = blockIdx.x % chunks;
-  const uint64_t beg = (uint64_t)ch * TK_CHUNK, end = min(beg + TK_CHUNK, n);
+  const uint64_t beg = (uint64_t)ch * TK_CHUNK, end = beg + TK_CHUNK < n ? beg + TK_CHUNK : n;
   const uint64_t t = thr[row];
   const uint32_t nd = need[row], base = eq_base[blockIdx.x];
   const uint32_t w = threadIdx.x / WAVE, lane = lane_id();
