@@ -191,11 +191,21 @@ __global__ void k_fill(T* p, uint64_t n, T v) {
     p[i] = v;
 }
 
-// Plain copy (16-byte vectors); the known-size kernel for bandwidth checks.
+// Plain copy (16-byte vectors, four loads in flight per lane); the
+// known-size kernel for bandwidth checks.
 __global__ void __launch_bounds__(BLOCK) la3_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                   uint64_t n) {
-  for (uint64_t i = blockIdx.x * (uint64_t)BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK)
-    dst[i] = src[i];
+  constexpr int V = 4;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  uint64_t i = blockIdx.x * (uint64_t)BLOCK + threadIdx.x;
+  for (; i + (V - 1) * stride < n; i += V * stride) {
+    uint4 t[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) t[v] = src[i + v * stride];
+#pragma unroll
+    for (int v = 0; v < V; v++) dst[i + v * stride] = t[v];
+  }
+  for (; i < n; i += stride) dst[i] = src[i];
 }
 
 // out[id] = v[slot(id)] for every original id (coalesced writes).
