@@ -113,6 +113,24 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_push(DG g, const uint64_t* item
   em_finish(em, touched);
 }
 
+// Sets level byte q of a 64-query level row to v for every bit q of m,
+// eight bytes at a time (read-modify-write of the 8-byte groups holding set
+// bits; one lane owns the row in the step).
+__device__ __forceinline__ void set_levels(uint8_t* row, uint64_t m, uint8_t v) {
+  uint64_t* w8 = (uint64_t*)row;  // level rows are 64-byte aligned
+  const uint64_t vv = 0x0101010101010101ull * v;
+#pragma unroll
+  for (int g = 0; g < 8; g++) {
+    uint64_t x = (m >> (8 * g)) & 0xffu;
+    if (!x) continue;
+    x = (x | (x << 28)) & 0x0000000f0000000full;  // bit i -> byte i
+    x = (x | (x << 14)) & 0x0003000300030003ull;
+    x = (x | (x << 7)) & 0x0101010101010101ull;
+    const uint64_t sel = x * 0xffu;
+    w8[g] = (w8[g] & ~sel) | (vv & sel);
+  }
+}
+
 // Marks the new bits of each touched vertex visited, records their level and
 // emits the vertex's column as push work for the next step. New bits are the
 // next-frontier bits not yet visited; a sink's next-frontier words are
@@ -138,11 +156,7 @@ __device__ __forceinline__ void msbfs_apply_body(const DG& g, Part p, Emitter& e
         if (!m) continue;
         act = true;
         vis[(size_t)r * W + w] = v | m;
-        while (m) {
-          const int q = __ffsll((long long)m) - 1;
-          lev[(size_t)r * 64 * W + w * 64 + q] = nxt;
-          m &= m - 1;
-        }
+        set_levels(lev + (size_t)r * 64 * W + w * 64, m, nxt);
       }
       if (act && has_col) {
         const uint32_t c = col_of_slot(g, r);
@@ -207,30 +221,46 @@ __device__ __forceinline__ void msbfs_pull_body(const DG& g, Part p, Emitter& em
         any |= need[w] != 0;
       }
       if (any && e - b <= LONG_ROW) {
-        for (uint32_t j = b; j < e; j++) {
+        // UB in-neighbours at a time (their frontier words load together),
+        // stopping once every needed query is covered: further entries
+        // cannot change acc then, so the result is that of the serial scan.
+        constexpr uint32_t UB = W <= 2 ? 4 : 2;
+        bool done = false;
+        uint32_t j = b;
+        for (; j + UB <= e && !done; j += UB) {
+          uint32_t src[UB];
+#pragma unroll
+          for (uint32_t t = 0; t < UB; t++) src[t] = g.row_cols[j + t];
+          done = true;
+#pragma unroll
+          for (int w = 0; w < W; w++) {
+            uint64_t f = 0;
+#pragma unroll
+            for (uint32_t t = 0; t < UB; t++) f |= fa[(size_t)src[t] * W + w];
+            acc[w] |= f & need[w];
+            done &= acc[w] == need[w];
+          }
+          scanned += UB;
+        }
+        for (; j < e && !done; j++) {
           const uint32_t src = g.row_cols[j];
-          bool done = true;
+          done = true;
           scanned++;
 #pragma unroll
           for (int w = 0; w < W; w++) {
             acc[w] |= fa[(size_t)src * W + w] & need[w];
             done &= acc[w] == need[w];
           }
-          if (done) break;
         }
         const bool has_col = slot_has_col(g, r);
 #pragma unroll
         for (int w = 0; w < W; w++) {
-          uint64_t m = acc[w];
+          const uint64_t m = acc[w];
           if (!m) continue;
           act = true;
           vis[(size_t)r * W + w] |= m;
           if (has_col) fb[(size_t)r * W + w] = m;
-          while (m) {
-            const int q = __ffsll((long long)m) - 1;
-            lev[(size_t)r * 64 * W + w * 64 + q] = nxt;
-            m &= m - 1;
-          }
+          set_levels(lev + (size_t)r * 64 * W + w * 64, m, nxt);
         }
       }
       if (act && slot_has_col(g, r)) {
@@ -304,11 +334,7 @@ __device__ __forceinline__ void msbfs_pull_long_body(const DG& g, Part p, Emitte
         act = true;
         vis[(size_t)r * W + w] |= m;
         if (has_col) fb[(size_t)r * W + w] = m;
-        while (m) {
-          const int q = __ffsll((long long)m) - 1;
-          lev[(size_t)r * 64 * W + w * 64 + q] = nxt;
-          m &= m - 1;
-        }
+        set_levels(lev + (size_t)r * 64 * W + w * 64, m, nxt);
       }
       if (act && has_col) {
         const uint32_t c = col_of_slot(g, r);

@@ -240,7 +240,8 @@ def test_bfs_batch_undirected(gpu_ctx, oracle):
         assert (hops[q] == oracle.bfs(o, roots[q])[0]).all()
 
 
-@pytest.mark.parametrize("scale,seed,k", [(10, 41, 1), (12, 42, 64), (14, 43, 100), (16, 44, 128)])
+@pytest.mark.parametrize("scale,seed,k", [(10, 41, 1), (12, 42, 64), (14, 43, 100), (16, 44, 128),
+                                          (13, 45, 256), (12, 46, 1024)])
 def test_sssp_batch_equals_single(gpu_ctx, oracle, scale, seed, k):
     rec = _rmat(scale, seed, weighted=True)
     n = 1 << scale
