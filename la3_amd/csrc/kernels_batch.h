@@ -603,23 +603,34 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_sinks(DG g, uint32_t* dist) {
 
 // out[q * nslots + id] = v[slot(id) * K + q] for q < k, through a 64 x 64
 // LDS tile: rows are read whole (lane = query, coalesced) and written per
-// query as runs of 64 consecutive ids (lane = id, coalesced).
+// query as runs of 64 consecutive ids (lane = id, coalesced). The tile's 64
+// slots are read once, one per lane, and each wavefront keeps its 16 row
+// loads (and then its 16 stores) in flight together.
 template <class T>
 __device__ __forceinline__ void k_batch_output_body(const DG& g, const T* v, uint32_t K, uint32_t k, T* out) {
   __shared__ T tile[64][65];
   const uint32_t lane = lane_id(), wv = threadIdx.x / WAVE;
+  constexpr uint32_t PER = 64 / (BLOCK / WAVE);  // rows (then queries) per wavefront
   const uint32_t ntile = (g.nslots + 63) / 64;
   for (uint32_t t = blockIdx.x; t < ntile; t += gridDim.x) {
     const uint32_t id0 = t * 64;
+    const uint32_t nid = min(64u, g.nslots - id0);
+    const uint32_t msl = lane < nid ? g.slot_of_id[id0 + lane] : 0;
     for (uint32_t q0 = 0; q0 < k; q0 += 64) {
-      for (uint32_t i = wv; i < 64; i += BLOCK / WAVE) {
-        const uint32_t id = id0 + i;
-        if (id < g.nslots && q0 + lane < k) tile[i][lane] = v[(size_t)g.slot_of_id[id] * K + q0 + lane];
+      T val[PER];
+#pragma unroll
+      for (uint32_t x = 0; x < PER; x++) {
+        const uint32_t i = wv + (BLOCK / WAVE) * x;
+        const uint32_t sl = rdlane(msl, i);
+        val[x] = (i < nid && q0 + lane < k) ? v[(size_t)sl * K + q0 + lane] : T();
       }
+#pragma unroll
+      for (uint32_t x = 0; x < PER; x++) tile[wv + (BLOCK / WAVE) * x][lane] = val[x];
       __syncthreads();
-      for (uint32_t qi = wv; qi < 64 && q0 + qi < k; qi += BLOCK / WAVE) {
-        const uint32_t id = id0 + lane;
-        if (id < g.nslots) out[(size_t)(q0 + qi) * g.nslots + id] = tile[lane][qi];
+#pragma unroll
+      for (uint32_t x = 0; x < PER; x++) {
+        const uint32_t qi = wv + (BLOCK / WAVE) * x;
+        if (q0 + qi < k && lane < nid) out[(size_t)(q0 + qi) * g.nslots + id0 + lane] = tile[lane][qi];
       }
       __syncthreads();
     }
