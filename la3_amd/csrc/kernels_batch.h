@@ -522,13 +522,20 @@ __device__ __forceinline__ void mssp_push_body(const DG& g, Part p, Emitter& em,
               vbits[j] = lane == u ? b : vbits[j];
             }
           }
+          // With one mask word per row, the row is queued by the lane whose
+          // atomicOr finds the word still zero: the first improvement of the
+          // row in this step (nm starts the step zeroed, far bits included).
+          // Wider rows are queued on every improvement (waiting on QW
+          // returned words costs more than it saves). k_mssp_items keeps
+          // the first occurrence of a row either way.
           bool any = false;
 #pragma unroll
           for (int j = 0; j < QW; j++) {
-            if (vbits[j]) atomicOr((unsigned long long*)&nm[(size_t)vrow * QW + j], (unsigned long long)vbits[j]);
-            any |= vbits[j] != 0;
+            if (!vbits[j]) continue;
+            unsigned long long* w = (unsigned long long*)&nm[(size_t)vrow * QW + j];
+            if (QW == 1) any = atomicOr(w, (unsigned long long)vbits[j]) == 0;
+            else { atomicOr(w, (unsigned long long)vbits[j]); any = true; }
           }
-          // queued on every improvement; k_mssp_items keeps the first per row
           em_push_lanes(em, touched, vrow, any);
         }
       }
