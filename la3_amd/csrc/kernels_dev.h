@@ -447,40 +447,64 @@ __global__ void k_sssp_init(DG g, uint32_t* dist, uint32_t* stamp, uint32_t root
 // the host raises T when nothing below it remains. T = ~0 is plain
 // Bellman-Ford. Every improvement is still relaxed eventually, so the fixed
 // point — the exact distances — is unchanged.
+//
+// Wavefront w takes items w, w + nw, ... of the work list, 64 at a time, one
+// item per lane (its vertex distance and entry range load together); the
+// entries of those items are then packed into passes of 64 lanes (lane l
+// of a pass finds its item by a binary search over the lanes' exclusive
+// offsets), so short columns do not leave most lanes idle.
 __global__ void __launch_bounds__(BLOCK) k_sssp_push(DG g, const uint64_t* items, uint32_t nitems,
                                                      uint32_t* dist, uint32_t* stamp, uint32_t round,
                                                      uint32_t T, Frontier nf, uint32_t* zc) {
   zero_ctr(zc);
-  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t nw = gridDim.x * (BLOCK / WAVE);
   const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  const uint32_t lane = lane_id();
   const uint32_t tag = round + 1;
   LA3_EMITTER(em);
-  for (uint32_t it = wid; it < nitems; it += nwaves) {
-    const uint64_t item = items[it];
-    const uint32_t s = (uint32_t)item, ch = (uint32_t)(item >> 32);
-    const uint32_t c = col_of_slot(g, s);
-    const uint32_t du = ld_relaxed(&dist[s]);
-    if (du >= T) {  // wave-uniform: far, keep it for a later round
-      bool again = false;
-      uint32_t nit = 0, ne = 0;
-      if (ch == 0 && lane_id() == 0 && atomicMax(&stamp[s], tag) < tag) {
-        again = true;
-        ne = g.col_split[c] - g.col_ptr[c];
-        nit = (ne + CHUNK - 1) / CHUNK;
-        em.nmin_inv = max(em.nmin_inv, ~du);
+  const uint32_t mine = wid < nitems ? (nitems - wid + nw - 1) / nw : 0;
+  for (uint32_t b0 = 0; b0 < mine; b0 += WAVE) {  // wave-uniform
+    const uint32_t nb = min((uint32_t)WAVE, mine - b0);
+    uint32_t s = 0, du = SSSP_INF, beg = 0, len = 0, fnit = 0, fne = 0;
+    bool again = false;
+    if (lane < nb) {
+      const uint64_t item = items[wid + (uint64_t)(b0 + lane) * nw];
+      s = (uint32_t)item;
+      const uint32_t ch = (uint32_t)(item >> 32);
+      const uint32_t c = col_of_slot(g, s);
+      du = ld_relaxed(&dist[s]);
+      const uint32_t cb = g.col_ptr[c], cs = g.col_split[c];
+      if (du >= T) {  // far: keep it for a later round
+        if (ch == 0 && atomicMax(&stamp[s], tag) < tag) {
+          again = true;
+          fne = cs - cb;
+          fnit = (fne + CHUNK - 1) / CHUNK;
+          em.nmin_inv = max(em.nmin_inv, ~du);
+        }
+      } else {
+        beg = cb + ch * CHUNK;
+        const uint32_t end = min(beg + CHUNK, cs);
+        len = end > beg ? end - beg : 0;
       }
-      em_push(em, nf, s, nit, ne, again);
-      continue;
     }
-    const uint32_t beg = g.col_ptr[c] + ch * CHUNK;
-    const uint32_t end = min(beg + CHUNK, g.col_split[c]);
-    for (uint32_t e0 = beg; e0 < end; e0 += WAVE) {
-      const uint32_t e = e0 + lane_id();
+    em_push(em, nf, s, fnit, fne, again);
+    const uint32_t incl = wave_incl_scan(len);
+    const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
+    const uint32_t excl = incl - len;
+    for (uint32_t p0 = 0; p0 < total; p0 += WAVE) {  // wave-uniform
+      const uint32_t gi = p0 + lane;
+      // item of entry gi: the last lane x with excl[x] <= gi (excl[0] = 0)
+      uint32_t x = 0;
+#pragma unroll
+      for (uint32_t step = WAVE / 2; step > 0; step >>= 1)
+        if (__shfl(excl, x + step, WAVE) <= gi) x += step;
+      const uint32_t ib = __shfl(beg, x, WAVE), ie = __shfl(excl, x, WAVE), idu = __shfl(du, x, WAVE);
       bool push = false;
       uint32_t r = 0;
-      if (e < end) {
+      if (gi < total) {
+        const uint32_t e = ib + (gi - ie);
         r = g.col_rows[e];
-        const uint32_t nd = du + g.col_w[e];
+        const uint32_t nd = idu + g.col_w[e];
         if (nd < ld_relaxed(&dist[r])) {
           // Fire-and-forget: the comparison with the loaded value marks the
           // improvement (a stale load only re-activates r harmlessly).
