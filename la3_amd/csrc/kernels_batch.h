@@ -717,9 +717,10 @@ __device__ __forceinline__ float ppr_msg(float h, float l, uint32_t d) {
   return q + (r + l) / fd;
 }
 
+// seed: this lane's queries' seed slots (~0 past k); deg: out-degree of r.
 template <int QW>
-__device__ __forceinline__ void ppr_finish(const PprBatchArgs& a, uint32_t r, const float* s,
-                                           const float* c, bool loop, double* res) {
+__device__ __forceinline__ void ppr_finish(const PprBatchArgs& a, uint32_t r, const float* s, const float* c,
+                                           bool loop, double* res, const uint32_t* seed, uint32_t deg) {
   constexpr uint32_t K = 64 * QW;
   const uint32_t lane = lane_id();
 #pragma unroll
@@ -727,14 +728,23 @@ __device__ __forceinline__ void ppr_finish(const PprBatchArgs& a, uint32_t r, co
     const uint32_t q = 64 * w + lane;
     if (q >= a.k) continue;
     const size_t o = (size_t)r * K + q;
-    const bool seed = a.seeds[q] == r;
+    const bool sd = seed[w] == r;
     float nh, nl;
-    ppr_apply(seed ? a.alpha_hi : 0.f, seed ? a.alpha_lo : 0.f, a.damp_hi, a.damp_lo, s[w], c[w], nh, nl);
+    ppr_apply(sd ? a.alpha_hi : 0.f, sd ? a.alpha_lo : 0.f, a.damp_hi, a.damp_lo, s[w], c[w], nh, nl);
     const double d = fabs(((double)nh + nl) - ((double)a.hi[o] + a.lo[o]));
     res[w] += d;
     a.hi[o] = nh;
     a.lo[o] = nl;
-    if (loop) a.xnext[o] = ppr_msg(nh, nl, a.deg[r]);
+    if (loop) a.xnext[o] = ppr_msg(nh, nl, deg);
+  }
+}
+
+template <int QW>
+__device__ __forceinline__ void ppr_seeds(const PprBatchArgs& a, uint32_t* seed) {
+#pragma unroll
+  for (int w = 0; w < QW; w++) {
+    const uint32_t q = 64 * w + lane_id();
+    seed[w] = q < a.k ? a.seeds[q] : 0xffffffffu;
   }
 }
 
@@ -761,23 +771,26 @@ template <int QW>
 __device__ __forceinline__ void ppr_row_sum(const DG& g, const PprBatchArgs& a, uint32_t b, uint32_t e,
                                             float* s, float* c) {
   constexpr uint32_t K = 64 * QW;
+  // messages of UR entries loaded before any is added (adds stay in entry
+  // order): UR k-wide rows in flight per wavefront
+  constexpr uint32_t UR = QW <= 2 ? 8 : 4;
   const uint32_t lane = lane_id();
   bool live[QW];
 #pragma unroll
   for (int w = 0; w < QW; w++) live[w] = 64 * w + lane < a.k;
   uint32_t j = b;
-  for (; j + 4 <= e; j += 4) {
-    const uint32_t s0 = g.row_cols[j], s1 = g.row_cols[j + 1], s2 = g.row_cols[j + 2],
-                   s3 = g.row_cols[j + 3];
+  for (; j + UR <= e; j += UR) {
+    uint32_t sc[UR];
+#pragma unroll
+    for (uint32_t u = 0; u < UR; u++) sc[u] = g.row_cols[j + u];
 #pragma unroll
     for (int w = 0; w < QW; w++) {
       if (!live[w]) continue;
-      const float v0 = a.x[(size_t)s0 * K + 64 * w + lane], v1 = a.x[(size_t)s1 * K + 64 * w + lane];
-      const float v2 = a.x[(size_t)s2 * K + 64 * w + lane], v3 = a.x[(size_t)s3 * K + 64 * w + lane];
-      kahan_add(s[w], c[w], v0);
-      kahan_add(s[w], c[w], v1);
-      kahan_add(s[w], c[w], v2);
-      kahan_add(s[w], c[w], v3);
+      float v[UR];
+#pragma unroll
+      for (uint32_t u = 0; u < UR; u++) v[u] = a.x[(size_t)sc[u] * K + 64 * w + lane];
+#pragma unroll
+      for (uint32_t u = 0; u < UR; u++) kahan_add(s[w], c[w], v[u]);
     }
   }
   for (; j < e; j++) {
@@ -788,21 +801,40 @@ __device__ __forceinline__ void ppr_row_sum(const DG& g, const PprBatchArgs& a, 
   }
 }
 
-// Rows [r0, r1) with at most PSEG entries: one wavefront per row.
+// Rows [r0, r1) with at most PSEG entries: one wavefront per row (rows
+// r0 + w, r0 + w + nw, ... for wavefront w). The rows' entry ranges and
+// out-degrees are read 64 at a time, one row per lane, and handed out with
+// readlane, so a row's first message loads do not wait behind its pointers.
 template <int QW>
 __device__ __forceinline__ void mppr_rows_body(const DG& g, Part p, const PprBatchArgs& a, uint32_t r0,
                                                uint32_t r1, bool loop, double* red) {
   double res[QW];
+  uint32_t seed[QW];
 #pragma unroll
   for (int w = 0; w < QW; w++) res[w] = 0.0;
-  for (uint32_t r = r0 + p.wid(); r < r1; r += p.nwaves()) {
-    const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1];
-    if (e - b > PSEG) continue;  // wave-uniform; segmented path
-    float s[QW], c[QW];
+  ppr_seeds<QW>(a, seed);
+  const uint32_t lane = lane_id();
+  const uint32_t nw = p.nwaves(), wid = p.wid();
+  const uint32_t nrow = r1 > r0 ? r1 - r0 : 0;
+  const uint32_t mine = wid < nrow ? (nrow - wid + nw - 1) / nw : 0;
+  for (uint32_t b0 = 0; b0 < mine; b0 += WAVE) {  // wave-uniform
+    const uint32_t nb = min((uint32_t)WAVE, mine - b0);
+    uint32_t mr = 0, mb = 0, me = 0, md = 1;
+    if (lane < nb) {
+      mr = r0 + wid + (b0 + lane) * nw;
+      mb = g.row_ptr[mr];
+      me = g.row_ptr[mr + 1];
+      md = a.deg[mr];
+    }
+    for (uint32_t x = 0; x < nb; x++) {
+      const uint32_t r = rdlane(mr, x), b = rdlane(mb, x), e = rdlane(me, x);
+      if (e - b > PSEG) continue;  // wave-uniform; segmented path
+      float s[QW], c[QW];
 #pragma unroll
-    for (int w = 0; w < QW; w++) { s[w] = 0.f; c[w] = 0.f; }
-    ppr_row_sum<QW>(g, a, b, e, s, c);
-    ppr_finish<QW>(a, r, s, c, loop, res);
+      for (int w = 0; w < QW; w++) { s[w] = 0.f; c[w] = 0.f; }
+      ppr_row_sum<QW>(g, a, b, e, s, c);
+      ppr_finish<QW>(a, r, s, c, loop, res, seed, rdlane(md, x));
+    }
   }
   ppr_resid_flush<QW>(a, res, p.bid, red);
 }
@@ -847,8 +879,10 @@ __device__ __forceinline__ void mppr_combine_body(const DG& g, Part p, const Ppr
   constexpr uint32_t K = 64 * QW;
   const uint32_t lane = lane_id();
   double res[QW];
+  uint32_t seed[QW];
 #pragma unroll
   for (int w = 0; w < QW; w++) res[w] = 0.0;
+  ppr_seeds<QW>(a, seed);
   for (uint32_t i = p.wid(); i < a.nlong_ppr; i += p.nwaves()) {
     const uint32_t r = a.long_ppr[i];
     if (r < r0 || r >= r1) continue;
@@ -863,7 +897,7 @@ __device__ __forceinline__ void mppr_combine_body(const DG& g, Part p, const Ppr
         kahan_add(s[w], c[w], -a.part[(size_t)sg * 2 * K + K + 64 * w + lane]);
       }
     }
-    ppr_finish<QW>(a, r, s, c, loop, res);
+    ppr_finish<QW>(a, r, s, c, loop, res, seed, a.deg[r]);
   }
   ppr_resid_flush<QW>(a, res, p.bid, red);
 }
