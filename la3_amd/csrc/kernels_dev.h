@@ -587,8 +587,18 @@ __device__ __forceinline__ void pr_range(const DG& g, uint32_t r, uint32_t& lo, 
 }
 
 // y + x[cols[lo]] + ... + x[cols[hi - 1]], added in entry order; the loads
-// of four entries are issued before their adds.
+// of eight (then four) entries are issued before their adds.
 __device__ __forceinline__ double seq_sum(const DG& g, const double* x, uint32_t j, uint32_t hi, double y) {
+  for (; j + 8 <= hi; j += 8) {
+    uint32_t cc[8];
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) cc[u] = g.row_cols[j + u];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = x[cc[u]];
+#pragma unroll
+    for (int u = 0; u < 8; u++) y = __dadd_rn(y, v[u]);
+  }
   for (; j + 4 <= hi; j += 4) {
     const uint32_t c0 = g.row_cols[j], c1 = g.row_cols[j + 1], c2 = g.row_cols[j + 2], c3 = g.row_cols[j + 3];
     const double v0 = x[c0], v1 = x[c1], v2 = x[c2], v3 = x[c3];
@@ -602,11 +612,21 @@ __device__ __forceinline__ double seq_sum(const DG& g, const double* x, uint32_t
 }
 
 // Wavefront sum of x[cols[j]], j in [lo, hi): lane l adds entries lo + l,
-// lo + l + 64, ... in order (four loads in flight), then a fixed-shape
-// wavefront reduction.
+// lo + l + 64, ... in order (eight, then four, loads in flight), then a
+// fixed-shape wavefront reduction.
 __device__ __forceinline__ double wave_range_sum(const DG& g, const double* x, uint32_t lo, uint32_t hi) {
   double p = 0.0;
   uint32_t j = lo + lane_id();
+  for (; j + 7 * WAVE < hi; j += 8 * WAVE) {
+    uint32_t cc[8];
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) cc[u] = g.row_cols[j + u * WAVE];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = x[cc[u]];
+#pragma unroll
+    for (int u = 0; u < 8; u++) p = __dadd_rn(p, v[u]);
+  }
   for (; j + 3 * WAVE < hi; j += 4 * WAVE) {
     const uint32_t c0 = g.row_cols[j], c1 = g.row_cols[j + WAVE], c2 = g.row_cols[j + 2 * WAVE],
                    c3 = g.row_cols[j + 3 * WAVE];
