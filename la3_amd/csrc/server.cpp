@@ -128,6 +128,7 @@ void complete(la3_server* s, const std::vector<Query>& qs, uint32_t batch_size, 
 void worker_fused(la3_server* s, la3_ctx* ctx) {
   struct Running {
     std::vector<Query> qs;
+    GraphEntry ge{};  // copied under the lock: add_graph may grow the table
     uint64_t bytes = 0;
     std::vector<uint8_t> out;
     double start = 0;
@@ -158,6 +159,7 @@ void worker_fused(la3_server* s, la3_ctx* ctx) {
           }
           free -= ((uint32_t)qs.size() + 255) / 256;
           Running r;
+          r.ge = s->graphs[qs.front().graph];
           r.qs = std::move(qs);
           r.bytes = bytes;
           admit.push_back(std::move(r));
@@ -171,7 +173,7 @@ void worker_fused(la3_server* s, la3_ctx* ctx) {
     std::vector<std::pair<Running, la3_status>> failed;
     for (Running& r : admit) {
       const Query& h = r.qs.front();
-      const GraphEntry& ge = s->graphs[h.graph];
+      const GraphEntry& ge = r.ge;
       std::vector<uint32_t> roots;
       for (const Query& q : r.qs) roots.push_back(q.root);
       if (s->cfg.keep_results) r.out.resize((size_t)r.qs.size() * ge.nslots * elem_bytes(h.kind));
@@ -233,6 +235,7 @@ void worker_main(la3_server* s, la3_ctx* ctx) {
   for (;;) {
     std::vector<Query> batch;
     uint64_t bytes = 0;
+    GraphEntry ge{};  // copied under the lock: add_graph may grow the table
     {
       std::unique_lock<std::mutex> lk(s->mu);
       for (;;) {
@@ -246,10 +249,10 @@ void worker_main(la3_server* s, la3_ctx* ctx) {
         if (!batch.empty()) break;
         s->cv.wait(lk);
       }
+      ge = s->graphs[batch.front().graph];
     }
     const Query& h = batch.front();
     const uint32_t k = (uint32_t)batch.size();
-    const GraphEntry& ge = s->graphs[h.graph];
     std::vector<uint32_t> roots(k);
     for (uint32_t i = 0; i < k; i++) roots[i] = batch[i].root;
     const double start = s->now();
@@ -321,7 +324,9 @@ la3_status la3_server_add_graph(la3_server* s, la3_graph* g, uint32_t* graph_id)
 }
 
 uint64_t la3_server_query_bytes(const la3_server* s, la3_query_kind kind, uint32_t graph_id) {
-  if (!s || graph_id >= s->graphs.size()) return 0;
+  if (!s) return 0;
+  std::lock_guard<std::mutex> lk(const_cast<la3_server*>(s)->mu);
+  if (graph_id >= s->graphs.size()) return 0;
   return (uint64_t)(per_vertex_bytes(kind) * s->graphs[graph_id].nslots);
 }
 

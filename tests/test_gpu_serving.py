@@ -68,3 +68,31 @@ def test_admission_budget_limits_batch(gpu_ctx, graphs, fused):
     # a budget below one query still admits lone queries (nothing else runs), one at a time
     assert len(done) == 20 and max(d["batch_size"] for d in done) == 1
     srv.close()
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_graphs_added_while_serving(gpu_ctx, graphs, fused):
+    """Graphs registered while workers run batches: the workers copy a
+    graph's entry under the server lock, so growing the table is safe."""
+    srv = L.Server(0, nworkers=2, max_batch=8, state_budget=1 << 30, keep_results=True, fused=fused)
+    g0, roots0 = graphs[0]
+    gid0 = srv.add_graph(g0)
+    rng = np.random.default_rng(11)
+    subs = []
+    for i in range(40):
+        root = int(rng.choice(roots0))
+        subs.append((srv.submit(L.Q_BFS, gid0, root, 0), g0, root))
+        if i % 10 == 5:
+            for g, roots in graphs[1:]:
+                gid = srv.add_graph(g)
+                r = int(rng.choice(roots))
+                subs.append((srv.submit(L.Q_SSSP, gid, r, 0), g, r))
+    srv.drain()
+    info = {d["qid"]: d for d in srv.completed()}
+    assert len(info) == len(subs) and all(d["status"] == 0 for d in info.values())
+    for qid, g, root in subs:
+        kind = L.Q_BFS if g is g0 else L.Q_SSSP
+        got = srv.result(qid, kind, g.nslots)
+        want = L.bfs(g, root, parents=False)[0] if kind == L.Q_BFS else L.sssp(g, root)[0]
+        assert (got == want).all()
+    srv.close()
