@@ -134,6 +134,22 @@ def test_tc_counts(gpu_ctx, oracle, scale, seed):
     assert (per == eper).all()
 
 
+def test_tc_dense_lists_closed_form(gpu_ctx):
+    """Complete graph on ids 1..n: every pair of long neighbour lists goes
+    through the wavefront intersections (binary searches and the merge
+    split). Closed form: entry (u, v), v < u, adds v - 1 to v, so vertex v
+    counts (n - v)(v - 1), and the total is C(n, 3)."""
+    n = 2200
+    ii, jj = np.triu_indices(n, 1)
+    rec = np.stack([ii + 1, jj + 1], axis=1).astype(np.uint32)
+    g = L.Graph.from_records(gpu_ctx, rec, n, False, L.ACYCLIC)
+    total, per = L.tc(g)
+    assert total == n * (n - 1) * (n - 2) // 6
+    v = np.arange(n + 1, dtype=np.int64)
+    want = np.where(v >= 1, (n - v) * (v - 1), 0)
+    assert (per.astype(np.int64) == want).all()
+
+
 # ---------------------------------------------------------- PR / degree ---
 def test_pagerank_c1_config(gpu_ctx, oracle):
     """PageRank, 20 iterations, on the scale-16 seed-1 R-MAT graph."""
