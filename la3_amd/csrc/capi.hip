@@ -1462,16 +1462,20 @@ la3_status la3_sssp_batch_ctx(la3_ctx* c, la3_graph* g, const uint32_t* roots, u
 
 }  // extern "C"
 
-template <int QW>
+// LPR < 64: batches of at most LPR queries on the narrow-row kernels
+// (k_mppr_sub_*), K = LPR lanes per vertex; else K = 64 * QW.
+template <int QW, int LPR = 64>
 static la3_status mppr_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>& seed_slots, uint32_t k,
                            uint32_t iters, double alpha, float* rank, float* comp, double* residual) {
   const uint32_t S = g->nslots;
-  constexpr uint32_t K = 64 * QW;
+  constexpr bool SUB = LPR < 64;
+  constexpr uint32_t K = SUB ? LPR : 64 * QW;
+  constexpr uint32_t RPW = SUB ? 64 / LPR : 1;  // rows per wavefront
   const DG& d = g->dg;
   const size_t SK = (size_t)S * K;
   const uint32_t wave_blocks = 4 * c->ncu * 8;
-  const uint32_t rblocks = std::max<uint32_t>(1, std::min<uint32_t>((g->nrows + 3) / 4, wave_blocks));
-  const uint32_t cblocks = std::max<uint32_t>(1, std::min<uint32_t>((d.nplong + 3) / 4, wave_blocks));
+  const uint32_t rblocks = std::max<uint32_t>(1, std::min<uint32_t>((g->nrows + 4 * RPW - 1) / (4 * RPW), wave_blocks));
+  const uint32_t cblocks = std::max<uint32_t>(1, std::min<uint32_t>((d.nplong + 4 * RPW - 1) / (4 * RPW), wave_blocks));
   float* x0 = (float*)scratch(c, "mppr.x0", 4 * SK);
   float* x1 = (float*)scratch(c, "mppr.x1", 4 * SK);
   float* hi = (float*)scratch(c, "mppr.hi", 4 * SK);
@@ -1499,22 +1503,25 @@ static la3_status mppr_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   a.part = part; a.seg_row = d.pseg_row; a.seg_beg = d.pseg_beg; a.long_ppr = d.plong;
   a.nlong_ppr = d.nplong; a.seg_first = d.pseg_first;
   launch(c, "outdeg", k_outdeg, blocks_for(S), d, deg);
-  launch(c, "mppr_init", k_mppr_init<QW>, blocks_for(SK), d, a);
+  launch(c, "mppr_init", k_mppr_init_k<K>, blocks_for(SK), d, a);
   float* xa = x0;
   float* xb = x1;
   int ev_rows = -1, ev_segs = -1;
   auto step = [&](uint32_t r0, uint32_t r1, uint32_t s0, uint32_t s1, int loop, double* rr, double* rc) {
     a.x = xa; a.xnext = xb;
     a.resid = rr;
-    ev_rows = launch(c, "mppr_rows", k_mppr_rows<QW>, rblocks, d, a, r0, r1, loop);
+    if constexpr (SUB) ev_rows = launch(c, "mppr_rows", k_mppr_sub_rows<LPR>, rblocks, d, a, r0, r1, loop);
+    else ev_rows = launch(c, "mppr_rows", k_mppr_rows<QW>, rblocks, d, a, r0, r1, loop);
     ev_segs = -1;
     if (s1 > s0) {
       PprBatchArgs as = a;
       as.seg_row = d.pseg_row + s0; as.seg_beg = d.pseg_beg + s0; as.part = part + (size_t)s0 * 2 * K;
-      ev_segs = launch(c, "mppr_segs", k_mppr_segs<QW>, std::max<uint32_t>(1, std::min<uint32_t>((s1 - s0 + 3) / 4, wave_blocks)),
-                       d, as, s1 - s0);
+      const uint32_t sblocks = std::max<uint32_t>(1, std::min<uint32_t>((s1 - s0 + 4 * RPW - 1) / (4 * RPW), wave_blocks));
+      if constexpr (SUB) ev_segs = launch(c, "mppr_segs", k_mppr_sub_segs<LPR>, sblocks, d, as, s1 - s0);
+      else ev_segs = launch(c, "mppr_segs", k_mppr_segs<QW>, sblocks, d, as, s1 - s0);
       a.resid = rc;
-      launch(c, "mppr_combine", k_mppr_combine<QW>, cblocks, d, a, r0, r1, loop);
+      if constexpr (SUB) launch(c, "mppr_combine", k_mppr_sub_combine<LPR>, cblocks, d, a, r0, r1, loop);
+      else launch(c, "mppr_combine", k_mppr_combine<QW>, cblocks, d, a, r0, r1, loop);
     }
   };
   // Algorithmic bytes per iteration: 4 B index + 4k B message row per entry
@@ -1582,7 +1589,10 @@ la3_status la3_ppr_batch_ctx(la3_ctx* c, la3_graph* g, const uint32_t* seeds, ui
   timing_begin(c);
   la3_status st;
   const uint32_t QW = (k + 63) / 64;
-  if (QW <= 1) st = mppr_run<1>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
+  if (k <= 8) st = mppr_run<1, 8>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
+  else if (k <= 16) st = mppr_run<1, 16>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
+  else if (k <= 32) st = mppr_run<1, 32>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
+  else if (QW <= 1) st = mppr_run<1>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
   else if (QW <= 2) st = mppr_run<2>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
   else if (QW <= 4) st = mppr_run<4>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
   else if (QW <= 8) st = mppr_run<8>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);

@@ -910,6 +910,150 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_combine(DG g, PprBatchArgs a, ui
 }
 
 // hi = lo = tele; x = message for every vertex with a column (constant for sources).
+// ------------------------------------------- batched PPR, narrow rows ---
+// Batches of at most 32 queries: LPR (8, 16 or 32) lanes per row, one per
+// query, and 64 / LPR rows per wavefront, so a small batch does not leave
+// most lanes of every row idle. Per query the arithmetic is that of the
+// 64-lane kernels above (same entries, same order, same updates), so a
+// query's ranks do not depend on the width of the batch it ran in; the FP64
+// residual partials are grouped differently.
+template <int LPR>
+__device__ __forceinline__ void ppr_sub_row_sum(const DG& g, const PprBatchArgs& a, uint32_t b, uint32_t e,
+                                                bool live, float& s, float& c) {
+  constexpr uint32_t UR = 8;
+  const uint32_t q = lane_id() % LPR;
+  uint32_t j = b;
+  for (; j + UR <= e; j += UR) {
+    uint32_t sc[UR];
+#pragma unroll
+    for (uint32_t u = 0; u < UR; u++) sc[u] = g.row_cols[j + u];
+    if (!live) continue;
+    float v[UR];
+#pragma unroll
+    for (uint32_t u = 0; u < UR; u++) v[u] = a.x[(size_t)sc[u] * LPR + q];
+#pragma unroll
+    for (uint32_t u = 0; u < UR; u++) kahan_add(s, c, v[u]);
+  }
+  for (; j < e; j++)
+    if (live) kahan_add(s, c, a.x[(size_t)g.row_cols[j] * LPR + q]);
+}
+
+// Lane `q` of a row group: the row's update for query q (q < k).
+template <int LPR>
+__device__ __forceinline__ void ppr_sub_finish(const PprBatchArgs& a, uint32_t r, float s, float c, bool loop,
+                                               uint32_t seed, double& res) {
+  const uint32_t q = lane_id() % LPR;
+  const size_t o = (size_t)r * LPR + q;
+  const bool sd = seed == r;
+  float nh, nl;
+  ppr_apply(sd ? a.alpha_hi : 0.f, sd ? a.alpha_lo : 0.f, a.damp_hi, a.damp_lo, s, c, nh, nl);
+  res += fabs(((double)nh + nl) - ((double)a.hi[o] + a.lo[o]));
+  a.hi[o] = nh;
+  a.lo[o] = nl;
+  if (loop) a.xnext[o] = ppr_msg(nh, nl, a.deg[r]);
+}
+
+// Per-block, per-query FP64 residual: the row groups of a wavefront by a
+// fixed butterfly, then the wavefronts in order. red: (BLOCK / WAVE) x LPR.
+template <int LPR>
+__device__ __forceinline__ void ppr_sub_resid_flush(const PprBatchArgs& a, double res, uint32_t bid, double* red) {
+#pragma unroll
+  for (int d = LPR; d < WAVE; d <<= 1) res += __shfl_xor(res, d, WAVE);
+  const uint32_t w0 = threadIdx.x / WAVE, lane = lane_id();
+  if (lane < LPR) red[w0 * LPR + lane] = res;
+  __syncthreads();
+  if (threadIdx.x < LPR) {
+    double t = 0;
+    for (uint32_t j = 0; j < BLOCK / WAVE; j++) t += red[j * LPR + threadIdx.x];
+    a.resid[(size_t)bid * LPR + threadIdx.x] = t;
+  }
+}
+
+// Rows [r0, r1) with at most PSEG entries; row group g of wavefront w takes
+// rows r0 + (w + t * nw) * (64 / LPR) + g.
+template <int LPR>
+__global__ void __launch_bounds__(BLOCK) k_mppr_sub_rows(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1, int loop) {
+  __shared__ double red[(BLOCK / WAVE) * LPR];
+  constexpr uint32_t RPW = WAVE / LPR;
+  const uint32_t lane = lane_id(), grp = lane / LPR, q = lane % LPR;
+  const bool live = q < a.k;
+  const uint32_t seed = live ? a.seeds[q] : 0xffffffffu;
+  const uint32_t nw = gridDim.x * (BLOCK / WAVE), wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  double res = 0.0;
+  for (uint64_t base = r0 + (uint64_t)wid * RPW; base < r1; base += (uint64_t)nw * RPW) {
+    const uint32_t r = (uint32_t)base + grp;
+    if (r >= r1) continue;
+    const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1];
+    if (e - b > PSEG) continue;  // segmented path
+    float s = 0.f, c = 0.f;
+    ppr_sub_row_sum<LPR>(g, a, b, e, live, s, c);
+    if (live) ppr_sub_finish<LPR>(a, r, s, c, loop != 0, seed, res);
+  }
+  ppr_sub_resid_flush<LPR>(a, res, blockIdx.x, red);
+}
+
+// PSEG-entry segments [0, nseg) of the long rows: partial (sum, comp).
+template <int LPR>
+__global__ void __launch_bounds__(BLOCK) k_mppr_sub_segs(DG g, PprBatchArgs a, uint32_t nseg) {
+  constexpr uint32_t RPW = WAVE / LPR;
+  const uint32_t lane = lane_id(), grp = lane / LPR, q = lane % LPR;
+  const bool live = q < a.k;
+  const uint32_t nw = gridDim.x * (BLOCK / WAVE), wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  for (uint64_t base = (uint64_t)wid * RPW; base < nseg; base += (uint64_t)nw * RPW) {
+    const uint32_t sg = (uint32_t)base + grp;
+    if (sg >= nseg) continue;
+    const uint32_t r = a.seg_row[sg], b = a.seg_beg[sg];
+    const uint32_t e = min(b + PSEG, g.row_ptr[r + 1]);
+    float s = 0.f, c = 0.f;
+    ppr_sub_row_sum<LPR>(g, a, b, e, live, s, c);
+    a.part[(size_t)sg * 2 * LPR + q] = s;
+    a.part[(size_t)sg * 2 * LPR + LPR + q] = c;
+  }
+}
+
+// Each long row's segment partials in segment order, then its update.
+template <int LPR>
+__global__ void __launch_bounds__(BLOCK) k_mppr_sub_combine(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1,
+                                                            int loop) {
+  __shared__ double red[(BLOCK / WAVE) * LPR];
+  constexpr uint32_t RPW = WAVE / LPR;
+  const uint32_t lane = lane_id(), grp = lane / LPR, q = lane % LPR;
+  const bool live = q < a.k;
+  const uint32_t seed = live ? a.seeds[q] : 0xffffffffu;
+  const uint32_t nw = gridDim.x * (BLOCK / WAVE), wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  double res = 0.0;
+  for (uint64_t base = (uint64_t)wid * RPW; base < a.nlong_ppr; base += (uint64_t)nw * RPW) {
+    const uint32_t i = (uint32_t)base + grp;
+    if (i >= a.nlong_ppr) continue;
+    const uint32_t r = a.long_ppr[i];
+    if (r < r0 || r >= r1 || !live) continue;
+    float s = 0.f, c = 0.f;
+    for (uint32_t sg = a.seg_first[i]; sg < a.seg_first[i + 1]; sg++) {
+      kahan_add(s, c, a.part[(size_t)sg * 2 * LPR + q]);
+      kahan_add(s, c, -a.part[(size_t)sg * 2 * LPR + LPR + q]);
+    }
+    ppr_sub_finish<LPR>(a, r, s, c, loop != 0, seed, res);
+  }
+  ppr_sub_resid_flush<LPR>(a, res, blockIdx.x, red);
+}
+
+// Initial state for K lanes per vertex (K = 64 * QW, or LPR).
+template <uint32_t K>
+__global__ void k_mppr_init_k(DG g, PprBatchArgs a) {
+  const uint64_t total = (uint64_t)g.nslots * K;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = (uint32_t)(i / K), q = (uint32_t)(i % K);
+    const bool seed = q < a.k && a.seeds[q] == s;
+    const float th = seed ? a.alpha_hi : 0.f, tl = seed ? a.alpha_lo : 0.f;
+    a.hi[i] = th;
+    a.lo[i] = tl;
+    const float x = slot_has_col(g, s) ? ppr_msg(th, tl, a.deg[s]) : 0.f;
+    ((float*)a.x)[i] = x;
+    a.xnext[i] = x;
+  }
+}
+
 template <int QW>
 __global__ void k_mppr_init(DG g, PprBatchArgs a) {
   constexpr uint32_t K = 64 * QW;

@@ -278,7 +278,7 @@ def test_sssp_batch_equals_single(gpu_ctx, oracle, scale, seed, k):
 PPR_L1_TOL = 1e-7
 
 
-@pytest.mark.parametrize("scale,seed,k", [(12, 51, 5), (14, 52, 64), (16, 53, 128)])
+@pytest.mark.parametrize("scale,seed,k", [(12, 51, 5), (13, 55, 20), (14, 52, 64), (16, 53, 128)])
 def test_ppr_batch_fp32_vs_fp64_oracle(gpu_ctx, oracle, scale, seed, k):
     """FP32 + compensation within L1 <= 1e-7 of the FP64 oracle, per query."""
     rec = _rmat(scale, seed)
@@ -306,6 +306,22 @@ def test_ppr_batch_fp32_vs_fp64_oracle(gpu_ctx, oracle, scale, seed, k):
     # a query in a batch equals the same query alone
     r1, _, _ = L.ppr_batch(g, [seeds[3]], 30)
     assert (r1[0] == rank[3]).all()
+
+
+def test_ppr_batch_width_does_not_change_ranks(gpu_ctx):
+    """Batches of <= 32 queries run on narrow rows (8, 16 or 32 lanes per
+    row, several rows per wavefront); every query's ranks and compensation
+    terms are bit-identical to the same query in a 64-lane batch."""
+    rec = _rmat(15, 56)
+    n = 1 << 15
+    g = L.Graph.from_records(gpu_ctx, rec, n, False, L.DIRECTED)
+    seeds = _roots(rec, 80, 56)
+    wide, wcomp, _ = L.ppr_batch(g, seeds, 12, comp=True)
+    for k in (1, 7, 8, 9, 16, 25, 32):
+        narrow, ncomp, resid = L.ppr_batch(g, seeds[:k], 12, comp=True)
+        assert (narrow == wide[:k]).all(), f"k={k}: ranks differ from the 64-lane batch"
+        assert (ncomp == wcomp[:k]).all(), f"k={k}: compensation differs"
+        assert np.isfinite(resid).all() and (resid >= 0).all()
 
 
 def test_ppr_batch_matches_fp64_single(gpu_ctx):
