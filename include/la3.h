@@ -164,6 +164,13 @@ la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
  * convergence. dist: u32, 2^31-1 for unreachable. */
 la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters_run);
 
+/* The same two programs on another context's stream and scratch (same
+ * device as the graph's): concurrent callers, such as the serving loop's
+ * workers, use one context each. No counterpart in the reference. */
+la3_status la3_bfs_ctx(la3_ctx* ctx, la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
+                       uint32_t* iters_run);
+la3_status la3_sssp_ctx(la3_ctx* ctx, la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters_run);
+
 /* CcVertex (cc.h:22-38) on an UNDIRECTED load: min original id per connected
  * component (label 0 is the reference's "empty" accumulator, cc.h:31; a
  * graph whose vertex 0 has edges is outside the reference's defined

@@ -32,7 +32,7 @@ EXPORTS = [
     "la3_last_timing", "la3_set_profiling", "la3_rmat_generate",
     "la3_kernel_stats", "la3_kernel_stats_reset", "la3_host_alloc", "la3_host_free",
     "la3_copy_bandwidth", "la3_result_fetch",
-    "la3_bfs_batch_ctx", "la3_sssp_batch_ctx", "la3_ppr_batch_ctx",
+    "la3_bfs_batch_ctx", "la3_sssp_batch_ctx", "la3_ppr_batch_ctx", "la3_bfs_ctx", "la3_sssp_ctx",
     "la3_server_create", "la3_server_add_graph", "la3_server_query_bytes", "la3_server_submit",
     "la3_server_drain", "la3_server_completed", "la3_server_result", "la3_server_clock",
     "la3_server_destroy", "la3_graph_set_tuning", "la3_graph_get_tuning", "la3_autotune",
@@ -142,6 +142,8 @@ def lib():
             "la3_host_free": (None, [P]),
             "la3_copy_bandwidth": (i32, [P, u64, u32, ctypes.POINTER(ctypes.c_double)]),
             "la3_result_fetch": (i32, [P, P, u64]),
+            "la3_bfs_ctx": (i32, [P, P, u32, P, P, P]),
+            "la3_sssp_ctx": (i32, [P, P, u32, P, P]),
             "la3_bfs_batch_ctx": (i32, [P, P, P, u32, P, P]),
             "la3_sssp_batch_ctx": (i32, [P, P, P, u32, P, P]),
             "la3_ppr_batch_ctx": (i32, [P, P, P, u32, u32, ctypes.c_double, P, P, P]),

@@ -895,7 +895,13 @@ la3_status la3_degree(la3_graph* g, uint32_t* degree) {
 // ---------------------------------------------------------------- BFS ---
 la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent, uint32_t* iters_run) {
   if (!g) return LA3_EINVAL;
-  la3_ctx* c = g->ctx;
+  return la3_bfs_ctx(g->ctx, g, root, hops, parent, iters_run);
+}
+
+la3_status la3_bfs_ctx(la3_ctx* c, la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
+                       uint32_t* iters_run) {
+  if (!c || !g) return LA3_EINVAL;
+  if (c->device != g->ctx->device) return fail(c, LA3_EINVAL, "graph lives on another device");
   if (root > g->n) return fail(c, LA3_EINVAL, "root id out of range");
   if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
   HIPC(c, hipSetDevice(c->device));
@@ -967,7 +973,12 @@ la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
 // --------------------------------------------------------------- SSSP ---
 la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters_run) {
   if (!g) return LA3_EINVAL;
-  la3_ctx* c = g->ctx;
+  return la3_sssp_ctx(g->ctx, g, root, dist, iters_run);
+}
+
+la3_status la3_sssp_ctx(la3_ctx* c, la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters_run) {
+  if (!c || !g) return LA3_EINVAL;
+  if (c->device != g->ctx->device) return fail(c, LA3_EINVAL, "graph lives on another device");
   if (root > g->n) return fail(c, LA3_EINVAL, "root id out of range");
   if (!g->weighted) return fail(c, LA3_EINVAL, "SSSP needs a weighted graph");
   if (!g->directed) return fail(c, LA3_EINVAL, "SSSP runs on a directed load");

@@ -267,8 +267,15 @@ void worker_main(la3_server* s, la3_ctx* ctx) {
       }
       out = host;
     }
+    // A lone BFS or SSSP query runs on the single-query program, which is
+    // faster for one root; its levels / distances are exactly the batched
+    // ones (integer results, tested equal for every root).
     la3_status st;
     if (s->cfg.keep_results && !out) st = LA3_ENOMEM;
+    else if (h.kind == LA3_Q_BFS && k == 1)
+      st = la3_bfs_ctx(ctx, ge.g, roots[0], (uint8_t*)out, nullptr, nullptr);
+    else if (h.kind == LA3_Q_SSSP && k == 1)
+      st = la3_sssp_ctx(ctx, ge.g, roots[0], (uint32_t*)out, nullptr);
     else if (h.kind == LA3_Q_BFS)
       st = la3_bfs_batch_ctx(ctx, ge.g, roots.data(), k, (uint8_t*)out, nullptr);
     else if (h.kind == LA3_Q_SSSP)
