@@ -1361,11 +1361,14 @@ la3_status la3_bfs_batch_ctx(la3_ctx* c, la3_graph* g, const uint32_t* roots, ui
 
 }  // extern "C"
 
-template <int QW>
+// LPR < 64: batches of at most LPR queries on the narrow-row kernels
+// (k_mssp_sub_*), K = LPR lanes per vertex; else K = 64 * QW.
+template <int QW, int LPR = 64>
 static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>& root_slots, uint32_t k,
                            uint32_t* dist_out, uint32_t* iters_run) {
   const uint32_t S = g->nslots;
-  constexpr uint32_t K = 64 * QW;
+  constexpr bool SUB = LPR < 64;
+  constexpr uint32_t K = SUB ? LPR : 64 * QW;
   const DG& d = g->dg;
   uint32_t* dist = (uint32_t*)scratch(c, "mssp.dist", sizeof(uint32_t) * (size_t)S * K);
   uint64_t* cm = (uint64_t*)scratch(c, "mssp.cm", sizeof(uint64_t) * (size_t)S * QW);
@@ -1391,7 +1394,7 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for((uint64_t)S * QW), (unsigned long long*)nm,
          (uint64_t)S * QW, 0ull);
   launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(S), stamp, (uint64_t)S, 0u);
-  launch(c, "mssp_roots", k_mssp_roots<QW>, 1, d, (const uint32_t*)rs, k, dist, cm);
+  launch(c, "mssp_roots", k_mssp_roots_k, 1, d, (const uint32_t*)rs, k, K, dist, cm);
   zero_counts(c, F.f[0]);
   zero_counts(c, F.f[1]);
   zero_counts(c, T);
@@ -1405,8 +1408,13 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   while (fc.items > 0) {
     const Frontier& nf = F.f[cur ^ 1];
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
-    const int ev = launch(c, "mssp_push", k_mssp_push<QW>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items,
-                          dist, (const uint64_t*)cm, nm, thr, T, F.f[cur].nitems);
+    int ev;
+    if constexpr (SUB)
+      ev = launch(c, "mssp_push", k_mssp_sub_push<LPR>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dist,
+                  (const uint64_t*)cm, nm, thr, T, F.f[cur].nitems);
+    else
+      ev = launch(c, "mssp_push", k_mssp_push<QW>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dist,
+                  (const uint64_t*)cm, nm, thr, T, F.f[cur].nitems);
     FrontCounts tc;
     if ((st = read_counts(c, T, tc))) return st;
     // The touched list becomes the next frontier's work items; clearing T's
@@ -1429,7 +1437,13 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
     fc = nc;
     cur ^= 1;
   }
-  if (g->nsink) launch(c, "mssp_sinks", k_mssp_sinks<QW>, std::min<uint32_t>((g->nsink + 3) / 4, wave_blocks), d, dist);
+  if (g->nsink) {
+    if constexpr (SUB)
+      launch(c, "mssp_sinks", k_mssp_sub_sinks<LPR>,
+             std::min<uint32_t>((g->nsink + 4 * (64 / LPR) - 1) / (4 * (64 / LPR)), wave_blocks), d, dist);
+    else
+      launch(c, "mssp_sinks", k_mssp_sinks<QW>, std::min<uint32_t>((g->nsink + 3) / 4, wave_blocks), d, dist);
+  }
   launch(c, "mssp_output", k_batch_output<uint32_t>, blocks_for(S), d, (const uint32_t*)dist, K, k, out);
   if ((st = deliver(c, dist_out, out, sizeof(uint32_t) * (size_t)S * k))) return st;
   HIPC(c, hipStreamSynchronize(c->stream));
@@ -1462,7 +1476,10 @@ la3_status la3_sssp_batch_ctx(la3_ctx* c, la3_graph* g, const uint32_t* roots, u
   timing_begin(c);
   la3_status st;
   const uint32_t QW = (k + 63) / 64;
-  if (QW <= 1) st = mssp_run<1>(c, g, rs, k, dist, iters_run);
+  if (k <= 8) st = mssp_run<1, 8>(c, g, rs, k, dist, iters_run);
+  else if (k <= 16) st = mssp_run<1, 16>(c, g, rs, k, dist, iters_run);
+  else if (k <= 32) st = mssp_run<1, 32>(c, g, rs, k, dist, iters_run);
+  else if (QW <= 1) st = mssp_run<1>(c, g, rs, k, dist, iters_run);
   else if (QW <= 2) st = mssp_run<2>(c, g, rs, k, dist, iters_run);
   else if (QW <= 4) st = mssp_run<4>(c, g, rs, k, dist, iters_run);
   else if (QW <= 8) st = mssp_run<8>(c, g, rs, k, dist, iters_run);

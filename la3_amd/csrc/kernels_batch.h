@@ -553,6 +553,140 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items
   em_finish(em, touched);
 }
 
+// ------------------------------------------- batched SSSP, narrow rows ---
+// Batches of at most 32 queries: a distance row holds LPR (8, 16 or 32)
+// queries, lane l works on query l % LPR of entry group l / LPR, and a
+// wavefront relaxes 64 / LPR entries of a column at once. A small batch
+// then reads LPR x 4 B per entry instead of a 256-B row, and its lanes are
+// busy. The relaxation rule, the near/far split and the queueing are those
+// of mssp_push_body (one mask word per row); distances are exact either way.
+template <int LPR>
+__device__ __forceinline__ void mssp_sub_push_body(const DG& g, Part p, Emitter& em, const uint64_t* items,
+                                                   uint32_t nitems, uint32_t* dist, const uint64_t* cm,
+                                                   uint64_t* nm, uint32_t T, const Frontier& touched) {
+  constexpr uint32_t EPP = WAVE / LPR;  // entries per pass
+  constexpr uint32_t U = 4;             // passes in flight
+  constexpr uint64_t QMASK = LPR == 64 ? ~0ull : ((1ull << LPR) - 1);
+  const uint32_t lane = lane_id(), grp = lane / LPR, q = lane % LPR;
+  const uint32_t nw = p.nwaves(), wid = p.wid();
+  const uint32_t mine = wid < nitems ? (nitems - wid + nw - 1) / nw : 0;
+  for (uint32_t b0 = 0; b0 < mine; b0 += WAVE) {  // wave-uniform
+    const uint32_t nb = min((uint32_t)WAVE, mine - b0);
+    uint32_t ms = 0, mbeg = 0, mend = 0, mfirst = 0;
+    if (lane < nb) {
+      const uint64_t item = items[wid + (uint64_t)(b0 + lane) * nw];
+      ms = (uint32_t)item;
+      const uint32_t ch = (uint32_t)(item >> 32);
+      const uint32_t c = col_of_slot(g, ms);
+      mbeg = g.col_ptr[c] + ch * CHUNK;
+      mend = max(mbeg, min(mbeg + CHUNK, g.col_split[c]));
+      mfirst = ch == 0;
+    }
+    for (uint32_t x = 0; x < nb; x++) {
+      const uint32_t s = rdlane(ms, x), beg = rdlane(mbeg, x), end = rdlane(mend, x);
+      const bool first = rdlane(mfirst, x) != 0;
+      uint32_t ds = ld_relaxed(&dist[(size_t)s * LPR + q]);
+      const uint64_t qm = cm[s];
+      uint32_t ne = min((uint32_t)WAVE, end - beg);
+      uint32_t rv = lane < ne ? g.col_rows[beg + lane] : 0;
+      uint32_t wv = lane < ne ? g.col_w[beg + lane] : 0;
+      const bool a0 = (qm >> q) & 1;
+      if (!a0) ds = SSSP_INF;
+      const bool far = a0 && ds >= T;
+      const bool act = a0 && !far;
+      if (far && first) em.nmin_inv = max(em.nmin_inv, ~ds);
+      const uint64_t fb = __ballot(far) & QMASK;  // group 0 holds every query
+      if (fb && first && lane == 0) atomicOr((unsigned long long*)&nm[s], (unsigned long long)fb);
+      em_push_one(em, touched, s, fb != 0 && first);
+      if ((__ballot(act) & QMASK) == 0) continue;  // nothing near: no column scan
+      for (uint32_t e0 = beg; e0 < end; e0 += WAVE) {  // wave-uniform
+        if (e0 != beg) {
+          ne = min((uint32_t)WAVE, end - e0);
+          rv = lane < ne ? g.col_rows[e0 + lane] : 0;
+          wv = lane < ne ? g.col_w[e0 + lane] : 0;
+        }
+        for (uint32_t u0 = 0; u0 < ne; u0 += U * EPP) {
+          const uint32_t n = min(U * EPP, ne - u0);
+          uint32_t r[U], w[U], cur[U];
+#pragma unroll
+          for (uint32_t u = 0; u < U; u++) {
+            const uint32_t e = u * EPP + grp;  // this lane's entry in the group
+            const uint32_t src = min(u0 + e, (uint32_t)WAVE - 1);
+            r[u] = __shfl(rv, src, WAVE);
+            w[u] = __shfl(wv, src, WAVE);
+            cur[u] = (u * EPP < n && e < n) ? ld_relaxed(&dist[(size_t)r[u] * LPR + q]) : 0u;
+          }
+          uint32_t vrow = 0;
+          uint64_t vbits = 0;
+#pragma unroll
+          for (uint32_t u = 0; u < U; u++) {
+            if (u * EPP >= n) continue;  // wave-uniform
+            const uint32_t e = u * EPP + grp;
+            const uint32_t nd = ds + w[u];
+            const bool imp = act && e < n && nd < cur[u];
+            if (imp) {
+              atomicMin(&dist[(size_t)r[u] * LPR + q], nd);
+              em.nmin_inv = max(em.nmin_inv, ~nd);
+            }
+            const uint64_t b = __ballot(imp);
+#pragma unroll
+            for (uint32_t gg = 0; gg < EPP; gg++) {
+              const uint32_t at = u * EPP + gg;  // collecting lane of entry group gg
+              vrow = lane == at ? rdlane(r[u], gg * LPR) : vrow;
+              vbits = lane == at ? (LPR == 64 ? b : (b >> (gg * LPR)) & QMASK) : vbits;
+            }
+          }
+          // The lane whose atomicOr finds the row's word zero queues the row
+          // (its first improvement in this step); k_mssp_items keeps the
+          // first occurrence of a row either way.
+          bool any = false;
+          if (vbits) any = atomicOr((unsigned long long*)&nm[vrow], (unsigned long long)vbits) == 0;
+          em_push_lanes(em, touched, vrow, any);
+        }
+      }
+    }
+  }
+}
+
+template <int LPR>
+__global__ void __launch_bounds__(BLOCK) k_mssp_sub_push(DG g, const uint64_t* items, uint32_t nitems,
+                                                         uint32_t* dist, const uint64_t* cm, uint64_t* nm,
+                                                         uint32_t T, Frontier touched, uint32_t* zc) {
+  zero_ctr(zc);
+  LA3_EMITTER(em);
+  mssp_sub_push_body<LPR>(g, grid_part(), em, items, nitems, dist, cm, nm, T, touched);
+  em_finish(em, touched);
+}
+
+// Roots: dist[slot * K + q] = 0 and the query's mask bit (K lanes per row).
+__global__ void k_mssp_roots_k(DG g, const uint32_t* root_slots, uint32_t k, uint32_t K, uint32_t* dist,
+                               uint64_t* cm) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (uint32_t q = 0; q < k; q++) {
+    const uint32_t s = root_slots[q];
+    dist[(size_t)s * K + q] = 0;
+    cm[(size_t)s * (K / 64 ? K / 64 : 1) + q / 64] |= 1ull << (q & 63);
+  }
+}
+
+// Sink rows from their final in-neighbours, LPR lanes per sink row.
+template <int LPR>
+__global__ void k_mssp_sub_sinks(DG g, uint32_t* dist) {
+  constexpr uint32_t RPW = WAVE / LPR;
+  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  const uint32_t lane = lane_id(), grp = lane / LPR, q = lane % LPR;
+  for (uint64_t base = (uint64_t)wid * RPW; base < g.nsink; base += (uint64_t)nwaves * RPW) {
+    if (base + grp >= g.nsink) continue;
+    const uint32_t r = g.nreg + (uint32_t)base + grp;
+    uint32_t y = SSSP_INF;
+    for (uint32_t e = g.row_ptr[r]; e < g.row_ptr[r + 1]; e++)
+      y = min(y, dist[(size_t)g.row_cols[e] * LPR + q] + g.row_w[e]);
+    uint32_t* pd = &dist[(size_t)r * LPR + q];
+    *pd = min(*pd, y);
+  }
+}
+
 // Turns the touched list (rows may repeat) into push work items for the
 // regular-row part of each column, keeping the first occurrence of a row in
 // this step (stamp).

@@ -274,6 +274,29 @@ def test_sssp_batch_equals_single(gpu_ctx, oracle, scale, seed, k):
         assert (dist[q] == oracle.sssp(o, roots[q])[0]).all()
 
 
+@pytest.mark.parametrize("delta", [None, 1, 0])
+def test_sssp_batch_narrow_rows_equal_wide(gpu_ctx, oracle, delta):
+    """Batches of <= 32 queries run on narrow distance rows (8, 16 or 32
+    lanes per row, several column entries per wavefront); every query's
+    distances equal the same query in a 64-lane batch and the oracle, with
+    the near/far split on (delta 1: nearly every pair waits as far), off
+    (delta 0) or at the graph's default."""
+    rec = _rmat(15, 57, weighted=True)
+    n = 1 << 15
+    g, o = _both(gpu_ctx, oracle, rec, n, True, L.DIRECTED)
+    if delta is not None:
+        g.set_tuning(L.TUNE_BATCH_SSSP_DELTA, delta)
+    roots = _roots(rec, 80, 57)
+    roots[1] = roots[0]
+    roots[2] = n
+    wide, _ = L.sssp_batch(g, roots)
+    for k in (1, 7, 8, 9, 16, 25, 32):
+        narrow, _ = L.sssp_batch(g, roots[:k])
+        assert (narrow == wide[:k]).all(), f"k={k}: distances differ from the 64-lane batch"
+    for q in (0, 3, 31):
+        assert (wide[q] == oracle.sssp(o, roots[q])[0]).all()
+
+
 # ---------------------------------------------------------- batched PPR ---
 PPR_L1_TOL = 1e-7
 
