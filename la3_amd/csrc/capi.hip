@@ -1387,8 +1387,11 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
     return fail(c, LA3_ENOMEM, "batched SSSP: device allocation failed");
   HIPC(c, hipMemcpyAsync(rs, root_slots.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream));
   const uint64_t inf2 = ((uint64_t)SSSP_INF << 32) | SSSP_INF;
-  launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for((uint64_t)S * K / 2), (unsigned long long*)dist,
-         (uint64_t)S * K / 2, (unsigned long long)inf2);
+  if constexpr (K % 2)  // one lane per row: S may be odd
+    launch(c, "fill_u32", k_fill<uint32_t>, blocks_for((uint64_t)S * K), dist, (uint64_t)S * K, SSSP_INF);
+  else
+    launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for((uint64_t)S * K / 2), (unsigned long long*)dist,
+           (uint64_t)S * K / 2, (unsigned long long)inf2);
   launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for((uint64_t)S * QW), (unsigned long long*)cm,
          (uint64_t)S * QW, 0ull);
   launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for((uint64_t)S * QW), (unsigned long long*)nm,
@@ -1476,7 +1479,10 @@ la3_status la3_sssp_batch_ctx(la3_ctx* c, la3_graph* g, const uint32_t* roots, u
   timing_begin(c);
   la3_status st;
   const uint32_t QW = (k + 63) / 64;
-  if (k <= 8) st = mssp_run<1, 8>(c, g, rs, k, dist, iters_run);
+  if (k == 1) st = mssp_run<1, 1>(c, g, rs, k, dist, iters_run);
+  else if (k == 2) st = mssp_run<1, 2>(c, g, rs, k, dist, iters_run);
+  else if (k <= 4) st = mssp_run<1, 4>(c, g, rs, k, dist, iters_run);
+  else if (k <= 8) st = mssp_run<1, 8>(c, g, rs, k, dist, iters_run);
   else if (k <= 16) st = mssp_run<1, 16>(c, g, rs, k, dist, iters_run);
   else if (k <= 32) st = mssp_run<1, 32>(c, g, rs, k, dist, iters_run);
   else if (QW <= 1) st = mssp_run<1>(c, g, rs, k, dist, iters_run);
@@ -1617,7 +1623,10 @@ la3_status la3_ppr_batch_ctx(la3_ctx* c, la3_graph* g, const uint32_t* seeds, ui
   timing_begin(c);
   la3_status st;
   const uint32_t QW = (k + 63) / 64;
-  if (k <= 8) st = mppr_run<1, 8>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
+  if (k == 1) st = mppr_run<1, 1>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
+  else if (k == 2) st = mppr_run<1, 2>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
+  else if (k <= 4) st = mppr_run<1, 4>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
+  else if (k <= 8) st = mppr_run<1, 8>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
   else if (k <= 16) st = mppr_run<1, 16>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
   else if (k <= 32) st = mppr_run<1, 32>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);
   else if (QW <= 1) st = mppr_run<1>(c, g, ss, k, iters, alpha, rank, rank_comp, residual);

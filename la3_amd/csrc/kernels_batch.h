@@ -554,8 +554,8 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items
 }
 
 // ------------------------------------------- batched SSSP, narrow rows ---
-// Batches of at most 32 queries: a distance row holds LPR (8, 16 or 32)
-// queries, lane l works on query l % LPR of entry group l / LPR, and a
+// Batches of at most 32 queries: a distance row holds LPR (1, 2, 4, 8, 16
+// or 32) queries, lane l works on query l % LPR of entry group l / LPR, and a
 // wavefront relaxes 64 / LPR entries of a column at once. A small batch
 // then reads LPR x 4 B per entry instead of a 256-B row, and its lanes are
 // busy. The relaxation rule, the near/far split and the queueing are those
@@ -564,8 +564,8 @@ template <int LPR>
 __device__ __forceinline__ void mssp_sub_push_body(const DG& g, Part p, Emitter& em, const uint64_t* items,
                                                    uint32_t nitems, uint32_t* dist, const uint64_t* cm,
                                                    uint64_t* nm, uint32_t T, const Frontier& touched) {
-  constexpr uint32_t EPP = WAVE / LPR;  // entries per pass
-  constexpr uint32_t U = 4;             // passes in flight
+  constexpr uint32_t EPP = WAVE / LPR;       // entries per pass
+  constexpr uint32_t U = LPR >= 4 ? 4 : LPR;  // passes in flight (U * EPP <= 64)
   constexpr uint64_t QMASK = LPR == 64 ? ~0ull : ((1ull << LPR) - 1);
   const uint32_t lane = lane_id(), grp = lane / LPR, q = lane % LPR;
   const uint32_t nw = p.nwaves(), wid = p.wid();
@@ -628,12 +628,17 @@ __device__ __forceinline__ void mssp_sub_push_body(const DG& g, Part p, Emitter&
               atomicMin(&dist[(size_t)r[u] * LPR + q], nd);
               em.nmin_inv = max(em.nmin_inv, ~nd);
             }
-            const uint64_t b = __ballot(imp);
+            if constexpr (LPR == 1) {  // one pass; every lane collects its own entry
+              vrow = r[u];
+              vbits = imp ? 1 : 0;
+            } else {
+              const uint64_t b = __ballot(imp);
 #pragma unroll
-            for (uint32_t gg = 0; gg < EPP; gg++) {
-              const uint32_t at = u * EPP + gg;  // collecting lane of entry group gg
-              vrow = lane == at ? rdlane(r[u], gg * LPR) : vrow;
-              vbits = lane == at ? (LPR == 64 ? b : (b >> (gg * LPR)) & QMASK) : vbits;
+              for (uint32_t gg = 0; gg < EPP; gg++) {
+                const uint32_t at = u * EPP + gg;  // collecting lane of entry group gg
+                vrow = lane == at ? rdlane(r[u], gg * LPR) : vrow;
+                vbits = lane == at ? (b >> (gg * LPR)) & QMASK : vbits;
+              }
             }
           }
           // The lane whose atomicOr finds the row's word zero queues the row
@@ -1043,10 +1048,9 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_combine(DG g, PprBatchArgs a, ui
   mppr_combine_body<QW>(g, grid_part(), a, r0, r1, loop != 0, red);
 }
 
-// hi = lo = tele; x = message for every vertex with a column (constant for sources).
 // ------------------------------------------- batched PPR, narrow rows ---
-// Batches of at most 32 queries: LPR (8, 16 or 32) lanes per row, one per
-// query, and 64 / LPR rows per wavefront, so a small batch does not leave
+// Batches of at most 32 queries: LPR (1, 2, 4, 8, 16 or 32) lanes per row,
+// one per query, and 64 / LPR rows per wavefront, so a small batch does not leave
 // most lanes of every row idle. Per query the arithmetic is that of the
 // 64-lane kernels above (same entries, same order, same updates), so a
 // query's ranks do not depend on the width of the batch it ran in; the FP64

@@ -276,8 +276,8 @@ def test_sssp_batch_equals_single(gpu_ctx, oracle, scale, seed, k):
 
 @pytest.mark.parametrize("delta", [None, 1, 0])
 def test_sssp_batch_narrow_rows_equal_wide(gpu_ctx, oracle, delta):
-    """Batches of <= 32 queries run on narrow distance rows (8, 16 or 32
-    lanes per row, several column entries per wavefront); every query's
+    """Batches of <= 32 queries run on narrow distance rows (1, 2, 4, 8, 16
+    or 32 lanes per row, several column entries per wavefront); every query's
     distances equal the same query in a 64-lane batch and the oracle, with
     the near/far split on (delta 1: nearly every pair waits as far), off
     (delta 0) or at the graph's default."""
@@ -290,7 +290,7 @@ def test_sssp_batch_narrow_rows_equal_wide(gpu_ctx, oracle, delta):
     roots[1] = roots[0]
     roots[2] = n
     wide, _ = L.sssp_batch(g, roots)
-    for k in (1, 7, 8, 9, 16, 25, 32):
+    for k in (1, 2, 3, 4, 7, 8, 9, 16, 25, 32):
         narrow, _ = L.sssp_batch(g, roots[:k])
         assert (narrow == wide[:k]).all(), f"k={k}: distances differ from the 64-lane batch"
     for q in (0, 3, 31):
@@ -332,15 +332,15 @@ def test_ppr_batch_fp32_vs_fp64_oracle(gpu_ctx, oracle, scale, seed, k):
 
 
 def test_ppr_batch_width_does_not_change_ranks(gpu_ctx):
-    """Batches of <= 32 queries run on narrow rows (8, 16 or 32 lanes per
-    row, several rows per wavefront); every query's ranks and compensation
+    """Batches of <= 32 queries run on narrow rows (1, 2, 4, 8, 16 or 32
+    lanes per row, several rows per wavefront); every query's ranks and compensation
     terms are bit-identical to the same query in a 64-lane batch."""
     rec = _rmat(15, 56)
     n = 1 << 15
     g = L.Graph.from_records(gpu_ctx, rec, n, False, L.DIRECTED)
     seeds = _roots(rec, 80, 56)
     wide, wcomp, _ = L.ppr_batch(g, seeds, 12, comp=True)
-    for k in (1, 7, 8, 9, 16, 25, 32):
+    for k in (1, 2, 3, 4, 7, 8, 9, 16, 25, 32):
         narrow, ncomp, resid = L.ppr_batch(g, seeds[:k], 12, comp=True)
         assert (narrow == wide[:k]).all(), f"k={k}: ranks differ from the 64-lane batch"
         assert (ncomp == wcomp[:k]).all(), f"k={k}: compensation differs"
