@@ -410,6 +410,7 @@ def workload_serve(args, L):
                      "six resident R-MAT graphs s18-22,24 seeds 10-15, Zipf(1.1) by size rank; "
                      "continuous batching, batches of one (program, graph), 3 workers",
                      queries_per_s=round(nq / span_b, 2),
+                     offered_qps=round(nq / float(arrivals[-1]), 2),
                      p50_ms=round(float(np.percentile(lat_b, 50)), 3),
                      p99_ms=round(float(np.percentile(lat_b, 99)), 3),
                      mean_batch=round(float(bs.mean()), 2), max_batch=int(bs.max()),

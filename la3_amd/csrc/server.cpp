@@ -8,7 +8,9 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <set>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/la3.h"
@@ -43,6 +45,8 @@ struct la3_server {
   std::map<uint64_t, std::vector<uint8_t>> results;
   uint64_t next_qid = 1, submitted = 0, completed = 0, in_flight_bytes = 0;
   uint32_t running_batches = 0;
+  // (kind, graph, iters) groups with a batch running on a worker
+  std::set<std::tuple<int, uint32_t, uint32_t>> running_groups;
   bool stop = false;
   std::vector<std::thread> workers;
 
@@ -70,17 +74,26 @@ uint64_t batch_bytes(const la3_server* s, la3_query_kind k, uint32_t graph, uint
 
 size_t elem_bytes(la3_query_kind k) { return k == LA3_Q_BFS ? 1 : 4; }
 
+std::tuple<int, uint32_t, uint32_t> group_of(const Query& q) { return {(int)q.kind, q.graph, q.iters}; }
+
 // Takes, with the lock held, the oldest pending query and the later pending
 // queries with its (kind, graph, iters), up to `cap`, while the admitted state
-// bytes fit the budget (a lone query is admitted when nothing runs). Returns
-// the picked queries (arrival order) and their state bytes.
-std::vector<Query> pick_group(la3_server* s, uint32_t cap, uint64_t& bytes) {
+// bytes fit the budget (a lone query is admitted when nothing runs). With
+// `exclusive`, queries of a group that has a batch running are left pending:
+// they go together in that group's next batch instead of a second sweep of
+// the same graph beside it. Returns the picked queries (arrival order) and
+// their state bytes.
+std::vector<Query> pick_group(la3_server* s, uint32_t cap, uint64_t& bytes, bool exclusive) {
   std::vector<Query> batch;
   bytes = 0;
   if (s->pending.empty() || cap == 0) return batch;
-  const Query head = s->pending.front();
+  size_t h = 0;
+  if (exclusive)
+    while (h < s->pending.size() && s->running_groups.count(group_of(s->pending[h]))) h++;
+  if (h == s->pending.size()) return batch;
+  const Query head = s->pending[h];
   std::vector<size_t> pick;
-  for (size_t i = 0; i < s->pending.size() && pick.size() < cap; i++) {
+  for (size_t i = h; i < s->pending.size() && pick.size() < cap; i++) {
     const Query& q = s->pending[i];
     if (q.kind != head.kind || q.graph != head.graph || q.iters != head.iters) continue;
     const uint64_t b = batch_bytes(s, head.kind, head.graph, (uint32_t)pick.size() + 1);
@@ -96,6 +109,7 @@ std::vector<Query> pick_group(la3_server* s, uint32_t cap, uint64_t& bytes) {
   std::reverse(batch.begin(), batch.end());
   s->in_flight_bytes += bytes;
   s->running_batches++;
+  if (exclusive) s->running_groups.insert(group_of(head));
   return batch;
 }
 
@@ -151,7 +165,7 @@ void worker_fused(la3_server* s, la3_ctx* ctx) {
         while (free > 0 || !sess) {
           uint64_t bytes = 0;
           const uint32_t cap = sess ? std::min<uint32_t>(s->cfg.max_batch, 256 * free) : s->cfg.max_batch;
-          std::vector<Query> qs = pick_group(s, cap, bytes);
+          std::vector<Query> qs = pick_group(s, cap, bytes, false);
           if (qs.empty()) break;
           if (!sess) {  // no session: every query fails with the creation status
             complete(s, qs, (uint32_t)qs.size(), cst, s->now(), s->now(), nullptr, bytes);
@@ -245,7 +259,7 @@ void worker_main(la3_server* s, la3_ctx* ctx) {
           la3_ctx_destroy(ctx);
           return;
         }
-        batch = pick_group(s, s->cfg.max_batch, bytes);
+        batch = pick_group(s, s->cfg.max_batch, bytes, true);
         if (!batch.empty()) break;
         s->cv.wait(lk);
       }
@@ -288,6 +302,7 @@ void worker_main(la3_server* s, la3_ctx* ctx) {
     {
       std::lock_guard<std::mutex> lk(s->mu);
       complete(s, batch, k, st, start, fin, (const uint8_t*)out, bytes);
+      s->running_groups.erase(group_of(h));
     }
     s->cv.notify_all();
   }
