@@ -932,11 +932,21 @@ __device__ __forceinline__ void ppr_row_sum(const DG& g, const PprBatchArgs& a, 
       for (uint32_t u = 0; u < UR; u++) kahan_add(s[w], c[w], v[u]);
     }
   }
-  for (; j < e; j++) {
-    const uint32_t s0 = g.row_cols[j];
+  if (j < e) {  // the last < UR entries: loaded together (clamped to e - 1), added in order
+    const uint32_t rem = e - j;
+    uint32_t sc[UR];
 #pragma unroll
-    for (int w = 0; w < QW; w++)
-      if (live[w]) kahan_add(s[w], c[w], a.x[(size_t)s0 * K + 64 * w + lane]);
+    for (uint32_t u = 0; u < UR; u++) sc[u] = g.row_cols[min(j + u, e - 1)];
+#pragma unroll
+    for (int w = 0; w < QW; w++) {
+      if (!live[w]) continue;
+      float v[UR];
+#pragma unroll
+      for (uint32_t u = 0; u < UR; u++) v[u] = a.x[(size_t)sc[u] * K + 64 * w + lane];
+#pragma unroll
+      for (uint32_t u = 0; u < UR; u++)
+        if (u < rem) kahan_add(s[w], c[w], v[u]);
+    }
   }
 }
 
@@ -1072,8 +1082,20 @@ __device__ __forceinline__ void ppr_sub_row_sum(const DG& g, const PprBatchArgs&
 #pragma unroll
     for (uint32_t u = 0; u < UR; u++) kahan_add(s, c, v[u]);
   }
-  for (; j < e; j++)
-    if (live) kahan_add(s, c, a.x[(size_t)g.row_cols[j] * LPR + q]);
+  if (j < e) {  // the last < UR entries: loaded together (clamped to e - 1), added in order
+    const uint32_t rem = e - j;
+    uint32_t sc[UR];
+#pragma unroll
+    for (uint32_t u = 0; u < UR; u++) sc[u] = g.row_cols[min(j + u, e - 1)];
+    if (live) {
+      float v[UR];
+#pragma unroll
+      for (uint32_t u = 0; u < UR; u++) v[u] = a.x[(size_t)sc[u] * LPR + q];
+#pragma unroll
+      for (uint32_t u = 0; u < UR; u++)
+        if (u < rem) kahan_add(s, c, v[u]);
+    }
+  }
 }
 
 // Lane `q` of a row group: the row's update for query q (q < k).

@@ -587,7 +587,8 @@ __device__ __forceinline__ void pr_range(const DG& g, uint32_t r, uint32_t& lo, 
 }
 
 // y + x[cols[lo]] + ... + x[cols[hi - 1]], added in entry order; the loads
-// of eight (then four) entries are issued before their adds.
+// of eight entries (the last < 8 with indices clamped to hi - 1) are issued
+// before their adds.
 __device__ __forceinline__ double seq_sum(const DG& g, const double* x, uint32_t j, uint32_t hi, double y) {
   for (; j + 8 <= hi; j += 8) {
     uint32_t cc[8];
@@ -599,15 +600,18 @@ __device__ __forceinline__ double seq_sum(const DG& g, const double* x, uint32_t
 #pragma unroll
     for (int u = 0; u < 8; u++) y = __dadd_rn(y, v[u]);
   }
-  for (; j + 4 <= hi; j += 4) {
-    const uint32_t c0 = g.row_cols[j], c1 = g.row_cols[j + 1], c2 = g.row_cols[j + 2], c3 = g.row_cols[j + 3];
-    const double v0 = x[c0], v1 = x[c1], v2 = x[c2], v3 = x[c3];
-    y = __dadd_rn(y, v0);
-    y = __dadd_rn(y, v1);
-    y = __dadd_rn(y, v2);
-    y = __dadd_rn(y, v3);
+  if (j < hi) {
+    const uint32_t rem = hi - j;
+    uint32_t cc[8];
+    double v[8];
+#pragma unroll
+    for (uint32_t u = 0; u < 8; u++) cc[u] = g.row_cols[min(j + u, hi - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = x[cc[u]];
+#pragma unroll
+    for (uint32_t u = 0; u < 8; u++)
+      if (u < rem) y = __dadd_rn(y, v[u]);
   }
-  for (; j < hi; j++) y = __dadd_rn(y, x[g.row_cols[j]]);
   return y;
 }
 
