@@ -1065,10 +1065,9 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_combine(DG g, PprBatchArgs a, ui
 // 64-lane kernels above (same entries, same order, same updates), so a
 // query's ranks do not depend on the width of the batch it ran in; the FP64
 // residual partials are grouped differently.
-template <int LPR>
+template <int LPR, uint32_t UR = 8>
 __device__ __forceinline__ void ppr_sub_row_sum(const DG& g, const PprBatchArgs& a, uint32_t b, uint32_t e,
                                                 bool live, float& s, float& c) {
-  constexpr uint32_t UR = 8;
   const uint32_t q = lane_id() % LPR;
   uint32_t j = b;
   for (; j + UR <= e; j += UR) {
@@ -1155,6 +1154,7 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_sub_rows(DG g, PprBatchArgs a, u
 // PSEG-entry segments [0, nseg) of the long rows: partial (sum, comp).
 template <int LPR>
 __global__ void __launch_bounds__(BLOCK) k_mppr_sub_segs(DG g, PprBatchArgs a, uint32_t nseg) {
+  constexpr uint32_t SEG_UR = LPR <= 8 ? 32 : 8;  // entries in flight per lane along a segment
   constexpr uint32_t RPW = WAVE / LPR;
   const uint32_t lane = lane_id(), grp = lane / LPR, q = lane % LPR;
   const bool live = q < a.k;
@@ -1165,7 +1165,7 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_sub_segs(DG g, PprBatchArgs a, u
     const uint32_t r = a.seg_row[sg], b = a.seg_beg[sg];
     const uint32_t e = min(b + PSEG, g.row_ptr[r + 1]);
     float s = 0.f, c = 0.f;
-    ppr_sub_row_sum<LPR>(g, a, b, e, live, s, c);
+    ppr_sub_row_sum<LPR, SEG_UR>(g, a, b, e, live, s, c);
     a.part[(size_t)sg * 2 * LPR + q] = s;
     a.part[(size_t)sg * 2 * LPR + LPR + q] = c;
   }
