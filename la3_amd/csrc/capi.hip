@@ -1380,8 +1380,8 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   la3_status st = get_fronts(c, g, F, "bfs");
   if (st) return st;
   Frontier T;
-  // One entry per improving scanned entry at most (rows repeat until
-  // k_mssp_items deduplicates them).
+  // One entry per improving scanned entry at most (with more than one mask
+  // word per row, rows repeat until k_mssp_items deduplicates them).
   if ((st = get_queue(c, "mssp.touched", (uint64_t)S + g->nnz + 1, T))) return st;
   if (!dist || !cm || !nm || !stamp || !rs || !out)
     return fail(c, LA3_ENOMEM, "batched SSSP: device allocation failed");
@@ -1423,7 +1423,7 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
     // The touched list becomes the next frontier's work items; clearing T's
     // counters happens inside that launch.
     launch(c, "mssp_items", k_mssp_items, blocks_for(std::max<uint32_t>(tc.items, 1)), d, (const uint64_t*)T.items,
-           tc.items, stamp, round + 2, nf, T.nitems);
+           tc.items, stamp, round + 2, (int)(QW > 1), nf, T.nitems);
     // cm of the finished step is cleared at the work-list vertices (the only
     // ones whose masks a push reads); nm becomes the next step's cm.
     launch(c, "mssp_clear", k_clear_items<QW>, blocks_for(fc.items), (const uint64_t*)F.f[cur].items, fc.items, cm);

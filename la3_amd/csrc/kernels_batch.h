@@ -453,7 +453,7 @@ __device__ __forceinline__ void mssp_push_body(const DG& g, Part p, Emitter& em,
       uint32_t rv = lane < ne ? g.col_rows[beg + lane] : 0;
       uint32_t wv = lane < ne ? g.col_w[beg + lane] : 0;
       bool act[QW];
-      bool any_near = false, any_far = false;
+      bool any_near = false, requeue = false;  // requeue: wave-uniform
 #pragma unroll
       for (int j = 0; j < QW; j++) {
         const bool a0 = (qm[j] >> lane) & 1;
@@ -461,15 +461,18 @@ __device__ __forceinline__ void mssp_push_body(const DG& g, Part p, Emitter& em,
         const bool far = a0 && ds[j] >= T;
         act[j] = a0 && !far;
         any_near |= act[j];
-        if (far && first) {
-          any_far = true;
-          em.nmin_inv = max(em.nmin_inv, ~ds[j]);
-        }
+        if (far && first) em.nmin_inv = max(em.nmin_inv, ~ds[j]);
         const uint64_t fb = __ballot(far);
-        if (fb && first && lane == 0) atomicOr((unsigned long long*)&nm[(size_t)s * QW + j], (unsigned long long)fb);
+        if (fb && first) {
+          unsigned long long old = 0;
+          if (lane == 0) old = atomicOr((unsigned long long*)&nm[(size_t)s * QW + j], (unsigned long long)fb);
+          // one mask word: queued only by the step's first OR into it (as
+          // the improvements below), so the touched list has no repeats
+          requeue |= QW == 1 ? __ballot(lane == 0 && old == 0) != 0 : true;
+        }
       }
-      // far pairs of this vertex: queue it again (k_mssp_items drops repeats)
-      em_push_one(em, touched, s, __ballot(any_far) != 0);
+      // far pairs of this vertex: queue it again
+      em_push_one(em, touched, s, requeue);
       if (__ballot(any_near) == 0) continue;  // nothing near: no column scan
       bool actw[QW];  // wave-uniform: some lane relaxes in word j
 #pragma unroll
@@ -526,8 +529,8 @@ __device__ __forceinline__ void mssp_push_body(const DG& g, Part p, Emitter& em,
           // atomicOr finds the word still zero: the first improvement of the
           // row in this step (nm starts the step zeroed, far bits included).
           // Wider rows are queued on every improvement (waiting on QW
-          // returned words costs more than it saves). k_mssp_items keeps
-          // the first occurrence of a row either way.
+          // returned words costs more than it saves) and k_mssp_items drops
+          // the repeats; with one word the touched list has none.
           bool any = false;
 #pragma unroll
           for (int j = 0; j < QW; j++) {
@@ -596,8 +599,13 @@ __device__ __forceinline__ void mssp_sub_push_body(const DG& g, Part p, Emitter&
       const bool act = a0 && !far;
       if (far && first) em.nmin_inv = max(em.nmin_inv, ~ds);
       const uint64_t fb = __ballot(far) & QMASK;  // group 0 holds every query
-      if (fb && first && lane == 0) atomicOr((unsigned long long*)&nm[s], (unsigned long long)fb);
-      em_push_one(em, touched, s, fb != 0 && first);
+      bool requeue = false;  // wave-uniform: the step's first OR into nm[s]
+      if (fb && first) {
+        unsigned long long old = 0;
+        if (lane == 0) old = atomicOr((unsigned long long*)&nm[s], (unsigned long long)fb);
+        requeue = __ballot(lane == 0 && old == 0) != 0;
+      }
+      em_push_one(em, touched, s, requeue);
       if ((__ballot(act) & QMASK) == 0) continue;  // nothing near: no column scan
       for (uint32_t e0 = beg; e0 < end; e0 += WAVE) {  // wave-uniform
         if (e0 != beg) {
@@ -642,8 +650,8 @@ __device__ __forceinline__ void mssp_sub_push_body(const DG& g, Part p, Emitter&
             }
           }
           // The lane whose atomicOr finds the row's word zero queues the row
-          // (its first improvement in this step); k_mssp_items keeps the
-          // first occurrence of a row either way.
+          // (its first improvement in this step), so the touched list holds
+          // each row once.
           bool any = false;
           if (vbits) any = atomicOr((unsigned long long*)&nm[vrow], (unsigned long long)vbits) == 0;
           em_push_lanes(em, touched, vrow, any);
@@ -692,11 +700,14 @@ __global__ void k_mssp_sub_sinks(DG g, uint32_t* dist) {
   }
 }
 
-// Turns the touched list (rows may repeat) into push work items for the
-// regular-row part of each column, keeping the first occurrence of a row in
-// this step (stamp).
+// Turns the touched list into push work items for the regular-row part of
+// each column. With `dedup` a row may repeat in the list and the first
+// occurrence in this step is kept (stamp); without, the list holds each row
+// once (rows of one mask word: the push queues a row only on the step's
+// first OR into its word, which starts the step zero at every row that has
+// items, since the finished step's masks are cleared at its work list).
 __device__ __forceinline__ void mssp_items_body(const DG& g, Part p, Emitter& em, const uint64_t* touched,
-                                                uint32_t ntouched, uint32_t* stamp, uint32_t tag,
+                                                uint32_t ntouched, uint32_t* stamp, uint32_t tag, bool dedup,
                                                 const Frontier& nf) {
   for (uint32_t base = p.bid * BLOCK; base < ntouched; base += p.nthreads()) {
     const uint32_t i = base + threadIdx.x;
@@ -704,7 +715,7 @@ __device__ __forceinline__ void mssp_items_body(const DG& g, Part p, Emitter& em
     bool first = false;
     if (i < ntouched) {
       r = (uint32_t)touched[i];
-      first = atomicMax(&stamp[r], tag) < tag;
+      first = !dedup || atomicMax(&stamp[r], tag) < tag;
       if (first) {
         ne = g.col_split[r] - g.col_ptr[r];  // touched rows are regular: column index == slot
         nit = (ne + CHUNK - 1) / CHUNK;
@@ -715,11 +726,11 @@ __device__ __forceinline__ void mssp_items_body(const DG& g, Part p, Emitter& em
 }
 
 __global__ void __launch_bounds__(BLOCK) k_mssp_items(DG g, const uint64_t* touched, uint32_t ntouched,
-                                                      uint32_t* stamp, uint32_t tag, Frontier nf,
+                                                      uint32_t* stamp, uint32_t tag, int dedup, Frontier nf,
                                                       uint32_t* zc) {
   zero_ctr(zc);
   LA3_EMITTER(em);
-  mssp_items_body(g, grid_part(), em, touched, ntouched, stamp, tag, nf);
+  mssp_items_body(g, grid_part(), em, touched, ntouched, stamp, tag, dedup != 0, nf);
   em_finish(em, nf);
 }
 

@@ -124,7 +124,7 @@ __device__ __forceinline__ void fused_sssp_b(const FTask& T, const FCtl& C, Part
   const uint32_t cur = C.cur;
   const DG g = T.g;
   const Frontier nf = T.q[cur ^ 1];
-  mssp_items_body(g, p, em, T.touched.items, *T.touched.nitems, T.stamp, C.iter + 2, nf);
+  mssp_items_body(g, p, em, T.touched.items, *T.touched.nitems, T.stamp, C.iter + 2, QW > 1, nf);
   em_finish(em, nf, p.bid);
   clear_items_body<QW>(p, T.q[cur].items, C.nitems, T.m[cur]);
 }
