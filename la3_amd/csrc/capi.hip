@@ -1182,13 +1182,37 @@ la3_status la3_cc(la3_graph* g, uint32_t* label, uint32_t* iters_run) {
   uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
   if (!p || !out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   launch(c, "cc_init", k_cc_init, blocks_for(S), d, p);
-  // Algorithmic bytes: 4 B column + 4 B parent read per entry, 8 B row
-  // pointers and a parent word per row (the finds' extra hops not counted).
-  const int ev_s = launch(c, "cc_hook", k_cc_hook, blocks_for(g->nrows), d, p);
-  const int ev_l = d.nlong ? launch(c, "cc_hook_long", k_cc_hook_long, std::min<uint32_t>((d.nlong + 3) / 4, 4096), d, p)
-                           : -1;
-  annotate(c, ev_s, 8.0 * (g->nnz - g->nnz_long_reg) + 12.0 * g->nrows, g->nnz - g->nnz_long_reg);
-  annotate(c, ev_l, 8.0 * g->nnz_long_reg, g->nnz_long_reg);
+  if (g->nrows) {
+    // Link every row to a smaller-id vertex among its first entries,
+    // compress, and take the most frequent root of a sample of rows as the
+    // largest tree; then only rows outside it union with all their entries.
+    // Algorithmic bytes: row pointers, the scanned entries and their ids per
+    // row (the finds' hops not counted); the finish pass's entries of rows
+    // outside the largest tree are not counted either.
+    constexpr uint32_t LINK_SCAN = 8, NSAMPLE = 512;
+    const int ev = launch(c, "cc_link", k_cc_link, blocks_for(g->nrows), d, p, LINK_SCAN);
+    annotate(c, ev, 8.0 * g->nrows + 8.0 * std::min<uint64_t>(g->nnz, (uint64_t)LINK_SCAN * g->nrows), g->nrows);
+    const int evc = launch(c, "cc_compress", k_cc_compress, blocks_for(S), d, p);
+    annotate(c, evc, 8.0 * S, 0);
+    uint32_t* smp = (uint32_t*)scratch(c, "cc.sample", sizeof(uint32_t) * NSAMPLE);
+    if (!smp) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+    launch(c, "cc_sample", k_cc_sample, 2, d, (const uint32_t*)p, NSAMPLE, 2654435761u, smp);
+    HIPC(c, hipMemcpyAsync(c->host_ctr, smp, sizeof(uint32_t) * NSAMPLE, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    std::vector<uint32_t> roots((const uint32_t*)c->host_ctr, (const uint32_t*)c->host_ctr + NSAMPLE);
+    std::sort(roots.begin(), roots.end());
+    uint32_t big = roots[0], best = 0;
+    for (size_t i = 0; i < roots.size();) {
+      size_t j = i;
+      while (j < roots.size() && roots[j] == roots[i]) j++;
+      if (j - i > best) { best = (uint32_t)(j - i); big = roots[i]; }
+      i = j;
+    }
+    const int evf = launch(c, "cc_finish", k_cc_finish, blocks_for(g->nrows), d, p, big, 0u);
+    annotate(c, evf, 12.0 * g->nrows, g->nnz);
+    if (d.nlong)
+      launch(c, "cc_finish_long", k_cc_finish_long, std::min<uint32_t>((d.nlong + 3) / 4, 4096), d, p, big, 0u);
+  }
   launch(c, "cc_output", k_cc_output, blocks_for(S), d, p, out);
   c->edges_scanned = g->nnz;
   la3_status st = deliver(c, label, out, sizeof(uint32_t) * S);

@@ -738,26 +738,63 @@ __global__ void k_cc_init(DG g, uint32_t* p) {
     p[s] = s;
 }
 
-__global__ void __launch_bounds__(BLOCK) k_cc_hook(DG g, uint32_t* p) {
+// Sampling-based union-find (the Afforest scheme: link a few neighbours,
+// compress, find the largest tree, then process the remaining edges of the
+// vertices outside it). Edges of an undirected load are stored in both rows,
+// so an edge with an endpoint outside the largest tree is seen from that
+// endpoint; two endpoints inside it are connected already. The hooking rule
+// (larger-id root under smaller-id root) is unchanged, so every root is its
+// component's minimum-id vertex whatever order the unions run in.
+
+// Every row r unions with the smallest-id vertex among its first `scan`
+// entries when that id is below its own (lane per row). Each union then
+// hooks the row's own root, so the threads do not compete for one word (a
+// plain first-entry link hooks a hub's root under each smaller-id neighbour
+// in turn, one compare-and-swap winner at a time).
+__global__ void __launch_bounds__(BLOCK) k_cc_link(DG g, uint32_t* p, uint32_t scan) {
   for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < g.nrows; r += gridDim.x * blockDim.x) {
-    const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1];
-    if (e - b > LONG_ROW) continue;
+    const uint32_t b = g.row_ptr[r], e = min(g.row_ptr[r + 1], g.row_ptr[r] + scan);
+    uint32_t best = r, bid = g.id_of[r];
     for (uint32_t j = b; j < e; j++) {
-      const uint32_t s = g.row_cols[j];
-      if (s < r) uf_union(g, p, r, s);
+      const uint32_t s = g.row_cols[j], id = g.id_of[s];
+      if (id < bid) { bid = id; best = s; }
     }
+    if (best != r) uf_union(g, p, r, best);
   }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_cc_hook_long(DG g, uint32_t* p) {
+// p[s] = root of s (full compression; the unions have finished).
+__global__ void k_cc_compress(DG g, uint32_t* p) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
+    const uint32_t x = uf_find(p, s);  // other threads only shorten paths to the same root
+    p[s] = x;
+  }
+}
+
+// Roots of n sampled rows (row i * stride mod nrows).
+__global__ void k_cc_sample(DG g, const uint32_t* p, uint32_t n, uint32_t stride, uint32_t* out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = p[(uint32_t)(((uint64_t)i * stride) % g.nrows)];
+}
+
+// Rows whose root is not `big`: union with entries from `from` on. Rows of at
+// most LONG_ROW entries lane per row, longer rows a wavefront each.
+__global__ void __launch_bounds__(BLOCK) k_cc_finish(DG g, uint32_t* p, uint32_t big, uint32_t from) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < g.nrows; r += gridDim.x * blockDim.x) {
+    const uint32_t b = g.row_ptr[r], e = g.row_ptr[r + 1];
+    if (e - b <= from || e - b > LONG_ROW || uf_find(p, r) == big) continue;
+    for (uint32_t j = b + from; j < e; j++) uf_union(g, p, r, g.row_cols[j]);
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cc_finish_long(DG g, uint32_t* p, uint32_t big, uint32_t from) {
   const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
   const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
   for (uint32_t i = wid; i < g.nlong; i += nwaves) {
     const uint32_t r = g.long_rows[i];
-    for (uint32_t j = g.row_ptr[r] + lane_id(); j < g.row_ptr[r + 1]; j += WAVE) {
-      const uint32_t s = g.row_cols[j];
-      if (s < r) uf_union(g, p, r, s);
-    }
+    if ((uint32_t)__builtin_amdgcn_readfirstlane((int)uf_find(p, r)) == big) continue;  // wave-uniform
+    for (uint32_t j = g.row_ptr[r] + from + lane_id(); j < g.row_ptr[r + 1]; j += WAVE)
+      uf_union(g, p, r, g.row_cols[j]);
   }
 }
 

@@ -121,6 +121,26 @@ def test_cc_labels(gpu_ctx, oracle, scale, seed):
     assert (lab == oracle.cc(o)[0]).all()
 
 
+def test_cc_many_components(gpu_ctx, oracle):
+    """Sparse graphs with no giant component (many small trees, the sampled
+    'largest tree' holds few vertices) and a graph of two equal halves: the
+    rows outside the sampled tree carry most of the unions."""
+    rng = np.random.default_rng(9)
+    n = 40000
+    a = rng.integers(1, n + 1, 15000)
+    b = rng.integers(1, n + 1, 15000)
+    sparse = np.stack([a, b], 1).astype(np.uint32)
+    half = n // 2
+    c1 = rng.integers(1, half + 1, (200000, 2))
+    c2 = rng.integers(half + 1, n + 1, (200000, 2))
+    halves = np.concatenate([c1, c2]).astype(np.uint32)
+    for rec in (sparse, halves, random_records(24, 3000, 1500)):
+        nn = int(rec.max())
+        g, o = _both(gpu_ctx, oracle, rec, nn, False, L.UNDIRECTED)
+        lab, _ = L.cc(g)
+        assert (lab == oracle.cc(o)[0]).all()
+
+
 @pytest.mark.parametrize("scale,seed", [(10, 9), (14, 10)])
 def test_tc_counts(gpu_ctx, oracle, scale, seed):
     rec = _rmat(scale, seed)
