@@ -1239,7 +1239,8 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
   if (!row_of || !per || !tot) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(S), per, (uint64_t)S, 0u);
   launch(c, "fill_u64", k_fill<unsigned long long>, 1, tot, (uint64_t)2, 0ull);
-  launch(c, "tc_rows", k_tc_entry_rows, blocks_for(g->nrows), d, row_of);
+  launch(c, "tc_rows", k_tc_entry_rows, std::max<uint32_t>(1, std::min<uint32_t>((g->nrows + 255) / 256, 4 * c->ncu * 8)), d,
+         row_of);
   const int ev = launch(c, "tc_count", k_tc_count, blocks_for(g->nnz, 4 * c->ncu * 8), d, (const uint32_t*)row_of,
                         per, tot);
   c->edges_scanned = g->nnz;

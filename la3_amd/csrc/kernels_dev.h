@@ -828,9 +828,31 @@ __device__ uint32_t tc_intersect(const DG& g, uint32_t ab, uint32_t ae, uint32_t
   return cnt;
 }
 
+// row_of[j] = the row of entry j. A wavefront takes 64 consecutive rows,
+// whose entries are one contiguous range, and writes it 64 entries at a
+// time; lane i holds row r0 + i's first entry and each entry finds its row
+// by a binary search over the lanes (the last row starting at or before it,
+// so empty rows are skipped).
 __global__ void __launch_bounds__(BLOCK) k_tc_entry_rows(DG g, uint32_t* row_of) {
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < g.nrows; r += gridDim.x * blockDim.x)
-    for (uint32_t j = g.row_ptr[r]; j < g.row_ptr[r + 1]; j++) row_of[j] = r;
+  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
+  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  const uint32_t lane = lane_id();
+  for (uint64_t r0 = (uint64_t)wid * WAVE; r0 < g.nrows; r0 += (uint64_t)nwaves * WAVE) {
+    const uint32_t nr = (uint32_t)min((uint64_t)WAVE, g.nrows - r0);
+    const uint32_t rp = g.row_ptr[r0 + min(lane, nr - 1)];
+    const uint32_t b = g.row_ptr[r0], e = g.row_ptr[r0 + nr];
+    for (uint32_t j0 = b; j0 < e; j0 += WAVE) {  // wave-uniform
+      const uint32_t j = j0 + lane;
+      uint32_t lo = 0;
+#pragma unroll
+      for (uint32_t step = WAVE / 2; step > 0; step >>= 1) {
+        const uint32_t cand = lo + step;
+        const uint32_t v = __shfl(rp, (int)min(cand, (uint32_t)WAVE - 1), WAVE);
+        if (cand < nr && v <= j) lo = cand;
+      }
+      if (j < e) row_of[j] = (uint32_t)r0 + lo;
+    }
+  }
 }
 
 // Number of keys of list A = cols[ab, ae) found in list B = cols[bb, be)
