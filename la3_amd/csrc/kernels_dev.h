@@ -913,9 +913,11 @@ __device__ uint32_t tc_intersect_merge(const DG& g, uint32_t ab, uint32_t ae, ui
 // total[0] += triangles; total[1] += sum over entries of |N-(u)| + |N-(v)|
 // (the list entries a merge intersection reads: the algorithmic bytes).
 // Lane per entry (u, v); a pair whose shorter list has more than
-// TC_WAVE_MIN entries is intersected by the whole wavefront instead, so a
-// pair of hub lists is not one lane's serial chain of binary searches.
-constexpr uint32_t TC_WAVE_MIN = 32;
+// TC_WAVE_MIN entries, or whose searches would take one lane more than
+// TC_LANE_STEPS dependent loads, is intersected by the whole wavefront
+// instead, so a pair of hub lists is not one lane's serial chain of binary
+// searches (128 measured best of 64, 128, 256 and no limit at scale 20).
+constexpr uint32_t TC_WAVE_MIN = 32, TC_LANE_STEPS = 128;
 
 __global__ void __launch_bounds__(BLOCK) k_tc_count(DG g, const uint32_t* row_of, uint32_t* per_slot,
                                                     unsigned long long* total) {
@@ -934,7 +936,11 @@ __global__ void __launch_bounds__(BLOCK) k_tc_count(DG g, const uint32_t* row_of
         work += (ue - ub) + (ve - vb);
       }
     }
-    const bool heavy = pair && min(ue - ub, ve - vb) > TC_WAVE_MIN;
+    // A lane's searches cost about min * log2(max) dependent loads; past
+    // TC_LANE_STEPS the pair goes to the wavefront (log2(max) loads per 64
+    // keys) even when its shorter list is short.
+    const uint32_t mn = min(ue - ub, ve - vb), mx = max(ue - ub, ve - vb);
+    const bool heavy = pair && (mn > TC_WAVE_MIN || mn * (32 - __clz(mx)) > TC_LANE_STEPS);
     if (pair && !heavy) {
       const uint32_t c = tc_intersect(g, ub, ue, vb, ve);
       if (c) { atomicAdd(&per_slot[v], c); acc += c; }
