@@ -75,6 +75,7 @@ struct la3_graph {
   Hasher h;
   uint64_t nnz = 0, nnz_reg = 0, nnz_sink = 0, nrecords = 0, device_bytes = 0;
   uint64_t pnnz_reg = 0, nnz_long_reg = 0;
+  uint32_t max_pseg = 0;  // most PSEG_ENTRIES-entry segments of one row
   // Near/far thresholds of the SSSP relaxation (0 = plain Bellman-Ford),
   // single-query and batched; set by la3_graph_set_tuning or la3_autotune.
   uint32_t sssp_delta = 0, mssp_delta = 0;
@@ -257,6 +258,8 @@ la3_status graph_from_layout(la3_ctx* ctx, HostLayout& L, la3_graph** out) {
   d.nrows = L.nrows; d.nnz = L.nnz; d.nlong = (uint32_t)L.long_rows.size();
   d.nplong = (uint32_t)L.plong.size(); d.nplong_reg = L.nplong_reg;
   d.npseg = (uint32_t)L.pseg_row.size(); d.npseg_reg = L.npseg_reg;
+  for (size_t i = 0; i + 1 < L.pseg_first.size(); i++)
+    g->max_pseg = std::max(g->max_pseg, L.pseg_first[i + 1] - L.pseg_first[i]);
   la3_status s;
   auto up = [&](const std::vector<uint32_t>& v, const uint32_t** o) { return upload(g.get(), v, o); };
   if ((s = up(L.col_ptr, &d.col_ptr)) || (s = up(L.col_split, &d.col_split)) ||
@@ -509,6 +512,7 @@ la3_status ingest_device(la3_ctx* c, const uint32_t* rec, uint64_t nrec, uint32_
   for (uint32_t col = 0; col < NC; col++) nnz_reg += hcs[col] - hcp[col];
   uint32_t nplong_reg = 0;
   while (nplong_reg < np && hplong[nplong_reg] < nreg) nplong_reg++;
+  for (uint64_t i = 0; i < np; i++) g->max_pseg = std::max(g->max_pseg, hpf[i + 1] - hpf[i]);
   g->nnz = nnz; g->nnz_reg = nnz_reg; g->nnz_sink = nnz - nnz_reg;
   g->nnz_long_reg = hs[0]; g->pnnz_reg = hs[1];
   DG& d = g->dg;
@@ -1029,20 +1033,28 @@ la3_status la3_sssp_ctx(la3_ctx* c, la3_graph* g, uint32_t root, uint32_t* dist,
 // ----------------------------------------------------------- PageRank ---
 }  // extern "C"
 
-// One PR pass in MODE over the regular rows (sinks = false) or the sink rows:
-// k_pr_step, then k_pr_combine for the rows longer than PSEG_ENTRIES.
-// Returns the event slot of the k_pr_step launch.
+// One PR pass in MODE over the regular rows (sinks = false) or the sink rows.
+// Rows longer than PSEG_ENTRIES: when no row has more than PR_BLOCK_SEGS
+// segments, a block per row inside k_pr_step (one launch per pass); else
+// their segments in k_pr_step, then k_pr_combine. Returns the event slot of
+// the k_pr_step launch.
+constexpr uint32_t PR_BLOCK_SEGS = 64;
 template <int MODE>
 static int pr_pass(la3_ctx* c, la3_graph* g, const PrArgs& a, bool sinks, double* part, const char* name) {
   const DG& d = g->dg;
   const uint32_t r0 = sinks ? g->nreg : 0, r1 = sinks ? g->nrows : g->nreg;
-  const uint32_t s0 = sinks ? d.npseg_reg : 0, s1 = sinks ? d.npseg : d.npseg_reg;
+  uint32_t s0 = sinks ? d.npseg_reg : 0, s1 = sinks ? d.npseg : d.npseg_reg;
   const uint32_t p0 = sinks ? d.nplong_reg : 0, p1 = sinks ? d.nplong : d.nplong_reg;
   if (r1 <= r0) return -1;
+  const bool block_rows = p1 > p0 && g->max_pseg <= PR_BLOCK_SEGS;
+  if (block_rows) s1 = s0;  // no segments
   const uint32_t nbr = blocks_for(r1 - r0);
   const uint32_t nbw = std::min<uint32_t>(((uint64_t)d.nlong + (s1 - s0) + 3) / 4, 4096);
-  const int ev = launch(c, name, k_pr_step<MODE>, nbr + nbw, d, a, r0, r1, nbr, s0, s1, part);
-  if (p1 > p0) launch(c, "pr_combine", k_pr_combine<MODE>, blocks_for(p1 - p0), d, a, p0, p1, (const double*)part);
+  const uint32_t nbb = block_rows ? std::min<uint32_t>(p1 - p0, 4096) : 0;
+  const int ev = launch(c, name, k_pr_step<MODE>, nbr + nbw + nbb, d, a, r0, r1, nbr, s0, s1, part, nbw, p0,
+                        block_rows ? p1 : p0);
+  if (p1 > p0 && !block_rows)
+    launch(c, "pr_combine", k_pr_combine<MODE>, blocks_for(p1 - p0), d, a, p0, p1, (const double*)part);
   return ev;
 }
 

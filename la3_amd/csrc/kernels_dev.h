@@ -644,15 +644,58 @@ __device__ __forceinline__ double wave_range_sum(const DG& g, const double* x, u
   return wave_sum(p);
 }
 
+// Block sum of x[cols[j]], j in [lo, hi): thread t adds entries lo + t,
+// lo + t + BLOCK, ... in order (eight loads in flight), then the wavefront
+// sums, then the wavefronts' sums in order. The result is on thread 0.
+__device__ __forceinline__ double block_range_sum(const DG& g, const double* x, uint32_t lo, uint32_t hi,
+                                                  double* red) {
+  double p = 0.0;
+  uint32_t j = lo + threadIdx.x;
+  for (; j + 7 * BLOCK < hi; j += 8 * BLOCK) {
+    uint32_t cc[8];
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) cc[u] = g.row_cols[j + u * BLOCK];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = x[cc[u]];
+#pragma unroll
+    for (int u = 0; u < 8; u++) p = __dadd_rn(p, v[u]);
+  }
+  for (; j < hi; j += BLOCK) p = __dadd_rn(p, x[g.row_cols[j]]);
+  p = wave_sum(p);
+  if (lane_id() == 0) red[threadIdx.x / WAVE] = p;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (uint32_t w = 0; w < BLOCK / WAVE; w++) s = __dadd_rn(s, red[w]);
+  __syncthreads();  // red is reused by the next row
+  return s;
+}
+
 // One PR step over rows [r0, r1) (regular rows, or sink rows for MODE 2):
 //   blocks [0, nbr): rows with <= LONG_ROW entries, lane per row;
-//   blocks [nbr, grid): long rows with <= PSEG_ENTRIES entries, wavefront
-//   per row; then the PSEG_ENTRIES-entry segments [s0, s1) of the longer
-//   rows, wavefront per segment, partial sums into part[segment]
-//   (k_pr_combine finishes those rows).
+//   blocks [nbr, nbr + nbw): long rows with <= PSEG_ENTRIES entries,
+//   wavefront per row; then the PSEG_ENTRIES-entry segments [s0, s1) of the
+//   longer rows, wavefront per segment, partial sums into part[segment]
+//   (k_pr_combine finishes those rows);
+//   blocks [nbr + nbw, grid): the rows plong[p0, p1), longer than
+//   PSEG_ENTRIES, a block per row (when the host passes no segments).
 template <int MODE>
 __global__ void __launch_bounds__(BLOCK) k_pr_step(DG g, PrArgs a, uint32_t r0, uint32_t r1, uint32_t nbr,
-                                                   uint32_t s0, uint32_t s1, double* part) {
+                                                   uint32_t s0, uint32_t s1, double* part, uint32_t nbw,
+                                                   uint32_t p0, uint32_t p1) {
+  if (blockIdx.x >= nbr + nbw) {
+    __shared__ double red[BLOCK / WAVE];
+    const uint32_t nbb = gridDim.x - nbr - nbw;
+    for (uint32_t i = p0 + (blockIdx.x - nbr - nbw); i < p1; i += nbb) {  // block-uniform
+      const uint32_t r = g.plong[i];
+      uint32_t lo, hi;
+      pr_range<MODE>(g, r, lo, hi);
+      const double p = block_range_sum(g, a.x, lo, hi, red);
+      if (threadIdx.x == 0) pr_finish<MODE>(g, a, r, MODE == 1 ? __dadd_rn(a.srcsum[r], p) : p);
+    }
+    return;
+  }
   if (blockIdx.x < nbr) {
     for (uint32_t r = r0 + blockIdx.x * BLOCK + threadIdx.x; r < r1; r += nbr * BLOCK) {
       if (g.row_ptr[r + 1] - g.row_ptr[r] > LONG_ROW) continue;
@@ -662,7 +705,7 @@ __global__ void __launch_bounds__(BLOCK) k_pr_step(DG g, PrArgs a, uint32_t r0, 
     }
     return;
   }
-  const uint32_t nwaves = (gridDim.x - nbr) * (BLOCK / WAVE);
+  const uint32_t nwaves = nbw * (BLOCK / WAVE);
   const uint32_t wid = (blockIdx.x - nbr) * (BLOCK / WAVE) + threadIdx.x / WAVE;
   for (uint32_t i = wid; i < g.nlong; i += nwaves) {
     const uint32_t r = g.long_rows[i];
