@@ -96,3 +96,24 @@ def test_graphs_added_while_serving(gpu_ctx, graphs, fused):
         want = L.bfs(g, root, parents=False)[0] if kind == L.Q_BFS else L.sssp(g, root)[0]
         assert (got == want).all()
     srv.close()
+
+
+def test_one_running_batch_per_group(gpu_ctx, graphs):
+    """Per-(program, graph) mode: a group never has two batches on the GPU at
+    once; queries arriving while its batch runs form its next batch."""
+    g, roots = graphs[2]
+    srv = L.Server(0, nworkers=3, max_batch=8, state_budget=1 << 30, keep_results=False, fused=False)
+    gid = srv.add_graph(g)
+    rng = np.random.default_rng(11)
+    for i in range(48):
+        kind = (L.Q_BFS, L.Q_PPR)[i % 2]
+        srv.submit(kind, gid, int(rng.choice(roots)), 30 if kind == L.Q_PPR else 0)
+    srv.drain()
+    done = srv.completed()
+    assert len(done) == 48 and all(d["status"] == 0 for d in done)
+    for kind in (L.Q_BFS, L.Q_PPR):
+        spans = sorted({(d["start_s"], d["done_s"]) for d in done if d["kind"] == kind})
+        for (s0, e0), (s1, e1) in zip(spans, spans[1:]):
+            assert s1 >= e0, f"kind {kind}: batch [{s1}, {e1}] overlaps [{s0}, {e0}]"
+    assert max(d["batch_size"] for d in done) > 1
+    srv.close()
