@@ -1246,18 +1246,24 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
   const uint32_t S = g->nslots;
   const DG& d = g->dg;
   uint32_t* row_of = (uint32_t*)scratch(c, "tc.row_of", sizeof(uint32_t) * (g->nnz + 1));
+  uint32_t* col_of = (uint32_t*)scratch(c, "tc.col_of", sizeof(uint32_t) * (g->nnz + 1));
   uint32_t* per = (uint32_t*)scratch(c, "tc.per", sizeof(uint32_t) * S);
   unsigned long long* tot = (unsigned long long*)scratch(c, "tc.total", 64);
-  if (!row_of || !per || !tot) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  if (!row_of || !col_of || !per || !tot) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(S), per, (uint64_t)S, 0u);
-  launch(c, "fill_u64", k_fill<unsigned long long>, 1, tot, (uint64_t)2, 0ull);
-  launch(c, "tc_rows", k_tc_entry_rows, std::max<uint32_t>(1, std::min<uint32_t>((g->nrows + 255) / 256, 4 * c->ncu * 8)), d,
-         row_of);
-  const int ev = launch(c, "tc_count", k_tc_count, blocks_for(g->nnz, 4 * c->ncu * 8), d, (const uint32_t*)row_of,
-                        per, tot);
+  launch(c, "fill_u64", k_fill<unsigned long long>, 1, tot, (uint64_t)3, 0ull);
+  const uint32_t wmax = 4 * c->ncu * 8;
+  launch(c, "tc_rows", k_tc_entry_owner, std::max<uint32_t>(1, std::min<uint32_t>((g->nrows + 255) / 256, wmax)),
+         d.row_ptr, g->nrows, row_of);
+  launch(c, "tc_cols", k_tc_entry_owner, std::max<uint32_t>(1, std::min<uint32_t>((d.ncols + 255) / 256, wmax)),
+         d.col_ptr, d.ncols, col_of);
+  const int ev = launch(c, "tc_count", k_tc_count, blocks_for(g->nnz, wmax), d, (const uint32_t*)row_of, per, tot);
+  const uint32_t hb = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((g->nnz + BLOCK - 1) / BLOCK), 16 * c->ncu));
+  const int evh = launch(c, "tc_hash", k_tc_hash<0>, hb, d, (const uint32_t*)row_of, per, tot);
+  const int evh2 = launch(c, "tc_hash_cols", k_tc_hash<1>, hb, d, (const uint32_t*)col_of, per, tot);
   c->edges_scanned = g->nnz;
-  unsigned long long h[2] = {0, 0};
-  HIPC(c, hipMemcpyAsync(c->host_ctr, tot, 16, hipMemcpyDeviceToHost, c->stream));
+  unsigned long long h[3] = {0, 0, 0};
+  HIPC(c, hipMemcpyAsync(c->host_ctr, tot, 24, hipMemcpyDeviceToHost, c->stream));
   if (per_vertex) {
     uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
     if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
@@ -1266,11 +1272,17 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
     if (st) return st;
   }
   HIPC(c, hipStreamSynchronize(c->stream));
-  std::memcpy(h, c->host_ctr, 16);
+  std::memcpy(h, c->host_ctr, 24);
   *total = h[0];
-  // Algorithmic bytes: 4 B per list entry a merge intersection reads, plus
-  // row id, column and both row pointers (16 B) per entry.
+  // Algorithmic bytes: 4 B per list entry a merge intersection of the pair
+  // reads, plus row id, column and both row pointers (16 B) per entry, by
+  // the kernel that counts the entry (h[1]: k_tc_count, h[2]: the hash
+  // passes, shared between their two launches in proportion to nothing
+  // known on the host, so all charged to the row pass; the entry metadata
+  // to k_tc_count, which reads it for every entry).
   annotate(c, ev, 4.0 * h[1] + 16.0 * g->nnz, g->nnz);
+  annotate(c, evh, 4.0 * h[2], 0);
+  (void)evh2;
   timing_end(c);
   return LA3_OK;
 }

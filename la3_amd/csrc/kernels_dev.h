@@ -871,19 +871,20 @@ __device__ uint32_t tc_intersect(const DG& g, uint32_t ab, uint32_t ae, uint32_t
   return cnt;
 }
 
-// row_of[j] = the row of entry j. A wavefront takes 64 consecutive rows,
-// whose entries are one contiguous range, and writes it 64 entries at a
-// time; lane i holds row r0 + i's first entry and each entry finds its row
-// by a binary search over the lanes (the last row starting at or before it,
-// so empty rows are skipped).
-__global__ void __launch_bounds__(BLOCK) k_tc_entry_rows(DG g, uint32_t* row_of) {
+// owner[j] = the row (or column) whose entries [ptr[i], ptr[i + 1]) hold
+// entry j, for i < n. A wavefront takes 64 consecutive owners, whose entries
+// are one contiguous range, and writes it 64 entries at a time; lane i holds
+// owner i's first entry and each entry finds its owner by a binary search
+// over the lanes (the last owner starting at or before it, so empty ones
+// are skipped).
+__global__ void __launch_bounds__(BLOCK) k_tc_entry_owner(const uint32_t* ptr, uint32_t n, uint32_t* owner) {
   const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
   const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
   const uint32_t lane = lane_id();
-  for (uint64_t r0 = (uint64_t)wid * WAVE; r0 < g.nrows; r0 += (uint64_t)nwaves * WAVE) {
-    const uint32_t nr = (uint32_t)min((uint64_t)WAVE, g.nrows - r0);
-    const uint32_t rp = g.row_ptr[r0 + min(lane, nr - 1)];
-    const uint32_t b = g.row_ptr[r0], e = g.row_ptr[r0 + nr];
+  for (uint64_t r0 = (uint64_t)wid * WAVE; r0 < n; r0 += (uint64_t)nwaves * WAVE) {
+    const uint32_t nr = (uint32_t)min((uint64_t)WAVE, n - r0);
+    const uint32_t rp = ptr[r0 + min(lane, nr - 1)];
+    const uint32_t b = ptr[r0], e = ptr[r0 + nr];
     for (uint32_t j0 = b; j0 < e; j0 += WAVE) {  // wave-uniform
       const uint32_t j = j0 + lane;
       uint32_t lo = 0;
@@ -893,7 +894,7 @@ __global__ void __launch_bounds__(BLOCK) k_tc_entry_rows(DG g, uint32_t* row_of)
         const uint32_t v = __shfl(rp, (int)min(cand, (uint32_t)WAVE - 1), WAVE);
         if (cand < nr && v <= j) lo = cand;
       }
-      if (j < e) row_of[j] = (uint32_t)r0 + lo;
+      if (j < e) owner[j] = (uint32_t)r0 + lo;
     }
   }
 }
@@ -962,9 +963,26 @@ __device__ uint32_t tc_intersect_merge(const DG& g, uint32_t ab, uint32_t ae, ui
 // searches (128 measured best of 64, 128, 256 and no limit at scale 20).
 constexpr uint32_t TC_WAVE_MIN = 32, TC_LANE_STEPS = 128;
 
+// Entries go to the LDS hash passes (k_tc_hash) when the endpoint with the
+// longer list N-(x), TC_HASH_MIN < d(x) <= TC_HASH_MAX, can hold it in a
+// table that several entries share: x = u (the row) when d(v) <= d(u); x = v
+// (the column) when d(u) < d(v) and column v has more than TC_HASH_MIN
+// entries. The other endpoint's list is then probed into the table, d(y)
+// probes instead of d(y) binary searches. Every other entry is counted here.
+constexpr uint32_t TC_HASH_MIN = 32, TC_HASH_MAX = 4096, TC_TABLE = 2 * TC_HASH_MAX;
+
+__device__ __forceinline__ bool tc_hashed(uint32_t d) { return d > TC_HASH_MIN && d <= TC_HASH_MAX; }
+
+__device__ __forceinline__ bool tc_by_hash(const DG& g, uint32_t v, uint32_t du, uint32_t dv) {
+  if (dv <= du) return tc_hashed(du);
+  if (!tc_hashed(dv)) return false;
+  const uint32_t c = col_of_slot(g, v);
+  return g.col_ptr[c + 1] - g.col_ptr[c] > TC_HASH_MIN;
+}
+
 __global__ void __launch_bounds__(BLOCK) k_tc_count(DG g, const uint32_t* row_of, uint32_t* per_slot,
                                                     unsigned long long* total) {
-  unsigned long long acc = 0, work = 0;
+  unsigned long long acc = 0, work = 0, hwork = 0;
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < g.nnz; base += stride) {
     const uint64_t j = base + threadIdx.x;
@@ -974,9 +992,13 @@ __global__ void __launch_bounds__(BLOCK) k_tc_count(DG g, const uint32_t* row_of
       u = row_of[j];
       v = g.row_cols[j];
       if (v < g.nrows) {
-        pair = true;
         ub = g.row_ptr[u]; ue = g.row_ptr[u + 1]; vb = g.row_ptr[v]; ve = g.row_ptr[v + 1];
-        work += (ue - ub) + (ve - vb);
+        if (tc_by_hash(g, v, ue - ub, ve - vb)) {
+          hwork += (ue - ub) + (ve - vb);  // counted by k_tc_hash
+        } else {
+          pair = true;
+          work += (ue - ub) + (ve - vb);
+        }
       }
     }
     // A lane's searches cost about min * log2(max) dependent loads; past
@@ -1008,8 +1030,169 @@ __global__ void __launch_bounds__(BLOCK) k_tc_count(DG g, const uint32_t* row_of
   }
   acc = wave_sum(acc);
   work = wave_sum(work);
+  hwork = wave_sum(hwork);
   if (lane_id() == 0 && acc) atomicAdd(&total[0], acc);
   if (lane_id() == 0 && work) atomicAdd(&total[1], work);
+  if (lane_id() == 0 && hwork) atomicAdd(&total[2], hwork);
+}
+
+// Multiplicative (Fibonacci) hash of a key to 32 - shift bits.
+__device__ __forceinline__ uint32_t tc_slot_hash(uint32_t key, uint32_t shift) { return (key * 0x9E3779B1u) >> shift; }
+
+// Exclusive prefix over the block of one value per thread; tot = the sum.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* ws, uint32_t& tot) {
+  const uint32_t lane = lane_id(), w = threadIdx.x / WAVE;
+  const uint32_t incl = wave_incl_scan(v);
+  if (lane == WAVE - 1) ws[w] = incl;
+  __syncthreads();
+  uint32_t off = 0;
+  tot = 0;
+#pragma unroll
+  for (uint32_t i = 0; i < BLOCK / WAVE; i++) {
+    const uint32_t x = ws[i];
+    off += i < w ? x : 0u;
+    tot += x;
+  }
+  __syncthreads();
+  return off + incl - v;
+}
+
+// LDS table of N-(x) (slots + 1, open addressing, at most half full);
+// returns its size T through lg (T = 2^lg).
+__device__ __forceinline__ void tc_build_table(const DG& g, uint32_t x, uint32_t* table, uint32_t& T,
+                                               uint32_t& shift) {
+  const uint32_t xb = g.row_ptr[x], xe = g.row_ptr[x + 1], dx = xe - xb;
+  uint32_t lg = 6;
+  while ((1u << lg) < 2 * dx) lg++;
+  T = 1u << lg;
+  shift = 32 - lg;
+  for (uint32_t i = threadIdx.x; i < T; i += BLOCK) table[i] = 0;
+  __syncthreads();
+  for (uint32_t j = xb + threadIdx.x; j < xe; j += BLOCK) {  // keys are distinct
+    const uint32_t key = g.row_cols[j] + 1;
+    uint32_t h = tc_slot_hash(key, shift);
+    while (atomicCAS(&table[h], 0u, key) != 0u) h = (h + 1) & (T - 1);
+  }
+  __syncthreads();
+}
+
+// Thread t brings list cols[lb, lb + len) (len = 0: none); the block probes
+// every element of every list into the table, four positions per thread in
+// flight, and thread t gets the number of its list's elements found.
+__device__ uint32_t tc_probe_block(const DG& g, const uint32_t* table, uint32_t T, uint32_t shift,
+                                   uint32_t* excl, uint32_t* lbeg, uint32_t* cnt, uint32_t* ws, uint32_t lb,
+                                   uint32_t len) {
+  const uint32_t t = threadIdx.x;
+  uint32_t W;
+  const uint32_t ex = block_excl_scan(len, ws, W);
+  excl[t] = ex;
+  lbeg[t] = lb;
+  cnt[t] = 0;
+  __syncthreads();
+  constexpr uint32_t PQ = 4;
+  for (uint32_t p0 = 0; p0 < W; p0 += PQ * BLOCK) {  // block-uniform
+    uint32_t o[PQ], key[PQ];
+#pragma unroll
+    for (uint32_t q = 0; q < PQ; q++) {
+      const uint32_t p = p0 + q * BLOCK + t;
+      // owner: the last thread whose prefix is <= p (threads with empty lists
+      // share their prefix with the next thread, which owns those positions)
+      o[q] = 0;
+#pragma unroll
+      for (uint32_t step = BLOCK / 2; step; step >>= 1)
+        if (excl[o[q] + step] <= p) o[q] += step;
+      key[q] = p < W ? g.row_cols[lbeg[o[q]] + (p - excl[o[q]])] + 1 : 0u;
+    }
+#pragma unroll
+    for (uint32_t q = 0; q < PQ; q++) {
+      if (!key[q]) continue;
+      uint32_t h = tc_slot_hash(key[q], shift);
+      for (;;) {  // at most half the table is used: an empty slot ends every probe
+        const uint32_t sv = table[h];
+        if (sv == key[q]) { atomicAdd(&cnt[o[q]], 1u); break; }
+        if (sv == 0) break;
+        h = (h + 1) & (T - 1);
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t c = cnt[t];
+  __syncthreads();  // excl / lbeg / cnt are rewritten by the next call
+  return c;
+}
+
+// The hash passes over 256-entry ranges of the entries: SIDE 0 row-major
+// (owner_of = row of each entry, x = u, y = v, counts to v), SIDE 1
+// column-major (owner_of = column of each entry, x = v, y = u, counts to
+// v). A block builds the table of each owner x of its range that passes the
+// rule of k_tc_count's filter (at most 10 per range: such owners have more
+// than 32 entries) and probes its eligible entries' N-(y).
+template <int SIDE>
+__global__ void __launch_bounds__(BLOCK) k_tc_hash(DG g, const uint32_t* owner_of, uint32_t* per_slot,
+                                                   unsigned long long* total) {
+  __shared__ uint32_t table[TC_TABLE];
+  __shared__ uint32_t excl[BLOCK], lbeg[BLOCK], cnt[BLOCK], xs[BLOCK];
+  __shared__ uint32_t ws[BLOCK / WAVE];
+  __shared__ uint32_t hx[BLOCK], hneed[BLOCK], hsum[BLOCK];
+  __shared__ uint32_t nh;
+  const uint32_t t = threadIdx.x;
+  unsigned long long acc = 0;
+  for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < g.nnz; base += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t j = base + t;
+    uint32_t x = 0, y = 0, dx = 0, dy = 0, yb = 0;
+    bool own = false;
+    if (j < g.nnz) {
+      if (SIDE == 0) {
+        x = owner_of[j];
+        y = g.row_cols[j];
+      } else {
+        x = slot_of_col(g, owner_of[j]);
+        y = g.col_rows[j];
+      }
+      if (x < g.nrows) dx = g.row_ptr[x + 1] - g.row_ptr[x];
+      if (y < g.nrows) { yb = g.row_ptr[y]; dy = g.row_ptr[y + 1] - yb; }
+      if (SIDE == 0) {
+        own = tc_hashed(dx);
+      } else {
+        const uint32_t c = owner_of[j];
+        own = tc_hashed(dx) && g.col_ptr[c + 1] - g.col_ptr[c] > TC_HASH_MIN;
+      }
+    }
+    const bool elig = own && (SIDE == 0 ? (y < g.nrows && dy <= dx) : dy < dx);
+    if (t == 0) nh = 0;
+    xs[t] = own ? x : 0xffffffffu;
+    __syncthreads();
+    if (own && (t == 0 || xs[t - 1] != x)) {  // first entry of an owner's run
+      const uint32_t i = atomicAdd(&nh, 1u);
+      hx[i] = x;
+      hneed[i] = 0;
+      hsum[i] = 0;
+    }
+    __syncthreads();
+    const uint32_t n = nh;
+    uint32_t mine = 0xffffffffu;
+    if (elig)
+      for (uint32_t i = 0; i < n; i++)
+        if (hx[i] == x) mine = i;
+    if (elig) atomicAdd(&hneed[mine], 1u);
+    __syncthreads();
+    for (uint32_t i = 0; i < n; i++) {  // block-uniform
+      if (!hneed[i]) continue;
+      uint32_t T, shift;
+      tc_build_table(g, hx[i], table, T, shift);
+      const bool go = mine == i;
+      const uint32_t c = tc_probe_block(g, table, T, shift, excl, lbeg, cnt, ws, go ? yb : 0u, go ? dy : 0u);
+      if (c) {
+        if (SIDE == 0) { atomicAdd(&per_slot[y], c); acc += c; }
+        else atomicAdd(&hsum[i], c);
+      }
+      __syncthreads();
+      if (SIDE == 1 && t == 0 && hsum[i]) { atomicAdd(&per_slot[hx[i]], hsum[i]); acc += hsum[i]; }
+    }
+    __syncthreads();  // nh, xs, hx are rewritten for the next range
+  }
+  acc = wave_sum(acc);
+  if (lane_id() == 0 && acc) atomicAdd(&total[0], acc);
 }
 
 }  // namespace la3
