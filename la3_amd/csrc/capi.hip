@@ -1253,10 +1253,28 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
   launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(S), per, (uint64_t)S, 0u);
   launch(c, "fill_u64", k_fill<unsigned long long>, 1, tot, (uint64_t)3, 0ull);
   const uint32_t wmax = 4 * c->ncu * 8;
-  launch(c, "tc_rows", k_tc_entry_owner, std::max<uint32_t>(1, std::min<uint32_t>((g->nrows + 255) / 256, wmax)),
-         d.row_ptr, g->nrows, row_of);
-  launch(c, "tc_cols", k_tc_entry_owner, std::max<uint32_t>(1, std::min<uint32_t>((d.ncols + 255) / 256, wmax)),
-         d.col_ptr, d.ncols, col_of);
+  // Entry -> row and entry -> column maps: each owner's first entry marked
+  // with its index, then an inclusive max-scan (balanced whatever the list
+  // lengths; a wavefront per 64 owners left hub columns on one wavefront).
+  if (g->nnz > (uint64_t)INT32_MAX) return fail(c, LA3_EINVAL, "TC: more than 2^31 - 1 entries");
+  uint32_t* mark = (uint32_t*)scratch(c, "tc.mark", sizeof(uint32_t) * (g->nnz + 1));
+  size_t need = 0;
+  if (hipcub::DeviceScan::InclusiveScan(nullptr, need, mark, row_of, hipcub::Max(), (int)g->nnz, c->stream) !=
+      hipSuccess)
+    return fail(c, LA3_EDEVICE, "TC: scan sizing failed");
+  void* stmp = scratch(c, "tc.scan_tmp", std::max<size_t>(need, 16));
+  if (!mark || !stmp) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  const std::pair<const uint32_t*, uint32_t> owners[2] = {{d.row_ptr, g->nrows}, {d.col_ptr, d.ncols}};
+  uint32_t* maps[2] = {row_of, col_of};
+  for (int m = 0; m < 2; m++) {
+    if (!g->nnz) break;
+    launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(g->nnz), mark, (uint64_t)g->nnz, 0u);
+    launch(c, "tc_marks", k_tc_mark_owner, blocks_for(std::max<uint32_t>(owners[m].second, 1)), owners[m].first,
+           owners[m].second, mark);
+    if (hipcub::DeviceScan::InclusiveScan(stmp, need, mark, maps[m], hipcub::Max(), (int)g->nnz, c->stream) !=
+        hipSuccess)
+      return fail(c, LA3_EDEVICE, "TC: owner scan failed");
+  }
   const int ev = launch(c, "tc_count", k_tc_count, blocks_for(g->nnz, wmax), d, (const uint32_t*)row_of, per, tot);
   const uint32_t hb = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((g->nnz + BLOCK - 1) / BLOCK), 16 * c->ncu));
   const int evh = launch(c, "tc_hash", k_tc_hash<0>, hb, d, (const uint32_t*)row_of, per, tot);

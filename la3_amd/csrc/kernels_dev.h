@@ -871,31 +871,13 @@ __device__ uint32_t tc_intersect(const DG& g, uint32_t ab, uint32_t ae, uint32_t
   return cnt;
 }
 
-// owner[j] = the row (or column) whose entries [ptr[i], ptr[i + 1]) hold
-// entry j, for i < n. A wavefront takes 64 consecutive owners, whose entries
-// are one contiguous range, and writes it 64 entries at a time; lane i holds
-// owner i's first entry and each entry finds its owner by a binary search
-// over the lanes (the last owner starting at or before it, so empty ones
-// are skipped).
-__global__ void __launch_bounds__(BLOCK) k_tc_entry_owner(const uint32_t* ptr, uint32_t n, uint32_t* owner) {
-  const uint32_t nwaves = gridDim.x * (BLOCK / WAVE);
-  const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
-  const uint32_t lane = lane_id();
-  for (uint64_t r0 = (uint64_t)wid * WAVE; r0 < n; r0 += (uint64_t)nwaves * WAVE) {
-    const uint32_t nr = (uint32_t)min((uint64_t)WAVE, n - r0);
-    const uint32_t rp = ptr[r0 + min(lane, nr - 1)];
-    const uint32_t b = ptr[r0], e = ptr[r0 + nr];
-    for (uint32_t j0 = b; j0 < e; j0 += WAVE) {  // wave-uniform
-      const uint32_t j = j0 + lane;
-      uint32_t lo = 0;
-#pragma unroll
-      for (uint32_t step = WAVE / 2; step > 0; step >>= 1) {
-        const uint32_t cand = lo + step;
-        const uint32_t v = __shfl(rp, (int)min(cand, (uint32_t)WAVE - 1), WAVE);
-        if (cand < nr && v <= j) lo = cand;
-      }
-      if (j < e) owner[j] = (uint32_t)r0 + lo;
-    }
+// mark[ptr[i]] = i for every non-empty owner i < n (a row or column whose
+// entries are [ptr[i], ptr[i + 1])); an inclusive max-scan of the marks (0
+// elsewhere) then gives every entry its owner.
+__global__ void k_tc_mark_owner(const uint32_t* ptr, uint32_t n, uint32_t* mark) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t b = ptr[i];
+    if (ptr[i + 1] > b) mark[b] = i;
   }
 }
 
@@ -1105,13 +1087,24 @@ __device__ uint32_t tc_probe_block(const DG& g, const uint32_t* table, uint32_t 
     }
 #pragma unroll
     for (uint32_t q = 0; q < PQ; q++) {
-      if (!key[q]) continue;
-      uint32_t h = tc_slot_hash(key[q], shift);
-      for (;;) {  // at most half the table is used: an empty slot ends every probe
-        const uint32_t sv = table[h];
-        if (sv == key[q]) { atomicAdd(&cnt[o[q]], 1u); break; }
-        if (sv == 0) break;
-        h = (h + 1) & (T - 1);
+      bool hit = false;
+      if (key[q]) {
+        uint32_t h = tc_slot_hash(key[q], shift);
+        for (;;) {  // at most half the table is used: an empty slot ends every probe
+          const uint32_t sv = table[h];
+          if (sv == key[q]) { hit = true; break; }
+          if (sv == 0) break;
+          h = (h + 1) & (T - 1);
+        }
+      }
+      // a wavefront's 64 consecutive positions mostly share one list: then
+      // one add of their hit count
+      const uint32_t o0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)o[q]);
+      if (__ballot(o[q] != o0) == 0) {
+        const uint64_t hb = __ballot(hit);
+        if (hb && lane_id() == 0) atomicAdd(&cnt[o0], (uint32_t)__popcll(hb));
+      } else if (hit) {
+        atomicAdd(&cnt[o[q]], 1u);
       }
     }
   }
