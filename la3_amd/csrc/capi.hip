@@ -1251,7 +1251,7 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
   unsigned long long* tot = (unsigned long long*)scratch(c, "tc.total", 64);
   if (!row_of || !col_of || !per || !tot) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(S), per, (uint64_t)S, 0u);
-  launch(c, "fill_u64", k_fill<unsigned long long>, 1, tot, (uint64_t)3, 0ull);
+  launch(c, "fill_u64", k_fill<unsigned long long>, 1, tot, (uint64_t)6, 0ull);
   const uint32_t wmax = 4 * c->ncu * 8;
   // Entry -> row and entry -> column maps: each owner's first entry marked
   // with its index, then an inclusive max-scan (balanced whatever the list
@@ -1276,12 +1276,19 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
       return fail(c, LA3_EDEVICE, "TC: owner scan failed");
   }
   const int ev = launch(c, "tc_count", k_tc_count, blocks_for(g->nnz, wmax), d, (const uint32_t*)row_of, per, tot);
-  const uint32_t hb = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((g->nnz + BLOCK - 1) / BLOCK), 16 * c->ncu));
-  const int evh = launch(c, "tc_hash", k_tc_hash<0>, hb, d, (const uint32_t*)row_of, per, tot);
-  const int evh2 = launch(c, "tc_hash_cols", k_tc_hash<1>, hb, d, (const uint32_t*)col_of, per, tot);
+  const uint32_t hb = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((g->nnz + BLOCK - 1) / BLOCK), 32 * c->ncu));
+  // Owners of 33..1024 entries (8-KB tables: 8 blocks per CU), then
+  // 1025..4096 (32 KB: 4 per CU). A third band (1025..2048, 16 KB) cost more
+  // in its extra pass over the ranges than its occupancy gained.
+  const int evh[4] = {
+      launch(c, "tc_hash", k_tc_hash<0, TC_HASH_MIN, TC_HASH_SPLIT>, hb, d, (const uint32_t*)row_of, per, tot),
+      launch(c, "tc_hash_big", k_tc_hash<0, TC_HASH_SPLIT, TC_HASH_MAX>, hb, d, (const uint32_t*)row_of, per, tot),
+      launch(c, "tc_hash_cols", k_tc_hash<1, TC_HASH_MIN, TC_HASH_SPLIT>, hb, d, (const uint32_t*)col_of, per, tot),
+      launch(c, "tc_hash_cols_big", k_tc_hash<1, TC_HASH_SPLIT, TC_HASH_MAX>, hb, d, (const uint32_t*)col_of, per,
+             tot)};
   c->edges_scanned = g->nnz;
-  unsigned long long h[3] = {0, 0, 0};
-  HIPC(c, hipMemcpyAsync(c->host_ctr, tot, 24, hipMemcpyDeviceToHost, c->stream));
+  unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
+  HIPC(c, hipMemcpyAsync(c->host_ctr, tot, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   if (per_vertex) {
     uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
     if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
@@ -1290,17 +1297,14 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
     if (st) return st;
   }
   HIPC(c, hipStreamSynchronize(c->stream));
-  std::memcpy(h, c->host_ctr, 24);
+  std::memcpy(h, c->host_ctr, sizeof(h));
   *total = h[0];
   // Algorithmic bytes: 4 B per list entry a merge intersection of the pair
-  // reads, plus row id, column and both row pointers (16 B) per entry, by
-  // the kernel that counts the entry (h[1]: k_tc_count, h[2]: the hash
-  // passes, shared between their two launches in proportion to nothing
-  // known on the host, so all charged to the row pass; the entry metadata
-  // to k_tc_count, which reads it for every entry).
+  // reads, by the kernel that counts the entry (h[1]: k_tc_count, h[2..5]:
+  // the four hash launches), plus row id, column and both row pointers
+  // (16 B) per entry for k_tc_count, which reads them for every entry.
   annotate(c, ev, 4.0 * h[1] + 16.0 * g->nnz, g->nnz);
-  annotate(c, evh, 4.0 * h[2], 0);
-  (void)evh2;
+  for (int k = 0; k < 4; k++) annotate(c, evh[k], 4.0 * h[2 + k], 0);
   timing_end(c);
   return LA3_OK;
 }

@@ -951,7 +951,7 @@ constexpr uint32_t TC_WAVE_MIN = 32, TC_LANE_STEPS = 128;
 // (the column) when d(u) < d(v) and column v has more than TC_HASH_MIN
 // entries. The other endpoint's list is then probed into the table, d(y)
 // probes instead of d(y) binary searches. Every other entry is counted here.
-constexpr uint32_t TC_HASH_MIN = 32, TC_HASH_MAX = 4096, TC_TABLE = 2 * TC_HASH_MAX;
+constexpr uint32_t TC_HASH_MIN = 32, TC_HASH_SPLIT = 1024, TC_HASH_MAX = 4096;
 
 __device__ __forceinline__ bool tc_hashed(uint32_t d) { return d > TC_HASH_MIN && d <= TC_HASH_MAX; }
 
@@ -964,7 +964,7 @@ __device__ __forceinline__ bool tc_by_hash(const DG& g, uint32_t v, uint32_t du,
 
 __global__ void __launch_bounds__(BLOCK) k_tc_count(DG g, const uint32_t* row_of, uint32_t* per_slot,
                                                     unsigned long long* total) {
-  unsigned long long acc = 0, work = 0, hwork = 0;
+  unsigned long long acc = 0, work = 0, hwork[4] = {0, 0, 0, 0};
   const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
   for (uint64_t base = (uint64_t)blockIdx.x * BLOCK; base < g.nnz; base += stride) {
     const uint64_t j = base + threadIdx.x;
@@ -975,8 +975,10 @@ __global__ void __launch_bounds__(BLOCK) k_tc_count(DG g, const uint32_t* row_of
       v = g.row_cols[j];
       if (v < g.nrows) {
         ub = g.row_ptr[u]; ue = g.row_ptr[u + 1]; vb = g.row_ptr[v]; ve = g.row_ptr[v + 1];
-        if (tc_by_hash(g, v, ue - ub, ve - vb)) {
-          hwork += (ue - ub) + (ve - vb);  // counted by k_tc_hash
+        if (tc_by_hash(g, v, ue - ub, ve - vb)) {  // counted by a k_tc_hash launch
+          const uint32_t du = ue - ub, dv = ve - vb, dx = max(du, dv);
+          const uint32_t k = (dv <= du ? 0u : 2u) + (dx > TC_HASH_SPLIT ? 1u : 0u);
+          hwork[k] += du + dv;
         } else {
           pair = true;
           work += (ue - ub) + (ve - vb);
@@ -1012,10 +1014,13 @@ __global__ void __launch_bounds__(BLOCK) k_tc_count(DG g, const uint32_t* row_of
   }
   acc = wave_sum(acc);
   work = wave_sum(work);
-  hwork = wave_sum(hwork);
   if (lane_id() == 0 && acc) atomicAdd(&total[0], acc);
   if (lane_id() == 0 && work) atomicAdd(&total[1], work);
-  if (lane_id() == 0 && hwork) atomicAdd(&total[2], hwork);
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const unsigned long long h = wave_sum(hwork[k]);
+    if (lane_id() == 0 && h) atomicAdd(&total[2 + k], h);
+  }
 }
 
 // Multiplicative (Fibonacci) hash of a key to 32 - shift bits.
@@ -1120,10 +1125,12 @@ __device__ uint32_t tc_probe_block(const DG& g, const uint32_t* table, uint32_t 
 // v). A block builds the table of each owner x of its range that passes the
 // rule of k_tc_count's filter (at most 10 per range: such owners have more
 // than 32 entries) and probes its eligible entries' N-(y).
-template <int SIDE>
+// The launch takes the owners with DMIN < d(x) <= DMAX: smaller tables for
+// the smaller owners leave room for more blocks per CU.
+template <int SIDE, uint32_t DMIN, uint32_t DMAX>
 __global__ void __launch_bounds__(BLOCK) k_tc_hash(DG g, const uint32_t* owner_of, uint32_t* per_slot,
                                                    unsigned long long* total) {
-  __shared__ uint32_t table[TC_TABLE];
+  __shared__ uint32_t table[2 * DMAX];
   __shared__ uint32_t excl[BLOCK], lbeg[BLOCK], cnt[BLOCK], xs[BLOCK];
   __shared__ uint32_t ws[BLOCK / WAVE];
   __shared__ uint32_t hx[BLOCK], hneed[BLOCK], hsum[BLOCK];
@@ -1144,11 +1151,12 @@ __global__ void __launch_bounds__(BLOCK) k_tc_hash(DG g, const uint32_t* owner_o
       }
       if (x < g.nrows) dx = g.row_ptr[x + 1] - g.row_ptr[x];
       if (y < g.nrows) { yb = g.row_ptr[y]; dy = g.row_ptr[y + 1] - yb; }
+      const bool band = tc_hashed(dx) && dx > DMIN && dx <= DMAX;
       if (SIDE == 0) {
-        own = tc_hashed(dx);
+        own = band;
       } else {
         const uint32_t c = owner_of[j];
-        own = tc_hashed(dx) && g.col_ptr[c + 1] - g.col_ptr[c] > TC_HASH_MIN;
+        own = band && g.col_ptr[c + 1] - g.col_ptr[c] > TC_HASH_MIN;
       }
     }
     const bool elig = own && (SIDE == 0 ? (y < g.nrows && dy <= dx) : dy < dx);
