@@ -263,6 +263,97 @@ class CtVertex : public ProgramBase<Graph<Empty>, CtState> {
   uint64_t total_ = 0;
 };
 
+// ------------------------------------------------------------- batched ---
+// B queries of one program as one batched run (la3_*_batch). The state of
+// vertex vid holds the B per-query values. The batched programs have no
+// counterpart program in the reference; btopk keeps VertexProgram::btopk's
+// signature and algorithm (vertex_program.hpp:683-767) at one rank.
+template <uint32_t B, class V>
+struct BatchState {
+  V v[B];
+};
+
+template <class G, uint32_t B, class V>
+class BatchProgramBase : public ProgramBase<G, BatchState<B, V>> {
+ public:
+  explicit BatchProgramBase(const G* graph) : ProgramBase<G, BatchState<B, V>>(graph) {}
+  uint32_t roots[B] = {};  // query q starts at roots[q] (its seed for PPR)
+
+  // Per query b < BATCH_SIZE: (id, map(id, state)[b]) for every id in slot
+  // order, std::partial_sort by cmp, the first k, into topk[b].
+  template <uint32_t BATCH_SIZE, class I, class VV, class Mapper, class Comparator>
+  void btopk(uint32_t k, std::vector<std::pair<I, VV>>* topk, Mapper map, Comparator cmp,
+             bool /*active_only*/ = false) const {
+    static_assert(BATCH_SIZE <= B, "btopk: BATCH_SIZE exceeds the batch");
+    std::vector<std::pair<I, VV>> ivs[BATCH_SIZE];
+    for (uint32_t vid : this->G_->slot_ids()) {
+      const VV* vals = map(vid, this->states_[vid]);
+      for (uint32_t b = 0; b < BATCH_SIZE; b++) ivs[b].push_back(std::pair<I, VV>((I)vid, vals[b]));
+    }
+    for (uint32_t b = 0; b < BATCH_SIZE; b++) {
+      const size_t kk = std::min<size_t>(k, ivs[b].size());
+      std::partial_sort(ivs[b].begin(), ivs[b].begin() + kk, ivs[b].end(), cmp);
+      ivs[b].resize(kk);
+      topk[b] = ivs[b];
+    }
+  }
+  uint32_t iterations() const { return iters_; }
+
+ protected:
+  uint32_t iters_ = 0;
+  // rows[q * nslots + id], query-major as the batch calls return them
+  void fill(const std::vector<V>& rows) {
+    const uint32_t S = this->nslots();
+    this->states_.assign(S, BatchState<B, V>());
+    for (uint32_t q = 0; q < B; q++)
+      for (uint32_t i = 0; i < S; i++) this->states_[i].v[q] = rows[(size_t)q * S + i];
+  }
+};
+
+// B BFS queries (levels, 255 = unreached), one k-bit traversal.
+template <uint32_t B>
+class BfsBatchVertex : public BatchProgramBase<Graph<Empty>, B, uint8_t> {
+ public:
+  BfsBatchVertex(const Graph<Empty>* G, bool = false) : BatchProgramBase<Graph<Empty>, B, uint8_t>(G) {}
+  void initialize() {}
+  void execute(uint32_t = 0) {
+    std::vector<uint8_t> h((size_t)B * this->nslots());
+    this->ctx().check(la3_bfs_batch(this->G_->handle(), this->roots, B, h.data(), &this->iters_), "bfs_batch");
+    this->fill(h);
+  }
+};
+
+// B SSSP queries (distances, 2^31 - 1 = unreachable), one k-wide traversal.
+template <uint32_t B>
+class SpBatchVertex : public BatchProgramBase<Graph<uint32_t>, B, uint32_t> {
+ public:
+  SpBatchVertex(const Graph<uint32_t>* G, bool = false) : BatchProgramBase<Graph<uint32_t>, B, uint32_t>(G) {}
+  void initialize() {}
+  void execute(uint32_t = 0) {
+    std::vector<uint32_t> d((size_t)B * this->nslots());
+    this->ctx().check(la3_sssp_batch(this->G_->handle(), this->roots, B, d.data(), &this->iters_),
+                      "sssp_batch");
+    this->fill(d);
+  }
+};
+
+// B personalized-PageRank queries (FP32 with compensation), max_iters
+// iterations of the PR step with teleport alpha at each query's seed.
+template <uint32_t B>
+class PprBatchVertex : public BatchProgramBase<Graph<Empty>, B, float> {
+ public:
+  PprBatchVertex(const Graph<Empty>* G, bool = true) : BatchProgramBase<Graph<Empty>, B, float>(G) {}
+  double alpha = 0.15;
+  void initialize() {}
+  void execute(uint32_t max_iters) {
+    std::vector<float> r((size_t)B * this->nslots());
+    this->ctx().check(la3_ppr_batch(this->G_->handle(), this->roots, B, max_iters, alpha, r.data(), nullptr, nullptr),
+                      "ppr_batch");
+    this->iters_ = max_iters;
+    this->fill(r);
+  }
+};
+
 }  // namespace la3
 
 #endif  // LA3_HPP

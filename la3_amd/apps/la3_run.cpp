@@ -7,6 +7,9 @@
 //   la3_run cc     <filepath> <num_vertices>
 //   la3_run tc     <filepath> <num_vertices>
 //   la3_run degree <filepath> <num_vertices> [-o]
+//   la3_run ppr_topk <filepath> <num_vertices> <iterations> <k> <seed0..seed3>
+//     (four personalized-PageRank queries as one batch, per-query top-k by
+//      btopk, vertex_program.hpp:683-767; no reference driver)
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -29,7 +32,7 @@ static void timer(const char* name, double t0) {
 
 int main(int argc, char** argv) {
   if (argc < 4) {
-    fprintf(stderr, "Usage: %s <pr|bfs|sssp|cc|tc|degree> <filepath> <num_vertices> [args]\n", argv[0]);
+    fprintf(stderr, "Usage: %s <pr|bfs|sssp|cc|tc|degree|ppr_topk> <filepath> <num_vertices> [args]\n", argv[0]);
     return 1;
   }
   const std::string app = argv[1], path = argv[2];
@@ -137,6 +140,26 @@ int main(int argc, char** argv) {
                                   [](const iv_t& a, const iv_t& b) { return a.second > b.second; });
       printf("Top-%i: \n", k);
       for (const auto& iv : top) printf("idx %u: degree %u \n", iv.first, iv.second);
+    } else if (app == "ppr_topk") {
+      constexpr uint32_t B = 4;
+      if (argc < 6 + (int)B) { fprintf(stderr, "missing <iterations> <k> <seed0..seed3>\n"); return 1; }
+      const uint32_t niters = (uint32_t)atoi(argv[4]), k = (uint32_t)atoi(argv[5]);
+      la3::Graph<Empty> G(ctx);
+      G.load_directed(true, path, n);
+      la3::PprBatchVertex<B> vp(&G);
+      for (uint32_t q = 0; q < B; q++) vp.roots[q] = (uint32_t)std::atol(argv[6 + q]);
+      vp.initialize();
+      double t = now();
+      vp.execute(niters);
+      timer("PPR Batch Execution", t);
+      using iv_t = std::pair<uint32_t, float>;
+      std::vector<iv_t> top[B];
+      vp.btopk<B, uint32_t, float>(k, top, [](uint32_t, const la3::BatchState<B, float>& s) { return s.v; },
+                                   [](const iv_t& a, const iv_t& b) { return a.second > b.second; });
+      for (uint32_t q = 0; q < B; q++) {
+        printf("Query %u (seed %u) top-%u: \n", q, vp.roots[q], k);
+        for (const auto& iv : top[q]) printf("idx %u: rank %.9g \n", iv.first, (double)iv.second);
+      }
     } else {
       fprintf(stderr, "unknown app %s\n", app.c_str());
       return 1;

@@ -2,8 +2,10 @@
 counterpart of the reference's app drivers: same arguments, same output
 lines, and the g1 checksums the reference prints at np=1."""
 import os
+import re
 import subprocess
 
+import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -55,8 +57,6 @@ def test_cc_tc_degree_drivers(g1_files, g1):
 def test_degree_driver_top3(g1_files, g1):
     """degree.cpp:34-43: Top-3 by in-degree (std::partial_sort over the
     states in slot order; equal degrees may come in either order)."""
-    import re
-    import numpy as np
     out = run("degree", g1_files[0], 8)
     top = [(int(a), int(b)) for a, b in re.findall(r"idx (\d+): degree (\d+)", out)]
     assert "Top-3:" in out and len(top) == 3
@@ -65,3 +65,31 @@ def test_degree_driver_top3(g1_files, g1):
     assert [d for _, d in top] == sorted(indeg, reverse=True)[:3]
     assert all(indeg[i] == d for i, d in top)
 
+
+
+def test_ppr_batch_btopk_driver(tmp_path, gpu_ctx):
+    """la3_run ppr_topk: four PPR queries as one batch through
+    la3::PprBatchVertex<4>, per-query top-k by btopk; each printed (id, rank)
+    is that query's rank from la3_ppr_batch, the list is in descending rank
+    order and holds the k largest ranks."""
+    import la3_amd as L
+    rec = L.rmat(12, 16, seed=81)
+    n = 1 << 12
+    path = tmp_path / "g.bin"
+    rec.tofile(path)
+    src = np.unique(rec[:, 0])
+    seeds = [int(x) for x in np.random.default_rng(81).choice(src, size=4, replace=False)]
+    k = 10
+    out = run("ppr_topk", path, n, 20, k, *seeds)
+    g = L.Graph.from_records(gpu_ctx, rec, n, False, L.DIRECTED)
+    rank, _, _ = L.ppr_batch(g, seeds, 20)
+    blocks = re.split(r"Query \d+ \(seed \d+\) top-\d+:", out)[1:]
+    assert len(blocks) == 4
+    for q, blk in enumerate(blocks):
+        top = [(int(a), float(b)) for a, b in re.findall(r"idx (\d+): rank (\S+)", blk)]
+        assert len(top) == k
+        vals = [v for _, v in top]
+        assert vals == sorted(vals, reverse=True)
+        for i, v in top:
+            assert np.float32(v) == rank[q][i]
+        assert np.float32(vals[-1]) == np.sort(rank[q])[::-1][k - 1]
