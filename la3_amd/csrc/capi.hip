@@ -1278,8 +1278,8 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
   const int ev = launch(c, "tc_count", k_tc_count, blocks_for(g->nnz, wmax), d, (const uint32_t*)row_of, per, tot);
   const uint32_t hb = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((g->nnz + BLOCK - 1) / BLOCK), 32 * c->ncu));
   // Owners of 33..1024 entries (8-KB tables: 8 blocks per CU), then
-  // 1025..4096 (32 KB: 4 per CU). A third band (1025..2048, 16 KB) cost more
-  // in its extra pass over the ranges than its occupancy gained.
+  // 1025..4096 (32 KB: 4 per CU). Splitting the upper band once more cost
+  // more in its extra pass over the ranges than its occupancy gained.
   const int evh[4] = {
       launch(c, "tc_hash", k_tc_hash<0, TC_HASH_MIN, TC_HASH_SPLIT>, hb, d, (const uint32_t*)row_of, per, tot),
       launch(c, "tc_hash_big", k_tc_hash<0, TC_HASH_SPLIT, TC_HASH_MAX>, hb, d, (const uint32_t*)row_of, per, tot),

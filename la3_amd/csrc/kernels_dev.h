@@ -951,6 +951,10 @@ constexpr uint32_t TC_WAVE_MIN = 32, TC_LANE_STEPS = 128;
 // (the column) when d(u) < d(v) and column v has more than TC_HASH_MIN
 // entries. The other endpoint's list is then probed into the table, d(y)
 // probes instead of d(y) binary searches. Every other entry is counted here.
+// Owner bands of the hash launches: (MIN, SPLIT] with 8-KB tables and
+// (SPLIT, MAX] with 32 KB. Longer lists stay with k_tc_count: a third band
+// to 16384 (128-KB tables, one block per CU) took 6.6 ms for pairs that the
+// wavefront merges of k_tc_count count in about 0.4 ms.
 constexpr uint32_t TC_HASH_MIN = 32, TC_HASH_SPLIT = 1024, TC_HASH_MAX = 4096;
 
 __device__ __forceinline__ bool tc_hashed(uint32_t d) { return d > TC_HASH_MIN && d <= TC_HASH_MAX; }
@@ -1142,21 +1146,18 @@ __global__ void __launch_bounds__(BLOCK) k_tc_hash(DG g, const uint32_t* owner_o
     uint32_t x = 0, y = 0, dx = 0, dy = 0, yb = 0;
     bool own = false;
     if (j < g.nnz) {
-      if (SIDE == 0) {
-        x = owner_of[j];
-        y = g.row_cols[j];
-      } else {
-        x = slot_of_col(g, owner_of[j]);
-        y = g.col_rows[j];
-      }
+      // the owner's fields are shared by a run of entries (cached); the other
+      // endpoint's list is read only for entries this launch owns
+      x = SIDE == 0 ? owner_of[j] : slot_of_col(g, owner_of[j]);
       if (x < g.nrows) dx = g.row_ptr[x + 1] - g.row_ptr[x];
-      if (y < g.nrows) { yb = g.row_ptr[y]; dy = g.row_ptr[y + 1] - yb; }
-      const bool band = tc_hashed(dx) && dx > DMIN && dx <= DMAX;
-      if (SIDE == 0) {
-        own = band;
-      } else {
+      own = tc_hashed(dx) && dx > DMIN && dx <= DMAX;
+      if (SIDE == 1 && own) {
         const uint32_t c = owner_of[j];
-        own = band && g.col_ptr[c + 1] - g.col_ptr[c] > TC_HASH_MIN;
+        own = g.col_ptr[c + 1] - g.col_ptr[c] > TC_HASH_MIN;
+      }
+      if (own) {
+        y = SIDE == 0 ? g.row_cols[j] : g.col_rows[j];
+        if (y < g.nrows) { yb = g.row_ptr[y]; dy = g.row_ptr[y + 1] - yb; }
       }
     }
     const bool elig = own && (SIDE == 0 ? (y < g.nrows && dy <= dx) : dy < dx);

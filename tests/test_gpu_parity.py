@@ -154,12 +154,12 @@ def test_tc_counts(gpu_ctx, oracle, scale, seed):
     assert (per == eper).all()
 
 
-def test_tc_dense_lists_closed_form(gpu_ctx):
+@pytest.mark.parametrize("n", [2200, 4300])
+def test_tc_dense_lists_closed_form(gpu_ctx, n):
     """Complete graph on ids 1..n: every pair of long neighbour lists goes
-    through the wavefront intersections (binary searches and the merge
-    split). Closed form: entry (u, v), v < u, adds v - 1 to v, so vertex v
-    counts (n - v)(v - 1), and the total is C(n, 3)."""
-    n = 2200
+    through the LDS hash passes of both owner bands, and (n = 4300: lists of
+    more than 4096 entries) through the wavefront intersections. Closed form: entry (u, v), v < u, adds v - 1 to v, so
+    vertex v counts (n - v)(v - 1), and the total is C(n, 3)."""
     ii, jj = np.triu_indices(n, 1)
     rec = np.stack([ii + 1, jj + 1], axis=1).astype(np.uint32)
     g = L.Graph.from_records(gpu_ctx, rec, n, False, L.ACYCLIC)
