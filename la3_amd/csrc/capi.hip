@@ -1757,9 +1757,12 @@ la3_status la3_autotune(la3_graph* g, const uint32_t* roots, uint32_t nroots, la
     if (rep) rep->single_ms[i] = ts * 1e3 / nsingle;
     if (ts < best_s) { best_s = ts; pick_s = cand[i]; }
     g->mssp_delta = cand[i];
-    t0 = clock();
-    if ((st = la3_sssp_batch(g, roots, kb, nullptr, nullptr))) return st;
-    const double tb = clock() - t0;
+    double tb = 1e30;  // best of three: candidates often differ by a few percent
+    for (int r = 0; r < 3; r++) {
+      t0 = clock();
+      if ((st = la3_sssp_batch(g, roots, kb, nullptr, nullptr))) return st;
+      tb = std::min(tb, clock() - t0);
+    }
     if (rep) rep->batch_ms[i] = tb * 1e3;
     if (tb < best_b) { best_b = tb; pick_b = cand[i]; }
   }
@@ -1781,9 +1784,12 @@ la3_status la3_autotune(la3_graph* g, const uint32_t* roots, uint32_t nroots, la
   auto run = [&](std::vector<la3_batch>& bs, double& ms) -> la3_status {
     la3_status st = la3_batch_run(c, bs.data(), (uint32_t)bs.size(), 0.15);  // warm (scratch, graph capture)
     if (st) return st;
-    const double t0 = clock();
-    if ((st = la3_batch_run(c, bs.data(), (uint32_t)bs.size(), 0.15))) return st;
-    ms = (clock() - t0) * 1e3;
+    ms = 1e30;
+    for (int r = 0; r < 3; r++) {  // best of three
+      const double t0 = clock();
+      if ((st = la3_batch_run(c, bs.data(), (uint32_t)bs.size(), 0.15))) return st;
+      ms = std::min(ms, (clock() - t0) * 1e3);
+    }
     return LA3_OK;
   };
   auto batch = [&](la3_query_kind kind) {
