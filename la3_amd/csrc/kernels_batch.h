@@ -1049,7 +1049,27 @@ __device__ __forceinline__ void mppr_combine_body(const DG& g, Part p, const Ppr
     float s[QW], c[QW];
 #pragma unroll
     for (int w = 0; w < QW; w++) { s[w] = 0.f; c[w] = 0.f; }
-    for (uint32_t sg = a.seg_first[i]; sg < a.seg_first[i + 1]; sg++) {
+    // the partials of four segments loaded before their adds (in order)
+    const uint32_t f1 = a.seg_first[i + 1];
+    uint32_t sg = a.seg_first[i];
+    for (; sg + 4 <= f1; sg += 4) {
+#pragma unroll
+      for (int w = 0; w < QW; w++) {
+        if (64 * w + lane >= a.k) continue;
+        float ps[4], pc[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+          ps[u] = a.part[(size_t)(sg + u) * 2 * K + 64 * w + lane];
+          pc[u] = a.part[(size_t)(sg + u) * 2 * K + K + 64 * w + lane];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; u++) {
+          kahan_add(s[w], c[w], ps[u]);
+          kahan_add(s[w], c[w], -pc[u]);
+        }
+      }
+    }
+    for (; sg < f1; sg++) {
 #pragma unroll
       for (int w = 0; w < QW; w++) {
         if (64 * w + lane >= a.k) continue;
@@ -1199,7 +1219,23 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_sub_combine(DG g, PprBatchArgs a
     const uint32_t r = a.long_ppr[i];
     if (r < r0 || r >= r1 || !live) continue;
     float s = 0.f, c = 0.f;
-    for (uint32_t sg = a.seg_first[i]; sg < a.seg_first[i + 1]; sg++) {
+    // the partials of eight segments loaded before their adds (in order)
+    const uint32_t f1 = a.seg_first[i + 1];
+    uint32_t sg = a.seg_first[i];
+    for (; sg + 8 <= f1; sg += 8) {
+      float ps[8], pc[8];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; u++) {
+        ps[u] = a.part[(size_t)(sg + u) * 2 * LPR + q];
+        pc[u] = a.part[(size_t)(sg + u) * 2 * LPR + LPR + q];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 8; u++) {
+        kahan_add(s, c, ps[u]);
+        kahan_add(s, c, -pc[u]);
+      }
+    }
+    for (; sg < f1; sg++) {
       kahan_add(s, c, a.part[(size_t)sg * 2 * LPR + q]);
       kahan_add(s, c, -a.part[(size_t)sg * 2 * LPR + LPR + q]);
     }
