@@ -734,7 +734,17 @@ __global__ void __launch_bounds__(BLOCK) k_pr_combine(DG g, PrArgs a, uint32_t p
   for (uint32_t i = p0 + blockIdx.x * BLOCK + threadIdx.x; i < p1; i += gridDim.x * BLOCK) {
     const uint32_t r = g.plong[i];
     double y = MODE == 1 ? a.srcsum[r] : 0.0;
-    for (uint32_t sg = g.pseg_first[i]; sg < g.pseg_first[i + 1]; sg++) y = __dadd_rn(y, part[sg]);
+    // eight partials loaded before their adds (in segment order)
+    const uint32_t f1 = g.pseg_first[i + 1];
+    uint32_t sg = g.pseg_first[i];
+    for (; sg + 8 <= f1; sg += 8) {
+      double v[8];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; u++) v[u] = part[sg + u];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; u++) y = __dadd_rn(y, v[u]);
+    }
+    for (; sg < f1; sg++) y = __dadd_rn(y, part[sg]);
     pr_finish<MODE>(g, a, r, y);
   }
 }
