@@ -693,8 +693,16 @@ __global__ void k_mssp_sub_sinks(DG g, uint32_t* dist) {
     if (base + grp >= g.nsink) continue;
     const uint32_t r = g.nreg + (uint32_t)base + grp;
     uint32_t y = SSSP_INF;
-    for (uint32_t e = g.row_ptr[r]; e < g.row_ptr[r + 1]; e++)
-      y = min(y, dist[(size_t)g.row_cols[e] * LPR + q] + g.row_w[e]);
+    const uint32_t e1 = g.row_ptr[r + 1];
+    uint32_t e = g.row_ptr[r];
+    for (; e + 8 <= e1; e += 8) {  // eight in-neighbours in flight (min: any order)
+      uint32_t v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = dist[(size_t)g.row_cols[e + u] * LPR + q] + g.row_w[e + u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) y = min(y, v[u]);
+    }
+    for (; e < e1; e++) y = min(y, dist[(size_t)g.row_cols[e] * LPR + q] + g.row_w[e]);
     uint32_t* pd = &dist[(size_t)r * LPR + q];
     *pd = min(*pd, y);
   }
@@ -745,7 +753,22 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_sinks(DG g, uint32_t* dist) {
     uint32_t y[QW];
 #pragma unroll
     for (int j = 0; j < QW; j++) y[j] = SSSP_INF;
-    for (uint32_t e = g.row_ptr[r]; e < g.row_ptr[r + 1]; e++) {
+    const uint32_t e1 = g.row_ptr[r + 1];
+    uint32_t e = g.row_ptr[r];
+    for (; e + 8 <= e1; e += 8) {  // eight in-neighbours' rows in flight (min: any order)
+      uint32_t src[8], w[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) { src[u] = g.row_cols[e + u]; w[u] = g.row_w[e + u]; }
+#pragma unroll
+      for (int j = 0; j < QW; j++) {
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = dist[(size_t)src[u] * K + 64 * j + lane];
+#pragma unroll
+        for (int u = 0; u < 8; u++) y[j] = min(y[j], v[u] + w[u]);
+      }
+    }
+    for (; e < e1; e++) {
       const uint32_t src = g.row_cols[e], w = g.row_w[e];
 #pragma unroll
       for (int j = 0; j < QW; j++) y[j] = min(y[j], dist[(size_t)src * K + 64 * j + lane] + w);
