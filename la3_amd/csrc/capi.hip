@@ -1501,13 +1501,27 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   while (fc.items > 0) {
     const Frontier& nf = F.f[cur ^ 1];
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
-    int ev;
-    if constexpr (SUB)
-      ev = launch(c, "mssp_push", k_mssp_sub_push<LPR>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dist,
-                  (const uint64_t*)cm, nm, thr, T, F.f[cur].nitems);
-    else
-      ev = launch(c, "mssp_push", k_mssp_push<QW>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dist,
-                  (const uint64_t*)cm, nm, thr, T, F.f[cur].nitems);
+    if constexpr (SUB) {
+      // Narrow rows: the push queues each improved row once and emits its
+      // work items into nf itself (no touched list, no items pass, one
+      // counter read per step).
+      const int ev = launch(c, "mssp_push", k_mssp_sub_push<LPR>, blocks, d, (const uint64_t*)F.f[cur].items,
+                            fc.items, dist, (const uint64_t*)cm, nm, thr, nf, F.f[cur].nitems);
+      launch(c, "mssp_clear", k_clear_items<QW>, blocks_for(fc.items), (const uint64_t*)F.f[cur].items, fc.items,
+             cm);
+      std::swap(cm, nm);
+      FrontCounts nc;
+      if ((st = read_counts(c, nf, nc))) return st;
+      annotate(c, ev, (8.0 + 4.0 * K) * fc.edges + (8.0 + 4.0 * K) * fc.cols + (K / 8.0 + 4) * nc.verts, fc.edges);
+      c->edges_scanned += fc.edges;
+      thr = next_threshold(thr, delta, nc.nmin);
+      round++;
+      fc = nc;
+      cur ^= 1;
+      continue;
+    }
+    const int ev = launch(c, "mssp_push", k_mssp_push<QW>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items,
+                          dist, (const uint64_t*)cm, nm, thr, T, F.f[cur].nitems);
     FrontCounts tc;
     if ((st = read_counts(c, T, tc))) return st;
     // The touched list becomes the next frontier's work items; clearing T's

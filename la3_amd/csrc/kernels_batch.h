@@ -566,7 +566,7 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items
 template <int LPR>
 __device__ __forceinline__ void mssp_sub_push_body(const DG& g, Part p, Emitter& em, const uint64_t* items,
                                                    uint32_t nitems, uint32_t* dist, const uint64_t* cm,
-                                                   uint64_t* nm, uint32_t T, const Frontier& touched) {
+                                                   uint64_t* nm, uint32_t T, const Frontier& next) {
   constexpr uint32_t EPP = WAVE / LPR;       // entries per pass
   constexpr uint32_t U = LPR >= 4 ? 4 : LPR;  // passes in flight (U * EPP <= 64)
   constexpr uint64_t QMASK = LPR == 64 ? ~0ull : ((1ull << LPR) - 1);
@@ -605,7 +605,10 @@ __device__ __forceinline__ void mssp_sub_push_body(const DG& g, Part p, Emitter&
         if (lane == 0) old = atomicOr((unsigned long long*)&nm[s], (unsigned long long)fb);
         requeue = __ballot(lane == 0 && old == 0) != 0;
       }
-      em_push_one(em, touched, s, requeue);
+      if (requeue) {  // s's work items go straight into the next work list
+        const uint32_t ne = g.col_split[s] - g.col_ptr[s];
+        em_push(em, next, s, lane == 0 ? (ne + CHUNK - 1) / CHUNK : 0u, lane == 0 ? ne : 0u, lane == 0);
+      }
       if ((__ballot(act) & QMASK) == 0) continue;  // nothing near: no column scan
       for (uint32_t e0 = beg; e0 < end; e0 += WAVE) {  // wave-uniform
         if (e0 != beg) {
@@ -650,25 +653,33 @@ __device__ __forceinline__ void mssp_sub_push_body(const DG& g, Part p, Emitter&
             }
           }
           // The lane whose atomicOr finds the row's word zero queues the row
-          // (its first improvement in this step), so the touched list holds
-          // each row once.
+          // (its first improvement in this step), so each row is queued
+          // once: its work items (the regular-row part of its column, a
+          // row's column index being its slot) go straight into the next
+          // work list, with no touched list and no items pass.
           bool any = false;
           if (vbits) any = atomicOr((unsigned long long*)&nm[vrow], (unsigned long long)vbits) == 0;
-          em_push_lanes(em, touched, vrow, any);
+          if (__ballot(any)) {  // wave-uniform
+            uint32_t ne = 0;
+            if (any) ne = g.col_split[vrow] - g.col_ptr[vrow];
+            em_push(em, next, vrow, (ne + CHUNK - 1) / CHUNK, ne, any);
+          }
         }
       }
     }
   }
 }
 
+// next: the next step's work list (its counters, the smallest queued
+// distance included, are read by the host after the launch).
 template <int LPR>
 __global__ void __launch_bounds__(BLOCK) k_mssp_sub_push(DG g, const uint64_t* items, uint32_t nitems,
                                                          uint32_t* dist, const uint64_t* cm, uint64_t* nm,
-                                                         uint32_t T, Frontier touched, uint32_t* zc) {
+                                                         uint32_t T, Frontier next, uint32_t* zc) {
   zero_ctr(zc);
   LA3_EMITTER(em);
-  mssp_sub_push_body<LPR>(g, grid_part(), em, items, nitems, dist, cm, nm, T, touched);
-  em_finish(em, touched);
+  mssp_sub_push_body<LPR>(g, grid_part(), em, items, nitems, dist, cm, nm, T, next);
+  em_finish(em, next);
 }
 
 // Roots: dist[slot * K + q] = 0 and the query's mask bit (K lanes per row).
