@@ -1501,12 +1501,17 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   while (fc.items > 0) {
     const Frontier& nf = F.f[cur ^ 1];
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
-    if constexpr (SUB) {
-      // Narrow rows: the push queues each improved row once and emits its
-      // work items into nf itself (no touched list, no items pass, one
-      // counter read per step).
-      const int ev = launch(c, "mssp_push", k_mssp_sub_push<LPR>, blocks, d, (const uint64_t*)F.f[cur].items,
-                            fc.items, dist, (const uint64_t*)cm, nm, thr, nf, F.f[cur].nitems);
+    if constexpr (SUB || QW == 1) {
+      // One mask word per row: the push queues each improved row once and
+      // emits its work items into nf itself (no touched list, no items pass,
+      // one counter read per step).
+      int ev;
+      if constexpr (SUB)
+        ev = launch(c, "mssp_push", k_mssp_sub_push<LPR>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items,
+                    dist, (const uint64_t*)cm, nm, thr, nf, F.f[cur].nitems);
+      else
+        ev = launch(c, "mssp_push", k_mssp_push<QW>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dist,
+                    (const uint64_t*)cm, nm, thr, nf, F.f[cur].nitems);
       launch(c, "mssp_clear", k_clear_items<QW>, blocks_for(fc.items), (const uint64_t*)F.f[cur].items, fc.items,
              cm);
       std::swap(cm, nm);

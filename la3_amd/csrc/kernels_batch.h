@@ -415,10 +415,13 @@ __device__ __forceinline__ uint64_t rdlane64(uint64_t v, uint32_t l) {
 // together, instead of behind a chain of dependent loads (item, column
 // pointers, mask, distances, entries), and the entry indices live in one
 // register, not U.
-template <int QW>
+// DIRECT (one mask word per row only): queued rows' work items go straight
+// into `touched`, which is then the next step's work list (no items pass).
+template <int QW, bool DIRECT = false>
 __device__ __forceinline__ void mssp_push_body(const DG& g, Part p, Emitter& em, const uint64_t* items,
                                                uint32_t nitems, uint32_t* dist, const uint64_t* cm,
                                                uint64_t* nm, uint32_t T, const Frontier& touched) {
+  static_assert(!DIRECT || QW == 1, "direct emission needs one mask word per row");
   constexpr uint32_t K = 64 * QW;
   // Edges in groups of U: the U distance rows are loaded before any is
   // used, so a wavefront keeps U row loads in flight.
@@ -472,7 +475,14 @@ __device__ __forceinline__ void mssp_push_body(const DG& g, Part p, Emitter& em,
         }
       }
       // far pairs of this vertex: queue it again
-      em_push_one(em, touched, s, requeue);
+      if (DIRECT) {
+        if (requeue) {
+          const uint32_t ne = g.col_split[s] - g.col_ptr[s];
+          em_push(em, touched, s, lane == 0 ? (ne + CHUNK - 1) / CHUNK : 0u, lane == 0 ? ne : 0u, lane == 0);
+        }
+      } else {
+        em_push_one(em, touched, s, requeue);
+      }
       if (__ballot(any_near) == 0) continue;  // nothing near: no column scan
       bool actw[QW];  // wave-uniform: some lane relaxes in word j
 #pragma unroll
@@ -539,7 +549,15 @@ __device__ __forceinline__ void mssp_push_body(const DG& g, Part p, Emitter& em,
             if (QW == 1) any = atomicOr(w, (unsigned long long)vbits[j]) == 0;
             else { atomicOr(w, (unsigned long long)vbits[j]); any = true; }
           }
-          em_push_lanes(em, touched, vrow, any);
+          if (DIRECT) {
+            if (__ballot(any)) {  // wave-uniform; a regular row's column index is its slot
+              uint32_t ne = 0;
+              if (any) ne = g.col_split[vrow] - g.col_ptr[vrow];
+              em_push(em, touched, vrow, (ne + CHUNK - 1) / CHUNK, ne, any);
+            }
+          } else {
+            em_push_lanes(em, touched, vrow, any);
+          }
         }
       }
     }
@@ -552,7 +570,7 @@ __global__ void __launch_bounds__(BLOCK) k_mssp_push(DG g, const uint64_t* items
                                                      uint32_t T, Frontier touched, uint32_t* zc) {
   zero_ctr(zc);
   LA3_EMITTER(em);
-  mssp_push_body<QW>(g, grid_part(), em, items, nitems, dist, cm, nm, T, touched);
+  mssp_push_body<QW, QW == 1>(g, grid_part(), em, items, nitems, dist, cm, nm, T, touched);
   em_finish(em, touched);
 }
 
