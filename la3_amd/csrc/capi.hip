@@ -48,7 +48,6 @@ struct la3_ctx {
   la3_timing last{};
   uint64_t edges_scanned = 0;
   std::map<std::string, DevBuf> scratch;
-  std::vector<void*> retired;  // blocks of grown scratch, freed on release / destroy
   void* host_ctr = nullptr;  // pinned, for frontier counters
   int ncu = 256;
   int refs = 1;              // the caller's reference plus one per live graph
@@ -104,7 +103,6 @@ void ctx_release(la3_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (auto& kv : c->scratch) if (kv.second.p) (void)hipFree(kv.second.p);
-    for (void* q : c->retired) (void)hipFree(q);
     for (auto& e : c->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
     for (auto& kv : c->fz_exec) (void)hipGraphExecDestroy(kv.second);
@@ -200,38 +198,21 @@ void timing_end(la3_ctx* c) {
 }
 
 // Grow-only named device scratch.
-// Grow-only device scratch by name. A grown buffer's old block is kept until
-// the context releases its scratch or is destroyed: hipFree waits for the
-// whole device, which would stall every other context's stream (the serving
-// loop's workers) at each growth. Growth takes at least twice the old size,
-// so a buffer grows a few times at most.
 void* scratch(la3_ctx* c, const char* name, size_t bytes) {
   DevBuf& b = c->scratch[name];
   if (b.bytes >= bytes) return b.p;
-  const size_t need = std::max<size_t>(bytes, 256);
-  const size_t want = b.bytes ? std::max(need, 2 * b.bytes) : need;
-  void* p = nullptr;
-  size_t got = want;
-  if (hipMalloc(&p, want) != hipSuccess) {  // no room for the headroom: the size asked
-    (void)hipGetLastError();
-    p = nullptr;
-    got = need;
-    if (want == need || hipMalloc(&p, need) != hipSuccess) {
-      (void)hipGetLastError();
-      return nullptr;
-    }
-  }
-  if (b.p) c->retired.push_back(b.p);
-  b.p = p;
-  b.bytes = got;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+  size_t want = std::max<size_t>(bytes, 256);
+  if (hipMalloc(&b.p, want) != hipSuccess) { b.p = nullptr; return nullptr; }
+  b.bytes = want;
   return b.p;
 }
 
 // Frees the cached scratch whose names start with `prefix` (all with "").
 void scratch_release(la3_ctx* c, const std::string& prefix) {
   (void)hipStreamSynchronize(c->stream);
-  for (void* q : c->retired) (void)hipFree(q);
-  c->retired.clear();
   for (auto it = c->scratch.begin(); it != c->scratch.end();) {
     if (it->first.compare(0, prefix.size(), prefix) == 0) {
       if (it->second.p) (void)hipFree(it->second.p);
