@@ -16,7 +16,7 @@
  *  - The library owns all device memory; the caller owns every host buffer.
  *  - Per-vertex outputs are indexed by ORIGINAL vertex id and have
  *    la3_graph_info.nslots entries (ids 0..n, the one-rank tile of
- *    src/matrix/matrix2d.hpp:66-70 with n/1+1 slots).
+ *    src/matrix/matrix2d.hpp:6-8 with n/1+1 slots).
  *  - One host thread per context at a time (calls on a context serialize on its
  *    stream); results do not depend on batch composition.
  */
@@ -33,14 +33,14 @@ typedef enum {
   LA3_OK = 0,
   LA3_EINVAL = 1,   /* bad argument: null pointer, id out of range, wrong graph mode */
   LA3_ENOMEM = 2,   /* device or host allocation failed, or admission rejected */
-  LA3_EIO = 3,      /* file could not be opened / read / stat'ed (graph.hpp:186-203) */
+  LA3_EIO = 3,      /* file could not be opened / read / stat'ed (graph.hpp:69-98) */
   LA3_EDEVICE = 4,  /* a HIP runtime call failed */
   LA3_ENOTSUP = 5,  /* size beyond what this build supports (e.g. nnz >= 2^32) */
   LA3_EAGAIN = 6    /* serving: no completed query yet */
 } la3_status;
 
 /* Load modes: the flag combinations of Graph<W>::load_directed(reverse_edges,
- * remove_cycles) and load_undirected (src/matrix/graph.hpp:342-364). */
+ * remove_cycles) and load_undirected (src/matrix/graph.hpp:224-246). */
 typedef enum {
   LA3_DIRECTED = 0,          /* load_directed(reverse=false): stores A^T, messages flow a->b */
   LA3_REVERSED = 1,          /* load_directed(reverse=true):  messages flow b->a */
@@ -58,7 +58,7 @@ typedef struct la3_graph la3_graph;
 typedef struct {
   uint32_t n;          /* nvertices as passed (or read from the header) */
   uint32_t nslots;     /* n + 1 */
-  uint32_t nregular;   /* vertex classes (src/matrix/processed_matrix2d.hpp:292-317) */
+  uint32_t nregular;   /* vertex classes (src/matrix/processed_matrix2d.hpp:182-229) */
   uint32_t nsink;
   uint32_t nsource;
   uint32_t nisolated;
@@ -96,7 +96,7 @@ la3_status la3_ctx_release_scratch(la3_ctx* ctx);
 la3_status la3_ctx_set_ingest(la3_ctx* ctx, int on_device);
 
 /* -------------------------------------------------------------- graphs --- */
-/* Graph<W>::load_binary (src/matrix/graph.hpp:155-332) at np=1: reads the LA3
+/* Graph<W>::load_binary (src/matrix/graph.hpp:38-214) at np=1: reads the LA3
  * binary file (u32 pairs, or u32 triples when weighted != 0), drops
  * self-loops, orients per `mode`, hashes ids, deduplicates with the first
  * record winning, classifies vertices and builds the device-resident DCSC and
@@ -108,7 +108,7 @@ la3_status la3_graph_load_file(la3_ctx* ctx, const char* path, uint32_t nvertice
 la3_status la3_graph_load_records(la3_ctx* ctx, const uint32_t* records, uint64_t nrecords,
                                   uint32_t nvertices, int weighted, la3_mode mode,
                                   la3_hashing hashing, la3_graph** out);
-/* Graph<W>::free (src/matrix/graph.hpp:146-152). */
+/* Graph<W>::free (src/matrix/graph.hpp:28-34). */
 void la3_graph_free(la3_graph* g);
 la3_status la3_graph_info_get(const la3_graph* g, la3_graph_info* out);
 /* Layout export for parity checks: hid_of[nslots] maps a vertex-state slot
@@ -135,7 +135,7 @@ typedef enum {
 } la3_array;
 la3_status la3_graph_array(const la3_graph* g, la3_array which, uint32_t* out, uint64_t cap,
                            uint64_t* count);
-/* hasher->hash / unhash (src/matrix/hashers.h:432-444). */
+/* hasher->hash / unhash (src/matrix/hashers.h:55-67). */
 uint32_t la3_graph_hash(const la3_graph* g, uint32_t v);
 uint32_t la3_graph_unhash(const la3_graph* g, uint32_t v);
 

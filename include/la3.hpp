@@ -94,7 +94,7 @@ class Graph {
 
  private:
   void load(bool binary, const std::string& path, uint32_t n, la3_mode m, Hashing h) {
-    if (!binary) throw Error(LA3_ENOTSUP, "text input is not implemented (as in graph.hpp:335-339)");
+    if (!binary) throw Error(LA3_ENOTSUP, "text input is not implemented (as in graph.hpp:216-221)");
     free();
     ctx_->check(la3_graph_load_file(ctx_->get(), path.c_str(), n, weighted, m, (la3_hashing)h, &g_),
                 "graph_load_file");

@@ -797,7 +797,7 @@ la3_status la3_graph_load_file(la3_ctx* ctx, const char* path, uint32_t nvertice
   uint64_t off = 0;
   uint32_t n = nvertices;
   if (n == 0) {
-    // 16-byte header {u32 nrows-1, u32 ncols-1, u64 nnz} (graph.hpp:207-234).
+    // 16-byte header {u32 nrows-1, u32 ncols-1, u64 nnz} (graph.hpp:89-117).
     if (size < 16) { if (base) munmap((void*)base, size); return fail(ctx, LA3_EIO, "fread() failure: short header"); }
     uint32_t hr, hc;
     std::memcpy(&hr, base, 4);

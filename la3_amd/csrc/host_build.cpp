@@ -3,11 +3,11 @@
 // the output is identical for any thread count.
 //
 // Semantics follow the reference at one rank:
-//   record filtering / orientation / hashing   src/matrix/graph.hpp:280-309
-//   vertex classes                            src/matrix/processed_matrix2d.hpp:292-317
+//   record filtering / orientation / hashing   src/matrix/graph.hpp:151-191
+//   vertex classes                            src/matrix/processed_matrix2d.hpp:182-229
 //   slot order (dashboard locator)            src/utils/locator.h:84-120
-//   dedup, first record wins                  src/matrix/csc_matrix2d.hpp:265-277 with
-//                                             unordered_set::insert (src/utils/common.h:141)
+//   dedup, first record wins                  src/matrix/csc_matrix2d.hpp:60-72 with
+//                                             unordered_set::insert (src/matrix/csc_matrix2d.hpp:62-72)
 //   DCSC columns in col-group locator order,  src/utils/csc.h:94-125
 //   entries by row position
 #include <algorithm>
@@ -186,7 +186,7 @@ std::string build_layout(const uint32_t* rec, uint64_t nrec, uint32_t n, bool we
   const uint64_t m = kept[T] * per;
   if (m >= 0xffffffffull) return "more than 2^32-1 stored entries";
 
-  // 2. Orient and hash (graph.hpp:283-309), preserving record order.
+  // 2. Orient and hash (graph.hpp:165-191), preserving record order.
   std::vector<uint32_t> R(m), C(m), Wt(weighted ? m : 0);
   parallel_chunks(nrec, [&](int t, uint64_t b, uint64_t e) {
     uint64_t o = kept[t] * per;

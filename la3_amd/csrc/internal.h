@@ -13,7 +13,7 @@ constexpr uint32_t SSSP_INF = 0x7fffffffu;   // sssp.h:14 (UINT32_MAX / 2)
 constexpr uint32_t LONG_ROW = 64;            // rows longer than this get a whole wavefront
 constexpr uint32_t CHUNK = 256;              // entries per traversal work item
 
-// SimpleBucketHasher(n, nranks = 1) (src/matrix/hashers.h:416-445).
+// SimpleBucketHasher(n, nranks = 1) (src/matrix/hashers.h:39-67).
 struct Hasher {
   bool bucket = true;
   int64_t nparts = 128, height = 0, max_range = 0;

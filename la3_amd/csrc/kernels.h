@@ -1,6 +1,6 @@
 // Device kernels for the vertex-program path on gfx950 (wave64, CDNA4).
 //
-// Data layout in HBM (one graph = one rank's tile set, src/matrix/matrix2d.hpp:66-70
+// Data layout in HBM (one graph = one rank's tile set, src/matrix/matrix2d.hpp:6-8
 // at np=1): every per-vertex array is indexed by SLOT, the reference's
 // vertex-state order [regular | sink | source | isolated]
 // (src/utils/locator.h:84-120). Non-empty columns c are numbered

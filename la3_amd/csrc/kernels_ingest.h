@@ -1,7 +1,7 @@
 // Device ingest: LA3 binary records -> the graph layout of kernels.h, built
 // on the GPU. Same result, byte for byte, as the host builder
-// (host_build.cpp, which restates src/matrix/graph.hpp:280-309,
-// processed_matrix2d.hpp:292-317, locator.h:84-120, csc_matrix2d.hpp:265-277
+// (host_build.cpp, which restates src/matrix/graph.hpp:151-191,
+// processed_matrix2d.hpp:182-229, locator.h:84-120, csc_matrix2d.hpp:60-72
 // and csc.h:94-125):
 //   1. drop self-loops, orient (directed: row = destination; acyclic: row =
 //      larger id, or smaller when reversed), hash, undirected both ways, in

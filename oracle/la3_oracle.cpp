@@ -6,15 +6,15 @@
 // cpu_baseline leg use it, and only as the checker.
 //
 // What it restates (reference paths are relative to the LA3 tree):
-//   ingest            src/matrix/graph.hpp:155-332   (self-loop drop, transpose,
+//   ingest            src/matrix/graph.hpp:38-214   (self-loop drop, transpose,
 //                                                     reverse, acyclic orientation,
 //                                                     bucket hashing, mirroring)
-//   hashing           src/matrix/hashers.h:416-445   (SimpleBucketHasher, nranks=1)
+//   hashing           src/matrix/hashers.h:39-67   (SimpleBucketHasher, nranks=1)
 //   tiling            src/matrix/matrix2d.hpp:6-41   (np=1: one (n+1)x(n+1) tile)
-//   vertex classes    src/matrix/processed_matrix2d.hpp:292-317
+//   vertex classes    src/matrix/processed_matrix2d.hpp:182-229
 //   locators          src/utils/locator.h:84-155 and
 //                     src/matrix/processed_matrix2d_details.hpp:156-216
-//   DCSC build        src/matrix/csc_matrix2d.hpp:224-294, src/utils/csc.h:76-130
+//   DCSC build        src/matrix/csc_matrix2d.hpp:20-88, src/utils/csc.h:76-130
 //                     (first-inserted-wins dedup, regular/sink split, entries
 //                      sorted by global_idx)
 //   initialize        src/vprogram/vertex_program.hpp:84-174, 417-508
@@ -29,12 +29,26 @@
 // PPR is not in the reference. It is defined here as PR with a per-query
 // teleport vector (alpha at the seed, 0 elsewhere), same iteration structure.
 //
-// Pinning: tests/test_oracle.py checks every function here against the g1
-// known answers (tests/golden/g1_kat.json), textbook BFS / Dijkstra /
-// union-find / brute-force triangle counts on random graphs, a dense-matrix
-// PageRank formulation with the post-loop sink step, and degree counts.
-// The DCSC/class layout (ora_layout) is pinned by the class KATs of g1 and by
-// brute-force recomputation of classes and per-column row sets.
+// Pinning (every function here is checked against answers that do not come
+// from this file):
+//   hashing            hand-worked hashed ids for n = 8, 300, 5000, 2^16+3 and
+//                      the bucket construction over every id, inverse and
+//                      identity above max_range (tests/test_oracle_layout.py)
+//   classes, slot order, both DCSCs (colptrs, colidxs, global_idx, idx,
+//                      first-record weights)   brute-force recomputation from
+//                      the records for all five load modes, bucket and no
+//                      hashing, ids 0 and >= max_range (test_oracle_layout.py)
+//   programs           g1 known answers (tests/golden/g1_kat.json); scipy BFS,
+//                      Dijkstra and connected components, brute-force and trace
+//                      triangle counts, degree counts, a dense PR/PPR iteration
+//                      with the post-loop sink step, BFS parents recomputed
+//                      from classes and the bucket construction
+//                      (tests/test_oracle.py)
+//   execute(max_iters) hop-bounded BFS levels and Bellman-Ford distances,
+//                      m-round min-label propagation and its round count,
+//                      single-iteration PR (tests/test_oracle_bounded.py)
+// Not pinned: nothing on the np = 1 path; the reference's own tests hold no
+// vectors (it has none), so these closed forms are the pins.
 
 #include <algorithm>
 #include <cmath>
@@ -46,7 +60,7 @@
 namespace {
 
 // ---------------------------------------------------------------- hashing ---
-// hashers.h:416-445. nbuckets = nranks = 1 here.
+// hashers.h:39-67. nbuckets = nranks = 1 here.
 struct Hasher {
   bool bucket;
   int64_t nparts = 128, height = 0, max_range = 0;
@@ -120,13 +134,13 @@ void build_csc(OCSC& c, uint32_t ncols, const std::vector<Rec>& triples, const O
               [](const Entry& a, const Entry& b) { return a.global_idx < b.global_idx; });
 }
 
-// graph.hpp:155-332 (np=1), processed_matrix2d*.hpp, csc_matrix2d.hpp.
+// graph.hpp:38-214 (np=1), processed_matrix2d*.hpp, csc_matrix2d.hpp.
 // Returns nullptr if an id exceeds n (the reference writes out of bounds there).
 OGraph* build_graph(const uint32_t* data, uint64_t nrec, uint32_t n, bool weighted, int mode,
                     bool bucket) {
   auto* G = new OGraph();
   G->n = n;
-  G->nslots = n + 1;  // matrix2d.hpp:69 tile_height = n/1 + 1
+  G->nslots = n + 1;  // matrix2d.hpp:8 tile_height = n/1 + 1
   G->weighted = weighted;
   G->h = Hasher(n, bucket);
   const bool directed = mode != UNDIRECTED;
@@ -140,9 +154,9 @@ OGraph* build_graph(const uint32_t* data, uint64_t nrec, uint32_t n, bool weight
   for (uint64_t k = 0; k < nrec; k++) {
     Rec t{data[k * rw], data[k * rw + 1], weighted ? data[k * rw + 2] : 0u};
     if (t.row > n || t.col > n) { delete G; return nullptr; }
-    if (t.row == t.col) continue;                      // graph.hpp:284
-    if (directed && !reverse) std::swap(t.row, t.col);  // graph.hpp:289
-    if (acyclic) {                                      // graph.hpp:292-297
+    if (t.row == t.col) continue;                      // graph.hpp:165-167
+    if (directed && !reverse) std::swap(t.row, t.col);  // graph.hpp:171-172
+    if (acyclic) {                                      // graph.hpp:174-179
       if ((!reverse && t.col > t.row) || (reverse && t.col < t.row)) std::swap(t.row, t.col);
     }
     t.row = (uint32_t)G->h.hash(t.row);
@@ -151,7 +165,7 @@ OGraph* build_graph(const uint32_t* data, uint64_t nrec, uint32_t n, bool weight
     if (!directed) { std::swap(t.row, t.col); tri.push_back(t); }
   }
 
-  // Classes over the stored matrix (processed_matrix2d.hpp:292-317).
+  // Classes over the stored matrix (processed_matrix2d.hpp:182-229).
   const uint32_t S = G->nslots;
   std::vector<uint8_t> rows(S, 0), cols(S, 0);
   for (auto& t : tri) { rows[t.row] = 1; cols[t.col] = 1; }
@@ -288,7 +302,7 @@ struct BfsProg {
   }
 };
 
-// sssp.h:11-46 (IntegerWrapper<INF> default-constructs to INF, common.h:215-236)
+// sssp.h:11-46 (IntegerWrapper<INF> default-constructs to INF, common.h:106-127)
 constexpr uint32_t SSSP_INF = UINT32_MAX / 2;
 struct SsspProg {
   using M = uint32_t; using A = uint32_t;
@@ -617,13 +631,14 @@ int ora_ppr(void* g, void* gr, uint32_t seed, uint32_t iters, double alpha, doub
 
 // bfs.cpp:11-37 (on whatever load mode g was built with). parent is the
 // hashed id of the last in-neighbour processed, as the reference stores it.
-int ora_bfs(void* h, uint32_t root, uint8_t* hops, uint32_t* parent, uint32_t* iters_run) {
+int ora_bfs(void* h, uint32_t root, uint32_t max_iters, uint8_t* hops, uint32_t* parent,
+            uint32_t* iters_run) {
   const OGraph& G = *((Handle*)h)->G;
   BfsProg p;
   p.root = root;
   Engine<BfsProg> e(p, G);
   e.initialize();
-  e.execute(0);
+  e.execute(max_iters);
   for (uint32_t i = 0; i < G.nslots; i++) {
     uint32_t id = e.vid_of(i);
     hops[id] = e.v[i].hops;
@@ -634,25 +649,25 @@ int ora_bfs(void* h, uint32_t root, uint8_t* hops, uint32_t* parent, uint32_t* i
 }
 
 // sssp.cpp:11-37
-int ora_sssp(void* h, uint32_t root, uint32_t* dist, uint32_t* iters_run) {
+int ora_sssp(void* h, uint32_t root, uint32_t max_iters, uint32_t* dist, uint32_t* iters_run) {
   const OGraph& G = *((Handle*)h)->G;
   SsspProg p;
   p.root = root;
   Engine<SsspProg> e(p, G);
   e.initialize();
-  e.execute(0);
+  e.execute(max_iters);
   for (uint32_t i = 0; i < G.nslots; i++) dist[e.vid_of(i)] = e.v[i].dist;
   if (iters_run) *iters_run = e.iters_run;
   return 0;
 }
 
 // cc.cpp:11-31
-int ora_cc(void* h, uint32_t* label, uint32_t* iters_run) {
+int ora_cc(void* h, uint32_t max_iters, uint32_t* label, uint32_t* iters_run) {
   const OGraph& G = *((Handle*)h)->G;
   CcProg p;
   Engine<CcProg> e(p, G);
   e.initialize();
-  e.execute(0);
+  e.execute(max_iters);
   for (uint32_t i = 0; i < G.nslots; i++) label[e.vid_of(i)] = e.v[i].label;
   if (iters_run) *iters_run = e.iters_run;
   return 0;

@@ -45,9 +45,9 @@ def lib():
         L.ora_degree.argtypes = [P, P]
         L.ora_pr.argtypes = [P, P, u32, P, P, P]
         L.ora_ppr.argtypes = [P, P, u32, u32, ctypes.c_double, P, P]
-        L.ora_bfs.argtypes = [P, u32, P, P, P]
-        L.ora_sssp.argtypes = [P, u32, P, P]
-        L.ora_cc.argtypes = [P, P, P]
+        L.ora_bfs.argtypes = [P, u32, u32, P, P, P]
+        L.ora_sssp.argtypes = [P, u32, u32, P, P]
+        L.ora_cc.argtypes = [P, u32, P, P]
         L.ora_tc.argtypes = [P, P, P, P]
         _lib = L
     return _lib
@@ -125,25 +125,27 @@ def ppr(g, gr, seed, iters, alpha=0.15):
     return rank, int(it[0])
 
 
-def bfs(g, root):
+def bfs(g, root, max_iters=0):
+    """BfsVertex execute(max_iters): 0 runs until convergence, 1 is
+    execute_single, m > 1 stops after m iterations."""
     hops = np.zeros(g.nslots, dtype=np.uint8)
     parent = np.zeros(g.nslots, dtype=np.uint32)
     it = np.zeros(1, dtype=np.uint32)
-    lib().ora_bfs(g.h, int(root), _ptr(hops), _ptr(parent), _ptr(it))
+    lib().ora_bfs(g.h, int(root), int(max_iters), _ptr(hops), _ptr(parent), _ptr(it))
     return hops, parent, int(it[0])
 
 
-def sssp(g, root):
+def sssp(g, root, max_iters=0):
     dist = np.zeros(g.nslots, dtype=np.uint32)
     it = np.zeros(1, dtype=np.uint32)
-    lib().ora_sssp(g.h, int(root), _ptr(dist), _ptr(it))
+    lib().ora_sssp(g.h, int(root), int(max_iters), _ptr(dist), _ptr(it))
     return dist, int(it[0])
 
 
-def cc(g):
+def cc(g, max_iters=0):
     label = np.zeros(g.nslots, dtype=np.uint32)
     it = np.zeros(1, dtype=np.uint32)
-    lib().ora_cc(g.h, _ptr(label), _ptr(it))
+    lib().ora_cc(g.h, int(max_iters), _ptr(label), _ptr(it))
     return label, int(it[0])
 
 

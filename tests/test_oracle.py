@@ -118,11 +118,12 @@ def test_bfs_matches_textbook(oracle, seed):
             assert it == 1 + (int(hops[reg_reached].max()) if reg_reached.any() else 0)
 
 
-def _message_order_keys(rec, n, g):
+def _message_order_keys(rec, n):
     """Reference per-row message order of each sender (sources first, then
-    regular, each by increasing hashed id), computed from classes alone."""
+    regular, each by increasing hashed id), computed from classes and the
+    bucket construction of the hash alone (no oracle code)."""
     cls = CF.classes(rec, n)
-    hashed = np.array([g.hash(v) for v in range(n + 1)])
+    hashed = CF.bucket_hash_table(n)
     key = np.full(n + 1, -1, dtype=np.int64)
     src = np.flatnonzero(cls["source"])
     reg = np.flatnonzero(cls["regular"])
@@ -138,7 +139,7 @@ def test_bfs_parent_is_last_writer(oracle, seed):
     n = 5000
     rec = random_records(seed, n, 30000)
     g = oracle.OracleGraph(rec, n, False, oracle.DIRECTED)
-    key, hashed = _message_order_keys(rec, n, g)
+    key, hashed = _message_order_keys(rec, n)
     d = CF.dedup_first(rec)
     for root in (1, 3):
         hops, parent, _ = oracle.bfs(g, root)
