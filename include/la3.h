@@ -135,6 +135,48 @@ typedef enum {
 } la3_array;
 la3_status la3_graph_array(const la3_graph* g, la3_array which, uint32_t* out, uint64_t cap,
                            uint64_t* count);
+/* Device view of a graph's layout, for device code compiled outside the
+ * library (the VertexProgram template of include/la3_vertex_program.hpp):
+ * the arrays the library's kernels read, by vertex slot (order [regular |
+ * sink | source | isolated], each by increasing hashed id,
+ * src/utils/locator.h:84-120). Columns c = [regular | source]; column c's
+ * sender slot is c (regular) or c + nsink (source). Rows are slots
+ * [0, nreg + nsink). col_ptr/col_split/col_rows: the DCSC, entries of a
+ * column by row slot, [col_ptr, col_split) the regular rows (the reference's
+ * `csc`), [col_split, col_ptr + 1) the sink rows (`sink_csc`,
+ * src/utils/csc.h:61-130). row_ptr/row_split/row_cols: the row-major
+ * companion, a row's senders in the reference's message order (source
+ * columns, then regular columns, each ascending,
+ * src/vprogram/vertex_program_execute.hpp:464-483); row_split marks the
+ * first regular sender. id_of / hid_of: original / hashed id of a slot;
+ * slot_of_id: slot of an original id. long_rows: the rows with more than 64
+ * entries. The pointers stay valid until la3_graph_free. No counterpart in
+ * the reference (its CSC<W> arrays are host memory). */
+typedef struct {
+  int device;
+  uint32_t nslots, nreg, nsink, nsrc, ncols, nrows;
+  uint64_t nnz;
+  uint32_t directed, weighted;
+  const uint32_t* col_ptr;
+  const uint32_t* col_split;
+  const uint32_t* col_rows;
+  const uint32_t* col_w;      /* NULL when unweighted */
+  const uint32_t* row_ptr;
+  const uint32_t* row_split;
+  const uint32_t* row_cols;
+  const uint32_t* row_w;      /* NULL when unweighted */
+  const uint32_t* id_of;
+  const uint32_t* hid_of;
+  const uint32_t* slot_of_id;
+  const uint32_t* long_rows;
+  uint32_t nlong;
+} la3_device_view;
+la3_status la3_graph_device_view(const la3_graph* g, la3_device_view* out);
+/* The context's HIP stream (a hipStream_t), on which every call on the
+ * context runs; device code launched by the caller on it is ordered with
+ * those calls. NULL for a host-only context. */
+void* la3_ctx_stream(const la3_ctx* ctx);
+
 /* hasher->hash / unhash (src/matrix/hashers.h:55-67). */
 uint32_t la3_graph_hash(const la3_graph* g, uint32_t v);
 uint32_t la3_graph_unhash(const la3_graph* g, uint32_t v);
@@ -153,29 +195,55 @@ la3_status la3_degree(la3_graph* g, uint32_t* degree);
 la3_status la3_pagerank(la3_graph* g, uint32_t iters, double* rank, uint32_t* degree,
                         uint32_t* iters_run);
 
-/* BfsVertex (bfs.h:27-49), execute() until convergence, on any load mode
- * (bfs.cpp:14 uses UNDIRECTED). hops: u8 level, 255 = unreached. parent (may
- * be NULL): the hashed id of the in-neighbour whose message the reference
- * combines last (last writer wins, bfs.h:40), 0 for the root and unreached. */
-la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
+/* Every traversal takes the reference's max_iters
+ * (VertexProgram::execute, src/vprogram/vertex_program_execute.hpp:11-52):
+ * 0 runs until an iteration activates no regular vertex; 1 is
+ * execute_single (one step for regular and sink rows from the initial
+ * messages); m > 1 runs exactly m iterations (and iters_run reports m even if
+ * the traversal settled earlier, as the reference's loop does). */
+
+/* BfsVertex (bfs.h:27-49), execute(max_iters), on any load mode
+ * (bfs.cpp:14 uses UNDIRECTED). hops: u8 level, 255 = unreached (levels
+ * above max_iters stay 255). parent (may be NULL): the hashed id of the
+ * in-neighbour whose message the reference combines last (last writer wins,
+ * bfs.h:40), 0 for the root and unreached. */
+la3_status la3_bfs(la3_graph* g, uint32_t root, uint32_t max_iters, uint8_t* hops, uint32_t* parent,
                    uint32_t* iters_run);
 
-/* SpVertex (sssp.h:26-46) on a weighted DIRECTED load, execute() until
- * convergence. dist: u32, 2^31-1 for unreachable. */
-la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters_run);
+/* SpVertex (sssp.h:26-46) on a weighted DIRECTED load, execute(max_iters).
+ * dist: u32, 2^31-1 for unreachable. With max_iters = m > 1 the regular
+ * vertices hold the shortest distances over paths of at most m edges and the
+ * sinks one edge more (the post-loop sink step); with max_iters = 1 every
+ * row is one edge from the root. Bounded runs relax in the reference's
+ * synchronous order (each iteration reads the distances the previous one
+ * left); unbounded runs relax asynchronously (same distances, fewer device
+ * rounds), and iters_run then counts device rounds, not the reference's
+ * iterations. */
+la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t max_iters, uint32_t* dist, uint32_t* iters_run);
 
 /* The same two programs on another context's stream and scratch (same
  * device as the graph's): concurrent callers, such as the serving loop's
  * workers, use one context each. No counterpart in the reference. */
-la3_status la3_bfs_ctx(la3_ctx* ctx, la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
-                       uint32_t* iters_run);
-la3_status la3_sssp_ctx(la3_ctx* ctx, la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters_run);
+la3_status la3_bfs_ctx(la3_ctx* ctx, la3_graph* g, uint32_t root, uint32_t max_iters, uint8_t* hops,
+                       uint32_t* parent, uint32_t* iters_run);
+la3_status la3_sssp_ctx(la3_ctx* ctx, la3_graph* g, uint32_t root, uint32_t max_iters, uint32_t* dist,
+                        uint32_t* iters_run);
 
-/* CcVertex (cc.h:22-38) on an UNDIRECTED load: min original id per connected
- * component (label 0 is the reference's "empty" accumulator, cc.h:31; a
- * graph whose vertex 0 has edges is outside the reference's defined
- * behaviour). */
-la3_status la3_cc(la3_graph* g, uint32_t* label, uint32_t* iters_run);
+/* CcVertex (cc.h:22-38), execute(max_iters), run by the vertex-program
+ * engine of include/la3_vertex_program.hpp with the reference's operations:
+ * each row combines its messages in the reference's order with cc.h:31's
+ * rule (an accumulator of 0 counts as empty), so labels and iters_run equal
+ * the reference's on any load, including graphs where vertex 0 has edges
+ * (its label 0 then survives only where it is the last message a row
+ * combines). On an UNDIRECTED load with vertex 0 isolated, the labels are
+ * the smallest original id of each connected component. */
+la3_status la3_cc(la3_graph* g, uint32_t max_iters, uint32_t* label, uint32_t* iters_run);
+
+/* Connected components by union-find (no counterpart in the reference):
+ * label = smallest original id in the component, the fixed point CcVertex
+ * reaches when vertex 0 is isolated; UNDIRECTED loads only. Faster than
+ * la3_cc (no iterations); no iteration count. */
+la3_status la3_components(la3_graph* g, uint32_t* label);
 
 /* GnVertex + CtVertex (tc.h:47-99, tc.cpp:11-52) on an ACYCLIC load: total
  * triangle count (u64) and, if per_vertex != NULL, the count each vertex holds
@@ -190,11 +258,14 @@ la3_status la3_ppr(la3_graph* g, uint32_t seed, uint32_t iters, double alpha, do
 /* ------------------------------------------------------- batched queries --- */
 /* k BFS roots in one traversal with k-bit frontiers (k <= 1024). hops is
  * k x nslots, query-major: hops[q * nslots + id]. Identical per query to
- * la3_bfs levels. */
-la3_status la3_bfs_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint8_t* hops,
+ * la3_bfs levels with the same max_iters. */
+la3_status la3_bfs_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint32_t max_iters, uint8_t* hops,
                          uint32_t* iters_run);
-/* k SSSP roots with k-wide u32 distances (k <= 1024); dist[q * nslots + id]. */
-la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint32_t* dist,
+/* k SSSP roots with k-wide u32 distances (k <= 1024); dist[q * nslots + id].
+ * Identical per query to la3_sssp with the same max_iters; a bounded batch
+ * (max_iters > 0) runs its queries one after another in the synchronous
+ * order. */
+la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint32_t max_iters, uint32_t* dist,
                           uint32_t* iters_run);
 /* k PPR seeds (k <= 1024), `iters` iterations, FP32 with compensation: each
  * vertex and query carries its rank (f32) and the rank's compensation term
@@ -211,9 +282,9 @@ la3_status la3_ppr_batch(la3_graph* g, const uint32_t* seeds, uint32_t k, uint32
  * host threads (each with its own context on the same device) can run
  * batches on one resident graph concurrently. */
 la3_status la3_bfs_batch_ctx(la3_ctx* ctx, la3_graph* g, const uint32_t* roots, uint32_t k,
-                             uint8_t* hops, uint32_t* iters_run);
+                             uint32_t max_iters, uint8_t* hops, uint32_t* iters_run);
 la3_status la3_sssp_batch_ctx(la3_ctx* ctx, la3_graph* g, const uint32_t* roots, uint32_t k,
-                              uint32_t* dist, uint32_t* iters_run);
+                              uint32_t max_iters, uint32_t* dist, uint32_t* iters_run);
 la3_status la3_ppr_batch_ctx(la3_ctx* ctx, la3_graph* g, const uint32_t* seeds, uint32_t k,
                              uint32_t iters, double alpha, float* rank, float* rank_comp,
                              double* residual);
@@ -258,9 +329,13 @@ la3_status la3_batch_run(la3_ctx* ctx, la3_batch* batches, uint32_t nbatches, do
  * session at a time (LA3_EAGAIN otherwise); a session has 64 task slots of
  * 256 queries (la3_session_add returns LA3_EAGAIN when the batch does not
  * fit). Each batch's results are those of la3_batch_run for it alone. A
- * task slot keeps its device buffers for the next task up to 256 MB and
- * frees larger ones when its task finishes (la3_batch_run keeps them all
- * until la3_ctx_release_scratch). */
+ * task slot keeps its device buffers for the next task up to 256 MB (the
+ * environment variable LA3_SESSION_KEEP_BYTES, read at session creation,
+ * changes the limit) and frees larger ones when its task finishes
+ * (la3_batch_run keeps them all until la3_ctx_release_scratch). A finished
+ * batch whose `out` was NULL leaves its rows as the context's last result
+ * only while its slot's buffers are kept: once they are freed,
+ * la3_result_fetch and la3_result_topk return LA3_EINVAL. */
 typedef struct la3_session la3_session;
 la3_status la3_session_create(la3_ctx* ctx, double ppr_alpha, la3_session** out);
 la3_status la3_session_add(la3_session* s, const la3_batch* batch, uint64_t* id);
@@ -401,7 +476,9 @@ la3_status la3_kernel_stats_reset(la3_ctx* ctx);
 /* Every program's main per-vertex output (hops, dist, rank, label, ...) may be
  * NULL: the result then stays in device memory, and la3_result_fetch copies
  * the last program's main output (as laid out for the host buffer) into
- * `dst`. Device-resident results let a caller time or chain programs without
+ * `dst`. The result lives in the context's scratch: it is gone (LA3_EINVAL)
+ * after la3_ctx_release_scratch or when a later call frees or regrows that
+ * buffer. Device-resident results let a caller time or chain programs without
  * the PCIe transfer. */
 la3_status la3_result_fetch(la3_ctx* ctx, void* dst, uint64_t bytes);
 

@@ -6,8 +6,11 @@
 //       (src/matrix/graph.h:38-109)
 //   DegVertex, PrVertex, BfsVertex, SpVertex, CcVertex, GnVertex, CtVertex
 //       (src/apps/graph_analytics/*.h) with initialize(), initialize(other),
-//       execute(max_iters), reduce(mapper, reducer) and a public `root` where
-//       the reference has one (src/vprogram/vertex_program.h:158-202).
+//       execute(max_iters), reset(), free(), reduce(mapper, reducer), topk and
+//       a public `root` where the reference has one
+//       (src/vprogram/vertex_program.h:158-214).
+// Programs with other operations are written against la3::VertexProgram<P>
+// in la3_vertex_program.hpp (HIP device code).
 // States are exposed per vertex with the reference's field names (degree,
 // rank, parent, hops, distance, label, ntriangles). Errors throw la3::Error
 // (the reference exits or asserts; a C++ caller can catch here).
@@ -117,6 +120,12 @@ class ProgramBase {
   const std::vector<S>& states() const { return states_; }
   const S& state(uint32_t vid) const { return states_.at(vid); }
 
+  // reset() (vertex_program.hpp:51-63): the next execute() starts from
+  // initialize() again — every execute() here does. free() (:38-48)
+  // releases the states.
+  void reset() {}
+  void free() { std::vector<S>().swap(states_); }
+
   template <class Value, class Mapper, class Reducer>
   Value reduce(Mapper map, Reducer red, bool /*active_only*/ = false) const {
     Value acc = Value();
@@ -159,7 +168,10 @@ class DegVertex : public ProgramBase<Graph<W>, DegState> {
  public:
   DegVertex(const Graph<W>* G, bool /*stationary*/ = true) : ProgramBase<Graph<W>, DegState>(G) {}
   void initialize() {}
-  void execute(uint32_t /*max_iters*/ = 1) {
+  // Every max_iters >= 1 gives the same degrees; 0 (until convergence)
+  // never ends in the reference, whose apply always activates.
+  void execute(uint32_t max_iters = 1) {
+    if (max_iters == 0) throw Error(LA3_EINVAL, "DegVertex::execute(0) does not converge");
     std::vector<uint32_t> d(this->nslots());
     this->ctx().check(la3_degree(this->G_->handle(), d.data()), "degree");
     this->states_.assign(d.size(), DegState());
@@ -195,10 +207,10 @@ class BfsVertex : public ProgramBase<Graph<Empty>, BfsState> {
   BfsVertex(const Graph<Empty>* G, bool = false) : ProgramBase(G) {}
   uint32_t root = 0;
   void initialize() {}
-  void execute(uint32_t = 0) {
+  void execute(uint32_t max_iters = 0) {
     std::vector<uint8_t> h(nslots());
     std::vector<uint32_t> p(nslots());
-    ctx().check(la3_bfs(G_->handle(), root, h.data(), p.data(), &iters_), "bfs");
+    ctx().check(la3_bfs(G_->handle(), root, max_iters, h.data(), p.data(), &iters_), "bfs");
     states_.assign(h.size(), BfsState());
     for (size_t i = 0; i < h.size(); i++) { states_[i].hops = h[i]; states_[i].parent = p[i]; }
   }
@@ -214,12 +226,16 @@ class SpVertex : public ProgramBase<Graph<uint32_t>, SpState> {
   SpVertex(const Graph<uint32_t>* G, bool = false) : ProgramBase(G) {}
   uint32_t root = 0;
   void initialize() {}
-  void execute(uint32_t = 0) {
+  void execute(uint32_t max_iters = 0) {
     std::vector<uint32_t> d(nslots());
-    ctx().check(la3_sssp(G_->handle(), root, d.data(), nullptr), "sssp");
+    ctx().check(la3_sssp(G_->handle(), root, max_iters, d.data(), &iters_), "sssp");
     states_.assign(d.size(), SpState());
     for (size_t i = 0; i < d.size(); i++) states_[i].distance = d[i];
   }
+  uint32_t iterations() const { return iters_; }
+
+ private:
+  uint32_t iters_ = 0;
 };
 
 // CcVertex (cc.h:22-38).
@@ -227,12 +243,16 @@ class CcVertex : public ProgramBase<Graph<Empty>, CcState> {
  public:
   CcVertex(const Graph<Empty>* G, bool = false) : ProgramBase(G) {}
   void initialize() {}
-  void execute(uint32_t = 0) {
+  void execute(uint32_t max_iters = 0) {
     std::vector<uint32_t> l(nslots());
-    ctx().check(la3_cc(G_->handle(), l.data(), nullptr), "cc");
+    ctx().check(la3_cc(G_->handle(), max_iters, l.data(), &iters_), "cc");
     states_.assign(l.size(), CcState());
     for (size_t i = 0; i < l.size(); i++) states_[i].label = l[i];
   }
+  uint32_t iterations() const { return iters_; }
+
+ private:
+  uint32_t iters_ = 0;
 };
 
 // GnVertex + CtVertex (tc.h:47-99): the pair runs as one masked SpGEMM on the
@@ -316,9 +336,10 @@ class BfsBatchVertex : public BatchProgramBase<Graph<Empty>, B, uint8_t> {
  public:
   BfsBatchVertex(const Graph<Empty>* G, bool = false) : BatchProgramBase<Graph<Empty>, B, uint8_t>(G) {}
   void initialize() {}
-  void execute(uint32_t = 0) {
+  void execute(uint32_t max_iters = 0) {
     std::vector<uint8_t> h((size_t)B * this->nslots());
-    this->ctx().check(la3_bfs_batch(this->G_->handle(), this->roots, B, h.data(), &this->iters_), "bfs_batch");
+    this->ctx().check(la3_bfs_batch(this->G_->handle(), this->roots, B, max_iters, h.data(), &this->iters_),
+                      "bfs_batch");
     this->fill(h);
   }
 };
@@ -329,9 +350,9 @@ class SpBatchVertex : public BatchProgramBase<Graph<uint32_t>, B, uint32_t> {
  public:
   SpBatchVertex(const Graph<uint32_t>* G, bool = false) : BatchProgramBase<Graph<uint32_t>, B, uint32_t>(G) {}
   void initialize() {}
-  void execute(uint32_t = 0) {
+  void execute(uint32_t max_iters = 0) {
     std::vector<uint32_t> d((size_t)B * this->nslots());
-    this->ctx().check(la3_sssp_batch(this->G_->handle(), this->roots, B, d.data(), &this->iters_),
+    this->ctx().check(la3_sssp_batch(this->G_->handle(), this->roots, B, max_iters, d.data(), &this->iters_),
                       "sssp_batch");
     this->fill(d);
   }

@@ -38,7 +38,8 @@ EXPORTS = [
     "la3_server_destroy", "la3_graph_set_tuning", "la3_graph_get_tuning", "la3_autotune",
     "la3_batch_run", "la3_session_create", "la3_session_add", "la3_session_step",
     "la3_session_poll", "la3_session_free_slots", "la3_session_destroy", "la3_ctx_release_scratch",
-    "la3_result_topk", "la3_ctx_set_ingest", "la3_graph_array",
+    "la3_result_topk", "la3_ctx_set_ingest", "la3_graph_array", "la3_components",
+    "la3_graph_device_view", "la3_ctx_stream",
 ]
 (TUNE_SSSP_DELTA, TUNE_BATCH_SSSP_DELTA, TUNE_BFS_TASK_K, TUNE_SSSP_TASK_K, TUNE_PPR_TASK_K,
  TUNE_FUSED_BLOCKS) = range(6)
@@ -126,13 +127,16 @@ def lib():
             "la3_graph_unhash": (u32, [P, u32]),
             "la3_degree": (i32, [P, P]),
             "la3_pagerank": (i32, [P, u32, P, P, P]),
-            "la3_bfs": (i32, [P, u32, P, P, P]),
-            "la3_sssp": (i32, [P, u32, P, P]),
-            "la3_cc": (i32, [P, P, P]),
+            "la3_bfs": (i32, [P, u32, u32, P, P, P]),
+            "la3_sssp": (i32, [P, u32, u32, P, P]),
+            "la3_cc": (i32, [P, u32, P, P]),
+            "la3_components": (i32, [P, P]),
+            "la3_graph_device_view": (i32, [P, P]),
+            "la3_ctx_stream": (P, [P]),
             "la3_tc": (i32, [P, P, P]),
             "la3_ppr": (i32, [P, u32, u32, ctypes.c_double, P]),
-            "la3_bfs_batch": (i32, [P, P, u32, P, P]),
-            "la3_sssp_batch": (i32, [P, P, u32, P, P]),
+            "la3_bfs_batch": (i32, [P, P, u32, u32, P, P]),
+            "la3_sssp_batch": (i32, [P, P, u32, u32, P, P]),
             "la3_ppr_batch": (i32, [P, P, u32, u32, ctypes.c_double, P, P, P]),
             "la3_last_timing": (i32, [P, ctypes.POINTER(Timing)]),
             "la3_set_profiling": (i32, [P, i32]),
@@ -142,10 +146,10 @@ def lib():
             "la3_host_free": (None, [P]),
             "la3_copy_bandwidth": (i32, [P, u64, u32, ctypes.POINTER(ctypes.c_double)]),
             "la3_result_fetch": (i32, [P, P, u64]),
-            "la3_bfs_ctx": (i32, [P, P, u32, P, P, P]),
-            "la3_sssp_ctx": (i32, [P, P, u32, P, P]),
-            "la3_bfs_batch_ctx": (i32, [P, P, P, u32, P, P]),
-            "la3_sssp_batch_ctx": (i32, [P, P, P, u32, P, P]),
+            "la3_bfs_ctx": (i32, [P, P, u32, u32, P, P, P]),
+            "la3_sssp_ctx": (i32, [P, P, u32, u32, P, P]),
+            "la3_bfs_batch_ctx": (i32, [P, P, P, u32, u32, P, P]),
+            "la3_sssp_batch_ctx": (i32, [P, P, P, u32, u32, P, P]),
             "la3_ppr_batch_ctx": (i32, [P, P, P, u32, u32, ctypes.c_double, P, P, P]),
             "la3_server_create": (i32, [i32, ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]),
             "la3_server_add_graph": (i32, [P, P, ctypes.POINTER(u32)]),
@@ -440,26 +444,36 @@ def ppr(g, seed, iters, alpha=0.15, out=None):
     return rank
 
 
-def bfs(g, root, parents=True, out=None):
+def bfs(g, root, parents=True, out=None, max_iters=0):
+    """BfsVertex execute(max_iters): (hops, parents or None, iterations)."""
     hops = _out(out, (g.nslots,), np.uint8)
     par = np.zeros(g.nslots, dtype=np.uint32) if parents else None
     it = np.zeros(1, dtype=np.uint32)
-    g.ctx.check(lib().la3_bfs(g.h, int(root), _ptr(hops), _ptr(par), _ptr(it)), "bfs")
+    g.ctx.check(lib().la3_bfs(g.h, int(root), int(max_iters), _ptr(hops), _ptr(par), _ptr(it)), "bfs")
     return hops, par, int(it[0])
 
 
-def sssp(g, root, out=None):
+def sssp(g, root, out=None, max_iters=0):
+    """SpVertex execute(max_iters): (distances, iterations)."""
     dist = _out(out, (g.nslots,), np.uint32)
     it = np.zeros(1, dtype=np.uint32)
-    g.ctx.check(lib().la3_sssp(g.h, int(root), _ptr(dist), _ptr(it)), "sssp")
+    g.ctx.check(lib().la3_sssp(g.h, int(root), int(max_iters), _ptr(dist), _ptr(it)), "sssp")
     return dist, int(it[0])
 
 
-def cc(g, out=None):
+def cc(g, out=None, max_iters=0):
+    """CcVertex execute(max_iters) on the vertex-program engine: (labels, iterations)."""
     label = _out(out, (g.nslots,), np.uint32)
     it = np.zeros(1, dtype=np.uint32)
-    g.ctx.check(lib().la3_cc(g.h, _ptr(label), _ptr(it)), "cc")
+    g.ctx.check(lib().la3_cc(g.h, int(max_iters), _ptr(label), _ptr(it)), "cc")
     return label, int(it[0])
+
+
+def components(g, out=None):
+    """Union-find components (min original id per component), undirected loads."""
+    label = _out(out, (g.nslots,), np.uint32)
+    g.ctx.check(lib().la3_components(g.h, _ptr(label)), "components")
+    return label
 
 
 def tc(g, per_vertex=True):
@@ -469,20 +483,20 @@ def tc(g, per_vertex=True):
     return int(total[0]), per
 
 
-def bfs_batch(g, roots, out=None):
+def bfs_batch(g, roots, out=None, max_iters=0):
     roots = np.ascontiguousarray(roots, dtype=np.uint32)
     hops = _out(out, (roots.size, g.nslots), np.uint8)
     it = np.zeros(1, dtype=np.uint32)
-    g.ctx.check(lib().la3_bfs_batch(g.h, _ptr(roots), roots.size, _ptr(hops), _ptr(it)),
+    g.ctx.check(lib().la3_bfs_batch(g.h, _ptr(roots), roots.size, int(max_iters), _ptr(hops), _ptr(it)),
                 "bfs_batch")
     return hops, int(it[0])
 
 
-def sssp_batch(g, roots, out=None):
+def sssp_batch(g, roots, out=None, max_iters=0):
     roots = np.ascontiguousarray(roots, dtype=np.uint32)
     dist = _out(out, (roots.size, g.nslots), np.uint32)
     it = np.zeros(1, dtype=np.uint32)
-    g.ctx.check(lib().la3_sssp_batch(g.h, _ptr(roots), roots.size, _ptr(dist), _ptr(it)),
+    g.ctx.check(lib().la3_sssp_batch(g.h, _ptr(roots), roots.size, int(max_iters), _ptr(dist), _ptr(it)),
                 "sssp_batch")
     return dist, int(it[0])
 

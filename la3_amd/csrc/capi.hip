@@ -5,10 +5,12 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -18,6 +20,7 @@
 #include <unistd.h>
 
 #include "../../include/la3.h"
+#include "../../include/la3_vertex_program.hpp"
 #include "internal.h"
 #include "kernels_dev.h"
 #include "kernels_batch.h"
@@ -28,6 +31,99 @@
 #include <hipcub/hipcub.hpp>
 
 using namespace la3;
+
+// ------------------------------------------------------- device blocks ---
+// Every device block the library allocates (context scratch, graph arrays,
+// ingest temporaries) is registered, with its size and owner, from hipMalloc
+// to hipFree. Every host <-> device copy and memset checks its device range
+// against the registry first: a range that is not inside one live block (a
+// stale pointer to a freed or regrown buffer) fails with LA3_EDEVICE and a
+// message naming the range, instead of reaching the runtime, which treats an
+// unknown pointer as host memory. Contexts and threads share one registry.
+namespace {
+
+struct BlockInfo {
+  size_t bytes;
+  const char* owner;  // static string
+};
+std::mutex g_blocks_mu;
+std::map<uintptr_t, BlockInfo> g_blocks;
+thread_local std::string g_copy_err;
+
+hipError_t dev_alloc(void** p, size_t bytes, const char* owner) {
+  *p = nullptr;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return e;
+  }
+  std::lock_guard<std::mutex> lk(g_blocks_mu);
+  g_blocks[(uintptr_t)*p] = BlockInfo{bytes, owner};
+  return hipSuccess;
+}
+
+void dev_free(void* p) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_blocks_mu);
+    g_blocks.erase((uintptr_t)p);
+  }
+  (void)hipFree(p);
+}
+
+// True iff [p, p + bytes) lies inside one live block.
+bool dev_covers(const void* p, size_t bytes) {
+  const uintptr_t a = (uintptr_t)p;
+  std::lock_guard<std::mutex> lk(g_blocks_mu);
+  auto it = g_blocks.upper_bound(a);
+  if (it == g_blocks.begin()) return false;
+  --it;
+  return a >= it->first && a - it->first <= it->second.bytes && bytes <= it->second.bytes - (a - it->first);
+}
+
+hipError_t range_error(const void* p, size_t bytes, const char* what) {
+  char buf[160];
+  snprintf(buf, sizeof(buf), "%s: device range [%p, +%zu) is not inside a live device block", what, p, bytes);
+  g_copy_err = buf;
+  return hipErrorInvalidValue;
+}
+
+// hipMemcpy(Async) with the device side(s) checked against the registry.
+// Zero-byte copies are no-ops.
+hipError_t dcopy_async(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s) {
+  if (bytes == 0) return hipSuccess;
+  if ((kind == hipMemcpyDeviceToHost || kind == hipMemcpyDeviceToDevice) && !dev_covers(src, bytes))
+    return range_error(src, bytes, "copy source");
+  if ((kind == hipMemcpyHostToDevice || kind == hipMemcpyDeviceToDevice) && !dev_covers(dst, bytes))
+    return range_error(dst, bytes, "copy destination");
+  return hipMemcpyAsync(dst, src, bytes, kind, s);
+}
+
+hipError_t dcopy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  if (bytes == 0) return hipSuccess;
+  if ((kind == hipMemcpyDeviceToHost || kind == hipMemcpyDeviceToDevice) && !dev_covers(src, bytes))
+    return range_error(src, bytes, "copy source");
+  if ((kind == hipMemcpyHostToDevice || kind == hipMemcpyDeviceToDevice) && !dev_covers(dst, bytes))
+    return range_error(dst, bytes, "copy destination");
+  return hipMemcpy(dst, src, bytes, kind);
+}
+
+hipError_t dset_async(void* dst, int value, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return hipSuccess;
+  if (!dev_covers(dst, bytes)) return range_error(dst, bytes, "memset");
+  return hipMemsetAsync(dst, value, bytes, s);
+}
+
+std::string hip_error_text(hipError_t e) {
+  std::string m = hipGetErrorString(e);
+  if (!g_copy_err.empty()) {
+    m = g_copy_err;
+    g_copy_err.clear();
+  }
+  return m;
+}
+
+}  // namespace
 
 // ------------------------------------------------------------------ ctx ---
 struct DevBuf {
@@ -102,7 +198,7 @@ void ctx_release(la3_ctx* c) {
   if (c->device >= 0) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    for (auto& kv : c->scratch) if (kv.second.p) (void)hipFree(kv.second.p);
+    for (auto& kv : c->scratch) dev_free(kv.second.p);
     for (auto& e : c->evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
     for (auto& kv : c->fz_exec) (void)hipGraphExecDestroy(kv.second);
@@ -121,7 +217,7 @@ la3_status fail(la3_ctx* c, la3_status s, const std::string& msg) {
   do {                                                                                   \
     hipError_t e_ = (call);                                                              \
     if (e_ != hipSuccess)                                                                \
-      return fail((ctx), LA3_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+      return fail((ctx), LA3_EDEVICE, std::string(#call) + ": " + hip_error_text(e_));     \
   } while (0)
 
 hipEvent_t get_event(la3_ctx* c) {
@@ -197,15 +293,32 @@ void timing_end(la3_ctx* c) {
   c->last = t;
 }
 
+// A device-resident result (la3_result_fetch / la3_result_topk) stops being
+// one when the block holding it is freed: forget it then.
+void forget_result_in(la3_ctx* c, const void* p, size_t bytes) {
+  const uintptr_t r = (uintptr_t)c->last_result, a = (uintptr_t)p;
+  if (c->last_result && r >= a && r - a < bytes) {
+    c->last_result = nullptr;
+    c->last_result_bytes = 0;
+  }
+}
+
 // Grow-only named device scratch.
 void* scratch(la3_ctx* c, const char* name, size_t bytes) {
   DevBuf& b = c->scratch[name];
   if (b.bytes >= bytes) return b.p;
-  if (b.p) (void)hipFree(b.p);
+  if (b.p) {
+    forget_result_in(c, b.p, b.bytes);
+    dev_free(b.p);
+  }
   b.p = nullptr;
   b.bytes = 0;
   size_t want = std::max<size_t>(bytes, 256);
-  if (hipMalloc(&b.p, want) != hipSuccess) { b.p = nullptr; return nullptr; }
+  if (dev_alloc(&b.p, want, "context scratch") != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    return nullptr;
+  }
   b.bytes = want;
   return b.p;
 }
@@ -215,7 +328,10 @@ void scratch_release(la3_ctx* c, const std::string& prefix) {
   (void)hipStreamSynchronize(c->stream);
   for (auto it = c->scratch.begin(); it != c->scratch.end();) {
     if (it->first.compare(0, prefix.size(), prefix) == 0) {
-      if (it->second.p) (void)hipFree(it->second.p);
+      if (it->second.p) {
+        forget_result_in(c, it->second.p, it->second.bytes);
+        dev_free(it->second.p);
+      }
       it = c->scratch.erase(it);
     } else {
       ++it;
@@ -227,10 +343,10 @@ template <class T>
 la3_status upload(la3_graph* g, const std::vector<T>& v, const T** out) {
   void* p = nullptr;
   size_t bytes = std::max<size_t>(v.size() * sizeof(T), 4);
-  HIPC(g->ctx, hipMalloc(&p, bytes));
+  HIPC(g->ctx, dev_alloc(&p, bytes, "graph array"));
   g->allocs.push_back(p);
   g->device_bytes += bytes;
-  if (!v.empty()) HIPC(g->ctx, hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (!v.empty()) HIPC(g->ctx, dcopy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   *out = (const T*)p;
   return LA3_OK;
 }
@@ -269,12 +385,12 @@ la3_status graph_from_layout(la3_ctx* ctx, HostLayout& L, la3_graph** out) {
       (s = up(L.long_rows, &d.long_rows)) || (s = up(L.slot_of_id, &d.slot_of_id)) ||
       (s = up(L.plong, &d.plong)) || (s = up(L.pseg_first, &d.pseg_first)) ||
       (s = up(L.pseg_row, &d.pseg_row)) || (s = up(L.pseg_beg, &d.pseg_beg))) {
-    for (void* p : g->allocs) (void)hipFree(p);
+    for (void* p : g->allocs) dev_free(p);
     return s;
   }
   if (L.weighted) {
     if ((s = up(L.col_w, &d.col_w)) || (s = up(L.row_w, &d.row_w))) {
-      for (void* p : g->allocs) (void)hipFree(p);
+      for (void* p : g->allocs) dev_free(p);
       return s;
     }
   }
@@ -290,11 +406,14 @@ la3_status graph_from_layout(la3_ctx* ctx, HostLayout& L, la3_graph** out) {
 // Device buffers of one ingest, freed together unless kept for the graph.
 struct DevTemps {
   std::vector<void*> ptrs;
-  ~DevTemps() { for (void* p : ptrs) (void)hipFree(p); }
+  ~DevTemps() { for (void* p : ptrs) dev_free(p); }
   template <class T>
   T* get(size_t count) {
     void* p = nullptr;
-    if (hipMalloc(&p, std::max<size_t>(count * sizeof(T), 4)) != hipSuccess) return nullptr;
+    if (dev_alloc(&p, std::max<size_t>(count * sizeof(T), 4), "graph build") != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
     ptrs.push_back(p);
     return (T*)p;
   }
@@ -345,8 +464,8 @@ la3_status ingest_device(la3_ctx* c, const uint32_t* rec, uint64_t nrec, uint32_
     out = 0;
     if (cnt == 0) return LA3_OK;
     uint32_t a = 0, b = 0;
-    HIPC(c, hipMemcpyAsync(&a, in + cnt - 1, 4, hipMemcpyDeviceToHost, st));
-    HIPC(c, hipMemcpyAsync(&b, ex + cnt - 1, 4, hipMemcpyDeviceToHost, st));
+    HIPC(c, dcopy_async(&a, in + cnt - 1, 4, hipMemcpyDeviceToHost, st));
+    HIPC(c, dcopy_async(&b, ex + cnt - 1, 4, hipMemcpyDeviceToHost, st));
     HIPC(c, hipStreamSynchronize(st));
     out = (uint64_t)a + b;
     return LA3_OK;
@@ -358,15 +477,15 @@ la3_status ingest_device(la3_ctx* c, const uint32_t* rec, uint64_t nrec, uint32_
   uint32_t* pos = T.get<uint32_t>(nrec);
   uint32_t* bad = T.get<uint32_t>(1);
   if (!drec || !keep || !pos || !bad) return fail(c, LA3_ENOMEM, oom);
-  if (nrec) HIPC(c, hipMemcpyAsync(drec, rec, 4 * nrec * RW, hipMemcpyHostToDevice, st));
-  HIPC(c, hipMemsetAsync(bad, 0, 4, st));
+  if (nrec) HIPC(c, dcopy_async(drec, rec, 4 * nrec * RW, hipMemcpyHostToDevice, st));
+  HIPC(c, dset_async(bad, 0, 4, st));
   launch(c, "ingest", k_ing_keep, blocks_for(nrec), (const uint32_t*)drec, nrec, RW, n, directed ? 1u : 2u, keep, bad);
   if (!exclusive_sum(keep, pos, nrec)) return fail(c, LA3_EDEVICE, "device ingest: scan failed");
   uint64_t m = 0;
   la3_status e;
   if ((e = total(keep, pos, nrec, m))) return e;
   uint32_t hbad = 0;
-  HIPC(c, hipMemcpy(&hbad, bad, 4, hipMemcpyDeviceToHost));
+  HIPC(c, dcopy(&hbad, bad, 4, hipMemcpyDeviceToHost));
   if (hbad) return fail(c, LA3_EINVAL, "vertex id out of range [0, nvertices]");
   if (m >= 0xffffffffull) return fail(c, LA3_ENOTSUP, "more than 2^32-1 stored entries");
   uint32_t* R = T.get<uint32_t>(m);
@@ -385,8 +504,8 @@ la3_status ingest_device(la3_ctx* c, const uint32_t* rec, uint64_t nrec, uint32_
   uint32_t* id_of = T.get<uint32_t>(S);
   uint32_t* slot_of_id = T.get<uint32_t>(S);
   if (!rowf || !colf || !cls || !rank || !slot || !hid_of || !id_of || !slot_of_id) return fail(c, LA3_ENOMEM, oom);
-  HIPC(c, hipMemsetAsync(rowf, 0, S, st));
-  HIPC(c, hipMemsetAsync(colf, 0, S, st));
+  HIPC(c, dset_async(rowf, 0, S, st));
+  HIPC(c, dset_async(colf, 0, S, st));
   launch(c, "ingest", k_ing_occupy, blocks_for(m), (const uint32_t*)R, (const uint32_t*)C, m, rowf, colf);
   launch(c, "ingest", k_ing_class, blocks_for(S), (const uint8_t*)rowf, (const uint8_t*)colf, S, cls);
   uint64_t cnt[4];
@@ -428,7 +547,7 @@ la3_status ingest_device(la3_ctx* c, const uint32_t* rec, uint64_t nrec, uint32_
   if (!col_rows || (weighted && !col_w) || !col_of || !col_ptr || !col_split) return fail(c, LA3_ENOMEM, oom);
   launch(c, "ingest", k_ing_compact, blocks_for(m), (const uint64_t*)kb.Current(), (const uint32_t*)vb.Current(),
          (const uint32_t*)first, (const uint32_t*)upos, m, b, (const uint32_t*)W, col_rows, col_w, col_of);
-  HIPC(c, hipMemsetAsync(col_ptr, 0, 4ull * (NC + 1), st));
+  HIPC(c, dset_async(col_ptr, 0, 4ull * (NC + 1), st));
   launch(c, "ingest", k_ing_ptr, blocks_for(nnz), (const uint32_t*)col_of, nnz, NC, col_ptr);
   launch(c, "ingest", k_ing_split, blocks_for(NC), (const uint32_t*)col_of, (const uint32_t*)col_rows,
          (const uint32_t*)col_ptr, nnz, NC, nreg, col_split);
@@ -449,7 +568,7 @@ la3_status ingest_device(la3_ctx* c, const uint32_t* rec, uint64_t nrec, uint32_
   if (!sort_pairs(kr, vr, nnz, 2 * b)) return fail(c, LA3_EDEVICE, "device ingest: sort failed");
   launch(c, "ingest", k_ing_rowfill, blocks_for(nnz), (const uint64_t*)kr.Current(), (const uint32_t*)vr.Current(), nnz,
          b, (const uint32_t*)col_of, (const uint32_t*)col_w, nreg, nsink, row_cols, row_w, row_of, mkey);
-  HIPC(c, hipMemsetAsync(row_ptr, 0, 4ull * (NR + 1), st));
+  HIPC(c, dset_async(row_ptr, 0, 4ull * (NR + 1), st));
   launch(c, "ingest", k_ing_ptr, blocks_for(nnz), (const uint32_t*)row_of, nnz, NR, row_ptr);
   launch(c, "ingest", k_ing_split, blocks_for(NR), (const uint32_t*)row_of, (const uint32_t*)mkey,
          (const uint32_t*)row_ptr, nnz, NR, nsrc, row_split);
@@ -483,14 +602,14 @@ la3_status ingest_device(la3_ctx* c, const uint32_t* rec, uint64_t nrec, uint32_
   uint64_t npseg = 0;
   if ((e = total(nseg, pseg_first, np, npseg))) return e;
   const uint32_t npseg32 = (uint32_t)npseg;
-  HIPC(c, hipMemcpyAsync(pseg_first + np, &npseg32, 4, hipMemcpyHostToDevice, st));
+  HIPC(c, dcopy_async(pseg_first + np, &npseg32, 4, hipMemcpyHostToDevice, st));
   uint32_t* pseg_row = T.get<uint32_t>(npseg);
   uint32_t* pseg_beg = T.get<uint32_t>(npseg);
   unsigned long long* sums = T.get<unsigned long long>(2);
   if (!pseg_row || !pseg_beg || !sums) return fail(c, LA3_ENOMEM, oom);
   launch(c, "ingest", k_ing_segfill, blocks_for(np), (const uint32_t*)plong, (uint32_t)np, (const uint32_t*)row_ptr,
          (const uint32_t*)pseg_first, PSEG_ENTRIES, pseg_row, pseg_beg);
-  HIPC(c, hipMemsetAsync(sums, 0, 16, st));
+  HIPC(c, dset_async(sums, 0, 16, st));
   launch(c, "ingest", k_ing_lensum, blocks_for(nlong), (const uint32_t*)long_rows, (uint32_t)nlong,
          (const uint32_t*)row_ptr, nreg, sums);
   launch(c, "ingest", k_ing_lensum, blocks_for(np), (const uint32_t*)plong, (uint32_t)np, (const uint32_t*)row_ptr,
@@ -499,14 +618,14 @@ la3_status ingest_device(la3_ctx* c, const uint32_t* rec, uint64_t nrec, uint32_
   unsigned long long hs[2];
   std::vector<uint32_t> hplong(np), hpf(np + 1), hcp((size_t)NC + 1), hcs(NC);
   g->hid_of.resize(S); g->id_of.resize(S); g->slot_of_id.resize(S);
-  HIPC(c, hipMemcpyAsync(hs, sums, 16, hipMemcpyDeviceToHost, st));
-  if (np) HIPC(c, hipMemcpyAsync(hplong.data(), plong, 4 * np, hipMemcpyDeviceToHost, st));
-  HIPC(c, hipMemcpyAsync(hpf.data(), pseg_first, 4 * (np + 1), hipMemcpyDeviceToHost, st));
-  HIPC(c, hipMemcpyAsync(hcp.data(), col_ptr, 4ull * (NC + 1), hipMemcpyDeviceToHost, st));
-  if (NC) HIPC(c, hipMemcpyAsync(hcs.data(), col_split, 4ull * NC, hipMemcpyDeviceToHost, st));
-  HIPC(c, hipMemcpyAsync(g->hid_of.data(), hid_of, 4ull * S, hipMemcpyDeviceToHost, st));
-  HIPC(c, hipMemcpyAsync(g->id_of.data(), id_of, 4ull * S, hipMemcpyDeviceToHost, st));
-  HIPC(c, hipMemcpyAsync(g->slot_of_id.data(), slot_of_id, 4ull * S, hipMemcpyDeviceToHost, st));
+  HIPC(c, dcopy_async(hs, sums, 16, hipMemcpyDeviceToHost, st));
+  if (np) HIPC(c, dcopy_async(hplong.data(), plong, 4 * np, hipMemcpyDeviceToHost, st));
+  HIPC(c, dcopy_async(hpf.data(), pseg_first, 4 * (np + 1), hipMemcpyDeviceToHost, st));
+  HIPC(c, dcopy_async(hcp.data(), col_ptr, 4ull * (NC + 1), hipMemcpyDeviceToHost, st));
+  if (NC) HIPC(c, dcopy_async(hcs.data(), col_split, 4ull * NC, hipMemcpyDeviceToHost, st));
+  HIPC(c, dcopy_async(g->hid_of.data(), hid_of, 4ull * S, hipMemcpyDeviceToHost, st));
+  HIPC(c, dcopy_async(g->id_of.data(), id_of, 4ull * S, hipMemcpyDeviceToHost, st));
+  HIPC(c, dcopy_async(g->slot_of_id.data(), slot_of_id, 4ull * S, hipMemcpyDeviceToHost, st));
   HIPC(c, hipStreamSynchronize(st));
   uint64_t nnz_reg = 0;
   for (uint32_t col = 0; col < NC; col++) nnz_reg += hcs[col] - hcp[col];
@@ -551,7 +670,7 @@ la3_status load_records(la3_ctx* ctx, const uint32_t* rec, uint64_t nrec, uint32
     g->ctx = ctx;
     la3_status st = ingest_device(ctx, rec, nrec, n, weighted != 0, (int)mode, hashing != LA3_HASH_NONE, g.get());
     if (st) {
-      for (void* p : g->allocs) (void)hipFree(p);
+      for (void* p : g->allocs) dev_free(p);
       return st;
     }
     ctx->refs++;
@@ -599,7 +718,7 @@ struct FrontCounts {
 };
 
 la3_status read_counts(la3_ctx* c, const Frontier& f, FrontCounts& out) {
-  HIPC(c, hipMemcpyAsync(c->host_ctr, f.nitems, CTR_BYTES, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, dcopy_async(c->host_ctr, f.nitems, CTR_BYTES, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   const uint8_t* h = (const uint8_t*)c->host_ctr;
   out = FrontCounts();
@@ -637,7 +756,7 @@ uint32_t next_threshold(uint32_t T, uint32_t delta, uint32_t nmin) {
 la3_status deliver(la3_ctx* c, void* dst, const void* dev, size_t bytes) {
   c->last_result = dev;
   c->last_result_bytes = bytes;
-  if (dst) HIPC(c, hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+  if (dst) HIPC(c, dcopy_async(dst, dev, bytes, hipMemcpyDeviceToHost, c->stream));
   return LA3_OK;
 }
 
@@ -708,7 +827,7 @@ la3_status la3_graph_array(const la3_graph* g, la3_array which, uint32_t* out, u
     return LA3_OK;
   }
   HIPC(c, hipSetDevice(c->device));
-  HIPC(c, hipMemcpy(out, dev, 4 * n, hipMemcpyDeviceToHost));
+  HIPC(c, dcopy(out, dev, 4 * n, hipMemcpyDeviceToHost));
   return LA3_OK;
 }
 
@@ -820,7 +939,7 @@ void la3_graph_free(la3_graph* g) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (g->pr_exec) (void)hipGraphExecDestroy(g->pr_exec);
-    for (void* p : g->allocs) (void)hipFree(p);
+    for (void* p : g->allocs) dev_free(p);
   }
   delete g;
   ctx_release(c);
@@ -855,12 +974,12 @@ la3_status la3_graph_csc(const la3_graph* g, int which, uint32_t* colptrs, uint3
   } else {
     HIPC(c, hipSetDevice(c->device));
     cp.resize(NC + 1); cs.resize(NC); rows.resize(g->nnz);
-    HIPC(c, hipMemcpy(cp.data(), g->dg.col_ptr, sizeof(uint32_t) * (NC + 1), hipMemcpyDeviceToHost));
-    HIPC(c, hipMemcpy(cs.data(), g->dg.col_split, sizeof(uint32_t) * NC, hipMemcpyDeviceToHost));
-    if (g->nnz) HIPC(c, hipMemcpy(rows.data(), g->dg.col_rows, sizeof(uint32_t) * g->nnz, hipMemcpyDeviceToHost));
+    HIPC(c, dcopy(cp.data(), g->dg.col_ptr, sizeof(uint32_t) * (NC + 1), hipMemcpyDeviceToHost));
+    HIPC(c, dcopy(cs.data(), g->dg.col_split, sizeof(uint32_t) * NC, hipMemcpyDeviceToHost));
+    if (g->nnz) HIPC(c, dcopy(rows.data(), g->dg.col_rows, sizeof(uint32_t) * g->nnz, hipMemcpyDeviceToHost));
     if (g->weighted && g->nnz) {
       w.resize(g->nnz);
-      HIPC(c, hipMemcpy(w.data(), g->dg.col_w, sizeof(uint32_t) * g->nnz, hipMemcpyDeviceToHost));
+      HIPC(c, dcopy(w.data(), g->dg.col_w, sizeof(uint32_t) * g->nnz, hipMemcpyDeviceToHost));
     }
   }
   uint32_t k = 0;
@@ -878,6 +997,78 @@ la3_status la3_graph_csc(const la3_graph* g, int which, uint32_t* colptrs, uint3
   colptrs[NC] = k;
   return LA3_OK;
 }
+
+// ----------------------------------------------- vertex-program engine ---
+}  // extern "C"
+namespace {
+
+la3_device_view device_view(const la3_graph* g) {
+  const DG& d = g->dg;
+  la3_device_view v;
+  std::memset(&v, 0, sizeof(v));
+  v.device = g->ctx->device;
+  v.nslots = d.nslots; v.nreg = d.nreg; v.nsink = d.nsink; v.nsrc = d.nsrc; v.ncols = d.ncols; v.nrows = d.nrows;
+  v.nnz = d.nnz;
+  v.directed = g->directed; v.weighted = g->weighted;
+  v.col_ptr = d.col_ptr; v.col_split = d.col_split; v.col_rows = d.col_rows; v.col_w = g->weighted ? d.col_w : nullptr;
+  v.row_ptr = d.row_ptr; v.row_split = d.row_split; v.row_cols = d.row_cols; v.row_w = g->weighted ? d.row_w : nullptr;
+  v.id_of = d.id_of; v.hid_of = d.hid_of; v.slot_of_id = d.slot_of_id;
+  v.long_rows = d.long_rows; v.nlong = d.nlong;
+  return v;
+}
+static_assert(LONG_ROW == vp_detail::VP_LONG, "the layout's long-row list must match the engine's lane-per-row limit");
+
+// Engine buffers from the context's scratch, named under `prefix`; the
+// scratch keeps them for the next call, so release does nothing. `sa` must
+// outlive the engine.
+struct ScratchAlloc {
+  la3_ctx* c;
+  std::string prefix;
+};
+vp_detail::Allocator scratch_allocator(ScratchAlloc& sa) {
+  vp_detail::Allocator a;
+  a.self = &sa;
+  a.alloc = [](void* self, const char* name, size_t bytes) -> void* {
+    ScratchAlloc* s = (ScratchAlloc*)self;
+    return scratch(s->c, (s->prefix + name).c_str(), bytes);
+  };
+  a.release = [](void*, void*) {};
+  return a;
+}
+
+// CcVertex's operations (src/apps/graph_analytics/cc.h:28-37).
+struct CcOps {
+  using W = Empty;
+  using M = uint32_t;
+  using A = uint32_t;
+  struct S {
+    uint32_t label = 0;
+  };
+  __device__ bool init(uint32_t vid, S& s) const {
+    s.label = vid;
+    return true;
+  }
+  __device__ M scatter(const S& s) const { return s.label; }
+  __device__ A gather(const Edge<Empty>&, const M& m) const { return m; }
+  __device__ void combine(const A& y1, A& y2) const { y2 = y2 == 0 ? y1 : min(y1, y2); }
+  __device__ bool apply(const A& y, S& s) const {
+    const uint32_t t = s.label;
+    s.label = min(s.label, y);
+    return t != s.label;
+  }
+};
+
+}  // namespace
+extern "C" {
+
+la3_status la3_graph_device_view(const la3_graph* g, la3_device_view* out) {
+  if (!g || !out) return LA3_EINVAL;
+  if (g->host) return fail(g->ctx, LA3_EDEVICE, "graph lives on a host-only context");
+  *out = device_view(g);
+  return LA3_OK;
+}
+
+void* la3_ctx_stream(const la3_ctx* c) { return (c && c->device >= 0) ? (void*)c->stream : nullptr; }
 
 // ------------------------------------------------------------- degree ---
 la3_status la3_degree(la3_graph* g, uint32_t* degree) {
@@ -897,13 +1088,14 @@ la3_status la3_degree(la3_graph* g, uint32_t* degree) {
 }
 
 // ---------------------------------------------------------------- BFS ---
-la3_status la3_bfs(la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent, uint32_t* iters_run) {
+la3_status la3_bfs(la3_graph* g, uint32_t root, uint32_t max_iters, uint8_t* hops, uint32_t* parent,
+                   uint32_t* iters_run) {
   if (!g) return LA3_EINVAL;
-  return la3_bfs_ctx(g->ctx, g, root, hops, parent, iters_run);
+  return la3_bfs_ctx(g->ctx, g, root, max_iters, hops, parent, iters_run);
 }
 
-la3_status la3_bfs_ctx(la3_ctx* c, la3_graph* g, uint32_t root, uint8_t* hops, uint32_t* parent,
-                       uint32_t* iters_run) {
+la3_status la3_bfs_ctx(la3_ctx* c, la3_graph* g, uint32_t root, uint32_t max_iters, uint8_t* hops,
+                       uint32_t* parent, uint32_t* iters_run) {
   if (!c || !g) return LA3_EINVAL;
   if (c->device != g->ctx->device) return fail(c, LA3_EINVAL, "graph lives on another device");
   if (root > g->n) return fail(c, LA3_EINVAL, "root id out of range");
@@ -928,8 +1120,10 @@ la3_status la3_bfs_ctx(la3_ctx* c, la3_graph* g, uint32_t root, uint8_t* hops, u
   const uint32_t wave_blocks = 4 * c->ncu * 8;
   // The reference stops after the first iteration that activates no regular
   // vertex (vertex_program_execute.hpp:211-212); a regular vertex is exactly one
-  // that emits work items, so the loop runs while items exist.
-  while (fc.items > 0) {
+  // that emits work items, so the loop runs while items exist. Step `iter`
+  // gives level iter + 1, so max_iters = m stops after level m (a bounded
+  // loop that settles earlier only repeats empty iterations).
+  while (fc.items > 0 && (max_iters == 0 || iter < max_iters)) {
     const Frontier& nf = F.f[cur ^ 1];
     int ev = -1;
     if (!bottom_up) {
@@ -967,20 +1161,21 @@ la3_status la3_bfs_ctx(la3_ctx* c, la3_graph* g, uint32_t root, uint8_t* hops, u
   if (!oh || (parent && !op)) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   launch(c, "bfs_output", k_bfs_output, blocks_for(S), d, (const uint8_t*)level, (const uint32_t*)mark, oh, op);
   if ((st = deliver(c, hops, oh, S))) return st;
-  if (parent) HIPC(c, hipMemcpyAsync(parent, op, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
+  if (parent) HIPC(c, dcopy_async(parent, op, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   timing_end(c);
-  if (iters_run) *iters_run = std::max<uint32_t>(iter, 1);
+  if (iters_run) *iters_run = max_iters ? max_iters : std::max<uint32_t>(iter, 1);
   return LA3_OK;
 }
 
 // --------------------------------------------------------------- SSSP ---
-la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters_run) {
+la3_status la3_sssp(la3_graph* g, uint32_t root, uint32_t max_iters, uint32_t* dist, uint32_t* iters_run) {
   if (!g) return LA3_EINVAL;
-  return la3_sssp_ctx(g->ctx, g, root, dist, iters_run);
+  return la3_sssp_ctx(g->ctx, g, root, max_iters, dist, iters_run);
 }
 
-la3_status la3_sssp_ctx(la3_ctx* c, la3_graph* g, uint32_t root, uint32_t* dist, uint32_t* iters_run) {
+la3_status la3_sssp_ctx(la3_ctx* c, la3_graph* g, uint32_t root, uint32_t max_iters, uint32_t* dist,
+                        uint32_t* iters_run) {
   if (!c || !g) return LA3_EINVAL;
   if (c->device != g->ctx->device) return fail(c, LA3_EINVAL, "graph lives on another device");
   if (root > g->n) return fail(c, LA3_EINVAL, "root id out of range");
@@ -992,25 +1187,47 @@ la3_status la3_sssp_ctx(la3_ctx* c, la3_graph* g, uint32_t root, uint32_t* dist,
   const uint32_t S = g->nslots;
   uint32_t* dv = (uint32_t*)scratch(c, "sssp.dist", sizeof(uint32_t) * S);
   uint32_t* stamp = (uint32_t*)scratch(c, "sssp.stamp", sizeof(uint32_t) * S);
+  // Bounded runs relax synchronously: messages are the distances at the end
+  // of the previous step (sssp.msg), so m steps are the reference's m
+  // iterations; unbounded runs relax asynchronously (same fixed point).
+  const bool sync = max_iters > 0;
+  uint32_t* msg = sync ? (uint32_t*)scratch(c, "sssp.msg", sizeof(uint32_t) * S) : nullptr;
   Fronts F;
   la3_status st = get_fronts(c, g, F, "bfs");
   if (st) return st;
-  if (!dv || !stamp) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  if (!dv || !stamp || (sync && !msg)) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   const DG& d = g->dg;
   launch(c, "sssp_init", k_sssp_init, blocks_for(S), d, dv, stamp, g->slot_of_id[root], F.f[0], F.f[1].nitems);
   FrontCounts fc;
   if ((st = read_counts(c, F.f[0], fc))) return st;
+  if (sync) {
+    if (fc.items) launch(c, "sssp_snap", k_sssp_snap, blocks_for(fc.items), (const uint64_t*)F.f[0].items, fc.items,
+                         (const uint32_t*)dv, msg);
+    // execute(1) is execute_single (vertex_program_execute.hpp:20-32): the sink
+    // rows take the root's initial message, so their step runs before the
+    // regular rows' one.
+    if (max_iters == 1 && g->nsink) launch(c, "sssp_sinks", k_sssp_sinks, blocks_for(g->nsink), d, dv);
+  }
   uint32_t round = 0, cur = 0;
   const uint32_t wave_blocks = 4 * c->ncu * 8;
-  const uint32_t delta = g->sssp_delta;
+  const uint32_t delta = sync ? 0 : g->sssp_delta;
   uint32_t T = delta ? delta : 0xffffffffu;
-  while (fc.items > 0) {
+  while (fc.items > 0 && (!sync || round < max_iters)) {
     const Frontier& nf = F.f[cur ^ 1];
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((fc.items + 3) / 4, wave_blocks));
-    const int ev = launch(c, "sssp_push", k_sssp_push, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dv, stamp, round, T, nf, F.f[cur].nitems);
+    int ev;
+    if (sync)
+      ev = launch(c, "sssp_push", k_sssp_push<true>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dv, stamp,
+                  round, T, nf, F.f[cur].nitems, (const uint32_t*)msg);
+    else
+      ev = launch(c, "sssp_push", k_sssp_push<false>, blocks, d, (const uint64_t*)F.f[cur].items, fc.items, dv, stamp,
+                  round, T, nf, F.f[cur].nitems, (const uint32_t*)nullptr);
     c->edges_scanned += fc.edges;
     FrontCounts nc;
     if ((st = read_counts(c, nf, nc))) return st;
+    if (sync && nc.items)
+      launch(c, "sssp_snap", k_sssp_snap, blocks_for(nc.items), (const uint64_t*)nf.items, nc.items,
+             (const uint32_t*)dv, msg);
     // Algorithmic bytes (w = m = a = s_r = s_w = 4 B): (4 + w) per entry,
     // 8 B pointers + m per active column, (a + s_r + s_w) + m per activated row.
     annotate(c, ev, 8.0 * fc.edges + 12.0 * fc.cols + 16.0 * nc.verts, fc.edges);
@@ -1019,14 +1236,14 @@ la3_status la3_sssp_ctx(la3_ctx* c, la3_graph* g, uint32_t root, uint32_t* dist,
     round++;
     cur ^= 1;
   }
-  if (g->nsink) launch(c, "sssp_sinks", k_sssp_sinks, blocks_for(g->nsink), d, dv);
+  if (g->nsink && max_iters != 1) launch(c, "sssp_sinks", k_sssp_sinks, blocks_for(g->nsink), d, dv);
   uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
   if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
   launch(c, "gather_u32", k_gather_by_id<uint32_t>, blocks_for(S), d, (const uint32_t*)dv, out);
   if ((st = deliver(c, dist, out, sizeof(uint32_t) * S))) return st;
   HIPC(c, hipStreamSynchronize(c->stream));
   timing_end(c);
-  if (iters_run) *iters_run = round + 1;
+  if (iters_run) *iters_run = sync ? max_iters : round + 1;
   return LA3_OK;
 }
 
@@ -1136,16 +1353,19 @@ static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t
     it++;
     if (until) {
       uint32_t h = 0;
-      HIPC(c, hipMemcpyAsync(c->host_ctr, act, 4, hipMemcpyDeviceToHost, c->stream));
+      HIPC(c, dcopy_async(c->host_ctr, act, 4, hipMemcpyDeviceToHost, c->stream));
       HIPC(c, hipStreamSynchronize(c->stream));
       std::memcpy(&h, c->host_ctr, 4);
       if (!h) break;
     }
   }
   c->edges_scanned = (uint64_t)it * (g->nnz_reg) + g->nnz_sink;
-  // Sink step: xa now holds the final regular messages.
-  a.x = xa;
-  a.xnext = xb;
+  // Sink step: xa now holds the final regular messages. execute(1) is the
+  // reference's execute_single (vertex_program_execute.hpp:20-32), whose sink
+  // rows take the same initial messages as the regular rows: xb still holds
+  // them after the one step.
+  a.x = iters == 1 ? xb : xa;
+  a.xnext = iters == 1 ? xa : xb;
   if (g->nsink) {
     pr_pass<2>(c, g, a, true, part, "pr_sinks");
   }
@@ -1158,7 +1378,7 @@ static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t
     uint32_t* outu = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
     if (!outu) return fail(c, LA3_ENOMEM, "scratch allocation failed");
     launch(c, "gather_u32", k_gather_by_id<uint32_t>, blocks_for(S), d, (const uint32_t*)deg, outu);
-    HIPC(c, hipMemcpyAsync(degree, outu, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, dcopy_async(degree, outu, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
   }
   HIPC(c, hipStreamSynchronize(c->stream));
   timing_end(c);
@@ -1181,7 +1401,8 @@ la3_status la3_ppr(la3_graph* g, uint32_t seed, uint32_t iters, double alpha, do
 }
 
 // ----------------------------------------------------------------- CC ---
-la3_status la3_cc(la3_graph* g, uint32_t* label, uint32_t* iters_run) {
+// Union-find components (no counterpart in the reference).
+la3_status la3_components(la3_graph* g, uint32_t* label) {
   if (!g) return LA3_EINVAL;
   la3_ctx* c = g->ctx;
   if (g->directed) return fail(c, LA3_EINVAL, "CC runs on an undirected load");
@@ -1209,7 +1430,7 @@ la3_status la3_cc(la3_graph* g, uint32_t* label, uint32_t* iters_run) {
     uint32_t* smp = (uint32_t*)scratch(c, "cc.sample", sizeof(uint32_t) * NSAMPLE);
     if (!smp) return fail(c, LA3_ENOMEM, "scratch allocation failed");
     launch(c, "cc_sample", k_cc_sample, 2, d, (const uint32_t*)p, NSAMPLE, 2654435761u, smp);
-    HIPC(c, hipMemcpyAsync(c->host_ctr, smp, sizeof(uint32_t) * NSAMPLE, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, dcopy_async(c->host_ctr, smp, sizeof(uint32_t) * NSAMPLE, hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     std::vector<uint32_t> roots((const uint32_t*)c->host_ctr, (const uint32_t*)c->host_ctr + NSAMPLE);
     std::sort(roots.begin(), roots.end());
@@ -1231,7 +1452,32 @@ la3_status la3_cc(la3_graph* g, uint32_t* label, uint32_t* iters_run) {
   if (st) return st;
   HIPC(c, hipStreamSynchronize(c->stream));
   timing_end(c);
-  if (iters_run) *iters_run = 1;
+  return LA3_OK;
+}
+
+// CcVertex (src/apps/graph_analytics/cc.h:22-38) on the vertex-program
+// engine of include/la3_vertex_program.hpp, with the reference's operations.
+la3_status la3_cc(la3_graph* g, uint32_t max_iters, uint32_t* label, uint32_t* iters_run) {
+  if (!g) return LA3_EINVAL;
+  la3_ctx* c = g->ctx;
+  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+  HIPC(c, hipSetDevice(c->device));
+  timing_begin(c);
+  const uint32_t S = g->nslots;
+  ScratchAlloc sa{c, "cc."};
+  vp_detail::Engine<CcOps> e(device_view(g), c->stream, false, CcOps(), scratch_allocator(sa));
+  hipError_t he = e.execute(max_iters);
+  if (he != hipSuccess) return fail(c, LA3_EDEVICE, "CC: " + e.err);
+  uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
+  if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  static_assert(sizeof(CcOps::S) == sizeof(uint32_t), "CC state is one label");
+  launch(c, "vp_by_id", vp_detail::k_vp_by_id<CcOps::S>, blocks_for(S), device_view(g),
+         (const CcOps::S*)e.bufs().v, (CcOps::S*)out);
+  la3_status st = deliver(c, label, out, sizeof(uint32_t) * S);
+  if (st) return st;
+  HIPC(c, hipStreamSynchronize(c->stream));
+  timing_end(c);
+  if (iters_run) *iters_run = e.iters_run;
   return LA3_OK;
 }
 
@@ -1288,7 +1534,7 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
              tot)};
   c->edges_scanned = g->nnz;
   unsigned long long h[6] = {0, 0, 0, 0, 0, 0};
-  HIPC(c, hipMemcpyAsync(c->host_ctr, tot, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, dcopy_async(c->host_ctr, tot, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   if (per_vertex) {
     uint32_t* out = (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S);
     if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
@@ -1314,7 +1560,7 @@ la3_status la3_tc(la3_graph* g, uint64_t* total, uint32_t* per_vertex) {
 
 template <int W>
 static la3_status msbfs_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>& root_slots, uint32_t k,
-                            uint8_t* hops, uint32_t* iters_run) {
+                            uint32_t max_iters, uint8_t* hops, uint32_t* iters_run) {
   const uint32_t S = g->nslots;
   const uint32_t K = 64 * W;
   const DG& d = g->dg;
@@ -1338,8 +1584,8 @@ static la3_status msbfs_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t
     const int bits = std::max(0, std::min(64, (int)k - 64 * w));
     hv[w] = bits == 64 ? ~0ull : ((1ull << bits) - 1);
   }
-  HIPC(c, hipMemcpyAsync(valid, hv, sizeof(hv), hipMemcpyHostToDevice, c->stream));
-  HIPC(c, hipMemcpyAsync(rs, root_slots.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, dcopy_async(valid, hv, sizeof(hv), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, dcopy_async(rs, root_slots.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream));
   launch(c, "msbfs_init", k_msbfs_init<W>, blocks_for(S), d, vis, fa, fb, (uint64_t*)lev, stamp);
   launch(c, "msbfs_roots", k_msbfs_roots<W>, 1, d, (const uint32_t*)rs, k, vis, fa, lev);
   zero_counts(c, F.f[0]);
@@ -1349,7 +1595,7 @@ static la3_status msbfs_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t
   uint32_t iter = 0, cur = 0;
   bool bottom_up = false;
   const uint32_t wave_blocks = 4 * c->ncu * 8;
-  while (fc.items > 0) {
+  while (fc.items > 0 && (max_iters == 0 || iter < max_iters)) {
     const Frontier& nf = F.f[cur ^ 1];
     zero_counts(c, nf);
     int ev = -1;
@@ -1394,20 +1640,20 @@ static la3_status msbfs_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t
   launch(c, "msbfs_output", k_msbfs_output, blocks_for(S), d, (const uint8_t*)lev, K, k, out);
   if ((st = deliver(c, hops, out, (size_t)S * k))) return st;
   HIPC(c, hipStreamSynchronize(c->stream));
-  if (iters_run) *iters_run = std::max<uint32_t>(iter, 1);
+  if (iters_run) *iters_run = max_iters ? max_iters : std::max<uint32_t>(iter, 1);
   return LA3_OK;
 }
 
 extern "C" {
 
-la3_status la3_bfs_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint8_t* hops,
+la3_status la3_bfs_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint32_t max_iters, uint8_t* hops,
                          uint32_t* iters_run) {
   if (!g) return LA3_EINVAL;
-  return la3_bfs_batch_ctx(g->ctx, g, roots, k, hops, iters_run);
+  return la3_bfs_batch_ctx(g->ctx, g, roots, k, max_iters, hops, iters_run);
 }
 
 la3_status la3_bfs_batch_ctx(la3_ctx* c, la3_graph* g, const uint32_t* roots, uint32_t k,
-                             uint8_t* hops, uint32_t* iters_run) {
+                             uint32_t max_iters, uint8_t* hops, uint32_t* iters_run) {
   if (!c || !g || !roots) return LA3_EINVAL;
   if (c->device != g->ctx->device) return fail(c, LA3_EINVAL, "graph lives on another device");
   if (k == 0 || k > 1024) return fail(c, LA3_EINVAL, "batch size must be in [1, 1024]");
@@ -1421,11 +1667,11 @@ la3_status la3_bfs_batch_ctx(la3_ctx* c, la3_graph* g, const uint32_t* roots, ui
   timing_begin(c);
   la3_status st;
   const uint32_t W = (k + 63) / 64;
-  if (W <= 1) st = msbfs_run<1>(c, g, rs, k, hops, iters_run);
-  else if (W <= 2) st = msbfs_run<2>(c, g, rs, k, hops, iters_run);
-  else if (W <= 4) st = msbfs_run<4>(c, g, rs, k, hops, iters_run);
-  else if (W <= 8) st = msbfs_run<8>(c, g, rs, k, hops, iters_run);
-  else st = msbfs_run<16>(c, g, rs, k, hops, iters_run);
+  if (W <= 1) st = msbfs_run<1>(c, g, rs, k, max_iters, hops, iters_run);
+  else if (W <= 2) st = msbfs_run<2>(c, g, rs, k, max_iters, hops, iters_run);
+  else if (W <= 4) st = msbfs_run<4>(c, g, rs, k, max_iters, hops, iters_run);
+  else if (W <= 8) st = msbfs_run<8>(c, g, rs, k, max_iters, hops, iters_run);
+  else st = msbfs_run<16>(c, g, rs, k, max_iters, hops, iters_run);
   timing_end(c);
   return st;
 }
@@ -1456,7 +1702,7 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   if ((st = get_queue(c, "mssp.touched", (uint64_t)S + g->nnz + 1, T))) return st;
   if (!dist || !cm || !nm || !stamp || !rs || !out)
     return fail(c, LA3_ENOMEM, "batched SSSP: device allocation failed");
-  HIPC(c, hipMemcpyAsync(rs, root_slots.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, dcopy_async(rs, root_slots.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream));
   const uint64_t inf2 = ((uint64_t)SSSP_INF << 32) | SSSP_INF;
   if constexpr (K % 2)  // one lane per row: S may be odd
     launch(c, "fill_u32", k_fill<uint32_t>, blocks_for((uint64_t)S * K), dist, (uint64_t)S * K, SSSP_INF);
@@ -1546,14 +1792,14 @@ static la3_status mssp_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
 
 extern "C" {
 
-la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint32_t* dist,
+la3_status la3_sssp_batch(la3_graph* g, const uint32_t* roots, uint32_t k, uint32_t max_iters, uint32_t* dist,
                           uint32_t* iters_run) {
   if (!g) return LA3_EINVAL;
-  return la3_sssp_batch_ctx(g->ctx, g, roots, k, dist, iters_run);
+  return la3_sssp_batch_ctx(g->ctx, g, roots, k, max_iters, dist, iters_run);
 }
 
 la3_status la3_sssp_batch_ctx(la3_ctx* c, la3_graph* g, const uint32_t* roots, uint32_t k,
-                              uint32_t* dist, uint32_t* iters_run) {
+                              uint32_t max_iters, uint32_t* dist, uint32_t* iters_run) {
   if (!c || !g || !roots) return LA3_EINVAL;
   if (c->device != g->ctx->device) return fail(c, LA3_EINVAL, "graph lives on another device");
   if (k == 0 || k > 1024) return fail(c, LA3_EINVAL, "batch size must be in [1, 1024]");
@@ -1566,8 +1812,26 @@ la3_status la3_sssp_batch_ctx(la3_ctx* c, la3_graph* g, const uint32_t* roots, u
   }
   if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
   HIPC(c, hipSetDevice(c->device));
-  timing_begin(c);
   la3_status st;
+  if (max_iters) {
+    // Bounded: the queries one after another on the synchronous single-query
+    // path (the k-wide push relaxes asynchronously), rows gathered on the
+    // device in query order.
+    const size_t S = g->nslots;
+    for (uint32_t q = 0; q < k; q++) {
+      if ((st = la3_sssp_ctx(c, g, roots[q], max_iters, nullptr, nullptr))) return st;
+      uint32_t* rows = (uint32_t*)scratch(c, "msspb.out", sizeof(uint32_t) * S * k);
+      if (!rows) return fail(c, LA3_ENOMEM, "batched SSSP: device allocation failed");
+      HIPC(c, dcopy_async(rows + S * q, c->last_result, sizeof(uint32_t) * S, hipMemcpyDeviceToDevice, c->stream));
+      if (q + 1 == k) {
+        if ((st = deliver(c, dist, rows, sizeof(uint32_t) * S * k))) return st;
+        HIPC(c, hipStreamSynchronize(c->stream));
+      }
+    }
+    if (iters_run) *iters_run = max_iters;
+    return LA3_OK;
+  }
+  timing_begin(c);
   const uint32_t QW = (k + 63) / 64;
   if (k == 1) st = mssp_run<1, 1>(c, g, rs, k, dist, iters_run);
   else if (k == 2) st = mssp_run<1, 2>(c, g, rs, k, dist, iters_run);
@@ -1616,7 +1880,7 @@ static la3_status mppr_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
     return fail(c, LA3_ENOMEM, "batched PPR: device allocation failed");
   std::vector<uint32_t> seeds(K, 0xffffffffu);
   std::copy(seed_slots.begin(), seed_slots.end(), seeds.begin());
-  HIPC(c, hipMemcpyAsync(sd, seeds.data(), sizeof(uint32_t) * K, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, dcopy_async(sd, seeds.data(), sizeof(uint32_t) * K, hipMemcpyHostToDevice, c->stream));
   PprBatchArgs a{};
   a.x = x0; a.xnext = x1; a.hi = hi; a.lo = lo; a.deg = deg; a.seeds = sd; a.k = k;
   a.alpha_hi = (float)alpha;
@@ -1663,8 +1927,8 @@ static la3_status mppr_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   }
   if (residual) {
     std::vector<double> hr((size_t)rblocks * K), hc((size_t)cblocks * K);
-    HIPC(c, hipMemcpyAsync(hr.data(), res_r, sizeof(double) * hr.size(), hipMemcpyDeviceToHost, c->stream));
-    HIPC(c, hipMemcpyAsync(hc.data(), res_c, sizeof(double) * hc.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, dcopy_async(hr.data(), res_r, sizeof(double) * hr.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, dcopy_async(hc.data(), res_c, sizeof(double) * hc.size(), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     for (uint32_t q = 0; q < k; q++) {
       double t = 0;
@@ -1673,14 +1937,19 @@ static la3_status mppr_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
       residual[q] = iters ? t : 0.0;
     }
   }
-  // Sink step from the final messages (vertex_program_execute.hpp:131-148).
-  if (g->nsink) step(g->nreg, g->nrows, d.npseg_reg, d.npseg, 0, res_s, res_s);
+  // Sink step from the final messages (vertex_program_execute.hpp:131-148);
+  // with one iteration (execute_single, vertex_program_execute.hpp:20-32)
+  // from the initial ones, which the other buffer still holds.
+  if (g->nsink) {
+    if (iters == 1) std::swap(xa, xb);
+    step(g->nreg, g->nrows, d.npseg_reg, d.npseg, 0, res_s, res_s);
+  }
   launch(c, "mppr_output", k_batch_output<float>, blocks_for(S), d, (const float*)hi, K, k, out);
   la3_status st = deliver(c, rank, out, sizeof(float) * (size_t)S * k);
   if (st) return st;
   if (comp) {
     launch(c, "mppr_output", k_batch_output<float>, blocks_for(S), d, (const float*)lo, K, k, outc);
-    HIPC(c, hipMemcpyAsync(comp, outc, sizeof(float) * (size_t)S * k, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, dcopy_async(comp, outc, sizeof(float) * (size_t)S * k, hipMemcpyDeviceToHost, c->stream));
   }
   HIPC(c, hipStreamSynchronize(c->stream));
   return LA3_OK;
@@ -1767,11 +2036,11 @@ la3_status la3_autotune(la3_graph* g, const uint32_t* roots, uint32_t nroots, la
   uint32_t pick_s = 0, pick_b = 0;
   for (uint32_t i = 0; i < nc; i++) {
     g->sssp_delta = cand[i];
-    la3_status st = la3_sssp(g, roots[0], nullptr, nullptr);  // warm
+    la3_status st = la3_sssp(g, roots[0], 0, nullptr, nullptr);  // warm
     if (st) return st;
     double t0 = clock();
     for (uint32_t r = 0; r < nsingle; r++)
-      if ((st = la3_sssp(g, roots[r], nullptr, nullptr))) return st;
+      if ((st = la3_sssp(g, roots[r], 0, nullptr, nullptr))) return st;
     const double ts = clock() - t0;
     if (rep) rep->single_ms[i] = ts * 1e3 / nsingle;
     if (ts < best_s) { best_s = ts; pick_s = cand[i]; }
@@ -1779,7 +2048,7 @@ la3_status la3_autotune(la3_graph* g, const uint32_t* roots, uint32_t nroots, la
     double tb = 1e30;  // best of three: candidates often differ by a few percent
     for (int r = 0; r < 3; r++) {
       t0 = clock();
-      if ((st = la3_sssp_batch(g, roots, kb, nullptr, nullptr))) return st;
+      if ((st = la3_sssp_batch(g, roots, kb, 0, nullptr, nullptr))) return st;
       tb = std::min(tb, clock() - t0);
     }
     if (rep) rep->batch_ms[i] = tb * 1e3;
@@ -1888,7 +2157,8 @@ la3_status la3_result_fetch(la3_ctx* c, void* dst, uint64_t bytes) {
   if (!c || !dst) return LA3_EINVAL;
   if (!c->last_result) return fail(c, LA3_EINVAL, "no result on the device");
   if (bytes > c->last_result_bytes) return fail(c, LA3_EINVAL, "fetch larger than the last result");
-  HIPC(c, hipMemcpyAsync(dst, c->last_result, bytes, hipMemcpyDeviceToHost, c->stream));
+  if (!dev_covers(c->last_result, bytes)) return fail(c, LA3_EINVAL, "the last result is no longer in device memory");
+  HIPC(c, dcopy_async(dst, c->last_result, bytes, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   return LA3_OK;
 }
@@ -1983,7 +2253,7 @@ la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t
     if ((st = get_queue(c, fz_name(t, "q0"), qcap, T.q[0])) || (st = get_queue(c, fz_name(t, "q1"), qcap, T.q[1])) ||
         (st = get_queue(c, fz_name(t, "touched"), bfs ? (uint64_t)S + 1 : (uint64_t)S + g->nnz + 1, T.touched)))
       return st;
-    HIPC(c, hipMemcpyAsync(rs, slots.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, dcopy_async(rs, slots.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream));
     if (bfs) {
       T.vis = (uint64_t*)buf("vis", 8 * SW);
       T.fr[0] = (uint64_t*)buf("fr0", 8 * SW);
@@ -1996,7 +2266,7 @@ la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t
         const int bits = std::max(0, std::min(64, (int)k - 64 * (int)w));
         hv[w] = bits == 64 ? ~0ull : ((1ull << bits) - 1);
       }
-      HIPC(c, hipMemcpyAsync(valid, hv, 8 * W, hipMemcpyHostToDevice, c->stream));
+      HIPC(c, dcopy_async(valid, hv, 8 * W, hipMemcpyHostToDevice, c->stream));
       T.valid = valid;
       if (W == 1) {
         launch(c, "msbfs_init", k_msbfs_init<1>, blocks_for(S), d, T.vis, T.fr[0], T.fr[1], (uint64_t*)T.lev, T.stamp);
@@ -2046,7 +2316,7 @@ la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t
     return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
   std::vector<uint32_t> seeds(K, 0xffffffffu);
   std::copy(slots.begin(), slots.end(), seeds.begin());
-  HIPC(c, hipMemcpyAsync(sd, seeds.data(), sizeof(uint32_t) * K, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, dcopy_async(sd, seeds.data(), sizeof(uint32_t) * K, hipMemcpyHostToDevice, c->stream));
   PprBatchArgs& a = T.pa;
   a.x = T.xs[0]; a.xnext = T.xs[1]; a.hi = hi; a.lo = lo; a.deg = deg; a.seeds = sd; a.k = k;
   a.alpha_hi = (float)alpha;
@@ -2085,11 +2355,13 @@ la3_status fz_finish(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t 
     if (g->nsink) launch(c, "mssp_sinks", k_mssp_sinks<QW>, std::min<uint32_t>((g->nsink + 3) / 4, wave_blocks), d, T.dist);
     launch(c, "mssp_output", k_batch_output<uint32_t>, blocks_for(S), d, (const uint32_t*)T.dist, K, k, (uint32_t*)out);
   } else {
-    // Sink step from the final messages (vertex_program_execute.hpp:131-148).
+    // Sink step from the final messages (vertex_program_execute.hpp:131-148);
+    // with one iteration (execute_single) from the initial ones.
     if (g->nsink) {
       PprBatchArgs a = T.pa;
-      a.x = T.xs[C.cur];
-      a.xnext = T.xs[C.cur ^ 1];
+      const uint32_t xi = T.iters == 1 ? C.cur ^ 1 : C.cur;
+      a.x = T.xs[xi];
+      a.xnext = T.xs[xi ^ 1];
       a.resid = (double*)scratch(c, fz_name(t, "res_sink").c_str(), sizeof(double) * wave_blocks * K);
       if (!a.resid) return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
       const uint32_t rblocks = std::max<uint32_t>(1, std::min<uint32_t>((g->nrows + 3) / 4, wave_blocks));
@@ -2109,11 +2381,11 @@ la3_status fz_finish(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t 
       float* oc = (float*)scratch(c, fz_name(t, "outc").c_str(), sizeof(float) * (size_t)S * k);
       if (!oc) return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
       launch(c, "mppr_output", k_batch_output<float>, blocks_for(S), d, (const float*)T.pa.lo, K, k, oc);
-      HIPC(c, hipMemcpyAsync(b.out_comp + (size_t)S * ft.q0, oc, sizeof(float) * (size_t)S * k, hipMemcpyDeviceToHost,
+      HIPC(c, dcopy_async(b.out_comp + (size_t)S * ft.q0, oc, sizeof(float) * (size_t)S * k, hipMemcpyDeviceToHost,
                              c->stream));
     }
     if (b.residual)
-      HIPC(c, hipMemcpyAsync(b.residual + ft.q0, T.resid, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
+      HIPC(c, dcopy_async(b.residual + ft.q0, T.resid, sizeof(double) * k, hipMemcpyDeviceToHost, c->stream));
   }
   return deliver(c, host, out, esz * (size_t)S * k);
 }
@@ -2212,6 +2484,7 @@ struct la3_session {
   uint32_t ntasks = 0;      // slots [0, ntasks) are known to the device
   uint32_t steps = 1;       // steps per replay
   bool keep_slots = false;  // keep every slot's buffers (la3_batch_run: the caller releases)
+  size_t keep_bytes = FZ_SLOT_KEEP_BYTES;  // a finished slot keeps its buffers up to this much
   bool pending_init = false;
   std::map<uint64_t, FzBatch> batches;
   std::deque<uint64_t> done;
@@ -2251,7 +2524,7 @@ la3_status sess_add(la3_session* s, const la3_batch& in, uint64_t* id) {
       s->fresh[u] = FCtl{};
       s->fresh[u].state = FS_FREE;
       s->fresh[u].done = 1;
-      (void)hipMemcpyAsync(dc + u, &s->fresh[u], sizeof(FCtl), hipMemcpyHostToDevice, c->stream);
+      (void)dcopy_async(dc + u, &s->fresh[u], sizeof(FCtl), hipMemcpyHostToDevice, c->stream);
       scratch_release(c, fz_name(u, ""));
     }
     s->batches.erase(bid);
@@ -2272,13 +2545,13 @@ la3_status sess_add(la3_session* s, const la3_batch& in, uint64_t* id) {
     sl.max_steps = T.kind == FK_PPR ? (uint64_t)T.iters + 8 : 8ull * T.g.nslots + 4096;
     s->fresh[t] = FCtl{};
     s->fresh[t].state = FS_NEW;
-    if (hipMemcpyAsync(c->fz_tasks + t, &s->ht[t], sizeof(FTask), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-        hipMemcpyAsync(dc + t, &s->fresh[t], sizeof(FCtl), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+    if (dcopy_async(c->fz_tasks + t, &s->ht[t], sizeof(FTask), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        dcopy_async(dc + t, &s->fresh[t], sizeof(FCtl), hipMemcpyHostToDevice, c->stream) != hipSuccess)
       return undo(fail(c, LA3_EDEVICE, "session: task upload failed"));
     s->ntasks = std::max(s->ntasks, t + 1);
   }
   s->hdr.ntasks = s->ntasks;
-  HIPC(c, hipMemcpyAsync(&dh->ntasks, &s->hdr.ntasks, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, dcopy_async(&dh->ntasks, &s->hdr.ntasks, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
   s->pending_init = true;
   if (id) *id = bid;
   return LA3_OK;
@@ -2314,7 +2587,7 @@ la3_status sess_step(la3_session* s) {
       for (uint32_t i = 0; i < s->steps; i++) fz_step(c, mask, s->hdr.ga, true, &last_a, &last_pull);
     }
   }
-  HIPC(c, hipMemcpyAsync(s->hc, dc, sizeof(FCtl) * s->ntasks, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, dcopy_async(s->hc, dc, sizeof(FCtl) * s->ntasks, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   double bytes = 0, pbytes = 0;
   uint64_t edges = 0;
@@ -2343,7 +2616,7 @@ la3_status sess_step(la3_session* s) {
     s->fresh[t] = FCtl{};
     s->fresh[t].state = FS_FREE;
     s->fresh[t].done = 1;
-    HIPC(c, hipMemcpyAsync(dc + t, &s->fresh[t], sizeof(FCtl), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, dcopy_async(dc + t, &s->fresh[t], sizeof(FCtl), hipMemcpyHostToDevice, c->stream));
     sl.used = false;
     if (--fb.tasks_left == 0) s->done.push_back(sl.batch);
     finished = true;
@@ -2364,7 +2637,7 @@ la3_status sess_step(la3_session* s) {
     size_t held = 0;
     for (const auto& kv : c->scratch)
       if (kv.first.compare(0, pre.size(), pre) == 0) held += kv.second.bytes;
-    if (held > FZ_SLOT_KEEP_BYTES) scratch_release(c, pre);
+    if (held > s->keep_bytes) scratch_release(c, pre);
   }
   return LA3_OK;
 }
@@ -2383,10 +2656,12 @@ la3_status sess_init(la3_ctx* c, double alpha, uint32_t blocks_per_cu, la3_sessi
   }
   s->c = c;
   s->alpha = alpha;
+  // LA3_SESSION_KEEP_BYTES overrides how much a finished slot keeps.
+  if (const char* kb = getenv("LA3_SESSION_KEEP_BYTES")) s->keep_bytes = (size_t)strtoull(kb, nullptr, 10);
   s->hdr = FHdr{};
   s->hdr.ga = fz_ga(c, blocks_per_cu);
   s->hdr.gb = fz_gb(c);
-  HIPC(c, hipMemcpyAsync(c->fz_ctl, &s->hdr, sizeof(FHdr), hipMemcpyHostToDevice, c->stream));
+  HIPC(c, dcopy_async(c->fz_ctl, &s->hdr, sizeof(FHdr), hipMemcpyHostToDevice, c->stream));
   c->fz_busy = true;
   return LA3_OK;
 }
@@ -2503,6 +2778,8 @@ la3_status la3_result_topk(la3_ctx* c, la3_dtype dtype, uint32_t rows, uint64_t 
   const uint32_t esz = dtype == LA3_DT_U8 ? 1 : dtype == LA3_DT_F64 ? 8 : 4;
   if (!c->last_result || (uint64_t)rows * n * esz > c->last_result_bytes)
     return fail(c, LA3_EINVAL, "no device-resident result of that size (call a program with a NULL output first)");
+  if (!dev_covers(c->last_result, (size_t)rows * n * esz))
+    return fail(c, LA3_EINVAL, "the last result is no longer in device memory");
   HIPC(c, hipSetDevice(c->device));
   timing_begin(c);
   const void* v = c->last_result;
@@ -2524,11 +2801,11 @@ la3_status la3_result_topk(la3_ctx* c, la3_dtype dtype, uint32_t rows, uint64_t 
   std::vector<uint32_t> above(rows, 0), hist(256ull * rows);
   uint64_t hi_mask = 0;
   for (int shift = (int)bits - 8; shift >= 0; shift -= 8) {
-    HIPC(c, hipMemcpyAsync(dprefix, prefix.data(), 8ull * rows, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, dcopy_async(dprefix, prefix.data(), 8ull * rows, hipMemcpyHostToDevice, c->stream));
     launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(256ull * rows), dhist, 256ull * rows, 0u);
     launch(c, "topk_hist", k_topk_hist, nb, v, (uint32_t)dtype, largest, n, chunks, (const uint64_t*)dprefix, hi_mask,
            (uint32_t)shift, dhist);
-    HIPC(c, hipMemcpyAsync(hist.data(), dhist, 4ull * 256 * rows, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, dcopy_async(hist.data(), dhist, 4ull * 256 * rows, hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     for (uint32_t r = 0; r < rows; r++) {
       uint32_t acc = above[r];
@@ -2544,9 +2821,9 @@ la3_status la3_result_topk(la3_ctx* c, la3_dtype dtype, uint32_t rows, uint64_t 
   }
   // 2. ordered compaction: every key above the threshold, equal keys by id
   std::vector<uint32_t> small(4ull * rows, 0), ceq(nb), base(nb);
-  HIPC(c, hipMemcpyAsync(dprefix, prefix.data(), 8ull * rows, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, dcopy_async(dprefix, prefix.data(), 8ull * rows, hipMemcpyHostToDevice, c->stream));
   launch(c, "topk_count", k_topk_count_eq, nb, v, (uint32_t)dtype, largest, n, chunks, (const uint64_t*)dprefix, dceq);
-  HIPC(c, hipMemcpyAsync(ceq.data(), dceq, 4ull * nb, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, dcopy_async(ceq.data(), dceq, 4ull * nb, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   for (uint32_t r = 0; r < rows; r++) {
     uint32_t b = 0;
@@ -2555,16 +2832,16 @@ la3_status la3_result_topk(la3_ctx* c, la3_dtype dtype, uint32_t rows, uint64_t 
     small[rows + r] = above[r];         // gtc
     small[3ull * rows + r] = kk;        // cnt
   }
-  HIPC(c, hipMemcpyAsync(dsmall, small.data(), 16ull * rows, hipMemcpyHostToDevice, c->stream));
-  HIPC(c, hipMemcpyAsync(dbase, base.data(), 4ull * nb, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, dcopy_async(dsmall, small.data(), 16ull * rows, hipMemcpyHostToDevice, c->stream));
+  HIPC(c, dcopy_async(dbase, base.data(), 4ull * nb, hipMemcpyHostToDevice, c->stream));
   launch(c, "topk_write", k_topk_write, nb, v, (uint32_t)dtype, largest, n, chunks, (const uint64_t*)dprefix,
          (const uint32_t*)dsmall, (const uint32_t*)(dsmall + rows), (const uint32_t*)dbase, dsmall + 2 * rows, k, dids);
   launch(c, "topk_gather", k_topk_gather, blocks_for((uint64_t)rows * k), v, esz, n, rows, k,
          (const uint32_t*)(dsmall + 3 * rows), (const uint32_t*)dids, dvals);
   std::vector<uint32_t> hid((size_t)rows * k);
   std::vector<uint8_t> hv((size_t)rows * k * esz);
-  HIPC(c, hipMemcpyAsync(hid.data(), dids, 4ull * rows * k, hipMemcpyDeviceToHost, c->stream));
-  HIPC(c, hipMemcpyAsync(hv.data(), dvals, (size_t)esz * rows * k, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, dcopy_async(hid.data(), dids, 4ull * rows * k, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, dcopy_async(hv.data(), dvals, (size_t)esz * rows * k, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   timing_end(c);
   // 3. order each row's kk entries: value (largest or smallest first), then id

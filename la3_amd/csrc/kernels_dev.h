@@ -453,9 +453,15 @@ __global__ void k_sssp_init(DG g, uint32_t* dist, uint32_t* stamp, uint32_t root
 // entries of those items are then packed into passes of 64 lanes (lane l
 // of a pass finds its item by a binary search over the lanes' exclusive
 // offsets), so short columns do not leave most lanes idle.
+//
+// SYNC = true is the reference's synchronous iteration: a sender's message is
+// msg[s], its distance when the previous step ended (k_sssp_snap), not a
+// distance lowered earlier in this step; bounded runs use it so that m steps
+// give exactly the reference's m iterations (T must then be ~0).
+template <bool SYNC>
 __global__ void __launch_bounds__(BLOCK) k_sssp_push(DG g, const uint64_t* items, uint32_t nitems,
                                                      uint32_t* dist, uint32_t* stamp, uint32_t round,
-                                                     uint32_t T, Frontier nf, uint32_t* zc) {
+                                                     uint32_t T, Frontier nf, uint32_t* zc, const uint32_t* msg) {
   zero_ctr(zc);
   const uint32_t nw = gridDim.x * (BLOCK / WAVE);
   const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
@@ -472,7 +478,7 @@ __global__ void __launch_bounds__(BLOCK) k_sssp_push(DG g, const uint64_t* items
       s = (uint32_t)item;
       const uint32_t ch = (uint32_t)(item >> 32);
       const uint32_t c = col_of_slot(g, s);
-      du = ld_relaxed(&dist[s]);
+      du = SYNC ? msg[s] : ld_relaxed(&dist[s]);
       const uint32_t cb = g.col_ptr[c], cs = g.col_split[c];
       if (du >= T) {  // far: keep it for a later round
         if (ch == 0 && atomicMax(&stamp[s], tag) < tag) {
@@ -522,6 +528,15 @@ __global__ void __launch_bounds__(BLOCK) k_sssp_push(DG g, const uint64_t* items
     }
   }
   em_finish(em, nf);
+}
+
+// Messages of the next synchronous step: the distance each frontier vertex
+// holds now that the step that queued it has ended.
+__global__ void k_sssp_snap(const uint64_t* items, uint32_t nitems, const uint32_t* dist, uint32_t* msg) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += gridDim.x * blockDim.x) {
+    const uint32_t s = (uint32_t)items[i];
+    msg[s] = dist[s];
+  }
 }
 
 // Sink step (vertex_program_execute.hpp:131-148): every sink row takes the

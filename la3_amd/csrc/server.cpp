@@ -287,13 +287,13 @@ void worker_main(la3_server* s, la3_ctx* ctx) {
     la3_status st;
     if (s->cfg.keep_results && !out) st = LA3_ENOMEM;
     else if (h.kind == LA3_Q_BFS && k == 1)
-      st = la3_bfs_ctx(ctx, ge.g, roots[0], (uint8_t*)out, nullptr, nullptr);
+      st = la3_bfs_ctx(ctx, ge.g, roots[0], 0, (uint8_t*)out, nullptr, nullptr);
     else if (h.kind == LA3_Q_SSSP && k == 1)
-      st = la3_sssp_ctx(ctx, ge.g, roots[0], (uint32_t*)out, nullptr);
+      st = la3_sssp_ctx(ctx, ge.g, roots[0], 0, (uint32_t*)out, nullptr);
     else if (h.kind == LA3_Q_BFS)
-      st = la3_bfs_batch_ctx(ctx, ge.g, roots.data(), k, (uint8_t*)out, nullptr);
+      st = la3_bfs_batch_ctx(ctx, ge.g, roots.data(), k, 0, (uint8_t*)out, nullptr);
     else if (h.kind == LA3_Q_SSSP)
-      st = la3_sssp_batch_ctx(ctx, ge.g, roots.data(), k, (uint32_t*)out, nullptr);
+      st = la3_sssp_batch_ctx(ctx, ge.g, roots.data(), k, 0, (uint32_t*)out, nullptr);
     else
       st = la3_ppr_batch_ctx(ctx, ge.g, roots.data(), k, h.iters, s->cfg.ppr_alpha, (float*)out,
                              nullptr, nullptr);

@@ -117,8 +117,10 @@ def test_cc_labels(gpu_ctx, oracle, scale, seed):
     rec = _rmat(scale, seed)
     n = 1 << scale
     g, o = _both(gpu_ctx, oracle, rec, n, False, L.UNDIRECTED)
-    lab, _ = L.cc(g)
-    assert (lab == oracle.cc(o)[0]).all()
+    lab, it = L.cc(g)
+    elab, eit = oracle.cc(o)
+    assert (lab == elab).all() and it == eit
+    assert (L.components(g) == elab).all()
 
 
 def test_cc_many_components(gpu_ctx, oracle):
@@ -137,8 +139,10 @@ def test_cc_many_components(gpu_ctx, oracle):
     for rec in (sparse, halves, random_records(24, 3000, 1500)):
         nn = int(rec.max())
         g, o = _both(gpu_ctx, oracle, rec, nn, False, L.UNDIRECTED)
-        lab, _ = L.cc(g)
-        assert (lab == oracle.cc(o)[0]).all()
+        lab, it = L.cc(g)
+        elab, eit = oracle.cc(o)
+        assert (lab == elab).all() and it == eit
+        assert (L.components(g) == elab).all()
 
 
 @pytest.mark.parametrize("scale,seed", [(10, 9), (14, 10)])
@@ -233,7 +237,7 @@ def test_edge_cases(gpu_ctx, oracle):
         L.bfs(g, 5)
     # wrong load mode for the program
     with pytest.raises(L.LA3Error):
-        L.cc(g)
+        L.components(g)
     with pytest.raises(L.LA3Error):
         L.tc(g)
     # sink / source / isolated roots
