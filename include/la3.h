@@ -229,6 +229,21 @@ la3_status la3_bfs_ctx(la3_ctx* ctx, la3_graph* g, uint32_t root, uint32_t max_i
 la3_status la3_sssp_ctx(la3_ctx* ctx, la3_graph* g, uint32_t root, uint32_t max_iters, uint32_t* dist,
                         uint32_t* iters_run);
 
+/* la3_bfs_ctx that allocates nothing. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * traversal's work buffers (levels, marks, two frontier queues: about
+ * 21 bytes per vertex slot plus nnz/16 bytes) live in one fixed device array
+ * of 64 MiB per device, held by the library from its load to the end of the
+ * process; larger graphs return LA3_ENOTSUP. hops_dev (nslots bytes) and
+ * parent_dev (nslots u32, may be NULL) are DEVICE pointers the caller
+ * allocated; results are by original id, as la3_bfs_ctx returns them. Every
+ * call on a device shares that device's array: calls that overlap on one
+ * device (from any context) race. The array's device address is looked up
+ * when a context is created on the device. */
+la3_status la3_bfs_noalloc(la3_ctx* ctx, la3_graph* g, uint32_t root, uint32_t max_iters, uint8_t* hops_dev,
+                           uint32_t* parent_dev, uint32_t* iters_run);
+
 /* CcVertex (cc.h:22-38), execute(max_iters), run by the vertex-program
  * engine of include/la3_vertex_program.hpp with the reference's operations:
  * each row combines its messages in the reference's order with cc.h:31's

@@ -39,7 +39,7 @@ EXPORTS = [
     "la3_batch_run", "la3_session_create", "la3_session_add", "la3_session_step",
     "la3_session_poll", "la3_session_free_slots", "la3_session_destroy", "la3_ctx_release_scratch",
     "la3_result_topk", "la3_ctx_set_ingest", "la3_graph_array", "la3_components",
-    "la3_graph_device_view", "la3_ctx_stream",
+    "la3_graph_device_view", "la3_ctx_stream", "la3_bfs_noalloc",
 ]
 (TUNE_SSSP_DELTA, TUNE_BATCH_SSSP_DELTA, TUNE_BFS_TASK_K, TUNE_SSSP_TASK_K, TUNE_PPR_TASK_K,
  TUNE_FUSED_BLOCKS) = range(6)
@@ -133,6 +133,7 @@ def lib():
             "la3_components": (i32, [P, P]),
             "la3_graph_device_view": (i32, [P, P]),
             "la3_ctx_stream": (P, [P]),
+            "la3_bfs_noalloc": (i32, [P, P, u32, u32, P, P, P]),
             "la3_tc": (i32, [P, P, P]),
             "la3_ppr": (i32, [P, u32, u32, ctypes.c_double, P]),
             "la3_bfs_batch": (i32, [P, P, u32, u32, P, P]),

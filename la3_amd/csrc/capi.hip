@@ -125,6 +125,48 @@ std::string hip_error_text(hipError_t e) {
 
 }  // namespace
 
+// ---------------------------------------------- allocation-free scratch ---
+// Work buffers of la3_bfs_noalloc (levels, marks and the two frontier
+// queues), for callers that cannot allocate device memory. The array exists
+// once per device from the library's load to the end of the process; every
+// call on a device uses the same copy, so calls that overlap on one device
+// race. Host code uses the address hipGetSymbolAddress returns for the
+// device (noalloc::base), never the array's name.
+namespace noalloc {
+constexpr size_t SCRATCH_BYTES = 64ull << 20;
+constexpr int MAX_DEVICES = 64;
+static __device__ __align__(256) uint8_t scratch[SCRATCH_BYTES];
+std::mutex mu;
+void* dev_ptr[MAX_DEVICES] = {};
+bool looked_up[MAX_DEVICES] = {};
+
+// Called with `device` current (la3_ctx_create): looks the array's address
+// up once per device, checks the result and registers the block so that
+// counter copies out of it pass the range checks.
+hipError_t setup(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(mu);
+  if (looked_up[device]) return dev_ptr[device] ? hipSuccess : hipErrorInvalidSymbol;
+  looked_up[device] = true;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return e != hipSuccess ? e : hipErrorInvalidSymbol;
+  }
+  dev_ptr[device] = p;
+  std::lock_guard<std::mutex> lb(g_blocks_mu);
+  g_blocks[(uintptr_t)p] = BlockInfo{SCRATCH_BYTES, "allocation-free scratch"};
+  return hipSuccess;
+}
+
+void* base(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  return dev_ptr[device];
+}
+}  // namespace noalloc
+
 // ------------------------------------------------------------------ ctx ---
 struct DevBuf {
   void* p = nullptr;
@@ -774,6 +816,9 @@ la3_status la3_ctx_create(int device, la3_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) return LA3_EDEVICE;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return LA3_EDEVICE;
   if (hipHostMalloc(&c->host_ctr, 4096, hipHostMallocDefault) != hipSuccess) return LA3_EDEVICE;
+  // The allocation-free entry point's array: its address on this device
+  // (a failure only disables la3_bfs_noalloc, which then reports it).
+  (void)noalloc::setup(device);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->ncu = prop.multiProcessorCount;
   *out = c.release();
@@ -1094,21 +1139,15 @@ la3_status la3_bfs(la3_graph* g, uint32_t root, uint32_t max_iters, uint8_t* hop
   return la3_bfs_ctx(g->ctx, g, root, max_iters, hops, parent, iters_run);
 }
 
-la3_status la3_bfs_ctx(la3_ctx* c, la3_graph* g, uint32_t root, uint32_t max_iters, uint8_t* hops,
-                       uint32_t* parent, uint32_t* iters_run) {
-  if (!c || !g) return LA3_EINVAL;
-  if (c->device != g->ctx->device) return fail(c, LA3_EINVAL, "graph lives on another device");
-  if (root > g->n) return fail(c, LA3_EINVAL, "root id out of range");
-  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
-  HIPC(c, hipSetDevice(c->device));
-  timing_begin(c);
+}  // extern "C"
+namespace {
+
+// The BFS traversal of la3_bfs_ctx on caller-chosen buffers: level[nslots],
+// mark[nslots] and the two frontier queues F; *steps = traversal steps run.
+la3_status bfs_core(la3_ctx* c, la3_graph* g, uint32_t root, uint32_t max_iters, uint8_t* level, uint32_t* mark,
+                    const Fronts& F, uint32_t* steps) {
   const uint32_t S = g->nslots;
-  uint8_t* level = (uint8_t*)scratch(c, "bfs.level", S);
-  uint32_t* mark = (uint32_t*)scratch(c, "bfs.mark", sizeof(uint32_t) * S);
-  Fronts F;
-  la3_status st = get_fronts(c, g, F, "bfs");
-  if (st) return st;
-  if (!level || !mark) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  la3_status st;
   const DG& d = g->dg;
   const uint32_t rs = g->slot_of_id[root];
   launch(c, "bfs_init", k_bfs_init, blocks_for(S), d, level, mark, rs, F.f[0], F.f[1].nitems);
@@ -1156,14 +1195,82 @@ la3_status la3_bfs_ctx(la3_ctx* c, la3_graph* g, uint32_t root, uint32_t max_ite
     fc = nc;
     cur ^= 1;
   }
+  *steps = iter;
+  return LA3_OK;
+}
+
+}  // namespace
+extern "C" {
+
+la3_status la3_bfs_ctx(la3_ctx* c, la3_graph* g, uint32_t root, uint32_t max_iters, uint8_t* hops,
+                       uint32_t* parent, uint32_t* iters_run) {
+  if (!c || !g) return LA3_EINVAL;
+  if (c->device != g->ctx->device) return fail(c, LA3_EINVAL, "graph lives on another device");
+  if (root > g->n) return fail(c, LA3_EINVAL, "root id out of range");
+  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+  HIPC(c, hipSetDevice(c->device));
+  timing_begin(c);
+  const uint32_t S = g->nslots;
+  uint8_t* level = (uint8_t*)scratch(c, "bfs.level", S);
+  uint32_t* mark = (uint32_t*)scratch(c, "bfs.mark", sizeof(uint32_t) * S);
+  Fronts F;
+  la3_status st = get_fronts(c, g, F, "bfs");
+  if (st) return st;
+  if (!level || !mark) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  uint32_t iter = 0;
+  if ((st = bfs_core(c, g, root, max_iters, level, mark, F, &iter))) return st;
   uint8_t* oh = (uint8_t*)scratch(c, "out.u8", S);
   uint32_t* op = parent ? (uint32_t*)scratch(c, "out.u32", sizeof(uint32_t) * S) : nullptr;
   if (!oh || (parent && !op)) return fail(c, LA3_ENOMEM, "scratch allocation failed");
-  launch(c, "bfs_output", k_bfs_output, blocks_for(S), d, (const uint8_t*)level, (const uint32_t*)mark, oh, op);
+  launch(c, "bfs_output", k_bfs_output, blocks_for(S), g->dg, (const uint8_t*)level, (const uint32_t*)mark, oh, op);
   if ((st = deliver(c, hops, oh, S))) return st;
   if (parent) HIPC(c, dcopy_async(parent, op, sizeof(uint32_t) * S, hipMemcpyDeviceToHost, c->stream));
   HIPC(c, hipStreamSynchronize(c->stream));
   timing_end(c);
+  if (iters_run) *iters_run = max_iters ? max_iters : std::max<uint32_t>(iter, 1);
+  return LA3_OK;
+}
+
+la3_status la3_bfs_noalloc(la3_ctx* c, la3_graph* g, uint32_t root, uint32_t max_iters, uint8_t* hops_dev,
+                           uint32_t* parent_dev, uint32_t* iters_run) {
+  if (!c || !g || !hops_dev) return LA3_EINVAL;
+  if (c->device != g->ctx->device) return fail(c, LA3_EINVAL, "graph lives on another device");
+  if (root > g->n) return fail(c, LA3_EINVAL, "root id out of range");
+  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+  HIPC(c, hipSetDevice(c->device));
+  uint8_t* base = (uint8_t*)noalloc::base(c->device);
+  if (!base) return fail(c, LA3_EDEVICE, "allocation-free scratch: hipGetSymbolAddress failed on this device");
+  // Carve levels, marks and the two frontier queues out of the array.
+  const uint64_t S = g->nslots;
+  const uint64_t cap = S + g->nnz / CHUNK + 2;
+  auto al = [](uint64_t b) { return (b + 255) & ~255ull; };
+  const uint64_t need = al(S) + al(4 * S) + 2 * (al(8 * cap) + al(CTR_BYTES));
+  if (need > noalloc::SCRATCH_BYTES)
+    return fail(c, LA3_ENOTSUP, "la3_bfs_noalloc: the graph needs " + std::to_string(need) +
+                                    " bytes of work buffers, more than the " +
+                                    std::to_string(noalloc::SCRATCH_BYTES) + "-byte allocation-free scratch");
+  uint64_t off = 0;
+  auto take = [&](uint64_t b) { uint8_t* p = base + off; off += al(b); return p; };
+  uint8_t* level = take(S);
+  uint32_t* mark = (uint32_t*)take(4 * S);
+  Fronts F;
+  F.cap = cap;
+  for (int i = 0; i < 2; i++) {
+    F.f[i].items = (uint64_t*)take(8 * cap);
+    uint8_t* ctr = take(CTR_BYTES);
+    F.f[i].nitems = (uint32_t*)ctr;
+    F.f[i].slots = (CtrSlot*)(ctr + 64);
+  }
+  // No per-launch profiling events here (event creation allocates).
+  const bool prof = c->profiling;
+  c->profiling = false;
+  uint32_t iter = 0;
+  la3_status st = bfs_core(c, g, root, max_iters, level, mark, F, &iter);
+  if (!st) launch(c, "bfs_output", k_bfs_output, blocks_for(S), g->dg, (const uint8_t*)level, (const uint32_t*)mark,
+                  hops_dev, parent_dev);
+  c->profiling = prof;
+  if (st) return st;
+  HIPC(c, hipStreamSynchronize(c->stream));
   if (iters_run) *iters_run = max_iters ? max_iters : std::max<uint32_t>(iter, 1);
   return LA3_OK;
 }
