@@ -53,6 +53,19 @@ def build(verbose=True):
     return OUT
 
 
+def build_program(src, out):
+    """Compiles a source file of VertexProgram<P> programs
+    (include/la3_vertex_program.hpp) into a shared library linked against
+    libla3.so: the way a caller builds programs of its own for the engine."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    rel = os.path.relpath(HERE, os.path.dirname(os.path.abspath(out)))
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", f"-I{inc}", src, "-o", tmp, f"-L{HERE}", "-lla3",
+          f"-Wl,-rpath,$ORIGIN/{rel}", "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     build()
     sys.exit(0)

@@ -373,6 +373,84 @@ struct CtProg {
   bool apply(const A& y, S& s) { s.ntriangles = y; return false; }
 };
 
+// Programs written only against the vertex-program interface, with no app
+// of the reference behind them; the same operations are compiled against
+// include/la3_vertex_program.hpp in tests/vp_programs.hip. They exercise the
+// engine's paths: weights and execute_single (WDeg), execute_ with the sink
+// step and execute_non_opt (MaxLabel), apply with the iteration and an
+// order-dependent combine (Level), floating-point sums and
+// initialize(other) (PrAlpha), and a state-reading gather (CountGreater).
+struct WDegProg {
+  using M = Empty; using A = uint32_t;
+  struct S { uint32_t wdeg = 0; };
+  static constexpr bool stationary = true, gather_state = false, apply_iter = false;
+  A identity() const { return 0; }
+  bool init(uint32_t, S&) { return stationary; }
+  M scatter(const S&) { return M(); }
+  A gather(const EdgeV& e, const M&) { return e.w; }
+  void combine(const A& y1, A& y2) { y2 += y1; }
+  bool apply(const A& y, S& s) { s.wdeg = y; return true; }
+};
+
+struct MaxLabelProg {
+  using M = uint32_t; using A = uint32_t;
+  struct S { uint32_t label = 0; };
+  static constexpr bool stationary = false, gather_state = false, apply_iter = false;
+  A identity() const { return 0; }
+  bool init(uint32_t vid, S& s) { s.label = vid; return true; }
+  M scatter(const S& s) { return s.label; }
+  A gather(const EdgeV&, const M& m) { return m; }
+  void combine(const A& y1, A& y2) { y2 = std::max(y1, y2); }
+  bool apply(const A& y, S& s) { uint32_t t = s.label; s.label = std::max(s.label, y); return t != s.label; }
+};
+
+struct LevelProg {
+  using M = Empty; using A = uint32_t;
+  struct S { uint32_t level = 0xffffffffu; uint32_t via = 0; };
+  static constexpr bool stationary = false, gather_state = false, apply_iter = true;
+  uint32_t root = 0;
+  A identity() const { return 0; }
+  bool init(uint32_t vid, S& s) { if (vid == root) { s.level = 0; return true; } return false; }
+  M scatter(const S&) { return M(); }
+  A gather(const EdgeV& e, const M&) { return e.src; }
+  void combine(const A& y1, A& y2) { y2 = y1; }
+  bool apply(const A& y, S& s, uint32_t iter) {
+    if (s.level != 0xffffffffu) return false;
+    s.level = iter + 1;
+    s.via = y;
+    return true;
+  }
+};
+
+struct PrAlphaProg {
+  using M = double; using A = double;
+  struct S { uint32_t degree = 0; double rank = 0.2; };
+  static constexpr bool stationary = true, gather_state = false, apply_iter = false;
+  A identity() const { return 0.0; }
+  bool init(uint32_t, S&) { return stationary; }
+  bool init_other(uint32_t, const DegProg::S& o, S& s) { s.degree = o.degree; return true; }
+  M scatter(const S& s) { return s.rank / s.degree; }
+  A gather(const EdgeV&, const M& m) { return m; }
+  void combine(const A& y1, A& y2) { y2 += y1; }
+  bool apply(const A& y, S& s) {
+    double t = s.rank;
+    s.rank = 0.2 + 0.8 * y;
+    return std::fabs(s.rank - t) > 1e-9;
+  }
+};
+
+struct CountGreaterProg {
+  using M = uint32_t; using A = uint32_t;
+  struct S { uint32_t val = 0; uint32_t cnt = 0; };
+  static constexpr bool stationary = true, gather_state = true, apply_iter = false;
+  A identity() const { return 0; }
+  bool init(uint32_t vid, S& s) { s.val = vid % 7; return true; }
+  M scatter(const S& s) { return s.val; }
+  A gather(const EdgeV&, const M& m, const S& s) { return m > s.val ? 1u : 0u; }
+  void combine(const A& y1, A& y2) { y2 += y1; }
+  bool apply(const A& y, S& s) { s.cnt += y; s.val = y % 7; return false; }
+};
+
 // ---------------------------------------------------------------- engine ---
 template <class P>
 struct Engine {
@@ -380,6 +458,7 @@ struct Engine {
   P& p;
   const OGraph& G;
   std::vector<S> v;                                  // dashboard positions
+  std::vector<S> vm;  // mirrors read by a state-reading gather (vertex_program.hpp:159-173)
   std::vector<std::pair<uint32_t, M>> xreg, xsrc;    // StreamingArray (pushes ascending)
   std::vector<A> yreg, ysink;
   std::vector<uint8_t> treg, tsink;                  // activity bitvectors
@@ -394,6 +473,16 @@ struct Engine {
 
   uint32_t vid_of(uint32_t pos) const { return (uint32_t)G.h.unhash(G.hid_of[pos]); }
 
+  // Mirrors are filled once, at initialize, with the active states: all
+  // of them for a stationary program (activate_all), none otherwise (init
+  // activates no vertex, vertex_program.hpp:103-104), whose mirrors keep S().
+  void mirror() {
+    if constexpr (P::gather_state) {
+      vm.assign(G.nslots, S());
+      if (P::stationary) vm = v;
+    }
+  }
+
   // vertex_program.hpp:84-157
   void initialize() {
     xreg.clear(); xsrc.clear();
@@ -407,6 +496,7 @@ struct Engine {
       }
     }
     for (uint32_t i = G.nreg + G.nsink + G.nsrc; i < G.nslots; i++) p.init(vid_of(i), v[i]);
+    mirror();
   }
 
   // vertex_program.hpp:417-491: positions swap sink(G) <-> source(G reversed).
@@ -426,6 +516,7 @@ struct Engine {
       }
     }
     for (uint32_t i = G.nreg + G.nsink + G.nsrc; i < G.nslots; i++) p.init_other(vid_of(i), o[i], v[i]);
+    mirror();
   }
 
   // vertex_program_execute.hpp:564-603
@@ -437,7 +528,7 @@ struct Engine {
         const Entry& e = csc.entries[j];
         EdgeV ev{csc.colidxs[c], e.idx, e.w};
         if constexpr (P::gather_state)
-          p.combine(p.gather(ev, im.second, v[mirror_base + e.global_idx]), y[e.global_idx]);
+          p.combine(p.gather(ev, im.second, vm[mirror_base + e.global_idx]), y[e.global_idx]);
         else
           p.combine(p.gather(ev, im.second), y[e.global_idx]);
         t[e.global_idx] = 1;
@@ -691,6 +782,74 @@ int ora_tc(void* g, void* gr, uint64_t* total, uint32_t* per_vertex) {
     if (per_vertex) per_vertex[ce.vid_of(i)] = ce.v[i].ntriangles;
   }
   *total = t;
+  return 0;
+}
+
+// The interface-only programs above; per-vertex outputs by original id.
+int ora_vp_wdeg(void* h, uint32_t max_iters, uint32_t* wdeg, uint32_t* iters_run) {
+  const OGraph& G = *((Handle*)h)->G;
+  WDegProg p;
+  Engine<WDegProg> e(p, G);
+  e.initialize();
+  e.execute(max_iters);
+  for (uint32_t i = 0; i < G.nslots; i++) wdeg[e.vid_of(i)] = e.v[i].wdeg;
+  if (iters_run) *iters_run = e.iters_run;
+  return 0;
+}
+
+int ora_vp_maxlabel(void* h, uint32_t max_iters, uint32_t* label, uint32_t* iters_run) {
+  const OGraph& G = *((Handle*)h)->G;
+  MaxLabelProg p;
+  Engine<MaxLabelProg> e(p, G);
+  e.initialize();
+  e.execute(max_iters);
+  for (uint32_t i = 0; i < G.nslots; i++) label[e.vid_of(i)] = e.v[i].label;
+  if (iters_run) *iters_run = e.iters_run;
+  return 0;
+}
+
+int ora_vp_level(void* h, uint32_t root, uint32_t max_iters, uint32_t* level, uint32_t* via, uint32_t* iters_run) {
+  const OGraph& G = *((Handle*)h)->G;
+  LevelProg p;
+  p.root = root;
+  Engine<LevelProg> e(p, G);
+  e.initialize();
+  e.execute(max_iters);
+  for (uint32_t i = 0; i < G.nslots; i++) {
+    level[e.vid_of(i)] = e.v[i].level;
+    via[e.vid_of(i)] = e.v[i].via;
+  }
+  if (iters_run) *iters_run = e.iters_run;
+  return 0;
+}
+
+int ora_vp_pralpha(void* g, void* gr, uint32_t max_iters, double* rank, uint32_t* iters_run) {
+  const OGraph& G = *((Handle*)g)->G;
+  const OGraph& GR = *((Handle*)gr)->G;
+  DegProg dp;
+  Engine<DegProg> de(dp, GR);
+  de.initialize();
+  de.execute(1);
+  PrAlphaProg p;
+  Engine<PrAlphaProg> e(p, G);
+  e.initialize_other(de.v);
+  e.execute(max_iters);
+  for (uint32_t i = 0; i < G.nslots; i++) rank[e.vid_of(i)] = e.v[i].rank;
+  if (iters_run) *iters_run = e.iters_run;
+  return 0;
+}
+
+int ora_vp_countgreater(void* h, uint32_t max_iters, uint32_t* val, uint32_t* cnt, uint32_t* iters_run) {
+  const OGraph& G = *((Handle*)h)->G;
+  CountGreaterProg p;
+  Engine<CountGreaterProg> e(p, G);
+  e.initialize();
+  e.execute(max_iters);
+  for (uint32_t i = 0; i < G.nslots; i++) {
+    val[e.vid_of(i)] = e.v[i].val;
+    cnt[e.vid_of(i)] = e.v[i].cnt;
+  }
+  if (iters_run) *iters_run = e.iters_run;
   return 0;
 }
 

@@ -49,6 +49,11 @@ def lib():
         L.ora_sssp.argtypes = [P, u32, u32, P, P]
         L.ora_cc.argtypes = [P, u32, P, P]
         L.ora_tc.argtypes = [P, P, P, P]
+        L.ora_vp_wdeg.argtypes = [P, u32, P, P]
+        L.ora_vp_maxlabel.argtypes = [P, u32, P, P]
+        L.ora_vp_level.argtypes = [P, u32, u32, P, P, P]
+        L.ora_vp_pralpha.argtypes = [P, P, u32, P, P]
+        L.ora_vp_countgreater.argtypes = [P, u32, P, P, P]
         _lib = L
     return _lib
 
@@ -154,3 +159,43 @@ def tc(g_acyclic, gr_acyclic):
     per = np.zeros(g_acyclic.nslots, dtype=np.uint32)
     lib().ora_tc(g_acyclic.h, gr_acyclic.h, _ptr(total), _ptr(per))
     return int(total[0]), per
+
+
+# --------------------------------------- programs of the interface alone ---
+def _u32(n):
+    return np.zeros(n, dtype=np.uint32)
+
+
+def vp_wdeg(g, max_iters=1):
+    """Weighted in-degree (stationary; W = u32 weight)."""
+    out, it = _u32(g.nslots), _u32(1)
+    lib().ora_vp_wdeg(g.h, int(max_iters), _ptr(out), _ptr(it))
+    return out, int(it[0])
+
+
+def vp_maxlabel(g, max_iters=0):
+    """Largest id reached by max-label propagation."""
+    out, it = _u32(g.nslots), _u32(1)
+    lib().ora_vp_maxlabel(g.h, int(max_iters), _ptr(out), _ptr(it))
+    return out, int(it[0])
+
+
+def vp_level(g, root, max_iters=0):
+    """Level (iteration of first reach, ~0 unreached) and the last-combined sender."""
+    lev, via, it = _u32(g.nslots), _u32(g.nslots), _u32(1)
+    lib().ora_vp_level(g.h, int(root), int(max_iters), _ptr(lev), _ptr(via), _ptr(it))
+    return lev, via, int(it[0])
+
+
+def vp_pralpha(g, gr, max_iters):
+    """PageRank with alpha 0.2 and tolerance 1e-9, degrees via initialize(other)."""
+    rank, it = np.zeros(g.nslots, dtype=np.float64), _u32(1)
+    lib().ora_vp_pralpha(g.h, gr.h, int(max_iters), _ptr(rank), _ptr(it))
+    return rank, int(it[0])
+
+
+def vp_countgreater(g, max_iters):
+    """State-reading gather: counts messages larger than the mirrored value."""
+    val, cnt, it = _u32(g.nslots), _u32(g.nslots), _u32(1)
+    lib().ora_vp_countgreater(g.h, int(max_iters), _ptr(val), _ptr(cnt), _ptr(it))
+    return val, cnt, int(it[0])

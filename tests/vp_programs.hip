@@ -1,0 +1,232 @@
+// Vertex programs written only against include/la3_vertex_program.hpp (the
+// reference's VertexProgram<W, M, A, S> interface, vertex_program.h:21-137),
+// compiled outside the library as a caller would, for tests/test_gpu_vp.py.
+// None of them is one of the library's programs; the oracle holds the same
+// operations (oracle/la3_oracle.cpp: WDegProg, MaxLabelProg, LevelProg,
+// PrAlphaProg, CountGreaterProg). Each entry point returns 0 or a la3_status
+// and writes per-vertex outputs by original id.
+#include <cstdio>
+#include <string>
+
+#include "la3_vertex_program.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+// Weighted in-degree: stationary, weights, execute_single.
+struct WDeg {
+  using W = uint32_t;
+  using M = la3::Empty;
+  using A = uint32_t;
+  struct S {
+    uint32_t wdeg = 0;
+  };
+  __device__ A gather(const la3::Edge<W>& e, const M&) const { return e.weight; }
+  __device__ void combine(const A& y1, A& y2) const { y2 += y1; }
+  __device__ bool apply(const A& y, S& s) const {
+    s.wdeg = y;
+    return true;
+  }
+};
+
+// Max-label propagation: execute_ with the sink step on directed loads,
+// execute_non_opt on undirected ones.
+struct MaxLabel {
+  using W = la3::Empty;
+  using M = uint32_t;
+  using A = uint32_t;
+  struct S {
+    uint32_t label = 0;
+  };
+  __device__ bool init(uint32_t vid, S& s) const {
+    s.label = vid;
+    return true;
+  }
+  __device__ M scatter(const S& s) const { return s.label; }
+  __device__ A gather(const la3::Edge<W>&, const M& m) const { return m; }
+  __device__ void combine(const A& y1, A& y2) const { y2 = max(y1, y2); }
+  __device__ bool apply(const A& y, S& s) const {
+    const uint32_t t = s.label;
+    s.label = max(s.label, y);
+    return t != s.label;
+  }
+};
+
+// First-reach level with the last combined sender: apply reads the
+// iteration, and the combine keeps the last message, so the result depends
+// on the order in which a row combines its messages.
+struct Level {
+  using W = la3::Empty;
+  using M = la3::Empty;
+  using A = uint32_t;
+  struct S {
+    uint32_t level = 0xffffffffu;
+    uint32_t via = 0;
+  };
+  uint32_t root = 0;
+  __device__ bool init(uint32_t vid, S& s) const {
+    if (vid != root) return false;
+    s.level = 0;
+    return true;
+  }
+  __device__ A gather(const la3::Edge<W>& e, const M&) const { return e.src; }
+  __device__ void combine(const A& y1, A& y2) const { y2 = y1; }
+  __device__ bool apply(const A& y, S& s, uint32_t iter) const {
+    if (s.level != 0xffffffffu) return false;
+    s.level = iter + 1;
+    s.via = y;
+    return true;
+  }
+};
+
+// Degrees on the reversed load, handed to PrAlpha through initialize(other).
+struct Deg {
+  using W = la3::Empty;
+  using M = la3::Empty;
+  using A = uint32_t;
+  struct S {
+    uint32_t degree = 0;
+  };
+  __device__ A gather(const la3::Edge<W>&, const M&) const { return 1; }
+  __device__ void combine(const A& y1, A& y2) const { y2 += y1; }
+  __device__ bool apply(const A& y, S& s) const {
+    s.degree = y;
+    return true;
+  }
+};
+
+struct PrAlpha {
+  using W = la3::Empty;
+  using M = double;
+  using A = double;
+  struct S {
+    uint32_t degree = 0;
+    double rank = 0.2;
+  };
+  __device__ bool init(uint32_t, const Deg::S& o, S& s) const {
+    s.degree = o.degree;
+    return true;
+  }
+  __device__ M scatter(const S& s) const { return s.rank / s.degree; }
+  __device__ A gather(const la3::Edge<W>&, const M& m) const { return m; }
+  __device__ void combine(const A& y1, A& y2) const { y2 += y1; }
+  __device__ bool apply(const A& y, S& s) const {
+    const double t = s.rank;
+    s.rank = 0.2 + 0.8 * y;
+    return fabs(s.rank - t) > 1e-9;
+  }
+};
+
+// A gather that reads the row's state (its mirror as initialize left it).
+struct CountGreater {
+  using W = la3::Empty;
+  using M = uint32_t;
+  using A = uint32_t;
+  struct S {
+    uint32_t val = 0;
+    uint32_t cnt = 0;
+  };
+  __device__ bool init(uint32_t vid, S& s) const {
+    s.val = vid % 7;
+    return true;
+  }
+  __device__ M scatter(const S& s) const { return s.val; }
+  __device__ A gather(const la3::Edge<W>&, const M& m, const S& s) const { return m > s.val ? 1u : 0u; }
+  __device__ void combine(const A& y1, A& y2) const { y2 += y1; }
+  __device__ bool apply(const A& y, S& s) const {
+    s.cnt += y;
+    s.val = y % 7;
+    return false;
+  }
+};
+
+template <class F>
+int guarded(F f) {
+  try {
+    f();
+    return 0;
+  } catch (const la3::Error& e) {
+    g_err = e.what();
+    return (int)e.status;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return (int)LA3_EDEVICE;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vp_error() { return g_err.c_str(); }
+
+int vp_wdeg(la3_ctx* c, la3_graph* g, uint32_t max_iters, uint32_t* wdeg, uint32_t* iters) {
+  return guarded([&] {
+    la3::VertexProgram<WDeg> vp(c, g, true);
+    vp.initialize();
+    vp.execute(max_iters);
+    auto st = vp.states();
+    for (size_t i = 0; i < st.size(); i++) wdeg[i] = st[i].wdeg;
+    *iters = vp.iterations();
+  });
+}
+
+int vp_maxlabel(la3_ctx* c, la3_graph* g, uint32_t max_iters, uint32_t* label, uint32_t* iters) {
+  return guarded([&] {
+    la3::VertexProgram<MaxLabel> vp(c, g);
+    vp.initialize();
+    vp.execute(max_iters);
+    auto st = vp.states();
+    for (size_t i = 0; i < st.size(); i++) label[i] = st[i].label;
+    *iters = vp.iterations();
+  });
+}
+
+int vp_level(la3_ctx* c, la3_graph* g, uint32_t root, uint32_t max_iters, uint32_t* level, uint32_t* via,
+             uint32_t* iters) {
+  return guarded([&] {
+    Level ops;
+    ops.root = root;
+    la3::VertexProgram<Level> vp(c, g, false, ops);
+    vp.initialize();
+    vp.execute(max_iters);
+    auto st = vp.states();
+    for (size_t i = 0; i < st.size(); i++) {
+      level[i] = st[i].level;
+      via[i] = st[i].via;
+    }
+    *iters = vp.iterations();
+  });
+}
+
+// g: directed load, gr: the same file reversed (as pr.cpp:15-27 loads them).
+int vp_pralpha(la3_ctx* c, la3_graph* g, la3_graph* gr, uint32_t max_iters, double* rank, uint32_t* iters) {
+  return guarded([&] {
+    la3::VertexProgram<Deg> deg(c, gr, true);
+    deg.initialize();
+    deg.execute(1);
+    la3::VertexProgram<PrAlpha> vp(c, g, true);
+    vp.initialize(deg);
+    vp.execute(max_iters);
+    auto st = vp.states();
+    for (size_t i = 0; i < st.size(); i++) rank[i] = st[i].rank;
+    *iters = vp.iterations();
+  });
+}
+
+int vp_countgreater(la3_ctx* c, la3_graph* g, uint32_t max_iters, uint32_t* val, uint32_t* cnt, uint32_t* iters) {
+  return guarded([&] {
+    la3::VertexProgram<CountGreater> vp(c, g, true);
+    vp.initialize();
+    vp.execute(max_iters);
+    auto st = vp.states();
+    for (size_t i = 0; i < st.size(); i++) {
+      val[i] = st[i].val;
+      cnt[i] = st[i].cnt;
+    }
+    *iters = vp.iterations();
+  });
+}
+
+}  // extern "C"
