@@ -1,25 +1,36 @@
 #!/usr/bin/env python3
 """Benchmark of the vertex-program hot path on one MI355X.
 
-Default workload (BFS + SSSP on a directed R-MAT graph): scale 22 (4,194,304
-vertices), edge factor 16, a=0.57 b=0.19 c=0.19, seed 2, u32 weights in
-[1, 128]; 64 roots drawn by seed from the vertices with out-degree > 0. One
-step = the 64 BFS queries then the 64 SSSP queries. Two paths are timed in
-the same run on the same roots: batched (all roots in one traversal per
-program; this is `value`) and single-query (one call of the plain GPU vertex
-program per root; reported under single_query). Before timing, every root's
-batched result is checked equal to its single-query result.
+Default workload (the batched-queries configuration): a directed R-MAT graph
+of scale 24 (16,777,216 vertices), edge factor 16, a=0.57 b=0.19 c=0.19,
+seed 4, u32 weights in [1, 128]; 256 BFS roots, 256 SSSP roots and 128
+personalized-PageRank seeds (30 iterations, FP32 ranks with compensated
+sums) drawn by seed from the vertices with out-degree > 0. One step = all 640
+queries as ONE fused run (one fused launch per traversal step, steps replayed
+from a captured hipGraph; results stay in HBM). Timed in the same run on the
+same queries: the per-family batched calls one after the other, and the
+plain GPU vertex program one query at a time. Before timing, every BFS/SSSP
+query's batched and fused results are checked equal to its single-query
+result, and PPR is checked against the FP64 single-query program.
 
 Prints one JSON line:
-  value       GTEPS = sum over queries of the out-degrees of the vertices the
-              query reached, divided by the timed wall time;
-  roofline    the dominant kernel's algorithmic bytes over its summed
-              HIP-event time (events recorded on the library's stream),
-              against 8 TB/s;
-  cpu_baseline the oracle (CPU restatement of the reference's algorithm,
-              one thread) on a sample of the same roots of the same graph.
+  value        GTEPS = sum over queries of the entries traversed (out-degrees
+               of the vertices a BFS/SSSP query reached; nnz of regular rows
+               x iterations for PPR), divided by the timed wall time;
+  roofline     the fused launch's algorithmic bytes (the compulsory traffic of
+               each vertex-program step: (4 + w) E_scan + (8 + m) C_act +
+               (a + s_r + s_w) R_touch + m R_act, per-vertex terms scaled by
+               the live queries, R_touch counted on the device) over its summed
+               HIP-event time on the library stream, against 8 TB/s; traffic =
+               HBM bytes per launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs;
+  cpu_baseline the oracle (CPU restatement of the reference's algorithm) on a
+               sample of the same BFS/SSSP queries on the same graph, one
+               query per thread on the host's cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Other workloads (--workload): bfs_sssp_s22 (64 BFS + 64 SSSP roots, scale
+22), pr_s16 (PageRank), cc_tc_s20 (CC and TC), serve (serving loop).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
 import argparse
 import json
@@ -72,16 +83,23 @@ def cpu_cores():
     return max(1, min(16, n))
 
 
-def cpu_baseline(rec, n, roots, outdeg, gpu_hops, gpu_dist):
-    """The oracle (CPU restatement of the reference's np=1 path) on `roots`,
-    one query per thread across the host cores (queries are independent, as
-    the reference's np x tpp ranks and threads would split them); also checks
-    that the GPU results for those roots equal the oracle's."""
-    from concurrent.futures import ThreadPoolExecutor
+def cpu_baseline(rec, n, roots, outdeg, gpu_hops, gpu_dist, scale):
+    """The oracle (CPU restatement of the reference's np=1 path) on `roots`
+    (each as a BFS and as an SSSP query) on the graph of `rec`."""
     from oracle import oracle as O
     t = time.time()
     o = O.OracleGraph(rec, n, True, O.DIRECTED)
     log(f"oracle ingest {time.time() - t:.1f}s (not timed)")
+    return cpu_baseline_on(o, outdeg, roots, gpu_hops, gpu_dist, scale)
+
+
+def cpu_baseline_on(o, outdeg, roots, gpu_hops, gpu_dist, scale):
+    """The oracle on `roots` (each as a BFS and as an SSSP query), one query
+    per thread across the host cores (queries are independent, as the
+    reference's np x tpp ranks and threads would split them); also checks
+    that the GPU results for those roots equal the oracle's."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
     cores = cpu_cores()
 
     def one(job):
@@ -100,8 +118,8 @@ def cpu_baseline(rec, n, roots, outdeg, gpu_hops, gpu_dist):
     edges = sum(e for e, _ in out)
     exact = all(x for _, x in out)
     return dict(value=edges / secs / 1e9, unit="GTEPS", cores=cores, kind="port",
-                sample=f"{len(roots)} BFS + {len(roots)} SSSP roots of the step, full scale-22 graph, "
-                       f"oracle (CPU restatement), one query per thread on {cores} threads, "
+                sample=f"{len(roots)} BFS + {len(roots)} SSSP queries (the same roots) on the full scale-{scale} "
+                       f"graph, oracle (CPU restatement), one query per thread on {cores} threads, "
                        f"{secs:.1f} s wall",
                 exact_match=exact)
 
@@ -252,44 +270,69 @@ def workload_cc_tc_s20(args, L):
 
 
 def workload_batched_s24(args, L):
-    """256 BFS + 256 SSSP + 128 PPR (30 iterations) on R-MAT scale 24 seed 4,
-    batched vs one query at a time."""
+    """256 BFS + 256 SSSP + 128 PPR (30 iterations) on R-MAT scale 24 seed 4:
+    one fused run (value) vs the per-family batched calls vs one query at a
+    time, with the oracle on a sample of the queries as the CPU baseline."""
+    from concurrent.futures import ThreadPoolExecutor
     scale, seed, n = 24, 4, 1 << 24
+    t = time.time()
     rec = L.rmat(scale, 16, 0.57, 0.19, 0.19, seed, weighted=True)
+    log(f"R-MAT scale {scale}: {rec.shape[0]} edges in {time.time() - t:.1f}s")
     outdeg = np.bincount(rec[:, 0], minlength=n + 1)
+    # The oracle's graph for the CPU baseline builds on one host core while
+    # the GPU legs run (not timed).
+    pool = ThreadPoolExecutor(max_workers=1)
+    oracle_graph = None
+    if not args.no_cpu_baseline:
+        def build_oracle():
+            from oracle import oracle as O
+            t0 = time.time()
+            o = O.OracleGraph(rec, n, True, O.DIRECTED)
+            log(f"oracle ingest {time.time() - t0:.1f}s (background, not timed)")
+            return o
+        oracle_graph = pool.submit(build_oracle)
     ctx = L.Context(0)
+    t = time.time()
     g = L.Graph.from_records(ctx, rec, n, True, L.DIRECTED)
-    log(f"scale 24 loaded: {g.info}")
+    log(f"ingest {time.time() - t:.1f}s: {g.info}")
     bfs_r, sssp_r, ppr_s = choose_roots(rec, 256, 41), choose_roots(rec, 256, 42), choose_roots(rec, 128, 43)
+    t = time.time()
     tune = g.autotune(sssp_r)
-    log(f"autotune: {tune}")
+    log(f"autotune ({time.time() - t:.1f}s, not timed): {tune}")
     D = L.DEVICE
+    t = time.time()
     hb, _ = L.bfs_batch(g, bfs_r)
     db, _ = L.sssp_batch(g, sssp_r)
     pb, _, resid = L.ppr_batch(g, ppr_s, 30)
     edges = 0
+    sample = sssp_r[:args.cpu_roots]
+    gpu_hops, gpu_dist = {}, {}
     for q, r in enumerate(bfs_r):
         h1 = L.bfs(g, r, parents=False)[0]
-        assert (hb[q] == h1).all()
+        assert (hb[q] == h1).all(), f"batched BFS differs from single-query for root {r}"
         edges += traversed_edges(outdeg, h1 != L.BFS_INF)
     for q, r in enumerate(sssp_r):
         d1 = L.sssp(g, r)[0]
-        assert (db[q] == d1).all()
+        assert (db[q] == d1).all(), f"batched SSSP differs from single-query for root {r}"
         edges += traversed_edges(outdeg, d1 != L.SSSP_INF)
+        if r in sample:
+            gpu_dist[r] = d1
+            gpu_hops[r] = L.bfs(g, r, parents=False)[0]
     l1s, rels = [], []
     for q in range(0, 128, 16):
         r64 = L.ppr(g, ppr_s[q], 30)
         l1s.append(float(np.abs(pb[q] - r64).sum()))
         nz = r64 > 0
         rels.append(float((np.abs(pb[q] - r64)[nz] / r64[nz]).max()))
-    assert max(l1s) <= 1e-7
+    assert max(l1s) <= 1e-7, f"PPR FP32+compensation vs FP64: L1 {max(l1s)} > 1e-7"
     ppr_edges = g.info["nnz_regular"] * 30
     edges += ppr_edges * 128
     mix = [(g, L.Q_BFS, bfs_r), (g, L.Q_SSSP, sssp_r), (g, L.Q_PPR, ppr_s, 30)]
     fr = L.batch_run(ctx, mix)
     assert (fr[0]["out"] == hb).all() and (fr[1]["out"] == db).all() and (fr[2]["out"] == pb).all(), \
         "fused run differs from the per-family batched calls"
-    log("exact: fused run == per-family batched calls for all 640 queries")
+    log(f"exact ({time.time() - t:.1f}s): fused run == per-family batched calls == single-query for all "
+        f"512 BFS/SSSP queries; PPR L1 vs FP64 {max(l1s):.2e}")
     del hb, db, pb, fr
 
     def fused(k):
@@ -311,33 +354,54 @@ def workload_batched_s24(args, L):
             for s_ in ppr_s:
                 L.ppr(g, s_, 30, out=D)
 
-    fused(1)
-    el_f, _, _ = timed(ctx, fused, args.steps, profile=False)
+    for _ in range(max(args.warmup, 1)):
+        fused(1)
+    el_f, lat_f, _ = timed(ctx, lambda k: [(time.perf_counter(), fused(1), time.perf_counter()) for _ in range(k)],
+                           args.steps, profile=False)
+    step_ms = [(b - a) * 1e3 for a, _, b in lat_f]
     log(f"fused: {el_f * 1e3 / args.steps:.1f} ms/step, {edges / (el_f / args.steps) / 1e9:.1f} GTEPS")
-    el_fp, _, st_f = timed(ctx, fused, args.steps)
-    log(f"fused, profiled launches: {el_fp * 1e3 / args.steps:.1f} ms/step; kernels {json.dumps(kernel_table(st_f))}")
-    el_b, _, st_b = timed(ctx, batched, args.steps)
-    log(f"batched (sequential): {el_b * 1e3 / args.steps:.1f} ms/step; kernels {json.dumps(kernel_table(st_b))}")
-    el_s, _, st_s = timed(ctx, single, 1)
-    return base_line(args, edges * args.steps / el_f / 1e9, "GTEPS", el_f * 1e3 / args.steps, "u32/f32",
-                     "256 BFS + 256 SSSP + 128 PPR(30 it, FP32+comp), directed R-MAT scale 24 ef 16 seed 4, "
-                     "u32 weights [1,128]; all 640 queries in one fused run, one fused launch per step, "
-                     "steps replayed from a captured hipGraph",
-                     queries_per_s=round(640 * args.steps / el_f, 2),
-                     batched_sequential={"value": round(edges * args.steps / el_b / 1e9, 4), "unit": "GTEPS",
-                                         "ms_per_step": round(el_b * 1e3 / args.steps, 3),
-                                         "roofline": roofline(st_b), "kernels": kernel_table(st_b)},
-                     single_query={"value": round(edges / el_s / 1e9, 4), "unit": "GTEPS",
-                                   "queries_per_s": round(640 / el_s, 2), "ms": round(el_s * 1e3, 1),
-                                   "roofline": roofline(st_s)},
-                     speedup_vs_single=round(el_s * args.steps / el_f, 2),
-                     ppr_l1_vs_fp64=max(l1s), ppr_max_rel=max(rels),
-                     ppr_residual_last_iter=float(np.max(resid)),
-                     roofline=roofline(st_f), kernels=kernel_table(st_f),
-                     autotune={"sssp_delta": tune["sssp_delta"], "batch_sssp_delta": tune["batch_sssp_delta"],
-                               "task_k": tune["task_k"], "fused_blocks_per_cu": tune["fused_blocks"],
-                               "task_ms_64_128_256": [[round(x, 3) for x in r] for r in tune["task_ms"]],
-                               "fused_ms_8_16_32": [round(x, 3) for x in tune["fused_ms"]]})
+    # the same steps as separate launches with per-kernel events and the
+    # device's touched-row counts (roofline)
+    el_fp, _, st_f = timed(ctx, fused, 1)
+    log(f"fused, profiled launches: {el_fp * 1e3:.1f} ms/step; kernels {json.dumps(kernel_table(st_f))}")
+    el_b, _, st_b = timed(ctx, batched, args.steps, profile=False)
+    log(f"batched (sequential): {el_b * 1e3 / args.steps:.1f} ms/step")
+    single(0)
+    el_s, _, _ = timed(ctx, single, 1, profile=False)
+    log(f"single query: {el_s * 1e3:.1f} ms for 640 queries")
+    copy_gbs = ctx.copy_bandwidth(1 << 30, 10)
+    log(f"device copy bandwidth {copy_gbs:.0f} GB/s (1 GiB buffer, read + write)")
+    out = base_line(args, edges * args.steps / el_f / 1e9, "GTEPS", el_f * 1e3 / args.steps, "u32/f32",
+                    "batched queries: 256 BFS + 256 SSSP + 128 PPR (30 iterations, FP32 + compensated sums), "
+                    "directed R-MAT scale 24 ef 16 (a=.57 b=.19 c=.19) seed 4, u32 weights [1,128]; all 640 "
+                    "queries in one fused run per step, one fused launch per traversal step, steps replayed "
+                    "from a captured hipGraph, results left in HBM",
+                    queries_per_s=round(640 * args.steps / el_f, 2),
+                    p50_ms=round(float(np.percentile(step_ms, 50)), 3),
+                    p99_ms=round(float(np.percentile(step_ms, 99)), 3),
+                    latency_note="a query's latency is its fused run's wall time (all 640 finish together)",
+                    batched_sequential={"value": round(edges * args.steps / el_b / 1e9, 4), "unit": "GTEPS",
+                                        "ms_per_step": round(el_b * 1e3 / args.steps, 3),
+                                        "note": "la3_bfs_batch, la3_sssp_batch, la3_ppr_batch one after another"},
+                    single_query={"value": round(edges / el_s / 1e9, 4), "unit": "GTEPS",
+                                  "queries_per_s": round(640 / el_s, 2), "ms": round(el_s * 1e3, 1),
+                                  "note": "the plain GPU vertex program, one query per call (FP64 PPR)"},
+                    speedup_vs_single=round(el_s * args.steps / el_f, 2),
+                    ppr_l1_vs_fp64=max(l1s), ppr_max_rel=max(rels),
+                    ppr_residual_last_iter=float(np.max(resid)),
+                    roofline=roofline(st_f, "batched_s24"), kernels=kernel_table(st_f),
+                    device_copy_gbs=round(copy_gbs, 1),
+                    autotune={"sssp_delta": tune["sssp_delta"], "batch_sssp_delta": tune["batch_sssp_delta"],
+                              "task_k": tune["task_k"], "fused_blocks_per_cu": tune["fused_blocks"],
+                              "task_ms_64_128_256": [[round(x, 3) for x in r] for r in tune["task_ms"]],
+                              "fused_ms_8_16_32": [round(x, 3) for x in tune["fused_ms"]]})
+    out["config"]["graph_edges"] = int(g.info["nnz"])
+    out["config"]["queries_per_step"] = 640
+    if oracle_graph is not None:
+        o = oracle_graph.result()
+        out["cpu_baseline"] = cpu_baseline_on(o, outdeg, sample, gpu_hops, gpu_dist, scale)
+    pool.shutdown()
+    return out
 
 
 def workload_serve(args, L):
@@ -427,36 +491,8 @@ def workload_serve(args, L):
                                    "p99_ms": round(float(np.percentile(lat_s, 99)), 3)})
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--scale", type=int, default=22)
-    ap.add_argument("--roots", type=int, default=64)
-    ap.add_argument("--cpu-roots", type=int, default=32)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-single", action="store_true", help="skip the single-query baseline")
-    ap.add_argument("--workload", default="bfs_sssp_s22",
-                    choices=["bfs_sssp_s22", "pr_s16", "cc_tc_s20", "batched_s24", "serve"])
-    ap.add_argument("--qps", type=float, default=400.0)
-    ap.add_argument("--serve-queries", type=int, default=2000)
-    ap.add_argument("--serve-batch", type=int, default=64)
-    ap.add_argument("--serve-workers", type=int, default=1,
-                    help="workers (sessions) of the fused serving mode")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the whole graph is one rank's tile set on one "
-              "MI355X and this build has no multi-GPU path. Run with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
-
-    import la3_amd as L
-    other = {"pr_s16": workload_pr_s16, "cc_tc_s20": workload_cc_tc_s20,
-             "batched_s24": workload_batched_s24, "serve": workload_serve}
-    if args.workload in other:
-        print(json.dumps(other[args.workload](args, L)), flush=True)
-        return
-
+def workload_bfs_sssp_s22(args, L):
+    """64 BFS + 64 SSSP roots on R-MAT scale 22 seed 2: fused run vs batched vs single-query."""
     scale, seed = args.scale, 2
     n = 1 << scale
     t = time.time()
@@ -530,7 +566,7 @@ def main():
             "queries_per_s": round(2 * k * args.steps / el_s, 2),
             "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
             "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
-            "roofline": roofline(st_s),
+            "kernels": kernel_table(st_s),
         }
         log(f"single: {el_s * 1e3 / args.steps:.1f} ms/step, {gteps_s:.2f} GTEPS; kernels {json.dumps(kernel_table(st_s))}")
 
@@ -570,7 +606,7 @@ def main():
         "batched_sequential": {"value": round(gteps_b, 4), "unit": "GTEPS",
                                "ms_per_step": round(el_b * 1e3 / args.steps, 3),
                                "note": "la3_bfs_batch then la3_sssp_batch (host-driven steps)",
-                               "roofline": roofline(st_b, "bfs_sssp_s22")},
+                               "kernels": kernel_table(st_b)},
         "single_query": single,
         "speedup_vs_single": round(gteps_f / single["value"], 2) if single else None,
         "roofline": roofline(st_f, "bfs_sssp_s22"),
@@ -584,8 +620,38 @@ def main():
                      "fused_ms_8_16_32": [round(x, 3) for x in tune["fused_ms"]]},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(rec, n, roots[:args.cpu_roots], outdeg, hops, dist)
-    print(json.dumps(out), flush=True)
+        out["cpu_baseline"] = cpu_baseline(rec, n, roots[:args.cpu_roots], outdeg, hops, dist, scale)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--roots", type=int, default=64)
+    ap.add_argument("--cpu-roots", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-single", action="store_true", help="skip the single-query baseline")
+    ap.add_argument("--workload", default="batched_s24",
+                    choices=["bfs_sssp_s22", "pr_s16", "cc_tc_s20", "batched_s24", "serve"])
+    ap.add_argument("--qps", type=float, default=400.0)
+    ap.add_argument("--serve-queries", type=int, default=2000)
+    ap.add_argument("--serve-batch", type=int, default=64)
+    ap.add_argument("--serve-workers", type=int, default=1,
+                    help="workers (sessions) of the fused serving mode")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the whole graph is one rank's tile set on one "
+              "MI355X and this build has no multi-GPU path. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import la3_amd as L
+    work = {"batched_s24": workload_batched_s24, "bfs_sssp_s22": workload_bfs_sssp_s22,
+            "pr_s16": workload_pr_s16, "cc_tc_s20": workload_cc_tc_s20, "serve": workload_serve}
+    print(json.dumps(work[args.workload](args, L)), flush=True)
+
 
 
 if __name__ == "__main__":

@@ -2356,6 +2356,14 @@ la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t
     uint32_t* rs = (uint32_t*)buf("roots", sizeof(uint32_t) * k);
     T.stamp = (uint32_t*)buf("stamp", sizeof(uint32_t) * S);
     if (!rs || !T.stamp) return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
+    // Measurement runs (profiling on) count the rows that receive a message
+    // in each step, for the algorithmic-byte accounting.
+    T.tstamp = nullptr;
+    if (c->profiling) {
+      T.tstamp = (uint32_t*)buf("tstamp", sizeof(uint32_t) * S);
+      if (!T.tstamp) return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
+      launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(S), T.tstamp, (uint64_t)S, 0u);
+    }
     const uint64_t qcap = (uint64_t)S + g->nnz / CHUNK + 2;
     if ((st = get_queue(c, fz_name(t, "q0"), qcap, T.q[0])) || (st = get_queue(c, fz_name(t, "q1"), qcap, T.q[1])) ||
         (st = get_queue(c, fz_name(t, "touched"), bfs ? (uint64_t)S + 1 : (uint64_t)S + g->nnz + 1, T.touched)))
@@ -2433,10 +2441,12 @@ la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t
   a.damp_lo = (float)(damp - (double)a.damp_hi);
   a.part = part; a.seg_row = d.pseg_row; a.seg_beg = d.pseg_beg; a.long_ppr = d.plong;
   a.nlong_ppr = d.nplong; a.seg_first = d.pseg_first;
-  // Bytes and entries per iteration, as la3_ppr_batch accounts them.
+  // Algorithmic bytes per iteration ((4 + w) E + (8 + m) C + (a + s_r + s_w)
+  // R_touch + m R_act): every regular-row entry, every column and every
+  // regular row, per live query an f32 message (m = 4k), a compensated sum
+  // (a = 8k) and the compensated rank read and written (s_r = s_w = 8k).
   const double kb = 4.0 * k;
-  const uint64_t nnz_short = g->nnz_reg - g->pnnz_reg;
-  T.step_bytes = (4.0 + kb) * nnz_short + 5.0 * kb * g->nreg + (4.0 + kb) * g->pnnz_reg + 2.0 * kb * d.npseg_reg;
+  T.step_bytes = 4.0 * g->nnz_reg + (8.0 + kb) * g->ncols + 6.0 * kb * g->nreg + kb * g->nreg;
   T.step_edges = g->nnz_reg;
   launch(c, "outdeg", k_outdeg, blocks_for(S), d, deg);
   if (W == 1) launch(c, "mppr_init", k_mppr_init<1>, blocks_for(SK), d, a);

@@ -69,12 +69,15 @@ __global__ void __launch_bounds__(BLOCK) k_msbfs_seed(DG g, const uint32_t* root
 
 // Top-down step: every entry (c -> r) of an active column ORs c's frontier
 // bits that r has not yet seen into r's next-frontier words. The first
-// toucher of r in this step (stamp) appends r to the touched list.
+// toucher of r in this step (stamp) appends r to the touched list. With
+// tst != nullptr (measurement runs) rows that receive a message are counted
+// once per step (tst holds the last step tag ttag that counted the row).
 template <int W>
 __device__ __forceinline__ void msbfs_push_body(const DG& g, Part p, Emitter& em, const uint64_t* items,
                                                 uint32_t nitems, const uint64_t* vis, const uint64_t* fa,
                                                 uint64_t* fb, uint32_t* stamp, uint32_t tag,
-                                                const Frontier& touched) {
+                                                const Frontier& touched, uint32_t* tst = nullptr,
+                                                uint32_t ttag = 0) {
   for (uint32_t it = p.wid(); it < nitems; it += p.nwaves()) {
     const uint64_t item = items[it];
     const uint32_t s = (uint32_t)item, ch = (uint32_t)(item >> 32);
@@ -97,6 +100,7 @@ __device__ __forceinline__ void msbfs_push_body(const DG& g, Part p, Emitter& em
           if (m) { atomicOr((unsigned long long*)&fb[(size_t)r * W + w], (unsigned long long)m); any = true; }
         }
         if (any) first = atomicMax(&stamp[r], tag) < tag;
+        if (tst && atomicMax(&tst[r], ttag) < ttag) em.touched++;
       }
       em_push(em, touched, r, first ? 1u : 0u, 0, first);
     }
@@ -420,7 +424,8 @@ __device__ __forceinline__ uint64_t rdlane64(uint64_t v, uint32_t l) {
 template <int QW, bool DIRECT = false>
 __device__ __forceinline__ void mssp_push_body(const DG& g, Part p, Emitter& em, const uint64_t* items,
                                                uint32_t nitems, uint32_t* dist, const uint64_t* cm,
-                                               uint64_t* nm, uint32_t T, const Frontier& touched) {
+                                               uint64_t* nm, uint32_t T, const Frontier& touched,
+                                               uint32_t* tst = nullptr, uint32_t ttag = 0) {
   static_assert(!DIRECT || QW == 1, "direct emission needs one mask word per row");
   constexpr uint32_t K = 64 * QW;
   // Edges in groups of U: the U distance rows are loaded before any is
@@ -535,6 +540,9 @@ __device__ __forceinline__ void mssp_push_body(const DG& g, Part p, Emitter& em,
               vbits[j] = lane == u ? b : vbits[j];
             }
           }
+          // Measurement runs: lane u counts entry u's row once per step (every
+          // entry of a column with a near query carries a message).
+          if (tst && lane < n && atomicMax(&tst[vrow], ttag) < ttag) em.touched++;
           // With one mask word per row, the row is queued by the lane whose
           // atomicOr finds the word still zero: the first improvement of the
           // row in this step (nm starts the step zeroed, far bits included).

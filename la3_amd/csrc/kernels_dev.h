@@ -77,7 +77,7 @@ struct CtrSlot {
   unsigned long long edges;      // entries behind the emitted items
   unsigned long long scanned;    // row entries read by pull steps
   uint32_t nmin_inv;             // ~(min distance in the next frontier), via atomicMax
-  uint32_t pad;
+  uint32_t touched;              // rows that received a message (measurement runs only)
 };
 struct Frontier {
   uint64_t* items;               // work items: (chunk << 32) | slot
@@ -92,6 +92,7 @@ struct Emitter {
   uint32_t verts, cols;
   unsigned long long edges, scanned;
   uint32_t nmin_inv;  // ~(smallest distance this lane put in the next frontier)
+  uint32_t touched;   // rows this lane saw receive their first message of the step
 };
 
 __device__ __forceinline__ void em_init(Emitter& em, uint64_t* lds) {
@@ -100,6 +101,7 @@ __device__ __forceinline__ void em_init(Emitter& em, uint64_t* lds) {
   em.verts = em.cols = 0;
   em.edges = em.scanned = 0;
   em.nmin_inv = 0;
+  em.touched = 0;
 }
 
 __device__ __forceinline__ void em_flush(Emitter& em, const Frontier& f) {
@@ -140,28 +142,33 @@ __device__ __forceinline__ void em_push(Emitter& em, const Frontier& f, uint32_t
 // Must be called once by every thread of the block (contains a block barrier).
 __device__ __forceinline__ void em_finish(Emitter& em, const Frontier& f, uint32_t bid) {
   em_flush(em, f);
-  __shared__ unsigned long long red[5][BLOCK / WAVE];
+  __shared__ unsigned long long red[6][BLOCK / WAVE];
   const uint32_t w = threadIdx.x / WAVE;
   const unsigned long long v0 = wave_sum((unsigned long long)em.verts);
   const unsigned long long v1 = wave_sum((unsigned long long)em.cols);
   const unsigned long long v2 = wave_sum(em.edges);
   const unsigned long long v3 = wave_sum(em.scanned);
+  const unsigned long long v5 = wave_sum((unsigned long long)em.touched);
   uint32_t mx = em.nmin_inv;
 #pragma unroll
   for (int d = WAVE / 2; d > 0; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, WAVE));
-  if (lane_id() == 0) { red[0][w] = v0; red[1][w] = v1; red[2][w] = v2; red[3][w] = v3; red[4][w] = mx; }
+  if (lane_id() == 0) {
+    red[0][w] = v0; red[1][w] = v1; red[2][w] = v2; red[3][w] = v3; red[4][w] = mx; red[5][w] = v5;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long t[5] = {0, 0, 0, 0, 0};
+    unsigned long long t[6] = {0, 0, 0, 0, 0, 0};
     for (int i = 0; i < 4; i++)
       for (uint32_t j = 0; j < BLOCK / WAVE; j++) t[i] += red[i][j];
     for (uint32_t j = 0; j < BLOCK / WAVE; j++) t[4] = max(t[4], red[4][j]);
+    for (uint32_t j = 0; j < BLOCK / WAVE; j++) t[5] += red[5][j];
     CtrSlot& cs = f.slots[bid % NSLOT];
     if (t[0]) atomicAdd(&cs.verts, (uint32_t)t[0]);
     if (t[1]) atomicAdd(&cs.cols, (uint32_t)t[1]);
     if (t[2]) atomicAdd(&cs.edges, t[2]);
     if (t[3]) atomicAdd(&cs.scanned, t[3]);
     if (t[4]) atomicMax(&cs.nmin_inv, (uint32_t)t[4]);
+    if (t[5]) atomicAdd(&cs.touched, (uint32_t)t[5]);
   }
 }
 

@@ -29,6 +29,7 @@ struct FTask {
   uint32_t delta;                  // SSSP near/far delta (0 = Bellman-Ford)
   uint32_t nslots_pad;
   uint32_t* stamp;
+  uint32_t* tstamp;                // measurement runs: last step that counted a row as touched
   Frontier q[2];                   // work lists, ping-pong by FCtl::cur
   Frontier touched;
   // BFS
@@ -96,7 +97,7 @@ template <int W>
 __device__ __forceinline__ void fused_bfs_a(const FTask& T, const FCtl& C, Part p, Emitter& em) {
   const uint32_t cur = C.cur;
   msbfs_push_body<W>(T.g, p, em, T.q[cur].items, C.nitems, T.vis, T.fr[cur], T.fr[cur ^ 1], T.stamp,
-                     C.iter + 2, T.touched);
+                     C.iter + 2, T.touched, T.tstamp, C.iter + 1);
   em_finish(em, T.touched, p.bid);
 }
 
@@ -115,7 +116,8 @@ __device__ __forceinline__ void fused_bfs_b(const FTask& T, const FCtl& C, Part 
 template <int QW>
 __device__ __forceinline__ void fused_sssp_a(const FTask& T, const FCtl& C, Part p, Emitter& em) {
   const uint32_t cur = C.cur;
-  mssp_push_body<QW>(T.g, p, em, T.q[cur].items, C.nitems, T.dist, T.m[cur], T.m[cur ^ 1], C.thr, T.touched);
+  mssp_push_body<QW>(T.g, p, em, T.q[cur].items, C.nitems, T.dist, T.m[cur], T.m[cur ^ 1], C.thr, T.touched,
+                     T.tstamp, C.iter + 1);
   em_finish(em, T.touched, p.bid);
 }
 
@@ -210,7 +212,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused_b(const FTask* __restrict__ tas
 
 // Sums of one queue's counter slots (one wavefront; NSLOT == WAVE).
 struct FSums {
-  uint32_t verts, cols, nmin;
+  uint32_t verts, cols, nmin, touched;
   unsigned long long edges, scanned;
 };
 __device__ __forceinline__ FSums fsum(const Frontier& f) {
@@ -220,6 +222,7 @@ __device__ __forceinline__ FSums fsum(const Frontier& f) {
   o.cols = wave_sum(s.cols);
   o.edges = wave_sum(s.edges);
   o.scanned = wave_sum(s.scanned);
+  o.touched = wave_sum(s.touched);
   uint32_t mx = s.nmin_inv;
 #pragma unroll
   for (int d = WAVE / 2; d > 0; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, d, WAVE));
@@ -278,20 +281,29 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
     if (!init) {
       const FSums ts = fsum(T.touched);  // push counters: smallest queued distance
       if (lane == 0) {
+        // Algorithmic bytes of the step (the compulsory traffic of one
+        // vertex-program iteration with perfect reuse of the vectors):
+        //   (4 + w) E_scan + (8 + m) C_act + (a + s_r + s_w) R_touch + m R_act,
+        // per-vertex terms per live query: BFS bit sets (m = a = s_r = s_w =
+        // k/8 B), SSSP u32 distances (w = 4, m = a = s_r = s_w = 4k B).
+        // R_touch (rows that received a message) is counted by the push in
+        // measurement runs (tstamp); a bottom-up step touches exactly the
+        // rows it activates.
         const double kb = T.k / 8.0;
         double b = 0;
         unsigned long long e = 0;
         if (T.kind == FK_BFS) {
           if (!C.bottom_up) {
-            b = (4.0 + kb) * C.fedges + (8.0 + kb) * C.fcols + 3 * kb * ns.verts;
+            b = 4.0 * C.fedges + (8.0 + kb) * C.fcols + 3 * kb * ts.touched + kb * ns.verts;
             e = C.fedges;
           } else {
-            b = (4.0 + kb) * ns.scanned + (8.0 + kb) * g.nrows + 2 * kb * ns.verts;
+            b = 4.0 * ns.scanned + (8.0 + kb) * C.fcols + 4 * kb * ns.verts;
             e = ns.scanned;
             C.pbytes += b;
           }
         } else {
-          b = (8.0 + 4.0 * K) * C.fedges + (8.0 + 4.0 * K) * C.fcols + (K / 8.0 + 4) * ns.verts;
+          const double k4 = 4.0 * T.k;
+          b = 8.0 * C.fedges + (8.0 + k4) * C.fcols + 3 * k4 * ts.touched + k4 * ns.verts;
           e = C.fedges;
         }
         C.bytes += b;
