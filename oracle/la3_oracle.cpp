@@ -56,6 +56,9 @@
 #include <cstring>
 #include <numeric>
 #include <vector>
+#ifdef _OPENMP
+#include <parallel/algorithm>
+#endif
 
 namespace {
 
@@ -129,6 +132,9 @@ void build_csc(OCSC& c, uint32_t ncols, const std::vector<Rec>& triples, const O
     uint32_t col = G.colloc[t.col];
     c.entries[--c.colptrs[col]] = Entry{loc - off, t.row, t.w};
   }
+  // Each column's entries by global_idx (unique within a column), one
+  // column per task.
+#pragma omp parallel for schedule(dynamic, 1024)
   for (uint32_t i = 0; i < ncols; i++)
     std::sort(c.entries.begin() + c.colptrs[i], c.entries.begin() + c.colptrs[i + 1],
               [](const Entry& a, const Entry& b) { return a.global_idx < b.global_idx; });
@@ -193,21 +199,20 @@ OGraph* build_graph(const uint32_t* data, uint64_t nrec, uint32_t n, bool weight
   const uint32_t ncols = pos;
   for (uint32_t i = 0; i < S; i++) if (!cols[i]) G->colloc[i] = pos++;
 
-  // Dedup with first-inserted-wins (unordered_set::insert keeps the first);
-  // a stable ordering by (row, col) keeps the first occurrence of each key.
-  std::vector<uint64_t> order(tri.size());
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) {
-    if (tri[a].row != tri[b].row) return tri[a].row < tri[b].row;
-    return tri[a].col < tri[b].col;
-  });
+  // Dedup with first-inserted-wins (unordered_set::insert keeps the first):
+  // ordering by (row, col, insertion index) puts the first occurrence of each
+  // key first. The pairs are unique, so any sort gives this one order.
+  std::vector<std::pair<uint64_t, uint64_t>> order(tri.size());
+  for (uint64_t k = 0; k < tri.size(); k++) order[k] = {((uint64_t)tri[k].row << 32) | tri[k].col, k};
+#ifdef _OPENMP
+  __gnu_parallel::sort(order.begin(), order.end());
+#else
+  std::sort(order.begin(), order.end());
+#endif
   std::vector<Rec> regset, sinkset;
   for (uint64_t k = 0; k < order.size(); k++) {
-    const Rec& t = tri[order[k]];
-    if (k > 0) {
-      const Rec& p = tri[order[k - 1]];
-      if (p.row == t.row && p.col == t.col) continue;
-    }
+    if (k > 0 && order[k - 1].first == order[k].first) continue;
+    const Rec& t = tri[order[k].second];
     if (G->rowloc[t.row] < G->nreg) regset.push_back(t); else sinkset.push_back(t);
   }
   build_csc(G->reg, ncols, regset, *G);
