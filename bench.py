@@ -296,6 +296,9 @@ def workload_batched_s24(args, L):
     g = L.Graph.from_records(ctx, rec, n, True, L.DIRECTED)
     log(f"ingest {time.time() - t:.1f}s: {g.info}")
     bfs_r, sssp_r, ppr_s = choose_roots(rec, 256, 41), choose_roots(rec, 256, 42), choose_roots(rec, 128, 43)
+    if args.fused_only:
+        return fused_only(args, L, ctx, g, [(g, L.Q_BFS, bfs_r), (g, L.Q_SSSP, sssp_r), (g, L.Q_PPR, ppr_s, 30)],
+                          "batched_s24")
     t = time.time()
     tune = g.autotune(sssp_r)
     log(f"autotune ({time.time() - t:.1f}s, not timed): {tune}")
@@ -402,6 +405,30 @@ def workload_batched_s24(args, L):
         out["cpu_baseline"] = cpu_baseline_on(o, outdeg, sample, gpu_hops, gpu_dist, scale)
     pool.shutdown()
     return out
+
+
+def fused_only(args, L, ctx, g, mix, workload):
+    """Profiling runs (rocprofv3 kernel traces and counter passes): the
+    tuning the autotuner chose, given by --tune, instead of the autotuner;
+    then only fused steps, every one launched with per-kernel events and
+    touched-row counting, so a trace's average for the fused launch covers
+    exactly the launches behind the roofline; then the copy kernel of known
+    size (1 GiB) that calibrates the HBM counters."""
+    sd, bd, kb, ks, kp, blocks = (int(x) for x in args.tune.split(","))
+    g.set_tuning(L.TUNE_SSSP_DELTA, sd)
+    g.set_tuning(L.TUNE_BATCH_SSSP_DELTA, bd)
+    g.set_tuning(L.TUNE_BFS_TASK_K, kb)
+    g.set_tuning(L.TUNE_SSSP_TASK_K, ks)
+    g.set_tuning(L.TUNE_PPR_TASK_K, kp)
+    g.set_tuning(L.TUNE_FUSED_BLOCKS, blocks)
+    fused = lambda k: [L.batch_run(ctx, mix, out=L.DEVICE) for _ in range(k)]
+    timed(ctx, fused, max(args.warmup, 1))
+    el, _, st = timed(ctx, fused, args.steps)
+    log(f"fused, profiled launches: {el * 1e3 / args.steps:.1f} ms/step; kernels {json.dumps(kernel_table(st))}")
+    copy_gbs = ctx.copy_bandwidth(1 << 30, 1)
+    return {"profiling_run": workload, "tune": args.tune, "steps": args.steps,
+            "ms_per_step_profiled": round(el * 1e3 / args.steps, 3),
+            "roofline": roofline(st, workload), "kernels": kernel_table(st), "device_copy_gbs": round(copy_gbs, 1)}
 
 
 def workload_serve(args, L):
@@ -634,6 +661,10 @@ def main():
     ap.add_argument("--cpu-roots", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-single", action="store_true", help="skip the single-query baseline")
+    ap.add_argument("--fused-only", action="store_true",
+                    help="profiling runs: fixed tuning (--tune), only event-profiled fused steps")
+    ap.add_argument("--tune", default="512,512,256,128,128,32",
+                    help="--fused-only tuning: sssp_delta,batch_sssp_delta,bfs_k,sssp_k,ppr_k,blocks_per_cu")
     ap.add_argument("--workload", default="batched_s24",
                     choices=["bfs_sssp_s22", "pr_s16", "cc_tc_s20", "batched_s24", "serve"])
     ap.add_argument("--qps", type=float, default=400.0)
