@@ -298,7 +298,7 @@ __global__ void k_vp_init(la3_device_view g, P p, Bufs<P> b, uint32_t stationary
     bool a = false;
     if (!only || only[s]) a = op_init(p, stationary != 0, g.id_of[s], st);
     b.v[s] = st;
-    if (b.vm) b.vm[s] = st;
+    if (b.vm) b.vm[s] = stationary ? st : S();  // the active states (vertex_program.hpp:159-173)
     const bool sends = s < g.nreg || (g.directed && s >= src0 && s < src1);
     b.act[0][s] = sends && a;
     b.act[1][s] = 0;
@@ -325,7 +325,7 @@ __global__ void k_vp_init_other(la3_device_view g, P p, Bufs<P> b, uint32_t stat
     const uint32_t vid = g.id_of[s];
     const bool a = op_init_other(p, stationary != 0, vid, other[other_slot_of_id[vid]], st);
     b.v[s] = st;
-    if (b.vm) b.vm[s] = st;
+    if (b.vm) b.vm[s] = stationary ? st : S();  // the active states (vertex_program.hpp:159-173)
     const bool sends = s < g.nreg || (g.directed && s >= src0 && s < src1);
     b.act[0][s] = sends && a;
     b.act[1][s] = 0;

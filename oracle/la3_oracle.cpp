@@ -384,7 +384,9 @@ struct CtProg {
 // engine's paths: weights and execute_single (WDeg), execute_ with the sink
 // step and execute_non_opt (MaxLabel), apply with the iteration and an
 // order-dependent combine (Level), floating-point sums and
-// initialize(other) (PrAlpha), and a state-reading gather (CountGreater).
+// initialize(other) (PrAlpha), and a state-reading gather of a stationary
+// program (CountGreater: mirrors hold the initial states) and of a
+// non-stationary one (CountNonzero: mirrors hold S()).
 struct WDegProg {
   using M = Empty; using A = uint32_t;
   struct S { uint32_t wdeg = 0; };
@@ -454,6 +456,20 @@ struct CountGreaterProg {
   A gather(const EdgeV&, const M& m, const S& s) { return m > s.val ? 1u : 0u; }
   void combine(const A& y1, A& y2) { y2 += y1; }
   bool apply(const A& y, S& s) { s.cnt += y; s.val = y % 7; return false; }
+};
+
+// Not stationary, gather reads the row's state: the mirrors keep S() (val
+// 0), so a row counts the messages above 0, not above its own value.
+struct CountNonzeroProg {
+  using M = uint32_t; using A = uint32_t;
+  struct S { uint32_t val = 0; uint32_t cnt = 0; };
+  static constexpr bool stationary = false, gather_state = true, apply_iter = false;
+  A identity() const { return 0; }
+  bool init(uint32_t vid, S& s) { s.val = vid % 7; return vid % 3 == 0; }
+  M scatter(const S& s) { return s.val; }
+  A gather(const EdgeV&, const M& m, const S& s) { return m > s.val ? 1u : 0u; }
+  void combine(const A& y1, A& y2) { y2 += y1; }
+  bool apply(const A& y, S& s) { s.cnt += y; return y > 0; }
 };
 
 // ---------------------------------------------------------------- engine ---
@@ -854,6 +870,17 @@ int ora_vp_countgreater(void* h, uint32_t max_iters, uint32_t* val, uint32_t* cn
     val[e.vid_of(i)] = e.v[i].val;
     cnt[e.vid_of(i)] = e.v[i].cnt;
   }
+  if (iters_run) *iters_run = e.iters_run;
+  return 0;
+}
+
+int ora_vp_countnonzero(void* h, uint32_t max_iters, uint32_t* cnt, uint32_t* iters_run) {
+  const OGraph& G = *((Handle*)h)->G;
+  CountNonzeroProg p;
+  Engine<CountNonzeroProg> e(p, G);
+  e.initialize();
+  e.execute(max_iters);
+  for (uint32_t i = 0; i < G.nslots; i++) cnt[e.vid_of(i)] = e.v[i].cnt;
   if (iters_run) *iters_run = e.iters_run;
   return 0;
 }

@@ -54,6 +54,7 @@ def lib():
         L.ora_vp_level.argtypes = [P, u32, u32, P, P, P]
         L.ora_vp_pralpha.argtypes = [P, P, u32, P, P]
         L.ora_vp_countgreater.argtypes = [P, u32, P, P, P]
+        L.ora_vp_countnonzero.argtypes = [P, u32, P, P]
         _lib = L
     return _lib
 
@@ -199,3 +200,11 @@ def vp_countgreater(g, max_iters):
     val, cnt, it = _u32(g.nslots), _u32(g.nslots), _u32(1)
     lib().ora_vp_countgreater(g.h, int(max_iters), _ptr(val), _ptr(cnt), _ptr(it))
     return val, cnt, int(it[0])
+
+
+def vp_countnonzero(g, max_iters):
+    """Non-stationary state-reading gather: the mirrors keep S(), so each row
+    counts the messages above 0 (senders: vid % 3 == 0 at first)."""
+    cnt, it = _u32(g.nslots), _u32(1)
+    lib().ora_vp_countnonzero(g.h, int(max_iters), _ptr(cnt), _ptr(it))
+    return cnt, int(it[0])

@@ -30,6 +30,7 @@ def vp():
         lib.vp_level.argtypes = [P, P, u32, u32, P, P, P]
         lib.vp_pralpha.argtypes = [P, P, P, u32, P, P]
         lib.vp_countgreater.argtypes = [P, P, u32, P, P, P]
+        lib.vp_countnonzero.argtypes = [P, P, u32, P, P]
         _vp = lib
     return _vp
 
@@ -73,6 +74,12 @@ def countgreater(g, m):
     return val, cnt, int(it[0])
 
 
+def countnonzero(g, m):
+    cnt, it = np.zeros(g.nslots, np.uint32), np.zeros(1, np.uint32)
+    _call("vp_countnonzero", g.ctx.h, g.h, m, _p(cnt), _p(it))
+    return cnt, int(it[0])
+
+
 def _graphs(ctx, oracle, rec, n, weighted, mode):
     return L.Graph.from_records(ctx, rec, n, weighted, mode), oracle.OracleGraph(rec, n, weighted, mode)
 
@@ -108,6 +115,9 @@ def test_interface_programs_match_oracle(gpu_ctx, oracle, g1, case):
             val, cnt, it = countgreater(g, m)
             ev, ec, eit = oracle.vp_countgreater(o, m)
             assert (val == ev).all() and (cnt == ec).all() and it == eit, (name, mode, m)
+            cnt, it = countnonzero(g, m)
+            ec, eit = oracle.vp_countnonzero(o, m)
+            assert (cnt == ec).all() and it == eit, (name, mode, m)
     g, o = _graphs(gpu_ctx, oracle, rec, n, False, L.DIRECTED)
     gr, orr = _graphs(gpu_ctx, oracle, rec, n, False, L.REVERSED)
     for m in (1, 2, 20, 0):

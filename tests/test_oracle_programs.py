@@ -1,6 +1,6 @@
 """Pins the oracle's engine on programs written only against the
 vertex-program interface (oracle/la3_oracle.cpp: WDeg, MaxLabel, Level,
-PrAlpha, CountGreater) with closed forms computed from the records, before
+PrAlpha, CountGreater, CountNonzero) with closed forms computed from the records, before
 those programs check the device VertexProgram<P> template."""
 import numpy as np
 import pytest
@@ -106,3 +106,40 @@ def test_countgreater_reads_the_init_mirror(oracle, mode):
     np.add.at(y1, b, (msg1[a] > init[b]).astype(np.int64))
     val, cnt, it = oracle.vp_countgreater(g, 2)
     assert it == 2 and (cnt == y0 + y1).all() and (val == np.where(has_in, y1 % 7, init)).all()
+
+
+@pytest.mark.parametrize("mode", ["directed", "undirected"])
+def test_countnonzero_reads_default_mirrors(oracle, mode):
+    """A program that is not stationary has its mirrors left at S(): val 0,
+    so each row counts the messages above 0 (vertex_program.hpp:159-173).
+    The gather compares with the mirror; a mirror holding the row's own
+    initial value (vid % 7) would give other counts."""
+    rec = random_records(97, N, 1500)
+    und = mode == "undirected"
+    g = oracle.OracleGraph(rec, N, False, oracle.UNDIRECTED if und else oracle.DIRECTED)
+    d = CF.dedup_first(rec)
+    a, b = d[:, 0].astype(np.int64), d[:, 1].astype(np.int64)
+    if und:
+        a, b = np.concatenate([a, b]), np.concatenate([b, a])
+        key = np.unique(a * (N + 1) + b)
+        a, b = key // (N + 1), key % (N + 1)
+    ids = np.arange(N + 1)
+    init = ids % 7
+    has_in = np.bincount(b, minlength=N + 1) > 0
+    has_out = np.bincount(a, minlength=N + 1) > 0
+    source = has_out & ~has_in
+    sink = has_in & ~has_out
+    act0 = ids % 3 == 0
+    e0 = act0[a]
+    y0 = np.zeros(N + 1, dtype=np.int64)
+    np.add.at(y0, b[e0], (init[a[e0]] > 0).astype(np.int64))
+    cnt, it = oracle.vp_countnonzero(g, 1)
+    assert it == 1 and (cnt == y0).all()
+    # iteration 1 of execute_non_opt: regular senders whose apply returned
+    # true (y0 > 0); source messages reach sink rows only
+    activated = y0 > 0
+    e1 = (activated[a] & has_in[a] & has_out[a]) | (act0[a] & source[a] & sink[b])
+    y1 = np.zeros(N + 1, dtype=np.int64)
+    np.add.at(y1, b[e1], (init[a[e1]] > 0).astype(np.int64))
+    cnt, it = oracle.vp_countnonzero(g, 2)
+    assert it == 2 and (cnt == y0 + y1).all()

@@ -3,7 +3,7 @@
 // compiled outside the library as a caller would, for tests/test_gpu_vp.py.
 // None of them is one of the library's programs; the oracle holds the same
 // operations (oracle/la3_oracle.cpp: WDegProg, MaxLabelProg, LevelProg,
-// PrAlphaProg, CountGreaterProg). Each entry point returns 0 or a la3_status
+// PrAlphaProg, CountGreaterProg, CountNonzeroProg). Each entry point returns 0 or a la3_status
 // and writes per-vertex outputs by original id.
 #include <cstdio>
 #include <string>
@@ -141,6 +141,28 @@ struct CountGreater {
   }
 };
 
+// Not stationary, state-reading gather: the mirrors keep S() (val 0).
+struct CountNonzero {
+  using W = la3::Empty;
+  using M = uint32_t;
+  using A = uint32_t;
+  struct S {
+    uint32_t val = 0;
+    uint32_t cnt = 0;
+  };
+  __device__ bool init(uint32_t vid, S& s) const {
+    s.val = vid % 7;
+    return vid % 3 == 0;
+  }
+  __device__ M scatter(const S& s) const { return s.val; }
+  __device__ A gather(const la3::Edge<W>&, const M& m, const S& s) const { return m > s.val ? 1u : 0u; }
+  __device__ void combine(const A& y1, A& y2) const { y2 += y1; }
+  __device__ bool apply(const A& y, S& s) const {
+    s.cnt += y;
+    return y > 0;
+  }
+};
+
 template <class F>
 int guarded(F f) {
   try {
@@ -225,6 +247,17 @@ int vp_countgreater(la3_ctx* c, la3_graph* g, uint32_t max_iters, uint32_t* val,
       val[i] = st[i].val;
       cnt[i] = st[i].cnt;
     }
+    *iters = vp.iterations();
+  });
+}
+
+int vp_countnonzero(la3_ctx* c, la3_graph* g, uint32_t max_iters, uint32_t* cnt, uint32_t* iters) {
+  return guarded([&] {
+    la3::VertexProgram<CountNonzero> vp(c, g);
+    vp.initialize();
+    vp.execute(max_iters);
+    auto st = vp.states();
+    for (size_t i = 0; i < st.size(); i++) cnt[i] = st[i].cnt;
     *iters = vp.iterations();
   });
 }
