@@ -57,6 +57,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -179,17 +180,42 @@ struct Bufs {
   typename P::S* vm = nullptr;  // mirror (state-reading gather only)
   typename P::M* x[2] = {nullptr, nullptr};  // messages by slot (regular and source slots)
   uint8_t* act[2] = {nullptr, nullptr};      // message flags by slot
-  uint32_t* ctr = nullptr;      // [0] = some regular vertex was activated
+  // [0] some regular vertex was activated; [4 + k] active regular senders
+  // counted for a pass (two slots, used alternately); [6] active sources
+  uint32_t* ctr = nullptr;
+  uint32_t* stamp = nullptr;    // by row: the last sparse pass that reached it
 };
 
 // One pass over rows [r0, r1): read the messages of buffer `cur` (source
 // slots only when use_src), apply, and for regular rows write the next
-// messages into buffer `cur ^ 1` (unless single). Lane per row; rows longer
-// than VP_LONG are left to k_vp_pull_long.
+// messages into buffer `cur ^ 1` (unless single).
 struct PassArgs {
   uint32_t r0, r1, cur, iter;
   uint32_t use_src, single, stationary;
+  uint32_t tag, slot, dense_only;  // sparse-pass stamp, count slot, no sparse pass
 };
+
+// Dense or sparse pass, by the reference's density rule
+// (serializable_bitvector.h:54-55): dense when more than
+// (columns / 32) * 2 / 3 columns send. A sparse pass first stamps the rows
+// that the active senders' columns reach (k_vp_mark); the row kernels then
+// skip the other rows, which would receive nothing. Either way a processed
+// row folds all its messages in the same order, so results are the same.
+__device__ __forceinline__ bool vp_sparse(const la3_device_view& g, const uint32_t* ctr, const PassArgs& a) {
+  if (a.dense_only) return false;
+  const uint32_t n = ctr[4 + a.slot] + (a.use_src ? ctr[6] : 0);
+  return n <= (g.ncols / 32) * 2 / 3;
+}
+
+// Adds every thread's n into *dst, one atomic per block (all threads call).
+__device__ __forceinline__ void block_add(uint32_t n, uint32_t* dst) {
+  __shared__ uint32_t part;
+  if (threadIdx.x == 0) part = 0;
+  __syncthreads();
+  if (n) atomicAdd(&part, n);
+  __syncthreads();
+  if (threadIdx.x == 0 && part) atomicAdd(dst, part);
+}
 
 template <class P>
 __device__ __forceinline__ void vp_finish_row(const P& p, const la3_device_view& g, const Bufs<P>& b,
@@ -210,18 +236,43 @@ __device__ __forceinline__ void vp_finish_row(const P& p, const la3_device_view&
   }
 }
 
+// Short rows (at most VP_LONG entries): lane per row. Entries are read four
+// at a time (columns, then flags, then messages: independent loads in
+// flight together) and combined in entry order.
 template <class P>
 __global__ void __launch_bounds__(VP_BLOCK) k_vp_pull(la3_device_view g, P p, Bufs<P> b, PassArgs a) {
   using A = typename P::A;
-  const typename P::M* x = b.x[a.cur];
+  using M = typename P::M;
+  const M* x = b.x[a.cur];
   const uint8_t* act = b.act[a.cur];
+  const bool sparse = vp_sparse(g, b.ctr, a);
   for (uint32_t r = a.r0 + blockIdx.x * blockDim.x + threadIdx.x; r < a.r1; r += gridDim.x * blockDim.x) {
-    const uint32_t e0 = g.row_ptr[r], es = g.row_split[r], e1 = g.row_ptr[r + 1];
+    if (sparse && b.stamp[r] != a.tag) continue;
+    const uint32_t e0 = g.row_ptr[r], e1 = g.row_ptr[r + 1];
     if (e1 - e0 > VP_LONG) continue;
     A y = A();
     bool touched = false;
     const typename P::S* mir = b.vm ? b.vm + r : nullptr;
-    for (uint32_t e = a.use_src ? e0 : es; e < e1; e++) {
+    uint32_t e = a.use_src ? e0 : g.row_split[r];
+    for (; e + 4 <= e1; e += 4) {
+      uint32_t u[4];
+      bool on[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) u[j] = g.row_cols[e + j];
+#pragma unroll
+      for (int j = 0; j < 4; j++) on[j] = act[u[j]] != 0;
+      M m[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (on[j]) m[j] = x[u[j]];
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (on[j]) {
+          op_combine(p, op_gather(p, make_edge<typename P::W>(g, u[j], r, e + j), m[j], mir), y);
+          touched = true;
+        }
+    }
+    for (; e < e1; e++) {
       const uint32_t u = g.row_cols[e];
       if (!act[u]) continue;
       op_combine(p, op_gather(p, make_edge<typename P::W>(g, u, r, e), x[u], mir), y);
@@ -231,58 +282,277 @@ __global__ void __launch_bounds__(VP_BLOCK) k_vp_pull(la3_device_view g, P p, Bu
   }
 }
 
-// Rows longer than VP_LONG: one wavefront per row. Lanes gather 64 entries'
-// values at a time (in parallel: flags, messages, gathers); lane 0 then
-// combines them in entry order from LDS.
-// Orders the LDS writes of one wavefront's lanes before its lanes' reads.
-__device__ __forceinline__ void wave_sync_lds() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// Long rows (more than VP_LONG entries) in two phases, exact for any
+// combine: (A) every entry's gather, in parallel (a wavefront per
+// VP_SEG-entry segment), into a staging area: the value, and per 64-entry
+// chunk a mask of the entries whose sender was active; (B) a wavefront per
+// row folds the chunks' active values in entry order. Phase B's fold is the
+// only serial part; its loads are issued VP_DEPTH chunks ahead.
+constexpr uint32_t VP_SEG = 256;  // entries per phase-A segment: four chunks
+struct LongRow {
+  uint32_t r, e0, e1, cbase;  // row slot, entries [e0, e1), first staging chunk
+};
+template <class A>
+struct LongTabs {
+  const LongRow* rows = nullptr;   // by long-row index
+  const uint32_t* sbase = nullptr; // first segment of each long row
+  const uint32_t* seg = nullptr;   // long-row index of each segment
+  const uint32_t* nseg = nullptr;  // device: number of segments
+  A* val = nullptr;                // 64 values per chunk
+  uint64_t* mask = nullptr;        // one mask per chunk
+};
+
+// The long-row tables (once per program): chunk and segment offsets of each
+// long row, exclusive prefix sums in long-row order, by a three-kernel scan
+// over blocks of VP_TAB rows; totals[0] = chunks, totals[1] = segments.
+constexpr uint32_t VP_TAB = 1024;
+__device__ __forceinline__ void long_counts(const la3_device_view& g, uint32_t i, uint32_t& c, uint32_t& s) {
+  const uint32_t r = g.long_rows[i], len = g.row_ptr[r + 1] - g.row_ptr[r];
+  c = (len + VP_WAVE - 1) / VP_WAVE;
+  s = (len + VP_SEG - 1) / VP_SEG;
+}
+
+// In-block exclusive scan of two counters (VP_TAB threads).
+__device__ __forceinline__ void block_scan2(uint32_t& c, uint32_t& s, uint32_t& tc, uint32_t& ts) {
+  __shared__ uint32_t sc[VP_TAB], ss[VP_TAB];
+  const uint32_t t = threadIdx.x;
+  sc[t] = c;
+  ss[t] = s;
+  __syncthreads();
+  for (uint32_t off = 1; off < VP_TAB; off <<= 1) {  // inclusive (Hillis-Steele)
+    const uint32_t ac = t >= off ? sc[t - off] : 0, as = t >= off ? ss[t - off] : 0;
+    __syncthreads();
+    sc[t] += ac;
+    ss[t] += as;
+    __syncthreads();
+  }
+  tc = sc[VP_TAB - 1];
+  ts = ss[VP_TAB - 1];
+  c = sc[t] - c;
+  s = ss[t] - s;
+}
+
+__global__ void __launch_bounds__(VP_TAB) k_vp_long_sums(la3_device_view g, uint32_t* bsum) {
+  const uint32_t i = blockIdx.x * VP_TAB + threadIdx.x;
+  uint32_t c = 0, s = 0, tc, ts;
+  if (i < g.nlong) long_counts(g, i, c, s);
+  block_scan2(c, s, tc, ts);
+  if (threadIdx.x == 0) {
+    bsum[2 * blockIdx.x] = tc;
+    bsum[2 * blockIdx.x + 1] = ts;
+  }
+}
+
+// One block: exclusive scan of the nb block sums (in place), totals.
+__global__ void __launch_bounds__(VP_TAB) k_vp_long_scan(uint32_t* bsum, uint32_t nb, uint32_t* totals) {
+  uint32_t carry_c = 0, carry_s = 0;
+  for (uint32_t base = 0; base < nb; base += VP_TAB) {  // block-uniform
+    const uint32_t i = base + threadIdx.x;
+    uint32_t c = i < nb ? bsum[2 * i] : 0, s = i < nb ? bsum[2 * i + 1] : 0, tc, ts;
+    block_scan2(c, s, tc, ts);
+    if (i < nb) {
+      bsum[2 * i] = carry_c + c;
+      bsum[2 * i + 1] = carry_s + s;
+    }
+    carry_c += tc;
+    carry_s += ts;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    totals[0] = carry_c;
+    totals[1] = carry_s;
+  }
+}
+
+__global__ void __launch_bounds__(VP_TAB) k_vp_long_table(la3_device_view g, const uint32_t* bsum, LongRow* rows,
+                                                         uint32_t* sbase) {
+  const uint32_t i = blockIdx.x * VP_TAB + threadIdx.x;
+  uint32_t c = 0, s = 0, tc, ts;
+  if (i < g.nlong) long_counts(g, i, c, s);
+  block_scan2(c, s, tc, ts);
+  if (i < g.nlong) {
+    const uint32_t r = g.long_rows[i];
+    rows[i] = LongRow{r, g.row_ptr[r], g.row_ptr[r + 1], bsum[2 * blockIdx.x] + c};
+    sbase[i] = bsum[2 * blockIdx.x + 1] + s;
+  }
+}
+
+__global__ void k_vp_long_segs(la3_device_view g, const LongRow* rows, const uint32_t* sbase, uint32_t* seg) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.nlong; i += gridDim.x * blockDim.x) {
+    const uint32_t ns = (rows[i].e1 - rows[i].e0 + VP_SEG - 1) / VP_SEG;
+    for (uint32_t k = 0; k < ns; k++) seg[sbase[i] + k] = i;
+  }
+}
+
+__device__ __forceinline__ uint32_t vp_lane() { return threadIdx.x % VP_WAVE; }
+
+// A long-row record read by a whole wavefront, held in scalar registers.
+__device__ __forceinline__ LongRow uniform_row(const LongRow* rows, uint32_t li) {
+  const LongRow R = rows[li];
+  return LongRow{(uint32_t)__builtin_amdgcn_readfirstlane(R.r), (uint32_t)__builtin_amdgcn_readfirstlane(R.e0),
+                 (uint32_t)__builtin_amdgcn_readfirstlane(R.e1), (uint32_t)__builtin_amdgcn_readfirstlane(R.cbase)};
 }
 
 template <class P>
-__global__ void __launch_bounds__(VP_BLOCK) k_vp_pull_long(la3_device_view g, P p, Bufs<P> b, PassArgs a) {
-  using A = typename P::A;
-  __shared__ alignas(16) unsigned char raw[VP_BLOCK * sizeof(A)];
-  __shared__ uint8_t ok[VP_BLOCK];
-  A* val = reinterpret_cast<A*>(raw);
-  const uint32_t lane = threadIdx.x % VP_WAVE;
-  const uint32_t wv = threadIdx.x / VP_WAVE;
-  const uint32_t nw = gridDim.x * (VP_BLOCK / VP_WAVE);
-  const typename P::M* x = b.x[a.cur];
+__global__ void __launch_bounds__(VP_BLOCK) k_vp_gather_long(la3_device_view g, P p, Bufs<P> b, PassArgs a,
+                                                            LongTabs<typename P::A> t) {
+  using M = typename P::M;
+  const M* x = b.x[a.cur];
   const uint8_t* act = b.act[a.cur];
-  for (uint32_t li = blockIdx.x * (VP_BLOCK / VP_WAVE) + wv; li < g.nlong; li += nw) {
-    const uint32_t r = g.long_rows[li];
-    if (r < a.r0 || r >= a.r1) continue;  // wave-uniform
-    const uint32_t e0 = g.row_ptr[r], es = g.row_split[r], e1 = g.row_ptr[r + 1];
-    const typename P::S* mir = b.vm ? b.vm + r : nullptr;
+  const uint32_t lane = vp_lane();
+  const uint32_t nseg = *t.nseg;
+  const uint32_t nw = gridDim.x * (VP_BLOCK / VP_WAVE);
+  const bool sparse = vp_sparse(g, b.ctr, a);
+  for (uint32_t s = blockIdx.x * (VP_BLOCK / VP_WAVE) + threadIdx.x / VP_WAVE; s < nseg; s += nw) {
+    const uint32_t li = (uint32_t)__builtin_amdgcn_readfirstlane(t.seg[s]);
+    const LongRow R = uniform_row(t.rows, li);
+    if (R.r < a.r0 || R.r >= a.r1) continue;  // wave-uniform
+    if (sparse && (uint32_t)__builtin_amdgcn_readfirstlane(b.stamp[R.r]) != a.tag) continue;
+    const uint32_t k = s - t.sbase[li];
+    const uint32_t lo = a.use_src ? R.e0 : g.row_split[R.r];
+    const uint32_t eb = R.e0 + k * VP_SEG;
+    const typename P::S* mir = b.vm ? b.vm + R.r : nullptr;
+    uint32_t u[4];
+    bool on[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t e = eb + j * VP_WAVE + lane;
+      u[j] = e < R.e1 ? g.row_cols[e] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t e = eb + j * VP_WAVE + lane;
+      on[j] = e < R.e1 && e >= lo && act[u[j]] != 0;
+    }
+    M m[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      if (on[j]) m[j] = x[u[j]];
+    const uint64_t c0 = (uint64_t)R.cbase + k * (VP_SEG / VP_WAVE);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t e = eb + j * VP_WAVE + lane;
+      if (on[j]) t.val[(c0 + j) * VP_WAVE + lane] = op_gather(p, make_edge<typename P::W>(g, u[j], R.r, e), m[j], mir);
+      const uint64_t bm = __ballot(on[j]);
+      if (lane == 0 && eb + j * VP_WAVE < R.e1) t.mask[c0 + j] = bm;
+    }
+  }
+}
+
+// One lane's copy of v from lane `src` (src uniform across the wavefront).
+template <class T>
+__device__ __forceinline__ T wave_read(const T& v, uint32_t src) {
+  constexpr int NW = (sizeof(T) + 3) / 4;
+  uint32_t w[NW];
+  w[NW - 1] = 0;
+  memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < NW; i++) w[i] = __builtin_amdgcn_readlane(w[i], src);
+  T out;
+  memcpy(&out, w, sizeof(T));
+  return out;
+}
+
+template <class P>
+__global__ void __launch_bounds__(VP_BLOCK) k_vp_fold_long(la3_device_view g, P p, Bufs<P> b, PassArgs a,
+                                                          LongTabs<typename P::A> t) {
+  using A = typename P::A;
+  constexpr uint32_t D = sizeof(A) <= 8 ? 8 : 2;  // chunks loaded ahead of the fold
+  const uint32_t lane = vp_lane();
+  const uint32_t nw = gridDim.x * (VP_BLOCK / VP_WAVE);
+  const bool sparse = vp_sparse(g, b.ctr, a);
+  for (uint32_t li = blockIdx.x * (VP_BLOCK / VP_WAVE) + threadIdx.x / VP_WAVE; li < g.nlong; li += nw) {
+    const LongRow R = uniform_row(t.rows, li);
+    if (R.r < a.r0 || R.r >= a.r1) continue;  // wave-uniform
+    if (sparse && (uint32_t)__builtin_amdgcn_readfirstlane(b.stamp[R.r]) != a.tag) continue;
+    const uint32_t nch = (R.e1 - R.e0 + VP_WAVE - 1) / VP_WAVE;
     A y = A();
     bool touched = false;
-    for (uint32_t base = a.use_src ? e0 : es; base < e1; base += VP_WAVE) {  // wave-uniform
-      const uint32_t e = base + lane;
-      bool have = false;
-      if (e < e1) {
-        const uint32_t u = g.row_cols[e];
-        if (act[u]) {
-          val[threadIdx.x] = op_gather(p, make_edge<typename P::W>(g, u, r, e), x[u], mir);
-          have = true;
-        }
-      }
-      ok[threadIdx.x] = have;
-      wave_sync_lds();
-      if (lane == 0) {
-        const uint32_t n = min((uint32_t)VP_WAVE, e1 - base);
-        for (uint32_t j = 0; j < n; j++)
-          if (ok[wv * VP_WAVE + j]) {
-            op_combine(p, val[wv * VP_WAVE + j], y);
-            touched = true;
-          }
-      }
-      wave_sync_lds();
+    // Loads of chunks past the row's last read its last chunk again (and are
+    // not folded): unconditional loads keep the wait counts exact, so the
+    // next group's loads stay in flight while this group is folded.
+    uint64_t m[D];
+    A v[D];
+#pragma unroll
+    for (uint32_t j = 0; j < D; j++) {
+      const uint64_t ch = (uint64_t)R.cbase + min(j, nch - 1);
+      m[j] = t.mask[ch];
+      v[j] = t.val[ch * VP_WAVE + lane];
     }
-    if (lane == 0 && touched) vp_finish_row(p, g, b, a, r, y);
+    for (uint32_t c = 0; c < nch; c += D) {
+      uint64_t mn[D];
+      A vn[D];
+#pragma unroll
+      for (uint32_t j = 0; j < D; j++) {
+        const uint64_t ch = (uint64_t)R.cbase + min(c + D + j, nch - 1);
+        mn[j] = t.mask[ch];
+        vn[j] = t.val[ch * VP_WAVE + lane];
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < D; j++)
+        if (c + j < nch) {
+          // the mask is the same in every lane: in scalar registers the
+          // fold loop and a uniform accumulator run on the scalar unit
+          uint64_t mm = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(m[j] >> 32)) << 32) |
+                        (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)m[j]);
+          touched |= mm != 0;
+          if (mm == ~0ull) {  // every entry active: a straight-line fold, lanes in order
+#pragma unroll
+            for (uint32_t l = 0; l < VP_WAVE; l++) op_combine(p, wave_read(v[j], l), y);
+          } else {
+            while (mm) {
+              const uint32_t src = (uint32_t)__builtin_ctzll(mm);
+              mm &= mm - 1;
+              op_combine(p, wave_read(v[j], src), y);
+            }
+          }
+        }
+#pragma unroll
+      for (uint32_t j = 0; j < D; j++) {
+        m[j] = mn[j];
+        v[j] = vn[j];
+      }
+    }
+    if (lane == 0 && touched) vp_finish_row(p, g, b, a, R.r, y);
   }
+}
+
+// Sparse passes: stamps every row of [r0, r1) that an active sender's column
+// reaches (columns [regular | source]: column i is sender slot i, or
+// i + nsink for a source). Columns of at most VP_WAVE entries are walked by
+// their lane, longer ones by the whole wavefront.
+template <class P>
+__global__ void __launch_bounds__(VP_BLOCK) k_vp_mark(la3_device_view g, Bufs<P> b, PassArgs a) {
+  if (!vp_sparse(g, b.ctr, a)) return;
+  const uint8_t* act = b.act[a.cur];
+  const bool sink_rows = a.r0 >= g.nreg;
+  const uint32_t nc = g.nreg + (a.use_src ? g.nsrc : 0);
+  const uint32_t lane = vp_lane();
+  for (uint32_t base = blockIdx.x * blockDim.x; base < nc; base += gridDim.x * blockDim.x) {  // block-uniform
+    const uint32_t i = base + threadIdx.x;
+    uint32_t lo = 0, hi = 0;
+    if (i < nc && act[i < g.nreg ? i : i + g.nsink]) {
+      lo = sink_rows ? g.col_split[i] : g.col_ptr[i];
+      hi = sink_rows ? g.col_ptr[i + 1] : g.col_split[i];
+    }
+    if (hi - lo <= VP_WAVE) {
+      for (uint32_t e = lo; e < hi; e++) b.stamp[g.col_rows[e]] = a.tag;
+      lo = hi = 0;
+    }
+    uint64_t wide = __ballot(hi > lo);
+    while (wide) {
+      const uint32_t src = (uint32_t)__builtin_ctzll(wide);
+      wide &= wide - 1;
+      const uint32_t l0 = __builtin_amdgcn_readlane(lo, src), l1 = __builtin_amdgcn_readlane(hi, src);
+      for (uint32_t e = l0 + lane; e < l1; e += VP_WAVE) b.stamp[g.col_rows[e]] = a.tag;
+    }
+  }
+}
+
+// Zeroes the counters (before initialize).
+template <class P>
+__global__ void k_vp_reset_ctr(Bufs<P> b) {
+  if (threadIdx.x < 16) b.ctr[threadIdx.x] = 0;
 }
 
 // initialize() / initialize(vids) (vertex_program.hpp:84-157, 177-236): init
@@ -293,6 +563,7 @@ template <class P>
 __global__ void k_vp_init(la3_device_view g, P p, Bufs<P> b, uint32_t stationary, const uint8_t* only) {
   using S = typename P::S;
   const uint32_t src0 = g.nreg + g.nsink, src1 = src0 + g.nsrc;
+  uint32_t nsrc_act = 0;
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
     S st = S();
     bool a = false;
@@ -307,8 +578,12 @@ __global__ void k_vp_init(la3_device_view g, P p, Bufs<P> b, uint32_t stationary
       b.x[0][s] = m;
       if (s >= g.nreg) b.x[1][s] = m;  // source messages are read from either buffer
     }
-    if (sends && a && s >= g.nreg) b.act[1][s] = 1;
+    if (sends && a && s >= g.nreg) {
+      b.act[1][s] = 1;
+      nsrc_act++;
+    }
   }
+  block_add(nsrc_act, b.ctr + 6);
 }
 
 // initialize(other) (vertex_program.hpp:410-508): init(vid, other's state of
@@ -320,6 +595,7 @@ __global__ void k_vp_init_other(la3_device_view g, P p, Bufs<P> b, uint32_t stat
                                 const uint32_t* other_slot_of_id) {
   using S = typename P::S;
   const uint32_t src0 = g.nreg + g.nsink, src1 = src0 + g.nsrc;
+  uint32_t nsrc_act = 0;
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
     S st = S();
     const uint32_t vid = g.id_of[s];
@@ -334,16 +610,35 @@ __global__ void k_vp_init_other(la3_device_view g, P p, Bufs<P> b, uint32_t stat
       b.x[0][s] = m;
       if (s >= g.nreg) b.x[1][s] = m;
     }
-    if (sends && a && s >= g.nreg) b.act[1][s] = 1;
+    if (sends && a && s >= g.nreg) {
+      b.act[1][s] = 1;
+      nsrc_act++;
+    }
   }
+  block_add(nsrc_act, b.ctr + 6);
 }
 
-// Clears the regular flags of buffer `which` and the activation counter.
+// Before a pass that reads buffer which ^ 1 and writes buffer `which`:
+// clears the regular flags of `which` and the activation flag, counts the
+// active regular senders of which ^ 1 into count slot `slot` (zeroed by the
+// previous call, or at initialize) and zeroes the other slot.
 template <class P>
-__global__ void k_vp_clear(la3_device_view g, Bufs<P> b, uint32_t which) {
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nreg; s += gridDim.x * blockDim.x)
+__global__ void k_vp_clear(la3_device_view g, Bufs<P> b, uint32_t which, uint32_t slot) {
+  uint32_t n = 0;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nreg; s += gridDim.x * blockDim.x) {
     b.act[which][s] = 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) b.ctr[0] = 0;
+    n += b.act[which ^ 1][s];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    b.ctr[0] = 0;
+    b.ctr[4 + (slot ^ 1)] = 0;
+  }
+  block_add(n, b.ctr + 4 + slot);
+}
+
+__global__ void k_vp_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 // Every regular vertex scatters its final state into buffer `which` (the
@@ -424,6 +719,7 @@ class Engine {
   hipError_t initialize(const uint8_t* only_slots = nullptr) {
     hipError_t e = ensure();
     if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_vp_reset_ctr<P>, dim3(1), dim3(64), 0, stream_, b_);
     hipLaunchKernelGGL(k_vp_init<P>, dim3(vp_blocks(g_.nslots)), dim3(VP_BLOCK), 0, stream_, g_, p_, b_,
                        (uint32_t)stationary_, only_slots);
     initialized_ = true;
@@ -434,6 +730,7 @@ class Engine {
   hipError_t initialize_other(const OS* other_states, const uint32_t* other_slot_of_id) {
     hipError_t e = ensure();
     if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_vp_reset_ctr<P>, dim3(1), dim3(64), 0, stream_, b_);
     hipLaunchKernelGGL((k_vp_init_other<P, OS>), dim3(vp_blocks(g_.nslots)), dim3(VP_BLOCK), 0, stream_, g_, p_,
                        b_, (uint32_t)stationary_, other_states, other_slot_of_id);
     initialized_ = true;
@@ -476,7 +773,7 @@ class Engine {
       // Sink step: every regular vertex re-scatters its final state; the
       // sink rows take source and regular messages once (:131-148).
       hipLaunchKernelGGL(k_vp_scatter_all<P>, dim3(vp_blocks(g_.nreg)), dim3(VP_BLOCK), 0, stream_, g_, p_, b_, cur_);
-      pass(g_.nreg, nrows, 1, iter, false);
+      pass(g_.nreg, nrows, 1, iter, false, true);
     }
     return sync_error();
   }
@@ -505,6 +802,16 @@ class Engine {
     al_.release(al_.self, b_.act[0]);
     al_.release(al_.self, b_.act[1]);
     al_.release(al_.self, b_.ctr);
+    al_.release(al_.self, b_.stamp);
+    al_.release(al_.self, (void*)lt_.rows);
+    al_.release(al_.self, (void*)lt_.sbase);
+    al_.release(al_.self, (void*)ltot_);
+    al_.release(al_.self, (void*)lt_.seg);
+    al_.release(al_.self, lt_.val);
+    al_.release(al_.self, lt_.mask);
+    lt_ = LongTabs<A>();
+    ltot_ = nullptr;
+    nseg_ = 0;
     b_ = Bufs<P>();
     allocated_ = false;
     initialized_ = false;
@@ -517,6 +824,11 @@ class Engine {
   P p_;
   Allocator al_;
   Bufs<P> b_;
+  LongTabs<A> lt_;
+  uint32_t* ltot_ = nullptr;
+  uint32_t nseg_ = 0;
+  uint32_t tag_ = 0;                   // the last pass's stamp
+  uint32_t ccount_ = 0, cslot_ = 0;    // count slots used alternately
   bool allocated_ = false, initialized_ = false;
   uint32_t cur_ = 0;
 
@@ -535,26 +847,87 @@ class Engine {
     b_.act[0] = (uint8_t*)al_.alloc(al_.self, "vp.a0", S_ + 16);
     b_.act[1] = (uint8_t*)al_.alloc(al_.self, "vp.a1", S_ + 16);
     b_.ctr = (uint32_t*)al_.alloc(al_.self, "vp.ctr", 64);
+    b_.stamp = (uint32_t*)al_.alloc(al_.self, "vp.stamp", 4 * S_);
     allocated_ = true;
-    if (!b_.v || (gather_with_state && !b_.vm) || !b_.x[0] || !b_.x[1] || !b_.act[0] || !b_.act[1] || !b_.ctr) {
+    if (!b_.v || (gather_with_state && !b_.vm) || !b_.x[0] || !b_.x[1] || !b_.act[0] || !b_.act[1] || !b_.ctr ||
+        !b_.stamp) {
       release();
       err = "device allocation failed";
       return hipErrorOutOfMemory;
+    }
+    tag_ = 0;
+    hipLaunchKernelGGL(k_vp_fill_u32, dim3(vp_blocks(S_)), dim3(VP_BLOCK), 0, stream_, b_.stamp, (uint64_t)S_, 0u);
+    const hipError_t e = ensure_long();
+    if (e != hipSuccess) {
+      release();
+      err = e == hipErrorOutOfMemory ? "device allocation failed" : hipGetErrorString(e);
+      return e;
     }
     return hipSuccess;
   }
 
   void clear(uint32_t which) {
-    hipLaunchKernelGGL(k_vp_clear<P>, dim3(vp_blocks(g_.nreg)), dim3(VP_BLOCK), 0, stream_, g_, b_, which);
+    cslot_ = (ccount_++) & 1;
+    hipLaunchKernelGGL(k_vp_clear<P>, dim3(std::min<uint32_t>(vp_blocks(g_.nreg), 1024)), dim3(VP_BLOCK), 0, stream_,
+                       g_, b_, which, cslot_);
   }
 
-  void pass(uint32_t r0, uint32_t r1, bool use_src, uint32_t iter, bool single) {
+  // One pass over rows [r0, r1): with dense_only, every row (the sink step
+  // after every regular vertex re-scattered); otherwise dense or sparse by
+  // the counted senders.
+  void pass(uint32_t r0, uint32_t r1, bool use_src, uint32_t iter, bool single, bool dense_only = false) {
     if (r1 <= r0) return;
-    PassArgs a{r0, r1, cur_, iter, (uint32_t)use_src, (uint32_t)single, (uint32_t)stationary_};
+    if (++tag_ == 0) {  // stamps wrapped: clear them
+      hipLaunchKernelGGL(k_vp_fill_u32, dim3(vp_blocks(g_.nslots)), dim3(VP_BLOCK), 0, stream_, b_.stamp,
+                         (uint64_t)g_.nslots, 0u);
+      tag_ = 1;
+    }
+    PassArgs a{r0, r1, cur_, iter, (uint32_t)use_src, (uint32_t)single, (uint32_t)stationary_,
+               tag_, cslot_, (uint32_t)dense_only};
+    if (!dense_only) {
+      const uint32_t nc = g_.nreg + (use_src ? g_.nsrc : 0);
+      hipLaunchKernelGGL(k_vp_mark<P>, dim3(vp_blocks(nc)), dim3(VP_BLOCK), 0, stream_, g_, b_, a);
+    }
     hipLaunchKernelGGL(k_vp_pull<P>, dim3(vp_blocks(r1 - r0)), dim3(VP_BLOCK), 0, stream_, g_, p_, b_, a);
-    if (g_.nlong)
-      hipLaunchKernelGGL(k_vp_pull_long<P>, dim3(vp_blocks((uint64_t)g_.nlong * VP_WAVE)), dim3(VP_BLOCK), 0,
-                         stream_, g_, p_, b_, a);
+    if (g_.nlong) {
+      hipLaunchKernelGGL(k_vp_gather_long<P>, dim3(vp_blocks((uint64_t)nseg_ * VP_WAVE)), dim3(VP_BLOCK), 0,
+                         stream_, g_, p_, b_, a, lt_);
+      hipLaunchKernelGGL(k_vp_fold_long<P>, dim3(vp_blocks((uint64_t)g_.nlong * VP_WAVE)), dim3(VP_BLOCK), 0,
+                         stream_, g_, p_, b_, a, lt_);
+    }
+  }
+
+  // The long-row tables and staging (once per program; one synchronization
+  // to size the staging).
+  hipError_t ensure_long() {
+    if (!g_.nlong) return hipSuccess;
+    const uint32_t nb = (g_.nlong + VP_TAB - 1) / VP_TAB;
+    LongRow* rows = (LongRow*)al_.alloc(al_.self, "vp.lrows", sizeof(LongRow) * (size_t)g_.nlong);
+    uint32_t* sbase = (uint32_t*)al_.alloc(al_.self, "vp.lsbase", 4ull * g_.nlong);
+    uint32_t* totals = (uint32_t*)al_.alloc(al_.self, "vp.ltot", 16 + 8ull * nb);  // totals, block sums
+    lt_.rows = rows;
+    lt_.sbase = sbase;
+    lt_.nseg = totals ? totals + 1 : nullptr;
+    ltot_ = totals;
+    if (!rows || !sbase || !totals) return hipErrorOutOfMemory;
+    hipLaunchKernelGGL(k_vp_long_sums, dim3(nb), dim3(VP_TAB), 0, stream_, g_, totals + 4);
+    hipLaunchKernelGGL(k_vp_long_scan, dim3(1), dim3(VP_TAB), 0, stream_, totals + 4, nb, totals);
+    hipLaunchKernelGGL(k_vp_long_table, dim3(nb), dim3(VP_TAB), 0, stream_, g_, (const uint32_t*)(totals + 4),
+                       rows, sbase);
+    uint32_t tot[2] = {0, 0};
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(tot, totals, 8, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    if (e != hipSuccess) return e;
+    nseg_ = tot[1];
+    uint32_t* seg = (uint32_t*)al_.alloc(al_.self, "vp.lseg", 4ull * (nseg_ ? nseg_ : 1));
+    lt_.seg = seg;
+    lt_.val = (A*)al_.alloc(al_.self, "vp.lval", sizeof(A) * VP_WAVE * (size_t)(tot[0] ? tot[0] : 1));
+    lt_.mask = (uint64_t*)al_.alloc(al_.self, "vp.lmask", 8ull * (tot[0] ? tot[0] : 1));
+    if (!seg || !lt_.val || !lt_.mask) return hipErrorOutOfMemory;
+    hipLaunchKernelGGL(k_vp_long_segs, dim3(vp_blocks(g_.nlong)), dim3(VP_BLOCK), 0, stream_, g_,
+                       (const LongRow*)rows, (const uint32_t*)sbase, seg);
+    return hipGetLastError();
   }
 
   hipError_t sync_error() {
