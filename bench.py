@@ -466,6 +466,8 @@ def workload_serve(args, L):
                 srv.submit(kd, gids[i], int(graphs[i][1][0]), 30 if kd == L.Q_PPR else 0)
         srv.drain()
         srv.completed()
+        if not args.serve_cold:
+            srv.warm()  # every worker's scratch at working size: no growth frees in the stream
         t0 = srv.clock()
         for j in range(nq):
             while srv.clock() - t0 < arrivals[j]:
@@ -500,7 +502,7 @@ def workload_serve(args, L):
                      f"serving: {nq} queries, Poisson {qps} QPS, 50/30/20 BFS/SSSP/PPR(30 it), "
                      "six resident R-MAT graphs s18-22,24 seeds 10-15, Zipf(1.1) by size rank; "
                      "continuous batching, batches of one (program, graph), 3 workers",
-                     queries_per_s=round(nq / span_b, 2),
+                     queries_per_s=round(nq / span_b, 2), warm_workers=not args.serve_cold,
                      offered_qps=round(nq / float(arrivals[-1]), 2),
                      p50_ms=round(float(np.percentile(lat_b, 50)), 3),
                      p99_ms=round(float(np.percentile(lat_b, 99)), 3),
@@ -670,6 +672,8 @@ def main():
     ap.add_argument("--qps", type=float, default=400.0)
     ap.add_argument("--serve-queries", type=int, default=2000)
     ap.add_argument("--serve-batch", type=int, default=64)
+    ap.add_argument("--serve-cold", action="store_true",
+                    help="serve: skip the server warm-up (workers grow scratch during the stream)")
     ap.add_argument("--serve-workers", type=int, default=1,
                     help="workers (sessions) of the fused serving mode")
     args = ap.parse_args()

@@ -442,6 +442,14 @@ typedef struct {
 la3_status la3_server_create(int device, const la3_server_config* cfg, la3_server** out);
 /* Registers a resident graph (loaded on any context of the same device). */
 la3_status la3_server_add_graph(la3_server* s, la3_graph* g, uint32_t* graph_id);
+/* Warms every worker before a stream: each runs, on its own context, one
+ * query and a batch of max_batch queries of each program on each registered
+ * graph, so that its scratch has reached its working size and no scratch
+ * growth (whose hipFree waits for the whole device and stalls the other
+ * workers) falls into the stream. Blocks until every worker is done; call
+ * it with no queries pending. A no-op in fused mode. No counterpart in the
+ * reference. */
+la3_status la3_server_warm(la3_server* s);
 /* Bytes of per-query state a query of `kind` on the graph is admitted with. */
 uint64_t la3_server_query_bytes(const la3_server* s, la3_query_kind kind, uint32_t graph_id);
 la3_status la3_server_submit(la3_server* s, la3_query_kind kind, uint32_t graph_id, uint32_t root,

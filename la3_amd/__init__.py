@@ -35,7 +35,7 @@ EXPORTS = [
     "la3_bfs_batch_ctx", "la3_sssp_batch_ctx", "la3_ppr_batch_ctx", "la3_bfs_ctx", "la3_sssp_ctx",
     "la3_server_create", "la3_server_add_graph", "la3_server_query_bytes", "la3_server_submit",
     "la3_server_drain", "la3_server_completed", "la3_server_result", "la3_server_clock",
-    "la3_server_destroy", "la3_graph_set_tuning", "la3_graph_get_tuning", "la3_autotune",
+    "la3_server_destroy", "la3_server_warm", "la3_graph_set_tuning", "la3_graph_get_tuning", "la3_autotune",
     "la3_batch_run", "la3_session_create", "la3_session_add", "la3_session_step",
     "la3_session_poll", "la3_session_free_slots", "la3_session_destroy", "la3_ctx_release_scratch",
     "la3_result_topk", "la3_ctx_set_ingest", "la3_graph_array", "la3_components",
@@ -161,6 +161,7 @@ def lib():
             "la3_server_result": (i32, [P, u64, P, u64]),
             "la3_server_clock": (ctypes.c_double, [P]),
             "la3_server_destroy": (None, [P]),
+            "la3_server_warm": (i32, [P]),
             "la3_graph_set_tuning": (i32, [P, i32, u32]),
             "la3_graph_get_tuning": (i32, [P, i32, ctypes.POINTER(u32)]),
             "la3_autotune": (i32, [P, P, u32, ctypes.POINTER(AutotuneReport)]),
@@ -630,6 +631,14 @@ class Server:
             raise LA3Error(st, "la3_server_add_graph")
         self.graphs.append(g)  # keeps the graph alive while served
         return gid.value
+
+    def warm(self):
+        """Every worker runs each program on each graph once, single and
+        max_batch wide (la3_server_warm): scratch at working size before a
+        stream."""
+        st = lib().la3_server_warm(self.h)
+        if st:
+            raise LA3Error(st, "la3_server_warm")
 
     def query_bytes(self, kind, gid):
         return int(lib().la3_server_query_bytes(self.h, kind, gid))

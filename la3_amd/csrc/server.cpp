@@ -29,6 +29,7 @@ struct Query {
 struct GraphEntry {
   la3_graph* g;
   uint32_t nslots;
+  bool weighted, directed;
 };
 
 }  // namespace
@@ -49,6 +50,7 @@ struct la3_server {
   std::set<std::tuple<int, uint32_t, uint32_t>> running_groups;
   bool stop = false;
   std::vector<std::thread> workers;
+  uint64_t warm_gen = 0, warm_done = 0;  // la3_server_warm requests / worker completions
 
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -242,10 +244,31 @@ void worker_fused(la3_server* s, la3_ctx* ctx) {
   }
 }
 
+// la3_server_warm on one worker: each program on each graph, one query and
+// max_batch queries (roots spread over the vertex ids), on the worker's
+// context, results left on the device; statuses are not kept (a program a
+// graph does not support fails at once).
+void warm_worker(la3_server* s, la3_ctx* ctx, const std::vector<GraphEntry>& graphs) {
+  const uint32_t kmax = s->cfg.max_batch;
+  for (const GraphEntry& ge : graphs) {
+    std::vector<uint32_t> roots(kmax);
+    for (uint32_t i = 0; i < kmax; i++) roots[i] = (uint32_t)(((uint64_t)i * ge.nslots) / kmax);
+    (void)la3_bfs_ctx(ctx, ge.g, roots[0], 0, nullptr, nullptr, nullptr);
+    (void)la3_bfs_batch_ctx(ctx, ge.g, roots.data(), kmax, 0, nullptr, nullptr);
+    if (ge.weighted && ge.directed) {
+      (void)la3_sssp_ctx(ctx, ge.g, roots[0], 0, nullptr, nullptr);
+      (void)la3_sssp_batch_ctx(ctx, ge.g, roots.data(), kmax, 0, nullptr, nullptr);
+    }
+    if (ge.directed) (void)la3_ppr_batch_ctx(ctx, ge.g, roots.data(), kmax, 1, s->cfg.ppr_alpha, nullptr, nullptr, nullptr);
+  }
+  (void)la3_ctx_synchronize(ctx);
+}
+
 void worker_main(la3_server* s, la3_ctx* ctx) {
   if (s->cfg.fused) return worker_fused(s, ctx);
   void* host = nullptr;
   uint64_t host_bytes = 0;
+  uint64_t warmed = 0;
   for (;;) {
     std::vector<Query> batch;
     uint64_t bytes = 0;
@@ -258,6 +281,16 @@ void worker_main(la3_server* s, la3_ctx* ctx) {
           if (host) la3_host_free(host);
           la3_ctx_destroy(ctx);
           return;
+        }
+        if (warmed < s->warm_gen) {
+          const std::vector<GraphEntry> graphs = s->graphs;
+          warmed = s->warm_gen;
+          lk.unlock();
+          warm_worker(s, ctx, graphs);
+          lk.lock();
+          s->warm_done++;
+          s->cv.notify_all();
+          continue;
         }
         batch = pick_group(s, s->cfg.max_batch, bytes, true);
         if (!batch.empty()) break;
@@ -341,7 +374,18 @@ la3_status la3_server_add_graph(la3_server* s, la3_graph* g, uint32_t* graph_id)
   if (st) return st;
   std::lock_guard<std::mutex> lk(s->mu);
   *graph_id = (uint32_t)s->graphs.size();
-  s->graphs.push_back({g, info.nslots});
+  s->graphs.push_back({g, info.nslots, info.weighted != 0, info.directed != 0});
+  return LA3_OK;
+}
+
+la3_status la3_server_warm(la3_server* s) {
+  if (!s) return LA3_EINVAL;
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (s->cfg.fused) return LA3_OK;
+  s->warm_gen++;
+  const uint64_t target = s->warm_gen * s->workers.size();
+  s->cv.notify_all();
+  s->cv.wait(lk, [&] { return s->warm_done >= target || s->stop; });
   return LA3_OK;
 }
 

@@ -26,6 +26,8 @@ def graphs(gpu_ctx):
 def test_mixed_stream_equals_alone(gpu_ctx, graphs, fused):
     srv = L.Server(0, nworkers=3, max_batch=16, state_budget=1 << 30, keep_results=True, fused=fused)
     gids = [srv.add_graph(g) for g, _ in graphs]
+    srv.warm()  # every worker runs each program on each graph (a no-op when fused)
+    assert srv.completed() == []  # warm-up runs are not queries
     rng = np.random.default_rng(7)
     subs = []
     for i in range(120):
