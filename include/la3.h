@@ -520,6 +520,22 @@ typedef enum { LA3_DT_U8 = 0, LA3_DT_U32 = 1, LA3_DT_F32 = 2, LA3_DT_F64 = 3 } l
 la3_status la3_result_topk(la3_ctx* ctx, la3_dtype dtype, uint32_t rows, uint64_t n, uint32_t k, int largest,
                            uint32_t* ids, void* values);
 
+/* Per-row reductions of the last device-resident result, for the checksums
+ * the reference's drivers print with VertexProgram::reduce
+ * (vertex_program.hpp:558-604; bfs.cpp:28-31, sssp.cpp:28-36, cc.cpp:27-30,
+ * pr.cpp:51-54) without copying the result to the host: the result is read
+ * as rows x n values of `dtype` (by original id, as every program writes
+ * it). LA3_REDUCE_SUM: the sum of each row, into uint64_t out[rows] for
+ * LA3_DT_U8 / LA3_DT_U32 (exact) and double out[rows] for LA3_DT_F32 /
+ * LA3_DT_F64 (FP64, in a fixed order: the same on every run, possibly not in
+ * the last bits the reference's sequential sum). LA3_REDUCE_COUNT_NE: the
+ * number of values other than `value` (converted to `dtype`), into
+ * uint64_t out[rows] (e.g. reachable vertices: hops != 255, distances !=
+ * 2^31 - 1). */
+typedef enum { LA3_REDUCE_SUM = 0, LA3_REDUCE_COUNT_NE = 1 } la3_reduce_op;
+la3_status la3_result_reduce(la3_ctx* ctx, la3_dtype dtype, uint32_t rows, uint64_t n, la3_reduce_op op,
+                             double value, void* out);
+
 /* Page-locked host memory (hipHostMalloc) for result buffers: device-to-host
  * copies into it run at full link speed. Any la3_* output pointer may point
  * into it. */

@@ -35,7 +35,7 @@ EXPORTS = [
     "la3_bfs_batch_ctx", "la3_sssp_batch_ctx", "la3_ppr_batch_ctx", "la3_bfs_ctx", "la3_sssp_ctx",
     "la3_server_create", "la3_server_add_graph", "la3_server_query_bytes", "la3_server_submit",
     "la3_server_drain", "la3_server_completed", "la3_server_result", "la3_server_clock",
-    "la3_server_destroy", "la3_server_warm", "la3_graph_set_tuning", "la3_graph_get_tuning", "la3_autotune",
+    "la3_server_destroy", "la3_server_warm", "la3_result_reduce", "la3_graph_set_tuning", "la3_graph_get_tuning", "la3_autotune",
     "la3_batch_run", "la3_session_create", "la3_session_add", "la3_session_step",
     "la3_session_poll", "la3_session_free_slots", "la3_session_destroy", "la3_ctx_release_scratch",
     "la3_result_topk", "la3_ctx_set_ingest", "la3_graph_array", "la3_components",
@@ -162,6 +162,7 @@ def lib():
             "la3_server_clock": (ctypes.c_double, [P]),
             "la3_server_destroy": (None, [P]),
             "la3_server_warm": (i32, [P]),
+            "la3_result_reduce": (i32, [P, i32, u32, u64, i32, ctypes.c_double, P]),
             "la3_graph_set_tuning": (i32, [P, i32, u32]),
             "la3_graph_get_tuning": (i32, [P, i32, ctypes.POINTER(u32)]),
             "la3_autotune": (i32, [P, P, u32, ctypes.POINTER(AutotuneReport)]),
@@ -304,6 +305,19 @@ class Context:
         self.check(lib().la3_result_topk(self.h, code, int(rows), int(n), int(k), int(bool(largest)),
                                          _ptr(ids), _ptr(vals)), "result_topk")
         return ids, vals
+
+    def reduce(self, dtype, rows, n, op="sum", value=0):
+        """Per-row sum ("sum") or count of values != value ("count_ne") of
+        the last device-resident result (la3_result_reduce): uint64 sums of
+        integer results and counts, float64 sums of floating-point ones."""
+        dt = np.dtype(dtype)
+        code = {np.dtype(np.uint8): 0, np.dtype(np.uint32): 1, np.dtype(np.float32): 2,
+                np.dtype(np.float64): 3}[dt]
+        opc = {"sum": 0, "count_ne": 1}[op]
+        out = np.zeros(rows, dtype=np.float64 if (opc == 0 and dt.kind == "f") else np.uint64)
+        self.check(lib().la3_result_reduce(self.h, code, int(rows), int(n), opc, float(value), _ptr(out)),
+                   "result_reduce")
+        return out
 
     def copy_bandwidth(self, nbytes=1 << 30, reps=20):
         """Measured device copy GB/s (read + write) of a plain copy kernel."""

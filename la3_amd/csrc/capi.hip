@@ -2887,6 +2887,62 @@ void la3_session_destroy(la3_session* s) {
 // ============================================================== top-k ===
 extern "C" {
 
+}  // extern "C"
+
+namespace {
+template <class T, class Acc>
+la3_status row_reduce(la3_ctx* c, uint32_t rows, uint64_t n, la3_reduce_op op, T ne, Acc* out) {
+  const uint32_t nb = (uint32_t)((n + RD_CHUNK - 1) / RD_CHUNK);
+  Acc* part = (Acc*)scratch(c, "reduce.part", sizeof(Acc) * (size_t)rows * nb);
+  Acc* dout = (Acc*)scratch(c, "reduce.out", sizeof(Acc) * (size_t)rows);
+  if (!part || !dout) return fail(c, LA3_ENOMEM, "reduce: device allocation failed");
+  launch(c, "row_reduce", k_row_reduce<T, Acc>, rows * nb, (const T*)c->last_result, n, nb, (uint32_t)op, ne, part);
+  launch(c, "row_reduce_fin", k_row_reduce_fin<Acc>, (rows + BLOCK - 1) / BLOCK, (const Acc*)part, rows, nb, dout);
+  HIPC(c, dcopy_async(out, dout, sizeof(Acc) * (size_t)rows, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LA3_OK;
+}
+}  // namespace
+
+extern "C" {
+
+la3_status la3_result_reduce(la3_ctx* c, la3_dtype dtype, uint32_t rows, uint64_t n, la3_reduce_op op, double value,
+                             void* out) {
+  if (!c || !out || rows == 0 || n == 0 || rows > (1u << 20)) return LA3_EINVAL;
+  if (dtype < LA3_DT_U8 || dtype > LA3_DT_F64) return fail(c, LA3_EINVAL, "unknown value type");
+  if (op != LA3_REDUCE_SUM && op != LA3_REDUCE_COUNT_NE) return fail(c, LA3_EINVAL, "unknown reduction");
+  if (c->device < 0) return fail(c, LA3_EDEVICE, "host-only context");
+  const uint32_t esz = dtype == LA3_DT_U8 ? 1 : dtype == LA3_DT_F64 ? 8 : 4;
+  if (!c->last_result || (uint64_t)rows * n * esz > c->last_result_bytes)
+    return fail(c, LA3_EINVAL, "no device-resident result of that size (call a program with a NULL output first)");
+  if (!dev_covers(c->last_result, (size_t)rows * n * esz))
+    return fail(c, LA3_EINVAL, "the last result is no longer in device memory");
+  if ((n + RD_CHUNK - 1) / RD_CHUNK * (uint64_t)rows > 0x7fffffffull) return fail(c, LA3_EINVAL, "result too large");
+  HIPC(c, hipSetDevice(c->device));
+  timing_begin(c);
+  la3_status st;
+  const bool count = op == LA3_REDUCE_COUNT_NE;
+  switch (dtype) {
+    case LA3_DT_U8:
+      st = row_reduce<uint8_t, uint64_t>(c, rows, n, op, (uint8_t)value, (uint64_t*)out);
+      break;
+    case LA3_DT_U32:
+      st = row_reduce<uint32_t, uint64_t>(c, rows, n, op, (uint32_t)value, (uint64_t*)out);
+      break;
+    case LA3_DT_F32:
+      st = count ? row_reduce<float, uint64_t>(c, rows, n, op, (float)value, (uint64_t*)out)
+                 : row_reduce<float, double>(c, rows, n, op, (float)value, (double*)out);
+      break;
+    default:
+      st = count ? row_reduce<double, uint64_t>(c, rows, n, op, value, (uint64_t*)out)
+                 : row_reduce<double, double>(c, rows, n, op, value, (double*)out);
+      break;
+  }
+  if (st) return st;
+  timing_end(c);
+  return LA3_OK;
+}
+
 la3_status la3_result_topk(la3_ctx* c, la3_dtype dtype, uint32_t rows, uint64_t n, uint32_t k, int largest,
                            uint32_t* ids, void* values) {
   if (!c || !ids || rows == 0 || n == 0 || k == 0 || n > 0xffffffffull) return LA3_EINVAL;

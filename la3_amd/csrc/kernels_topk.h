@@ -134,4 +134,44 @@ __global__ void k_topk_gather(const void* v, uint32_t esz, uint64_t n, uint32_t 
   }
 }
 
+// Per-row reductions of a device-resident result (the checksums the
+// reference's drivers print): rows x n values of a type. Block b of row r
+// reduces entries [b * RD_CHUNK, (b + 1) * RD_CHUNK) of the row in a fixed
+// order (each thread over a fixed stride, then a fixed pairwise tree) into
+// part[r * nb + b]; k_row_reduce_fin adds a row's partials in block order.
+// Integer sums and counts are exact in 64 bits; floating-point values are
+// summed in FP64, in the same order on every run (the last bits may differ
+// from a sequential sum). op 0: sum; op 1: count of values != ne.
+constexpr uint32_t RD_CHUNK = 1u << 16;
+
+template <class T, class Acc>
+__global__ void __launch_bounds__(BLOCK) k_row_reduce(const T* v, uint64_t n, uint32_t nb, uint32_t op, T ne,
+                                                      Acc* part) {
+  __shared__ Acc red[BLOCK];
+  const uint32_t r = blockIdx.x / nb, b = blockIdx.x % nb;
+  const uint64_t lo = (uint64_t)b * RD_CHUNK, hi = min(n, lo + RD_CHUNK);
+  const T* row = v + (uint64_t)r * n;
+  Acc acc = 0;
+  for (uint64_t i = lo + threadIdx.x; i < hi; i += BLOCK) {
+    const T x = row[i];
+    acc += op == 0 ? (Acc)x : (Acc)(x != ne);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (uint32_t h = BLOCK / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+template <class Acc>
+__global__ void k_row_reduce_fin(const Acc* part, uint32_t rows, uint32_t nb, Acc* out) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x) {
+    Acc acc = 0;
+    for (uint32_t b = 0; b < nb; b++) acc += part[(uint64_t)r * nb + b];
+    out[r] = acc;
+  }
+}
+
 }  // namespace la3
