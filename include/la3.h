@@ -111,6 +111,32 @@ la3_status la3_graph_load_records(la3_ctx* ctx, const uint32_t* records, uint64_
 /* Graph<W>::free (src/matrix/graph.hpp:28-34). */
 void la3_graph_free(la3_graph* g);
 la3_status la3_graph_info_get(const la3_graph* g, la3_graph_info* out);
+/* A graph from the two CSCs of the reference's Graph<W> at np=1 (the
+ * arrays of CSCMatrix2D's tile, src/utils/csc.h:61-73, as la3_graph_csc
+ * exports them; [0] = regular rows (`csc`), [1] = sink rows (`sink_csc`)):
+ * for a caller whose graph is already built on the host. ncols = regular +
+ * source vertices; per CSC colptrs[ncols + 1], colidxs[ncols] (hashed column
+ * ids; unused for a column with no entries in that CSC) and per entry
+ * global_idx (position among the regular or the sink rows; may be NULL),
+ * idx (hashed row id) and weight (NULL when unweighted). `mode` is the load
+ * the arrays came from (directedness; TC needs LA3_ACYCLIC). The arrays are
+ * taken as the stored matrix: the graph is rebuilt from their entries with
+ * the same hashing, and its CSCs must equal the given arrays entry for
+ * entry, else LA3_EINVAL names the first difference (arrays that are not a
+ * reference layout are rejected, not reinterpreted). */
+typedef struct {
+  uint32_t nvertices;          /* n: vertex ids [0, n] */
+  int weighted;
+  la3_mode mode;
+  la3_hashing hashing;
+  uint32_t ncols;
+  const uint32_t* colptrs[2];
+  const uint32_t* colidxs[2];
+  const uint32_t* global_idx[2];
+  const uint32_t* idx[2];
+  const uint32_t* weight[2];
+} la3_csc_arrays;
+la3_status la3_graph_from_csc(la3_ctx* ctx, const la3_csc_arrays* arrays, la3_graph** out);
 /* Layout export for parity checks: hid_of[nslots] maps a vertex-state slot
  * (order [regular | sink | source | isolated], each by increasing hashed id,
  * src/utils/locator.h:84-120) to its hashed id. */

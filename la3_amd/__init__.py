@@ -35,7 +35,7 @@ EXPORTS = [
     "la3_bfs_batch_ctx", "la3_sssp_batch_ctx", "la3_ppr_batch_ctx", "la3_bfs_ctx", "la3_sssp_ctx",
     "la3_server_create", "la3_server_add_graph", "la3_server_query_bytes", "la3_server_submit",
     "la3_server_drain", "la3_server_completed", "la3_server_result", "la3_server_clock",
-    "la3_server_destroy", "la3_server_warm", "la3_result_reduce", "la3_graph_set_tuning", "la3_graph_get_tuning", "la3_autotune",
+    "la3_server_destroy", "la3_server_warm", "la3_result_reduce", "la3_graph_from_csc", "la3_graph_set_tuning", "la3_graph_get_tuning", "la3_autotune",
     "la3_batch_run", "la3_session_create", "la3_session_add", "la3_session_step",
     "la3_session_poll", "la3_session_free_slots", "la3_session_destroy", "la3_ctx_release_scratch",
     "la3_result_topk", "la3_ctx_set_ingest", "la3_graph_array", "la3_components",
@@ -69,6 +69,14 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("nworkers", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
                 ("state_budget", ctypes.c_uint64), ("keep_results", ctypes.c_int),
                 ("ppr_alpha", ctypes.c_double), ("fused", ctypes.c_int)]
+
+
+class CscArrays(ctypes.Structure):
+    _fields_ = [("nvertices", ctypes.c_uint32), ("weighted", ctypes.c_int), ("mode", ctypes.c_int),
+                ("hashing", ctypes.c_int), ("ncols", ctypes.c_uint32),
+                ("colptrs", ctypes.c_void_p * 2), ("colidxs", ctypes.c_void_p * 2),
+                ("global_idx", ctypes.c_void_p * 2), ("idx", ctypes.c_void_p * 2),
+                ("weight", ctypes.c_void_p * 2)]
 
 
 class QueryInfo(ctypes.Structure):
@@ -163,6 +171,7 @@ def lib():
             "la3_server_destroy": (None, [P]),
             "la3_server_warm": (i32, [P]),
             "la3_result_reduce": (i32, [P, i32, u32, u64, i32, ctypes.c_double, P]),
+            "la3_graph_from_csc": (i32, [P, P, ctypes.POINTER(P)]),
             "la3_graph_set_tuning": (i32, [P, i32, u32]),
             "la3_graph_get_tuning": (i32, [P, i32, ctypes.POINTER(u32)]),
             "la3_autotune": (i32, [P, P, u32, ctypes.POINTER(AutotuneReport)]),
@@ -349,6 +358,28 @@ class Graph:
         ctx.check(lib().la3_graph_load_records(ctx.h, _ptr(rec), nrec, int(n), int(weighted),
                                                int(mode), int(hashing), ctypes.byref(h)),
                   "graph_load_records")
+        return cls(ctx, h)
+
+    @classmethod
+    def from_csc(cls, ctx, n, csc, weighted=False, mode=DIRECTED, hashing=HASH_BUCKET):
+        """A graph from the reference's two CSCs (la3_graph_from_csc): csc =
+        (regular, sink), each a dict with colptrs, colidxs, global_idx, idx, w
+        as Graph.csc returns them."""
+        arr = CscArrays()
+        arr.nvertices, arr.weighted, arr.mode, arr.hashing = int(n), int(weighted), int(mode), int(hashing)
+        keep = []
+        arr.ncols = int(len(csc[0]["colptrs"]) - 1)
+        for w, c in enumerate(csc):
+            for field, key in (("colptrs", "colptrs"), ("colidxs", "colidxs"), ("global_idx", "global_idx"),
+                               ("idx", "idx"), ("weight", "w")):
+                a = c.get(key)
+                if a is None or (key == "w" and not weighted):
+                    continue
+                a = np.ascontiguousarray(a, dtype=np.uint32)
+                keep.append(a)
+                getattr(arr, field)[w] = a.ctypes.data
+        h = ctypes.c_void_p()
+        ctx.check(lib().la3_graph_from_csc(ctx.h, ctypes.byref(arr), ctypes.byref(h)), "graph_from_csc")
         return cls(ctx, h)
 
     @classmethod

@@ -1043,6 +1043,86 @@ la3_status la3_graph_csc(const la3_graph* g, int which, uint32_t* colptrs, uint3
   return LA3_OK;
 }
 
+la3_status la3_graph_from_csc(la3_ctx* c, const la3_csc_arrays* a, la3_graph** out) {
+  if (!c || !a || !out) return fail(c, LA3_EINVAL, "null argument");
+  *out = nullptr;
+  if ((int)a->mode < 0 || (int)a->mode > 4) return fail(c, LA3_EINVAL, "bad load mode");
+  if (a->nvertices == 0xffffffffu) return fail(c, LA3_ENOTSUP, "nvertices must be < 2^32-1");
+  const uint64_t S = (uint64_t)a->nvertices + 1;
+  const uint32_t NC = a->ncols;
+  if (NC > S) return fail(c, LA3_EINVAL, "more columns than vertices");
+  const Hasher h(a->nvertices, a->hashing != LA3_HASH_NONE);
+  // The stored matrix's entries as records (column vertex, row vertex[, w])
+  // by original id: loaded directed and not reversed, they store exactly
+  // these entries (graph.hpp:150-192), and the builder then derives the
+  // classes, slots and both CSCs from them.
+  const int wd = a->weighted ? 3 : 2;
+  std::vector<uint32_t> rec;
+  for (int which = 0; which < 2; which++) {
+    const uint32_t* cp = a->colptrs[which];
+    if (!cp || !a->colidxs[which]) return fail(c, LA3_EINVAL, "missing column arrays");
+    if (cp[0] != 0) return fail(c, LA3_EINVAL, "colptrs must start at 0");
+    for (uint32_t col = 0; col < NC; col++)
+      if (cp[col + 1] < cp[col]) return fail(c, LA3_EINVAL, "colptrs must not decrease");
+    const uint64_t nnz = cp[NC];
+    if (nnz && (!a->idx[which] || (a->weighted && !a->weight[which])))
+      return fail(c, LA3_EINVAL, "missing entry arrays");
+    rec.reserve(rec.size() + wd * nnz);
+    for (uint32_t col = 0; col < NC; col++) {
+      if (cp[col + 1] == cp[col]) continue;
+      const uint32_t hc = a->colidxs[which][col];
+      if (hc >= S) return fail(c, LA3_EINVAL, "column id out of range");
+      const uint32_t vc = (uint32_t)h.unhash(hc);
+      for (uint32_t j = cp[col]; j < cp[col + 1]; j++) {
+        const uint32_t hr = a->idx[which][j];
+        if (hr >= S) return fail(c, LA3_EINVAL, "row id out of range");
+        rec.push_back(vc);
+        rec.push_back((uint32_t)h.unhash(hr));
+        if (a->weighted) rec.push_back(a->weight[which][j]);
+      }
+    }
+  }
+  la3_graph* g = nullptr;
+  la3_status st = load_records(c, rec.data(), rec.size() / wd, a->nvertices, a->weighted, LA3_DIRECTED,
+                               a->hashing, &g);
+  if (st) return st;
+  rec = std::vector<uint32_t>();
+  // The load mode the arrays came from: directedness and TC's orientation.
+  g->mode = (int)a->mode;
+  g->directed = a->mode != LA3_UNDIRECTED;
+  if (g->host) {
+    g->host->mode = (int)a->mode;
+    g->host->directed = g->directed;
+  }
+  // Only a reference layout reproduces itself: compare both CSCs entry by entry.
+  std::string bad;
+  if (g->ncols != NC) bad = "column count " + std::to_string(g->ncols) + " != " + std::to_string(NC);
+  for (int which = 0; which < 2 && bad.empty(); which++) {
+    const uint32_t* cp = a->colptrs[which];
+    const uint64_t nnz = cp[NC];
+    std::vector<uint32_t> p(NC + 1), ci(NC + 1), gi(nnz + 1), ix(nnz + 1), w(nnz + 1);
+    if ((st = la3_graph_csc(g, which, p.data(), ci.data(), gi.data(), ix.data(), w.data()))) break;
+    const std::string name = which ? "sink CSC " : "regular CSC ";
+    if (p[NC] != nnz) { bad = name + "entry count"; break; }
+    for (uint32_t col = 0; col < NC && bad.empty(); col++) {
+      if (p[col] != cp[col]) bad = name + "colptrs[" + std::to_string(col) + "]";
+      else if (cp[col + 1] > cp[col] && ci[col] != a->colidxs[which][col]) bad = name + "colidxs[" + std::to_string(col) + "]";
+    }
+    for (uint64_t j = 0; j < nnz && bad.empty(); j++) {
+      if (a->global_idx[which] && gi[j] != a->global_idx[which][j]) bad = name + "global_idx[" + std::to_string(j) + "]";
+      else if (ix[j] != a->idx[which][j]) bad = name + "idx[" + std::to_string(j) + "]";
+      else if (a->weighted && w[j] != a->weight[which][j]) bad = name + "weight[" + std::to_string(j) + "]";
+    }
+  }
+  if (st || !bad.empty()) {
+    la3_graph_free(g);
+    if (st) return st;
+    return fail(c, LA3_EINVAL, "not a CSC layout of the reference (csc.h:76-130): " + bad + " differs");
+  }
+  *out = g;
+  return LA3_OK;
+}
+
 // ----------------------------------------------- vertex-program engine ---
 }  // extern "C"
 namespace {
