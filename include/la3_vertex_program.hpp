@@ -46,8 +46,11 @@
 // row-major companion holds a row's senders in exactly that order. So
 // results equal the reference's even for order-dependent combines
 // (last-writer-wins, floating-point sums). Rows of at most 64 entries are
-// folded by one lane; longer rows by a wavefront that gathers 64 messages at
-// a time in parallel and folds them in order from LDS.
+// folded by one lane; longer rows are gathered in parallel and folded in
+// entry order by a wavefront, or, for a program that declares its combine
+// associative or gives a segment type (seg_add / seg_join / seg_value),
+// folded as 256-entry runs by many wavefronts whose partials are joined in
+// order.
 //
 // Device memory: the states, two message buffers and two activity-flag
 // buffers (by vertex slot), plus a mirror of the states when gather reads
@@ -128,6 +131,28 @@ struct has_apply3<P, std::void_t<decltype(std::declval<const P&>().apply(
                          std::declval<const typename P::A&>(), std::declval<typename P::S&>(), uint32_t()))>>
     : std::true_type {};
 
+// Optional segmented fold, for long rows: a program whose combine is
+// associative (combine(a, combine(b, y)) == combine(combine(a, b), y)) may
+// declare `static constexpr bool associative = true`; any program may instead
+// give a segment type that reproduces its in-order fold:
+//   using Seg = ...;                                        // Seg() = no message
+//   __device__ void seg_add(const A& y, Seg& s) const;      // s := s then y
+//   __device__ void seg_join(const Seg& later, Seg& s) const; // s := s then later
+//   __device__ A seg_value(const Seg& s) const;             // the fold from A()
+// Rows of more than VP_LONG entries are then folded as 256-entry runs, by
+// many wavefronts at once, whose partial folds are joined in entry order.
+template <class P, class = void> struct has_seg : std::false_type {};
+template <class P>
+struct has_seg<P, std::void_t<typename P::Seg,
+                              decltype(std::declval<const P&>().seg_add(std::declval<const typename P::A&>(),
+                                                                        std::declval<typename P::Seg&>())),
+                              decltype(std::declval<const P&>().seg_join(std::declval<const typename P::Seg&>(),
+                                                                         std::declval<typename P::Seg&>())),
+                              decltype(std::declval<const P&>().seg_value(std::declval<const typename P::Seg&>()))>>
+    : std::true_type {};
+template <class P, class = void> struct is_assoc : std::false_type {};
+template <class P> struct is_assoc<P, std::enable_if_t<P::associative>> : std::true_type {};
+
 // ---- the operations with the reference's defaults ----
 template <class P>
 __device__ __forceinline__ bool op_init(const P& p, bool stationary, uint32_t vid, typename P::S& s) {
@@ -162,6 +187,29 @@ __device__ __forceinline__ bool op_apply(const P& p, const typename P::A& y, typ
   else if constexpr (has_apply2<P>::value) return p.apply(y, s);
   else return false;
 }
+
+// The segmented fold a program allows (on = false: none).
+template <class P, bool = has_seg<P>::value, bool = is_assoc<P>::value>
+struct SegFold {
+  static constexpr bool on = false;
+  using Seg = typename P::A;
+};
+template <class P, bool X>
+struct SegFold<P, true, X> {
+  static constexpr bool on = true;
+  using Seg = typename P::Seg;
+  __device__ static void add(const P& p, const typename P::A& y, Seg& s) { p.seg_add(y, s); }
+  __device__ static void join(const P& p, const Seg& later, Seg& s) { p.seg_join(later, s); }
+  __device__ static typename P::A value(const P& p, const Seg& s) { return p.seg_value(s); }
+};
+template <class P>
+struct SegFold<P, false, true> {  // associative combine: a run's fold is its accumulator
+  static constexpr bool on = true;
+  using Seg = typename P::A;
+  __device__ static void add(const P& p, const typename P::A& y, Seg& s) { op_combine(p, y, s); }
+  __device__ static void join(const P& p, const Seg& later, Seg& s) { op_combine(p, later, s); }
+  __device__ static typename P::A value(const P&, const Seg& s) { return s; }
+};
 
 template <class W>
 __device__ __forceinline__ Edge<W> make_edge(const la3_device_view& g, uint32_t sender, uint32_t row, uint32_t e) {
@@ -298,6 +346,8 @@ struct LongTabs {
   const uint32_t* sbase = nullptr; // first segment of each long row
   const uint32_t* seg = nullptr;   // long-row index of each segment
   const uint32_t* nseg = nullptr;  // device: number of segments
+  void* part = nullptr;            // segmented fold: one partial (SegFold::Seg) per segment
+  uint8_t* pany = nullptr;         // segmented fold: the segment held a message
   A* val = nullptr;                // 64 values per chunk
   uint64_t* mask = nullptr;        // one mask per chunk
 };
@@ -393,6 +443,40 @@ __device__ __forceinline__ LongRow uniform_row(const LongRow* rows, uint32_t li)
                  (uint32_t)__builtin_amdgcn_readfirstlane(R.e1), (uint32_t)__builtin_amdgcn_readfirstlane(R.cbase)};
 }
 
+// Lane l's copy of v from lane l + off (off < 64; lanes past 63 read their own).
+template <class T>
+__device__ __forceinline__ T wave_down(const T& v, uint32_t off) {
+  constexpr int NW = (sizeof(T) + 3) / 4;
+  int w[NW];
+  w[NW - 1] = 0;
+  memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < NW; i++) w[i] = __shfl_down(w[i], off, VP_WAVE);
+  T out;
+  memcpy(&out, w, sizeof(T));
+  return out;
+}
+
+// Ordered reduction of one partial fold per lane (lane order = entry order;
+// `ne` = the lane's run held a message): lane 0 ends with the fold of all
+// 64 runs. Empty runs are skipped, not joined (a join with Seg() is not the
+// identity for every combine, e.g. last-writer).
+template <class P>
+__device__ __forceinline__ void wave_fold(const P& p, typename SegFold<P>::Seg& s, bool& ne) {
+  using SF = SegFold<P>;
+  const uint32_t lane = threadIdx.x % VP_WAVE;
+#pragma unroll
+  for (uint32_t off = 1; off < VP_WAVE; off <<= 1) {
+    const typename SF::Seg o = wave_down(s, off);
+    const bool one = __shfl_down((int)ne, off, VP_WAVE) != 0;
+    if ((lane & (2 * off - 1)) == 0 && lane + off < VP_WAVE && one) {
+      if (ne) SF::join(p, o, s);
+      else s = o;
+      ne = true;
+    }
+  }
+}
+
 template <class P>
 __global__ void __launch_bounds__(VP_BLOCK) k_vp_gather_long(la3_device_view g, P p, Bufs<P> b, PassArgs a,
                                                             LongTabs<typename P::A> t) {
@@ -428,14 +512,78 @@ __global__ void __launch_bounds__(VP_BLOCK) k_vp_gather_long(la3_device_view g, 
 #pragma unroll
     for (int j = 0; j < 4; j++)
       if (on[j]) m[j] = x[u[j]];
-    const uint64_t c0 = (uint64_t)R.cbase + k * (VP_SEG / VP_WAVE);
+    if constexpr (SegFold<P>::on) {
+      // the segment's partial fold: per 64-entry chunk an ordered wave
+      // reduction, the chunks joined in order by lane 0
+      using SF = SegFold<P>;
+      typename SF::Seg acc = typename SF::Seg();
+      bool any = false;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const uint32_t e = eb + j * VP_WAVE + lane;
-      if (on[j]) t.val[(c0 + j) * VP_WAVE + lane] = op_gather(p, make_edge<typename P::W>(g, u[j], R.r, e), m[j], mir);
-      const uint64_t bm = __ballot(on[j]);
-      if (lane == 0 && eb + j * VP_WAVE < R.e1) t.mask[c0 + j] = bm;
+      for (int j = 0; j < 4; j++) {
+        const uint32_t e = eb + j * VP_WAVE + lane;
+        typename SF::Seg sj = typename SF::Seg();
+        bool ne = on[j];
+        if (on[j]) SF::add(p, op_gather(p, make_edge<typename P::W>(g, u[j], R.r, e), m[j], mir), sj);
+        wave_fold(p, sj, ne);
+        if (lane == 0 && ne) {
+          if (any) SF::join(p, sj, acc);
+          else acc = sj;
+          any = true;
+        }
+      }
+      if (lane == 0) {
+        ((typename SF::Seg*)t.part)[s] = acc;
+        t.pany[s] = any;
+      }
+    } else {
+      const uint64_t c0 = (uint64_t)R.cbase + k * (VP_SEG / VP_WAVE);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const uint32_t e = eb + j * VP_WAVE + lane;
+        if (on[j]) t.val[(c0 + j) * VP_WAVE + lane] = op_gather(p, make_edge<typename P::W>(g, u[j], R.r, e), m[j], mir);
+        const uint64_t bm = __ballot(on[j]);
+        if (lane == 0 && eb + j * VP_WAVE < R.e1) t.mask[c0 + j] = bm;
+      }
     }
+  }
+}
+
+// Long rows of a program with a segmented fold: a wavefront per row joins
+// its segments' partials in order (64 at a time, by an ordered wave
+// reduction), then applies.
+template <class P>
+__global__ void __launch_bounds__(VP_BLOCK) k_vp_join_long(la3_device_view g, P p, Bufs<P> b, PassArgs a,
+                                                          LongTabs<typename P::A> t) {
+  using SF = SegFold<P>;
+  const uint32_t lane = threadIdx.x % VP_WAVE;
+  const uint32_t nw = gridDim.x * (VP_BLOCK / VP_WAVE);
+  const bool sparse = vp_sparse(g, b.ctr, a);
+  const typename SF::Seg* part = (const typename SF::Seg*)t.part;
+  for (uint32_t li = blockIdx.x * (VP_BLOCK / VP_WAVE) + threadIdx.x / VP_WAVE; li < g.nlong; li += nw) {
+    const LongRow R = t.rows[li];
+    const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane(R.r);
+    if (r < a.r0 || r >= a.r1) continue;  // wave-uniform
+    if (sparse && (uint32_t)__builtin_amdgcn_readfirstlane(b.stamp[r]) != a.tag) continue;
+    const uint32_t s0 = (uint32_t)__builtin_amdgcn_readfirstlane(t.sbase[li]);
+    const uint32_t ns = ((uint32_t)__builtin_amdgcn_readfirstlane(R.e1) - (uint32_t)__builtin_amdgcn_readfirstlane(R.e0) +
+                         VP_SEG - 1) / VP_SEG;
+    typename SF::Seg acc = typename SF::Seg();
+    bool any = false;
+    for (uint32_t c = 0; c < ns; c += VP_WAVE) {  // wave-uniform
+      typename SF::Seg sc = typename SF::Seg();
+      bool ne = false;
+      if (c + lane < ns) {
+        ne = t.pany[s0 + c + lane] != 0;
+        if (ne) sc = part[s0 + c + lane];
+      }
+      wave_fold(p, sc, ne);
+      if (lane == 0 && ne) {
+        if (any) SF::join(p, sc, acc);
+        else acc = sc;
+        any = true;
+      }
+    }
+    if (lane == 0 && any) vp_finish_row(p, g, b, a, r, SF::value(p, acc));
   }
 }
 
@@ -809,6 +957,8 @@ class Engine {
     al_.release(al_.self, (void*)lt_.seg);
     al_.release(al_.self, lt_.val);
     al_.release(al_.self, lt_.mask);
+    al_.release(al_.self, lt_.part);
+    al_.release(al_.self, lt_.pany);
     lt_ = LongTabs<A>();
     ltot_ = nullptr;
     nseg_ = 0;
@@ -892,8 +1042,12 @@ class Engine {
     if (g_.nlong) {
       hipLaunchKernelGGL(k_vp_gather_long<P>, dim3(vp_blocks((uint64_t)nseg_ * VP_WAVE)), dim3(VP_BLOCK), 0,
                          stream_, g_, p_, b_, a, lt_);
-      hipLaunchKernelGGL(k_vp_fold_long<P>, dim3(vp_blocks((uint64_t)g_.nlong * VP_WAVE)), dim3(VP_BLOCK), 0,
-                         stream_, g_, p_, b_, a, lt_);
+      if constexpr (SegFold<P>::on)
+        hipLaunchKernelGGL(k_vp_join_long<P>, dim3(vp_blocks((uint64_t)g_.nlong * VP_WAVE)), dim3(VP_BLOCK), 0,
+                           stream_, g_, p_, b_, a, lt_);
+      else
+        hipLaunchKernelGGL(k_vp_fold_long<P>, dim3(vp_blocks((uint64_t)g_.nlong * VP_WAVE)), dim3(VP_BLOCK), 0,
+                           stream_, g_, p_, b_, a, lt_);
     }
   }
 
@@ -922,9 +1076,15 @@ class Engine {
     nseg_ = tot[1];
     uint32_t* seg = (uint32_t*)al_.alloc(al_.self, "vp.lseg", 4ull * (nseg_ ? nseg_ : 1));
     lt_.seg = seg;
-    lt_.val = (A*)al_.alloc(al_.self, "vp.lval", sizeof(A) * VP_WAVE * (size_t)(tot[0] ? tot[0] : 1));
-    lt_.mask = (uint64_t*)al_.alloc(al_.self, "vp.lmask", 8ull * (tot[0] ? tot[0] : 1));
-    if (!seg || !lt_.val || !lt_.mask) return hipErrorOutOfMemory;
+    if constexpr (SegFold<P>::on) {  // one partial per segment
+      lt_.part = al_.alloc(al_.self, "vp.lpart", sizeof(typename SegFold<P>::Seg) * (size_t)(nseg_ ? nseg_ : 1));
+      lt_.pany = (uint8_t*)al_.alloc(al_.self, "vp.lpany", nseg_ ? nseg_ : 1);
+      if (!seg || !lt_.part || !lt_.pany) return hipErrorOutOfMemory;
+    } else {  // every entry's value and per-chunk masks
+      lt_.val = (A*)al_.alloc(al_.self, "vp.lval", sizeof(A) * VP_WAVE * (size_t)(tot[0] ? tot[0] : 1));
+      lt_.mask = (uint64_t*)al_.alloc(al_.self, "vp.lmask", 8ull * (tot[0] ? tot[0] : 1));
+      if (!seg || !lt_.val || !lt_.mask) return hipErrorOutOfMemory;
+    }
     hipLaunchKernelGGL(k_vp_long_segs, dim3(vp_blocks(g_.nlong)), dim3(VP_BLOCK), 0, stream_, g_,
                        (const LongRow*)rows, (const uint32_t*)sbase, seg);
     return hipGetLastError();

@@ -1181,6 +1181,24 @@ struct CcOps {
     s.label = min(s.label, y);
     return t != s.label;
   }
+  // Runs of a long row's messages folded separately and joined in order
+  // (exactly the in-order fold): a 0 message empties the accumulator, so a
+  // run holding one folds to its own value whatever came before it; a run
+  // without one folds into the accumulator by combine.
+  struct Seg {
+    uint32_t v = 0;  // the run's fold from A() = 0
+    uint32_t z = 0;  // the run holds a 0 message
+  };
+  __device__ void seg_add(const A& y, Seg& sg) const {
+    combine(y, sg.v);
+    sg.z |= y == 0;
+  }
+  __device__ void seg_join(const Seg& later, Seg& sg) const {
+    if (later.z) sg.v = later.v;
+    else if (later.v != 0) combine(later.v, sg.v);  // else: the run held nothing
+    sg.z |= later.z;
+  }
+  __device__ A seg_value(const Seg& sg) const { return sg.v; }
 };
 
 }  // namespace

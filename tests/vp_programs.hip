@@ -14,8 +14,12 @@ namespace {
 
 thread_local std::string g_err;
 
-// Weighted in-degree: stationary, weights, execute_single.
+// Weighted in-degree: stationary, weights, execute_single. The three programs
+// below declare their combine associative (integer sum, max, last writer),
+// so their long rows take the engine's segmented fold; the others keep the
+// in-order fold of every message.
 struct WDeg {
+  static constexpr bool associative = true;
   using W = uint32_t;
   using M = la3::Empty;
   using A = uint32_t;
@@ -33,6 +37,7 @@ struct WDeg {
 // Max-label propagation: execute_ with the sink step on directed loads,
 // execute_non_opt on undirected ones.
 struct MaxLabel {
+  static constexpr bool associative = true;
   using W = la3::Empty;
   using M = uint32_t;
   using A = uint32_t;
@@ -57,6 +62,7 @@ struct MaxLabel {
 // iteration, and the combine keeps the last message, so the result depends
 // on the order in which a row combines its messages.
 struct Level {
+  static constexpr bool associative = true;
   using W = la3::Empty;
   using M = la3::Empty;
   using A = uint32_t;
