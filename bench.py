@@ -264,9 +264,9 @@ def workload_cc_tc_s20(args, L):
                      cc_exact=bool((lab == elab).all()), tc_total=total, tc_exact=total == etotal,
                      tc_ms=round(el_tc * 1e3 / args.steps, 3), roofline=None,
                      roofline_note="CC runs on the VertexProgram template, whose kernels are launched outside "
-                                   "the library's event timing; its dominant kernel, the long-row fold, folds a "
-                                   "hub row's messages in entry order on one wavefront (serial, not bandwidth-"
-                                   "bound); per-kernel times in profiles/vp_cc_tc_s20_kernel_stats.md",
+                                   "the library's event timing; a run is about 25 launches of 5-230 us (launch- "
+                                   "and latency-bound at this size); per-kernel times in "
+                                   "profiles/vp_cc_tc_s20_kernel_stats.md",
                      tc_roofline=roofline(st_tc),
                      cpu_baseline={"value": cc_e / cpu_cc / 1e9, "unit": "GTEPS", "cores": 1,
                                    "kind": "port", "sample": "one CC run (oracle label propagation)",
