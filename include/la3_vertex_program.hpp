@@ -703,21 +703,32 @@ __global__ void k_vp_reset_ctr(Bufs<P> b) {
   if (threadIdx.x < 16) b.ctr[threadIdx.x] = 0;
 }
 
+// States value-initialized, as the reference's vertex vector is on
+// allocation (random_access_array.h:26-28).
+template <class S>
+__global__ void k_vp_fill_states(S* v, uint32_t n) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) v[s] = S();
+}
+
 // initialize() / initialize(vids) (vertex_program.hpp:84-157, 177-236): init
-// per vertex (all, or the slots flagged in `only`); regular and, on a
-// directed graph, source vertices whose init returns true scatter into
-// message buffer 0. Flags and messages of buffer 1 start clear.
+// on each vertex's current state (all vertices, or the slots flagged in
+// `only`; the others keep theirs); regular and, on a directed graph, source
+// vertices whose init returns true scatter into message buffer 0; no other
+// vertex has a message pending. Flags and messages of buffer 1 start clear.
+// With `mirror` (the first initialize of the program's states), the mirrors
+// take the active states (vertex_program.hpp:159-173).
 template <class P>
-__global__ void k_vp_init(la3_device_view g, P p, Bufs<P> b, uint32_t stationary, const uint8_t* only) {
+__global__ void k_vp_init(la3_device_view g, P p, Bufs<P> b, uint32_t stationary, const uint8_t* only,
+                          uint32_t mirror) {
   using S = typename P::S;
   const uint32_t src0 = g.nreg + g.nsink, src1 = src0 + g.nsrc;
   uint32_t nsrc_act = 0;
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
-    S st = S();
+    S st = b.v[s];
     bool a = false;
     if (!only || only[s]) a = op_init(p, stationary != 0, g.id_of[s], st);
     b.v[s] = st;
-    if (b.vm) b.vm[s] = stationary ? st : S();  // the active states (vertex_program.hpp:159-173)
+    if (b.vm && mirror) b.vm[s] = stationary ? st : S();
     const bool sends = s < g.nreg || (g.directed && s >= src0 && s < src1);
     b.act[0][s] = sends && a;
     b.act[1][s] = 0;
@@ -740,16 +751,16 @@ __global__ void k_vp_init(la3_device_view g, P p, Bufs<P> b, uint32_t stationary
 // the same state to the same vertex.
 template <class P, class OS>
 __global__ void k_vp_init_other(la3_device_view g, P p, Bufs<P> b, uint32_t stationary, const OS* other,
-                                const uint32_t* other_slot_of_id) {
+                                const uint32_t* other_slot_of_id, uint32_t mirror) {
   using S = typename P::S;
   const uint32_t src0 = g.nreg + g.nsink, src1 = src0 + g.nsrc;
   uint32_t nsrc_act = 0;
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
-    S st = S();
+    S st = b.v[s];
     const uint32_t vid = g.id_of[s];
     const bool a = op_init_other(p, stationary != 0, vid, other[other_slot_of_id[vid]], st);
     b.v[s] = st;
-    if (b.vm) b.vm[s] = stationary ? st : S();  // the active states (vertex_program.hpp:159-173)
+    if (b.vm && mirror) b.vm[s] = stationary ? st : S();  // vertex_program.hpp:490-505
     const bool sends = s < g.nreg || (g.directed && s >= src0 && s < src1);
     b.act[0][s] = sends && a;
     b.act[1][s] = 0;
@@ -869,8 +880,10 @@ class Engine {
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_vp_reset_ctr<P>, dim3(1), dim3(64), 0, stream_, b_);
     hipLaunchKernelGGL(k_vp_init<P>, dim3(vp_blocks(g_.nslots)), dim3(VP_BLOCK), 0, stream_, g_, p_, b_,
-                       (uint32_t)stationary_, only_slots);
+                       (uint32_t)stationary_, only_slots, (uint32_t)!mirrored_);
     initialized_ = true;
+    mirrored_ = true;
+    cur_ = 0;
     return hipGetLastError();
   }
 
@@ -880,8 +893,10 @@ class Engine {
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_vp_reset_ctr<P>, dim3(1), dim3(64), 0, stream_, b_);
     hipLaunchKernelGGL((k_vp_init_other<P, OS>), dim3(vp_blocks(g_.nslots)), dim3(VP_BLOCK), 0, stream_, g_, p_,
-                       b_, (uint32_t)stationary_, other_states, other_slot_of_id);
+                       b_, (uint32_t)stationary_, other_states, other_slot_of_id, (uint32_t)!mirrored_);
     initialized_ = true;
+    mirrored_ = true;
+    cur_ = 0;
     return hipGetLastError();
   }
 
@@ -892,9 +907,10 @@ class Engine {
     const bool directed = g_.directed != 0;
     const bool opt = optimizable && directed && !gather_with_state && !apply_with_iter;
     const uint32_t nrows = g_.nreg + g_.nsink;
-    cur_ = 0;
+    // A second execute() without reset() continues from the messages the
+    // first one left (the reference initializes only once, :15-16).
     if (max_iters == 1) {  // execute_single / execute_single_undirected (:251-345)
-      clear(1);
+      clear(cur_ ^ 1);
       pass(0, g_.nreg, 1, 0, true);
       if (directed) pass(g_.nreg, nrows, 1, 0, true);
       iters_run = 1;
@@ -937,7 +953,8 @@ class Engine {
     return e == hipSuccess ? hipSuccess : fail(e);
   }
 
-  // reset() (vertex_program.hpp:51-63): the next execute() initializes again.
+  // reset() (vertex_program.hpp:51-63): the next execute() initializes again
+  // (pending messages dropped); the states stay.
   void reset() { initialized_ = false; }
 
   void release() {
@@ -965,6 +982,7 @@ class Engine {
     b_ = Bufs<P>();
     allocated_ = false;
     initialized_ = false;
+    mirrored_ = false;
   }
 
  private:
@@ -979,7 +997,7 @@ class Engine {
   uint32_t nseg_ = 0;
   uint32_t tag_ = 0;                   // the last pass's stamp
   uint32_t ccount_ = 0, cslot_ = 0;    // count slots used alternately
-  bool allocated_ = false, initialized_ = false;
+  bool allocated_ = false, initialized_ = false, mirrored_ = false;
   uint32_t cur_ = 0;
 
   hipError_t fail(hipError_t e) {
@@ -1007,6 +1025,7 @@ class Engine {
     }
     tag_ = 0;
     hipLaunchKernelGGL(k_vp_fill_u32, dim3(vp_blocks(S_)), dim3(VP_BLOCK), 0, stream_, b_.stamp, (uint64_t)S_, 0u);
+    hipLaunchKernelGGL(k_vp_fill_states<S>, dim3(vp_blocks(S_)), dim3(VP_BLOCK), 0, stream_, b_.v, (uint32_t)S_);
     const hipError_t e = ensure_long();
     if (e != hipSuccess) {
       release();
@@ -1117,15 +1136,21 @@ class VertexProgram {
         e_(view_, (hipStream_t)la3_ctx_stream(ctx), stationary, ops) {}
 
   // LA3-optimizable execution (vertex_program.h:140-148); cleared to force
-  // execute_non_opt.
+  // execute_non_opt. Every initialize ANDs in the graph's and the program's
+  // conditions, as initialize_flags does (vertex_program.hpp:66-81).
   bool optimizable = true;
 
-  // initialize() (vertex_program.hpp:84-174).
-  void initialize() { check(e_.initialize(), "initialize"); }
+  // initialize() (vertex_program.hpp:84-174): init on every vertex's
+  // current state (S() until the first execute).
+  void initialize() {
+    and_flags();
+    check(e_.initialize(), "initialize");
+  }
 
   // initialize(vids) (vertex_program.hpp:177-259): init only the listed
   // vertices; the others keep S() and send nothing.
   void initialize(const std::vector<uint32_t>& vids) {
+    and_flags();
     std::vector<uint32_t> slot(view_.nslots);
     check(hipMemcpy(slot.data(), view_.slot_of_id, 4ull * view_.nslots, hipMemcpyDeviceToHost), "initialize");
     std::vector<uint8_t> only(view_.nslots, 0);
@@ -1147,16 +1172,21 @@ class VertexProgram {
   // set (typically the reversed graph).
   template <class P2>
   void initialize(const VertexProgram<P2>& other) {
+    and_flags();
     check(e_.initialize_other(other.device_states(), other.view().slot_of_id), "initialize(other)");
   }
 
-  // execute(max_iters) (vertex_program_execute.hpp:11-52).
+  // execute(max_iters) (vertex_program_execute.hpp:11-52); without a prior
+  // initialize it initializes, and a second execute continues from the
+  // messages the first left.
   void execute(uint32_t max_iters = 0) {
+    and_flags();
     e_.optimizable = optimizable;
     check(e_.execute(max_iters), "execute");
   }
 
-  // reset() (vertex_program.hpp:51-63) and free() (:38-48).
+  // reset() (vertex_program.hpp:51-63): pending messages are dropped and the
+  // next execute initializes again; the states stay. free() (:38-48).
   void reset() { e_.reset(); }
   void free() { e_.release(); }
 
@@ -1191,6 +1221,11 @@ class VertexProgram {
   la3_graph* g_;
   la3_device_view view_;
   vp_detail::Engine<P> e_;
+
+  void and_flags() {
+    optimizable = optimizable && view_.directed && !vp_detail::Engine<P>::gather_with_state &&
+                  !vp_detail::Engine<P>::apply_with_iter;
+  }
 
   static la3_device_view view_of(la3_ctx* ctx, la3_graph* g) {
     la3_device_view v;

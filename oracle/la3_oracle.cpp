@@ -472,6 +472,21 @@ struct CountNonzeroProg {
   bool apply(const A& y, S& s) { s.cnt += y; return y > 0; }
 };
 
+// Stationary, gather reads the row's state, and init adds to the state it
+// finds: a second initialize (after reset) changes the states but not the
+// mirrors, which keep the first initialize's values.
+struct AccGreaterProg {
+  using M = uint32_t; using A = uint32_t;
+  struct S { uint32_t val = 0; uint32_t cnt = 0; };
+  static constexpr bool stationary = true, gather_state = true, apply_iter = false;
+  A identity() const { return 0; }
+  bool init(uint32_t vid, S& s) { s.val += vid % 5 + 1; return true; }
+  M scatter(const S& s) { return s.val; }
+  A gather(const EdgeV&, const M& m, const S& s) { return m > s.val ? 1u : 0u; }
+  void combine(const A& y1, A& y2) { y2 += y1; }
+  bool apply(const A& y, S& s) { s.cnt += y; s.val = y % 7; return false; }
+};
+
 // ---------------------------------------------------------------- engine ---
 template <class P>
 struct Engine {
@@ -494,14 +509,48 @@ struct Engine {
 
   uint32_t vid_of(uint32_t pos) const { return (uint32_t)G.h.unhash(G.hid_of[pos]); }
 
-  // Mirrors are filled once, at initialize, with the active states: all
-  // of them for a stationary program (activate_all), none otherwise (init
-  // activates no vertex, vertex_program.hpp:103-104), whose mirrors keep S().
+  // Mirrors are filled once, at the first initialize (`not
+  // v->mirrors_allocated`, vertex_program.hpp:159-173), with the active
+  // states: all of them for a stationary program (activate_all), none
+  // otherwise (init activates no vertex, vertex_program.hpp:103-104), whose
+  // mirrors keep S().
+  bool mirrored = false;
   void mirror() {
     if constexpr (P::gather_state) {
+      if (mirrored) return;
       vm.assign(G.nslots, S());
       if (P::stationary) vm = v;
     }
+    mirrored = true;
+  }
+
+  // reset() (vertex_program.hpp:51-63): messages and accumulators are
+  // dropped; the states stay.
+  void reset() {
+    xreg.clear();
+    xsrc.clear();
+  }
+
+  // initialize(vids) (vertex_program.hpp:177-259): init on the listed
+  // vertices only (flags by position), by class; regular and source
+  // vertices whose init returns true send, in increasing position.
+  void initialize_vids(const std::vector<uint8_t>& only) {
+    xreg.clear(); xsrc.clear();
+    for (uint32_t i = 0; i < G.nreg; i++)
+      if (only[i] && p.init(vid_of(i), v[i])) xreg.emplace_back(i, p.scatter(v[i]));
+    if (G.directed) {
+      for (uint32_t k = 0; k < G.nsink; k++) {
+        uint32_t i = G.nreg + k;
+        if (only[i]) p.init(vid_of(i), v[i]);
+      }
+      for (uint32_t k = 0; k < G.nsrc; k++) {
+        uint32_t i = G.nreg + G.nsink + k;
+        if (only[i] && p.init(vid_of(i), v[i])) xsrc.emplace_back(k, p.scatter(v[i]));
+      }
+    }
+    for (uint32_t i = G.nreg + G.nsink + G.nsrc; i < G.nslots; i++)
+      if (only[i]) p.init(vid_of(i), v[i]);
+    mirror();
   }
 
   // vertex_program.hpp:84-157
@@ -871,6 +920,54 @@ int ora_vp_countgreater(void* h, uint32_t max_iters, uint32_t* val, uint32_t* cn
     cnt[e.vid_of(i)] = e.v[i].cnt;
   }
   if (iters_run) *iters_run = e.iters_run;
+  return 0;
+}
+
+// MaxLabel run twice: initialize(); execute(m1); then either execute(m2)
+// (mode 0: the second run continues from the messages the first left) or
+// reset(); initialize(vids) (all vertices when vids is NULL); execute(m2)
+// (mode 1). vids are original ids.
+int ora_vp_maxlabel_seq(void* h, uint32_t m1, int mode, const uint32_t* vids, uint32_t nvids, uint32_t m2,
+                        uint32_t* label, uint32_t* iters_run) {
+  const OGraph& G = *((Handle*)h)->G;
+  MaxLabelProg p;
+  Engine<MaxLabelProg> e(p, G);
+  e.initialize();
+  e.execute(m1);
+  if (mode == 1) {
+    e.reset();
+    if (vids) {
+      std::vector<uint8_t> only(G.nslots, 0);
+      for (uint32_t i = 0; i < G.nslots; i++) {
+        const uint32_t id = e.vid_of(i);
+        for (uint32_t j = 0; j < nvids; j++)
+          if (vids[j] == id) only[i] = 1;
+      }
+      e.initialize_vids(only);
+    } else {
+      e.initialize();
+    }
+  }
+  e.execute(m2);
+  for (uint32_t i = 0; i < G.nslots; i++) label[e.vid_of(i)] = e.v[i].label;
+  if (iters_run) *iters_run = e.iters_run;
+  return 0;
+}
+
+// AccGreater: initialize(); execute(1); reset(); initialize(); execute(1).
+int ora_vp_accgreater_twice(void* h, uint32_t* val, uint32_t* cnt) {
+  const OGraph& G = *((Handle*)h)->G;
+  AccGreaterProg p;
+  Engine<AccGreaterProg> e(p, G);
+  e.initialize();
+  e.execute(1);
+  e.reset();
+  e.initialize();
+  e.execute(1);
+  for (uint32_t i = 0; i < G.nslots; i++) {
+    val[e.vid_of(i)] = e.v[i].val;
+    cnt[e.vid_of(i)] = e.v[i].cnt;
+  }
   return 0;
 }
 

@@ -55,6 +55,8 @@ def lib():
         L.ora_vp_pralpha.argtypes = [P, P, u32, P, P]
         L.ora_vp_countgreater.argtypes = [P, u32, P, P, P]
         L.ora_vp_countnonzero.argtypes = [P, u32, P, P]
+        L.ora_vp_maxlabel_seq.argtypes = [P, u32, ctypes.c_int, P, u32, u32, P, P]
+        L.ora_vp_accgreater_twice.argtypes = [P, P, P]
         _lib = L
     return _lib
 
@@ -208,3 +210,20 @@ def vp_countnonzero(g, max_iters):
     cnt, it = _u32(g.nslots), _u32(1)
     lib().ora_vp_countnonzero(g.h, int(max_iters), _ptr(cnt), _ptr(it))
     return cnt, int(it[0])
+
+
+def vp_maxlabel_seq(g, m1, m2, reset=False, vids=None):
+    """MaxLabel: initialize; execute(m1); then execute(m2) continuing, or
+    reset; initialize(vids) (all when None); execute(m2)."""
+    label, it = _u32(g.nslots), _u32(1)
+    v = None if vids is None else np.ascontiguousarray(vids, dtype=np.uint32)
+    lib().ora_vp_maxlabel_seq(g.h, int(m1), 1 if reset else 0, None if v is None else _ptr(v),
+                              0 if v is None else int(v.size), int(m2), _ptr(label), _ptr(it))
+    return label, int(it[0])
+
+
+def vp_accgreater_twice(g):
+    """AccGreater: initialize; execute(1); reset; initialize; execute(1)."""
+    val, cnt = _u32(g.nslots), _u32(g.nslots)
+    lib().ora_vp_accgreater_twice(g.h, _ptr(val), _ptr(cnt))
+    return val, cnt

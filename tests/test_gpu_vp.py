@@ -31,6 +31,8 @@ def vp():
         lib.vp_pralpha.argtypes = [P, P, P, u32, P, P]
         lib.vp_countgreater.argtypes = [P, P, u32, P, P, P]
         lib.vp_countnonzero.argtypes = [P, P, u32, P, P]
+        lib.vp_maxlabel_seq.argtypes = [P, P, u32, ctypes.c_int, P, u32, u32, P, P]
+        lib.vp_accgreater_twice.argtypes = [P, P, P, P]
         _vp = lib
     return _vp
 
@@ -124,3 +126,33 @@ def test_interface_programs_match_oracle(gpu_ctx, oracle, g1, case):
         r, it = pralpha(g, gr, m)
         er, eit = oracle.vp_pralpha(o, orr, m)
         assert np.abs(r - er).max() <= 1e-12 and it == eit, (name, m)
+
+
+def maxlabel_seq(g, m1, m2, reset=False, vids=None):
+    lab, it = np.zeros(g.nslots, np.uint32), np.zeros(1, np.uint32)
+    v = None if vids is None else np.ascontiguousarray(vids, dtype=np.uint32)
+    _call("vp_maxlabel_seq", g.ctx.h, g.h, m1, 1 if reset else 0, None if v is None else _p(v),
+          0 if v is None else v.size, m2, _p(lab), _p(it))
+    return lab, int(it[0])
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_reset_reinitialize_and_continue_match_oracle(gpu_ctx, oracle, g1, case):
+    """Second execute without reset (continues from the messages left),
+    reset + initialize(vids) (the other vertices keep their states), reset +
+    initialize, and mirrors kept from the first initialize (AccGreater)."""
+    name, recw, n, _ = _cases(g1)[case]
+    rec = np.ascontiguousarray(recw[:, :2])
+    g, o = _graphs(gpu_ctx, oracle, rec, n, False, L.UNDIRECTED)
+    ids = np.arange(n + 1)
+    vids = ids[(ids % 3 == 0) | (ids % 7 == 1)]
+    for m1, m2, reset, vv in ((2, 1, False, None), (1, 2, False, None), (3, 2, False, None),
+                              (0, 0, True, vids), (0, 0, True, None), (2, 0, True, vids)):
+        got = maxlabel_seq(g, m1, m2, reset, vv)
+        exp = oracle.vp_maxlabel_seq(o, m1, m2, reset, vv)
+        assert (got[0] == exp[0]).all() and got[1] == exp[1], (name, m1, m2, reset)
+    gd, od = _graphs(gpu_ctx, oracle, rec, n, False, L.DIRECTED)
+    val, cnt = np.zeros(gd.nslots, np.uint32), np.zeros(gd.nslots, np.uint32)
+    _call("vp_accgreater_twice", gd.ctx.h, gd.h, _p(val), _p(cnt))
+    ev, ec = oracle.vp_accgreater_twice(od)
+    assert (val == ev).all() and (cnt == ec).all(), name

@@ -143,3 +143,76 @@ def test_countnonzero_reads_default_mirrors(oracle, mode):
     np.add.at(y1, b[e1], (init[a[e1]] > 0).astype(np.int64))
     cnt, it = oracle.vp_countnonzero(g, 2)
     assert it == 2 and (cnt == y0 + y1).all()
+
+
+def test_maxlabel_second_execute_continues(oracle):
+    """execute(m1) then execute(m2) without reset continues from the
+    messages the first run left (the reference initializes once,
+    vertex_program_execute.hpp:15-16): on an undirected load, the largest id
+    within m1 + m2 hops."""
+    rec = random_records(98, N, 700)
+    g = oracle.OracleGraph(rec, N, False, oracle.UNDIRECTED)
+    A = CF.adjacency(rec, N, undirected=True)
+    d = csgraph.shortest_path(A, directed=False, unweighted=True)
+    for m1, m2 in ((2, 1), (2, 2), (3, 1)):
+        lab, it = oracle.vp_maxlabel_seq(g, m1, m2)
+        exp = np.where(d <= m1 + m2, np.arange(N + 1)[:, None], -1).max(axis=0)
+        assert (lab == exp).all() and it == m2, (m1, m2)
+    # execute(1) (execute_single) applies without scattering, and SpMV reads
+    # messages without consuming them (streaming_array.h:104-108): the next
+    # run re-sends the initial labels, which change nothing
+    for m2 in (1, 2):
+        lab, it = oracle.vp_maxlabel_seq(g, 1, m2)
+        exp = np.where(d <= 1, np.arange(N + 1)[:, None], -1).max(axis=0)
+        assert (lab == exp).all() and it == m2, m2
+
+
+def test_maxlabel_reset_then_initialize_vids(oracle):
+    """reset() keeps the states and initialize(vids) runs init on the listed
+    vertices only (vertex_program.hpp:51-63, 177-259): after convergence
+    (component maxima), re-initialized vertices hold their own ids, nobody
+    else sends, so a listed vertex ends with the largest id of its component
+    among the listed vertices it reaches through listed vertices, and every
+    other vertex keeps its component maximum."""
+    rec = random_records(99, N, 600)
+    g = oracle.OracleGraph(rec, N, False, oracle.UNDIRECTED)
+    A = CF.adjacency(rec, N, undirected=True)
+    ids = np.arange(N + 1)
+    comp = _max_ancestor(rec, N, True).astype(np.int64)
+    vids = ids[(ids % 3 == 0) | (ids % 7 == 1)]
+    listed = np.zeros(N + 1, bool)
+    listed[vids] = True
+    sub = A[listed][:, listed]
+    ncomp, lab_sub = csgraph.connected_components(sub, directed=False)
+    best = np.zeros(ncomp, dtype=np.int64)
+    np.maximum.at(best, lab_sub, vids)
+    exp = comp.copy()
+    exp[vids] = best[lab_sub]
+    lab, _ = oracle.vp_maxlabel_seq(g, 0, 0, reset=True, vids=vids)
+    assert (lab == exp).all()
+    # initialize() of every vertex after reset: back to the fixed point
+    lab, _ = oracle.vp_maxlabel_seq(g, 0, 0, reset=True)
+    assert (lab == comp).all()
+
+
+def test_accgreater_mirrors_keep_the_first_initialize(oracle):
+    """A stationary state-reading program whose init adds to the state it
+    finds, run as initialize; execute(1); reset; initialize; execute(1): the
+    second run compares with the mirrors of the first initialize
+    (vertex_program.hpp:159-173 mirror once), not with the new states."""
+    rec = random_records(100, N, 1800)
+    g = oracle.OracleGraph(rec, N, False, oracle.DIRECTED)
+    dd = CF.dedup_first(rec)
+    a, b = dd[:, 0].astype(np.int64), dd[:, 1].astype(np.int64)
+    ids = np.arange(N + 1)
+    init = ids % 5 + 1
+    has_in = np.bincount(b, minlength=N + 1) > 0
+    y1 = np.zeros(N + 1, dtype=np.int64)
+    np.add.at(y1, b, (init[a] > init[b]).astype(np.int64))
+    val1 = np.where(has_in, y1 % 7, init)
+    val2 = val1 + init
+    y2 = np.zeros(N + 1, dtype=np.int64)
+    np.add.at(y2, b, (val2[a] > init[b]).astype(np.int64))
+    val, cnt = oracle.vp_accgreater_twice(g)
+    assert (cnt == np.where(has_in, y1 + y2, 0)).all()
+    assert (val == np.where(has_in, y2 % 7, val2)).all()

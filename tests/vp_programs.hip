@@ -169,6 +169,30 @@ struct CountNonzero {
   }
 };
 
+// Stationary, state-reading gather, and an init that adds to the state it
+// finds: re-initialized after reset(), its mirrors keep the first values.
+struct AccGreater {
+  using W = la3::Empty;
+  using M = uint32_t;
+  using A = uint32_t;
+  struct S {
+    uint32_t val = 0;
+    uint32_t cnt = 0;
+  };
+  __device__ bool init(uint32_t vid, S& s) const {
+    s.val += vid % 5 + 1;
+    return true;
+  }
+  __device__ M scatter(const S& s) const { return s.val; }
+  __device__ A gather(const la3::Edge<W>&, const M& m, const S& s) const { return m > s.val ? 1u : 0u; }
+  __device__ void combine(const A& y1, A& y2) const { y2 += y1; }
+  __device__ bool apply(const A& y, S& s) const {
+    s.cnt += y;
+    s.val = y % 7;
+    return false;
+  }
+};
+
 template <class F>
 int guarded(F f) {
   try {
@@ -265,6 +289,42 @@ int vp_countnonzero(la3_ctx* c, la3_graph* g, uint32_t max_iters, uint32_t* cnt,
     auto st = vp.states();
     for (size_t i = 0; i < st.size(); i++) cnt[i] = st[i].cnt;
     *iters = vp.iterations();
+  });
+}
+
+// initialize(); execute(m1); then execute(m2) (mode 0), or reset();
+// initialize(vids) (all when vids is NULL); execute(m2) (mode 1).
+int vp_maxlabel_seq(la3_ctx* c, la3_graph* g, uint32_t m1, int mode, const uint32_t* vids, uint32_t nvids,
+                    uint32_t m2, uint32_t* label, uint32_t* iters) {
+  return guarded([&] {
+    la3::VertexProgram<MaxLabel> vp(c, g);
+    vp.initialize();
+    vp.execute(m1);
+    if (mode == 1) {
+      vp.reset();
+      if (vids) vp.initialize(std::vector<uint32_t>(vids, vids + nvids));
+      else vp.initialize();
+    }
+    vp.execute(m2);
+    auto st = vp.states();
+    for (size_t i = 0; i < st.size(); i++) label[i] = st[i].label;
+    *iters = vp.iterations();
+  });
+}
+
+int vp_accgreater_twice(la3_ctx* c, la3_graph* g, uint32_t* val, uint32_t* cnt) {
+  return guarded([&] {
+    la3::VertexProgram<AccGreater> vp(c, g, true);
+    vp.initialize();
+    vp.execute(1);
+    vp.reset();
+    vp.initialize();
+    vp.execute(1);
+    auto st = vp.states();
+    for (size_t i = 0; i < st.size(); i++) {
+      val[i] = st[i].val;
+      cnt[i] = st[i].cnt;
+    }
   });
 }
 
