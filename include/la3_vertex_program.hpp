@@ -63,6 +63,8 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <iostream>
+#include <memory>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -232,6 +234,13 @@ struct Bufs {
   // counted for a pass (two slots, used alternately); [6] active sources
   uint32_t* ctr = nullptr;
   uint32_t* stamp = nullptr;    // by row: the last sparse pass that reached it
+  // Vertex activity by slot (the state vector's activity set,
+  // random_access_array.h:14): set by apply for a regular vertex that was
+  // activated or is stationary and a sink that was activated
+  // (vertex_program_execute.hpp:693-713), kept across executes, cleared by
+  // reset_activity() and after the first mirroring; read by active_only
+  // reduce / topk and by the mirroring of a non-stationary program.
+  uint8_t* vact = nullptr;
 };
 
 // One pass over rows [r0, r1): read the messages of buffer `cur` (source
@@ -277,10 +286,15 @@ __device__ __forceinline__ void vp_finish_row(const P& p, const la3_device_view&
   b.v[r] = st;
   if (r < g.nreg) {
     if (act) b.ctr[0] = 1;
-    if (!a.single && (act || a.stationary)) {
-      b.x[a.cur ^ 1][r] = op_scatter(p, st);
-      b.act[a.cur ^ 1][r] = 1;
+    if (act || a.stationary) {
+      b.vact[r] = 1;
+      if (!a.single) {
+        b.x[a.cur ^ 1][r] = op_scatter(p, st);
+        b.act[a.cur ^ 1][r] = 1;
+      }
     }
+  } else if (act) {
+    b.vact[r] = 1;
   }
 }
 
@@ -382,7 +396,7 @@ __device__ __forceinline__ void block_scan2(uint32_t& c, uint32_t& s, uint32_t& 
   s = ss[t] - s;
 }
 
-__global__ void __launch_bounds__(VP_TAB) k_vp_long_sums(la3_device_view g, uint32_t* bsum) {
+static __global__ void __launch_bounds__(VP_TAB) k_vp_long_sums(la3_device_view g, uint32_t* bsum) {
   const uint32_t i = blockIdx.x * VP_TAB + threadIdx.x;
   uint32_t c = 0, s = 0, tc, ts;
   if (i < g.nlong) long_counts(g, i, c, s);
@@ -394,7 +408,7 @@ __global__ void __launch_bounds__(VP_TAB) k_vp_long_sums(la3_device_view g, uint
 }
 
 // One block: exclusive scan of the nb block sums (in place), totals.
-__global__ void __launch_bounds__(VP_TAB) k_vp_long_scan(uint32_t* bsum, uint32_t nb, uint32_t* totals) {
+static __global__ void __launch_bounds__(VP_TAB) k_vp_long_scan(uint32_t* bsum, uint32_t nb, uint32_t* totals) {
   uint32_t carry_c = 0, carry_s = 0;
   for (uint32_t base = 0; base < nb; base += VP_TAB) {  // block-uniform
     const uint32_t i = base + threadIdx.x;
@@ -414,7 +428,7 @@ __global__ void __launch_bounds__(VP_TAB) k_vp_long_scan(uint32_t* bsum, uint32_
   }
 }
 
-__global__ void __launch_bounds__(VP_TAB) k_vp_long_table(la3_device_view g, const uint32_t* bsum, LongRow* rows,
+static __global__ void __launch_bounds__(VP_TAB) k_vp_long_table(la3_device_view g, const uint32_t* bsum, LongRow* rows,
                                                          uint32_t* sbase) {
   const uint32_t i = blockIdx.x * VP_TAB + threadIdx.x;
   uint32_t c = 0, s = 0, tc, ts;
@@ -427,7 +441,7 @@ __global__ void __launch_bounds__(VP_TAB) k_vp_long_table(la3_device_view g, con
   }
 }
 
-__global__ void k_vp_long_segs(la3_device_view g, const LongRow* rows, const uint32_t* sbase, uint32_t* seg) {
+static __global__ void k_vp_long_segs(la3_device_view g, const LongRow* rows, const uint32_t* sbase, uint32_t* seg) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.nlong; i += gridDim.x * blockDim.x) {
     const uint32_t ns = (rows[i].e1 - rows[i].e0 + VP_SEG - 1) / VP_SEG;
     for (uint32_t k = 0; k < ns; k++) seg[sbase[i] + k] = i;
@@ -716,7 +730,9 @@ __global__ void k_vp_fill_states(S* v, uint32_t n) {
 // vertices whose init returns true scatter into message buffer 0; no other
 // vertex has a message pending. Flags and messages of buffer 1 start clear.
 // With `mirror` (the first initialize of the program's states), the mirrors
-// take the active states (vertex_program.hpp:159-173).
+// take the active states — every state when stationary (activate_all), else
+// those whose activity is set — and the activity is then cleared
+// (vertex_program.hpp:159-173).
 template <class P>
 __global__ void k_vp_init(la3_device_view g, P p, Bufs<P> b, uint32_t stationary, const uint8_t* only,
                           uint32_t mirror) {
@@ -728,7 +744,10 @@ __global__ void k_vp_init(la3_device_view g, P p, Bufs<P> b, uint32_t stationary
     bool a = false;
     if (!only || only[s]) a = op_init(p, stationary != 0, g.id_of[s], st);
     b.v[s] = st;
-    if (b.vm && mirror) b.vm[s] = stationary ? st : S();
+    if (b.vm && mirror) {
+      b.vm[s] = stationary || b.vact[s] ? st : S();
+      b.vact[s] = 0;
+    }
     const bool sends = s < g.nreg || (g.directed && s >= src0 && s < src1);
     b.act[0][s] = sends && a;
     b.act[1][s] = 0;
@@ -760,7 +779,10 @@ __global__ void k_vp_init_other(la3_device_view g, P p, Bufs<P> b, uint32_t stat
     const uint32_t vid = g.id_of[s];
     const bool a = op_init_other(p, stationary != 0, vid, other[other_slot_of_id[vid]], st);
     b.v[s] = st;
-    if (b.vm && mirror) b.vm[s] = stationary ? st : S();  // vertex_program.hpp:490-505
+    if (b.vm && mirror) {  // vertex_program.hpp:490-505
+      b.vm[s] = stationary || b.vact[s] ? st : S();
+      b.vact[s] = 0;
+    }
     const bool sends = s < g.nreg || (g.directed && s >= src0 && s < src1);
     b.act[0][s] = sends && a;
     b.act[1][s] = 0;
@@ -795,7 +817,7 @@ __global__ void k_vp_clear(la3_device_view g, Bufs<P> b, uint32_t which, uint32_
   block_add(n, b.ctr + 4 + slot);
 }
 
-__global__ void k_vp_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+static __global__ void k_vp_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
     p[i] = v;
 }
@@ -841,6 +863,52 @@ inline void hip_release(void*, void* p) {
   if (p) (void)hipFree(p);
 }
 
+static __global__ void k_vp_fill_u8(uint8_t* p, uint64_t n, uint8_t v) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// The vertex side of a program — states, their activity and the mirrors —
+// which a program constructed from another borrows (the reference's
+// VectorV, shared when owns_vertices is false, vertex_program.hpp:22-30).
+// Allocated by the first engine that needs it, with that engine's
+// allocator; freed when the last engine holding it lets it go.
+template <class S>
+struct VStore {
+  S* v = nullptr;
+  S* vm = nullptr;
+  uint8_t* vact = nullptr;
+  bool mirrored = false;  // v->mirrors_allocated (vertex_program.hpp:159-173)
+  uint32_t n = 0;
+  Allocator al;
+  VStore() = default;
+  VStore(const VStore&) = delete;
+  VStore& operator=(const VStore&) = delete;
+  ~VStore() {
+    if (!al.release) return;
+    al.release(al.self, v);
+    al.release(al.self, vm);
+    al.release(al.self, vact);
+  }
+  // States value-initialized and activity clear (random_access_array.h:26-28).
+  hipError_t ensure(const Allocator& a, uint32_t nslots, hipStream_t st) {
+    if (v) return hipSuccess;
+    al = a;
+    n = nslots ? nslots : 1;
+    v = (S*)al.alloc(al.self, "vp.v", sizeof(S) * (size_t)n);
+    vact = (uint8_t*)al.alloc(al.self, "vp.vact", (size_t)n + 16);
+    if (!v || !vact) return hipErrorOutOfMemory;
+    hipLaunchKernelGGL(k_vp_fill_states<S>, dim3(vp_blocks(n)), dim3(VP_BLOCK), 0, st, v, n);
+    hipLaunchKernelGGL(k_vp_fill_u8, dim3(vp_blocks(n)), dim3(VP_BLOCK), 0, st, vact, (uint64_t)n, (uint8_t)0);
+    return hipGetLastError();
+  }
+  hipError_t ensure_mirrors() {
+    if (vm) return hipSuccess;
+    vm = (S*)al.alloc(al.self, "vp.vm", sizeof(S) * (size_t)n);
+    return vm ? hipSuccess : hipErrorOutOfMemory;
+  }
+};
+
 // The execution engine (vertex_program_execute.hpp:11-345) over one graph's
 // device layout on one stream. Returns a hipError_t; `err` says what failed.
 template <class P>
@@ -858,12 +926,16 @@ class Engine {
   static constexpr bool gather_with_state = has_gather3<P>::value;
   static constexpr bool apply_with_iter = has_apply3<P>::value;
 
-  Engine(const la3_device_view& g, hipStream_t stream, bool stationary, const P& ops, Allocator al = Allocator())
-      : g_(g), stream_(stream), stationary_(stationary), p_(ops), al_(al) {
+  // store: the vertex side of another program on the same graph to borrow
+  // (null: this program owns its own).
+  Engine(const la3_device_view& g, hipStream_t stream, bool stationary, const P& ops, Allocator al = Allocator(),
+         std::shared_ptr<VStore<S>> store = nullptr)
+      : g_(g), stream_(stream), stationary_(stationary), p_(ops), al_(al), vs_(std::move(store)) {
     if (!al_.alloc) {
       al_.alloc = hip_alloc;
       al_.release = hip_release;
     }
+    if (!vs_) vs_ = std::make_shared<VStore<S>>();
   }
   ~Engine() { release(); }
   Engine(const Engine&) = delete;
@@ -874,15 +946,20 @@ class Engine {
   uint32_t iters_run = 0;
   const Bufs<P>& bufs() const { return b_; }
   const la3_device_view& view() const { return g_; }
+  const std::shared_ptr<VStore<S>>& store() const { return vs_; }
+  // The states, allocated (and value-initialized) on first use.
+  const S* states_ptr() {
+    if (!vs_) vs_ = std::make_shared<VStore<S>>();
+    return vs_->ensure(al_, g_.nslots, stream_) == hipSuccess ? vs_->v : nullptr;
+  }
 
   hipError_t initialize(const uint8_t* only_slots = nullptr) {
     hipError_t e = ensure();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_vp_reset_ctr<P>, dim3(1), dim3(64), 0, stream_, b_);
     hipLaunchKernelGGL(k_vp_init<P>, dim3(vp_blocks(g_.nslots)), dim3(VP_BLOCK), 0, stream_, g_, p_, b_,
-                       (uint32_t)stationary_, only_slots, (uint32_t)!mirrored_);
+                       (uint32_t)stationary_, only_slots, (uint32_t)mirror_now());
     initialized_ = true;
-    mirrored_ = true;
     cur_ = 0;
     return hipGetLastError();
   }
@@ -893,11 +970,31 @@ class Engine {
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_vp_reset_ctr<P>, dim3(1), dim3(64), 0, stream_, b_);
     hipLaunchKernelGGL((k_vp_init_other<P, OS>), dim3(vp_blocks(g_.nslots)), dim3(VP_BLOCK), 0, stream_, g_, p_,
-                       b_, (uint32_t)stationary_, other_states, other_slot_of_id, (uint32_t)!mirrored_);
+                       b_, (uint32_t)stationary_, other_states, other_slot_of_id, (uint32_t)mirror_now());
     initialized_ = true;
-    mirrored_ = true;
     cur_ = 0;
     return hipGetLastError();
+  }
+
+  // reset_activity() / activate_all() (vertex_program.hpp:549-555): clear or
+  // set the activity of every vertex state.
+  hipError_t set_activity(bool on) {
+    hipError_t e = ensure();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_vp_fill_u8, dim3(vp_blocks(g_.nslots)), dim3(VP_BLOCK), 0, stream_, b_.vact,
+                       (uint64_t)g_.nslots, (uint8_t)(on ? 1 : 0));
+    return sync_error();
+  }
+
+  // States and activity in slot order (the reference's state positions).
+  hipError_t by_slot(S* states, uint8_t* activity) {
+    hipError_t e = ensure();
+    if (e != hipSuccess) return e;
+    if (states) e = hipMemcpyAsync(states, b_.v, sizeof(S) * (size_t)g_.nslots, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess && activity)
+      e = hipMemcpyAsync(activity, b_.vact, (size_t)g_.nslots, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    return e == hipSuccess ? hipSuccess : fail(e);
   }
 
   // execute(max_iters) (vertex_program_execute.hpp:11-52).
@@ -943,6 +1040,8 @@ class Engine {
   }
 
   hipError_t states_by_id(S* host_out) {
+    hipError_t e0 = ensure();
+    if (e0 != hipSuccess) return e0;
     S* tmp = (S*)al_.alloc(al_.self, "vp.out", sizeof(S) * (size_t)g_.nslots);
     if (!tmp) return fail(hipErrorOutOfMemory);
     hipLaunchKernelGGL(k_vp_by_id<S>, dim3(vp_blocks(g_.nslots)), dim3(VP_BLOCK), 0, stream_, g_,
@@ -957,11 +1056,16 @@ class Engine {
   // (pending messages dropped); the states stay.
   void reset() { initialized_ = false; }
 
+  // free() (vertex_program.hpp:38-48): this program's buffers, and the
+  // states unless another program still holds them.
   void release() {
-    if (!allocated_) return;
+    if (!allocated_) {
+      if (vs_ && vs_->v) (void)hipStreamSynchronize(stream_);
+      vs_.reset();
+      return;
+    }
     (void)hipStreamSynchronize(stream_);
-    al_.release(al_.self, b_.v);
-    al_.release(al_.self, b_.vm);
+    vs_.reset();  // the states go with their last holder
     al_.release(al_.self, b_.x[0]);
     al_.release(al_.self, b_.x[1]);
     al_.release(al_.self, b_.act[0]);
@@ -982,7 +1086,6 @@ class Engine {
     b_ = Bufs<P>();
     allocated_ = false;
     initialized_ = false;
-    mirrored_ = false;
   }
 
  private:
@@ -992,13 +1095,22 @@ class Engine {
   P p_;
   Allocator al_;
   Bufs<P> b_;
+  std::shared_ptr<VStore<S>> vs_;
   LongTabs<A> lt_;
   uint32_t* ltot_ = nullptr;
   uint32_t nseg_ = 0;
   uint32_t tag_ = 0;                   // the last pass's stamp
   uint32_t ccount_ = 0, cslot_ = 0;    // count slots used alternately
-  bool allocated_ = false, initialized_ = false, mirrored_ = false;
+  bool allocated_ = false, initialized_ = false;
   uint32_t cur_ = 0;
+
+  // The first initialize of the (possibly shared) states of a state-reading
+  // program mirrors them; later ones do not.
+  bool mirror_now() {
+    if (!gather_with_state || vs_->mirrored) return false;
+    vs_->mirrored = true;
+    return true;
+  }
 
   hipError_t fail(hipError_t e) {
     if (err.empty()) err = hipGetErrorString(e);
@@ -1007,9 +1119,17 @@ class Engine {
 
   hipError_t ensure() {
     if (allocated_) return hipSuccess;
+    if (!vs_) vs_ = std::make_shared<VStore<S>>();  // after free(): fresh states
     const size_t S_ = g_.nslots ? g_.nslots : 1;
-    b_.v = (S*)al_.alloc(al_.self, "vp.v", sizeof(S) * S_);
-    if (gather_with_state) b_.vm = (S*)al_.alloc(al_.self, "vp.vm", sizeof(S) * S_);
+    hipError_t es = vs_->ensure(al_, g_.nslots, stream_);
+    if (es == hipSuccess && gather_with_state) es = vs_->ensure_mirrors();
+    if (es != hipSuccess) {
+      err = "device allocation failed";
+      return es;
+    }
+    b_.v = vs_->v;
+    b_.vm = vs_->vm;
+    b_.vact = vs_->vact;
     b_.x[0] = (M*)al_.alloc(al_.self, "vp.x0", sizeof(M) * S_ + 16);
     b_.x[1] = (M*)al_.alloc(al_.self, "vp.x1", sizeof(M) * S_ + 16);
     b_.act[0] = (uint8_t*)al_.alloc(al_.self, "vp.a0", S_ + 16);
@@ -1017,15 +1137,13 @@ class Engine {
     b_.ctr = (uint32_t*)al_.alloc(al_.self, "vp.ctr", 64);
     b_.stamp = (uint32_t*)al_.alloc(al_.self, "vp.stamp", 4 * S_);
     allocated_ = true;
-    if (!b_.v || (gather_with_state && !b_.vm) || !b_.x[0] || !b_.x[1] || !b_.act[0] || !b_.act[1] || !b_.ctr ||
-        !b_.stamp) {
+    if (!b_.x[0] || !b_.x[1] || !b_.act[0] || !b_.act[1] || !b_.ctr || !b_.stamp) {
       release();
       err = "device allocation failed";
       return hipErrorOutOfMemory;
     }
     tag_ = 0;
     hipLaunchKernelGGL(k_vp_fill_u32, dim3(vp_blocks(S_)), dim3(VP_BLOCK), 0, stream_, b_.stamp, (uint64_t)S_, 0u);
-    hipLaunchKernelGGL(k_vp_fill_states<S>, dim3(vp_blocks(S_)), dim3(VP_BLOCK), 0, stream_, b_.v, (uint32_t)S_);
     const hipError_t e = ensure_long();
     if (e != hipSuccess) {
       release();
@@ -1135,6 +1253,20 @@ class VertexProgram {
       : ctx_(ctx), g_(g), view_(view_of(ctx, g)),
         e_(view_, (hipStream_t)la3_ctx_stream(ctx), stationary, ops) {}
 
+  // VertexProgram(const VertexProgram<W, M2, A2, S>& other, bool stationary)
+  // (vertex_program.h:41, vertex_program.hpp:22-30): the same graph, and the
+  // other program's vertex states, their activity and mirrors, shared (the
+  // reference's borrowed VectorV); messages and accumulators of its own.
+  // The states stay allocated while either program holds them.
+  template <class P2>
+  VertexProgram(const VertexProgram<P2>& other, bool stationary = false, const P& ops = P())
+      : ctx_(other.context_handle()), g_(other.graph_handle()), view_(other.view()),
+        e_(view_, (hipStream_t)la3_ctx_stream(ctx_), stationary, ops, vp_detail::Allocator(),
+           other.shared_states()) {
+    static_assert(std::is_same<typename P2::S, S>::value, "a borrowing program has the other's state type");
+    static_assert(std::is_same<typename P2::W, W>::value, "a borrowing program has the other's weight type");
+  }
+
   // LA3-optimizable execution (vertex_program.h:140-148); cleared to force
   // execute_non_opt. Every initialize ANDs in the graph's and the program's
   // conditions, as initialize_flags does (vertex_program.hpp:66-81).
@@ -1192,31 +1324,146 @@ class VertexProgram {
 
   uint32_t iterations() const { return e_.iters_run; }
 
+  // reset_activity() / activate_all() (vertex_program.hpp:549-555): clear or
+  // set the activity of every vertex state. Apply sets it for a regular
+  // vertex that was activated or is stationary and for a sink that was
+  // activated (vertex_program_execute.hpp:693-713); it is kept across
+  // executes and resets, and cleared after the first mirroring of a
+  // state-reading program, which mirrors the active states.
+  void reset_activity() { check(e_.set_activity(false), "reset_activity"); }
+  void activate_all() { check(e_.set_activity(true), "activate_all"); }
+
   // States by original vertex id (nslots = n + 1 entries).
   std::vector<S> states() {
     std::vector<S> out(view_.nslots);
     check(e_.states_by_id(out.data()), "states");
     return out;
   }
-  const S* device_states() const { return e_.bufs().v; }
+  // Vertex activity by original vertex id (1 = active).
+  std::vector<uint8_t> activity() {
+    std::vector<uint8_t> a(view_.nslots), out(view_.nslots);
+    check(e_.by_slot(nullptr, a.data()), "activity");
+    const std::vector<uint32_t>& ids = ids_by_slot();
+    for (size_t p = 0; p < a.size(); p++) out[ids[p]] = a[p];
+    return out;
+  }
+  const S* device_states() const {
+    const S* v = const_cast<vp_detail::Engine<P>&>(e_).states_ptr();
+    if (!v) throw Error(LA3_ENOMEM, "device allocation failed");
+    return v;
+  }
   const la3_device_view& view() const { return view_; }
+  la3_ctx* context_handle() const { return ctx_; }
+  la3_graph* graph_handle() const { return g_; }
+  std::shared_ptr<vp_detail::VStore<S>> shared_states() const {
+    device_states();  // allocated, so that both programs hold the same states
+    return e_.store();
+  }
 
-  // reduce (vertex_program.hpp:558-604): map every vertex in the reference's
-  // slot order and fold with reduce.
+  // reduce (vertex_program.hpp:558-604) at one rank: map every vertex (or
+  // every active one) in the reference's slot order, with its original id,
+  // and fold with reduce into Value().
   template <class Value, class Mapper, class Reducer>
-  Value reduce(Mapper map, Reducer red) {
-    std::vector<S> st = states();
-    std::vector<uint32_t> hid(view_.nslots);
-    if (la3_graph_slots(g_, hid.data()) != LA3_OK) throw Error(LA3_EINVAL, "reduce: slots");
+  Value reduce(Mapper map, Reducer red, bool active_only = false) {
+    std::vector<S> st;
+    std::vector<uint8_t> act;
+    slot_states(st, act, active_only);
+    const std::vector<uint32_t>& ids = ids_by_slot();
     Value acc = Value();
-    for (uint32_t h : hid) {
-      const uint32_t vid = la3_graph_unhash(g_, h);
-      red(acc, map(vid, st[vid]));
-    }
+    for (size_t p = 0; p < st.size(); p++)
+      if (!active_only || act[p]) red(acc, map(ids[p], st[p]));
     return acc;
   }
 
+  // topk (vertex_program.hpp:607-680) at one rank, step for step:
+  // (id, map(id, state)) pairs in slot order (active vertices only if asked),
+  // std::partial_sort of the first min(k, count) by cmp, resized to k (pairs
+  // I(), V() appended when fewer), partial_sort of those k, resized to k.
+  template <class I, class V, class Mapper, class Comparator>
+  void topk(uint32_t k, std::vector<std::pair<I, V>>& out, Mapper map, Comparator cmp, bool active_only = false) {
+    std::vector<S> st;
+    std::vector<uint8_t> act;
+    slot_states(st, act, active_only);
+    const std::vector<uint32_t>& ids = ids_by_slot();
+    std::vector<std::pair<I, V>> ivs;
+    for (size_t p = 0; p < st.size(); p++)
+      if (!active_only || act[p]) ivs.push_back(std::pair<I, V>((I)ids[p], map(ids[p], st[p])));
+    out.clear();
+    select_k(ivs, k, cmp);
+    out = ivs;
+  }
+
+  // btopk<BATCH_SIZE> (vertex_program.hpp:683-767) at one rank: map returns
+  // a pointer to BATCH_SIZE values of a vertex; topk's steps per batch b
+  // into topk[b].
+  template <uint32_t BATCH_SIZE, class I, class V, class Mapper, class Comparator>
+  void btopk(uint32_t k, std::vector<std::pair<I, V>>* out, Mapper map, Comparator cmp, bool active_only = false) {
+    std::vector<S> st;
+    std::vector<uint8_t> act;
+    slot_states(st, act, active_only);
+    const std::vector<uint32_t>& ids = ids_by_slot();
+    std::vector<std::pair<I, V>> ivs[BATCH_SIZE];
+    for (uint32_t b = 0; b < BATCH_SIZE; b++) out[b].clear();
+    for (size_t p = 0; p < st.size(); p++) {
+      if (active_only && !act[p]) continue;
+      const V* vals = map(ids[p], st[p]);
+      for (uint32_t b = 0; b < BATCH_SIZE; b++) ivs[b].push_back(std::pair<I, V>((I)ids[p], vals[b]));
+    }
+    for (uint32_t b = 0; b < BATCH_SIZE; b++) {
+      select_k(ivs[b], k, cmp);
+      out[b] = ivs[b];
+    }
+  }
+
+  // display<values_only>(nvertices) (vertex_program.hpp:511-546): in slot
+  // order, every vertex whose original id j is at most min(nvertices, n) as
+  // "j: " + state.to_string() (values_only: the string alone, and one empty
+  // line after the list), one line each, to `os` (the reference logs them).
+  // S must have `std::string to_string() const`, as the reference's states do.
+  template <bool values_only = false>
+  void display(uint32_t nvertices = 30, std::ostream& os = std::cout) {
+    static_assert(has_to_string<S>::value, "display() needs S::to_string() const (as the reference's states)");
+    std::vector<S> st;
+    std::vector<uint8_t> act;
+    slot_states(st, act, false);
+    const std::vector<uint32_t>& ids = ids_by_slot();
+    const uint32_t nmax = std::min<uint32_t>(nvertices, view_.nslots ? view_.nslots - 1 : 0);
+    for (size_t p = 0; p < st.size(); p++) {
+      if (ids[p] > nmax) continue;
+      if (values_only) os << st[p].to_string() << "\n";
+      else os << ids[p] << ": " << st[p].to_string() << "\n";
+    }
+    if (values_only) os << "\n";
+  }
+
  private:
+  template <class T, class = void> struct has_to_string : std::false_type {};
+  template <class T>
+  struct has_to_string<T, std::void_t<decltype(std::declval<const T&>().to_string())>> : std::true_type {};
+
+  std::vector<uint32_t> ids_;  // original id by slot (the layout does not change)
+  const std::vector<uint32_t>& ids_by_slot() {
+    if (ids_.size() != view_.nslots) {
+      std::vector<uint32_t> hid(view_.nslots);
+      if (la3_graph_slots(g_, hid.data()) != LA3_OK) throw Error(LA3_EINVAL, "slot order");
+      ids_.resize(view_.nslots);
+      for (uint32_t p = 0; p < view_.nslots; p++) ids_[p] = la3_graph_unhash(g_, hid[p]);
+    }
+    return ids_;
+  }
+  void slot_states(std::vector<S>& st, std::vector<uint8_t>& act, bool with_activity) {
+    st.resize(view_.nslots);
+    if (with_activity) act.resize(view_.nslots);
+    check(e_.by_slot(st.data(), with_activity ? act.data() : nullptr), "states");
+  }
+  template <class IV, class Comparator>
+  static void select_k(std::vector<IV>& ivs, uint32_t k, Comparator cmp) {
+    std::partial_sort(ivs.begin(), ivs.begin() + std::min<size_t>(k, ivs.size()), ivs.end(), cmp);
+    ivs.resize(k);
+    std::partial_sort(ivs.begin(), ivs.begin() + std::min<size_t>(k, ivs.size()), ivs.end(), cmp);
+    ivs.resize(k);
+  }
+
   la3_ctx* ctx_;
   la3_graph* g_;
   la3_device_view view_;

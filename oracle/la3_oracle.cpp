@@ -498,9 +498,14 @@ struct Engine {
   std::vector<std::pair<uint32_t, M>> xreg, xsrc;    // StreamingArray (pushes ascending)
   std::vector<A> yreg, ysink;
   std::vector<uint8_t> treg, tsink;                  // activity bitvectors
+  // The state vector's activity set (random_access_array.h:14), by
+  // position: pushed by apply (vertex_program_execute.hpp:693-713), never
+  // cleared by execute or reset, cleared by reset_activity() and after the
+  // first mirroring (vertex_program.hpp:166-172).
+  std::vector<uint8_t> vact;
   uint32_t iters_run = 0;
 
-  Engine(P& p_, const OGraph& G_) : p(p_), G(G_), v(G_.nslots) {
+  Engine(P& p_, const OGraph& G_) : p(p_), G(G_), v(G_.nslots), vact(G_.nslots, 0) {
     yreg.assign(G.nreg + 1, p.identity());
     ysink.assign(G.nsink + 1, p.identity());
     treg.assign(G.nreg + 1, 0);
@@ -509,17 +514,25 @@ struct Engine {
 
   uint32_t vid_of(uint32_t pos) const { return (uint32_t)G.h.unhash(G.hid_of[pos]); }
 
+  // reset_activity() / activate_all() (vertex_program.hpp:549-555).
+  void reset_activity() { vact.assign(G.nslots, 0); }
+  void activate_all() { vact.assign(G.nslots, 1); }
+
   // Mirrors are filled once, at the first initialize (`not
-  // v->mirrors_allocated`, vertex_program.hpp:159-173), with the active
-  // states: all of them for a stationary program (activate_all), none
-  // otherwise (init activates no vertex, vertex_program.hpp:103-104), whose
-  // mirrors keep S().
+  // v->mirrors_allocated`, vertex_program.hpp:159-173): a stationary program
+  // first activates every vertex; the active states are broadcast (init
+  // itself activates none, vertex_program.hpp:103-104, so a non-stationary
+  // program mirrors only what an earlier execute or activate_all left
+  // active; the other mirrors keep S()); then the activity is cleared.
   bool mirrored = false;
   void mirror() {
     if constexpr (P::gather_state) {
       if (mirrored) return;
+      if (P::stationary) activate_all();
       vm.assign(G.nslots, S());
-      if (P::stationary) vm = v;
+      for (uint32_t i = 0; i < G.nslots; i++)
+        if (vact[i]) vm[i] = v[i];
+      reset_activity();
     }
     mirrored = true;
   }
@@ -632,7 +645,7 @@ struct Engine {
         if (!tsink[idx]) continue;
         A part = ysink[idx]; ysink[idx] = p.identity(); tsink[idx] = 0;
         A fin = p.identity(); p.combine(part, fin);
-        do_apply(fin, v[G.nreg + idx], iter);
+        if (do_apply(fin, v[G.nreg + idx], iter)) vact[G.nreg + idx] = 1;
       }
       return false;
     }
@@ -643,7 +656,10 @@ struct Engine {
       A fin = p.identity(); p.combine(part, fin);
       bool act = do_apply(fin, v[idx], iter);
       any |= act;
-      if ((act || P::stationary) && !single_iter) out.emplace_back(idx, p.scatter(v[idx]));
+      if (act || P::stationary) {
+        vact[idx] = 1;
+        if (!single_iter) out.emplace_back(idx, p.scatter(v[idx]));
+      }
     }
     if (!single_iter) xreg.swap(out);
     return any;
@@ -710,6 +726,22 @@ void run_pr_like(P& p, const OGraph& G, const OGraph& GR, uint32_t iters, double
     if (deg) deg[id] = e.v[i].degree;
   }
   if (iters_run) *iters_run = e.iters_run;
+}
+
+// One run of an interface program for ora_vp_activity.
+template <class P, class F>
+static void act_run(P& p, const OGraph& G, uint32_t max_iters, uint32_t flags, F field, uint32_t* out,
+                    uint8_t* act) {
+  Engine<P> e(p, G);
+  if (flags & 1) e.activate_all();
+  e.initialize();
+  e.execute(max_iters);
+  if (flags & 2) e.reset_activity();
+  if (flags & 4) e.activate_all();
+  for (uint32_t i = 0; i < G.nslots; i++) {
+    out[e.vid_of(i)] = field(e.v[i]);
+    act[e.vid_of(i)] = e.vact[i];
+  }
 }
 
 // ----------------------------------------------------------------- C API ---
@@ -967,6 +999,31 @@ int ora_vp_accgreater_twice(void* h, uint32_t* val, uint32_t* cnt) {
   for (uint32_t i = 0; i < G.nslots; i++) {
     val[e.vid_of(i)] = e.v[i].val;
     cnt[e.vid_of(i)] = e.v[i].cnt;
+  }
+  return 0;
+}
+
+// Vertex activity after a run of an interface program: prog 0 WDeg
+// (weighted load), 1 MaxLabel, 2 Level from root, 3 CountNonzero, 4
+// CountGreater. flags: 1 = activate_all() before initialize(), 2 =
+// reset_activity() after execute, 4 = activate_all() after execute. out: the
+// program's first state field by original id (wdeg, label, level, cnt,
+// cnt); act: the activity by original id.
+int ora_vp_activity(void* h, int prog, uint32_t root, uint32_t max_iters, uint32_t flags, uint32_t* out,
+                    uint8_t* act) {
+  const OGraph& G = *((Handle*)h)->G;
+  switch (prog) {
+    case 0: { WDegProg p; act_run(p, G, max_iters, flags, [](const WDegProg::S& s) { return s.wdeg; }, out, act); break; }
+    case 1: { MaxLabelProg p; act_run(p, G, max_iters, flags, [](const MaxLabelProg::S& s) { return s.label; }, out, act); break; }
+    case 2: {
+      LevelProg p;
+      p.root = root;
+      act_run(p, G, max_iters, flags, [](const LevelProg::S& s) { return s.level; }, out, act);
+      break;
+    }
+    case 3: { CountNonzeroProg p; act_run(p, G, max_iters, flags, [](const CountNonzeroProg::S& s) { return s.cnt; }, out, act); break; }
+    case 4: { CountGreaterProg p; act_run(p, G, max_iters, flags, [](const CountGreaterProg::S& s) { return s.cnt; }, out, act); break; }
+    default: return 1;
   }
   return 0;
 }

@@ -57,6 +57,7 @@ def lib():
         L.ora_vp_countnonzero.argtypes = [P, u32, P, P]
         L.ora_vp_maxlabel_seq.argtypes = [P, u32, ctypes.c_int, P, u32, u32, P, P]
         L.ora_vp_accgreater_twice.argtypes = [P, P, P]
+        L.ora_vp_activity.argtypes = [P, ctypes.c_int, u32, u32, u32, P, P]
         _lib = L
     return _lib
 
@@ -220,6 +221,21 @@ def vp_maxlabel_seq(g, m1, m2, reset=False, vids=None):
     lib().ora_vp_maxlabel_seq(g.h, int(m1), 1 if reset else 0, None if v is None else _ptr(v),
                               0 if v is None else int(v.size), int(m2), _ptr(label), _ptr(it))
     return label, int(it[0])
+
+
+VP_WDEG, VP_MAXLABEL, VP_LEVEL, VP_COUNTNONZERO, VP_COUNTGREATER = range(5)
+ACT_ALL_BEFORE, ACT_RESET_AFTER, ACT_ALL_AFTER = 1, 2, 4
+
+
+def vp_activity(g, prog, max_iters, flags=0, root=0):
+    """An interface program's first state field and its vertex activity (by
+    original id) after initialize; execute(max_iters), with activate_all()
+    before initialize and reset_activity() / activate_all() after execute as
+    flags ask."""
+    out, act = _u32(g.nslots), np.zeros(g.nslots, dtype=np.uint8)
+    st = lib().ora_vp_activity(g.h, int(prog), int(root), int(max_iters), int(flags), _ptr(out), _ptr(act))
+    assert st == 0
+    return out, act
 
 
 def vp_accgreater_twice(g):

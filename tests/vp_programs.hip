@@ -6,6 +6,8 @@
 // PrAlphaProg, CountGreaterProg, CountNonzeroProg). Each entry point returns 0 or a la3_status
 // and writes per-vertex outputs by original id.
 #include <cstdio>
+#include <cstring>
+#include <sstream>
 #include <string>
 
 #include "la3_vertex_program.hpp"
@@ -43,6 +45,7 @@ struct MaxLabel {
   using A = uint32_t;
   struct S {
     uint32_t label = 0;
+    std::string to_string() const { return "{label: " + std::to_string(label) + "}"; }
   };
   __device__ bool init(uint32_t vid, S& s) const {
     s.label = vid;
@@ -193,6 +196,19 @@ struct AccGreater {
   }
 };
 
+// Shares MaxLabel's states (the borrowing constructor): init adds 1000 to
+// the label it finds and activates nobody.
+struct Bump {
+  using W = la3::Empty;
+  using M = uint32_t;
+  using A = uint32_t;
+  using S = MaxLabel::S;
+  __device__ bool init(uint32_t, S& s) const {
+    s.label += 1000;
+    return false;
+  }
+};
+
 template <class F>
 int guarded(F f) {
   try {
@@ -325,6 +341,126 @@ int vp_accgreater_twice(la3_ctx* c, la3_graph* g, uint32_t* val, uint32_t* cnt) 
       val[i] = st[i].val;
       cnt[i] = st[i].cnt;
     }
+  });
+}
+
+// Vertex activity after a run (the oracle's ora_vp_activity): prog 0 WDeg,
+// 1 MaxLabel, 2 Level from root, 3 CountNonzero, 4 CountGreater; flags 1 =
+// activate_all() before initialize(), 2 = reset_activity() after execute,
+// 4 = activate_all() after execute. out: the first state field by id.
+int vp_activity(la3_ctx* c, la3_graph* g, int prog, uint32_t root, uint32_t max_iters, uint32_t flags,
+                uint32_t* out, uint8_t* act) {
+  return guarded([&] {
+    auto run = [&](auto& vp, auto field) {
+      if (flags & 1) vp.activate_all();
+      vp.initialize();
+      vp.execute(max_iters);
+      if (flags & 2) vp.reset_activity();
+      if (flags & 4) vp.activate_all();
+      auto st = vp.states();
+      auto a = vp.activity();
+      for (size_t i = 0; i < st.size(); i++) {
+        out[i] = field(st[i]);
+        act[i] = a[i];
+      }
+    };
+    if (prog == 0) {
+      la3::VertexProgram<WDeg> vp(c, g, true);
+      run(vp, [](const WDeg::S& s) { return s.wdeg; });
+    } else if (prog == 1) {
+      la3::VertexProgram<MaxLabel> vp(c, g);
+      run(vp, [](const MaxLabel::S& s) { return s.label; });
+    } else if (prog == 2) {
+      Level ops;
+      ops.root = root;
+      la3::VertexProgram<Level> vp(c, g, false, ops);
+      run(vp, [](const Level::S& s) { return s.level; });
+    } else if (prog == 3) {
+      la3::VertexProgram<CountNonzero> vp(c, g);
+      run(vp, [](const CountNonzero::S& s) { return s.cnt; });
+    } else if (prog == 4) {
+      la3::VertexProgram<CountGreater> vp(c, g, true);
+      run(vp, [](const CountGreater::S& s) { return s.cnt; });
+    } else {
+      throw la3::Error(LA3_EINVAL, "unknown program");
+    }
+  });
+}
+
+// MaxLabel until convergence, then: topk(k) of the labels (largest first;
+// active_only as asked) into ids/vals; btopk<2> of (label, id) into bids /
+// bvals (2 x k, batch-major); reduce of the labels (all and active) into
+// sums[0..1]; display<values_only>(nvertices) into text (cap bytes, NUL
+// terminated).
+int vp_extract(la3_ctx* c, la3_graph* g, uint32_t k, int active_only, uint32_t* ids, uint32_t* vals, uint32_t* bids,
+               uint32_t* bvals, uint64_t* sums, uint32_t nvertices, int values_only, char* text, uint64_t cap) {
+  return guarded([&] {
+    la3::VertexProgram<MaxLabel> vp(c, g);
+    vp.initialize();
+    vp.execute(0);
+    using IV = std::pair<uint32_t, uint32_t>;
+    auto greater = [](const IV& a, const IV& b) { return a.second > b.second; };
+    std::vector<IV> top;
+    vp.topk<uint32_t, uint32_t>(k, top, [](uint32_t, const MaxLabel::S& s) { return s.label; }, greater,
+                                active_only != 0);
+    if (top.size() != k) throw la3::Error(LA3_EDEVICE, "topk returned a wrong count");
+    for (uint32_t i = 0; i < k; i++) {
+      ids[i] = top[i].first;
+      vals[i] = top[i].second;
+    }
+    std::vector<IV> bt[2];
+    thread_local uint32_t pair[2];
+    vp.btopk<2, uint32_t, uint32_t>(
+        k, bt,
+        [](uint32_t vid, const MaxLabel::S& s) {
+          pair[0] = s.label;
+          pair[1] = vid;
+          return (const uint32_t*)pair;
+        },
+        greater, active_only != 0);
+    for (uint32_t b = 0; b < 2; b++) {
+      if (bt[b].size() != k) throw la3::Error(LA3_EDEVICE, "btopk returned a wrong count");
+      for (uint32_t i = 0; i < k; i++) {
+        bids[b * k + i] = bt[b][i].first;
+        bvals[b * k + i] = bt[b][i].second;
+      }
+    }
+    auto lab = [](uint32_t, const MaxLabel::S& s) { return (uint64_t)s.label; };
+    auto add = [](uint64_t& a, const uint64_t& b) { a += b; };
+    sums[0] = vp.reduce<uint64_t>(lab, add);
+    sums[1] = vp.reduce<uint64_t>(lab, add, true);
+    std::ostringstream os;
+    if (values_only) vp.display<true>(nvertices, os);
+    else vp.display(nvertices, os);
+    const std::string t = os.str();
+    if (t.size() + 1 > cap) throw la3::Error(LA3_EINVAL, "display text exceeds the buffer");
+    std::memcpy(text, t.c_str(), t.size() + 1);
+  });
+}
+
+// The borrowing constructor: MaxLabel runs to convergence into label0; a
+// Bump program built from it initializes (adding 1000 to every shared label)
+// and MaxLabel's states then read label1; Bump is freed, MaxLabel's states
+// still read label2; MaxLabel is freed and Bump (constructed before)
+// re-runs initialize on fresh states into label3 (every label 1000).
+int vp_borrow(la3_ctx* c, la3_graph* g, uint32_t* label0, uint32_t* label1, uint32_t* label2, uint32_t* label3) {
+  return guarded([&] {
+    auto copy = [](const std::vector<MaxLabel::S>& st, uint32_t* out) {
+      for (size_t i = 0; i < st.size(); i++) out[i] = st[i].label;
+    };
+    la3::VertexProgram<MaxLabel> ml(c, g);
+    ml.initialize();
+    ml.execute(0);
+    copy(ml.states(), label0);
+    la3::VertexProgram<Bump> bump(ml);
+    bump.initialize();
+    bump.execute(1);
+    copy(ml.states(), label1);
+    bump.free();
+    copy(ml.states(), label2);
+    ml.free();
+    bump.initialize();
+    copy(bump.states(), label3);
   });
 }
 
