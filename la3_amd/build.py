@@ -46,6 +46,10 @@ def build(verbose=True):
     _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "apps", "la3_run.cpp"),
           "-o", os.path.join(HERE, "bin", "la3_run"), f"-L{HERE}", "-lla3",
           "-Wl,-rpath,$ORIGIN/..", "-pthread"])
+    # R-MAT generator as a tool (writes LA3 binary edge files).
+    _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "apps", "la3_rmat.cpp"),
+          "-o", os.path.join(HERE, "bin", "la3_rmat"), f"-L{HERE}", "-lla3",
+          "-Wl,-rpath,$ORIGIN/..", "-pthread"])
     # Format tools (the reference's mtx2bin / bin2mtx), host only.
     for tool in ("la3_mtx2bin", "la3_bin2mtx"):
         _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "apps", tool + ".cpp"),

@@ -78,3 +78,32 @@ def test_bin2mtx_round_trip(tmp_path):
     assert b2.read_bytes() == b.read_bytes()
     tool("la3_bin2mtx", b, t, "-hi", "-wi")
     assert t.read_text().split("\n")[:3] == ["1 2", "3 4", "4 1"]
+
+
+@pytest.mark.parametrize("weighted,header", [(False, False), (True, True)])
+def test_rmat_tool_writes_the_generator_records(tmp_path, weighted, header):
+    """la3_rmat writes la3_rmat_generate's records (the same as la3_amd.rmat)
+    as an LA3 binary file, pairs or weighted triples, with the optional
+    16-byte header (u32 n, u32 n, u64 nedges), readable by the loader."""
+    import la3_amd as L
+    out = tmp_path / "g.bin"
+    args = [10, 8, 5, out] + (["-w"] if weighted else []) + (["-ho"] if header else [])
+    line = tool("la3_rmat", *args).split()
+    rec = L.rmat(10, 8, seed=5, weighted=weighted)
+    assert line == [str(1 << 10), str(rec.shape[0])]
+    raw = out.read_bytes()
+    if header:
+        hdr = np.frombuffer(raw[:16], dtype=np.uint32)
+        assert hdr[0] == 1 << 10 and hdr[1] == 1 << 10
+        assert int(np.frombuffer(raw[8:16], dtype=np.uint64)[0]) == rec.shape[0]
+        raw = raw[16:]
+    got = np.frombuffer(raw, dtype=np.uint32).reshape(-1, 3 if weighted else 2)
+    assert (got == rec).all()
+    assert got[:, :2].min() >= 1 and got[:, :2].max() <= 1 << 10 and (got[:, 0] != got[:, 1]).all()
+    if weighted:
+        assert got[:, 2].min() >= 1 and got[:, 2].max() <= 128
+    # the loader reads it; on the header path n is the header's count plus
+    # one, as the reference's loader takes it (graph.hpp:99-100)
+    ctx = L.Context(-1)
+    g = L.Graph.load_file(ctx, str(out), 0 if header else 1 << 10, weighted, L.DIRECTED)
+    assert g.info["nrecords"] == rec.shape[0] and g.info["n"] == (1 << 10) + (1 if header else 0)
