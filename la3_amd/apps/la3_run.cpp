@@ -1,6 +1,7 @@
 // la3_run — drivers of the graph-analytics apps over the C++ drop-in
 // (include/la3.hpp), with the reference's positional arguments and output
-// lines (src/apps/graph_analytics/{degree,pr,bfs,sssp,cc,tc}.cpp):
+// lines, display() of ids up to 30 included, without the logger's prefix
+// (src/apps/graph_analytics/{degree,pr,bfs,sssp,cc,tc}.cpp):
 //   la3_run pr     <filepath> <num_vertices> [<iterations>]
 //   la3_run bfs    <filepath> <num_vertices> <root>       (undirected, bfs.cpp:14)
 //   la3_run sssp   <filepath> <num_vertices> <root>       (weighted triples)
@@ -54,6 +55,7 @@ int main(int argc, char** argv) {
       vp.initialize(vp_degree);
       t = now();
       vp.execute(niters);
+      vp.display();  // pr.cpp:42
       timer("Pagerank Execution", t);
       long deg = vp.reduce<long>([](uint32_t, const la3::PrState& s) -> long { return s.degree; },
                                  [](long& a, const long& b) { a += b; });
@@ -73,6 +75,7 @@ int main(int argc, char** argv) {
         vp.initialize();
         double t = now();
         vp.execute();
+        vp.display();  // bfs.cpp:25
         timer("BFS Execution", t);
         reach = vp.reduce<long>([](uint32_t, const la3::BfsState& s) -> long { return s.hops != 255; },
                                 [](long& a, const long& b) { a += b; });
@@ -86,6 +89,7 @@ int main(int argc, char** argv) {
         vp.initialize();
         double t = now();
         vp.execute();
+        vp.display();  // sssp.cpp:25
         timer("SSSP Execution", t);
         reach = vp.reduce<long>([](uint32_t, const la3::SpState& s) -> long { return s.distance != 0x7fffffffu; },
                                 [](long& a, const long& b) { a += b; });
@@ -101,6 +105,7 @@ int main(int argc, char** argv) {
       vp.initialize();
       double t = now();
       vp.execute();
+      vp.display();  // cc.cpp:24
       timer("CC Execution", t);
       long sum = vp.reduce<long>([](uint32_t, const la3::CcState& s) -> long { return s.label; },
                                  [](long& a, const long& b) { a += b; });
@@ -128,6 +133,8 @@ int main(int argc, char** argv) {
       la3::DegVertex<Empty> vp(&G, true);
       double t = now();
       vp.execute(1);
+      printf(out ? "Out-Degrees: \n" : "In-Degrees: \n");  // degree.cpp:28-32
+      vp.display();
       timer("Degree Execution", t);
       long sum = vp.reduce<long>([](uint32_t, const la3::DegState& s) -> long { return s.degree; },
                                  [](long& a, const long& b) { a += b; });

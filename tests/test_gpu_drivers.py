@@ -93,3 +93,42 @@ def test_ppr_batch_btopk_driver(tmp_path, gpu_ctx):
         for i, v in top:
             assert np.float32(v) == rank[q][i]
         assert np.float32(vals[-1]) == np.sort(rank[q])[::-1][k - 1]
+
+
+def _display(out):
+    """The display() lines of a driver's output, in order."""
+    return [ln for ln in out.splitlines() if re.match(r"^\d+: \{", ln)]
+
+
+def test_drivers_display_g1(g1_files, g1):
+    """Every driver prints display() before its checksums, as the
+    reference's (bfs.cpp:25, sssp.cpp:25, cc.cpp:24, pr.cpp:42,
+    degree.cpp:28-33): ids 0..min(30, n) in the state vector's slot order
+    (regular, sink, source, isolated; hashed order is the id order at n = 8)
+    with the states' to_string formats. Expected lines from the g1 known
+    answers."""
+    cls = g1["classes_directed"]
+    directed = cls["regular"] + cls["sink"] + cls["source"] + cls["isolated"]
+    undirected = [1, 2, 3, 4, 5, 6, 8, 0, 7]  # every vertex with an edge is regular
+    sssp = {int(k): v for k, v in g1["sssp_root5"].items()}
+    out = run("sssp", g1_files[1], 8, 5)
+    assert _display(out) == [f"{i}: {{distance: {sssp.get(i, 2147483647)}}}" for i in directed]
+    hops = {int(k): v for k, v in g1["bfs_undirected_root5_hops"].items()}
+    par = {int(k): v for k, v in g1["bfs_undirected_root5_parent"].items()}
+    out = run("bfs", g1_files[0], 8, 5)
+    assert _display(out) == [f"{i}: {{parent: {par.get(i, 0)}, hops: {hops[i]}}}" for i in undirected]
+    lab = {int(k): v for k, v in g1["cc_labels"].items()}
+    out = run("cc", g1_files[0], 8)
+    assert _display(out) == [f"{i}: {{label: {lab[i]}}}" for i in undirected]
+    rank = {int(k): v for k, v in g1["pr_20"].items()}
+    deg = {int(k): v for k, v in g1["out_degree"].items()}
+    out = run("pr", g1_files[0], 8, 20)
+    shown = [re.match(r"^(\d+): \{rank: (\d+\.\d{6}), degree: (\d+)\}$", ln) for ln in _display(out)]
+    assert all(shown) and [int(m.group(1)) for m in shown] == directed
+    for m in shown:  # "%f" of the FP64 rank: within rounding of the 6th decimal
+        i = int(m.group(1))
+        assert abs(float(m.group(2)) - rank[i]) <= 5.1e-7 and int(m.group(3)) == deg.get(i, 0)
+    indeg = {int(k): v for k, v in g1["in_degree"].items()}
+    out = run("degree", g1_files[0], 8)
+    assert "In-Degrees:" in out
+    assert _display(out) == [f"{i}: {{degree: {indeg.get(i, 0)}}}" for i in directed]
