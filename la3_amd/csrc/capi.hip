@@ -2437,18 +2437,25 @@ la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t
   la3_graph* g = b.graph;
   const DG& d = g->dg;
   const uint32_t S = g->nslots;
-  const uint32_t k = ft.k, W = (k + 63) / 64 == 3 ? 4 : (k + 63) / 64, K = 64 * W;
+  // SSSP and PPR tasks of at most 32 queries take LPR lanes per vertex row
+  // (the batched calls' narrow rows); the others 64, 128 or 256.
+  const uint32_t k = ft.k;
+  const bool narrow = b.kind != LA3_Q_BFS && k <= 32;
+  const uint32_t LPR = !narrow ? 0u : k <= 1 ? 1u : k <= 2 ? 2u : k <= 4 ? 4u : k <= 8 ? 8u : k <= 16 ? 16u : 32u;
+  const uint32_t W = narrow ? FW_NARROW : (k + 63) / 64 == 3 ? 4 : (k + 63) / 64;
+  const uint32_t K = narrow ? LPR : 64 * W;
   std::memset(&T, 0, sizeof(T));
   T.g = d;
   T.kind = (uint32_t)b.kind;
   T.width = W;
+  T.lpr = LPR;
   T.k = k;
   T.iters = b.iters;
   std::vector<uint32_t> slots(k);
   for (uint32_t q = 0; q < k; q++) slots[q] = g->slot_of_id[b.roots[ft.q0 + q]];
   la3_status st;
   auto buf = [&](const char* what, size_t bytes) { return scratch(c, fz_name(t, what).c_str(), bytes); };
-  const size_t SW = (size_t)S * W, SK = (size_t)S * K;
+  const size_t SW = (size_t)S * (narrow ? 1 : W), SK = (size_t)S * K;  // mask words, lanes
   if (b.kind != LA3_Q_PPR) {
     const bool bfs = b.kind == LA3_Q_BFS;
     uint32_t* rs = (uint32_t*)buf("roots", sizeof(uint32_t) * k);
@@ -2463,8 +2470,11 @@ la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t
       launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(S), T.tstamp, (uint64_t)S, 0u);
     }
     const uint64_t qcap = (uint64_t)S + g->nnz / CHUNK + 2;
+    // (a narrow SSSP push queues rows straight into the next list: its
+    // touched list only holds counters)
+    const uint64_t tcap = narrow ? 64 : bfs ? (uint64_t)S + 1 : (uint64_t)S + g->nnz + 1;
     if ((st = get_queue(c, fz_name(t, "q0"), qcap, T.q[0])) || (st = get_queue(c, fz_name(t, "q1"), qcap, T.q[1])) ||
-        (st = get_queue(c, fz_name(t, "touched"), bfs ? (uint64_t)S + 1 : (uint64_t)S + g->nnz + 1, T.touched)))
+        (st = get_queue(c, fz_name(t, "touched"), tcap, T.touched)))
       return st;
     HIPC(c, dcopy_async(rs, slots.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream));
     if (bfs) {
@@ -2498,12 +2508,16 @@ la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t
       T.m[1] = (uint64_t*)buf("m1", 8 * SW);
       if (!T.dist || !T.m[0] || !T.m[1]) return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
       const uint64_t inf2 = ((uint64_t)SSSP_INF << 32) | SSSP_INF;
-      launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for(SK / 2), (unsigned long long*)T.dist,
-             (uint64_t)(SK / 2), (unsigned long long)inf2);
+      if (SK % 2)  // one lane per row: S may be odd
+        launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(SK), T.dist, (uint64_t)SK, SSSP_INF);
+      else
+        launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for(SK / 2), (unsigned long long*)T.dist,
+               (uint64_t)(SK / 2), (unsigned long long)inf2);
       launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for(SW), (unsigned long long*)T.m[0], (uint64_t)SW, 0ull);
       launch(c, "fill_u64", k_fill<unsigned long long>, blocks_for(SW), (unsigned long long*)T.m[1], (uint64_t)SW, 0ull);
       launch(c, "fill_u32", k_fill<uint32_t>, blocks_for(S), T.stamp, (uint64_t)S, 0u);
-      if (W == 1) launch(c, "mssp_roots", k_mssp_roots<1>, 1, d, (const uint32_t*)rs, k, T.dist, T.m[0]);
+      if (narrow) launch(c, "mssp_roots", k_mssp_roots_k, 1, d, (const uint32_t*)rs, k, K, T.dist, T.m[0]);
+      else if (W == 1) launch(c, "mssp_roots", k_mssp_roots<1>, 1, d, (const uint32_t*)rs, k, T.dist, T.m[0]);
       else if (W == 2) launch(c, "mssp_roots", k_mssp_roots<2>, 1, d, (const uint32_t*)rs, k, T.dist, T.m[0]);
       else launch(c, "mssp_roots", k_mssp_roots<4>, 1, d, (const uint32_t*)rs, k, T.dist, T.m[0]);
     }
@@ -2547,18 +2561,29 @@ la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t
   T.step_bytes = 4.0 * g->nnz_reg + (8.0 + kb) * g->ncols + 6.0 * kb * g->nreg + kb * g->nreg;
   T.step_edges = g->nnz_reg;
   launch(c, "outdeg", k_outdeg, blocks_for(S), d, deg);
-  if (W == 1) launch(c, "mppr_init", k_mppr_init<1>, blocks_for(SK), d, a);
-  else if (W == 2) launch(c, "mppr_init", k_mppr_init<2>, blocks_for(SK), d, a);
-  else launch(c, "mppr_init", k_mppr_init<4>, blocks_for(SK), d, a);
+  switch (narrow ? LPR : 64 * W) {
+    case 1: launch(c, "mppr_init", k_mppr_init_k<1>, blocks_for(SK), d, a); break;
+    case 2: launch(c, "mppr_init", k_mppr_init_k<2>, blocks_for(SK), d, a); break;
+    case 4: launch(c, "mppr_init", k_mppr_init_k<4>, blocks_for(SK), d, a); break;
+    case 8: launch(c, "mppr_init", k_mppr_init_k<8>, blocks_for(SK), d, a); break;
+    case 16: launch(c, "mppr_init", k_mppr_init_k<16>, blocks_for(SK), d, a); break;
+    case 32: launch(c, "mppr_init", k_mppr_init_k<32>, blocks_for(SK), d, a); break;
+    case 64: launch(c, "mppr_init", k_mppr_init<1>, blocks_for(SK), d, a); break;
+    case 128: launch(c, "mppr_init", k_mppr_init<2>, blocks_for(SK), d, a); break;
+    default: launch(c, "mppr_init", k_mppr_init<4>, blocks_for(SK), d, a); break;
+  }
   return LA3_OK;
 }
 
-// Post-loop work of one task and delivery of its results.
-template <int QW>
+// Post-loop work of one task and delivery of its results (LPR < 64: a
+// narrow task, LPR lanes per vertex row).
+template <int QW, int LPR = 64>
 la3_status fz_finish(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t, const FTask& T, const FCtl& C) {
   la3_graph* g = b.graph;
   const DG& d = g->dg;
-  const uint32_t S = g->nslots, K = 64 * QW, k = ft.k;
+  constexpr bool SUB = LPR < 64;
+  constexpr uint32_t RPW = SUB ? 64 / LPR : 1;  // rows per wavefront
+  const uint32_t S = g->nslots, K = SUB ? LPR : 64 * QW, k = ft.k;
   const uint32_t wave_blocks = 4 * c->ncu * 8;
   const size_t esz = b.kind == LA3_Q_BFS ? 1 : 4;
   void* out = scratch(c, fz_name(t, "out").c_str(), esz * (size_t)S * k);
@@ -2567,7 +2592,13 @@ la3_status fz_finish(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t 
   if (b.kind == LA3_Q_BFS) {
     launch(c, "msbfs_output", k_msbfs_output, blocks_for(S), d, (const uint8_t*)T.lev, K, k, (uint8_t*)out);
   } else if (b.kind == LA3_Q_SSSP) {
-    if (g->nsink) launch(c, "mssp_sinks", k_mssp_sinks<QW>, std::min<uint32_t>((g->nsink + 3) / 4, wave_blocks), d, T.dist);
+    if (g->nsink) {
+      if constexpr (SUB)
+        launch(c, "mssp_sinks", k_mssp_sub_sinks<LPR>,
+               std::min<uint32_t>((g->nsink + 4 * RPW - 1) / (4 * RPW), wave_blocks), d, T.dist);
+      else
+        launch(c, "mssp_sinks", k_mssp_sinks<QW>, std::min<uint32_t>((g->nsink + 3) / 4, wave_blocks), d, T.dist);
+    }
     launch(c, "mssp_output", k_batch_output<uint32_t>, blocks_for(S), d, (const uint32_t*)T.dist, K, k, (uint32_t*)out);
   } else {
     // Sink step from the final messages (vertex_program_execute.hpp:131-148);
@@ -2579,16 +2610,23 @@ la3_status fz_finish(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t 
       a.xnext = T.xs[xi ^ 1];
       a.resid = (double*)scratch(c, fz_name(t, "res_sink").c_str(), sizeof(double) * wave_blocks * K);
       if (!a.resid) return fail(c, LA3_ENOMEM, "fused run: device allocation failed");
-      const uint32_t rblocks = std::max<uint32_t>(1, std::min<uint32_t>((g->nrows + 3) / 4, wave_blocks));
-      const uint32_t cblocks = std::max<uint32_t>(1, std::min<uint32_t>((d.nplong + 3) / 4, wave_blocks));
-      launch(c, "mppr_rows", k_mppr_rows<QW>, rblocks, d, a, g->nreg, g->nrows, 0);
+      const uint32_t rblocks = std::max<uint32_t>(1, std::min<uint32_t>((g->nrows + 4 * RPW - 1) / (4 * RPW), wave_blocks));
+      const uint32_t cblocks = std::max<uint32_t>(1, std::min<uint32_t>((d.nplong + 4 * RPW - 1) / (4 * RPW), wave_blocks));
+      if constexpr (SUB) launch(c, "mppr_rows", k_mppr_sub_rows<LPR>, rblocks, d, a, g->nreg, g->nrows, 0);
+      else launch(c, "mppr_rows", k_mppr_rows<QW>, rblocks, d, a, g->nreg, g->nrows, 0);
       if (d.npseg > d.npseg_reg) {
         PprBatchArgs as = a;
         const uint32_t s0 = d.npseg_reg;
         as.seg_row = d.pseg_row + s0; as.seg_beg = d.pseg_beg + s0; as.part = a.part + (size_t)s0 * 2 * K;
-        launch(c, "mppr_segs", k_mppr_segs<QW>, std::max<uint32_t>(1, std::min<uint32_t>((d.npseg - s0 + 3) / 4, wave_blocks)),
-               d, as, d.npseg - s0);
-        launch(c, "mppr_combine", k_mppr_combine<QW>, cblocks, d, a, g->nreg, g->nrows, 0);
+        const uint32_t sblocks =
+            std::max<uint32_t>(1, std::min<uint32_t>((d.npseg - s0 + 4 * RPW - 1) / (4 * RPW), wave_blocks));
+        if constexpr (SUB) {
+          launch(c, "mppr_segs", k_mppr_sub_segs<LPR>, sblocks, d, as, d.npseg - s0);
+          launch(c, "mppr_combine", k_mppr_sub_combine<LPR>, cblocks, d, a, g->nreg, g->nrows, 0);
+        } else {
+          launch(c, "mppr_segs", k_mppr_segs<QW>, sblocks, d, as, d.npseg - s0);
+          launch(c, "mppr_combine", k_mppr_combine<QW>, cblocks, d, a, g->nreg, g->nrows, 0);
+        }
       }
     }
     launch(c, "mppr_output", k_batch_output<float>, blocks_for(S), d, (const float*)T.pa.hi, K, k, (float*)out);
@@ -2606,8 +2644,9 @@ la3_status fz_finish(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t 
 }
 
 // One step: the A and B launches of each width class in `mask` (bit 0: 64
-// queries, bit 1: 128, bit 2: 256), the bottom-up BFS launch of each width
-// class with BFS tasks (bits 3-5), then the control launch. With profiling
+// queries, bit 1: 128, bit 2: 256, bit 6: narrow tasks), the bottom-up BFS
+// launch of each width class with BFS tasks (bits 3-5), then the control
+// launch. With profiling
 // each launch is bracketed by events (last_a: the event slot of the last A).
 void fz_step(la3_ctx* c, uint32_t mask, uint32_t ga_blocks, bool events, int* last_a, int* last_p) {
   const uint32_t ga = ga_blocks + FMAX_TASKS, gb = fz_gb(c) + FMAX_TASKS;
@@ -2626,12 +2665,14 @@ void fz_step(la3_ctx* c, uint32_t mask, uint32_t ga_blocks, bool events, int* la
   if (mask & 1) go("fused_a", k_fused_a<1>, ga);
   if (mask & 2) go("fused_a", k_fused_a<2>, ga);
   if (mask & 4) go("fused_a", k_fused_a<4>, ga);
+  if (mask & 64) go("fused_a", k_fused_an, ga);
   if (mask & 8) go("fused_pull", k_fused_pull<1>, ga);
   if (mask & 16) go("fused_pull", k_fused_pull<2>, ga);
   if (mask & 32) go("fused_pull", k_fused_pull<4>, ga);
   if (mask & 1) go("fused_b", k_fused_b<1>, gb);
   if (mask & 2) go("fused_b", k_fused_b<2>, gb);
   if (mask & 4) go("fused_b", k_fused_b<4>, gb);
+  if (mask & 64) go("fused_b", k_fused_bn, gb);
   if (events) launch(c, "fused_c", k_fused_c, 1, tk, hdr, ctl, 0u);
   else hipLaunchKernelGGL(k_fused_c, dim3(1), dim3(BLOCK), 0, c->stream, tk, hdr, ctl, 0u);
 }
@@ -2787,8 +2828,8 @@ la3_status sess_step(la3_session* s) {
   for (uint32_t t = 0; t < s->ntasks; t++) {
     if (!s->slot[t].used) continue;
     const uint32_t w = s->ht[t].width;
-    const uint32_t wb = w == 1 ? 1u : w == 2 ? 2u : 4u;
-    mask |= wb | (s->ht[t].kind == FK_BFS ? wb << 3 : 0u);
+    const uint32_t wb = w == 1 ? 1u : w == 2 ? 2u : w == FW_NARROW ? 64u : 4u;
+    mask |= wb | (s->ht[t].kind == FK_BFS && w != FW_NARROW ? wb << 3 : 0u);
   }
   int last_a = -1, last_pull = -1;
   if (mask) {
@@ -2823,9 +2864,20 @@ la3_status sess_step(la3_session* s) {
     const uint32_t it = kind == LA3_Q_BFS ? std::max<uint32_t>(C.iter, 1) : kind == LA3_Q_SSSP ? C.iter + 1 : C.iter;
     fb.iters_run = std::max(fb.iters_run, it);
     const uint32_t W = s->ht[t].width;
-    st = W == 1 ? fz_finish<1>(c, fb.b, sl.ft, t, s->ht[t], C)
-         : W == 2 ? fz_finish<2>(c, fb.b, sl.ft, t, s->ht[t], C)
-                  : fz_finish<4>(c, fb.b, sl.ft, t, s->ht[t], C);
+    if (W == FW_NARROW) {
+      switch (s->ht[t].lpr) {
+        case 1: st = fz_finish<1, 1>(c, fb.b, sl.ft, t, s->ht[t], C); break;
+        case 2: st = fz_finish<1, 2>(c, fb.b, sl.ft, t, s->ht[t], C); break;
+        case 4: st = fz_finish<1, 4>(c, fb.b, sl.ft, t, s->ht[t], C); break;
+        case 8: st = fz_finish<1, 8>(c, fb.b, sl.ft, t, s->ht[t], C); break;
+        case 16: st = fz_finish<1, 16>(c, fb.b, sl.ft, t, s->ht[t], C); break;
+        default: st = fz_finish<1, 32>(c, fb.b, sl.ft, t, s->ht[t], C); break;
+      }
+    } else {
+      st = W == 1 ? fz_finish<1>(c, fb.b, sl.ft, t, s->ht[t], C)
+           : W == 2 ? fz_finish<2>(c, fb.b, sl.ft, t, s->ht[t], C)
+                    : fz_finish<4>(c, fb.b, sl.ft, t, s->ht[t], C);
+    }
     if (st) fb.st = st;
     // The slot is free again: the device skips it from now on.
     s->fresh[t] = FCtl{};

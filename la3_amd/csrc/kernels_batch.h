@@ -1220,15 +1220,16 @@ __device__ __forceinline__ void ppr_sub_resid_flush(const PprBatchArgs& a, doubl
 }
 
 // Rows [r0, r1) with at most PSEG entries; row group g of wavefront w takes
-// rows r0 + (w + t * nw) * (64 / LPR) + g.
+// rows r0 + (w + t * nw) * (64 / LPR) + g. red: LDS of (BLOCK / WAVE) x LPR
+// doubles (all threads of the block call).
 template <int LPR>
-__global__ void __launch_bounds__(BLOCK) k_mppr_sub_rows(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1, int loop) {
-  __shared__ double red[(BLOCK / WAVE) * LPR];
+__device__ __forceinline__ void mppr_sub_rows_body(const DG& g, Part p, const PprBatchArgs& a, uint32_t r0,
+                                                   uint32_t r1, bool loop, double* red) {
   constexpr uint32_t RPW = WAVE / LPR;
   const uint32_t lane = lane_id(), grp = lane / LPR, q = lane % LPR;
   const bool live = q < a.k;
   const uint32_t seed = live ? a.seeds[q] : 0xffffffffu;
-  const uint32_t nw = gridDim.x * (BLOCK / WAVE), wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  const uint32_t nw = p.nwaves(), wid = p.wid();
   double res = 0.0;
   for (uint64_t base = r0 + (uint64_t)wid * RPW; base < r1; base += (uint64_t)nw * RPW) {
     const uint32_t r = (uint32_t)base + grp;
@@ -1237,19 +1238,25 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_sub_rows(DG g, PprBatchArgs a, u
     if (e - b > PSEG) continue;  // segmented path
     float s = 0.f, c = 0.f;
     ppr_sub_row_sum<LPR>(g, a, b, e, live, s, c);
-    if (live) ppr_sub_finish<LPR>(a, r, s, c, loop != 0, seed, res);
+    if (live) ppr_sub_finish<LPR>(a, r, s, c, loop, seed, res);
   }
-  ppr_sub_resid_flush<LPR>(a, res, blockIdx.x, red);
+  ppr_sub_resid_flush<LPR>(a, res, p.bid, red);
+}
+
+template <int LPR>
+__global__ void __launch_bounds__(BLOCK) k_mppr_sub_rows(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1, int loop) {
+  __shared__ double red[(BLOCK / WAVE) * LPR];
+  mppr_sub_rows_body<LPR>(g, grid_part(), a, r0, r1, loop != 0, red);
 }
 
 // PSEG-entry segments [0, nseg) of the long rows: partial (sum, comp).
 template <int LPR>
-__global__ void __launch_bounds__(BLOCK) k_mppr_sub_segs(DG g, PprBatchArgs a, uint32_t nseg) {
+__device__ __forceinline__ void mppr_sub_segs_body(const DG& g, Part p, const PprBatchArgs& a, uint32_t nseg) {
   constexpr uint32_t SEG_UR = LPR <= 8 ? 32 : 8;  // entries in flight per lane along a segment
   constexpr uint32_t RPW = WAVE / LPR;
   const uint32_t lane = lane_id(), grp = lane / LPR, q = lane % LPR;
   const bool live = q < a.k;
-  const uint32_t nw = gridDim.x * (BLOCK / WAVE), wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  const uint32_t nw = p.nwaves(), wid = p.wid();
   for (uint64_t base = (uint64_t)wid * RPW; base < nseg; base += (uint64_t)nw * RPW) {
     const uint32_t sg = (uint32_t)base + grp;
     if (sg >= nseg) continue;
@@ -1262,16 +1269,20 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_sub_segs(DG g, PprBatchArgs a, u
   }
 }
 
+template <int LPR>
+__global__ void __launch_bounds__(BLOCK) k_mppr_sub_segs(DG g, PprBatchArgs a, uint32_t nseg) {
+  mppr_sub_segs_body<LPR>(g, grid_part(), a, nseg);
+}
+
 // Each long row's segment partials in segment order, then its update.
 template <int LPR>
-__global__ void __launch_bounds__(BLOCK) k_mppr_sub_combine(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1,
-                                                            int loop) {
-  __shared__ double red[(BLOCK / WAVE) * LPR];
+__device__ __forceinline__ void mppr_sub_combine_body(const DG& g, Part p, const PprBatchArgs& a, uint32_t r0,
+                                                      uint32_t r1, bool loop, double* red) {
   constexpr uint32_t RPW = WAVE / LPR;
   const uint32_t lane = lane_id(), grp = lane / LPR, q = lane % LPR;
   const bool live = q < a.k;
   const uint32_t seed = live ? a.seeds[q] : 0xffffffffu;
-  const uint32_t nw = gridDim.x * (BLOCK / WAVE), wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
+  const uint32_t nw = p.nwaves(), wid = p.wid();
   double res = 0.0;
   for (uint64_t base = (uint64_t)wid * RPW; base < a.nlong_ppr; base += (uint64_t)nw * RPW) {
     const uint32_t i = (uint32_t)base + grp;
@@ -1299,9 +1310,16 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_sub_combine(DG g, PprBatchArgs a
       kahan_add(s, c, a.part[(size_t)sg * 2 * LPR + q]);
       kahan_add(s, c, -a.part[(size_t)sg * 2 * LPR + LPR + q]);
     }
-    ppr_sub_finish<LPR>(a, r, s, c, loop != 0, seed, res);
+    ppr_sub_finish<LPR>(a, r, s, c, loop, seed, res);
   }
-  ppr_sub_resid_flush<LPR>(a, res, blockIdx.x, red);
+  ppr_sub_resid_flush<LPR>(a, res, p.bid, red);
+}
+
+template <int LPR>
+__global__ void __launch_bounds__(BLOCK) k_mppr_sub_combine(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1,
+                                                            int loop) {
+  __shared__ double red[(BLOCK / WAVE) * LPR];
+  mppr_sub_combine_body<LPR>(g, grid_part(), a, r0, r1, loop != 0, red);
 }
 
 // Initial state for K lanes per vertex (K = 64 * QW, or LPR).

@@ -21,11 +21,16 @@ namespace la3 {
 enum : uint32_t { FK_BFS = 0, FK_SSSP = 1, FK_PPR = 2 };
 constexpr uint32_t FMAX_TASKS = 64;   // batches of <= 256 queries in one run
 constexpr uint32_t FMAX_WIDTH = 4;    // 64-query words per batch
+// Width class of the narrow tasks: SSSP and PPR tasks of at most 32 queries
+// hold LPR (1, 2, 4, 8, 16 or 32) lanes per vertex row, as the batched
+// calls' narrow-row kernels do, instead of a 64-lane row.
+constexpr uint32_t FW_NARROW = 3;
 
 // One batch ("task"): immutable during a run.
 struct FTask {
   DG g;
-  uint32_t kind, width, k, iters;  // width = ceil(k / 64); iters: PPR iterations
+  uint32_t kind, width, k, iters;  // width = ceil(k / 64), or FW_NARROW; iters: PPR iterations
+  uint32_t lpr;                    // narrow tasks: lanes per vertex row (1..32); else 0
   uint32_t delta;                  // SSSP near/far delta (0 = Bellman-Ford)
   uint32_t nslots_pad;
   uint32_t* stamp;
@@ -155,6 +160,90 @@ __device__ __forceinline__ void fused_ppr_b(const FTask& T, const FCtl& C, Part 
   mppr_combine_body<QW>(g, p, a, 0, g.nreg, true, red);
 }
 
+// Narrow tasks: the narrow-row bodies of the batched calls, so a task of a
+// few queries reads LPR x 4 B per entry with busy lanes instead of a 64-lane
+// row. The SSSP push queues each improved row once and emits its work items
+// into the next list itself (no touched list, no items pass; the smallest
+// queued distance lands in that list's counters); B clears the consumed
+// masks (one word per row).
+template <int LPR>
+__device__ __forceinline__ void fused_sssp_an(const FTask& T, const FCtl& C, Part p, Emitter& em) {
+  const uint32_t cur = C.cur;
+  mssp_sub_push_body<LPR>(T.g, p, em, T.q[cur].items, C.nitems, T.dist, T.m[cur], T.m[cur ^ 1], C.thr,
+                          T.q[cur ^ 1]);
+  em_finish(em, T.q[cur ^ 1], p.bid);
+}
+
+template <int LPR>
+__device__ __forceinline__ void fused_ppr_an(const FTask& T, const FCtl& C, Part p, double* red) {
+  PprBatchArgs a = T.pa;
+  a.x = T.xs[C.cur];
+  a.xnext = T.xs[C.cur ^ 1];
+  a.resid = T.res_rows;
+  mppr_sub_rows_body<LPR>(T.g, p, a, 0, T.g.nreg, true, red);
+  mppr_sub_segs_body<LPR>(T.g, p, a, T.g.npseg_reg);
+}
+
+template <int LPR>
+__device__ __forceinline__ void fused_ppr_bn(const FTask& T, const FCtl& C, Part p, double* red) {
+  PprBatchArgs a = T.pa;
+  a.x = T.xs[C.cur];
+  a.xnext = T.xs[C.cur ^ 1];
+  a.resid = T.res_comb;
+  mppr_sub_combine_body<LPR>(T.g, p, a, 0, T.g.nreg, true, red);
+}
+
+template <int LPR>
+__device__ __forceinline__ void fused_an_task(const FTask& T, const FCtl& C, Part p, Emitter& em, double* red) {
+  if (T.kind == FK_SSSP) fused_sssp_an<LPR>(T, C, p, em);
+  else fused_ppr_an<LPR>(T, C, p, red);
+}
+
+template <int LPR>
+__device__ __forceinline__ void fused_bn_task(const FTask& T, const FCtl& C, Part p, double* red) {
+  if (T.kind == FK_SSSP) clear_items_body<1>(p, T.q[C.cur].items, C.nitems, T.m[C.cur]);
+  else fused_ppr_bn<LPR>(T, C, p, red);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fused_an(const FTask* __restrict__ tasks, const FHdr* hdr,
+                                                    const FCtl* ctl) {
+  Part p;
+  uint32_t t;
+  const FTask* T = find_task<FW_NARROW>(tasks, hdr, ctl, PH_A, p, t);
+  if (!T) return;  // block-uniform
+  const FCtl& C = ctl[t];
+  LA3_EMITTER(em);
+  double* red = (double*)em_lds;
+  switch (T->lpr) {
+    case 1: fused_an_task<1>(*T, C, p, em, red); break;
+    case 2: fused_an_task<2>(*T, C, p, em, red); break;
+    case 4: fused_an_task<4>(*T, C, p, em, red); break;
+    case 8: fused_an_task<8>(*T, C, p, em, red); break;
+    case 16: fused_an_task<16>(*T, C, p, em, red); break;
+    case 32: fused_an_task<32>(*T, C, p, em, red); break;
+    default: break;
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fused_bn(const FTask* __restrict__ tasks, const FHdr* hdr,
+                                                 const FCtl* ctl) {
+  Part p;
+  uint32_t t;
+  const FTask* T = find_task<FW_NARROW>(tasks, hdr, ctl, PH_B, p, t);
+  if (!T) return;  // block-uniform
+  const FCtl& C = ctl[t];
+  __shared__ double red[(BLOCK / WAVE) * 32];
+  switch (T->lpr) {
+    case 1: fused_bn_task<1>(*T, C, p, red); break;
+    case 2: fused_bn_task<2>(*T, C, p, red); break;
+    case 4: fused_bn_task<4>(*T, C, p, red); break;
+    case 8: fused_bn_task<8>(*T, C, p, red); break;
+    case 16: fused_bn_task<16>(*T, C, p, red); break;
+    case 32: fused_bn_task<32>(*T, C, p, red); break;
+    default: break;
+  }
+}
+
 // One launch per width class W present in the run (1, 2 or 4 query words):
 // each holds only the W-wide bodies, so its registers are those of its own
 // widest body.
@@ -250,7 +339,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
     const FTask& T = tasks[t];
     FCtl& C = ctl[t];
     const DG& g = T.g;
-    const uint32_t K = 64 * T.width;
+    const uint32_t K = T.lpr ? T.lpr : 64 * T.width;  // lanes per vertex row
     if (lane == 0) fin[t] = 0;
     const uint32_t state = C.state;
     const bool init = state == FS_NEW;
@@ -302,8 +391,11 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
             C.pbytes += b;
           }
         } else {
+          // narrow tasks queue the improved rows directly: R_touch taken as
+          // the rows queued (a lower bound)
           const double k4 = 4.0 * T.k;
-          b = 8.0 * C.fedges + (8.0 + k4) * C.fcols + 3 * k4 * ts.touched + k4 * ns.verts;
+          const uint32_t rt = T.lpr ? ns.verts : ts.touched;
+          b = 8.0 * C.fedges + (8.0 + k4) * C.fcols + 3 * k4 * rt + k4 * ns.verts;
           e = C.fedges;
         }
         C.bytes += b;
@@ -313,8 +405,11 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
           if (!C.bottom_up && ns.edges * 20 > g.nnz) C.bottom_up = 1;
           else if (C.bottom_up && (uint64_t)ns.verts * 24 < g.nslots) C.bottom_up = 0;
         } else if (T.delta) {
-          // Threshold unchanged while the new work list holds pairs below it.
-          if (ts.nmin >= C.thr) C.thr = (uint32_t)min(0xffffffffull, (unsigned long long)ts.nmin + T.delta);
+          // Threshold unchanged while the new work list holds pairs below it
+          // (the push's smallest queued distance: in the touched list, or for
+          // a narrow task in the next work list).
+          const uint32_t nmin = T.lpr ? ns.nmin : ts.nmin;
+          if (nmin >= C.thr) C.thr = (uint32_t)min(0xffffffffull, (unsigned long long)nmin + T.delta);
         }
         C.cur = nxt;
       }
@@ -345,7 +440,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
     if (!fin[t]) continue;  // block-uniform
     const FTask& T = tasks[t];
     const FCtl& C = ctl[t];
-    const uint32_t K = 64 * T.width;
+    const uint32_t K = T.lpr ? T.lpr : 64 * T.width;
     const uint32_t na = C.a_end - C.a_beg, nb = C.b_end - C.b_beg;
     for (uint32_t q = threadIdx.x; q < K; q += BLOCK) {
       double r = 0;
