@@ -530,6 +530,13 @@ la3_status la3_kernel_stats_reset(la3_ctx* ctx);
  * buffer. Device-resident results let a caller time or chain programs without
  * the PCIe transfer. */
 la3_status la3_result_fetch(la3_ctx* ctx, void* dst, uint64_t bytes);
+/* The first `rows` rows (nslots elements of elem_bytes = 1, 2, 4 or 8 each, by
+ * original id) of the last device-resident result, reordered on the device
+ * to the reference's master-slot order (the vertex-state positions
+ * [regular | sink | source | isolated], each class by hashed id,
+ * src/utils/locator.h:84-120; the order VertexProgram::reduce and topk walk,
+ * src/vprogram/vertex_program.hpp:558-680) and copied into `dst`. */
+la3_status la3_result_fetch_slots(la3_ctx* ctx, const la3_graph* g, uint32_t elem_bytes, uint32_t rows, void* dst);
 
 /* Top-k of each row of the last device-resident result (VertexProgram::topk,
  * vertex_program.hpp:607-680, one row per query for batched results): the

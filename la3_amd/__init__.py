@@ -31,7 +31,7 @@ EXPORTS = [
     "la3_bfs_batch", "la3_sssp_batch", "la3_ppr_batch",
     "la3_last_timing", "la3_set_profiling", "la3_rmat_generate",
     "la3_kernel_stats", "la3_kernel_stats_reset", "la3_host_alloc", "la3_host_free",
-    "la3_copy_bandwidth", "la3_result_fetch",
+    "la3_copy_bandwidth", "la3_result_fetch", "la3_result_fetch_slots",
     "la3_bfs_batch_ctx", "la3_sssp_batch_ctx", "la3_ppr_batch_ctx", "la3_bfs_ctx", "la3_sssp_ctx",
     "la3_server_create", "la3_server_add_graph", "la3_server_query_bytes", "la3_server_submit",
     "la3_server_drain", "la3_server_completed", "la3_server_result", "la3_server_clock",
@@ -155,6 +155,7 @@ def lib():
             "la3_host_free": (None, [P]),
             "la3_copy_bandwidth": (i32, [P, u64, u32, ctypes.POINTER(ctypes.c_double)]),
             "la3_result_fetch": (i32, [P, P, u64]),
+            "la3_result_fetch_slots": (i32, [P, P, u32, u32, P]),
             "la3_bfs_ctx": (i32, [P, P, u32, u32, P, P, P]),
             "la3_sssp_ctx": (i32, [P, P, u32, u32, P, P]),
             "la3_bfs_batch_ctx": (i32, [P, P, P, u32, u32, P, P]),
@@ -300,6 +301,14 @@ class Context:
     def fetch(self, out):
         """Copies the last program's device-resident main output into `out`."""
         self.check(lib().la3_result_fetch(self.h, _ptr(out), out.nbytes), "result_fetch")
+        return out
+
+    def fetch_slots(self, g, dtype, rows=1):
+        """The last device-resident result's first `rows` rows (g.nslots values
+        of numpy `dtype` by id) in the reference's master-slot order."""
+        dt = np.dtype(dtype)
+        out = np.empty((rows, g.nslots), dtype=dt)
+        self.check(lib().la3_result_fetch_slots(self.h, g.h, dt.itemsize, rows, _ptr(out)), "result_fetch_slots")
         return out
 
     def topk(self, dtype, rows, n, k, largest=True):

@@ -2368,6 +2368,26 @@ la3_status la3_result_fetch(la3_ctx* c, void* dst, uint64_t bytes) {
   return LA3_OK;
 }
 
+la3_status la3_result_fetch_slots(la3_ctx* c, const la3_graph* g, uint32_t elem_bytes, uint32_t rows, void* dst) {
+  if (!c || !g || !dst || rows == 0 || (elem_bytes != 1 && elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8))
+    return LA3_EINVAL;
+  if (g->host) return fail(c, LA3_EDEVICE, "graph lives on a host-only context");
+  if (g->ctx->device != c->device) return fail(c, LA3_EINVAL, "graph lives on another device");
+  if (!c->last_result) return fail(c, LA3_EINVAL, "no result on the device");
+  const uint64_t S = g->nslots, bytes = (uint64_t)elem_bytes * S * rows;
+  if (bytes > c->last_result_bytes) return fail(c, LA3_EINVAL, "fetch larger than the last result");
+  if (!dev_covers(c->last_result, bytes)) return fail(c, LA3_EINVAL, "the last result is no longer in device memory");
+  const void* src = c->last_result;  // the scratch call below must not move it
+  void* out = scratch(c, "out.slots", bytes);
+  if (!out) return fail(c, LA3_ENOMEM, "scratch allocation failed");
+  if (!dev_covers(src, bytes)) return fail(c, LA3_EINVAL, "the last result is no longer in device memory");
+  launch(c, "by_slot", k_rows_by_slot, blocks_for(S * rows), g->dg.id_of, (const uint8_t*)src, (uint8_t*)out, S,
+         (uint32_t)rows, elem_bytes);
+  HIPC(c, dcopy_async(dst, out, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return LA3_OK;
+}
+
 la3_status la3_host_alloc(uint64_t bytes, void** out) {
   if (!out) return LA3_EINVAL;
   *out = nullptr;

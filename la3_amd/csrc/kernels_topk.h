@@ -174,4 +174,21 @@ __global__ void k_row_reduce_fin(const Acc* part, uint32_t rows, uint32_t nb, Ac
   }
 }
 
+// Rows of a result by original id reordered to the vertex-state slot order
+// [regular | sink | source | isolated] (each class by hashed id):
+// out[r * S + p] = in[r * S + id_of[p]], elements of eb bytes.
+__global__ void k_rows_by_slot(const uint32_t* id_of, const uint8_t* in, uint8_t* out, uint64_t S, uint32_t rows,
+                               uint32_t eb) {
+  const uint64_t total = S * rows;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = i / S, p = i - r * S;
+    const uint64_t from = r * S + id_of[p];
+    if (eb == 1) out[i] = in[from];
+    else if (eb == 2) ((uint16_t*)out)[i] = ((const uint16_t*)in)[from];
+    else if (eb == 4) ((uint32_t*)out)[i] = ((const uint32_t*)in)[from];
+    else ((uint64_t*)out)[i] = ((const uint64_t*)in)[from];
+  }
+}
+
 }  // namespace la3

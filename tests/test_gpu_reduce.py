@@ -68,3 +68,33 @@ def test_reduce_errors(gpu_ctx, graph):
     gpu_ctx.release_scratch()
     with pytest.raises(L.LA3Error):   # the result is gone
         gpu_ctx.reduce(np.float64, 1, g.nslots, "sum")
+
+
+def test_fetch_in_slot_order(gpu_ctx, oracle):
+    """la3_result_fetch_slots: the last device-resident result reordered to
+    the master-slot order equals the id-order result indexed by the slot
+    order's ids (which the oracle's own layout gives), for u8 BFS levels, u32
+    batched SSSP rows, f64 PageRank and f32 PPR rows; errors for a result too
+    small."""
+    rec = L.rmat(12, 16, seed=91, weighted=True)
+    n = 1 << 12
+    g = L.Graph.from_records(gpu_ctx, rec, n, True, L.DIRECTED)
+    o = oracle.OracleGraph(rec, n, True, oracle.DIRECTED)
+    slot_ids = np.array([o.unhash(h) for h in o.hid_of()], dtype=np.int64)
+    assert (slot_ids == g.array("id_of")).all()
+    roots = [int(x) for x in np.unique(rec[:, 0])[:5]]
+    hops = L.bfs(g, roots[0], parents=False)[0]
+    L.bfs(g, roots[0], parents=False, out=L.DEVICE)
+    assert (gpu_ctx.fetch_slots(g, np.uint8)[0] == hops[slot_ids]).all()
+    dist = L.sssp_batch(g, roots)[0]
+    L.sssp_batch(g, roots, out=L.DEVICE)
+    got = gpu_ctx.fetch_slots(g, np.uint32, rows=len(roots))
+    assert (got == dist[:, slot_ids]).all()
+    rank = L.pagerank(g, 10)[0]
+    L.pagerank(g, 10, out=L.DEVICE)
+    assert (gpu_ctx.fetch_slots(g, np.float64)[0] == rank[slot_ids]).all()
+    pr = L.ppr_batch(g, roots[:3], 5)[0]
+    L.ppr_batch(g, roots[:3], 5, out=L.DEVICE)
+    assert (gpu_ctx.fetch_slots(g, np.float32, rows=3) == pr[:, slot_ids]).all()
+    with pytest.raises(L.LA3Error):
+        gpu_ctx.fetch_slots(g, np.float32, rows=4)
