@@ -75,7 +75,7 @@ def load_traffic(kernel, workload):
 
 
 def cpu_cores():
-    """Host cores this process may use, capped at the 16 a one-GPU box grants."""
+    """Host cores this process may use, at most 16 (the host share of one GPU)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:

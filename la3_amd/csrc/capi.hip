@@ -270,8 +270,9 @@ hipEvent_t get_event(la3_ctx* c) {
 }
 
 // Launches a kernel on the context's stream; with profiling on, brackets it
-// with an event pair. Returns the event slot (or -1) so that the driver can
-// attach the launch's algorithmic bytes once its counters are known.
+// with an event pair. Returns the event slot (or -1) so that the calling
+// program code can attach the launch's algorithmic bytes once its counters
+// are known.
 template <typename... KArgs, typename... Args>
 int launch(la3_ctx* c, const char* name, void (*k)(KArgs...), uint32_t blocks, Args... args) {
   hipEvent_t a = nullptr, b = nullptr;
