@@ -2470,6 +2470,7 @@ la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t
   T.kind = (uint32_t)b.kind;
   T.width = W;
   T.lpr = LPR;
+  T.cls = narrow ? narrow_cls(T.kind, LPR) : wide_cls(W);
   T.k = k;
   T.iters = b.iters;
   std::vector<uint32_t> slots(k);
@@ -2664,11 +2665,24 @@ la3_status fz_finish(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t 
   return deliver(c, host, out, esz * (size_t)S * k);
 }
 
-// One step: the A and B launches of each width class in `mask` (bit 0: 64
-// queries, bit 1: 128, bit 2: 256, bit 6: narrow tasks), the bottom-up BFS
-// launch of each width class with BFS tasks (bits 3-5), then the control
-// launch. With profiling
-// each launch is bracketed by events (last_a: the event slot of the last A).
+// Kernels of the narrow launch classes 3..14 (narrow_cls): SSSP, then PPR,
+// LPR = 1, 2, 4, 8, 16, 32.
+using FzKernel = void (*)(const FTask*, const FHdr*, const FCtl*);
+const FzKernel fz_narrow_a[12] = {
+    k_fused_an<FK_SSSP, 1>, k_fused_an<FK_SSSP, 2>, k_fused_an<FK_SSSP, 4>,  k_fused_an<FK_SSSP, 8>,
+    k_fused_an<FK_SSSP, 16>, k_fused_an<FK_SSSP, 32>, k_fused_an<FK_PPR, 1>, k_fused_an<FK_PPR, 2>,
+    k_fused_an<FK_PPR, 4>,  k_fused_an<FK_PPR, 8>,  k_fused_an<FK_PPR, 16>, k_fused_an<FK_PPR, 32>};
+const FzKernel fz_narrow_b[12] = {
+    k_fused_bn<FK_SSSP, 1>, k_fused_bn<FK_SSSP, 2>, k_fused_bn<FK_SSSP, 4>,  k_fused_bn<FK_SSSP, 8>,
+    k_fused_bn<FK_SSSP, 16>, k_fused_bn<FK_SSSP, 32>, k_fused_bn<FK_PPR, 1>, k_fused_bn<FK_PPR, 2>,
+    k_fused_bn<FK_PPR, 4>,  k_fused_bn<FK_PPR, 8>,  k_fused_bn<FK_PPR, 16>, k_fused_bn<FK_PPR, 32>};
+constexpr uint32_t FZ_PULL_BIT = NCLS;  // bits NCLS + 0..2: bottom-up BFS launches of the wide classes
+
+// One step: the A and B launches of each launch class in `mask` (bit c: class
+// c, wide classes 0-2 = 64, 128, 256 queries, narrow classes 3-14), the
+// bottom-up BFS launch of each wide class with BFS tasks (bits
+// FZ_PULL_BIT + 0..2), then the control launch. With profiling each launch is
+// bracketed by events (last_a: the event slot of the last A).
 void fz_step(la3_ctx* c, uint32_t mask, uint32_t ga_blocks, bool events, int* last_a, int* last_p) {
   const uint32_t ga = ga_blocks + FMAX_TASKS, gb = fz_gb(c) + FMAX_TASKS;
   const FTask* tk = (const FTask*)c->fz_tasks;
@@ -2686,20 +2700,22 @@ void fz_step(la3_ctx* c, uint32_t mask, uint32_t ga_blocks, bool events, int* la
   if (mask & 1) go("fused_a", k_fused_a<1>, ga);
   if (mask & 2) go("fused_a", k_fused_a<2>, ga);
   if (mask & 4) go("fused_a", k_fused_a<4>, ga);
-  if (mask & 64) go("fused_a", k_fused_an, ga);
-  if (mask & 8) go("fused_pull", k_fused_pull<1>, ga);
-  if (mask & 16) go("fused_pull", k_fused_pull<2>, ga);
-  if (mask & 32) go("fused_pull", k_fused_pull<4>, ga);
+  for (uint32_t n = 0; n < 12; n++)
+    if (mask >> (3 + n) & 1) go("fused_a", fz_narrow_a[n], ga);
+  if (mask >> FZ_PULL_BIT & 1) go("fused_pull", k_fused_pull<1>, ga);
+  if (mask >> (FZ_PULL_BIT + 1) & 1) go("fused_pull", k_fused_pull<2>, ga);
+  if (mask >> (FZ_PULL_BIT + 2) & 1) go("fused_pull", k_fused_pull<4>, ga);
   if (mask & 1) go("fused_b", k_fused_b<1>, gb);
   if (mask & 2) go("fused_b", k_fused_b<2>, gb);
   if (mask & 4) go("fused_b", k_fused_b<4>, gb);
-  if (mask & 64) go("fused_b", k_fused_bn, gb);
+  for (uint32_t n = 0; n < 12; n++)
+    if (mask >> (3 + n) & 1) go("fused_b", fz_narrow_b[n], gb);
   if (events) launch(c, "fused_c", k_fused_c, 1, tk, hdr, ctl, 0u);
   else hipLaunchKernelGGL(k_fused_c, dim3(1), dim3(BLOCK), 0, c->stream, tk, hdr, ctl, 0u);
 }
 
 la3_status fz_capture(la3_ctx* c, uint32_t mask, uint32_t steps, uint32_t ga_blocks, hipGraphExec_t* out) {
-  hipGraphExec_t& ex = c->fz_exec[((uint64_t)steps << 40) | ((uint64_t)ga_blocks << 8) | mask];
+  hipGraphExec_t& ex = c->fz_exec[((uint64_t)steps << 52) | ((uint64_t)ga_blocks << 24) | mask];
   *out = ex;
   if (ex) return LA3_OK;
   hipGraph_t graph = nullptr;
@@ -2848,9 +2864,9 @@ la3_status sess_step(la3_session* s) {
   uint32_t mask = 0;
   for (uint32_t t = 0; t < s->ntasks; t++) {
     if (!s->slot[t].used) continue;
-    const uint32_t w = s->ht[t].width;
-    const uint32_t wb = w == 1 ? 1u : w == 2 ? 2u : w == FW_NARROW ? 64u : 4u;
-    mask |= wb | (s->ht[t].kind == FK_BFS && w != FW_NARROW ? wb << 3 : 0u);
+    const uint32_t cls = s->ht[t].cls;
+    mask |= 1u << cls;
+    if (s->ht[t].kind == FK_BFS) mask |= 1u << (FZ_PULL_BIT + cls);  // BFS tasks are wide (cls 0-2)
   }
   int last_a = -1, last_pull = -1;
   if (mask) {

@@ -1250,9 +1250,11 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_sub_rows(DG g, PprBatchArgs a, u
 }
 
 // PSEG-entry segments [0, nseg) of the long rows: partial (sum, comp).
-template <int LPR>
+// UR: entries in flight per lane along a segment (0: 32 for LPR <= 8, else
+// 8); the sums are the same for any UR (loads batched, adds in order).
+template <int LPR, uint32_t UR = 0>
 __device__ __forceinline__ void mppr_sub_segs_body(const DG& g, Part p, const PprBatchArgs& a, uint32_t nseg) {
-  constexpr uint32_t SEG_UR = LPR <= 8 ? 32 : 8;  // entries in flight per lane along a segment
+  constexpr uint32_t SEG_UR = UR ? UR : LPR <= 8 ? 32 : 8;
   constexpr uint32_t RPW = WAVE / LPR;
   const uint32_t lane = lane_id(), grp = lane / LPR, q = lane % LPR;
   const bool live = q < a.k;

@@ -21,16 +21,30 @@ namespace la3 {
 enum : uint32_t { FK_BFS = 0, FK_SSSP = 1, FK_PPR = 2 };
 constexpr uint32_t FMAX_TASKS = 64;   // batches of <= 256 queries in one run
 constexpr uint32_t FMAX_WIDTH = 4;    // 64-query words per batch
-// Width class of the narrow tasks: SSSP and PPR tasks of at most 32 queries
-// hold LPR (1, 2, 4, 8, 16 or 32) lanes per vertex row, as the batched
-// calls' narrow-row kernels do, instead of a 64-lane row.
+// Width of the narrow tasks: SSSP and PPR tasks of at most 32 queries hold
+// LPR (1, 2, 4, 8, 16 or 32) lanes per vertex row, as the batched calls'
+// narrow-row kernels do, instead of a 64-lane row.
 constexpr uint32_t FW_NARROW = 3;
+
+// Launch classes: the wide tasks of 1, 2 and 4 query words (classes 0-2),
+// and one class per (program, LPR) of the narrow tasks (3-8 SSSP, 9-14 PPR),
+// each launched by a kernel of its own, so that each launch carries the
+// registers of one body only.
+constexpr uint32_t NCLS = 15;
+__host__ __device__ constexpr uint32_t lpr_log2(uint32_t lpr) {
+  return lpr <= 1 ? 0 : lpr == 2 ? 1 : lpr == 4 ? 2 : lpr == 8 ? 3 : lpr == 16 ? 4 : 5;
+}
+__host__ __device__ constexpr uint32_t wide_cls(uint32_t width) { return width == 1 ? 0 : width == 2 ? 1 : 2; }
+__host__ __device__ constexpr uint32_t narrow_cls(uint32_t kind, uint32_t lpr) {
+  return 3 + (kind == FK_PPR ? 6 : 0) + lpr_log2(lpr);
+}
 
 // One batch ("task"): immutable during a run.
 struct FTask {
   DG g;
   uint32_t kind, width, k, iters;  // width = ceil(k / 64), or FW_NARROW; iters: PPR iterations
   uint32_t lpr;                    // narrow tasks: lanes per vertex row (1..32); else 0
+  uint32_t cls;                    // launch class (wide_cls / narrow_cls)
   uint32_t delta;                  // SSSP near/far delta (0 = Bellman-Ford)
   uint32_t nslots_pad;
   uint32_t* stamp;
@@ -75,15 +89,15 @@ struct FHdr {
   uint32_t steps, pad[11];
 };
 
-// The task of width W whose share of this launch holds blockIdx.x.
+// The task of launch class CLS whose share of this launch holds blockIdx.x.
 enum : uint32_t { PH_A = 0, PH_B = 1, PH_PULL = 2 };
 
-template <int W>
+template <uint32_t CLS>
 __device__ __forceinline__ const FTask* find_task(const FTask* tasks, const FHdr* h, const FCtl* ctl,
                                                   uint32_t phase, Part& p, uint32_t& t) {
   const uint32_t nt = h->ntasks;
   for (t = 0; t < nt; t++) {
-    if (tasks[t].width != (uint32_t)W) continue;
+    if (tasks[t].cls != CLS) continue;
     const uint32_t b = phase == PH_A ? ctl[t].a_beg : phase == PH_B ? ctl[t].b_beg : ctl[t].p_beg;
     const uint32_t e = phase == PH_A ? ctl[t].a_end : phase == PH_B ? ctl[t].b_end : ctl[t].p_end;
     if (blockIdx.x >= b && blockIdx.x < e) {
@@ -181,7 +195,11 @@ __device__ __forceinline__ void fused_ppr_an(const FTask& T, const FCtl& C, Part
   a.xnext = T.xs[C.cur ^ 1];
   a.resid = T.res_rows;
   mppr_sub_rows_body<LPR>(T.g, p, a, 0, T.g.nreg, true, red);
-  mppr_sub_segs_body<LPR>(T.g, p, a, T.g.npseg_reg);
+  // entries in flight per lane along a segment: the launch holds both
+  // bodies, and 32 costs it three of its eight waves per SIMD, which pays
+  // only with one lane per row (scale 22, 30 iterations: 1 query 16.8 ms with
+  // 32 against 20.3 with 8; 8 queries 31.4 against 26.4)
+  mppr_sub_segs_body<LPR, LPR == 1 ? 32 : 8>(T.g, p, a, T.g.npseg_reg);
 }
 
 template <int LPR>
@@ -193,54 +211,37 @@ __device__ __forceinline__ void fused_ppr_bn(const FTask& T, const FCtl& C, Part
   mppr_sub_combine_body<LPR>(T.g, p, a, 0, T.g.nreg, true, red);
 }
 
-template <int LPR>
-__device__ __forceinline__ void fused_an_task(const FTask& T, const FCtl& C, Part p, Emitter& em, double* red) {
-  if (T.kind == FK_SSSP) fused_sssp_an<LPR>(T, C, p, em);
-  else fused_ppr_an<LPR>(T, C, p, red);
-}
-
-template <int LPR>
-__device__ __forceinline__ void fused_bn_task(const FTask& T, const FCtl& C, Part p, double* red) {
-  if (T.kind == FK_SSSP) clear_items_body<1>(p, T.q[C.cur].items, C.nitems, T.m[C.cur]);
-  else fused_ppr_bn<LPR>(T, C, p, red);
-}
-
+// One narrow class per kernel: program KIND with LPR lanes per row.
+template <uint32_t KIND, int LPR>
 __global__ void __launch_bounds__(BLOCK) k_fused_an(const FTask* __restrict__ tasks, const FHdr* hdr,
-                                                    const FCtl* ctl) {
+                                                 const FCtl* ctl) {
   Part p;
   uint32_t t;
-  const FTask* T = find_task<FW_NARROW>(tasks, hdr, ctl, PH_A, p, t);
+  const FTask* T = find_task<narrow_cls(KIND, LPR)>(tasks, hdr, ctl, PH_A, p, t);
   if (!T) return;  // block-uniform
   const FCtl& C = ctl[t];
-  LA3_EMITTER(em);
-  double* red = (double*)em_lds;
-  switch (T->lpr) {
-    case 1: fused_an_task<1>(*T, C, p, em, red); break;
-    case 2: fused_an_task<2>(*T, C, p, em, red); break;
-    case 4: fused_an_task<4>(*T, C, p, em, red); break;
-    case 8: fused_an_task<8>(*T, C, p, em, red); break;
-    case 16: fused_an_task<16>(*T, C, p, em, red); break;
-    case 32: fused_an_task<32>(*T, C, p, em, red); break;
-    default: break;
+  if constexpr (KIND == FK_SSSP) {
+    LA3_EMITTER(em);
+    fused_sssp_an<LPR>(*T, C, p, em);
+  } else {
+    __shared__ double red[(BLOCK / WAVE) * LPR];
+    fused_ppr_an<LPR>(*T, C, p, red);
   }
 }
 
+template <uint32_t KIND, int LPR>
 __global__ void __launch_bounds__(BLOCK) k_fused_bn(const FTask* __restrict__ tasks, const FHdr* hdr,
                                                  const FCtl* ctl) {
   Part p;
   uint32_t t;
-  const FTask* T = find_task<FW_NARROW>(tasks, hdr, ctl, PH_B, p, t);
+  const FTask* T = find_task<narrow_cls(KIND, LPR)>(tasks, hdr, ctl, PH_B, p, t);
   if (!T) return;  // block-uniform
   const FCtl& C = ctl[t];
-  __shared__ double red[(BLOCK / WAVE) * 32];
-  switch (T->lpr) {
-    case 1: fused_bn_task<1>(*T, C, p, red); break;
-    case 2: fused_bn_task<2>(*T, C, p, red); break;
-    case 4: fused_bn_task<4>(*T, C, p, red); break;
-    case 8: fused_bn_task<8>(*T, C, p, red); break;
-    case 16: fused_bn_task<16>(*T, C, p, red); break;
-    case 32: fused_bn_task<32>(*T, C, p, red); break;
-    default: break;
+  if constexpr (KIND == FK_SSSP) {
+    clear_items_body<1>(p, T->q[C.cur].items, C.nitems, T->m[C.cur]);
+  } else {
+    __shared__ double red[(BLOCK / WAVE) * LPR];
+    fused_ppr_bn<LPR>(*T, C, p, red);
   }
 }
 
@@ -252,7 +253,7 @@ __global__ void __launch_bounds__(BLOCK, 8) k_fused_a(const FTask* __restrict__ 
                                                    const FCtl* ctl) {
   Part p;
   uint32_t t;
-  const FTask* T = find_task<W>(tasks, hdr, ctl, PH_A, p, t);
+  const FTask* T = find_task<wide_cls(W)>(tasks, hdr, ctl, PH_A, p, t);
   if (!T) return;  // block-uniform
   const FCtl& C = ctl[t];
   LA3_EMITTER(em);
@@ -270,7 +271,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused_pull(const FTask* __restrict__ 
                                                       const FCtl* ctl) {
   Part p;
   uint32_t t;
-  const FTask* T = find_task<W>(tasks, hdr, ctl, PH_PULL, p, t);
+  const FTask* T = find_task<wide_cls(W)>(tasks, hdr, ctl, PH_PULL, p, t);
   if (!T) return;  // block-uniform
   const FCtl& C = ctl[t];
   const uint32_t cur = C.cur;
@@ -287,7 +288,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused_b(const FTask* __restrict__ tas
                                                    const FCtl* ctl) {
   Part p;
   uint32_t t;
-  const FTask* T = find_task<W>(tasks, hdr, ctl, PH_B, p, t);
+  const FTask* T = find_task<wide_cls(W)>(tasks, hdr, ctl, PH_B, p, t);
   if (!T) return;  // block-uniform
   const FCtl& C = ctl[t];
   LA3_EMITTER(em);
@@ -452,17 +453,16 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
   }
   __syncthreads();  // the partitions read above are rewritten below
   if (threadIdx.x == 0) {
-    // Blocks of each width class's launches, split among its running tasks:
+    // Blocks of each launch class's launches, split among its running tasks:
     // A (and, for bottom-up BFS steps, the pull launch) in proportion to the
     // estimated bytes (at least 1, at most the useful count), B evenly.
-    double tot[5] = {0, 0, 0, 0, 0}, totp[5] = {0, 0, 0, 0, 0};
-    uint32_t act[5] = {0, 0, 0, 0, 0}, a[5] = {0, 0, 0, 0, 0}, b[5] = {0, 0, 0, 0, 0};
-    uint32_t pp[5] = {0, 0, 0, 0, 0};
+    double tot[NCLS] = {}, totp[NCLS] = {};
+    uint32_t act[NCLS] = {}, a[NCLS] = {}, b[NCLS] = {}, pp[NCLS] = {};
     uint32_t active = 0;
     for (uint32_t t = 0; t < nt; t++) {
       const FCtl& C = ctl[t];
       if (C.state != FS_RUN || C.done) continue;
-      const uint32_t w = min(tasks[t].width, 4u);
+      const uint32_t w = tasks[t].cls;
       const bool pull = tasks[t].kind == FK_BFS && C.bottom_up;
       (pull ? totp : tot)[w] += C.est;
       act[w]++;
@@ -472,7 +472,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
     for (uint32_t t = 0; t < nt; t++) {
       FCtl& C = ctl[t];
       const bool run = C.state == FS_RUN && !C.done;
-      const uint32_t w = run ? min(tasks[t].width, 4u) : 0;
+      const uint32_t w = run ? tasks[t].cls : 0;
       const bool pull = run && tasks[t].kind == FK_BFS && C.bottom_up;
       uint32_t na = 0, nb = 0, np = 0;
       if (run) {
