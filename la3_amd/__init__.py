@@ -247,7 +247,10 @@ class Context:
 
     def __init__(self, device=0):
         h = ctypes.c_void_p()
-        st = lib().la3_ctx_create(device, ctypes.byref(h))
+        # The destructors release through this reference: module globals
+        # may already be gone when they run at interpreter exit.
+        self._L = lib()
+        st = self._L.la3_ctx_create(device, ctypes.byref(h))
         if st:
             raise LA3Error(st, f"la3_ctx_create(device={device}) failed")
         self.h = h
@@ -255,7 +258,7 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().la3_ctx_destroy(self.h)
+            self._L.la3_ctx_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -350,6 +353,7 @@ class Graph:
 
     def __init__(self, ctx, handle):
         self.ctx = ctx
+        self._L = lib()
         self.h = handle
         info = GraphInfo()
         ctx.check(lib().la3_graph_info_get(self.h, ctypes.byref(info)), "graph_info")
@@ -414,7 +418,7 @@ class Graph:
 
     def free(self):
         if getattr(self, "h", None):
-            lib().la3_graph_free(self.h)
+            self._L.la3_graph_free(self.h)
             self.h = None
 
     def __del__(self):
@@ -612,6 +616,7 @@ class Session:
 
     def __init__(self, ctx, alpha=0.15):
         self.ctx = ctx
+        self._L = lib()
         h = ctypes.c_void_p()
         ctx.check(lib().la3_session_create(ctx.h, float(alpha), ctypes.byref(h)), "session_create")
         self.h = h
@@ -656,7 +661,7 @@ class Session:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().la3_session_destroy(self.h)
+            self._L.la3_session_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -672,6 +677,7 @@ class Server:
         cfg = ServerConfig(nworkers, max_batch, state_budget, int(keep_results), ppr_alpha,
                            int(fused))
         h = ctypes.c_void_p()
+        self._L = lib()
         st = lib().la3_server_create(device, ctypes.byref(cfg), ctypes.byref(h))
         if st:
             raise LA3Error(st, "la3_server_create")
@@ -728,7 +734,7 @@ class Server:
 
     def close(self):
         if getattr(self, "h", None):
-            lib().la3_server_destroy(self.h)
+            self._L.la3_server_destroy(self.h)
             self.h = None
 
     def __del__(self):
