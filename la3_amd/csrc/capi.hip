@@ -394,6 +394,8 @@ la3_status upload(la3_graph* g, const std::vector<T>& v, const T** out) {
   return LA3_OK;
 }
 
+la3_status graph_msg_order(la3_ctx* c, la3_graph* g);
+
 la3_status graph_from_layout(la3_ctx* ctx, HostLayout& L, la3_graph** out) {
   auto g = std::make_unique<la3_graph>();
   g->ctx = ctx;
@@ -436,6 +438,10 @@ la3_status graph_from_layout(la3_ctx* ctx, HostLayout& L, la3_graph** out) {
       for (void* p : g->allocs) dev_free(p);
       return s;
     }
+  }
+  if ((s = graph_msg_order(ctx, g.get()))) {
+    for (void* p : g->allocs) dev_free(p);
+    return s;
   }
   g->hid_of.swap(L.hid_of);
   g->id_of.swap(L.id_of);
@@ -698,6 +704,52 @@ la3_status ingest_device(la3_ctx* c, const uint32_t* rec, uint64_t nrec, uint32_
   return LA3_OK;
 }
 
+// Message rows of the narrow PPR kernels by decreasing out-degree
+// (k_msg_keys): DG::rank_of and DG::row_rank, kept with the graph. A graph
+// of 2^31 slots or more (beyond the sort's int count) keeps slot order:
+// rank_of is the identity and row_rank is row_cols.
+la3_status graph_msg_order(la3_ctx* c, la3_graph* g) {
+  DG& d = g->dg;
+  const uint32_t S = d.nslots;
+  hipStream_t st = c->stream;
+  const char* oom = "message order: allocation failed";
+  DevTemps T;
+  uint32_t* rank_of = T.get<uint32_t>(S);
+  uint32_t* row_rank = nullptr;
+  if (!rank_of) return fail(c, LA3_ENOMEM, oom);
+  if (S < (1u << 31)) {
+    uint32_t b = 1;
+    while ((1ull << b) < S) b++;
+    uint64_t* k0 = T.get<uint64_t>(S);
+    uint64_t* k1 = T.get<uint64_t>(S);
+    row_rank = T.get<uint32_t>(d.nnz);
+    if (!k0 || !k1 || !row_rank) return fail(c, LA3_ENOMEM, oom);
+    launch(c, "msg_order", k_msg_keys, blocks_for(S), d, b, k0);
+    hipcub::DoubleBuffer<uint64_t> kb(k0, k1);
+    size_t need = 0;
+    if (hipcub::DeviceRadixSort::SortKeys(nullptr, need, kb, (int)S, 0, (int)(32 + b), st) != hipSuccess)
+      return fail(c, LA3_EDEVICE, "message order: sort failed");
+    void* tmp = T.get<uint8_t>(need);
+    if (!tmp) return fail(c, LA3_ENOMEM, oom);
+    if (hipcub::DeviceRadixSort::SortKeys(tmp, need, kb, (int)S, 0, (int)(32 + b), st) != hipSuccess)
+      return fail(c, LA3_EDEVICE, "message order: sort failed");
+    launch(c, "msg_order", k_msg_rank, blocks_for(S), (const uint64_t*)kb.Current(), S, b, rank_of);
+    if (d.nnz) launch(c, "msg_order", k_msg_entries, blocks_for(d.nnz), d.row_cols, d.nnz, (const uint32_t*)rank_of, row_rank);
+  } else {
+    launch(c, "msg_order", k_msg_identity, blocks_for(S), S, rank_of);
+  }
+  HIPC(c, hipStreamSynchronize(st));  // before the temporaries are freed
+  d.rank_of = rank_of;
+  d.row_rank = row_rank ? row_rank : d.row_cols;
+  for (void* p : {(void*)rank_of, (void*)row_rank}) {
+    if (!p) continue;
+    T.keep(p);
+    g->allocs.push_back(p);
+  }
+  g->device_bytes += 4ull * S + (row_rank ? 4 * std::max<uint64_t>(d.nnz, 1) : 0);
+  return LA3_OK;
+}
+
 la3_status load_records(la3_ctx* ctx, const uint32_t* rec, uint64_t nrec, uint32_t n, int weighted,
                         la3_mode mode, la3_hashing hashing, la3_graph** out) {
   if (!ctx || !out || (nrec && !rec)) return fail(ctx, LA3_EINVAL, "null argument");
@@ -712,6 +764,7 @@ la3_status load_records(la3_ctx* ctx, const uint32_t* rec, uint64_t nrec, uint32
     auto g = std::make_unique<la3_graph>();
     g->ctx = ctx;
     la3_status st = ingest_device(ctx, rec, nrec, n, weighted != 0, (int)mode, hashing != LA3_HASH_NONE, g.get());
+    if (!st) st = graph_msg_order(ctx, g.get());
     if (st) {
       for (void* p : g->allocs) dev_free(p);
       return st;

@@ -17,6 +17,11 @@
 //       reference's message order: source columns, then regular columns
 //       (src/vprogram/vertex_program_execute.hpp:477-482); row_split marks
 //       the first regular column. row_cols holds the sender's slot.
+//   rank_of[nslots], row_rank[nnz]
+//       Message rows of the narrow PPR kernels (at most 32 queries): slots
+//       by decreasing out-degree (ties by slot), rank_of[slot] = message row,
+//       row_rank[j] = rank_of[row_cols[j]]. The senders of most entries then
+//       share few cache lines.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -35,6 +40,8 @@ struct DG {
   const uint32_t* row_split;
   const uint32_t* row_cols;
   const uint32_t* row_w;
+  const uint32_t* rank_of;
+  const uint32_t* row_rank;
   const uint32_t* id_of;
   const uint32_t* hid_of;
   const uint32_t* long_rows;

@@ -882,7 +882,7 @@ namespace la3 {
 constexpr uint32_t PSEG = PSEG_ENTRIES;
 
 struct PprBatchArgs {
-  const float* x;        // messages [slot * K + q]
+  const float* x;        // messages [slot * K + q]; narrow rows: [rank_of[slot] * K + q]
   float* xnext;
   float* hi;             // rank [slot * K + q]
   float* lo;             // compensation
@@ -1155,7 +1155,9 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_combine(DG g, PprBatchArgs a, ui
 // most lanes of every row idle. Per query the arithmetic is that of the
 // 64-lane kernels above (same entries, same order, same updates), so a
 // query's ranks do not depend on the width of the batch it ran in; the FP64
-// residual partials are grouped differently.
+// residual partials are grouped differently. Message rows are stored by
+// sender rank (DG::rank_of, decreasing out-degree) and gathered through
+// DG::row_rank, so the senders of most entries share a few cached lines.
 template <int LPR, uint32_t UR = 8>
 __device__ __forceinline__ void ppr_sub_row_sum(const DG& g, const PprBatchArgs& a, uint32_t b, uint32_t e,
                                                 bool live, float& s, float& c) {
@@ -1164,7 +1166,7 @@ __device__ __forceinline__ void ppr_sub_row_sum(const DG& g, const PprBatchArgs&
   for (; j + UR <= e; j += UR) {
     uint32_t sc[UR];
 #pragma unroll
-    for (uint32_t u = 0; u < UR; u++) sc[u] = g.row_cols[j + u];
+    for (uint32_t u = 0; u < UR; u++) sc[u] = g.row_rank[j + u];
     if (!live) continue;
     float v[UR];
 #pragma unroll
@@ -1176,7 +1178,7 @@ __device__ __forceinline__ void ppr_sub_row_sum(const DG& g, const PprBatchArgs&
     const uint32_t rem = e - j;
     uint32_t sc[UR];
 #pragma unroll
-    for (uint32_t u = 0; u < UR; u++) sc[u] = g.row_cols[min(j + u, e - 1)];
+    for (uint32_t u = 0; u < UR; u++) sc[u] = g.row_rank[min(j + u, e - 1)];
     if (live) {
       float v[UR];
 #pragma unroll
@@ -1188,10 +1190,11 @@ __device__ __forceinline__ void ppr_sub_row_sum(const DG& g, const PprBatchArgs&
   }
 }
 
-// Lane `q` of a row group: the row's update for query q (q < k).
+// Lane `q` of a row group: the row's update for query q (q < k); xr is the
+// row's message row (rank_of[r]; read only when the messages advance).
 template <int LPR>
-__device__ __forceinline__ void ppr_sub_finish(const PprBatchArgs& a, uint32_t r, float s, float c, bool loop,
-                                               uint32_t seed, double& res) {
+__device__ __forceinline__ void ppr_sub_finish(const PprBatchArgs& a, uint32_t r, uint32_t xr, float s, float c,
+                                               bool loop, uint32_t seed, double& res) {
   const uint32_t q = lane_id() % LPR;
   const size_t o = (size_t)r * LPR + q;
   const bool sd = seed == r;
@@ -1200,7 +1203,7 @@ __device__ __forceinline__ void ppr_sub_finish(const PprBatchArgs& a, uint32_t r
   res += fabs(((double)nh + nl) - ((double)a.hi[o] + a.lo[o]));
   a.hi[o] = nh;
   a.lo[o] = nl;
-  if (loop) a.xnext[o] = ppr_msg(nh, nl, a.deg[r]);
+  if (loop) a.xnext[(size_t)xr * LPR + q] = ppr_msg(nh, nl, a.deg[r]);
 }
 
 // Per-block, per-query FP64 residual: the row groups of a wavefront by a
@@ -1238,7 +1241,7 @@ __device__ __forceinline__ void mppr_sub_rows_body(const DG& g, Part p, const Pp
     if (e - b > PSEG) continue;  // segmented path
     float s = 0.f, c = 0.f;
     ppr_sub_row_sum<LPR>(g, a, b, e, live, s, c);
-    if (live) ppr_sub_finish<LPR>(a, r, s, c, loop, seed, res);
+    if (live) ppr_sub_finish<LPR>(a, r, loop ? g.rank_of[r] : 0u, s, c, loop, seed, res);
   }
   ppr_sub_resid_flush<LPR>(a, res, p.bid, red);
 }
@@ -1312,7 +1315,7 @@ __device__ __forceinline__ void mppr_sub_combine_body(const DG& g, Part p, const
       kahan_add(s, c, a.part[(size_t)sg * 2 * LPR + q]);
       kahan_add(s, c, -a.part[(size_t)sg * 2 * LPR + LPR + q]);
     }
-    ppr_sub_finish<LPR>(a, r, s, c, loop, seed, res);
+    ppr_sub_finish<LPR>(a, r, loop ? g.rank_of[r] : 0u, s, c, loop, seed, res);
   }
   ppr_sub_resid_flush<LPR>(a, res, p.bid, red);
 }
@@ -1324,7 +1327,8 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_sub_combine(DG g, PprBatchArgs a
   mppr_sub_combine_body<LPR>(g, grid_part(), a, r0, r1, loop != 0, red);
 }
 
-// Initial state for K lanes per vertex (K = 64 * QW, or LPR).
+// Initial state for K lanes per vertex (K = 64 * QW, or LPR; narrow rows,
+// K < 64, store the messages by rank).
 template <uint32_t K>
 __global__ void k_mppr_init_k(DG g, PprBatchArgs a) {
   const uint64_t total = (uint64_t)g.nslots * K;
@@ -1336,8 +1340,9 @@ __global__ void k_mppr_init_k(DG g, PprBatchArgs a) {
     a.hi[i] = th;
     a.lo[i] = tl;
     const float x = slot_has_col(g, s) ? ppr_msg(th, tl, a.deg[s]) : 0.f;
-    ((float*)a.x)[i] = x;
-    a.xnext[i] = x;
+    const uint64_t xi = K < 64 ? (uint64_t)g.rank_of[s] * K + q : i;
+    ((float*)a.x)[xi] = x;
+    a.xnext[xi] = x;
   }
 }
 
