@@ -718,22 +718,24 @@ la3_status graph_msg_order(la3_ctx* c, la3_graph* g) {
   uint32_t* row_rank = nullptr;
   if (!rank_of) return fail(c, LA3_ENOMEM, oom);
   if (S < (1u << 31)) {
-    uint32_t b = 1;
-    while ((1ull << b) < S) b++;
+    // the key and value types of the ingest sorts (one instantiation)
     uint64_t* k0 = T.get<uint64_t>(S);
     uint64_t* k1 = T.get<uint64_t>(S);
+    uint32_t* v0 = T.get<uint32_t>(S);
+    uint32_t* v1 = T.get<uint32_t>(S);
     row_rank = T.get<uint32_t>(d.nnz);
-    if (!k0 || !k1 || !row_rank) return fail(c, LA3_ENOMEM, oom);
-    launch(c, "msg_order", k_msg_keys, blocks_for(S), d, b, k0);
+    if (!k0 || !k1 || !v0 || !v1 || !row_rank) return fail(c, LA3_ENOMEM, oom);
+    launch(c, "msg_order", k_msg_keys, blocks_for(S), d, k0, v0);
     hipcub::DoubleBuffer<uint64_t> kb(k0, k1);
+    hipcub::DoubleBuffer<uint32_t> vb(v0, v1);
     size_t need = 0;
-    if (hipcub::DeviceRadixSort::SortKeys(nullptr, need, kb, (int)S, 0, (int)(32 + b), st) != hipSuccess)
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, need, kb, vb, (int)S, 0, 32, st) != hipSuccess)
       return fail(c, LA3_EDEVICE, "message order: sort failed");
     void* tmp = T.get<uint8_t>(need);
     if (!tmp) return fail(c, LA3_ENOMEM, oom);
-    if (hipcub::DeviceRadixSort::SortKeys(tmp, need, kb, (int)S, 0, (int)(32 + b), st) != hipSuccess)
+    if (hipcub::DeviceRadixSort::SortPairs(tmp, need, kb, vb, (int)S, 0, 32, st) != hipSuccess)
       return fail(c, LA3_EDEVICE, "message order: sort failed");
-    launch(c, "msg_order", k_msg_rank, blocks_for(S), (const uint64_t*)kb.Current(), S, b, rank_of);
+    launch(c, "msg_order", k_msg_rank, blocks_for(S), (const uint32_t*)vb.Current(), S, rank_of);
     if (d.nnz) launch(c, "msg_order", k_msg_entries, blocks_for(d.nnz), d.row_cols, d.nnz, (const uint32_t*)rank_of, row_rank);
   } else {
     launch(c, "msg_order", k_msg_identity, blocks_for(S), S, rank_of);

@@ -232,20 +232,20 @@ __global__ void k_ing_lensum(const uint32_t* rows, uint32_t n, const uint32_t* r
 // 32 / LPR senders. In slot order (hashed ids) the senders of a line are
 // unrelated; ordered by decreasing out-degree, the senders of most entries
 // share few lines, which stay in L2 (R-MAT scale 24, one query: the 263M
-// gathers of an iteration take 2.4 ms against 4.1 ms in slot order). The
-// order: keys (~outdeg << b) | slot sorted ascending, b = bits of a slot.
-__global__ void k_msg_keys(DG g, uint32_t b, uint64_t* key) {
+// gathers of an iteration take 2.4 ms against 4.1 ms in a random order).
+// The order: (key ~outdeg, value slot) pairs in slot order, stably sorted
+// by key; then rank_of[slot] = position.
+__global__ void k_msg_keys(DG g, uint64_t* key, uint32_t* slot) {
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < g.nslots; s += gridDim.x * blockDim.x) {
     uint32_t d = 0;
     if (slot_has_col(g, s)) { const uint32_t c = col_of_slot(g, s); d = g.col_ptr[c + 1] - g.col_ptr[c]; }
-    key[s] = ((uint64_t)(~d) << b) | s;
+    key[s] = ~d;
+    slot[s] = s;
   }
 }
 
-__global__ void k_msg_rank(const uint64_t* key, uint32_t n, uint32_t b, uint32_t* rank_of) {
-  const uint64_t m = (1ull << b) - 1;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    rank_of[(uint32_t)(key[i] & m)] = i;
+__global__ void k_msg_rank(const uint32_t* slot, uint32_t n, uint32_t* rank_of) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) rank_of[slot[i]] = i;
 }
 
 __global__ void k_msg_identity(uint32_t n, uint32_t* rank_of) {
