@@ -2155,34 +2155,34 @@ static la3_status mppr_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   launch(c, "mppr_init", k_mppr_init_k<K>, blocks_for(SK), d, a);
   float* xa = x0;
   float* xb = x1;
-  int ev_rows = -1, ev_segs = -1;
+  int ev_rows = -1;
+  // Rows and long-row segments in one launch (residual partials of its
+  // rblocks blocks in rr), then the long rows' combine (partials in rc).
   auto step = [&](uint32_t r0, uint32_t r1, uint32_t s0, uint32_t s1, int loop, double* rr, double* rc) {
     a.x = xa; a.xnext = xb;
     a.resid = rr;
-    if constexpr (SUB) ev_rows = launch(c, "mppr_rows", k_mppr_sub_rows<LPR>, rblocks, d, a, r0, r1, loop);
-    else ev_rows = launch(c, "mppr_rows", k_mppr_rows<QW>, rblocks, d, a, r0, r1, loop);
-    ev_segs = -1;
-    if (s1 > s0) {
+    if (s1 == s0) {
+      if constexpr (SUB) ev_rows = launch(c, "mppr_rows", k_mppr_sub_rows<LPR>, rblocks, d, a, r0, r1, loop);
+      else ev_rows = launch(c, "mppr_rows", k_mppr_rows<QW>, rblocks, d, a, r0, r1, loop);
+    } else {
       PprBatchArgs as = a;
       as.seg_row = d.pseg_row + s0; as.seg_beg = d.pseg_beg + s0; as.part = part + (size_t)s0 * 2 * K;
-      const uint32_t sblocks = std::max<uint32_t>(1, std::min<uint32_t>((s1 - s0 + 4 * RPW - 1) / (4 * RPW), wave_blocks));
-      if constexpr (SUB) ev_segs = launch(c, "mppr_segs", k_mppr_sub_segs<LPR>, sblocks, d, as, s1 - s0);
-      else ev_segs = launch(c, "mppr_segs", k_mppr_segs<QW>, sblocks, d, as, s1 - s0);
+      if constexpr (SUB)
+        ev_rows = launch(c, "mppr_rows", k_mppr_sub_rowsegs<LPR>, rblocks, d, a, r0, r1, loop, as, s1 - s0);
+      else
+        ev_rows = launch(c, "mppr_rows", k_mppr_rowsegs<QW>, rblocks, d, a, r0, r1, loop, as, s1 - s0);
       a.resid = rc;
       if constexpr (SUB) launch(c, "mppr_combine", k_mppr_sub_combine<LPR>, cblocks, d, a, r0, r1, loop);
       else launch(c, "mppr_combine", k_mppr_combine<QW>, cblocks, d, a, r0, r1, loop);
     }
   };
-  // Algorithmic bytes per iteration: 4 B index + 4k B message row per entry
-  // of a regular row (live query lanes only), 20k B of state and messages
-  // per regular row; entries of rows longer than PSEG are read by
-  // mppr_segs, the rest by mppr_rows.
+  // Algorithmic bytes per iteration of the rows launch: 4 B index + 4k B
+  // message row per entry of a regular row (live query lanes only), 20k B
+  // of state and messages per regular row, 8k B per segment partial.
   const double kb = 4.0 * k;
-  const uint64_t nnz_short = g->nnz_reg - g->pnnz_reg;
   for (uint32_t it = 0; it < iters; it++) {
     step(0, g->nreg, 0, d.npseg_reg, 1, res_r, res_c);
-    annotate(c, ev_rows, (4.0 + kb) * nnz_short + 5.0 * kb * g->nreg, nnz_short);
-    annotate(c, ev_segs, (4.0 + kb) * g->pnnz_reg + 2.0 * kb * d.npseg_reg, g->pnnz_reg);
+    annotate(c, ev_rows, (4.0 + kb) * g->nnz_reg + 5.0 * kb * g->nreg + 2.0 * kb * d.npseg_reg, g->nnz_reg);
     c->edges_scanned += g->nnz_reg;
     std::swap(xa, xb);
   }

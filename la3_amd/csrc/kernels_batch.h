@@ -1092,6 +1092,18 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_segs(DG g, PprBatchArgs a, uint3
   mppr_segs_body<QW>(g, grid_part(), a, nseg);
 }
 
+// One iteration's row sums in one launch: the short rows, then the segments
+// of the long rows (`as`: the segment arrays from the first segment on), so
+// the two parts' tails overlap. The residual partials are those of the rows
+// part, one per block.
+template <int QW>
+__global__ void __launch_bounds__(BLOCK) k_mppr_rowsegs(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1, int loop,
+                                                        PprBatchArgs as, uint32_t nseg) {
+  __shared__ double red[(BLOCK / WAVE) * 64 * QW];
+  mppr_rows_body<QW>(g, grid_part(), a, r0, r1, loop != 0, red);
+  mppr_segs_body<QW>(g, grid_part(), as, nseg);
+}
+
 // Combines each long row's segment partials in segment order, then applies.
 template <int QW>
 __device__ __forceinline__ void mppr_combine_body(const DG& g, Part p, const PprBatchArgs& a, uint32_t r0,
@@ -1277,6 +1289,18 @@ __device__ __forceinline__ void mppr_sub_segs_body(const DG& g, Part p, const Pp
 template <int LPR>
 __global__ void __launch_bounds__(BLOCK) k_mppr_sub_segs(DG g, PprBatchArgs a, uint32_t nseg) {
   mppr_sub_segs_body<LPR>(g, grid_part(), a, nseg);
+}
+
+// Short rows and long-row segments in one launch, as k_mppr_rowsegs. The
+// segment part keeps 32 entries in flight per lane only with one lane per
+// row: in a launch that also holds the row part, 32 costs three of eight
+// waves per SIMD, which pays only there.
+template <int LPR>
+__global__ void __launch_bounds__(BLOCK) k_mppr_sub_rowsegs(DG g, PprBatchArgs a, uint32_t r0, uint32_t r1,
+                                                            int loop, PprBatchArgs as, uint32_t nseg) {
+  __shared__ double red[(BLOCK / WAVE) * LPR];
+  mppr_sub_rows_body<LPR>(g, grid_part(), a, r0, r1, loop != 0, red);
+  mppr_sub_segs_body<LPR, LPR == 1 ? 32 : 8>(g, grid_part(), as, nseg);
 }
 
 // Each long row's segment partials in segment order, then its update.
