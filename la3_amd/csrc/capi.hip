@@ -705,9 +705,10 @@ la3_status ingest_device(la3_ctx* c, const uint32_t* rec, uint64_t nrec, uint32_
 }
 
 // Message rows of the narrow PPR kernels by decreasing out-degree
-// (k_msg_keys): DG::rank_of and DG::row_rank, kept with the graph. A graph
-// of 2^31 slots or more (beyond the sort's int count) keeps slot order:
-// rank_of is the identity and row_rank is row_cols.
+// (k_msg_keys): DG::rank_of and DG::row_rank, kept with the graph. An
+// undirected load (PPR runs on directed loads only) and a graph of 2^31
+// slots or more (beyond the sort's int count) keep slot order: rank_of is
+// the identity and row_rank is row_cols.
 la3_status graph_msg_order(la3_ctx* c, la3_graph* g) {
   DG& d = g->dg;
   const uint32_t S = d.nslots;
@@ -717,7 +718,7 @@ la3_status graph_msg_order(la3_ctx* c, la3_graph* g) {
   uint32_t* rank_of = T.get<uint32_t>(S);
   uint32_t* row_rank = nullptr;
   if (!rank_of) return fail(c, LA3_ENOMEM, oom);
-  if (S < (1u << 31)) {
+  if (S < (1u << 31) && g->directed) {
     // the key and value types of the ingest sorts (one instantiation)
     uint64_t* k0 = T.get<uint64_t>(S);
     uint64_t* k1 = T.get<uint64_t>(S);
