@@ -566,10 +566,12 @@ __global__ void k_sssp_sinks(DG g, uint32_t* dist) {
 // base(r) = alpha for PR (seed == ~0), alpha at the seed only for PPR.
 // Rows with <= LONG_ROW entries: one lane per row, sequential sum in the
 // reference's message order (sources first). Longer rows: one wavefront,
-// lane-strided partial sums and a fixed butterfly.
+// lane-strided partial sums and a fixed butterfly. Messages are stored by
+// sender rank (DG::rank_of, decreasing out-degree) and gathered through
+// DG::row_rank: the senders of most entries then share a few cached lines.
 struct PrArgs {
-  const double* x;       // messages, by slot
-  double* xnext;         // next messages (regular rows), by slot
+  const double* x;       // messages, by rank_of[slot]
+  double* xnext;         // next messages (regular rows), by rank_of[slot]
   double* rank;          // by slot
   double* srcsum;        // per regular row: sum over source entries (constant)
   const uint32_t* deg;   // out-degree by slot
@@ -592,7 +594,7 @@ __device__ __forceinline__ void pr_finish(const DG& g, const PrArgs& a, uint32_t
   if (MODE == 1) {
     const double old = a.rank[r];
     a.rank[r] = nr;
-    a.xnext[r] = nr / (double)a.deg[r];
+    a.xnext[g.rank_of[r]] = nr / (double)a.deg[r];
     if (fabs(nr - old) > a.tol) a.active[0] = 1;
   } else {
     a.rank[r] = nr;
@@ -616,7 +618,7 @@ __device__ __forceinline__ double seq_sum(const DG& g, const double* x, uint32_t
     uint32_t cc[8];
     double v[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) cc[u] = g.row_cols[j + u];
+    for (int u = 0; u < 8; u++) cc[u] = g.row_rank[j + u];
 #pragma unroll
     for (int u = 0; u < 8; u++) v[u] = x[cc[u]];
 #pragma unroll
@@ -627,7 +629,7 @@ __device__ __forceinline__ double seq_sum(const DG& g, const double* x, uint32_t
     uint32_t cc[8];
     double v[8];
 #pragma unroll
-    for (uint32_t u = 0; u < 8; u++) cc[u] = g.row_cols[min(j + u, hi - 1)];
+    for (uint32_t u = 0; u < 8; u++) cc[u] = g.row_rank[min(j + u, hi - 1)];
 #pragma unroll
     for (int u = 0; u < 8; u++) v[u] = x[cc[u]];
 #pragma unroll
@@ -647,22 +649,22 @@ __device__ __forceinline__ double wave_range_sum(const DG& g, const double* x, u
     uint32_t cc[8];
     double v[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) cc[u] = g.row_cols[j + u * WAVE];
+    for (int u = 0; u < 8; u++) cc[u] = g.row_rank[j + u * WAVE];
 #pragma unroll
     for (int u = 0; u < 8; u++) v[u] = x[cc[u]];
 #pragma unroll
     for (int u = 0; u < 8; u++) p = __dadd_rn(p, v[u]);
   }
   for (; j + 3 * WAVE < hi; j += 4 * WAVE) {
-    const uint32_t c0 = g.row_cols[j], c1 = g.row_cols[j + WAVE], c2 = g.row_cols[j + 2 * WAVE],
-                   c3 = g.row_cols[j + 3 * WAVE];
+    const uint32_t c0 = g.row_rank[j], c1 = g.row_rank[j + WAVE], c2 = g.row_rank[j + 2 * WAVE],
+                   c3 = g.row_rank[j + 3 * WAVE];
     const double v0 = x[c0], v1 = x[c1], v2 = x[c2], v3 = x[c3];
     p = __dadd_rn(p, v0);
     p = __dadd_rn(p, v1);
     p = __dadd_rn(p, v2);
     p = __dadd_rn(p, v3);
   }
-  for (; j < hi; j += WAVE) p = __dadd_rn(p, x[g.row_cols[j]]);
+  for (; j < hi; j += WAVE) p = __dadd_rn(p, x[g.row_rank[j]]);
   return wave_sum(p);
 }
 
@@ -677,13 +679,13 @@ __device__ __forceinline__ double block_range_sum(const DG& g, const double* x, 
     uint32_t cc[8];
     double v[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) cc[u] = g.row_cols[j + u * BLOCK];
+    for (int u = 0; u < 8; u++) cc[u] = g.row_rank[j + u * BLOCK];
 #pragma unroll
     for (int u = 0; u < 8; u++) v[u] = x[cc[u]];
 #pragma unroll
     for (int u = 0; u < 8; u++) p = __dadd_rn(p, v[u]);
   }
-  for (; j < hi; j += BLOCK) p = __dadd_rn(p, x[g.row_cols[j]]);
+  for (; j < hi; j += BLOCK) p = __dadd_rn(p, x[g.row_rank[j]]);
   p = wave_sum(p);
   if (lane_id() == 0) red[threadIdx.x / WAVE] = p;
   __syncthreads();
@@ -777,8 +779,8 @@ __global__ void k_pr_init(DG g, PrArgs a) {
     a.rank[s] = r0;
     double x = 0.0;
     if (slot_has_col(g, s)) x = r0 / (double)a.deg[s];
-    ((double*)a.x)[s] = x;
-    a.xnext[s] = x;
+    ((double*)a.x)[g.rank_of[s]] = x;
+    a.xnext[g.rank_of[s]] = x;
   }
 }
 
