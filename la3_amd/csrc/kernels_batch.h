@@ -1234,6 +1234,27 @@ __device__ __forceinline__ void ppr_sub_resid_flush(const PprBatchArgs& a, doubl
   }
 }
 
+// Chunks of RPW consecutive items (segments, long rows) over the wavefronts
+// of a partition. With `spread` and fewer chunks than the first CHUNK_SPAN
+// wavefronts (2048 blocks, one full residency of the device), every
+// stride-th of those takes one, so the work spreads over all CUs instead of
+// the first few blocks; otherwise chunk c goes to wavefront c mod nw, as a
+// plain grid-stride loop. Wave-uniform. Spreading pays with one lane per
+// row, where a chunk is 64 long segments (R-MAT, one-query PPR, 30
+// iterations: scale 18 2.32 -> 1.48 ms, scale 20 5.7 -> 4.5 ms, 22 and 24
+// unchanged); with 4 lanes per row it cost 5-7% at scales 18-20.
+constexpr uint32_t CHUNK_SPAN = 8192;
+struct ChunkMap {
+  uint32_t stride, nwe;  // wavefronts wid % stride == 0 take chunks wid / stride + t * nwe
+  __device__ ChunkMap(uint64_t nchunks, uint32_t nw, bool spread) {
+    const uint32_t span = min(nw, CHUNK_SPAN);
+    stride = spread && nchunks < span ? span / (uint32_t)max(nchunks, (uint64_t)1) : 1u;
+    nwe = (nw + stride - 1) / stride;
+  }
+  __device__ bool takes(uint32_t wid) const { return wid % stride == 0; }
+  __device__ uint64_t first(uint32_t wid) const { return wid / stride; }
+};
+
 // Rows [r0, r1) with at most PSEG entries; row group g of wavefront w takes
 // rows r0 + (w + t * nw) * (64 / LPR) + g. red: LDS of (BLOCK / WAVE) x LPR
 // doubles (all threads of the block call).
@@ -1274,9 +1295,13 @@ __device__ __forceinline__ void mppr_sub_segs_body(const DG& g, Part p, const Pp
   const uint32_t lane = lane_id(), grp = lane / LPR, q = lane % LPR;
   const bool live = q < a.k;
   const uint32_t nw = p.nwaves(), wid = p.wid();
-  for (uint64_t base = (uint64_t)wid * RPW; base < nseg; base += (uint64_t)nw * RPW) {
-    const uint32_t sg = (uint32_t)base + grp;
-    if (sg >= nseg) continue;
+  const uint64_t nch = ((uint64_t)nseg + RPW - 1) / RPW;
+  const ChunkMap cm(nch, nw, LPR == 1);
+  if (!cm.takes(wid)) return;
+  for (uint64_t ch = cm.first(wid); ch < nch; ch += cm.nwe) {  // wave-uniform
+    const uint64_t si = ch * RPW + grp;
+    if (si >= nseg) continue;
+    const uint32_t sg = (uint32_t)si;
     const uint32_t r = a.seg_row[sg], b = a.seg_beg[sg];
     const uint32_t e = min(b + PSEG, g.row_ptr[r + 1]);
     float s = 0.f, c = 0.f;
@@ -1313,9 +1338,13 @@ __device__ __forceinline__ void mppr_sub_combine_body(const DG& g, Part p, const
   const uint32_t seed = live ? a.seeds[q] : 0xffffffffu;
   const uint32_t nw = p.nwaves(), wid = p.wid();
   double res = 0.0;
-  for (uint64_t base = (uint64_t)wid * RPW; base < a.nlong_ppr; base += (uint64_t)nw * RPW) {
-    const uint32_t i = (uint32_t)base + grp;
-    if (i >= a.nlong_ppr) continue;
+  // long rows spread over the grid as the segments are
+  const uint64_t nch = ((uint64_t)a.nlong_ppr + RPW - 1) / RPW;
+  const ChunkMap cm(nch, nw, LPR == 1);
+  for (uint64_t ch = cm.takes(wid) ? cm.first(wid) : nch; ch < nch; ch += cm.nwe) {  // wave-uniform
+    const uint64_t li = ch * RPW + grp;
+    if (li >= a.nlong_ppr) continue;
+    const uint32_t i = (uint32_t)li;
     const uint32_t r = a.long_ppr[i];
     if (r < r0 || r >= r1 || !live) continue;
     float s = 0.f, c = 0.f;
