@@ -465,10 +465,17 @@ __global__ void k_sssp_init(DG g, uint32_t* dist, uint32_t* stamp, uint32_t root
 // msg[s], its distance when the previous step ended (k_sssp_snap), not a
 // distance lowered earlier in this step; bounded runs use it so that m steps
 // give exactly the reference's m iterations (T must then be ~0).
+//
+// nitems_dev: when set, the item count is read from the device (the count
+// the previous step emitted) instead of `nitems`, so the host can enqueue
+// steps without waiting for counts; zc must then be null (the host clears
+// the consumed counters after the launch).
 template <bool SYNC>
 __global__ void __launch_bounds__(BLOCK) k_sssp_push(DG g, const uint64_t* items, uint32_t nitems,
                                                      uint32_t* dist, uint32_t* stamp, uint32_t round,
-                                                     uint32_t T, Frontier nf, uint32_t* zc, const uint32_t* msg) {
+                                                     uint32_t T, Frontier nf, uint32_t* zc, const uint32_t* msg,
+                                                     const uint32_t* nitems_dev) {
+  if (nitems_dev) nitems = *nitems_dev;
   zero_ctr(zc);
   const uint32_t nw = gridDim.x * (BLOCK / WAVE);
   const uint32_t wid = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE;
