@@ -458,6 +458,11 @@ typedef struct {
                              to max_batch while task slots and budget allow, and
                              retires finished groups, so queries of every program and
                              graph share each fused launch */
+  double max_wait_s;      /* batch order (fused = 0): 0 runs the group of the oldest
+                             pending query first; > 0 runs the group of least
+                             expected work first (the graph's entries times 1 for
+                             BFS, 4 for SSSP, the iterations for PPR), unless the
+                             oldest pending query has waited longer than this */
 } la3_server_config;
 typedef struct {
   uint64_t qid;

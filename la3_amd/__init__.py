@@ -68,7 +68,8 @@ class KernelStat(ctypes.Structure):
 class ServerConfig(ctypes.Structure):
     _fields_ = [("nworkers", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
                 ("state_budget", ctypes.c_uint64), ("keep_results", ctypes.c_int),
-                ("ppr_alpha", ctypes.c_double), ("fused", ctypes.c_int)]
+                ("ppr_alpha", ctypes.c_double), ("fused", ctypes.c_int),
+                ("max_wait_s", ctypes.c_double)]
 
 
 class CscArrays(ctypes.Structure):
@@ -673,9 +674,12 @@ class Server:
     batching by worker threads, admission by per-query state bytes."""
 
     def __init__(self, device=0, nworkers=2, max_batch=64, state_budget=32 << 30,
-                 keep_results=False, ppr_alpha=0.15, fused=False):
+                 keep_results=False, ppr_alpha=0.15, fused=False, max_wait_s=0.05):
+        """max_wait_s: batch order of the grouped mode (0: oldest query's group
+        first; > 0: least expected work first, unless the oldest query has
+        waited longer than max_wait_s seconds)."""
         cfg = ServerConfig(nworkers, max_batch, state_budget, int(keep_results), ppr_alpha,
-                           int(fused))
+                           int(fused), float(max_wait_s))
         h = ctypes.c_void_p()
         self._L = lib()
         st = lib().la3_server_create(device, ctypes.byref(cfg), ctypes.byref(h))

@@ -30,6 +30,7 @@ struct GraphEntry {
   la3_graph* g;
   uint32_t nslots;
   bool weighted, directed;
+  uint64_t nnz;
 };
 
 }  // namespace
@@ -78,20 +79,47 @@ size_t elem_bytes(la3_query_kind k) { return k == LA3_Q_BFS ? 1 : 4; }
 
 std::tuple<int, uint32_t, uint32_t> group_of(const Query& q) { return {(int)q.kind, q.graph, q.iters}; }
 
-// Takes, with the lock held, the oldest pending query and the later pending
-// queries with its (kind, graph, iters), up to `cap`, while the admitted state
-// bytes fit the budget (a lone query is admitted when nothing runs). With
-// `exclusive`, queries of a group that has a batch running are left pending:
-// they go together in that group's next batch instead of a second sweep of
-// the same graph beside it. Returns the picked queries (arrival order) and
-// their state bytes.
+// Expected work of a query: the graph's entries times the passes over them
+// (BFS 1, SSSP about 4 with its re-scans, PPR one per iteration).
+double expected_work(const la3_server* s, const Query& q) {
+  const double f = q.kind == LA3_Q_BFS ? 1.0 : q.kind == LA3_Q_SSSP ? 4.0 : (double)std::max(1u, q.iters);
+  return f * (double)std::max<uint64_t>(s->graphs[q.graph].nnz, 1);
+}
+
+// Takes, with the lock held, a head query and the later pending queries with
+// its (kind, graph, iters), up to `cap`, while the admitted state bytes fit
+// the budget (a lone query is admitted when nothing runs). The head is the
+// oldest pending query, or with max_wait_s > 0 (and `exclusive`) the one of
+// least expected work unless the oldest has waited longer than max_wait_s:
+// small graphs' queries then do not queue behind large PPR batches (at 400
+// QPS on the six-graph stream p50 8.2 -> 5.1-5.5 ms, p99 unchanged), and the
+// age bound keeps the large ones from starving. With `exclusive`, queries of
+// a group that has a batch running are left pending: they go together in
+// that group's next batch instead of a second sweep of the same graph beside
+// it. Returns the picked queries (arrival order) and their state bytes.
 std::vector<Query> pick_group(la3_server* s, uint32_t cap, uint64_t& bytes, bool exclusive) {
   std::vector<Query> batch;
   bytes = 0;
   if (s->pending.empty() || cap == 0) return batch;
   size_t h = 0;
-  if (exclusive)
-    while (h < s->pending.size() && s->running_groups.count(group_of(s->pending[h]))) h++;
+  if (exclusive) {
+    const size_t n = s->pending.size();
+    size_t oldest = n, least = n;
+    double lw = 0;
+    for (size_t i = 0; i < n; i++) {
+      const Query& q = s->pending[i];
+      if (s->running_groups.count(group_of(q))) continue;
+      if (oldest == n) {
+        oldest = i;
+        if (s->cfg.max_wait_s <= 0) break;
+      }
+      const double w = expected_work(s, q);
+      if (least == n || w < lw) { least = i; lw = w; }
+    }
+    h = oldest;
+    if (s->cfg.max_wait_s > 0 && oldest < n && s->now() - s->pending[oldest].submit_s <= s->cfg.max_wait_s)
+      h = least;
+  }
   if (h == s->pending.size()) return batch;
   const Query head = s->pending[h];
   std::vector<size_t> pick;
@@ -347,7 +375,7 @@ extern "C" {
 
 la3_status la3_server_create(int device, const la3_server_config* cfg, la3_server** out) {
   if (!cfg || !out || cfg->nworkers == 0 || cfg->nworkers > 64 || cfg->max_batch == 0 ||
-      cfg->max_batch > 1024 || !(cfg->ppr_alpha > 0 && cfg->ppr_alpha < 1))
+      cfg->max_batch > 1024 || !(cfg->ppr_alpha > 0 && cfg->ppr_alpha < 1) || !(cfg->max_wait_s >= 0))
     return LA3_EINVAL;
   std::vector<la3_ctx*> ctxs;
   for (uint32_t i = 0; i < cfg->nworkers; i++) {
@@ -374,7 +402,7 @@ la3_status la3_server_add_graph(la3_server* s, la3_graph* g, uint32_t* graph_id)
   if (st) return st;
   std::lock_guard<std::mutex> lk(s->mu);
   *graph_id = (uint32_t)s->graphs.size();
-  s->graphs.push_back({g, info.nslots, info.weighted != 0, info.directed != 0});
+  s->graphs.push_back({g, info.nslots, info.weighted != 0, info.directed != 0, info.nnz});
   return LA3_OK;
 }
 

@@ -4,6 +4,8 @@ same query run alone (bit-exact; PPR bit-identical to a batch of one and
 within L1 1e-7 of the FP64 single-query program). Both admission modes:
 per-(program, graph) batches, and mixed batches across programs and graphs
 run as one fused batch run."""
+import time
+
 import numpy as np
 import pytest
 
@@ -22,9 +24,10 @@ def graphs(gpu_ctx):
     return out
 
 
-@pytest.mark.parametrize("fused", [False, True])
-def test_mixed_stream_equals_alone(gpu_ctx, graphs, fused):
-    srv = L.Server(0, nworkers=3, max_batch=16, state_budget=1 << 30, keep_results=True, fused=fused)
+@pytest.mark.parametrize("fused,max_wait_s", [(False, 0.0), (False, 0.05), (True, 0.05)])
+def test_mixed_stream_equals_alone(gpu_ctx, graphs, fused, max_wait_s):
+    srv = L.Server(0, nworkers=3, max_batch=16, state_budget=1 << 30, keep_results=True, fused=fused,
+                   max_wait_s=max_wait_s)
     gids = [srv.add_graph(g) for g, _ in graphs]
     srv.warm()  # every worker runs each program on each graph (a no-op when fused)
     assert srv.completed() == []  # warm-up runs are not queries
@@ -97,6 +100,32 @@ def test_graphs_added_while_serving(gpu_ctx, graphs, fused):
         got = srv.result(qid, kind, g.nslots)
         want = L.bfs(g, root, parents=False)[0] if kind == L.Q_BFS else L.sssp(g, root)[0]
         assert (got == want).all()
+    srv.close()
+
+
+@pytest.mark.parametrize("max_wait_s", [0.0, 10.0])
+def test_batch_order(gpu_ctx, graphs, max_wait_s):
+    """Per-(program, graph) mode, one worker: while a long PPR batch runs, a
+    second PPR group on the large graph and a BFS group on the small graph
+    queue up. Oldest first (max_wait_s = 0) runs the PPR group next; least
+    expected work first (no query has waited max_wait_s) runs the BFS group
+    next."""
+    (gs, rs), (gl, rl) = graphs[0], graphs[2]
+    srv = L.Server(0, nworkers=1, max_batch=8, state_budget=1 << 30, keep_results=False, fused=False,
+                   max_wait_s=max_wait_s)
+    ids, il = srv.add_graph(gs), srv.add_graph(gl)
+    first = srv.submit(L.Q_PPR, il, int(rl[0]), 1000)  # long enough for the rest to queue
+    time.sleep(0.005)  # the worker has taken it
+    ppr = srv.submit(L.Q_PPR, il, int(rl[1]), 29)      # another (kind, graph, iters) group
+    bfs = [srv.submit(L.Q_BFS, ids, int(r), 0) for r in rs[:3]]
+    srv.drain()
+    done = {d["qid"]: d for d in srv.completed()}
+    assert len(done) == 5 and all(d["status"] == 0 for d in done.values())
+    assert done[first]["start_s"] <= min(done[q]["start_s"] for q in [ppr] + bfs)
+    if max_wait_s == 0:
+        assert done[ppr]["start_s"] < min(done[q]["start_s"] for q in bfs)
+    else:
+        assert max(done[q]["start_s"] for q in bfs) < done[ppr]["start_s"]
     srv.close()
 
 
