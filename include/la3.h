@@ -433,8 +433,9 @@ la3_status la3_autotune(la3_graph* g, const uint32_t* roots, uint32_t nroots,
 /* A request loop over resident graphs. Queries (BFS, SSSP, PPR) are
  * submitted one by one; `nworkers` host threads, each with its own context
  * (stream, scratch) on the device, repeatedly form a batch from the pending
- * queries — the oldest pending query whose key (graph, program and, for
- * PPR, iteration count) has no batch running fixes the key, and up to
+ * queries — a pending query whose key (graph, program and, for PPR,
+ * iteration count) has no batch running fixes the key (the oldest such
+ * query, or by max_wait_s the one of least expected work), and up to
  * max_batch pending queries with that key join it in arrival order; queries
  * of a key with a batch running wait for that batch to finish and form the
  * key's next one — as long as the per-query state bytes of
