@@ -1633,6 +1633,7 @@ static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t
       for (uint32_t i = 0; i < iters; i++) {
         b.x = (i & 1) ? xb : xa;
         b.xnext = (i & 1) ? xa : xb;
+        b.skip_rank = i + 1 < iters;  // only the last iteration's ranks are read
         pr_pass<1>(c, g, b, false, part, "pr_step");
       }
       hipError_t e = hipStreamEndCapture(c->stream, &graph);
@@ -1655,6 +1656,7 @@ static la3_status pr_common(la3_graph* g, uint32_t iters, double alpha, uint32_t
     if (until) launch(c, "fill_u32", k_fill<uint32_t>, 1, act, (uint64_t)1, 0u);
     a.x = xa;
     a.xnext = xb;
+    a.skip_rank = !until && it + 1 < iters;
     // Algorithmic bytes: 4 B index + 8 B message per regular-column entry,
     // 8 B source sum + 16 B rank read/written + 8 B next message per row.
     const int ev = pr_pass<1>(c, g, a, false, part, "pr_step");
@@ -2230,6 +2232,8 @@ static la3_status mppr_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   // of state and messages per regular row, 8k B per segment partial.
   const double kb = 4.0 * k;
   for (uint32_t it = 0; it < iters; it++) {
+    a.skip_rank = it + 2 < iters;   // the last two iterations store their ranks,
+    a.skip_resid = it + 1 < iters;  // the last one reads them back for its residual
     step(0, g->nreg, 0, d.npseg_reg, 1, res_r, res_c);
     annotate(c, ev_rows, (4.0 + kb) * g->nnz_reg + 5.0 * kb * g->nreg + 2.0 * kb * d.npseg_reg, g->nnz_reg);
     c->edges_scanned += g->nnz_reg;
@@ -2252,6 +2256,8 @@ static la3_status mppr_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   // from the initial ones, which the other buffer still holds.
   if (g->nsink) {
     if (iters == 1) std::swap(xa, xb);
+    a.skip_rank = 0;
+    a.skip_resid = 0;
     step(g->nreg, g->nrows, d.npseg_reg, d.npseg, 0, res_s, res_s);
   }
   launch(c, "mppr_output", k_batch_output<float>, blocks_for(S), d, (const float*)hi, K, k, out);

@@ -898,6 +898,12 @@ struct PprBatchArgs {
   const uint32_t* long_ppr; // rows with > PSEG entries
   uint32_t nlong_ppr;
   const uint32_t* seg_first;  // long row i -> first segment index (nlong_ppr + 1 entries)
+  // An iteration's new rank is a function of its row sum alone, and the
+  // next iteration's messages are formed from it in registers: only the
+  // last iteration's ranks and residual are ever read. Earlier iterations
+  // skip storing (hi, lo) (all but the last two) and reading the old pair
+  // for the residual (all but the last). Zero: store and compute both.
+  uint32_t skip_rank, skip_resid;
 };
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
@@ -941,10 +947,11 @@ __device__ __forceinline__ void ppr_finish(const PprBatchArgs& a, uint32_t r, co
     const bool sd = seed[w] == r;
     float nh, nl;
     ppr_apply(sd ? a.alpha_hi : 0.f, sd ? a.alpha_lo : 0.f, a.damp_hi, a.damp_lo, s[w], c[w], nh, nl);
-    const double d = fabs(((double)nh + nl) - ((double)a.hi[o] + a.lo[o]));
-    res[w] += d;
-    a.hi[o] = nh;
-    a.lo[o] = nl;
+    if (!a.skip_resid) res[w] += fabs(((double)nh + nl) - ((double)a.hi[o] + a.lo[o]));
+    if (!a.skip_rank) {
+      a.hi[o] = nh;
+      a.lo[o] = nl;
+    }
     if (loop) a.xnext[o] = ppr_msg(nh, nl, deg);
   }
 }
@@ -1212,9 +1219,11 @@ __device__ __forceinline__ void ppr_sub_finish(const PprBatchArgs& a, uint32_t r
   const bool sd = seed == r;
   float nh, nl;
   ppr_apply(sd ? a.alpha_hi : 0.f, sd ? a.alpha_lo : 0.f, a.damp_hi, a.damp_lo, s, c, nh, nl);
-  res += fabs(((double)nh + nl) - ((double)a.hi[o] + a.lo[o]));
-  a.hi[o] = nh;
-  a.lo[o] = nl;
+  if (!a.skip_resid) res += fabs(((double)nh + nl) - ((double)a.hi[o] + a.lo[o]));
+  if (!a.skip_rank) {
+    a.hi[o] = nh;
+    a.lo[o] = nl;
+  }
   if (loop) a.xnext[(size_t)xr * LPR + q] = ppr_msg(nh, nl, a.deg[r]);
 }
 

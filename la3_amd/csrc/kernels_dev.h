@@ -585,6 +585,11 @@ struct PrArgs {
   uint32_t* active;      // set when some |delta| > tol
   double alpha, damp, tol;
   uint32_t seed;         // slot of the PPR seed, ~0 for PR
+  // A fixed-iteration run reads no rank between iterations (the new rank is
+  // a function of the row sum; the next messages are formed in registers)
+  // and needs no convergence flag: all but its last iteration skip the
+  // rank's read and store. Zero: read, store and flag.
+  uint32_t skip_rank;
 };
 
 __device__ __forceinline__ double pr_base(const PrArgs& a, uint32_t r) {
@@ -599,10 +604,12 @@ __device__ __forceinline__ void pr_finish(const DG& g, const PrArgs& a, uint32_t
   if (MODE == 0) { a.srcsum[r] = y; return; }
   const double nr = __dadd_rn(pr_base(a, r), __dmul_rn(a.damp, y));
   if (MODE == 1) {
-    const double old = a.rank[r];
-    a.rank[r] = nr;
+    if (!a.skip_rank) {
+      const double old = a.rank[r];
+      a.rank[r] = nr;
+      if (fabs(nr - old) > a.tol) a.active[0] = 1;
+    }
     a.xnext[g.rank_of[r]] = nr / (double)a.deg[r];
-    if (fabs(nr - old) > a.tol) a.active[0] = 1;
   } else {
     a.rank[r] = nr;
   }

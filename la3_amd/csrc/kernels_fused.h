@@ -159,6 +159,8 @@ __device__ __forceinline__ void fused_ppr_a(const FTask& T, const FCtl& C, Part 
   PprBatchArgs a = T.pa;
   a.x = T.xs[C.cur];
   a.xnext = T.xs[C.cur ^ 1];
+  a.skip_rank = C.iter + 2 < T.iters;
+  a.skip_resid = C.iter + 1 < T.iters;
   a.resid = T.res_rows;
   mppr_rows_body<QW>(T.g, p, a, 0, T.g.nreg, true, red);
   mppr_segs_body<QW>(T.g, p, a, T.g.npseg_reg);
@@ -169,6 +171,8 @@ __device__ __forceinline__ void fused_ppr_b(const FTask& T, const FCtl& C, Part 
   PprBatchArgs a = T.pa;
   a.x = T.xs[C.cur];
   a.xnext = T.xs[C.cur ^ 1];
+  a.skip_rank = C.iter + 2 < T.iters;
+  a.skip_resid = C.iter + 1 < T.iters;
   a.resid = T.res_comb;
   const DG g = T.g;
   mppr_combine_body<QW>(g, p, a, 0, g.nreg, true, red);
@@ -193,6 +197,8 @@ __device__ __forceinline__ void fused_ppr_an(const FTask& T, const FCtl& C, Part
   PprBatchArgs a = T.pa;
   a.x = T.xs[C.cur];
   a.xnext = T.xs[C.cur ^ 1];
+  a.skip_rank = C.iter + 2 < T.iters;
+  a.skip_resid = C.iter + 1 < T.iters;
   a.resid = T.res_rows;
   mppr_sub_rows_body<LPR>(T.g, p, a, 0, T.g.nreg, true, red);
   // entries in flight per lane along a segment: the launch holds both
@@ -207,6 +213,8 @@ __device__ __forceinline__ void fused_ppr_bn(const FTask& T, const FCtl& C, Part
   PprBatchArgs a = T.pa;
   a.x = T.xs[C.cur];
   a.xnext = T.xs[C.cur ^ 1];
+  a.skip_rank = C.iter + 2 < T.iters;
+  a.skip_resid = C.iter + 1 < T.iters;
   a.resid = T.res_comb;
   mppr_sub_combine_body<LPR>(T.g, p, a, 0, T.g.nreg, true, red);
 }
