@@ -334,6 +334,22 @@ __device__ __forceinline__ void fzero(const Frontier& f) {
   for (uint32_t i = lane_id(); i < CTR_WORDS; i += WAVE) w[i] = 0;
 }
 
+// r + p[0] + p[stride] + ... + p[(n - 1) * stride], added in that order;
+// eight loads are issued before their adds (one thread sums thousands of
+// block partials).
+__device__ __forceinline__ double ordered_sum(const double* p, uint32_t n, uint32_t stride, double r) {
+  uint32_t b = 0;
+  for (; b + 8 <= n; b += 8) {
+    double v[8];
+#pragma unroll
+    for (uint32_t u = 0; u < 8; u++) v[u] = p[(size_t)(b + u) * stride];
+#pragma unroll
+    for (uint32_t u = 0; u < 8; u++) r += v[u];
+  }
+  for (; b < n; b++) r += p[(size_t)b * stride];
+  return r;
+}
+
 // Control step (one block of BLOCK threads). Tasks in state FS_NEW are set
 // up from their seeded work lists and become FS_RUN; with init_only == 0,
 // running tasks account for the step just run and advance (done tasks and
@@ -452,10 +468,8 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
     const uint32_t K = T.lpr ? T.lpr : 64 * T.width;
     const uint32_t na = C.a_end - C.a_beg, nb = C.b_end - C.b_beg;
     for (uint32_t q = threadIdx.x; q < K; q += BLOCK) {
-      double r = 0;
-      for (uint32_t b = 0; b < na; b++) r += T.res_rows[(size_t)b * K + q];
-      if (T.g.npseg_reg)
-        for (uint32_t b = 0; b < nb; b++) r += T.res_comb[(size_t)b * K + q];
+      double r = ordered_sum(T.res_rows + q, na, K, 0.0);
+      if (T.g.npseg_reg) r = ordered_sum(T.res_comb + q, nb, K, r);
       T.resid[q] = r;
     }
   }
