@@ -115,13 +115,14 @@ def test_batch_order(gpu_ctx, graphs, max_wait_s):
                    max_wait_s=max_wait_s)
     ids, il = srv.add_graph(gs), srv.add_graph(gl)
     first = srv.submit(L.Q_PPR, il, int(rl[0]), 1000)  # long enough for the rest to queue
-    time.sleep(0.005)  # the worker has taken it
+    time.sleep(0.005)  # usually the worker has taken it
     ppr = srv.submit(L.Q_PPR, il, int(rl[1]), 29)      # another (kind, graph, iters) group
     bfs = [srv.submit(L.Q_BFS, ids, int(r), 0) for r in rs[:3]]
     srv.drain()
     done = {d["qid"]: d for d in srv.completed()}
     assert len(done) == 5 and all(d["status"] == 0 for d in done.values())
-    assert done[first]["start_s"] <= min(done[q]["start_s"] for q in [ppr] + bfs)
+    # (whether or not the worker took `first` before the others arrived, the
+    # order between the PPR and BFS groups is fixed by the policy)
     if max_wait_s == 0:
         assert done[ppr]["start_s"] < min(done[q]["start_s"] for q in bfs)
     else:
