@@ -2203,7 +2203,7 @@ static la3_status mppr_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   a.part = part; a.seg_row = d.pseg_row; a.seg_beg = d.pseg_beg; a.long_ppr = d.plong;
   a.nlong_ppr = d.nplong; a.seg_first = d.pseg_first;
   launch(c, "outdeg", k_outdeg, blocks_for(S), d, deg);
-  launch(c, "mppr_init", k_mppr_init_k<K>, blocks_for(SK), d, a);
+  launch(c, "mppr_init", k_mppr_init_k<K>, blocks_for(SK), d, a, iters);
   float* xa = x0;
   float* xb = x1;
   int ev_rows = -1;
@@ -2257,7 +2257,7 @@ static la3_status mppr_run(la3_ctx* c, la3_graph* g, const std::vector<uint32_t>
   if (g->nsink) {
     if (iters == 1) std::swap(xa, xb);
     a.skip_rank = 0;
-    a.skip_resid = 0;
+    a.skip_resid = 1;  // sink ranks are stored; their residual is not reported
     step(g->nreg, g->nrows, d.npseg_reg, d.npseg, 0, res_s, res_s);
   }
   launch(c, "mppr_output", k_batch_output<float>, blocks_for(S), d, (const float*)hi, K, k, out);
@@ -2694,15 +2694,15 @@ la3_status fz_setup(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t t
   T.step_edges = g->nnz_reg;
   launch(c, "outdeg", k_outdeg, blocks_for(S), d, deg);
   switch (narrow ? LPR : 64 * W) {
-    case 1: launch(c, "mppr_init", k_mppr_init_k<1>, blocks_for(SK), d, a); break;
-    case 2: launch(c, "mppr_init", k_mppr_init_k<2>, blocks_for(SK), d, a); break;
-    case 4: launch(c, "mppr_init", k_mppr_init_k<4>, blocks_for(SK), d, a); break;
-    case 8: launch(c, "mppr_init", k_mppr_init_k<8>, blocks_for(SK), d, a); break;
-    case 16: launch(c, "mppr_init", k_mppr_init_k<16>, blocks_for(SK), d, a); break;
-    case 32: launch(c, "mppr_init", k_mppr_init_k<32>, blocks_for(SK), d, a); break;
-    case 64: launch(c, "mppr_init", k_mppr_init<1>, blocks_for(SK), d, a); break;
-    case 128: launch(c, "mppr_init", k_mppr_init<2>, blocks_for(SK), d, a); break;
-    default: launch(c, "mppr_init", k_mppr_init<4>, blocks_for(SK), d, a); break;
+    case 1: launch(c, "mppr_init", k_mppr_init_k<1>, blocks_for(SK), d, a, T.iters); break;
+    case 2: launch(c, "mppr_init", k_mppr_init_k<2>, blocks_for(SK), d, a, T.iters); break;
+    case 4: launch(c, "mppr_init", k_mppr_init_k<4>, blocks_for(SK), d, a, T.iters); break;
+    case 8: launch(c, "mppr_init", k_mppr_init_k<8>, blocks_for(SK), d, a, T.iters); break;
+    case 16: launch(c, "mppr_init", k_mppr_init_k<16>, blocks_for(SK), d, a, T.iters); break;
+    case 32: launch(c, "mppr_init", k_mppr_init_k<32>, blocks_for(SK), d, a, T.iters); break;
+    case 64: launch(c, "mppr_init", k_mppr_init<1>, blocks_for(SK), d, a, T.iters); break;
+    case 128: launch(c, "mppr_init", k_mppr_init<2>, blocks_for(SK), d, a, T.iters); break;
+    default: launch(c, "mppr_init", k_mppr_init<4>, blocks_for(SK), d, a, T.iters); break;
   }
   return LA3_OK;
 }
@@ -2737,6 +2737,7 @@ la3_status fz_finish(la3_ctx* c, const la3_batch& b, const FzTask& ft, uint32_t 
     // with one iteration (execute_single) from the initial ones.
     if (g->nsink) {
       PprBatchArgs a = T.pa;
+      a.skip_resid = 1;  // sink ranks are stored; their residual is not reported
       const uint32_t xi = T.iters == 1 ? C.cur ^ 1 : C.cur;
       a.x = T.xs[xi];
       a.xnext = T.xs[xi ^ 1];

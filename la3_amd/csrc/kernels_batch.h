@@ -1390,39 +1390,43 @@ __global__ void __launch_bounds__(BLOCK) k_mppr_sub_combine(DG g, PprBatchArgs a
 }
 
 // Initial state for K lanes per vertex (K = 64 * QW, or LPR; narrow rows,
-// K < 64, store the messages by rank).
+// K < 64, store the messages by rank), for a run of `iters` iterations.
+// Only what a later read needs is written: the messages of the senders
+// (regular and source slots), the next-step messages of the sources (no
+// row updates them), and the ranks of the slots no iteration writes
+// (sources, isolated; regular rows too when one iteration reads them back
+// for its residual). Sink ranks come from the sink step.
 template <uint32_t K>
-__global__ void k_mppr_init_k(DG g, PprBatchArgs a) {
+__device__ __forceinline__ void mppr_init_body(const DG& g, const PprBatchArgs& a, uint32_t iters) {
   const uint64_t total = (uint64_t)g.nslots * K;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
        i += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t s = (uint32_t)(i / K), q = (uint32_t)(i % K);
     const bool seed = q < a.k && a.seeds[q] == s;
     const float th = seed ? a.alpha_hi : 0.f, tl = seed ? a.alpha_lo : 0.f;
-    a.hi[i] = th;
-    a.lo[i] = tl;
-    const float x = slot_has_col(g, s) ? ppr_msg(th, tl, a.deg[s]) : 0.f;
-    const uint64_t xi = K < 64 ? (uint64_t)g.rank_of[s] * K + q : i;
-    ((float*)a.x)[xi] = x;
-    a.xnext[xi] = x;
+    const bool regular = s < g.nreg, sink = !regular && s < g.nrows;
+    const bool source = s >= g.nrows && s < g.nrows + g.nsrc;
+    if (!sink && !(regular && iters >= 2)) {
+      a.hi[i] = th;
+      a.lo[i] = tl;
+    }
+    if (regular || source) {
+      const float x = ppr_msg(th, tl, a.deg[s]);
+      const uint64_t xi = K < 64 ? (uint64_t)g.rank_of[s] * K + q : i;
+      ((float*)a.x)[xi] = x;
+      if (source) a.xnext[xi] = x;
+    }
   }
 }
 
+template <uint32_t K>
+__global__ void k_mppr_init_k(DG g, PprBatchArgs a, uint32_t iters) {
+  mppr_init_body<K>(g, a, iters);
+}
+
 template <int QW>
-__global__ void k_mppr_init(DG g, PprBatchArgs a) {
-  constexpr uint32_t K = 64 * QW;
-  const uint64_t total = (uint64_t)g.nslots * K;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t s = (uint32_t)(i / K), q = (uint32_t)(i % K);
-    const bool seed = q < a.k && a.seeds[q] == s;
-    const float th = seed ? a.alpha_hi : 0.f, tl = seed ? a.alpha_lo : 0.f;
-    a.hi[i] = th;
-    a.lo[i] = tl;
-    const float x = slot_has_col(g, s) ? ppr_msg(th, tl, a.deg[s]) : 0.f;
-    ((float*)a.x)[i] = x;
-    a.xnext[i] = x;
-  }
+__global__ void k_mppr_init(DG g, PprBatchArgs a, uint32_t iters) {
+  mppr_init_body<64 * QW>(g, a, iters);
 }
 
 }  // namespace la3
