@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -2547,6 +2548,10 @@ la3_status la3_rmat_generate(uint32_t scale, uint32_t edgefactor, double a, doub
 namespace {
 
 constexpr uint32_t FZ_STEPS = 4;
+// Session sub-steps: tasks on graphs with at most 1/SUB_RATIO of the entries
+// of the largest running task's graph, up to SUB_MAX sub-steps per step.
+constexpr uint64_t SUB_RATIO = 8;
+constexpr uint32_t SUB_MAX = 6;
 constexpr uint32_t FZ_BLOCKS_PER_CU = 32;
 constexpr size_t FZ_SLOT_KEEP_BYTES = 256ull << 20;  // a finished session slot keeps up to this much
 
@@ -2794,7 +2799,8 @@ constexpr uint32_t FZ_PULL_BIT = NCLS;  // bits NCLS + 0..2: bottom-up BFS launc
 // bottom-up BFS launch of each wide class with BFS tasks (bits
 // FZ_PULL_BIT + 0..2), then the control launch. With profiling each launch is
 // bracketed by events (last_a: the event slot of the last A).
-void fz_step(la3_ctx* c, uint32_t mask, uint32_t ga_blocks, bool events, int* last_a, int* last_p) {
+void fz_step(la3_ctx* c, uint32_t mask, uint32_t ga_blocks, bool events, int* last_a, int* last_p,
+             uint32_t cmode = FC_ALL) {
   const uint32_t ga = ga_blocks + FMAX_TASKS, gb = fz_gb(c) + FMAX_TASKS;
   const FTask* tk = (const FTask*)c->fz_tasks;
   FHdr* hdr = (FHdr*)c->fz_ctl;
@@ -2821,17 +2827,27 @@ void fz_step(la3_ctx* c, uint32_t mask, uint32_t ga_blocks, bool events, int* la
   if (mask & 4) go("fused_b", k_fused_b<4>, gb);
   for (uint32_t n = 0; n < 12; n++)
     if (mask >> (3 + n) & 1) go("fused_b", fz_narrow_b[n], gb);
-  if (events) launch(c, "fused_c", k_fused_c, 1, tk, hdr, ctl, 0u);
-  else hipLaunchKernelGGL(k_fused_c, dim3(1), dim3(BLOCK), 0, c->stream, tk, hdr, ctl, 0u);
+  if (events) launch(c, "fused_c", k_fused_c, 1, tk, hdr, ctl, cmode);
+  else hipLaunchKernelGGL(k_fused_c, dim3(1), dim3(BLOCK), 0, c->stream, tk, hdr, ctl, cmode);
 }
 
-la3_status fz_capture(la3_ctx* c, uint32_t mask, uint32_t steps, uint32_t ga_blocks, hipGraphExec_t* out) {
-  hipGraphExec_t& ex = c->fz_exec[((uint64_t)steps << 52) | ((uint64_t)ga_blocks << 24) | mask];
+// steps full steps, each followed by `sub` sub-steps in which only the tasks
+// of the header's sub_mask run and advance; the control pass before each
+// launch sequence splits the blocks among the tasks that sequence runs.
+// Captured once per key.
+la3_status fz_capture(la3_ctx* c, uint32_t mask, uint32_t steps, uint32_t ga_blocks, uint32_t sub,
+                      hipGraphExec_t* out) {
+  hipGraphExec_t& ex =
+      c->fz_exec[((uint64_t)steps << 52) | ((uint64_t)sub << 48) | ((uint64_t)ga_blocks << 24) | mask];
   *out = ex;
   if (ex) return LA3_OK;
   hipGraph_t graph = nullptr;
   HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-  for (uint32_t i = 0; i < steps; i++) fz_step(c, mask, ga_blocks, false, nullptr, nullptr);
+  for (uint32_t i = 0; i < steps; i++) {
+    fz_step(c, mask, ga_blocks, false, nullptr, nullptr, sub ? FC_SPLIT_SUB : FC_ALL);
+    for (uint32_t j = 0; j < sub; j++)
+      fz_step(c, mask, ga_blocks, false, nullptr, nullptr, FC_ADV_SUB | (j + 1 < sub ? FC_SPLIT_SUB : 0u));
+  }
   hipError_t e = hipStreamEndCapture(c->stream, &graph);
   if (e != hipSuccess) return fail(c, LA3_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
@@ -2890,6 +2906,8 @@ struct la3_session {
   bool keep_slots = false;  // keep every slot's buffers (la3_batch_run: the caller releases)
   size_t keep_bytes = FZ_SLOT_KEEP_BYTES;  // a finished slot keeps its buffers up to this much
   bool pending_init = false;
+  bool sub_steps = false;   // small tasks take sub-steps (sessions; not la3_batch_run)
+  uint64_t sub_mask = 0;    // the device header's sub_mask (host copy, source of its uploads)
   std::map<uint64_t, FzBatch> batches;
   std::deque<uint64_t> done;
   uint64_t next_id = 1;
@@ -2969,26 +2987,55 @@ la3_status sess_step(la3_session* s) {
   FCtl* dc = (FCtl*)(c->fz_ctl + sizeof(FHdr));
   la3_status st;
   if (s->pending_init) {
-    launch(c, "fused_c", k_fused_c, 1, (const FTask*)c->fz_tasks, dh, dc, 1u);
+    launch(c, "fused_c", k_fused_c, 1, (const FTask*)c->fz_tasks, dh, dc, (uint32_t)FC_INIT);
     s->pending_init = false;
   }
   uint32_t mask = 0;
+  uint64_t big = 0;  // entries of the largest running task's graph
   for (uint32_t t = 0; t < s->ntasks; t++) {
     if (!s->slot[t].used) continue;
     const uint32_t cls = s->ht[t].cls;
     mask |= 1u << cls;
     if (s->ht[t].kind == FK_BFS) mask |= 1u << (FZ_PULL_BIT + cls);  // BFS tasks are wide (cls 0-2)
+    big = std::max<uint64_t>(big, s->batches[s->slot[t].batch].b.graph->nnz);
+  }
+  // Sub-steps: a step lasts as long as its largest task's, so tasks on
+  // graphs of at most 1/SUB_RATIO of the largest running graph's entries
+  // also run `sub` sub-steps per step (about log2 of the size ratio, at
+  // most SUB_MAX): they advance several iterations while the large tasks
+  // advance one, instead of waiting on them every iteration.
+  uint64_t sub_mask = 0, small = 0;
+  for (uint32_t t = 0; t < s->ntasks; t++) {
+    if (!s->slot[t].used) continue;
+    const uint64_t e = s->batches[s->slot[t].batch].b.graph->nnz;
+    if (e * SUB_RATIO <= big) {
+      sub_mask |= 1ull << t;
+      small = std::max(small, e);
+    }
+  }
+  uint32_t sub = 0;
+  if (sub_mask && s->sub_steps) {
+    const double r = (double)big / (double)std::max<uint64_t>(small, 1);
+    sub = std::min<uint32_t>(SUB_MAX, std::max<uint32_t>(1, (uint32_t)std::log2(r)));
+  }
+  if (sub && sub_mask != s->sub_mask) {
+    s->sub_mask = sub_mask;
+    HIPC(c, dcopy_async(&dh->sub_mask, &s->sub_mask, sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
   }
   int last_a = -1, last_pull = -1;
   if (mask) {
     if (!c->profiling) {
       hipGraphExec_t ex = nullptr;
-      if ((st = fz_capture(c, mask, s->steps, s->hdr.ga, &ex))) return st;
+      if ((st = fz_capture(c, mask, s->steps, s->hdr.ga, sub, &ex))) return st;
       HIPC(c, hipGraphLaunch(ex, c->stream));
     } else {
       // Profiling: the same steps as separate launches, each bracketed by
       // events (per-kernel device time; the graph has none).
-      for (uint32_t i = 0; i < s->steps; i++) fz_step(c, mask, s->hdr.ga, true, &last_a, &last_pull);
+      for (uint32_t i = 0; i < s->steps; i++) {
+        fz_step(c, mask, s->hdr.ga, true, &last_a, &last_pull, sub ? FC_SPLIT_SUB : FC_ALL);
+        for (uint32_t j = 0; j < sub; j++)
+          fz_step(c, mask, s->hdr.ga, true, &last_a, &last_pull, FC_ADV_SUB | (j + 1 < sub ? FC_SPLIT_SUB : 0u));
+      }
     }
   }
   HIPC(c, dcopy_async(s->hc, dc, sizeof(FCtl) * s->ntasks, hipMemcpyDeviceToHost, c->stream));
@@ -3128,6 +3175,7 @@ la3_status la3_session_create(la3_ctx* c, double ppr_alpha, la3_session** out) {
   auto s = std::make_unique<la3_session>();
   la3_status st = sess_init(c, ppr_alpha, FZ_BLOCKS_PER_CU, s.get());
   if (st) return st;
+  s->sub_steps = true;
   *out = s.release();
   return LA3_OK;
 }

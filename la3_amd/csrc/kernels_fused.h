@@ -86,8 +86,18 @@ struct FCtl {
 
 struct FHdr {
   uint32_t ntasks, ga, gb, active;  // active: tasks not done
-  uint32_t steps, pad[11];
+  uint32_t steps, pad0;
+  uint64_t sub_mask;                // tasks that also run the sub-steps (FC_ADV_SUB, FC_SPLIT_SUB)
+  uint32_t pad[8];
 };
+
+// Control modes (flags): FC_INIT sets up new tasks only; FC_ADV_SUB
+// advances only the tasks of hdr->sub_mask (a sub-step ran: the others kept
+// their state); FC_SPLIT_SUB gives the next launches' blocks to those tasks
+// only (a sub-step comes next; the others get none). With neither, every
+// running task advances and shares the next full step. So small tasks take
+// several iterations per step of the large ones.
+enum : uint32_t { FC_ALL = 0, FC_INIT = 1, FC_ADV_SUB = 2, FC_SPLIT_SUB = 4 };
 
 // The task of launch class CLS whose share of this launch holds blockIdx.x.
 enum : uint32_t { PH_A = 0, PH_B = 1, PH_PULL = 2 };
@@ -351,15 +361,19 @@ __device__ __forceinline__ double ordered_sum(const double* p, uint32_t n, uint3
 }
 
 // Control step (one block of BLOCK threads). Tasks in state FS_NEW are set
-// up from their seeded work lists and become FS_RUN; with init_only == 0,
-// running tasks account for the step just run and advance (done tasks and
-// FS_FREE slots are skipped). Then the blocks of the next launches are split
-// among the running tasks.
+// up from their seeded work lists and become FS_RUN; running tasks account
+// for the step just run and advance (done tasks and FS_FREE slots are
+// skipped; with FC_ADV_SUB only the tasks of hdr->sub_mask, with FC_INIT
+// none). Then the blocks of the next launches are split among the running
+// tasks (with FC_SPLIT_SUB among those of the mask only).
 __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tasks, FHdr* hdr, FCtl* ctl,
-                                                   uint32_t init_only) {
+                                                   uint32_t mode) {
   __shared__ uint32_t fin[FMAX_TASKS];   // PPR tasks whose last iteration just ran
   const uint32_t nt = hdr->ntasks;
   const uint32_t lane = lane_id();
+  const uint64_t sub = (mode & FC_ADV_SUB) ? hdr->sub_mask : ~0ull;     // tasks that advance
+  const uint64_t next = (mode & FC_SPLIT_SUB) ? hdr->sub_mask : ~0ull;  // tasks of the next launches
+  const bool init_only = mode & FC_INIT;
   for (uint32_t t = threadIdx.x / WAVE; t < nt; t += BLOCK / WAVE) {
     const FTask& T = tasks[t];
     FCtl& C = ctl[t];
@@ -368,6 +382,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
     if (lane == 0) fin[t] = 0;
     const uint32_t state = C.state;
     const bool init = state == FS_NEW;
+    if (!(sub >> t & 1)) continue;
     if (state == FS_FREE || (state == FS_RUN && (C.done || init_only))) continue;
     if (T.kind == FK_PPR) {
       if (lane == 0) {
@@ -483,7 +498,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
     uint32_t active = 0;
     for (uint32_t t = 0; t < nt; t++) {
       const FCtl& C = ctl[t];
-      if (C.state != FS_RUN || C.done) continue;
+      if (C.state != FS_RUN || C.done || !(next >> t & 1)) continue;
       const uint32_t w = tasks[t].cls;
       const bool pull = tasks[t].kind == FK_BFS && C.bottom_up;
       (pull ? totp : tot)[w] += C.est;
@@ -493,7 +508,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
     const uint32_t ga = hdr->ga, gb = hdr->gb;
     for (uint32_t t = 0; t < nt; t++) {
       FCtl& C = ctl[t];
-      const bool run = C.state == FS_RUN && !C.done;
+      const bool run = C.state == FS_RUN && !C.done && (next >> t & 1);
       const uint32_t w = run ? tasks[t].cls : 0;
       const bool pull = run && tasks[t].kind == FK_BFS && C.bottom_up;
       uint32_t na = 0, nb = 0, np = 0;
@@ -508,8 +523,8 @@ __global__ void __launch_bounds__(BLOCK) k_fused_c(const FTask* __restrict__ tas
       C.p_beg = pp[w]; C.p_end = pp[w] + np; pp[w] += np;
       C.b_beg = b[w]; C.b_end = b[w] + nb; b[w] += nb;
     }
-    hdr->active = active;
-    if (!init_only && active) hdr->steps++;
+    if (next == ~0ull) hdr->active = active;
+    if (!init_only && sub == ~0ull && active) hdr->steps++;
   }
 }
 

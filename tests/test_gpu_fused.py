@@ -150,6 +150,52 @@ def test_session_join_while_running(gpu_ctx, two_graphs):
     L.batch_run(gpu_ctx, [(gb, L.Q_BFS, r2)])
 
 
+def test_session_sub_steps_on_small_graphs(gpu_ctx):
+    """A session running tasks on a graph 32x smaller than the largest
+    running one gives them sub-steps (several iterations per step of the
+    large tasks): they finish in fewer session steps than the large PPR
+    batch, and every result equals the batched call's."""
+    gl, rl = _graph(gpu_ctx, 15, 81)
+    gs, rs = _graph(gpu_ctx, 10, 82)
+    s = L.Session(gpu_ctx)
+    batches = [
+        (gl, L.Q_PPR, _roots(rl, 3, 83), 12),
+        (gl, L.Q_SSSP, _roots(rl, 20, 84)),
+        (gs, L.Q_BFS, _roots(rs, 70, 85)),
+        (gs, L.Q_SSSP, _roots(rs, 5, 86)),
+        (gs, L.Q_PPR, _roots(rs, 2, 87), 25),
+        (gs, L.Q_SSSP, _roots(rs, 100, 88)),
+    ]
+    ids, outs = [], []
+    for g, kind, roots, *rest in batches:
+        i, o = s.add(g, kind, roots, *rest)
+        ids.append(i)
+        outs.append(o)
+    finished, step = {}, 0
+    for _ in range(10000):
+        for i, it, st in s.poll():
+            assert st == 0
+            finished[i] = (step, it)
+        if len(finished) == len(batches):
+            break
+        s.step()
+        step += 1
+    assert len(finished) == len(batches)
+    for (g, kind, roots, *rest), i, o in zip(batches, ids, outs):
+        if kind == L.Q_BFS:
+            assert np.array_equal(o, L.bfs_batch(g, roots)[0])
+        elif kind == L.Q_SSSP:
+            assert np.array_equal(o, L.sssp_batch(g, roots)[0])
+        else:
+            assert np.array_equal(o, L.ppr_batch(g, roots, rest[0])[0])
+            assert finished[i][1] == rest[0]
+    big_ppr = finished[ids[0]][0]
+    assert finished[ids[0]][0] >= 12
+    for i in ids[2:]:
+        assert finished[i][0] < big_ppr, "small-graph batches should finish first"
+    s.close()
+
+
 def test_session_slots_exhausted(gpu_ctx, two_graphs):
     _, (gb, rb) = two_graphs
     s = L.Session(gpu_ctx)
