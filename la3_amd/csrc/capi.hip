@@ -2551,7 +2551,7 @@ constexpr uint32_t FZ_STEPS = 4;
 // Session sub-steps: tasks on graphs with at most 1/SUB_RATIO of the entries
 // of the largest running task's graph, up to SUB_MAX sub-steps per step.
 constexpr uint64_t SUB_RATIO = 8;
-constexpr uint32_t SUB_MAX = 6;
+constexpr uint32_t SUB_MAX = 3;
 constexpr uint32_t FZ_BLOCKS_PER_CU = 32;
 constexpr size_t FZ_SLOT_KEEP_BYTES = 256ull << 20;  // a finished session slot keeps up to this much
 
